@@ -1,0 +1,339 @@
+/*
+ * ghive.h -- C ABI of libghive_mi355x.so, the MI355X (gfx950) implementation of the
+ * operators GHive offloads from Hive's ql operator tree: filter with stream compaction,
+ * hash-join build and probe, hash group-by aggregation and key sort, plus the fused
+ * filter -> join -> group-by fragment those operators form inside one Tez map task.
+ *
+ * Reference interfaces replaced (paths relative to the GHive tree):
+ *   gpu_impl/inc/ghive_JNIInterface.h:17-18   Java_..._JNIInterface_GPUProcess (JNI bridge; the
+ *                                             JNI shim binds to the entry points below)
+ *   gpu_impl/inc/DataFlow/Column.hpp:5-35     ColumnType / Column       -> ghive_column
+ *   gpu_impl/inc/DataFlow/Table.hpp:7-53      Table, null_value/is_null -> ghive_table, sentinels
+ *   gpu_impl/inc/Operator/FilterPredicate.hpp:17-90   PredicateMode / FilterPredicate -> ghive_pred
+ *   gpu_impl/src/Operator/FilterPredicate.cu:434      FilterPredicate::process        -> ghive_filter
+ *   gpu_impl/src/Operator/FilterOperator.cu:36-93     FilterOperator::clean_data      -> ghive_filter_compact
+ *   gpu_impl/src/Operator/FilterOperator.cu:227-487   FilterOperator::generatePredicate -> ghive_parse_predicate
+ *   gpu_impl/inc/Operator/BaseJoinPredicate.hpp:11-88 BaseJoinType / BaseJoinPredicate::process -> ghive_join
+ *   gpu_impl/inc/Operator/AggregationDesc.hpp:6-36    AggregationType / AggregationDesc -> ghive_agg
+ *   gpu_impl/inc/Operator/GroupByPredicate.hpp:172    GroupByPredicate::gpu_execute   -> ghive_groupby
+ *   gpu_impl/src/Operator/SinkOperator.cu:135-288     SinkOperator ORDER BY sort      -> ghive_sort
+ *
+ * Conventions
+ *   - Every entry point returns an int status (GHIVE_OK == 0). On failure the message is
+ *     available from ghive_last_error(ctx). Nothing is printed.
+ *   - Column data pointers are DEVICE pointers (hipMalloc / ghive_malloc / any HIP allocation)
+ *     unless a parameter says otherwise. Counts returned through uint64_t* are host pointers.
+ *   - All work of a context is ordered on the context's stream; entry points that return a
+ *     count or a group number synchronise that stream before returning.
+ *   - A context is a handle: calls on different contexts may run concurrently from different
+ *     host threads; calls on one context must not overlap. There are no global statics.
+ *   - NULL is an in-band sentinel, as in the reference (Table.hpp:29-52): LONG_MAX for LONG,
+ *     INT_MAX for INT, NaN for DOUBLE/FLOAT, (start,end) = (-1,-1) for STRING.
+ *   - Row ids are uint32 (the reference's row_num is uint32); tables must have < 2^32-1 rows.
+ */
+#ifndef GHIVE_GHIVE_H
+#define GHIVE_GHIVE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GHIVE_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------------------- */
+enum {
+  GHIVE_OK = 0,
+  GHIVE_ERR_INVALID = 1,     /* malformed descriptor or argument                           */
+  GHIVE_ERR_UNSUPPORTED = 2, /* legal in the reference, not implemented by this library    */
+  GHIVE_ERR_OOM = 3,         /* device memory manager could not satisfy an allocation       */
+  GHIVE_ERR_CAPACITY = 4,    /* caller's output buffer too small: *out_count holds the need */
+  GHIVE_ERR_HIP = 5,         /* a HIP runtime call failed                                  */
+  GHIVE_ERR_ADMISSION = 6,   /* serving: request larger than the HBM admission budget      */
+  GHIVE_ERR_NO_DEVICE = 7    /* no usable gfx950 device                                    */
+};
+
+/* ---- column types: ColumnType codes (gpu_impl/inc/DataFlow/Column.hpp:5-11) ------------ */
+enum {
+  GHIVE_LONG = 0,   /* int64                                                       */
+  GHIVE_DOUBLE = 1, /* IEEE binary64                                               */
+  GHIVE_INT = 2,    /* int32                                                       */
+  GHIVE_STRING = 3, /* UTF-8 blob + int32 (start,end) per row; not computed on here */
+  GHIVE_FLOAT = 4   /* IEEE binary32 -- extension: the reference has no FLOAT type  */
+};
+
+/* Column (gpu_impl/inc/DataFlow/Column.hpp:13-35). data/aux are device pointers. */
+typedef struct ghive_column {
+  int32_t type;       /* GHIVE_LONG .. GHIVE_FLOAT                      */
+  uint32_t reserved;  /* must be 0                                      */
+  void *data;         /* row_num values (STRING: char blob)             */
+  void *aux;          /* STRING only: 2*row_num int32 (start,end) pairs */
+  uint64_t row_num;
+  uint64_t char_size; /* STRING only                                    */
+} ghive_column;
+
+/* Table (gpu_impl/inc/DataFlow/Table.hpp:7-53). */
+typedef struct ghive_table {
+  const ghive_column *columns;
+  uint32_t ncols;
+  uint32_t key_num; /* keys-first column count, as Table::key_num */
+  uint64_t row_num;
+} ghive_table;
+
+/* ---- context, errors, streams ------------------------------------------------------- */
+typedef struct ghive_ctx ghive_ctx;
+
+/* Opens device `device`. mem_limit_bytes caps the device memory manager (0 = 90% of the
+ * HBM free at init). Replaces the implicit CUDA context of the reference library. */
+int ghive_init(int device, uint64_t mem_limit_bytes, ghive_ctx **out);
+void ghive_shutdown(ghive_ctx *ctx);
+const char *ghive_last_error(const ghive_ctx *ctx);
+int ghive_abi_version(void);
+/* Use a caller-owned hipStream_t (NULL = the context's own stream). */
+int ghive_set_stream(ghive_ctx *ctx, void *hip_stream);
+void *ghive_get_stream(ghive_ctx *ctx);
+int ghive_sync(ghive_ctx *ctx);
+
+typedef struct ghive_device_info {
+  char name[64];
+  char arch[32];          /* e.g. "gfx950"                 */
+  int32_t compute_units;
+  int32_t wavefront_size;
+  uint64_t lds_per_block; /* bytes                          */
+  uint64_t hbm_bytes;
+  uint64_t l2_bytes;
+  int32_t clock_khz;
+  int32_t mem_clock_khz;
+  int32_t mem_bus_width;
+} ghive_device_info;
+int ghive_device_query(ghive_ctx *ctx, ghive_device_info *out);
+
+/* ---- device memory manager (the reference has none: gpu_impl leaks every cudaMalloc) -- */
+int ghive_malloc(ghive_ctx *ctx, uint64_t bytes, void **out);
+int ghive_free(ghive_ctx *ctx, void *ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device. Synchronous w.r.t. the stream. */
+int ghive_memcpy(ghive_ctx *ctx, void *dst, const void *src, uint64_t bytes, int kind);
+int ghive_mem_stats(ghive_ctx *ctx, uint64_t *in_use, uint64_t *peak, uint64_t *limit,
+                    uint64_t *reserved);
+
+/* ---- synthetic data: counter-based Philox4x32-10 (Salmon et al., SC'11) -------------- *
+ * Value of row r in column c is a pure function of (seed, c, r), so any slice can be     *
+ * regenerated anywhere. Uniform integers: lo + floor(u32 * (hi-lo+1) / 2^32).            */
+int ghive_gen_uniform_i32(ghive_ctx *ctx, uint64_t seed, uint32_t column_id, uint64_t row_begin,
+                          uint64_t n, int32_t lo, int32_t hi, int32_t *out);
+int ghive_gen_uniform_i64(ghive_ctx *ctx, uint64_t seed, uint32_t column_id, uint64_t row_begin,
+                          uint64_t n, int64_t lo, int64_t hi, int64_t *out);
+/* out[i] = start + row_begin + i (dense unique dimension keys). */
+int ghive_gen_sequence_i32(ghive_ctx *ctx, uint64_t row_begin, uint64_t n, int32_t start,
+                           int32_t *out);
+int ghive_gen_sequence_i64(ghive_ctx *ctx, uint64_t row_begin, uint64_t n, int64_t start,
+                           int64_t *out);
+/* Uniform [0,1) float32 with 24 random bits; standard normal float64 (Box-Muller). */
+int ghive_gen_uniform_f32(ghive_ctx *ctx, uint64_t seed, uint32_t column_id, uint64_t row_begin,
+                          uint64_t n, float *out);
+int ghive_gen_normal_f64(ghive_ctx *ctx, uint64_t seed, uint32_t column_id, uint64_t row_begin,
+                         uint64_t n, double *out);
+
+/* ---- filter ------------------------------------------------------------------------- */
+/* PredicateMode (gpu_impl/inc/Operator/FilterPredicate.hpp:17-58), same numbering. */
+enum {
+  GHIVE_FILTER_EQ = 0,
+  GHIVE_FILTER_LE = 1,
+  GHIVE_FILTER_GE = 2,
+  GHIVE_FILTER_LT = 3,
+  GHIVE_FILTER_GT = 4,
+  GHIVE_FILTER_RANGE = 5,        /* lo <  x <  hi                 */
+  GHIVE_FILTER_EQ_RANGE = 6,     /* lo <= x <= hi   (BETWEEN)     */
+  GHIVE_FILTER_NOT_EQ = 7,
+  GHIVE_FILTER_NOT_RANGE = 8,    /* x <  lo || x >  hi (NOT BETWEEN) */
+  GHIVE_FILTER_NOT_EQ_RANGE = 9, /* x <= lo || x >= hi             */
+  GHIVE_FILTER_NOT_NULL = 10,
+  GHIVE_FILTER_AND = 11,
+  GHIVE_FILTER_OR = 12
+};
+
+/* One node of a predicate tree: the fields of FilterPredicate (FilterPredicate.hpp:61-90).
+ * pred_col_num 0 = AND/OR over children[0..1]; 1 = column vs literal(s); 2 = column vs
+ * column (filter_col[0] OP filter_col[1]). data_type: 0 long, 1 double, 2 int, 3 string,
+ * 4 none. The comparison is done on values (an INT column against a long or double
+ * literal compares the converted values); a NULL operand makes the node false. */
+typedef struct ghive_pred {
+  int32_t mode;
+  int32_t pred_col_num;
+  uint32_t filter_col[2];
+  int32_t param_num;
+  int32_t data_type;
+  int64_t long_params[2];
+  int32_t int_params[2];
+  double double_params[2];
+  int32_t children[2]; /* node indexes (AND/OR), else -1 */
+} ghive_pred;
+
+/* FilterOperator::generatePredicate (FilterOperator.cu:227-487): parses Hive's EXPLAIN
+ * predicate text against the operator's output column names. Host-only, no device. */
+int ghive_parse_predicate(const char *text, const char *const *col_names, uint32_t ncols,
+                          ghive_pred *nodes, uint32_t capacity, uint32_t *nnodes,
+                          uint32_t *root, char *err, size_t err_cap);
+/* FilterPredicate::toString(0) (FilterPredicate.cu:670-710). Returns the length needed. */
+size_t ghive_predicate_to_string(const ghive_pred *nodes, uint32_t nnodes, uint32_t root,
+                                 char *buf, size_t cap);
+
+/* FilterPredicate::process + FilterOperator::clean_data, selection form: writes the ids of
+ * the rows that pass, in ascending order, to out_sel (device, capacity row_num). */
+int ghive_filter(ghive_ctx *ctx, const ghive_table *table, const ghive_pred *nodes,
+                 uint32_t nnodes, uint32_t root, uint32_t *out_sel, uint64_t *out_count);
+/* clean_data compaction form: every column of `table` gathered by the passing rows into
+ * out_cols[i].data (device buffers of capacity table->row_num, caller-owned). Unlike the
+ * reference the input is not modified in place. STRING columns: UNSUPPORTED. */
+int ghive_filter_compact(ghive_ctx *ctx, const ghive_table *table, const ghive_pred *nodes,
+                         uint32_t nnodes, uint32_t root, ghive_column *out_cols,
+                         uint64_t *out_count);
+/* Gather rows `sel[0..n)` of every column into out_cols (device). */
+int ghive_gather(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel, uint64_t n,
+                 ghive_column *out_cols);
+
+/* ---- hash join ---------------------------------------------------------------------- */
+/* BaseJoinType (gpu_impl/inc/Operator/BaseJoinPredicate.hpp:11-25), same numbering. */
+enum {
+  GHIVE_INNER_JOIN = 0,
+  GHIVE_LEFT_OUTER_JOIN = 1,
+  GHIVE_RIGHT_OUTER_JOIN = 2, /* UNSUPPORTED (empty stub in the reference) */
+  GHIVE_FULL_OUTER_JOIN = 3,  /* UNSUPPORTED (empty stub in the reference) */
+  GHIVE_LEFT_SEMI_JOIN = 4,
+  GHIVE_CROSS_JOIN = 5        /* UNSUPPORTED (empty stub in the reference) */
+};
+
+typedef struct ghive_hash_table ghive_hash_table;
+
+/* Open-addressing hash table on device over an INT or LONG key column. Rows `sel[0..n)`
+ * (sel == NULL: all rows) are inserted; NULL keys are skipped (they never join). Each slot
+ * carries a payload: the value of `payload` (INT/LONG column, may be NULL) at that row, or
+ * the build row id when payload == NULL. load_factor in (0, 0.95]; 0 = 0.5. */
+int ghive_hash_build(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, uint64_t n,
+                     const ghive_column *payload, float load_factor, ghive_hash_table **out);
+int ghive_hash_table_info(const ghive_hash_table *ht, uint64_t *entries, uint64_t *slots,
+                          uint64_t *bytes, int32_t *unique_keys);
+void ghive_hash_table_free(ghive_ctx *ctx, ghive_hash_table *ht);
+/* Inner-join probe: for every probe row (sel or all) and every matching build entry emits
+ * (probe row id, payload). Order is unspecified (compare as a multiset). out_* are device
+ * arrays of `capacity`; on GHIVE_ERR_CAPACITY *out_count is the number needed. */
+int ghive_hash_probe(ghive_ctx *ctx, const ghive_hash_table *ht, const ghive_column *key,
+                     const uint32_t *sel, uint64_t n, uint32_t *out_probe_rows,
+                     int64_t *out_payload, uint64_t capacity, uint64_t *out_count);
+
+/* One entry of join_types (BaseJoinPredicate.hpp:66-72): join `left` to `right`. */
+typedef struct ghive_join_cond {
+  int32_t type;
+  uint32_t left;
+  uint32_t right;
+} ghive_join_cond;
+
+/* BaseJoinPredicate::process (SortMergeJoinPredicate.cu:8-461), as hash joins: tables[t]
+ * joins on its column join_cols[t] (INT or LONG). Output is result_offsets: per result
+ * row, one int32 row id per non-semi table in table order (-1 = NULL side of a LEFT
+ * OUTER join), row-major, `capacity` rows. Rows come in unspecified order. */
+int ghive_join(ghive_ctx *ctx, const ghive_table *tables, uint32_t ntables,
+               const uint32_t *join_cols, const ghive_join_cond *conds, uint32_t nconds,
+               int32_t *out_offsets, uint64_t capacity, uint64_t *out_cardinality);
+
+/* ---- group-by ----------------------------------------------------------------------- */
+/* AggregationType (gpu_impl/inc/Operator/AggregationDesc.hpp:6-27), same numbering. */
+enum {
+  GHIVE_AGG_SUM = 0,
+  GHIVE_AGG_MAX = 1,
+  GHIVE_AGG_MIN = 2,
+  GHIVE_AGG_AVG = 3,
+  GHIVE_AGG_CNT = 4,
+  GHIVE_AGG_RANK = 5,   /* UNSUPPORTED */
+  GHIVE_AGG_UNKNOWN = 6 /* UNSUPPORTED */
+};
+#define GHIVE_COUNT_STAR 0xFFFFFFFFu /* ghive_agg.index for COUNT(*) */
+
+typedef struct ghive_agg {
+  int32_t type;
+  uint32_t index; /* input column */
+} ghive_agg;
+
+/* Output type of an aggregate over an input column type, following Hive (not the
+ * reference's int32 SUM/COUNT): SUM(INT|LONG) -> LONG, SUM(DOUBLE|FLOAT) -> DOUBLE,
+ * COUNT -> LONG, AVG -> DOUBLE, MIN/MAX -> input type. */
+int ghive_agg_output_type(int32_t agg_type, int32_t input_type);
+
+/* GroupByPredicate::gpu_execute (GroupByPredicate.cu:19-359) as a hash aggregation. Rows
+ * sel[0..n) (sel == NULL: all rows). Keys: INT/LONG/DOUBLE/FLOAT columns, total width
+ * <= 8 bytes; NULL keys form one group. Output: keys then aggregates into out_cols
+ * (types as ghive_agg_output_type; device buffers of `capacity` rows), groups in
+ * unspecified order. An aggregate with no non-NULL input is NULL (Hive semantics).
+ * SUM(LONG) wraps modulo 2^64 like a Java long. SUM/AVG(DOUBLE) are accumulated exactly
+ * in a 128-bit fixed-point sum per group (order-independent, reproducible), then rounded
+ * to nearest. */
+int ghive_groupby(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel, uint64_t n,
+                  const uint32_t *keys, uint32_t nkeys, const ghive_agg *aggs, uint32_t naggs,
+                  ghive_column *out_cols, uint64_t capacity, uint64_t *out_groups);
+
+/* ---- sort --------------------------------------------------------------------------- */
+/* SinkOperator ORDER BY (SinkOperator.cu:135-288; orders as SortOrder '+'/'-',
+ * GroupByPredicate.hpp:17-20): LSD radix sort; writes the permutation of row ids so that
+ * rows are in key order, ties in ascending row id (a total order). Keys: INT, LONG,
+ * DOUBLE, FLOAT columns; NULL sorts as its sentinel value (NaN last ascending). */
+int ghive_sort(ghive_ctx *ctx, const ghive_table *table, const uint32_t *key_cols,
+               const char *orders, uint32_t nkeys, uint32_t *out_perm);
+
+/* ---- fused fragment: filter -> star joins -> expression -> group-by ------------------- *
+ * One Tez map-task fragment of GHive's CS1 call stack (TableScan -> Filter -> MapJoin* ->
+ * Select -> GroupBy -> ReduceSink), executed as one streaming pass over the fact columns. */
+enum { GHIVE_SRC_FACT = 0, GHIVE_SRC_JOIN = 1 };
+enum { GHIVE_EXPR_COL = 0, GHIVE_EXPR_MUL = 1, GHIVE_EXPR_ONE = 2 };
+
+typedef struct ghive_fragment_join {
+  uint32_t fact_key_col;        /* fact column probed (INT/LONG)                       */
+  const ghive_hash_table *table; /* built with unique keys; payload = dimension attribute */
+} ghive_fragment_join;
+
+typedef struct ghive_fragment_key {
+  int32_t source; /* GHIVE_SRC_FACT: fact column `index`; GHIVE_SRC_JOIN: payload of join */
+  uint32_t index;
+} ghive_fragment_key;
+
+/* Aggregate over a fact expression: EXPR_COL a; EXPR_MUL a*b (int64 product of INT/LONG
+ * columns, as SelectPredicate's LONG multiply); EXPR_ONE counts rows. */
+typedef struct ghive_fragment_agg {
+  int32_t type; /* GHIVE_AGG_SUM / CNT / MIN / MAX */
+  int32_t expr;
+  uint32_t a;
+  uint32_t b;
+} ghive_fragment_agg;
+
+typedef struct ghive_fragment {
+  ghive_table fact;
+  const ghive_pred *filter; /* optional fact filter; nnodes 0 = none */
+  uint32_t filter_nnodes;
+  uint32_t filter_root;
+  const ghive_fragment_join *joins;
+  uint32_t njoins;
+  const ghive_fragment_key *keys; /* INT payload/columns; at most 2 keys */
+  uint32_t nkeys;
+  const ghive_fragment_agg *aggs;
+  uint32_t naggs;
+} ghive_fragment;
+
+/* Output: keys (INT each) then aggregates (LONG) into out_cols of `capacity` rows. */
+int ghive_fragment_execute(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_cols,
+                           uint64_t capacity, uint64_t *out_groups);
+
+/* Per-stage timing of the last fragment on this context (HIP events, milliseconds). */
+typedef struct ghive_stage_times {
+  float total_ms;
+  float main_kernel_ms; /* the fused scan/probe/aggregate kernel */
+  float finalize_ms;
+  uint64_t rows_in;
+  uint64_t rows_joined;
+} ghive_stage_times;
+int ghive_last_stage_times(ghive_ctx *ctx, ghive_stage_times *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GHIVE_GHIVE_H */

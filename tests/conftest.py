@@ -1,0 +1,38 @@
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP library)")
+
+
+@pytest.fixture(scope="session")
+def kats():
+    with open(os.path.join(GOLDEN, "reference_kats.json")) as fh:
+        return json.load(fh)
+
+
+@pytest.fixture(scope="session")
+def orc():
+    import oracle
+    oracle.build()
+    return oracle
+
+
+def np_col(spec):
+    """Fixture column spec -> numpy array (or ('string', blob, idx))."""
+    t = spec["type"]
+    if t == "STRING":
+        return ("string", spec["blob"].encode(), np.array(spec["idx"], dtype=np.int32))
+    dt = {"LONG": np.int64, "INT": np.int32, "DOUBLE": np.float64, "FLOAT": np.float32}[t]
+    return np.array(spec["values"], dtype=dt)
