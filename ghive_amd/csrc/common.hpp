@@ -1,0 +1,186 @@
+// Shared host-side infrastructure of libghive_mi355x.so: the context (stream, last error,
+// device memory manager, stage timers) and the status/error helpers used by every entry point.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ghive/ghive.h"
+
+namespace ghive {
+
+// Device memory manager. The reference has none (every cudaMalloc in gpu_impl leaks); here all
+// intermediates come from chunks obtained once with hipMalloc and recycled through a first-fit
+// free list with coalescing, so repeated fragments (batched and serving forms) do not touch the
+// HIP allocator and the footprint is accounted for admission control.
+class DeviceArena {
+ public:
+  explicit DeviceArena(uint64_t limit) : limit_(limit) {}
+  ~DeviceArena();
+  // Returns nullptr when the limit would be exceeded or hipMalloc fails.
+  void *alloc(uint64_t bytes);
+  bool free(void *p);
+  void release_all();
+  uint64_t in_use() const { return in_use_; }
+  uint64_t peak() const { return peak_; }
+  uint64_t limit() const { return limit_; }
+  uint64_t reserved() const { return reserved_; }
+  void set_limit(uint64_t l) { limit_ = l; }
+
+ private:
+  struct Chunk {
+    char *base;
+    uint64_t size;
+  };
+  std::mutex mu_;
+  std::vector<Chunk> chunks_;
+  std::map<uintptr_t, uint64_t> free_;  // address -> size (free blocks)
+  std::map<uintptr_t, uint64_t> used_;  // address -> size (live blocks)
+  uint64_t limit_, in_use_ = 0, peak_ = 0, reserved_ = 0;
+};
+
+}  // namespace ghive
+
+// Tunables of one context (set through ghive_set_option; the batched form's autotuner writes
+// them).
+struct ghive_tuning {
+  int64_t frag_lds_budget = 64 * 1024;  // bytes of LDS for the fragment's private group table
+  int64_t frag_late = 0;                // 1: read a column only for lanes holding live rows
+  int64_t frag_grid = 0;                // 0: one resident wave of workgroups
+  int64_t gb_lds_budget = 64 * 1024;    // bytes of LDS for the group-by's private table
+};
+
+struct ghive_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  ghive::DeviceArena *arena = nullptr;
+  hipDeviceProp_t props{};
+  int cus = 256;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  ghive_stage_times times{};
+  ghive_tuning tune{};
+  // reusable small device scratch for counters (8 x uint64) and its host mirror
+  uint64_t *d_counters = nullptr;
+};
+
+namespace ghive {
+
+int set_error(ghive_ctx *ctx, int code, const char *fmt, ...);
+
+#define GH_CHECK_CTX(ctx)                         \
+  do {                                            \
+    if (!(ctx)) return GHIVE_ERR_INVALID;         \
+    if (hipSetDevice((ctx)->device) != hipSuccess) \
+      return ghive::set_error(ctx, GHIVE_ERR_HIP, "hipSetDevice(%d) failed", (ctx)->device); \
+  } while (0)
+
+#define GH_HIP(ctx, call)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return ghive::set_error(ctx, GHIVE_ERR_HIP, "%s failed: %s (%s:%d)", #call,           \
+                              hipGetErrorString(e_), __FILE__, __LINE__);                   \
+  } while (0)
+
+#define GH_LAUNCH(ctx)                                                                      \
+  do {                                                                                      \
+    hipError_t e_ = hipGetLastError();                                                      \
+    if (e_ != hipSuccess)                                                                   \
+      return ghive::set_error(ctx, GHIVE_ERR_HIP, "kernel launch failed: %s (%s:%d)",       \
+                              hipGetErrorString(e_), __FILE__, __LINE__);                   \
+  } while (0)
+
+#define GH_TRY(expr)          \
+  do {                        \
+    int s_ = (expr);          \
+    if (s_ != GHIVE_OK) return s_; \
+  } while (0)
+
+// RAII device buffer from the context arena.
+class DevBuf {
+ public:
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : ctx_(o.ctx_), p_(o.p_) { o.p_ = nullptr; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) {
+      reset();
+      ctx_ = o.ctx_;
+      p_ = o.p_;
+      o.p_ = nullptr;
+    }
+    return *this;
+  }
+  ~DevBuf() { reset(); }
+  int alloc(ghive_ctx *ctx, uint64_t bytes) {
+    reset();
+    ctx_ = ctx;
+    p_ = ctx->arena->alloc(bytes ? bytes : 1);
+    if (!p_)
+      return set_error(ctx, GHIVE_ERR_OOM, "device memory manager: cannot allocate %llu bytes "
+                       "(in use %llu of limit %llu)", (unsigned long long)bytes,
+                       (unsigned long long)ctx->arena->in_use(), (unsigned long long)ctx->arena->limit());
+    return GHIVE_OK;
+  }
+  void reset() {
+    if (p_ && ctx_) ctx_->arena->free(p_);
+    p_ = nullptr;
+  }
+  void *release() {
+    void *p = p_;
+    p_ = nullptr;
+    return p;
+  }
+  template <class T>
+  T *as() const { return static_cast<T *>(p_); }
+  void *get() const { return p_; }
+
+ private:
+  ghive_ctx *ctx_ = nullptr;
+  void *p_ = nullptr;
+};
+
+inline int type_width(int t) {
+  switch (t) {
+    case GHIVE_LONG: case GHIVE_DOUBLE: return 8;
+    case GHIVE_INT: case GHIVE_FLOAT: return 4;
+    default: return 0;
+  }
+}
+inline bool is_int_type(int t) { return t == GHIVE_INT || t == GHIVE_LONG; }
+inline bool is_numeric_type(int t) {
+  return t == GHIVE_INT || t == GHIVE_LONG || t == GHIVE_DOUBLE || t == GHIVE_FLOAT;
+}
+
+constexpr uint64_t kMaxRows = 0xFFFFFFFEull;  // row ids are uint32
+
+int check_table(ghive_ctx *ctx, const ghive_table *t, const char *what);
+// Synchronous device->host copy on the context stream.
+int d2h(ghive_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int h2d(ghive_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+// Launch helpers (defined in kernels_util.hip).
+int fill_u64(ghive_ctx *ctx, uint64_t *p, uint64_t value, uint64_t n);
+int fill_u32(ghive_ctx *ctx, uint32_t *p, uint32_t value, uint64_t n);
+// Exclusive scan of n uint32 values into out (out may alias in); writes the total to
+// *d_total (device) when non-null.
+int exclusive_scan_u32(ghive_ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n,
+                       uint64_t *d_total);
+
+inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+inline uint64_t next_pow2(uint64_t v) {
+  uint64_t p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+}  // namespace ghive

@@ -1,0 +1,196 @@
+// Device-side building blocks shared by the kernels: wave64 helpers, typed column access with the
+// reference's NULL sentinels, the flattened predicate program, and Philox4x32-10.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/ghive/ghive.h"
+
+namespace ghive {
+
+constexpr int kWave = 64;
+
+// ---- wave64 helpers ---------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+// number of set bits of `mask` in lanes below this one
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+__device__ __forceinline__ int first_lane(uint64_t mask) { return __ffsll((unsigned long long)mask) - 1; }
+__device__ __forceinline__ uint64_t read_lane_u64(uint64_t v, int lane) {
+  uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, lane);
+  uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), lane);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint32_t read_lane_u32(uint32_t v, int lane) {
+  return __builtin_amdgcn_readlane(v, lane);
+}
+
+// Wave-aggregated append: every lane with `want` gets a distinct slot of *counter (one atomic
+// per wave). Returns the slot (valid only when want).
+__device__ __forceinline__ uint64_t wave_append(unsigned long long *counter, bool want) {
+  uint64_t m = ballot(want);
+  if (!m) return 0;
+  int leader = first_lane(m);
+  uint64_t base = 0;
+  if ((int)lane_id() == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+  base = read_lane_u64(base, leader);
+  return base + lanes_below(m);
+}
+
+// ---- columns and NULL sentinels (gpu_impl/inc/DataFlow/Table.hpp:29-52) -------------------
+__device__ __forceinline__ bool is_null_i32(int32_t v) { return v == 0x7fffffff; }
+__device__ __forceinline__ bool is_null_i64(int64_t v) { return v == 0x7fffffffffffffffll; }
+
+__device__ __forceinline__ bool col_load_i64(const void *p, int type, uint64_t r, int64_t &v) {
+  if (type == GHIVE_INT) {
+    int32_t x = static_cast<const int32_t *>(p)[r];
+    v = x;
+    return !is_null_i32(x);
+  }
+  int64_t x = static_cast<const int64_t *>(p)[r];
+  v = x;
+  return !is_null_i64(x);
+}
+
+__device__ __forceinline__ bool col_load_f64(const void *p, int type, uint64_t r, double &v) {
+  switch (type) {
+    case GHIVE_DOUBLE: v = static_cast<const double *>(p)[r]; return !__builtin_isnan(v);
+    case GHIVE_FLOAT: v = (double)static_cast<const float *>(p)[r]; return !__builtin_isnan(v);
+    case GHIVE_INT: {
+      int32_t x = static_cast<const int32_t *>(p)[r];
+      v = (double)x;
+      return !is_null_i32(x);
+    }
+    default: {
+      int64_t x = static_cast<const int64_t *>(p)[r];
+      v = (double)x;
+      return !is_null_i64(x);
+    }
+  }
+}
+
+// ---- flattened predicate program ----------------------------------------------------------
+constexpr int kMaxPredNodes = 32;
+constexpr int kMaxPredCols = 8;
+
+enum DevPredOp : int8_t {
+  kLeafInt = 0,   // column (as int64) OP literal(s)
+  kLeafDbl = 1,   // column (as double) OP literal(s)
+  kColColInt = 2,
+  kColColDbl = 3,
+  kNotNull = 4,
+  kAnd = 5,
+  kOr = 6,
+};
+
+struct DevPredNode {
+  int8_t op;
+  int8_t mode;
+  int8_t a;  // column slot, or child node index for AND/OR
+  int8_t b;
+  int32_t pad;
+  int64_t ia, ib;
+  double da, db;
+};
+
+// Nodes in evaluation order (children first); the value of the last node is the predicate.
+struct DevPred {
+  int32_t nnodes;
+  int32_t ncols;
+  const void *col_data[kMaxPredCols];
+  int32_t col_type[kMaxPredCols];
+  DevPredNode nodes[kMaxPredNodes];
+};
+
+template <class T>
+__device__ __forceinline__ bool cmp_mode(int mode, T x, T a, T b) {
+  switch (mode) {
+    case GHIVE_FILTER_EQ: return x == a;
+    case GHIVE_FILTER_LE: return x <= a;
+    case GHIVE_FILTER_GE: return x >= a;
+    case GHIVE_FILTER_LT: return x < a;
+    case GHIVE_FILTER_GT: return x > a;
+    case GHIVE_FILTER_RANGE: return x > a && x < b;
+    case GHIVE_FILTER_EQ_RANGE: return x >= a && x <= b;
+    case GHIVE_FILTER_NOT_EQ: return x != a;
+    case GHIVE_FILTER_NOT_RANGE: return x < a || x > b;
+    case GHIVE_FILTER_NOT_EQ_RANGE: return x <= a || x >= b;
+    default: return true;
+  }
+}
+
+__device__ __forceinline__ bool eval_pred(const DevPred &P, uint64_t r) {
+  uint32_t bits = 0;
+  for (int i = 0; i < P.nnodes; ++i) {
+    const DevPredNode &n = P.nodes[i];
+    bool v;
+    switch (n.op) {
+      case kLeafInt: {
+        int64_t x;
+        v = col_load_i64(P.col_data[n.a], P.col_type[n.a], r, x) && cmp_mode<int64_t>(n.mode, x, n.ia, n.ib);
+        break;
+      }
+      case kLeafDbl: {
+        double x;
+        v = col_load_f64(P.col_data[n.a], P.col_type[n.a], r, x) && cmp_mode<double>(n.mode, x, n.da, n.db);
+        break;
+      }
+      case kColColInt: {
+        int64_t x, y;
+        bool ok = col_load_i64(P.col_data[n.a], P.col_type[n.a], r, x);
+        ok = col_load_i64(P.col_data[n.b], P.col_type[n.b], r, y) && ok;
+        v = ok && cmp_mode<int64_t>(n.mode, x, y, 0);
+        break;
+      }
+      case kColColDbl: {
+        double x, y;
+        bool ok = col_load_f64(P.col_data[n.a], P.col_type[n.a], r, x);
+        ok = col_load_f64(P.col_data[n.b], P.col_type[n.b], r, y) && ok;
+        v = ok && cmp_mode<double>(n.mode, x, y, 0.0);
+        break;
+      }
+      case kNotNull: {
+        double x;
+        v = col_load_f64(P.col_data[n.a], P.col_type[n.a], r, x);
+        break;
+      }
+      case kAnd: v = ((bits >> n.a) & 1u) && ((bits >> n.b) & 1u); break;
+      default: v = ((bits >> n.a) & 1u) || ((bits >> n.b) & 1u); break;
+    }
+    bits |= (uint32_t)v << i;
+  }
+  return (bits >> (P.nnodes - 1)) & 1u;
+}
+
+// ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) -------------------------------------
+__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+    uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+    uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// ---- hashing -------------------------------------------------------------------------------
+// Fibonacci hashing: multiply by 2^64/phi and keep the top bits (Knuth, TAOCP 6.4).
+__device__ __forceinline__ uint64_t hash_u64(uint64_t k, int log2_slots) {
+  return (k * 0x9E3779B97F4A7C15ull) >> (64 - log2_slots);
+}
+__device__ __forceinline__ uint32_t hash_u32(uint32_t k, int log2_slots) {
+  return (uint32_t)(((uint64_t)(k * 0x9E3779B1u)) >> (32 - log2_slots));
+}
+
+}  // namespace ghive

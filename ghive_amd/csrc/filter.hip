@@ -1,0 +1,156 @@
+// Filter with order-preserving stream compaction (FilterPredicate::process + clean_data,
+// gpu_impl/src/Operator/FilterPredicate.cu:434-571, FilterOperator.cu:36-93).
+//
+// Pass 1 evaluates the whole predicate tree per row in registers (one coalesced read of each
+// referenced column) and stores one 64-bit wave ballot per 64 rows plus a popcount per 4096-row
+// tile. Pass 2 scans the tile counts. Pass 3 re-reads only the ballots (n/8 bytes) and writes the
+// selected row ids in ascending order. No decoupled look-back: nothing waits on another
+// workgroup.
+#include "common.hpp"
+#include "device.hpp"
+#include "filter.hpp"
+#include "pred_host.hpp"
+#include "scan.hpp"
+
+namespace ghive {
+
+constexpr int kFilterThreads = 256;
+constexpr int kFilterRowsPerThread = 16;
+constexpr uint64_t kFilterTile = (uint64_t)kFilterThreads * kFilterRowsPerThread;  // 4096
+constexpr uint64_t kWordsPerTile = kFilterTile / 64;                               // 64
+
+__global__ __launch_bounds__(kFilterThreads) void k_filter_mask(DevPred P, uint64_t n,
+                                                                uint64_t *masks, uint32_t *tile_counts) {
+  __shared__ uint32_t lds[kFilterThreads / kWave];
+  const uint64_t tile0 = (uint64_t)blockIdx.x * kFilterTile;
+  const uint32_t wave = threadIdx.x / kWave;
+  uint32_t cnt = 0;
+#pragma unroll 4
+  for (int j = 0; j < kFilterRowsPerThread; ++j) {
+    uint64_t r = tile0 + (uint64_t)j * kFilterThreads + threadIdx.x;
+    bool pass = r < n && eval_pred(P, r);
+    uint64_t m = ballot(pass);
+    uint64_t word = (tile0 >> 6) + (uint64_t)j * (kFilterThreads / kWave) + wave;
+    if (lane_id() == 0 && (word << 6) < n) masks[word] = m;
+    cnt += pass;
+  }
+  uint32_t total;
+  (void)block_exclusive_scan_256(cnt, lds, total);
+  if (threadIdx.x == 0) tile_counts[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kFilterThreads) void k_filter_write(const uint64_t *masks, uint64_t nwords,
+                                                                 const uint32_t *tile_offs, uint32_t *out) {
+  __shared__ uint32_t word_pre[kWordsPerTile];
+  const uint64_t w0 = (uint64_t)blockIdx.x * kWordsPerTile;
+  if (threadIdx.x < kWordsPerTile) {
+    uint64_t w = w0 + threadIdx.x;
+    uint32_t c = w < nwords ? (uint32_t)__popcll(masks[w]) : 0u;
+    uint32_t inc = wave_inclusive_scan(c);  // threads 0..63 are exactly wave 0
+    word_pre[threadIdx.x] = inc - c;
+  }
+  __syncthreads();
+  const uint32_t base = tile_offs[blockIdx.x];
+  const uint32_t wave = threadIdx.x / kWave, lane = lane_id();
+  for (uint32_t q = wave; q < kWordsPerTile; q += kFilterThreads / kWave) {
+    uint64_t w = w0 + q;
+    if (w >= nwords) break;
+    uint64_t m = masks[w];
+    if ((m >> lane) & 1ull) {
+      uint32_t below = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+      out[base + word_pre[q] + below] = (uint32_t)((w << 6) + lane);
+    }
+  }
+}
+
+template <class T>
+__global__ void k_gather(const T *in, const uint32_t *sel, uint64_t n, T *out) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) out[i] = in[sel[i]];
+}
+
+int filter_select(ghive_ctx *ctx, const ghive_table *t, const ghive_pred *nodes, uint32_t nnodes,
+                  uint32_t root, uint32_t *out_sel, uint64_t *out_count) {
+  DevPred P;
+  GH_TRY(flatten_pred(ctx, t, nodes, nnodes, root, P));
+  const uint64_t n = t->row_num;
+  *out_count = 0;
+  if (n == 0) return GHIVE_OK;
+  const uint32_t ntiles = ceil_div(n, kFilterTile);
+  const uint64_t nwords = (n + 63) / 64;
+  DevBuf masks, counts;
+  GH_TRY(masks.alloc(ctx, sizeof(uint64_t) * nwords));
+  GH_TRY(counts.alloc(ctx, sizeof(uint32_t) * ntiles));
+  k_filter_mask<<<ntiles, kFilterThreads, 0, ctx->stream>>>(P, n, masks.as<uint64_t>(), counts.as<uint32_t>());
+  GH_LAUNCH(ctx);
+  GH_TRY(exclusive_scan_u32(ctx, counts.as<uint32_t>(), counts.as<uint32_t>(), ntiles, ctx->d_counters));
+  k_filter_write<<<ntiles, kFilterThreads, 0, ctx->stream>>>(masks.as<uint64_t>(), nwords,
+                                                             counts.as<uint32_t>(), out_sel);
+  GH_LAUNCH(ctx);
+  GH_TRY(d2h(ctx, out_count, ctx->d_counters, sizeof(uint64_t)));
+  return GHIVE_OK;
+}
+
+int gather_columns(ghive_ctx *ctx, const ghive_table *t, const uint32_t *sel, uint64_t n,
+                   ghive_column *out_cols) {
+  for (uint32_t c = 0; c < t->ncols; ++c) {
+    const ghive_column &in = t->columns[c];
+    ghive_column &o = out_cols[c];
+    if (!is_numeric_type(in.type))
+      return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "gather: STRING columns are not supported");
+    o.type = in.type;
+    o.row_num = n;
+    if (n == 0) continue;
+    if (!o.data) return set_error(ctx, GHIVE_ERR_INVALID, "gather: output column %u has no buffer", c);
+    uint64_t g = (n + 255) / 256, cap = (uint64_t)ctx->cus * 16;
+    uint32_t grid = (uint32_t)(g > cap ? cap : g);
+    if (type_width(in.type) == 8)
+      k_gather<uint64_t><<<grid, 256, 0, ctx->stream>>>((const uint64_t *)in.data, sel, n, (uint64_t *)o.data);
+    else
+      k_gather<uint32_t><<<grid, 256, 0, ctx->stream>>>((const uint32_t *)in.data, sel, n, (uint32_t *)o.data);
+    GH_LAUNCH(ctx);
+  }
+  return GHIVE_OK;
+}
+
+}  // namespace ghive
+
+using namespace ghive;
+
+extern "C" {
+
+int ghive_filter(ghive_ctx *ctx, const ghive_table *table, const ghive_pred *nodes, uint32_t nnodes,
+                 uint32_t root, uint32_t *out_sel, uint64_t *out_count) {
+  GH_CHECK_CTX(ctx);
+  GH_TRY(check_table(ctx, table, "filter"));
+  if (!out_count) return set_error(ctx, GHIVE_ERR_INVALID, "filter: out_count is NULL");
+  if (table->row_num && !out_sel) return set_error(ctx, GHIVE_ERR_INVALID, "filter: out_sel is NULL");
+  return filter_select(ctx, table, nodes, nnodes, root, out_sel, out_count);
+}
+
+int ghive_filter_compact(ghive_ctx *ctx, const ghive_table *table, const ghive_pred *nodes,
+                         uint32_t nnodes, uint32_t root, ghive_column *out_cols, uint64_t *out_count) {
+  GH_CHECK_CTX(ctx);
+  GH_TRY(check_table(ctx, table, "filter_compact"));
+  if (!out_count || (table->ncols && !out_cols))
+    return set_error(ctx, GHIVE_ERR_INVALID, "filter_compact: NULL output");
+  DevBuf sel;
+  GH_TRY(sel.alloc(ctx, sizeof(uint32_t) * (table->row_num ? table->row_num : 1)));
+  GH_TRY(filter_select(ctx, table, nodes, nnodes, root, sel.as<uint32_t>(), out_count));
+  GH_TRY(gather_columns(ctx, table, sel.as<uint32_t>(), *out_count, out_cols));
+  GH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GHIVE_OK;
+}
+
+int ghive_gather(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel, uint64_t n,
+                 ghive_column *out_cols) {
+  GH_CHECK_CTX(ctx);
+  GH_TRY(check_table(ctx, table, "gather"));
+  if (n && (!sel || !out_cols)) return set_error(ctx, GHIVE_ERR_INVALID, "gather: NULL argument");
+  GH_TRY(gather_columns(ctx, table, sel, n, out_cols));
+  GH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GHIVE_OK;
+}
+
+}  // extern "C"

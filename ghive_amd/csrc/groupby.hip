@@ -1,0 +1,416 @@
+// Hash group-by (GroupByPredicate::gpu_execute, gpu_impl/src/Operator/GroupByPredicate.cu:19-359,
+// which sorts row ids with a comparator and runs reduce_by_key; here one pass aggregates into
+// LDS-private tables per workgroup that spill to and merge into one global table).
+#include <cmath>
+#include <cstring>
+
+#include "agg.hpp"
+#include "common.hpp"
+#include "device.hpp"
+#include "groupby.hpp"
+#include "scan.hpp"
+
+namespace ghive {
+
+constexpr int kGbThreads = 256;
+
+__device__ __forceinline__ uint64_t pack_key(const KeySpec &K, uint64_t r) {
+  uint64_t key = 0;
+  for (int k = 0; k < K.nkeys; ++k) {
+    uint64_t bits;
+    switch (K.type[k]) {
+      case GHIVE_INT: bits = (uint32_t) static_cast<const int32_t *>(K.data[k])[r]; break;
+      case GHIVE_LONG: bits = (uint64_t) static_cast<const int64_t *>(K.data[k])[r]; break;
+      case GHIVE_DOUBLE: {
+        double d = static_cast<const double *>(K.data[k])[r];
+        bits = __builtin_isnan(d) ? 0x7ff8000000000000ull : (uint64_t)__double_as_longlong(d);
+        break;
+      }
+      default: {
+        float f = static_cast<const float *>(K.data[k])[r];
+        bits = __builtin_isnan(f) ? 0x7fc00000u : (uint32_t)__float_as_uint(f);
+        break;
+      }
+    }
+    key |= bits << K.shift[k];
+  }
+  return key;
+}
+
+__device__ __forceinline__ AggVal agg_value(const AggSpec &A, uint64_t r) {
+  AggVal v;
+  v.valid = false;
+  v.neg = false;
+  v.a = v.b = 0;
+  if (A.col < 0) {
+    v.valid = true;
+    return v;
+  }
+  switch (A.kind) {
+    case AK_CNT: {
+      double d;
+      v.valid = col_load_f64(A.data, A.in_type, r, d);
+      break;
+    }
+    case AK_SUMI:
+    case AK_MINI:
+    case AK_MAXI: {
+      int64_t x;
+      v.valid = col_load_i64(A.data, A.in_type, r, x);
+      v.a = (uint64_t)x;
+      break;
+    }
+    case AK_MIND:
+    case AK_MAXD: {
+      double d;
+      v.valid = col_load_f64(A.data, A.in_type, r, d);
+      v.a = dbl_order(d);
+      break;
+    }
+    default: {
+      double d;
+      v.valid = col_load_f64(A.data, A.in_type, r, d);
+      if (v.valid) v.neg = to_fixed(d, A.scale_e, v.a, v.b);
+      break;
+    }
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(kGbThreads) void k_groupby(KeySpec K, GroupTableDev T, const uint32_t *sel,
+                                                        uint64_t n, int lcap_log2) {
+  extern __shared__ uint64_t lds[];
+  const int lstride = (1 << lcap_log2) + 1;
+  lds_table_init(lds, T, lcap_log2);
+  __syncthreads();
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    const uint64_t r = sel ? sel[i] : i;
+    const uint64_t key = pack_key(K, r);
+    const int ls = lds_find_insert(lds, key, lcap_log2);
+    if (ls >= 0) {
+      for (int a = 0; a < T.naggs; ++a) {
+        const AggSpec &A = T.aggs[a];
+        AggVal v = agg_value(A, r);
+        acc_apply(A, [&](int j) { return lds + (size_t)(1 + A.word0 + j) * lstride + ls; }, v);
+      }
+    } else {
+      int64_t g = gt_find_insert(T, key);
+      if (g < 0) continue;
+      for (int a = 0; a < T.naggs; ++a) {
+        const AggSpec &A = T.aggs[a];
+        AggVal v = agg_value(A, r);
+        acc_apply(A, [&](int j) { return T.acc + (size_t)(A.word0 + j) * T.stride + g; }, v);
+      }
+    }
+  }
+  __syncthreads();
+  lds_table_flush(lds, T, lcap_log2);
+}
+
+__global__ void k_gt_preinsert(GroupTableDev T, uint64_t key) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) (void)gt_find_insert(T, key);
+}
+
+// max |x| over the selected rows of a floating column, as ordered bits of a non-negative double
+__global__ void k_absmax(const void *data, int type, const uint32_t *sel, uint64_t n, unsigned long long *out) {
+  uint64_t m = 0;
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    double d;
+    if (col_load_f64(data, type, sel ? sel[i] : i, d)) {
+      uint64_t b = (uint64_t)__double_as_longlong(fabs(d));
+      m = b > m ? b : m;
+    }
+  }
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    uint64_t o = __shfl_xor((unsigned long long)m, d, kWave);
+    m = o > m ? o : m;
+  }
+  if (lane_id() == 0 && m) atomicMax(out, (unsigned long long)m);
+}
+
+__device__ __forceinline__ void write_typed(void *col, int type, uint64_t idx, int64_t iv, double dv, bool is_int) {
+  switch (type) {
+    case GHIVE_LONG: static_cast<int64_t *>(col)[idx] = is_int ? iv : (int64_t)dv; break;
+    case GHIVE_INT: static_cast<int32_t *>(col)[idx] = (int32_t)iv; break;
+    case GHIVE_DOUBLE: static_cast<double *>(col)[idx] = is_int ? (double)iv : dv; break;
+    default: static_cast<float *>(col)[idx] = (float)(is_int ? (double)iv : dv); break;
+  }
+}
+
+__device__ __forceinline__ void write_null(void *col, int type, uint64_t idx) {
+  switch (type) {
+    case GHIVE_LONG: static_cast<int64_t *>(col)[idx] = 0x7fffffffffffffffll; break;
+    case GHIVE_INT: static_cast<int32_t *>(col)[idx] = 0x7fffffff; break;
+    case GHIVE_DOUBLE: static_cast<double *>(col)[idx] = __longlong_as_double(0x7ff8000000000000ll); break;
+    default: static_cast<float *>(col)[idx] = __uint_as_float(0x7fc00000u); break;
+  }
+}
+
+__device__ __forceinline__ void finalize_agg(const AggSpec &A, const uint64_t *acc, uint64_t stride, uint64_t s,
+                                             void *col, uint64_t idx) {
+  auto W = [&](int j) { return acc[(size_t)(A.word0 + j) * stride + s]; };
+  switch (A.kind) {
+    case AK_CNT: static_cast<int64_t *>(col)[idx] = (int64_t)W(0); return;
+    case AK_SUMI: {
+      uint64_t c = W(1);
+      if (!c) return write_null(col, A.out_type, idx);
+      if (A.is_avg) static_cast<double *>(col)[idx] = (double)(int64_t)W(0) / (double)c;
+      else static_cast<int64_t *>(col)[idx] = (int64_t)W(0);
+      return;
+    }
+    case AK_MINI:
+    case AK_MAXI: {
+      if (!W(1)) return write_null(col, A.out_type, idx);
+      write_typed(col, A.out_type, idx, (int64_t)W(0), 0.0, true);
+      return;
+    }
+    case AK_MIND:
+    case AK_MAXD: {
+      if (!W(1)) return write_null(col, A.out_type, idx);
+      write_typed(col, A.out_type, idx, 0, dbl_unorder(W(0)), false);
+      return;
+    }
+    default: {
+      uint64_t c = W(4);
+      if (!c) return write_null(col, A.out_type, idx);
+      uint64_t plo = W(0), phi = W(1), nlo = W(2), nhi = W(3);
+      bool neg = nhi > phi || (nhi == phi && nlo > plo);
+      uint64_t lo, hi;
+      if (!neg) {
+        lo = plo - nlo;
+        hi = phi - nhi - (plo < nlo);
+      } else {
+        lo = nlo - plo;
+        hi = nhi - phi - (nlo < plo);
+      }
+      double d = fixed_to_double(lo, hi, A.scale_e);
+      if (neg) d = -d;
+      if (A.is_avg) d /= (double)c;
+      static_cast<double *>(col)[idx] = d;
+      return;
+    }
+  }
+}
+
+__global__ void k_groupby_finalize(KeySpec K, GroupTableDev T, OutSpec O, unsigned long long *counter, uint64_t cap) {
+  const uint64_t total = T.stride;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t iters = (total + stride - 1) / stride;
+  for (uint64_t it = 0; it < iters; ++it) {
+    uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x + it * stride;
+    bool used = false;
+    uint64_t key = 0;
+    if (s < total) {
+      key = T.keys[s];
+      used = (s == total - 1) ? key == 1ull : key != kEmptyKey;
+      if (s == total - 1) key = kEmptyKey;
+    }
+    uint64_t idx = wave_append(counter, used);
+    if (!used || idx >= cap) continue;
+    for (int k = 0; k < K.nkeys; ++k) {
+      uint64_t bits = key >> K.shift[k];
+      switch (K.type[k]) {
+        case GHIVE_INT: static_cast<int32_t *>(O.col[k])[idx] = (int32_t)(uint32_t)bits; break;
+        case GHIVE_LONG: static_cast<int64_t *>(O.col[k])[idx] = (int64_t)bits; break;
+        case GHIVE_DOUBLE: static_cast<uint64_t *>(O.col[k])[idx] = bits; break;
+        default: static_cast<uint32_t *>(O.col[k])[idx] = (uint32_t)bits; break;
+      }
+    }
+    for (int a = 0; a < T.naggs; ++a) finalize_agg(T.aggs[a], T.acc, T.stride, s, O.col[K.nkeys + a], idx);
+  }
+}
+
+int lds_log2_for(int nwords, int budget_bytes) {
+  int lg = 12;
+  while (lg > 4 && (((1 << lg) + 1) * (1 + nwords) * 8) > budget_bytes) --lg;
+  return lg;
+}
+
+int group_table_alloc(ghive_ctx *ctx, GroupTableDev &T, uint64_t capacity, DevBuf &keys, DevBuf &acc,
+                      DevBuf &counters) {
+  uint64_t slots = next_pow2(capacity * 2 < 64 ? 64 : capacity * 2);
+  T.stride = slots + 1;
+  T.mask = slots - 1;
+  T.log2 = __builtin_ctzll(slots);
+  T.capacity = capacity;
+  GH_TRY(keys.alloc(ctx, T.stride * 8));
+  GH_TRY(acc.alloc(ctx, T.stride * 8 * (uint64_t)(T.nwords ? T.nwords : 1)));
+  GH_TRY(counters.alloc(ctx, 2 * sizeof(unsigned long long)));
+  T.keys = keys.as<uint64_t>();
+  T.acc = acc.as<uint64_t>();
+  T.nent = counters.as<unsigned long long>();
+  T.overflow = T.nent + 1;
+  GH_TRY(fill_u64(ctx, T.keys, kEmptyKey, slots));
+  GH_TRY(fill_u64(ctx, T.keys + slots, 0, 1));
+  GH_TRY(fill_u64(ctx, reinterpret_cast<uint64_t *>(T.nent), 0, 2));
+  for (int a = 0; a < T.naggs; ++a) {
+    const AggSpec &A = T.aggs[a];
+    for (int w = 0; w < agg_words(A.kind); ++w) {
+      uint64_t v = 0;
+      if (w == 0 && A.kind == AK_MINI) v = 0x7fffffffffffffffull;
+      if (w == 0 && A.kind == AK_MAXI) v = 0x8000000000000000ull;
+      if (w == 0 && A.kind == AK_MIND) v = ~0ull;
+      GH_TRY(fill_u64(ctx, T.acc + (uint64_t)(A.word0 + w) * T.stride, v, T.stride));
+    }
+  }
+  return GHIVE_OK;
+}
+
+int group_table_preinsert(ghive_ctx *ctx, const GroupTableDev &T, uint64_t key) {
+  k_gt_preinsert<<<1, 64, 0, ctx->stream>>>(T, key);
+  GH_LAUNCH(ctx);
+  return GHIVE_OK;
+}
+
+int group_table_emit(ghive_ctx *ctx, const KeySpec &K, const GroupTableDev &T, const OutSpec &O,
+                     uint64_t capacity, uint64_t *out_groups) {
+  GH_TRY(fill_u64(ctx, ctx->d_counters, 0, 1));
+  uint64_t g = (T.stride + 255) / 256, cap = (uint64_t)ctx->cus * 16;
+  k_groupby_finalize<<<(uint32_t)(g > cap ? cap : g), 256, 0, ctx->stream>>>(
+      K, T, O, reinterpret_cast<unsigned long long *>(ctx->d_counters), capacity);
+  GH_LAUNCH(ctx);
+  uint64_t host[2] = {0, 0};
+  GH_TRY(d2h(ctx, &host[0], ctx->d_counters, sizeof(uint64_t)));
+  GH_TRY(d2h(ctx, &host[1], T.overflow, sizeof(uint64_t)));
+  *out_groups = host[0];
+  if (host[1] || host[0] > capacity) {
+    if (*out_groups <= capacity) *out_groups = capacity + 1;
+    return set_error(ctx, GHIVE_ERR_CAPACITY, "group-by: more than %llu groups (capacity)",
+                     (unsigned long long)capacity);
+  }
+  return GHIVE_OK;
+}
+
+}  // namespace ghive
+
+using namespace ghive;
+
+extern "C" {
+
+int ghive_groupby(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel, uint64_t n,
+                  const uint32_t *keys, uint32_t nkeys, const ghive_agg *aggs, uint32_t naggs,
+                  ghive_column *out_cols, uint64_t capacity, uint64_t *out_groups) {
+  GH_CHECK_CTX(ctx);
+  GH_TRY(check_table(ctx, table, "groupby"));
+  if (!out_groups) return set_error(ctx, GHIVE_ERR_INVALID, "groupby: out_groups is NULL");
+  if (!sel && n != table->row_num)
+    return set_error(ctx, GHIVE_ERR_INVALID, "groupby: n != rows without a selection");
+  if (nkeys > (uint32_t)kMaxKeys) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "groupby: %u keys (max %d)", nkeys, kMaxKeys);
+  if (naggs > (uint32_t)kMaxAggs) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "groupby: %u aggregates (max %d)", naggs, kMaxAggs);
+  if ((nkeys && !keys) || (naggs && !aggs)) return set_error(ctx, GHIVE_ERR_INVALID, "groupby: NULL keys/aggs");
+  if (nkeys + naggs && !out_cols) return set_error(ctx, GHIVE_ERR_INVALID, "groupby: out_cols is NULL");
+  KeySpec K;
+  memset(&K, 0, sizeof K);
+  K.nkeys = (int)nkeys;
+  int shift = 0;
+  for (uint32_t k = 0; k < nkeys; ++k) {
+    if (keys[k] >= table->ncols) return set_error(ctx, GHIVE_ERR_INVALID, "groupby: key column %u", keys[k]);
+    const ghive_column &c = table->columns[keys[k]];
+    if (!is_numeric_type(c.type))
+      return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "groupby: STRING keys are not supported");
+    K.type[k] = c.type;
+    K.data[k] = c.data;
+    K.shift[k] = shift;
+    shift += 8 * type_width(c.type);
+  }
+  if (shift > 64)
+    return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "groupby: keys are %d bits wide (max 64)", shift);
+  GroupTableDev T;
+  memset(&T, 0, sizeof T);
+  T.naggs = (int)naggs;
+  int word = 0;
+  for (uint32_t a = 0; a < naggs; ++a) {
+    AggSpec &A = T.aggs[a];
+    int ty = aggs[a].type;
+    bool star = ty == GHIVE_AGG_CNT && aggs[a].index == GHIVE_COUNT_STAR;
+    if (ty < GHIVE_AGG_SUM || ty > GHIVE_AGG_CNT)
+      return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "groupby: aggregation type %d", ty);
+    int in_type = GHIVE_LONG;
+    if (!star) {
+      if (aggs[a].index >= table->ncols)
+        return set_error(ctx, GHIVE_ERR_INVALID, "groupby: aggregate column %u", aggs[a].index);
+      in_type = table->columns[aggs[a].index].type;
+      if (!is_numeric_type(in_type))
+        return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "groupby: aggregates over STRING are not supported");
+      A.col = (int)aggs[a].index;
+      A.data = table->columns[aggs[a].index].data;
+    } else {
+      A.col = -1;
+    }
+    bool fp = in_type == GHIVE_DOUBLE || in_type == GHIVE_FLOAT;
+    A.in_type = (int8_t)in_type;
+    A.out_type = (int8_t)ghive_agg_output_type(ty, in_type);
+    A.is_avg = ty == GHIVE_AGG_AVG;
+    switch (ty) {
+      case GHIVE_AGG_CNT: A.kind = AK_CNT; break;
+      case GHIVE_AGG_SUM:
+      case GHIVE_AGG_AVG: A.kind = fp ? AK_SUMD : AK_SUMI; break;
+      case GHIVE_AGG_MIN: A.kind = fp ? AK_MIND : AK_MINI; break;
+      default: A.kind = fp ? AK_MAXD : AK_MAXI; break;
+    }
+    A.word0 = (int16_t)word;
+    word += agg_words(A.kind);
+    ghive_column &o = out_cols[nkeys + a];
+    if (capacity && !o.data) return set_error(ctx, GHIVE_ERR_INVALID, "groupby: output column %u has no buffer", nkeys + a);
+    o.type = A.out_type;
+  }
+  for (uint32_t k = 0; k < nkeys; ++k) {
+    if (capacity && !out_cols[k].data) return set_error(ctx, GHIVE_ERR_INVALID, "groupby: output column %u has no buffer", k);
+    out_cols[k].type = K.type[k];
+  }
+  T.nwords = word;
+  // exact double sums: pick the fixed-point LSB from max |x| and the row count
+  for (uint32_t a = 0; a < naggs; ++a) {
+    AggSpec &A = T.aggs[a];
+    if (A.kind != AK_SUMD) continue;
+    uint64_t m = 0;
+    if (n) {
+      GH_TRY(fill_u64(ctx, ctx->d_counters, 0, 1));
+      uint64_t g = (n + 255) / 256, cap = (uint64_t)ctx->cus * 8;
+      k_absmax<<<(uint32_t)(g > cap ? cap : g), 256, 0, ctx->stream>>>(A.data, A.in_type, sel, n,
+                                                                       reinterpret_cast<unsigned long long *>(ctx->d_counters));
+      GH_LAUNCH(ctx);
+      GH_TRY(d2h(ctx, &m, ctx->d_counters, sizeof m));
+    }
+    double mx;
+    memcpy(&mx, &m, sizeof mx);
+    if (!(mx > 0.0) || !std::isfinite(mx)) {
+      if (std::isinf(mx)) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "groupby: SUM over an infinite double");
+      A.scale_e = -1074;
+    } else {
+      int lg_n = 0;
+      while ((1ull << lg_n) < n + 1 && lg_n < 63) ++lg_n;
+      A.scale_e = std::ilogb(mx) + 1 + lg_n - 126;
+      if (A.scale_e < -1074) A.scale_e = -1074;
+    }
+  }
+  DevBuf dkeys, dacc, dcnt;
+  GH_TRY(group_table_alloc(ctx, T, capacity, dkeys, dacc, dcnt));
+  if (nkeys == 0) GH_TRY(group_table_preinsert(ctx, T, 0ull));  // a global aggregate: one group, even over no rows
+  if (n) {
+    int lg = lds_log2_for(T.nwords, (int)ctx->tune.gb_lds_budget);
+    size_t lds_bytes = (size_t)((1 << lg) + 1) * (1 + T.nwords) * 8;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_groupby, kGbThreads, lds_bytes) != hipSuccess || per_cu < 1)
+      per_cu = 1;
+    uint64_t want = (n + kGbThreads - 1) / kGbThreads, cap = (uint64_t)ctx->cus * per_cu;
+    k_groupby<<<(uint32_t)(want > cap ? cap : want), kGbThreads, lds_bytes, ctx->stream>>>(K, T, sel, n, lg);
+    GH_LAUNCH(ctx);
+  }
+  OutSpec O;
+  memset(&O, 0, sizeof O);
+  for (uint32_t c = 0; c < nkeys + naggs; ++c) {
+    O.col[c] = out_cols[c].data;
+    O.type[c] = out_cols[c].type;
+  }
+  int st = group_table_emit(ctx, K, T, O, capacity, out_groups);
+  for (uint32_t c = 0; c < nkeys + naggs; ++c) out_cols[c].row_num = st == GHIVE_OK ? *out_groups : 0;
+  return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,33 @@
+// Group-table helpers shared by the group-by operator and the fused fragment.
+#pragma once
+
+#include "agg.hpp"
+#include "common.hpp"
+
+namespace ghive {
+
+// Packing of up to kMaxKeys key columns into one uint64 (key k at bit K.shift[k]).
+struct KeySpec {
+  int32_t nkeys;
+  int32_t type[kMaxKeys];
+  int32_t shift[kMaxKeys];
+  const void *data[kMaxKeys];
+};
+
+struct OutSpec {
+  void *col[kMaxKeys + kMaxAggs];
+  int32_t type[kMaxKeys + kMaxAggs];
+};
+
+// Largest power-of-two LDS table (log2) whose keys + words fit in budget_bytes.
+int lds_log2_for(int nwords, int budget_bytes);
+// Allocates and initialises the global table for `capacity` groups (T.aggs/naggs/nwords set).
+int group_table_alloc(ghive_ctx *ctx, GroupTableDev &T, uint64_t capacity, DevBuf &keys, DevBuf &acc,
+                      DevBuf &counters);
+// Inserts `key` (so that a keyless aggregate yields its one group even over no rows).
+int group_table_preinsert(ghive_ctx *ctx, const GroupTableDev &T, uint64_t key);
+// Writes every group to the output columns; CAPACITY error when there are more than capacity.
+int group_table_emit(ghive_ctx *ctx, const KeySpec &K, const GroupTableDev &T, const OutSpec &O,
+                     uint64_t capacity, uint64_t *out_groups);
+
+}  // namespace ghive

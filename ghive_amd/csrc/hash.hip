@@ -1,0 +1,433 @@
+// Hash-join build and probe, and the N-way join of BaseJoinPredicate::process
+// (gpu_impl/src/Operator/SortMergeJoinPredicate.cu:8-461 produces the same result_offsets
+// with a GPU sort plus a single-threaded CPU merge; here both sides stay on the device).
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "hash.hpp"
+#include "scan.hpp"
+
+namespace ghive {
+
+constexpr int kHashThreads = 256;
+
+__device__ __forceinline__ long long wave_min_i64(long long v) {
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    long long o = __shfl_xor(v, d, kWave);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+__device__ __forceinline__ long long wave_max_i64(long long v) {
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    long long o = __shfl_xor(v, d, kWave);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// stats: [0] entries, [1] duplicate-key flag; mm: [0] key min, [1] key max, [2] payload min,
+// [3] payload max.
+__global__ __launch_bounds__(kHashThreads) void k_hash_build(
+    const void *keys, int key_type, const uint32_t *sel, uint64_t n, const void *pay, int pay_type,
+    int layout, uint64_t *slots, uint64_t mask, int log2, unsigned long long *stats, long long *mm) {
+  long long kmin = LLONG_MAX, kmax = LLONG_MIN, pmin = LLONG_MAX, pmax = LLONG_MIN;
+  uint32_t inserted = 0, dup = 0;
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    uint64_t r = sel ? sel[i] : i;
+    int64_t k;
+    if (!col_load_i64(keys, key_type, r, k)) continue;  // NULL keys never join
+    int64_t p;
+    if (pay) {
+      if (!col_load_i64(pay, pay_type, r, p)) p = pay_type == GHIVE_INT ? 0x7fffffffll : 0x7fffffffffffffffll;
+    } else {
+      p = (int64_t)r;
+    }
+    if (layout == 8) {
+      uint64_t word = (uint64_t)(uint32_t)(int32_t)k | ((uint64_t)(uint32_t)p << 32);
+      uint64_t s = hash_u32((uint32_t)k, log2);
+      for (uint64_t step = 0; step <= mask; ++step) {
+        unsigned long long old = atomicCAS((unsigned long long *)&slots[s], kEmpty8, word);
+        if (old == kEmpty8) break;
+        if ((int32_t)(uint32_t)old == (int32_t)k) dup = 1;
+        s = (s + 1) & mask;
+      }
+    } else {
+      uint64_t s = hash_u64((uint64_t)k, log2);
+      for (uint64_t step = 0; step <= mask; ++step) {
+        unsigned long long old = atomicCAS((unsigned long long *)&slots[2 * s], kEmpty16, (unsigned long long)k);
+        if (old == kEmpty16) {
+          slots[2 * s + 1] = (uint64_t)p;
+          break;
+        }
+        if ((int64_t)old == k) dup = 1;
+        s = (s + 1) & mask;
+      }
+    }
+    ++inserted;
+    kmin = k < kmin ? k : kmin;
+    kmax = k > kmax ? k : kmax;
+    pmin = p < pmin ? p : pmin;
+    pmax = p > pmax ? p : pmax;
+  }
+  kmin = wave_min_i64(kmin);
+  kmax = wave_max_i64(kmax);
+  pmin = wave_min_i64(pmin);
+  pmax = wave_max_i64(pmax);
+  unsigned long long ins = wave_sum_u64(inserted);
+  uint64_t anydup = ballot(dup != 0);
+  if (lane_id() == 0) {
+    if (ins) {
+      atomicAdd(&stats[0], ins);
+      atomicMin(&mm[0], kmin);
+      atomicMax(&mm[1], kmax);
+      atomicMin(&mm[2], pmin);
+      atomicMax(&mm[3], pmax);
+    }
+    if (anydup) atomicOr(&stats[1], 1ull);
+  }
+}
+
+// mode 0 inner: every match; 1 left semi: the first match only; 2 left outer: every match, or
+// one (row, -1) when there is none. Probe row i reads key row sel[i] (sel == nullptr: i) or
+// tup[i] (-1 = NULL side of an earlier outer join). Emits (emit_index ? i : row, payload).
+__global__ __launch_bounds__(kHashThreads) void k_hash_probe(
+    DevHash h, const void *keys, int key_type, const uint32_t *sel, const int32_t *tup, uint64_t n,
+    int mode, int emit_index, int unique, uint32_t *out_a, int64_t *out_b, uint64_t cap,
+    unsigned long long *counter) {
+  uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  // every lane of a wave runs the same number of outer iterations
+  const uint64_t iters = (n + stride - 1) / stride;
+  for (uint64_t it = 0; it < iters; ++it) {
+    uint64_t i = i0 + it * stride;
+    bool active = i < n;
+    uint64_t r = 0;
+    int64_t k = 0;
+    if (active) {
+      if (tup) {
+        int32_t t = tup[i];
+        active = t >= 0;
+        r = (uint64_t)(t < 0 ? 0 : t);
+      } else {
+        r = sel ? sel[i] : i;
+      }
+    }
+    bool valid = active && col_load_i64(keys, key_type, r, k) && k >= h.key_min && k <= h.key_max;
+    if (valid && h.layout == 8 && (k < INT32_MIN || k > INT32_MAX)) valid = false;
+    bool searching = valid;
+    bool any = false;
+    uint64_t s = valid ? home_slot(h, k) : 0;
+    uint64_t steps = 0;  // slots visited; a scan never wraps the whole table
+    bool pending_outer = (mode == 2) && (i < n);
+    for (;;) {
+      bool match = false;
+      int64_t payload = -1;
+      if (searching) {
+        for (;;) {
+          if (steps++ > h.mask) { searching = false; break; }
+          uint64_t kk, pp;
+          if (h.layout == 8) {
+            uint64_t w = h.slots[s];
+            kk = (uint64_t)(int64_t)(int32_t)(uint32_t)w;
+            pp = (uint64_t)(int64_t)(int32_t)(uint32_t)(w >> 32);
+            if ((uint32_t)w == 0x7fffffffu) { searching = false; break; }
+          } else {
+            kk = h.slots[2 * s];
+            if (kk == kEmpty16) { searching = false; break; }
+            pp = h.slots[2 * s + 1];
+          }
+          s = (s + 1) & h.mask;
+          if ((int64_t)kk == k) {
+            match = true;
+            payload = (int64_t)pp;
+            if (unique || mode == 1) searching = false;
+            break;
+          }
+        }
+      }
+      any |= match;
+      bool emit_outer = pending_outer && !searching && !any;
+      bool emit = match || emit_outer;
+      if (emit_outer) pending_outer = false;
+      if (match) pending_outer = false;
+      uint64_t pos = wave_append(counter, emit);
+      if (emit && pos < cap) {
+        out_a[pos] = emit_index ? (uint32_t)i : (uint32_t)r;
+        out_b[pos] = match ? payload : -1;
+      }
+      if (!ballot(searching)) break;
+    }
+  }
+}
+
+__global__ void k_seq_i32(int32_t *p, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = (int32_t)i;
+}
+
+// new[t][j] = old[t][pair_i[j]]; the joined table gets the build row (or -1).
+__global__ void k_expand_tuples(const int32_t *old_col, const uint32_t *pair_i, uint64_t n, int32_t *new_col) {
+  uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; j < n; j += stride) new_col[j] = old_col[pair_i[j]];
+}
+__global__ void k_payload_to_i32(const int64_t *pay, uint64_t n, int32_t *out) {
+  uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; j < n; j += stride) out[j] = (int32_t)pay[j];
+}
+__global__ void k_interleave(const int32_t *const *cols, uint32_t ncols, uint64_t n, int32_t *out) {
+  uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; j < n * ncols; j += stride) out[j] = cols[j % ncols][j / ncols];
+}
+
+static uint32_t grid_for(ghive_ctx *ctx, uint64_t n, uint32_t threads = kHashThreads) {
+  uint64_t g = (n + threads - 1) / threads, cap = (uint64_t)ctx->cus * 16;
+  return (uint32_t)(g < 1 ? 1 : (g > cap ? cap : g));
+}
+
+int hash_build(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, uint64_t n,
+               const ghive_column *payload, float load_factor, ghive_hash_table **out) {
+  if (!key || !out) return set_error(ctx, GHIVE_ERR_INVALID, "hash_build: NULL argument");
+  *out = nullptr;
+  if (!is_int_type(key->type))
+    return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "hash_build: key type %d (INT/LONG only)", key->type);
+  if (payload && !is_int_type(payload->type))
+    return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "hash_build: payload type %d (INT/LONG only)", payload->type);
+  if (payload && payload->row_num != key->row_num)
+    return set_error(ctx, GHIVE_ERR_INVALID, "hash_build: payload and key row counts differ");
+  if (key->row_num > kMaxRows) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "hash_build: too many rows");
+  if (!sel && n != key->row_num)
+    return set_error(ctx, GHIVE_ERR_INVALID, "hash_build: n (%llu) != rows (%llu) without a selection",
+                     (unsigned long long)n, (unsigned long long)key->row_num);
+  if (n && !key->data) return set_error(ctx, GHIVE_ERR_INVALID, "hash_build: key has no data");
+  if (load_factor <= 0.f) load_factor = 0.5f;
+  if (load_factor > 0.95f) return set_error(ctx, GHIVE_ERR_INVALID, "hash_build: load factor %.3f > 0.95", load_factor);
+  auto *h = new ghive_hash_table();
+  h->owner = ctx;
+  h->key_type = key->type;
+  h->source_rows = key->row_num;
+  h->payload_kind = payload ? (payload->type == GHIVE_INT ? 1 : 2) : 0;
+  h->layout = (key->type == GHIVE_INT && h->payload_kind != 2) ? 8 : 16;
+  uint64_t want = (uint64_t)((double)(n ? n : 1) / load_factor) + 1;
+  h->slots = next_pow2(want < 64 ? 64 : want);
+  h->log2_slots = __builtin_ctzll(h->slots);
+  uint64_t words = h->slots * (h->layout == 8 ? 1 : 2);
+  h->data = ctx->arena->alloc(words * 8);
+  if (!h->data) {
+    delete h;
+    return set_error(ctx, GHIVE_ERR_OOM, "hash_build: cannot allocate %llu slots", (unsigned long long)h->slots);
+  }
+  int st = GHIVE_OK;
+  if (h->layout == 8) {
+    st = fill_u64(ctx, static_cast<uint64_t *>(h->data), kEmpty8, h->slots);
+  } else {
+    // {key = LONG_MAX, payload = 0}: fill keys and payloads as one u64 array then fix payloads
+    st = fill_u64(ctx, static_cast<uint64_t *>(h->data), kEmpty16, words);
+  }
+  DevBuf stats;
+  if (st == GHIVE_OK) st = stats.alloc(ctx, 8 * sizeof(uint64_t));
+  long long init_mm[4] = {LLONG_MAX, LLONG_MIN, LLONG_MAX, LLONG_MIN};
+  unsigned long long init_st[2] = {0, 0};
+  if (st == GHIVE_OK) st = h2d(ctx, stats.as<char>(), init_st, sizeof init_st);
+  if (st == GHIVE_OK) st = h2d(ctx, stats.as<char>() + 16, init_mm, sizeof init_mm);
+  if (st == GHIVE_OK && n) {
+    k_hash_build<<<grid_for(ctx, n), kHashThreads, 0, ctx->stream>>>(
+        key->data, key->type, sel, n, payload ? payload->data : nullptr, payload ? payload->type : 0,
+        h->layout, static_cast<uint64_t *>(h->data), h->slots - 1, h->log2_slots,
+        stats.as<unsigned long long>(), reinterpret_cast<long long *>(stats.as<char>() + 16));
+    if (hipGetLastError() != hipSuccess) st = set_error(ctx, GHIVE_ERR_HIP, "hash_build: launch failed");
+  }
+  uint64_t host[6];
+  if (st == GHIVE_OK) st = d2h(ctx, host, stats.get(), sizeof host);
+  if (st != GHIVE_OK) {
+    ctx->arena->free(h->data);
+    delete h;
+    return st;
+  }
+  h->entries = host[0];
+  h->unique = host[1] == 0;
+  h->key_min = (int64_t)host[2];
+  h->key_max = (int64_t)host[3];
+  h->pay_min = (int64_t)host[4];
+  h->pay_max = (int64_t)host[5];
+  *out = h;
+  return GHIVE_OK;
+}
+
+// Probe into caller buffers; on overflow returns CAPACITY with the needed count in *count.
+static int probe_run(ghive_ctx *ctx, const ghive_hash_table *h, const ghive_column *key,
+                     const uint32_t *sel, const int32_t *tup, uint64_t n, int mode, int emit_index,
+                     uint32_t *out_a, int64_t *out_b, uint64_t cap, uint64_t *count) {
+  GH_TRY(fill_u64(ctx, ctx->d_counters, 0, 1));
+  if (n) {
+    k_hash_probe<<<grid_for(ctx, n), kHashThreads, 0, ctx->stream>>>(
+        dev_hash(h), key->data, key->type, sel, tup, n, mode, emit_index, h->unique, out_a, out_b,
+        cap, reinterpret_cast<unsigned long long *>(ctx->d_counters));
+    GH_LAUNCH(ctx);
+  }
+  GH_TRY(d2h(ctx, count, ctx->d_counters, sizeof(uint64_t)));
+  return *count > cap ? GHIVE_ERR_CAPACITY : GHIVE_OK;
+}
+
+}  // namespace ghive
+
+using namespace ghive;
+
+extern "C" {
+
+int ghive_hash_build(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, uint64_t n,
+                     const ghive_column *payload, float load_factor, ghive_hash_table **out) {
+  GH_CHECK_CTX(ctx);
+  return hash_build(ctx, key, sel, n, payload, load_factor, out);
+}
+
+int ghive_hash_table_info(const ghive_hash_table *ht, uint64_t *entries, uint64_t *slots,
+                          uint64_t *bytes, int32_t *unique_keys) {
+  if (!ht) return GHIVE_ERR_INVALID;
+  if (entries) *entries = ht->entries;
+  if (slots) *slots = ht->slots;
+  if (bytes) *bytes = ht->slots * (ht->layout == 8 ? 8 : 16);
+  if (unique_keys) *unique_keys = ht->unique;
+  return GHIVE_OK;
+}
+
+void ghive_hash_table_free(ghive_ctx *ctx, ghive_hash_table *ht) {
+  if (!ht) return;
+  ghive_ctx *c = ctx ? ctx : ht->owner;
+  if (c && ht->data) {
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->arena->free(ht->data);
+  }
+  delete ht;
+}
+
+int ghive_hash_probe(ghive_ctx *ctx, const ghive_hash_table *ht, const ghive_column *key,
+                     const uint32_t *sel, uint64_t n, uint32_t *out_probe_rows, int64_t *out_payload,
+                     uint64_t capacity, uint64_t *out_count) {
+  GH_CHECK_CTX(ctx);
+  if (!ht || !key || !out_count) return set_error(ctx, GHIVE_ERR_INVALID, "hash_probe: NULL argument");
+  if (!is_int_type(key->type))
+    return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "hash_probe: key type %d (INT/LONG only)", key->type);
+  if (!sel && n != key->row_num)
+    return set_error(ctx, GHIVE_ERR_INVALID, "hash_probe: n != rows without a selection");
+  if (key->row_num > kMaxRows) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "hash_probe: too many rows");
+  if (capacity && (!out_probe_rows || !out_payload))
+    return set_error(ctx, GHIVE_ERR_INVALID, "hash_probe: NULL output");
+  int s = probe_run(ctx, ht, key, sel, nullptr, n, 0, 0, out_probe_rows, out_payload, capacity, out_count);
+  if (s == GHIVE_ERR_CAPACITY)
+    set_error(ctx, s, "hash_probe: %llu matches exceed capacity %llu", (unsigned long long)*out_count,
+              (unsigned long long)capacity);
+  return s;
+}
+
+int ghive_join(ghive_ctx *ctx, const ghive_table *tables, uint32_t ntables, const uint32_t *join_cols,
+               const ghive_join_cond *conds, uint32_t nconds, int32_t *out_offsets, uint64_t capacity,
+               uint64_t *out_cardinality) {
+  GH_CHECK_CTX(ctx);
+  if (!tables || !join_cols || !conds || !out_cardinality)
+    return set_error(ctx, GHIVE_ERR_INVALID, "join: NULL argument");
+  if (ntables < 2 || nconds != ntables - 1)
+    return set_error(ctx, GHIVE_ERR_INVALID, "join: %u tables need %u conditions, got %u", ntables,
+                     ntables ? ntables - 1 : 0, nconds);
+  for (uint32_t t = 0; t < ntables; ++t) {
+    GH_TRY(check_table(ctx, &tables[t], "join"));
+    if (join_cols[t] >= tables[t].ncols)
+      return set_error(ctx, GHIVE_ERR_INVALID, "join: table %u has no column %u", t, join_cols[t]);
+    if (!is_int_type(tables[t].columns[join_cols[t]].type))
+      return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "join: table %u key type %d (INT/LONG only)", t,
+                       tables[t].columns[join_cols[t]].type);
+  }
+  std::vector<int> joined(ntables, 0), semi(ntables, 0);
+  joined[0] = 1;
+  for (uint32_t c = 0; c < nconds; ++c) {
+    const ghive_join_cond &jc = conds[c];
+    if (jc.type != GHIVE_INNER_JOIN && jc.type != GHIVE_LEFT_OUTER_JOIN && jc.type != GHIVE_LEFT_SEMI_JOIN)
+      return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "join: join type %d (INNER, LEFT OUTER, LEFT SEMI only)", jc.type);
+    if (jc.left >= ntables || jc.right >= ntables || !joined[jc.left] || joined[jc.right])
+      return set_error(ctx, GHIVE_ERR_INVALID, "join: condition %u (%u -> %u) is not left-deep", c, jc.left, jc.right);
+    joined[jc.right] = 1;
+    semi[jc.right] = jc.type == GHIVE_LEFT_SEMI_JOIN;
+  }
+  // tuples, column-major: tup[t][i] = row of table t in result row i (-1 = NULL)
+  const uint64_t n0 = tables[0].row_num;
+  std::vector<DevBuf> tup(ntables);
+  uint64_t cur = n0;
+  for (uint32_t t = 0; t < ntables; ++t) GH_TRY(tup[t].alloc(ctx, sizeof(int32_t) * (cur ? cur : 1)));
+  if (cur) {
+    k_seq_i32<<<grid_for(ctx, cur), 256, 0, ctx->stream>>>(tup[0].as<int32_t>(), cur);
+    GH_LAUNCH(ctx);
+  }
+  for (uint32_t c = 0; c < nconds; ++c) {
+    const ghive_join_cond &jc = conds[c];
+    const uint32_t L = jc.left, R = jc.right;
+    ghive_hash_table *ht = nullptr;
+    GH_TRY(hash_build(ctx, &tables[R].columns[join_cols[R]], nullptr, tables[R].row_num, nullptr, 0.5f, &ht));
+    const int mode = jc.type == GHIVE_LEFT_SEMI_JOIN ? 1 : (jc.type == GHIVE_LEFT_OUTER_JOIN ? 2 : 0);
+    const ghive_column *lk = &tables[L].columns[join_cols[L]];
+    uint64_t need = 0;
+    // count pass, then the emit pass into exactly sized buffers
+    int s = probe_run(ctx, ht, lk, nullptr, tup[L].as<int32_t>(), cur, mode, 1, nullptr, nullptr, 0, &need);
+    if (s != GHIVE_OK && s != GHIVE_ERR_CAPACITY) {
+      ghive_hash_table_free(ctx, ht);
+      return s;
+    }
+    DevBuf pa, pb;
+    s = pa.alloc(ctx, sizeof(uint32_t) * (need ? need : 1));
+    if (s == GHIVE_OK) s = pb.alloc(ctx, sizeof(int64_t) * (need ? need : 1));
+    uint64_t got = 0;
+    if (s == GHIVE_OK)
+      s = probe_run(ctx, ht, lk, nullptr, tup[L].as<int32_t>(), cur, mode, 1, pa.as<uint32_t>(),
+                    pb.as<int64_t>(), need, &got);
+    ghive_hash_table_free(ctx, ht);
+    if (s != GHIVE_OK) return s;
+    std::vector<DevBuf> nt(ntables);
+    for (uint32_t t = 0; t < ntables; ++t) {
+      GH_TRY(nt[t].alloc(ctx, sizeof(int32_t) * (got ? got : 1)));
+      if (!got) continue;
+      if (t == R) {
+        k_payload_to_i32<<<grid_for(ctx, got), 256, 0, ctx->stream>>>(pb.as<int64_t>(), got, nt[t].as<int32_t>());
+      } else {
+        k_expand_tuples<<<grid_for(ctx, got), 256, 0, ctx->stream>>>(tup[t].as<int32_t>(), pa.as<uint32_t>(), got,
+                                                                     nt[t].as<int32_t>());
+      }
+      GH_LAUNCH(ctx);
+    }
+    for (uint32_t t = 0; t < ntables; ++t) tup[t] = std::move(nt[t]);
+    cur = got;
+  }
+  uint32_t nout = 0;
+  std::vector<const int32_t *> cols;
+  for (uint32_t t = 0; t < ntables; ++t)
+    if (!semi[t]) {
+      cols.push_back(tup[t].as<int32_t>());
+      ++nout;
+    }
+  *out_cardinality = cur;
+  if (cur > capacity)
+    return set_error(ctx, GHIVE_ERR_CAPACITY, "join: cardinality %llu exceeds capacity %llu",
+                     (unsigned long long)cur, (unsigned long long)capacity);
+  if (cur) {
+    if (!out_offsets) return set_error(ctx, GHIVE_ERR_INVALID, "join: out_offsets is NULL");
+    DevBuf dcols;
+    GH_TRY(dcols.alloc(ctx, sizeof(void *) * nout));
+    GH_TRY(h2d(ctx, dcols.get(), cols.data(), sizeof(void *) * nout));
+    k_interleave<<<grid_for(ctx, cur * nout), 256, 0, ctx->stream>>>(dcols.as<const int32_t *>(), nout, cur,
+                                                                     out_offsets);
+    GH_LAUNCH(ctx);
+    GH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return GHIVE_OK;
+}
+
+}  // extern "C"

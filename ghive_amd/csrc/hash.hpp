@@ -1,0 +1,78 @@
+// Device hash tables for the join (build side resident in HBM, probed from L2/MALL).
+//
+// Two slot layouts, both open addressing with linear probing over a power-of-two slot array:
+//   L8  : one uint64 per slot = (uint32)key | (uint32)payload << 32, for INT keys whose payload
+//         fits 32 bits (an INT attribute or a build row id). One 8-byte load answers a probe and
+//         a 128-byte line holds 16 consecutive slots.
+//   L16 : two uint64 per slot = {key, payload}, for LONG keys or LONG payloads.
+// The empty marker is the NULL sentinel of the key (INT_MAX / LONG_MAX): NULL keys never join,
+// so no real key can be confused with an empty slot.
+#pragma once
+
+#include "common.hpp"
+#include "device.hpp"
+
+struct ghive_hash_table {
+  int layout;        // 8 or 16
+  int key_type;      // GHIVE_INT or GHIVE_LONG
+  int payload_kind;  // 0 build row id, 1 INT column, 2 LONG column
+  int log2_slots;
+  uint64_t slots;
+  uint64_t entries;
+  uint64_t source_rows;  // rows of the build-side table (hit-rate estimate for join ordering)
+  int unique;
+  int64_t key_min, key_max;
+  int64_t pay_min, pay_max;
+  void *data;  // device slot array
+  ghive_ctx *owner;
+};
+
+namespace ghive {
+
+constexpr uint64_t kEmpty8 = 0x000000007fffffffull;
+constexpr uint64_t kEmpty16 = 0x7fffffffffffffffull;
+
+struct DevHash {
+  const uint64_t *slots;
+  uint64_t mask;
+  int32_t log2;
+  int32_t layout;
+  int64_t key_min, key_max;  // inserted key range (probe pre-check, as Hive's min/max)
+};
+
+inline DevHash dev_hash(const ghive_hash_table *h) {
+  DevHash d;
+  d.slots = static_cast<const uint64_t *>(h->data);
+  d.mask = h->slots - 1;
+  d.log2 = h->log2_slots;
+  d.layout = h->layout;
+  d.key_min = h->key_min;
+  d.key_max = h->key_max;
+  return d;
+}
+
+__device__ __forceinline__ uint64_t home_slot(const DevHash &h, int64_t key) {
+  return h.layout == 8 ? (uint64_t)hash_u32((uint32_t)key, h.log2) : hash_u64((uint64_t)key, h.log2);
+}
+
+// First match of a unique-key table; returns false on a miss.
+__device__ __forceinline__ bool probe_unique8(const uint64_t *slots, uint64_t mask, int log2,
+                                              int32_t key, int32_t &payload) {
+  uint64_t s = hash_u32((uint32_t)key, log2);
+  for (uint64_t step = 0; step <= mask; ++step) {  // a table always has empty slots; bounded anyway
+    uint64_t w = slots[s];
+    int32_t k = (int32_t)(uint32_t)w;
+    if (k == key) {
+      payload = (int32_t)(w >> 32);
+      return true;
+    }
+    if (k == 0x7fffffff) return false;
+    s = (s + 1) & mask;
+  }
+  return false;
+}
+
+int hash_build(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, uint64_t n,
+               const ghive_column *payload, float load_factor, ghive_hash_table **out);
+
+}  // namespace ghive

@@ -1,0 +1,108 @@
+// Small device utilities: fill kernels (used instead of hipMemsetAsync so that the whole
+// operator sequence stays capturable in a HIP graph) and a reduce-then-scan exclusive prefix
+// sum over uint32 counts (filter tile counts, radix-sort digit histograms).
+#include "common.hpp"
+#include "device.hpp"
+#include "scan.hpp"
+
+namespace ghive {
+
+__global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+static uint32_t fill_grid(ghive_ctx *ctx, uint64_t n) {
+  uint64_t g = (n + 255) / 256;
+  uint64_t cap = (uint64_t)ctx->cus * 8;
+  return (uint32_t)(g < 1 ? 1 : (g > cap ? cap : g));
+}
+
+int fill_u64(ghive_ctx *ctx, uint64_t *p, uint64_t value, uint64_t n) {
+  if (!n) return GHIVE_OK;
+  k_fill_u64<<<fill_grid(ctx, n), 256, 0, ctx->stream>>>(p, value, n);
+  GH_LAUNCH(ctx);
+  return GHIVE_OK;
+}
+
+int fill_u32(ghive_ctx *ctx, uint32_t *p, uint32_t value, uint64_t n) {
+  if (!n) return GHIVE_OK;
+  k_fill_u32<<<fill_grid(ctx, n), 256, 0, ctx->stream>>>(p, value, n);
+  GH_LAUNCH(ctx);
+  return GHIVE_OK;
+}
+
+constexpr int kScanThreads = 256;
+constexpr int kScanPerThread = 16;
+constexpr uint64_t kScanChunk = (uint64_t)kScanThreads * kScanPerThread;
+
+// per-chunk totals
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t *in, uint64_t n,
+                                                              uint32_t *sums) {
+  __shared__ uint32_t lds[kScanThreads / kWave];
+  uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
+  uint32_t s = 0;
+  for (int j = 0; j < kScanPerThread; ++j) {
+    uint64_t i = base + (uint64_t)j * kScanThreads + threadIdx.x;
+    if (i < n) s += in[i];
+  }
+  uint32_t total;
+  (void)block_exclusive_scan_256(s, lds, total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// exclusive scan of one chunk, offset by offs[block] (offs == nullptr: 0)
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t *in, uint32_t *out,
+                                                             uint64_t n, const uint32_t *offs,
+                                                             uint64_t *d_total, uint32_t nblocks) {
+  __shared__ uint32_t lds[kScanThreads / kWave];
+  uint64_t base = (uint64_t)blockIdx.x * kScanChunk + (uint64_t)threadIdx.x * kScanPerThread;
+  uint32_t v[kScanPerThread];
+  uint32_t s = 0;
+  for (int j = 0; j < kScanPerThread; ++j) {
+    uint64_t i = base + j;
+    v[j] = i < n ? in[i] : 0u;
+    s += v[j];
+  }
+  uint32_t total;
+  uint32_t pre = block_exclusive_scan_256(s, lds, total);
+  uint32_t off = offs ? offs[blockIdx.x] : 0u;
+  uint32_t run = off + pre;
+  for (int j = 0; j < kScanPerThread; ++j) {
+    uint64_t i = base + j;
+    if (i < n) out[i] = run;
+    run += v[j];
+  }
+  if (d_total && blockIdx.x == nblocks - 1 && threadIdx.x == 0) *d_total = (uint64_t)off + total;
+}
+
+int exclusive_scan_u32(ghive_ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n,
+                       uint64_t *d_total) {
+  if (n == 0) {
+    if (d_total) return fill_u64(ctx, d_total, 0, 1);
+    return GHIVE_OK;
+  }
+  uint32_t nb = ceil_div(n, kScanChunk);
+  if (nb == 1) {
+    k_scan_apply<<<1, kScanThreads, 0, ctx->stream>>>(in, out, n, nullptr, d_total, 1);
+    GH_LAUNCH(ctx);
+    return GHIVE_OK;
+  }
+  DevBuf sums;
+  GH_TRY(sums.alloc(ctx, sizeof(uint32_t) * nb));
+  k_scan_reduce<<<nb, kScanThreads, 0, ctx->stream>>>(in, n, sums.as<uint32_t>());
+  GH_LAUNCH(ctx);
+  GH_TRY(exclusive_scan_u32(ctx, sums.as<uint32_t>(), sums.as<uint32_t>(), nb, nullptr));
+  k_scan_apply<<<nb, kScanThreads, 0, ctx->stream>>>(in, out, n, sums.as<uint32_t>(), d_total, nb);
+  GH_LAUNCH(ctx);
+  return GHIVE_OK;
+}
+
+}  // namespace ghive

@@ -1,0 +1,248 @@
+// ORDER BY as a stable LSD radix sort (replaces SinkOperator's thrust::sort of row ids with a
+// type-generic comparator plus a CPU permutation gather, gpu_impl/src/Operator/SinkOperator.cu:
+// 135-288). Keys are turned into one order-preserving unsigned bit string per row (first key in
+// the most significant bits; DESC inverts its field; NaN is canonical and sorts after +inf), then
+// sorted 8 bits at a time together with the row id. Digits that are equal in every row are
+// skipped. Each pass: per-tile LDS histogram -> scan of the digit x tile matrix -> stable scatter
+// ranked with wave64 ballots. Equal keys keep ascending row ids, so the result is a total order.
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "scan.hpp"
+
+namespace ghive {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortPerThread = 16;
+constexpr uint64_t kSortTile = (uint64_t)kSortThreads * kSortPerThread;
+constexpr int kSortWaves = kSortThreads / kWave;
+
+struct SortKeySpec {
+  int32_t nkeys;
+  int32_t type[8];
+  int32_t desc[8];
+  int32_t shift[8];  // bit offset of the field inside the 128-bit composite
+  int32_t width[8];
+  const void *data[8];
+};
+
+__device__ __forceinline__ uint64_t field_bits(int type, const void *p, uint64_t r) {
+  switch (type) {
+    case GHIVE_INT: return (uint32_t) static_cast<const int32_t *>(p)[r] ^ 0x80000000u;
+    case GHIVE_LONG: return (uint64_t) static_cast<const int64_t *>(p)[r] ^ 0x8000000000000000ull;
+    case GHIVE_FLOAT: {
+      float f = static_cast<const float *>(p)[r];
+      uint32_t b = __builtin_isnan(f) ? 0x7fc00000u : (f == 0.0f ? 0u : __float_as_uint(f));
+      return (b >> 31) ? (uint32_t)~b : (b | 0x80000000u);
+    }
+    default: {
+      double d = static_cast<const double *>(p)[r];
+      uint64_t b = __builtin_isnan(d) ? 0x7ff8000000000000ull : (d == 0.0 ? 0ull : (uint64_t)__double_as_longlong(d));
+      return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+    }
+  }
+}
+
+template <int KW>
+__global__ void k_sort_keys(SortKeySpec S, uint64_t n, uint64_t *keys, uint32_t *idx, unsigned long long *orand) {
+  uint64_t vor[2] = {0, 0}, vand[2] = {~0ull, ~0ull};
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    uint64_t w[2] = {0, 0};  // w[0] = low 64 bits, w[1] = high 64 bits
+    for (int k = 0; k < S.nkeys; ++k) {
+      uint64_t b = field_bits(S.type[k], S.data[k], i);
+      if (S.desc[k]) b = ~b;
+      if (S.width[k] < 64) b &= (1ull << S.width[k]) - 1;
+      int sh = S.shift[k];
+      if (sh >= 64) {
+        w[1] |= b << (sh - 64);
+      } else {
+        w[0] |= b << sh;
+        if (sh + S.width[k] > 64) w[1] |= b >> (64 - sh);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < KW; ++q) {
+      keys[(uint64_t)q * n + i] = w[q];
+      vor[q] |= w[q];
+      vand[q] &= w[q];
+    }
+    idx[i] = (uint32_t)i;
+  }
+#pragma unroll
+  for (int q = 0; q < KW; ++q) {
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+      vor[q] |= __shfl_xor((unsigned long long)vor[q], d, kWave);
+      vand[q] &= __shfl_xor((unsigned long long)vand[q], d, kWave);
+    }
+    if (lane_id() == 0) {
+      atomicOr(&orand[2 * q], (unsigned long long)vor[q]);
+      atomicAnd(&orand[2 * q + 1], (unsigned long long)vand[q]);
+    }
+  }
+}
+
+// hist[digit * nblocks + block]
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t *keys, uint64_t n, int word, int shift,
+                                                             uint32_t *hist, uint32_t nblocks) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t t0 = (uint64_t)blockIdx.x * kSortTile;
+  const uint64_t *kw = keys + (uint64_t)word * n;
+  for (int j = 0; j < kSortPerThread; ++j) {
+    uint64_t e = t0 + (uint64_t)j * kSortThreads + threadIdx.x;
+    if (e < n) atomicAdd(&h[(kw[e] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+template <int KW>
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *keys_in, const uint32_t *idx_in,
+                                                                uint64_t *keys_out, uint32_t *idx_out, uint64_t n,
+                                                                int word, int shift, const uint32_t *offs,
+                                                                uint32_t nblocks) {
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t wcnt[kSortWaves][256];
+  const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
+  run[threadIdx.x] = offs[(uint64_t)threadIdx.x * nblocks + blockIdx.x];
+  const uint64_t t0 = (uint64_t)blockIdx.x * kSortTile;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  for (int j = 0; j < kSortPerThread; ++j) {
+    const uint64_t e = t0 + (uint64_t)j * kSortThreads + threadIdx.x;
+    const bool valid = e < n;
+    uint64_t kv[2] = {0, 0};
+    uint32_t id = 0, digit = 0;
+    if (valid) {
+#pragma unroll
+      for (int q = 0; q < KW; ++q) kv[q] = keys_in[(uint64_t)q * n + e];
+      id = idx_in[e];
+      digit = (uint32_t)(kv[word] >> shift) & 255u;
+    }
+    uint64_t peers = ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      uint64_t bb = ballot((digit >> b) & 1u);
+      peers &= ((digit >> b) & 1u) ? bb : ~bb;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & lt);
+#pragma unroll
+    for (int q = 0; q < 256 / kWave; ++q) wcnt[wave][lane + q * kWave] = 0;
+    __syncthreads();
+    if (valid && rank == 0) wcnt[wave][digit] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = run[digit] + rank;
+      for (uint32_t w = 0; w < wave; ++w) pos += wcnt[w][digit];
+#pragma unroll
+      for (int q = 0; q < KW; ++q) keys_out[(uint64_t)q * n + pos] = kv[q];
+      idx_out[pos] = id;
+    }
+    __syncthreads();
+    uint32_t add = 0;
+#pragma unroll
+    for (int w = 0; w < kSortWaves; ++w) add += wcnt[w][threadIdx.x];
+    run[threadIdx.x] += add;
+    __syncthreads();
+  }
+}
+
+template <int KW>
+static int radix_passes(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, uint32_t *i0, uint32_t *i1,
+                        const std::vector<int> &digits, uint32_t **result) {
+  const uint32_t nblocks = ceil_div(n, kSortTile);
+  DevBuf hist;
+  GH_TRY(hist.alloc(ctx, sizeof(uint32_t) * 256ull * nblocks));
+  uint64_t *kin = k0, *kout = k1;
+  uint32_t *iin = i0, *iout = i1;
+  for (int d : digits) {
+    const int word = d / 8, shift = 8 * (d % 8);
+    k_radix_hist<<<nblocks, kSortThreads, 0, ctx->stream>>>(kin, n, word, shift, hist.as<uint32_t>(), nblocks);
+    GH_LAUNCH(ctx);
+    GH_TRY(exclusive_scan_u32(ctx, hist.as<uint32_t>(), hist.as<uint32_t>(), 256ull * nblocks, nullptr));
+    k_radix_scatter<KW><<<nblocks, kSortThreads, 0, ctx->stream>>>(kin, iin, kout, iout, n, word, shift,
+                                                                   hist.as<uint32_t>(), nblocks);
+    GH_LAUNCH(ctx);
+    std::swap(kin, kout);
+    std::swap(iin, iout);
+  }
+  *result = iin;
+  return GHIVE_OK;
+}
+
+}  // namespace ghive
+
+using namespace ghive;
+
+extern "C" {
+
+int ghive_sort(ghive_ctx *ctx, const ghive_table *table, const uint32_t *key_cols, const char *orders,
+               uint32_t nkeys, uint32_t *out_perm) {
+  GH_CHECK_CTX(ctx);
+  GH_TRY(check_table(ctx, table, "sort"));
+  if (nkeys == 0 || nkeys > 8 || !key_cols)
+    return set_error(ctx, GHIVE_ERR_INVALID, "sort: %u key columns (1..8)", nkeys);
+  const uint64_t n = table->row_num;
+  if (n && !out_perm) return set_error(ctx, GHIVE_ERR_INVALID, "sort: out_perm is NULL");
+  SortKeySpec S;
+  memset(&S, 0, sizeof S);
+  S.nkeys = (int)nkeys;
+  int bits = 0;
+  for (uint32_t k = 0; k < nkeys; ++k) {
+    if (key_cols[k] >= table->ncols) return set_error(ctx, GHIVE_ERR_INVALID, "sort: key column %u", key_cols[k]);
+    const ghive_column &c = table->columns[key_cols[k]];
+    if (!is_numeric_type(c.type)) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "sort: STRING keys are not supported");
+    char o = orders ? orders[k] : '+';
+    if (o != '+' && o != '-') return set_error(ctx, GHIVE_ERR_INVALID, "sort: order '%c' (use '+' or '-')", o);
+    S.type[k] = c.type;
+    S.desc[k] = o == '-';
+    S.width[k] = 8 * type_width(c.type);
+    S.data[k] = c.data;
+    bits += S.width[k];
+  }
+  if (bits > 128) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "sort: keys are %d bits wide (max 128)", bits);
+  if (n == 0) return GHIVE_OK;
+  int top = bits;
+  for (uint32_t k = 0; k < nkeys; ++k) {
+    top -= S.width[k];
+    S.shift[k] = top;
+  }
+  const int KW = bits > 64 ? 2 : 1;
+  DevBuf k0, k1, i1, orand;
+  GH_TRY(k0.alloc(ctx, 8ull * KW * n));
+  GH_TRY(k1.alloc(ctx, 8ull * KW * n));
+  GH_TRY(i1.alloc(ctx, 4ull * n));
+  GH_TRY(orand.alloc(ctx, 4 * sizeof(uint64_t)));
+  uint64_t init[4] = {0, ~0ull, 0, ~0ull};
+  GH_TRY(h2d(ctx, orand.get(), init, sizeof init));
+  uint64_t g = (n + 255) / 256, cap = (uint64_t)ctx->cus * 8;
+  uint32_t grid = (uint32_t)(g > cap ? cap : g);
+  if (KW == 1)
+    k_sort_keys<1><<<grid, 256, 0, ctx->stream>>>(S, n, k0.as<uint64_t>(), out_perm, orand.as<unsigned long long>());
+  else
+    k_sort_keys<2><<<grid, 256, 0, ctx->stream>>>(S, n, k0.as<uint64_t>(), out_perm, orand.as<unsigned long long>());
+  GH_LAUNCH(ctx);
+  uint64_t oa[4];
+  GH_TRY(d2h(ctx, oa, orand.get(), sizeof oa));
+  std::vector<int> digits;
+  for (int q = 0; q < KW; ++q) {
+    uint64_t varying = oa[2 * q] ^ oa[2 * q + 1];
+    for (int b = 0; b < 8; ++b)
+      if ((varying >> (8 * b)) & 255u) digits.push_back(8 * q + b);
+  }
+  uint32_t *res = out_perm;
+  if (KW == 1)
+    GH_TRY(radix_passes<1>(ctx, n, k0.as<uint64_t>(), k1.as<uint64_t>(), out_perm, i1.as<uint32_t>(), digits, &res));
+  else
+    GH_TRY(radix_passes<2>(ctx, n, k0.as<uint64_t>(), k1.as<uint64_t>(), out_perm, i1.as<uint32_t>(), digits, &res));
+  if (res != out_perm) GH_HIP(ctx, hipMemcpyAsync(out_perm, res, 4ull * n, hipMemcpyDeviceToDevice, ctx->stream));
+  GH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GHIVE_OK;
+}
+
+}  // extern "C"

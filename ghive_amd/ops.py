@@ -1,0 +1,355 @@
+"""Operator interface of the reference's gpu_impl, over libghive_mi355x.so.
+
+Class and method names follow the reference so that tests read like its own:
+FilterOperator.generatePredicate / FilterPredicate.process / FilterOperator.clean_data
+(gpu_impl/src/Operator/FilterOperator.cu, FilterPredicate.cu), JoinPredicate.process
+(gpu_impl/inc/Operator/BaseJoinPredicate.hpp:83), GroupByPredicate.gpu_execute
+(gpu_impl/inc/Operator/GroupByPredicate.hpp:172), SinkOperator.sort (SinkOperator.cu:135-288).
+Columns are torch tensors on the GPU (int32 = INT, int64 = LONG, float64 = DOUBLE,
+float32 = FLOAT); torch provides device memory and the stream, the compute is the HIP library.
+"""
+import ctypes as C
+
+import torch
+
+from . import abi
+from ._lib import GhiveError, lib
+
+_TYPE_OF_DTYPE = {torch.int64: abi.LONG, torch.float64: abi.DOUBLE, torch.int32: abi.INT,
+                  torch.float32: abi.FLOAT}
+_DTYPE_OF_TYPE = {v: k for k, v in _TYPE_OF_DTYPE.items()}
+
+
+class Context:
+    """One device context (ghive_init). Work is ordered on torch's current stream."""
+
+    def __init__(self, device=0, mem_limit=0):
+        self.device = device
+        h = C.c_void_p()
+        st = lib().ghive_init(device, mem_limit, C.byref(h))
+        if st != abi.OK:
+            raise GhiveError(st, "ghive_init failed")
+        self.h = h
+        torch.cuda.set_device(device)
+        self.use_stream(torch.cuda.current_stream(device))
+
+    def use_stream(self, stream):
+        self.check(lib().ghive_set_stream(self.h, C.c_void_p(stream.cuda_stream)))
+
+    def check(self, status):
+        if status != abi.OK:
+            raise GhiveError(status, lib().ghive_last_error(self.h).decode(errors="replace"))
+
+    def close(self):
+        if self.h:
+            lib().ghive_shutdown(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_info(self):
+        d = abi.DeviceInfo()
+        self.check(lib().ghive_device_query(self.h, C.byref(d)))
+        return {f: (getattr(d, f).decode() if isinstance(getattr(d, f), bytes) else getattr(d, f))
+                for f, _ in abi.DeviceInfo._fields_}
+
+    def mem_stats(self):
+        v = [C.c_uint64() for _ in range(4)]
+        self.check(lib().ghive_mem_stats(self.h, *[C.byref(x) for x in v]))
+        return dict(in_use=v[0].value, peak=v[1].value, limit=v[2].value, reserved=v[3].value)
+
+    def set_option(self, name, value):
+        self.check(lib().ghive_set_option(self.h, name.encode(), int(value)))
+
+    def get_option(self, name):
+        v = C.c_int64()
+        self.check(lib().ghive_get_option(self.h, name.encode(), C.byref(v)))
+        return v.value
+
+    def stage_times(self):
+        t = abi.StageTimes()
+        self.check(lib().ghive_last_stage_times(self.h, C.byref(t)))
+        return dict(total_ms=t.total_ms, main_kernel_ms=t.main_kernel_ms, finalize_ms=t.finalize_ms,
+                    rows_in=t.rows_in, rows_joined=t.rows_joined)
+
+    # ---- synthetic data (Philox4x32-10) -----------------------------------------------------
+    def gen_uniform_i32(self, seed, col, n, lo, hi, row_begin=0, out=None):
+        out = torch.empty(n, dtype=torch.int32, device=f"cuda:{self.device}") if out is None else out
+        self.check(lib().ghive_gen_uniform_i32(self.h, seed, col, row_begin, n, lo, hi, _p(out)))
+        return out
+
+    def gen_uniform_i64(self, seed, col, n, lo, hi, row_begin=0):
+        out = torch.empty(n, dtype=torch.int64, device=f"cuda:{self.device}")
+        self.check(lib().ghive_gen_uniform_i64(self.h, seed, col, row_begin, n, lo, hi, _p(out)))
+        return out
+
+    def gen_sequence_i32(self, n, start, row_begin=0):
+        out = torch.empty(n, dtype=torch.int32, device=f"cuda:{self.device}")
+        self.check(lib().ghive_gen_sequence_i32(self.h, row_begin, n, start, _p(out)))
+        return out
+
+    def gen_uniform_f32(self, seed, col, n, row_begin=0):
+        out = torch.empty(n, dtype=torch.float32, device=f"cuda:{self.device}")
+        self.check(lib().ghive_gen_uniform_f32(self.h, seed, col, row_begin, n, _p(out)))
+        return out
+
+    def gen_normal_f64(self, seed, col, n, row_begin=0):
+        out = torch.empty(n, dtype=torch.float64, device=f"cuda:{self.device}")
+        self.check(lib().ghive_gen_normal_f64(self.h, seed, col, row_begin, n, _p(out)))
+        return out
+
+
+def _p(t):
+    return C.c_void_p(t.data_ptr()) if t is not None and t.numel() else None
+
+
+class Table:
+    """Table of device columns (gpu_impl/inc/DataFlow/Table.hpp)."""
+
+    def __init__(self, columns, key_num=0):
+        self.columns = [c.contiguous() for c in columns]
+        n = len(self.columns)
+        rows = self.columns[0].numel() if n else 0
+        for c in self.columns:
+            if c.numel() != rows:
+                raise ValueError("ragged table")
+            if c.dtype not in _TYPE_OF_DTYPE:
+                raise TypeError(f"unsupported column dtype {c.dtype}")
+            if not c.is_cuda:
+                raise ValueError("columns must be on the GPU")
+        self.row_num = rows
+        self.cols = (abi.Column * max(n, 1))()
+        for i, c in enumerate(self.columns):
+            self.cols[i].type = _TYPE_OF_DTYPE[c.dtype]
+            self.cols[i].data = c.data_ptr() if c.numel() else None
+            self.cols[i].row_num = rows
+        self.table = abi.Table(self.cols, n, key_num, rows)
+
+    def ref(self):
+        return C.byref(self.table)
+
+
+class FilterPredicate:
+    """A predicate tree (FilterPredicate fields, gpu_impl/inc/Operator/FilterPredicate.hpp:61-90)."""
+
+    def __init__(self, nodes, root):
+        self.nodes = list(nodes)
+        self.root = root
+        self._arr = abi.pred_array(self.nodes)
+
+    def toString(self, level=0):
+        assert level == 0
+        n = lib().ghive_predicate_to_string(self._arr, len(self.nodes), self.root, None, 0)
+        buf = C.create_string_buffer(n)
+        lib().ghive_predicate_to_string(self._arr, len(self.nodes), self.root, buf, n)
+        return buf.value.decode()
+
+    def process(self, ctx, table):
+        """Ids of the rows that pass, ascending (int32 tensor holding uint32 row ids)."""
+        out = torch.empty(max(table.row_num, 1), dtype=torch.int32, device=table.columns[0].device
+                          if table.columns else "cuda")
+        cnt = C.c_uint64()
+        ctx.check(lib().ghive_filter(ctx.h, table.ref(), self._arr, len(self.nodes), self.root,
+                                     _p(out), C.byref(cnt)))
+        return out[: cnt.value]
+
+
+class FilterOperator:
+    """FilterOperator: EXPLAIN predicate text + the operator's output columns."""
+
+    def __init__(self, predicate_literal, output_cols):
+        self.predicatesLiteral = predicate_literal
+        self.output_cols = list(output_cols)
+
+    def generatePredicate(self):
+        names = (C.c_char_p * max(len(self.output_cols), 1))(*[c.encode() for c in self.output_cols])
+        cap = 64
+        nodes = (abi.Pred * cap)()
+        nn, root = C.c_uint32(), C.c_uint32()
+        err = C.create_string_buffer(512)
+        st = lib().ghive_parse_predicate(self.predicatesLiteral.encode(), names, len(self.output_cols),
+                                         nodes, cap, C.byref(nn), C.byref(root), err, 512)
+        if st != abi.OK:
+            raise GhiveError(st, err.value.decode())
+        return FilterPredicate([nodes[i] for i in range(nn.value)], root.value)
+
+    def clean_data(self, ctx, table, predicate=None):
+        """Table compacted to the passing rows (the input is not modified)."""
+        pred = predicate or self.generatePredicate()
+        outs = [torch.empty_like(c) for c in table.columns]
+        ocols = (abi.Column * max(len(outs), 1))()
+        for i, o in enumerate(outs):
+            ocols[i].data = o.data_ptr() if o.numel() else None
+        cnt = C.c_uint64()
+        ctx.check(lib().ghive_filter_compact(ctx.h, table.ref(), pred._arr, len(pred.nodes), pred.root,
+                                             ocols, C.byref(cnt)))
+        return Table([o[: cnt.value] for o in outs])
+
+
+class HashTable:
+    """Device hash table of a build side (unique or duplicate keys)."""
+
+    def __init__(self, ctx, key, sel=None, payload=None, load_factor=0.5):
+        self.ctx = ctx
+        kt = Table([key] + ([payload] if payload is not None else []))
+        self._keep = (kt, key, payload, sel)
+        n = sel.numel() if sel is not None else key.numel()
+        h = C.c_void_p()
+        pcol = C.byref(kt.cols[1]) if payload is not None else None
+        ctx.check(lib().ghive_hash_build(ctx.h, C.byref(kt.cols[0]), _p(sel), n, pcol, load_factor,
+                                         C.byref(h)))
+        self.h = h
+
+    def info(self):
+        e, s, b, u = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_int32()
+        self.ctx.check(lib().ghive_hash_table_info(self.h, C.byref(e), C.byref(s), C.byref(b), C.byref(u)))
+        return dict(entries=e.value, slots=s.value, bytes=b.value, unique=bool(u.value))
+
+    def probe(self, key, sel=None, capacity=None):
+        """(probe row ids, payloads) of every match, in unspecified order."""
+        kt = Table([key])
+        n = sel.numel() if sel is not None else key.numel()
+        cap = capacity if capacity is not None else max(n, 1)
+        while True:
+            rows = torch.empty(max(cap, 1), dtype=torch.int32, device=key.device)
+            pay = torch.empty(max(cap, 1), dtype=torch.int64, device=key.device)
+            cnt = C.c_uint64()
+            st = lib().ghive_hash_probe(self.ctx.h, self.h, C.byref(kt.cols[0]), _p(sel), n, _p(rows),
+                                        _p(pay), cap, C.byref(cnt))
+            if st == abi.ERR_CAPACITY and capacity is None:
+                cap = cnt.value
+                continue
+            self.ctx.check(st)
+            return rows[: cnt.value], pay[: cnt.value]
+
+    def free(self):
+        if self.h:
+            lib().ghive_hash_table_free(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class JoinPredicate:
+    """BaseJoinPredicate (join_cols, join_types as (type, left, right)); hash joins."""
+
+    def __init__(self, join_cols, join_types):
+        self.join_cols = list(join_cols)
+        self.join_types = [tuple(t) for t in join_types]
+        self.result_offsets = None
+        self.cardinality = 0
+
+    def process(self, ctx, tables, capacity=None):
+        tarr = (abi.Table * len(tables))(*[t.table for t in tables])
+        jc = torch.tensor(self.join_cols, dtype=torch.int32)
+        jc_arr = (C.c_uint32 * len(self.join_cols))(*self.join_cols)
+        conds = (abi.JoinCond * len(self.join_types))(*[abi.JoinCond(*c) for c in self.join_types])
+        nout = len(tables) - sum(1 for t in self.join_types if t[0] == abi.LEFT_SEMI_JOIN)
+        cap = capacity if capacity is not None else 1
+        dev = tables[0].columns[0].device
+        while True:
+            out = torch.empty(max(cap * nout, 1), dtype=torch.int32, device=dev)
+            card = C.c_uint64()
+            st = lib().ghive_join(ctx.h, tarr, len(tables), C.cast(jc_arr, C.c_void_p), conds,
+                                  len(self.join_types), _p(out), cap, C.byref(card))
+            if st == abi.ERR_CAPACITY and capacity is None:
+                cap = card.value
+                continue
+            ctx.check(st)
+            del jc
+            self.cardinality = card.value
+            self.result_offsets = out[: card.value * nout].view(card.value, nout)
+            return self.result_offsets
+
+
+class GroupByPredicate:
+    """GroupByPredicate(keys, aggregation_descs) with Hive output types."""
+
+    def __init__(self, keys, aggregation_descs):
+        self.keys = list(keys)
+        self.aggregation_descs = [tuple(a) for a in aggregation_descs]
+
+    def output_types(self, table):
+        types = [table.cols[k].type for k in self.keys]
+        for t, i in self.aggregation_descs:
+            it = abi.LONG if i == abi.COUNT_STAR else table.cols[i].type
+            types.append(lib().ghive_agg_output_type(t, it))
+        return types
+
+    def gpu_execute(self, ctx, table, sel=None, capacity=None):
+        """List of output tensors: keys then aggregates, one row per group (unordered)."""
+        n = sel.numel() if sel is not None else table.row_num
+        cap = capacity if capacity is not None else max(min(n, 1 << 20), 1)
+        types = self.output_types(table)
+        dev = table.columns[0].device if table.columns else torch.device("cuda")
+        karr = (C.c_uint32 * max(len(self.keys), 1))(*self.keys)
+        aarr = (abi.Agg * max(len(self.aggregation_descs), 1))(*[abi.Agg(*a) for a in self.aggregation_descs])
+        while True:
+            outs = [torch.empty(max(cap, 1), dtype=_DTYPE_OF_TYPE[t], device=dev) for t in types]
+            ocols = (abi.Column * max(len(outs), 1))()
+            for i, o in enumerate(outs):
+                ocols[i].data = o.data_ptr()
+            g = C.c_uint64()
+            st = lib().ghive_groupby(ctx.h, table.ref(), _p(sel), n, C.cast(karr, C.c_void_p), len(self.keys),
+                                     aarr, len(self.aggregation_descs), ocols, cap, C.byref(g))
+            if st == abi.ERR_CAPACITY and capacity is None and cap < max(n, 1):
+                cap = max(n, 1)
+                continue
+            ctx.check(st)
+            return [o[: g.value] for o in outs]
+
+
+def sort(ctx, table, key_cols, orders):
+    """SinkOperator ORDER BY: permutation of row ids (int32 tensor) in key order, ties by row id."""
+    perm = torch.empty(max(table.row_num, 1), dtype=torch.int32, device=table.columns[0].device)
+    karr = (C.c_uint32 * max(len(key_cols), 1))(*key_cols)
+    ctx.check(lib().ghive_sort(ctx.h, table.ref(), C.cast(karr, C.c_void_p), "".join(orders).encode(),
+                               len(key_cols), _p(perm)))
+    return perm[: table.row_num]
+
+
+class Fragment:
+    """Fused CS1 fragment: fact table, star joins (fact column, HashTable), keys, aggregates."""
+
+    def __init__(self, fact, joins, keys, aggs, fact_filter=None):
+        self.fact = fact
+        self.joins_py = list(joins)
+        self.jarr = (abi.FragmentJoin * max(len(joins), 1))(
+            *[abi.FragmentJoin(c, ht.h.value) for c, ht in joins])
+        self.karr = (abi.FragmentKey * max(len(keys), 1))(*[abi.FragmentKey(*k) for k in keys])
+        self.aarr = (abi.FragmentAgg * max(len(aggs), 1))(*[abi.FragmentAgg(*a) for a in aggs])
+        f = abi.Fragment()
+        f.fact = fact.table
+        if fact_filter is not None:
+            f.filter = C.cast(fact_filter._arr, C.POINTER(abi.Pred))
+            f.filter_nnodes, f.filter_root = len(fact_filter.nodes), fact_filter.root
+            self._filter = fact_filter
+        f.joins = C.cast(self.jarr, C.POINTER(abi.FragmentJoin))
+        f.njoins = len(joins)
+        f.keys = C.cast(self.karr, C.POINTER(abi.FragmentKey))
+        f.nkeys = len(keys)
+        f.aggs = C.cast(self.aarr, C.POINTER(abi.FragmentAgg))
+        f.naggs = len(aggs)
+        self.f = f
+        self.nkeys, self.naggs = len(keys), len(aggs)
+
+    def execute(self, ctx, capacity=4096, late=False, outs=None):
+        dev = self.fact.columns[0].device
+        if outs is None:
+            outs = ([torch.empty(capacity, dtype=torch.int32, device=dev) for _ in range(self.nkeys)] +
+                    [torch.empty(capacity, dtype=torch.int64, device=dev) for _ in range(self.naggs)])
+        ocols = (abi.Column * len(outs))()
+        for i, o in enumerate(outs):
+            ocols[i].data = o.data_ptr()
+        g = C.c_uint64()
+        fn = lib().ghive_fragment_execute_late if late else lib().ghive_fragment_execute
+        ctx.check(fn(ctx.h, C.byref(self.f), ocols, capacity, C.byref(g)))
+        return [o[: g.value] for o in outs]

@@ -1,0 +1,124 @@
+"""Synthetic workloads of the star-join fragment, generated on the device with Philox4x32-10.
+
+Column c of a table draws from counter (row, c) under the table's seed, so the host can reproduce
+any slice. Dimension keys are dense and unique (1..N), as in SSB.
+
+  small   : fact 6M rows (fk U[1,200000], measure U[1,100], price U[1e5,1e7]);
+            dim 200k rows (key, attr U[0,6], category U[0,24]);
+            filter dim.attr == 3 -> join on fk -> GROUP BY category SUM(measure*price) (int64)
+  star600m: fact 600M rows (fk_date, fk_cust, fk_supp, fk_part, quantity U[1,50],
+            price U[1e5,1e7], discount U[0,10]); date 2556 rows (year 1992..1998, month 1..12),
+            customer 3M, supplier 200k (region U[0,4]), part 1.4M (category U[0,24],
+            brand U[0,999]); filters year BETWEEN 1992 AND 1993, region = 1, category = 1;
+            three joins -> GROUP BY year, brand SUM(price*discount) (int64)
+"""
+from dataclasses import dataclass, field
+
+import torch
+
+from . import abi
+from . import ops
+
+SEED_BASE = 0x6768697665
+
+# column ids of the generator streams
+FACT_FK, FACT_MEASURE, FACT_PRICE = 0, 1, 2
+DIM_ATTR, DIM_CATEGORY = 10, 11
+F_DATE, F_CUST, F_SUPP, F_PART, F_QTY, F_PRICE, F_DISC = 0, 1, 2, 3, 4, 5, 6
+SUPP_REGION, PART_CATEGORY, PART_BRAND = 20, 30, 31
+
+STAR_DIMS = dict(date=2556, customer=3_000_000, supplier=200_000, part=1_400_000)
+
+
+@dataclass
+class SmallTables:
+    fact: ops.Table
+    dim: ops.Table
+    seed: int
+    fact_rows: int
+
+
+def small_tables(ctx, fact_rows=6_000_000, seed=SEED_BASE + 1, dim_rows=200_000, fact_begin=0):
+    fk = ctx.gen_uniform_i32(seed, FACT_FK, fact_rows, 1, dim_rows, row_begin=fact_begin)
+    measure = ctx.gen_uniform_i32(seed, FACT_MEASURE, fact_rows, 1, 100, row_begin=fact_begin)
+    price = ctx.gen_uniform_i32(seed, FACT_PRICE, fact_rows, 100_000, 10_000_000, row_begin=fact_begin)
+    key = ctx.gen_sequence_i32(dim_rows, 1)
+    attr = ctx.gen_uniform_i32(seed, DIM_ATTR, dim_rows, 0, 6)
+    cat = ctx.gen_uniform_i32(seed, DIM_CATEGORY, dim_rows, 0, 24)
+    return SmallTables(ops.Table([fk, measure, price]), ops.Table([key, attr, cat]), seed, fact_rows)
+
+
+def small_dim_table(ctx, dim, attr_value=3, load_factor=0.5):
+    """Filter dim.attr == attr_value and build the (key -> category) hash table."""
+    pred = ops.FilterPredicate([abi.make_pred(abi.FILTER_EQ, 1, [attr_value], data_type=2)], 0)
+    sel = pred.process(ctx, dim)
+    return ops.HashTable(ctx, dim.columns[0], sel=sel, payload=dim.columns[2], load_factor=load_factor)
+
+
+def small_fragment(fact, ht):
+    return ops.Fragment(fact, joins=[(0, ht)], keys=[(abi.SRC_JOIN, 0)],
+                        aggs=[(abi.AGG_SUM, abi.EXPR_MUL, FACT_MEASURE, FACT_PRICE)])
+
+
+@dataclass
+class StarTables:
+    fact: ops.Table
+    date: ops.Table      # key, year, month
+    supplier: ops.Table  # key, region
+    part: ops.Table      # key, category, brand
+    customer: ops.Table  # key
+    seed: int
+    fact_rows: int
+    extra: dict = field(default_factory=dict)
+
+
+def star_dims(ctx, seed=SEED_BASE + 2):
+    dev = f"cuda:{ctx.device}"
+    nd = STAR_DIMS["date"]
+    dkey = ctx.gen_sequence_i32(nd, 1)
+    day = torch.arange(nd, dtype=torch.int32, device=dev)
+    year = torch.clamp(1992 + day // 365, max=1998).to(torch.int32)
+    month = torch.clamp((day % 365) // 31 + 1, max=12).to(torch.int32)
+    date = ops.Table([dkey, year.contiguous(), month.contiguous()])
+    ns = STAR_DIMS["supplier"]
+    supplier = ops.Table([ctx.gen_sequence_i32(ns, 1), ctx.gen_uniform_i32(seed, SUPP_REGION, ns, 0, 4)])
+    npart = STAR_DIMS["part"]
+    part = ops.Table([ctx.gen_sequence_i32(npart, 1), ctx.gen_uniform_i32(seed, PART_CATEGORY, npart, 0, 24),
+                      ctx.gen_uniform_i32(seed, PART_BRAND, npart, 0, 999)])
+    customer = ops.Table([ctx.gen_sequence_i32(STAR_DIMS["customer"], 1)])
+    return date, supplier, part, customer
+
+
+def star_fact(ctx, fact_rows, seed=SEED_BASE + 2, fact_begin=0):
+    g = lambda col, lo, hi: ctx.gen_uniform_i32(seed, col, fact_rows, lo, hi, row_begin=fact_begin)
+    cols = [g(F_DATE, 1, STAR_DIMS["date"]), g(F_CUST, 1, STAR_DIMS["customer"]),
+            g(F_SUPP, 1, STAR_DIMS["supplier"]), g(F_PART, 1, STAR_DIMS["part"]),
+            g(F_QTY, 1, 50), g(F_PRICE, 100_000, 10_000_000), g(F_DISC, 0, 10)]
+    return ops.Table(cols)
+
+
+def star_tables(ctx, fact_rows=600_000_000, seed=SEED_BASE + 2):
+    date, supplier, part, customer = star_dims(ctx, seed)
+    return StarTables(star_fact(ctx, fact_rows, seed), date, supplier, part, customer, seed, fact_rows)
+
+
+def star_hash_tables(ctx, T, year_lo=1992, year_hi=1993, region=1, category=1, load_factor=0.5):
+    """Filtered dimension hash tables: date (-> year), supplier (existence), part (-> brand)."""
+    P = ops.FilterPredicate
+    dsel = P([abi.make_pred(abi.FILTER_EQ_RANGE, 1, [year_lo, year_hi], data_type=2)], 0).process(ctx, T.date)
+    ssel = P([abi.make_pred(abi.FILTER_EQ, 1, [region], data_type=2)], 0).process(ctx, T.supplier)
+    psel = P([abi.make_pred(abi.FILTER_EQ, 1, [category], data_type=2)], 0).process(ctx, T.part)
+    hd = ops.HashTable(ctx, T.date.columns[0], sel=dsel, payload=T.date.columns[1], load_factor=load_factor)
+    hs = ops.HashTable(ctx, T.supplier.columns[0], sel=ssel, payload=T.supplier.columns[1],
+                       load_factor=load_factor)
+    hp = ops.HashTable(ctx, T.part.columns[0], sel=psel, payload=T.part.columns[2], load_factor=load_factor)
+    return hd, hs, hp
+
+
+def star_fragment(fact, hd, hs, hp):
+    return ops.Fragment(fact, joins=[(F_DATE, hd), (F_SUPP, hs), (F_PART, hp)],
+                        keys=[(abi.SRC_JOIN, 0), (abi.SRC_JOIN, 2)],
+                        aggs=[(abi.AGG_SUM, abi.EXPR_MUL, F_PRICE, F_DISC)])
+
+
+STAR_REFERENCED_BYTES_PER_ROW = 20  # fk_date, fk_supp, fk_part, price, discount (int32)

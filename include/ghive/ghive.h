@@ -323,6 +323,16 @@ typedef struct ghive_fragment {
 int ghive_fragment_execute(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_cols,
                            uint64_t capacity, uint64_t *out_groups);
 
+/* Same operator with late materialisation: a fact column is read only for the lanes that
+ * still hold a row after the previous join (measured beside the default form). */
+int ghive_fragment_execute_late(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_cols,
+                                uint64_t capacity, uint64_t *out_groups);
+
+/* Tunables of a context: "fragment.lds_budget" (bytes, <= 160 KiB), "fragment.late" (0/1),
+ * "fragment.grid" (workgroups, 0 = one resident wave), "groupby.lds_budget" (bytes). */
+int ghive_set_option(ghive_ctx *ctx, const char *name, int64_t value);
+int ghive_get_option(ghive_ctx *ctx, const char *name, int64_t *value);
+
 /* Per-stage timing of the last fragment on this context (HIP events, milliseconds). */
 typedef struct ghive_stage_times {
   float total_ms;

@@ -36,3 +36,24 @@ def np_col(spec):
         return ("string", spec["blob"].encode(), np.array(spec["idx"], dtype=np.int32))
     dt = {"LONG": np.int64, "INT": np.int32, "DOUBLE": np.float64, "FLOAT": np.float32}[t]
     return np.array(spec["values"], dtype=dt)
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    """One HIP context for the GPU tests (session-wide; tests run in one process)."""
+    import torch
+    from ghive_amd import ops
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    c = ops.Context(0)
+    yield c
+    c.close()
+
+
+def to_dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def to_np(t):
+    return t.cpu().numpy()

@@ -1,0 +1,196 @@
+"""GPU parity of the hash group-by against the oracle: integer aggregates bit-exact, double
+SUM/AVG within 1e-12 relative of the correctly rounded oracle sum (BASELINE tolerance)."""
+import math
+from collections import defaultdict
+
+import numpy as np
+import pytest
+
+from conftest import np_col, to_dev, to_np
+from ghive_amd import abi
+
+pytestmark = pytest.mark.gpu
+INT_NULL, LONG_NULL = abi.INT_NULL, abi.LONG_NULL
+REL = 1e-12
+
+
+def _run(ctx, cols, keys, aggs, sel=None, capacity=None):
+    from ghive_amd import ops
+    t = ops.Table([to_dev(c) for c in cols])
+    outs = ops.GroupByPredicate(keys, aggs).gpu_execute(
+        ctx, t, sel=None if sel is None else to_dev(sel.astype(np.int32)), capacity=capacity)
+    return [to_np(o) for o in outs]
+
+
+def _as_dict(outs, nkeys):
+    res = {}
+    for g in range(len(outs[0]) if outs else 0):
+        key = tuple(o[g].item() for o in outs[:nkeys])
+        res[key] = [o[g].item() for o in outs[nkeys:]]
+    return res
+
+
+def _is_null(v):
+    return v is None or v == LONG_NULL or v == INT_NULL or (isinstance(v, float) and math.isnan(v))
+
+
+def _same(a, b):
+    if _is_null(a) or _is_null(b):
+        return _is_null(a) and _is_null(b)
+    if isinstance(a, float) or isinstance(b, float):
+        return a == b or abs(a - b) <= REL * max(abs(a), abs(b))
+    return a == b
+
+
+def _oracle_dict(orc, cols, keys, aggs, sel=None):
+    rep, vals, nulls = orc.groupby(cols, keys, aggs, sel=sel)
+    res = {}
+    for g, r in enumerate(rep):
+        key = tuple(cols[k][r].item() for k in keys)
+        key = tuple(float("nan") if isinstance(x, float) and math.isnan(x) else x for x in key)
+        res[key] = [None if nulls[a][g] else vals[a][g].item() for a in range(len(aggs))]
+    return res
+
+
+def _compare(got, want):
+    def norm(k):
+        return tuple("nan" if isinstance(x, float) and math.isnan(x) else x for x in k)
+    g = {norm(k): v for k, v in got.items()}
+    w = {norm(k): v for k, v in want.items()}
+    assert g.keys() == w.keys()
+    for k in w:
+        for a, b in zip(g[k], w[k]):
+            assert _same(a, b), (k, g[k], w[k])
+
+
+def test_gpu_impl_kats_numeric_keys(ctx, kats):
+    for k in kats["groupby_gpu_impl"]:
+        if any(c["type"] == "STRING" for c in k["columns"]):
+            continue  # STRING keys are out of scope on the device
+        cols = [np_col(c) for c in k["columns"]]
+        outs = _run(ctx, cols, k["keys"], [tuple(a) for a in k["aggs"]])
+        got = _as_dict(outs, len(k["keys"]))
+        want = {tuple(e["key"]): e["aggs"] for e in k["expect"]}
+        assert got == want, k["cite"]
+
+
+_AGG = {"sum": abi.AGG_SUM, "min": abi.AGG_MIN, "max": abi.AGG_MAX, "avg": abi.AGG_AVG,
+        "count": abi.AGG_CNT}
+
+
+def test_hive_aggregate_kats(ctx, kats):
+    for k in kats["hive_groupby"]:
+        dbl = k.get("dtype") == "DOUBLE"
+        vals = np.array([(np.nan if dbl else LONG_NULL) if v is None else v for v in k["values"]],
+                        dtype=np.float64 if dbl else np.int64)
+        cols, keys = [vals], []
+        if k["keys"] is not None:
+            cols.append(np.array([LONG_NULL if v is None else v for v in k["keys"]], dtype=np.int64))
+            keys = [1]
+        outs = _run(ctx, cols, keys, [(_AGG[k["agg"]], 0)])
+        if k["keys"] is None:
+            assert len(outs[0]) == 1
+            v = outs[0][0].item()
+            assert (None if _is_null(v) else v) == k["expect"], k["cite"]
+        else:
+            got = {(None if kk == LONG_NULL else kk): (None if _is_null(v) else v)
+                   for kk, v in zip(outs[0].tolist(), outs[1].tolist())}
+            assert got == {a: b for a, b in k["expect"]}, k["cite"]
+
+
+@pytest.mark.parametrize("card", [7, 3000, 200_000])
+@pytest.mark.parametrize("ktype", ["INT", "LONG", "DOUBLE", "INT2"])
+def test_groupby_all_aggs_vs_oracle(ctx, orc, card, ktype):
+    rng = np.random.default_rng(card + len(ktype))
+    n = 400_000
+    if ktype == "INT2":
+        k1 = rng.integers(0, max(card // 50, 2), n).astype(np.int32)
+        k2 = rng.integers(-25, 25, n).astype(np.int32)
+        kcols, keys = [k1, k2], [0, 1]
+    elif ktype == "DOUBLE":
+        k1 = rng.integers(0, card, n).astype(np.float64) / 4
+        k1[::211] = np.nan
+        kcols, keys = [k1], [0]
+    else:
+        dt = np.int32 if ktype == "INT" else np.int64
+        k1 = rng.integers(-card // 2, card - card // 2, n).astype(dt)
+        k1[::101] = INT_NULL if ktype == "INT" else LONG_NULL
+        kcols, keys = [k1], [0]
+    v_long = rng.integers(-2**62, 2**62, n, dtype=np.int64)
+    v_long[::37] = LONG_NULL
+    v_int = rng.integers(-1000, 1000, n).astype(np.int32)
+    v_dbl = rng.standard_normal(n) * 10.0 ** rng.integers(-3, 4, n)
+    v_dbl[::41] = np.nan
+    v_flt = rng.standard_normal(n).astype(np.float32)
+    cols = kcols + [v_long, v_int, v_dbl, v_flt]
+    b = len(kcols)
+    aggs = [(abi.AGG_SUM, b), (abi.AGG_MIN, b), (abi.AGG_MAX, b + 1), (abi.AGG_CNT, b),
+            (abi.AGG_AVG, b + 1), (abi.AGG_SUM, b + 2), (abi.AGG_CNT, abi.COUNT_STAR),
+            (abi.AGG_MAX, b + 2)]
+    outs = _run(ctx, cols, keys, aggs)
+    _compare(_as_dict(outs, len(keys)), _oracle_dict(orc, cols, keys, aggs))
+    aggs2 = [(abi.AGG_MIN, b + 3), (abi.AGG_AVG, b + 2), (abi.AGG_SUM, b + 3)]
+    outs = _run(ctx, cols, keys, aggs2)
+    _compare(_as_dict(outs, len(keys)), _oracle_dict(orc, cols, keys, aggs2))
+
+
+def test_groupby_with_selection_and_empty(ctx, orc):
+    rng = np.random.default_rng(2)
+    n = 100_000
+    k = rng.integers(0, 40, n).astype(np.int32)
+    v = rng.integers(0, 100, n).astype(np.int64)
+    sel = np.nonzero(rng.random(n) < 0.3)[0].astype(np.uint32)
+    outs = _run(ctx, [k, v], [0], [(abi.AGG_SUM, 1)], sel=sel)
+    _compare(_as_dict(outs, 1), _oracle_dict(orc, [k, v], [0], [(abi.AGG_SUM, 1)], sel=sel))
+    # keyless over no rows: one group, SUM NULL, COUNT 0
+    outs = _run(ctx, [k[:0], v[:0]], [], [(abi.AGG_SUM, 1), (abi.AGG_CNT, abi.COUNT_STAR)])
+    assert len(outs[0]) == 1 and outs[0][0] == LONG_NULL and outs[1][0] == 0
+    # keyed over no rows: no groups
+    outs = _run(ctx, [k[:0], v[:0]], [0], [(abi.AGG_SUM, 1)])
+    assert len(outs[0]) == 0
+
+
+def test_double_sum_order_independent_and_exact(ctx):
+    """Cancelling sums: every group's SUM equals math.fsum of its values (correct rounding)."""
+    rng = np.random.default_rng(12)
+    g = 5000
+    base = rng.standard_normal(g * 10) * 10.0 ** rng.integers(-8, 8, g * 10)
+    vals = np.concatenate([base, -base[: g * 5]])
+    keys = np.concatenate([np.repeat(np.arange(g), 10), np.repeat(np.arange(g), 10)[: g * 5]]).astype(np.int64)
+    perm = rng.permutation(len(vals))
+    outs1 = _run(ctx, [keys[perm], vals[perm]], [0], [(abi.AGG_SUM, 1)])
+    perm2 = rng.permutation(len(vals))
+    outs2 = _run(ctx, [keys[perm2], vals[perm2]], [0], [(abi.AGG_SUM, 1)])
+    d1, d2 = _as_dict(outs1, 1), _as_dict(outs2, 1)
+    assert d1 == d2  # bit-identical under a different arrival order
+    groups = defaultdict(list)
+    for kk, vv in zip(keys.tolist(), vals.tolist()):
+        groups[kk].append(vv)
+    worst = 0.0
+    for (kk,), (s,) in d1.items():
+        exact = math.fsum(groups[kk])
+        if exact != 0:
+            worst = max(worst, abs(s - exact) / abs(exact))
+    assert worst <= REL
+
+
+def test_groupby_capacity_error(ctx):
+    from ghive_amd._lib import GhiveError
+    k = np.arange(1000, dtype=np.int64)
+    with pytest.raises(GhiveError) as e:
+        _run(ctx, [k, k], [0], [(abi.AGG_SUM, 1)], capacity=100)
+    assert e.value.status == abi.ERR_CAPACITY
+
+
+def test_groupby_high_cardinality_1e7(ctx):
+    """10M rows, ~5M distinct LONG keys: Σcount = rows and ΣSUM = global SUM (int64, exact)."""
+    import torch
+    from ghive_amd import ops
+    n = 10_000_000
+    k = ctx.gen_uniform_i64(5, 0, n, 0, 5_000_000)
+    v = ctx.gen_uniform_i64(5, 1, n, -1000, 1000)
+    outs = ops.GroupByPredicate([0], [(abi.AGG_CNT, abi.COUNT_STAR), (abi.AGG_SUM, 1)]).gpu_execute(
+        ctx, ops.Table([k, v]), capacity=6_000_000)
+    assert int(outs[1].sum().item()) == n
+    assert int(outs[2].sum().item()) == int(v.sum().item())
+    assert outs[0].numel() == torch.unique(k).numel()

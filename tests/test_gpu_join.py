@@ -1,0 +1,104 @@
+"""GPU parity of hash build/probe and the N-way join against the oracle (row-pair multisets)."""
+import numpy as np
+import pytest
+
+from conftest import to_dev, to_np
+from ghive_amd import abi
+
+pytestmark = pytest.mark.gpu
+INT_NULL, LONG_NULL = abi.INT_NULL, abi.LONG_NULL
+
+
+def _pairs(rows, pay):
+    return sorted(zip(rows.astype(np.int64).tolist(), pay.astype(np.int64).tolist()))
+
+
+@pytest.mark.parametrize("ktype", ["INT", "LONG"])
+@pytest.mark.parametrize("payload", [None, "INT", "LONG"])
+def test_probe_duplicates_nulls_vs_oracle(ctx, orc, ktype, payload):
+    from ghive_amd import ops
+    rng = np.random.default_rng(11)
+    dt = np.int32 if ktype == "INT" else np.int64
+    null = INT_NULL if ktype == "INT" else LONG_NULL
+    bk = rng.integers(-500, 500, 30000).astype(dt)
+    bk[::50] = null
+    pk = rng.integers(-700, 700, 200000).astype(dt)
+    pk[::31] = null
+    pay = None
+    if payload == "INT":
+        pay = rng.integers(-2**31, 2**31 - 1, 30000).astype(np.int32)
+    elif payload == "LONG":
+        pay = rng.integers(-2**62, 2**62, 30000).astype(np.int64)
+    ht = ops.HashTable(ctx, to_dev(bk), payload=None if pay is None else to_dev(pay))
+    info = ht.info()
+    assert not info["unique"] and info["entries"] == int((bk != null).sum())
+    rows, vals = ht.probe(to_dev(pk))
+    want_rows, want_vals = orc.mapjoin_inner(bk, pk, payload=pay)
+    assert _pairs(to_np(rows), to_np(vals)) == _pairs(want_rows, want_vals)
+    ht.free()
+
+
+def test_probe_unique_with_selections(ctx, orc):
+    from ghive_amd import ops
+    rng = np.random.default_rng(3)
+    bk = (rng.permutation(400000)[:150000] * 3 + 1).astype(np.int32)
+    pk = rng.integers(0, 1200000, 1_000_000).astype(np.int32)
+    bsel = np.nonzero(rng.random(len(bk)) < 0.4)[0].astype(np.uint32)
+    psel = np.nonzero(rng.random(len(pk)) < 0.6)[0].astype(np.uint32)
+    for lf in (0.25, 0.5, 0.9):
+        ht = ops.HashTable(ctx, to_dev(bk), sel=to_dev(bsel.astype(np.int32)), load_factor=lf)
+        assert ht.info()["unique"]
+        rows, vals = ht.probe(to_dev(pk), sel=to_dev(psel.astype(np.int32)))
+        want = orc.mapjoin_inner(bk, pk, build_sel=bsel, probe_sel=psel)
+        assert _pairs(to_np(rows), to_np(vals)) == _pairs(*want)
+        ht.free()
+
+
+def test_probe_edge_cases(ctx):
+    from ghive_amd import ops
+    empty = to_dev(np.zeros(0, dtype=np.int64))
+    ht = ops.HashTable(ctx, empty)
+    rows, vals = ht.probe(to_dev(np.arange(1000, dtype=np.int64)))
+    assert rows.numel() == 0
+    ht2 = ops.HashTable(ctx, to_dev(np.array([INT_NULL, 5], dtype=np.int32)))
+    rows, vals = ht2.probe(to_dev(np.array([INT_NULL, 5, 6], dtype=np.int32)))  # NULL never joins
+    assert to_np(rows).tolist() == [1] and to_np(vals).tolist() == [1]
+    # a LONG probe key outside the INT range never matches an INT table
+    rows, _ = ht2.probe(to_dev(np.array([5 + 2**32, 5], dtype=np.int64)))
+    assert to_np(rows).tolist() == [1]
+
+
+def test_probe_capacity_error_reports_need(ctx):
+    from ghive_amd import ops
+    from ghive_amd._lib import GhiveError
+    ht = ops.HashTable(ctx, to_dev(np.zeros(100, dtype=np.int64)))  # one key, 100 entries
+    with pytest.raises(GhiveError) as e:
+        ht.probe(to_dev(np.zeros(10, dtype=np.int64)), capacity=50)
+    assert e.value.status == abi.ERR_CAPACITY
+    rows, _ = ht.probe(to_dev(np.zeros(10, dtype=np.int64)))
+    assert rows.numel() == 1000
+
+
+def test_join_reference_kat(ctx, kats):
+    from ghive_amd import ops
+    for k in kats["join"]:
+        tables = [ops.Table([to_dev(np.array(t, dtype=np.int64))]) for t in k["tables"]]
+        jp = ops.JoinPredicate(k["join_cols"], [tuple(c) for c in k["conds"]])
+        got = to_np(jp.process(ctx, tables))
+        want = np.array(k["expect_offsets"], dtype=np.int32).reshape(-1, k["width"])
+        assert sorted(map(tuple, got.tolist())) == sorted(map(tuple, want.tolist())), k["cite"]
+
+
+@pytest.mark.parametrize("types", [(0, 0), (1, 0), (4, 1), (0, 1), (1, 4)])
+def test_join_nway_vs_oracle(ctx, orc, types):
+    from ghive_amd import ops
+    rng = np.random.default_rng(sum(types) + 8)
+    t0 = rng.integers(0, 300, 20000).astype(np.int64)
+    t0[::101] = LONG_NULL
+    t1 = rng.integers(0, 300, 600).astype(np.int32)
+    t2 = rng.integers(100, 400, 500).astype(np.int64)
+    conds = [(types[0], 0, 1), (types[1], 0, 2)]
+    jp = ops.JoinPredicate([0, 0, 0], conds)
+    got = to_np(jp.process(ctx, [ops.Table([to_dev(t)]) for t in (t0, t1, t2)]))
+    want = orc.join([[t0], [t1], [t2]], [0, 0, 0], conds)
+    assert sorted(map(tuple, got.tolist())) == sorted(map(tuple, want.tolist()))

@@ -1,0 +1,72 @@
+"""GPU parity of the LSD radix ORDER BY: the permutation equals the oracle's total order
+(keys, then row id) exactly."""
+import numpy as np
+import pytest
+
+from conftest import to_dev, to_np
+from ghive_amd import abi
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(ctx, orc, cols, keys, orders):
+    from ghive_amd import ops
+    perm = to_np(ops.sort(ctx, ops.Table([to_dev(c) for c in cols]), keys, orders)).astype(np.uint32)
+    want = orc.sort_perm(cols, keys, orders)
+    assert np.array_equal(perm, want)
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 4095, 4096, 4097, 250_001])
+def test_sort_int64_float32(ctx, orc, n):
+    rng = np.random.default_rng(n)
+    k = rng.integers(-50, 50, n).astype(np.int64)
+    f = rng.random(n).astype(np.float32)
+    f[::53] = np.nan
+    _check(ctx, orc, [k, f], [0, 1], ["+", "+"])
+
+
+@pytest.mark.parametrize("ctype", ["INT", "LONG", "DOUBLE", "FLOAT"])
+@pytest.mark.parametrize("order", ["+", "-"])
+def test_sort_single_key(ctx, orc, ctype, order):
+    rng = np.random.default_rng(7)
+    n = 300_000
+    if ctype == "INT":
+        x = rng.integers(-2**31, 2**31 - 1, n).astype(np.int32)
+        x[::99] = abi.INT_NULL
+    elif ctype == "LONG":
+        x = rng.integers(-2**63, 2**63 - 1, n, dtype=np.int64)
+        x[::7] = x[::7] % 1000  # many duplicates
+    elif ctype == "DOUBLE":
+        x = rng.standard_normal(n) * 1e300 ** rng.random(n)
+        x[::31] = np.nan
+        x[::37] = -0.0
+        x[::41] = 0.0
+        x[::43] = np.inf
+        x[::47] = -np.inf
+    else:
+        x = (rng.standard_normal(n) * 100).astype(np.float32)
+        x[::31] = np.nan
+    _check(ctx, orc, [x], [0], [order])
+
+
+def test_sort_composite_mixed_orders(ctx, orc):
+    rng = np.random.default_rng(3)
+    n = 200_000
+    a = rng.integers(0, 5, n).astype(np.int32)
+    b = rng.integers(-3, 3, n).astype(np.int64)
+    c = (rng.integers(0, 4, n) / 2).astype(np.float64)
+    _check(ctx, orc, [a, b, c], [0, 1, 2], ["-", "+", "-"])
+    _check(ctx, orc, [a, b, c], [2, 0], ["+", "+"])
+
+
+def test_sort_large_sorted_and_permutation(ctx):
+    """20M rows: keys non-decreasing along the permutation, and it is a permutation."""
+    import torch
+    from ghive_amd import ops
+    n = 20_000_000
+    k = ctx.gen_uniform_i64(9, 0, n, -10**12, 10**12)
+    f = ctx.gen_uniform_f32(9, 1, n)
+    perm = ops.sort(ctx, ops.Table([k, f]), [0, 1], ["+", "+"]).to(torch.int64) & 0xFFFFFFFF
+    ks, fs = k[perm], f[perm]
+    assert bool(((ks[1:] > ks[:-1]) | ((ks[1:] == ks[:-1]) & (fs[1:] >= fs[:-1]))).all().item())
+    assert bool((torch.bincount(perm, minlength=n) == 1).all().item())
