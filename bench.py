@@ -66,7 +66,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rows", type=int, default=600_000_000)
     ap.add_argument("--late", action="store_true", help="late-materialising kernel variant")
-    ap.add_argument("--cpu-sample", type=int, default=60_000_000)
+    ap.add_argument("--no-dense", action="store_true", help="probe every dimension through its hash table")
+    ap.add_argument("--no-lds-bits", action="store_true", help="keep dimension bitmaps in HBM/L2")
+    ap.add_argument("--cpu-sample", type=int, default=600_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -80,6 +82,12 @@ def main():
     ctx = ops.Context(0)
     if args.late:
         ctx.set_option("fragment.late", 1)
+    if args.no_dense:
+        ctx.set_option("fragment.dense", 0)
+    if args.no_lds_bits:
+        ctx.set_option("fragment.lds_bits", 0)
+    variant = ("late" if args.late else "early") + ("/hash" if args.no_dense else
+                                                    ("/hbm-bitmap" if args.no_lds_bits else "/lds-bitmap"))
     info = ctx.device_info()
     log(f"device {info['name']} {info['arch']} CUs={info['compute_units']}")
     t0 = time.time()
@@ -115,7 +123,7 @@ def main():
 
     traffic = None
     prof = load_traffic("star600m_fragment_traffic.json")
-    if prof and prof.get("rows") == args.rows and bool(prof.get("late", False)) == args.late:
+    if prof and prof.get("rows") == args.rows and prof.get("variant") == variant:
         traffic = prof.get("bytes_per_launch")
 
     cpu = None
@@ -135,7 +143,7 @@ def main():
         "config": {"workload": "star-join fragment, plain form: 600M-row fact x date(2556) x "
                                "supplier(200k) x part(1.4M), GROUP BY year, brand SUM(price*discount)",
                    "fact_rows": args.rows, "groups": groups, "rows_joined": st["rows_joined"],
-                   "kernel_variant": "late" if args.late else "early",
+                   "kernel_variant": variant,
                    "fused_kernel_ms": k_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},

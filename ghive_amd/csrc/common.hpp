@@ -54,6 +54,8 @@ struct ghive_tuning {
   int64_t frag_lds_budget = 64 * 1024;  // bytes of LDS for the fragment's private group table
   int64_t frag_late = 0;                // 1: read a column only for lanes holding live rows
   int64_t frag_grid = 0;                // 0: one resident wave of workgroups
+  int64_t frag_lds_bits = 1;            // 1: copy small dimension bitmaps into LDS
+  int64_t frag_dense = 1;               // 1: probe dense dimensions through bitmap + payload array
   int64_t gb_lds_budget = 64 * 1024;    // bytes of LDS for the group-by's private table
 };
 

@@ -288,6 +288,10 @@ int ghive_set_option(ghive_ctx *ctx, const char *name, int64_t value) {
   } else if (k == "fragment.grid") {
     if (value < 0 || value > (1 << 24)) return set_error(ctx, GHIVE_ERR_INVALID, "option %s: %lld", name, (long long)value);
     ctx->tune.frag_grid = value;
+  } else if (k == "fragment.lds_bits") {
+    ctx->tune.frag_lds_bits = value ? 1 : 0;
+  } else if (k == "fragment.dense") {
+    ctx->tune.frag_dense = value ? 1 : 0;
   } else if (k == "groupby.lds_budget") {
     if (value < 0 || value > 160 * 1024) return set_error(ctx, GHIVE_ERR_INVALID, "option %s: %lld", name, (long long)value);
     ctx->tune.gb_lds_budget = value;
@@ -303,6 +307,8 @@ int ghive_get_option(ghive_ctx *ctx, const char *name, int64_t *value) {
   if (k == "fragment.lds_budget") *value = ctx->tune.frag_lds_budget;
   else if (k == "fragment.late") *value = ctx->tune.frag_late;
   else if (k == "fragment.grid") *value = ctx->tune.frag_grid;
+  else if (k == "fragment.lds_bits") *value = ctx->tune.frag_lds_bits;
+  else if (k == "fragment.dense") *value = ctx->tune.frag_dense;
   else if (k == "groupby.lds_budget") *value = ctx->tune.gb_lds_budget;
   else return set_error(ctx, GHIVE_ERR_INVALID, "unknown option '%s'", name);
   return GHIVE_OK;

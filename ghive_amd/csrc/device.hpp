@@ -184,6 +184,62 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
   out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
+// ---- keys and skew for the join/sort workloads ------------------------------------------------
+// A bijection of [0, 2^62): add a seed-derived offset, then xorshift / odd-multiply rounds, all
+// mod 2^62. Distinct ranks give distinct keys; keys stay below LONG_MAX (the NULL sentinel).
+__host__ __device__ __forceinline__ uint64_t key62(uint64_t seed, uint64_t rank) {
+  const uint64_t M = (1ull << 62) - 1;
+  uint64_t x = (rank + seed * 0x9E3779B97F4A7C15ull) & M;
+  x ^= x >> 31;
+  x = (x * 0x7fb5d329728ea185ull) & M;
+  x ^= x >> 27;
+  x = (x * 0x81dadef4bc2dd44dull) & M;
+  x ^= x >> 33;
+  return x;
+}
+
+// Rejection-inversion sampling of a Zipf(s) rank in [1, n] (W. Hoermann, G. Derflinger,
+// "Rejection-inversion to generate variates from monotone discrete distributions", ACM TOMACS
+// 6(3), 1996): invert the integral H of h(x) = x^-s, then accept or reject the rounded value.
+struct ZipfParams {
+  double s, n, h_x1, h_n, sx;
+};
+__host__ __device__ __forceinline__ double zipf_helper1(double x) {  // log1p(x)/x
+  return fabs(x) > 1e-8 ? log1p(x) / x : 1.0 - x * (0.5 - x * (1.0 / 3.0 - 0.25 * x));
+}
+__host__ __device__ __forceinline__ double zipf_helper2(double x) {  // expm1(x)/x
+  return fabs(x) > 1e-8 ? expm1(x) / x : 1.0 + x * 0.5 * (1.0 + x * (1.0 / 3.0) * (1.0 + 0.25 * x));
+}
+__host__ __device__ __forceinline__ double zipf_H(double s, double x) {
+  double lx = log(x);
+  return zipf_helper2((1.0 - s) * lx) * lx;
+}
+__host__ __device__ __forceinline__ double zipf_h(double s, double x) { return exp(-s * log(x)); }
+__host__ __device__ __forceinline__ double zipf_Hinv(double s, double y) {
+  double t = y * (1.0 - s);
+  if (t < -1.0) t = -1.0;
+  return exp(zipf_helper1(t) * y);
+}
+inline ZipfParams zipf_params(double s, uint64_t n) {
+  ZipfParams p;
+  p.s = s;
+  p.n = (double)n;
+  p.h_x1 = zipf_H(s, 1.5) - 1.0;
+  p.h_n = zipf_H(s, p.n + 0.5);
+  p.sx = 2.0 - zipf_Hinv(s, zipf_H(s, 2.5) - zipf_h(s, 2.0));
+  return p;
+}
+// One attempt with uniform u in [0,1); returns the rank, or 0 when rejected.
+__host__ __device__ __forceinline__ uint64_t zipf_try(const ZipfParams &p, double u) {
+  double uu = p.h_n + u * (p.h_x1 - p.h_n);
+  double x = zipf_Hinv(p.s, uu);
+  double kf = floor(x + 0.5);
+  if (kf < 1.0) kf = 1.0;
+  if (kf > p.n) kf = p.n;
+  if (kf - x <= p.sx || uu >= zipf_H(p.s, kf + 0.5) - zipf_h(p.s, kf)) return (uint64_t)kf;
+  return 0;
+}
+
 // ---- hashing -------------------------------------------------------------------------------
 // Fibonacci hashing: multiply by 2^64/phi and keep the top bits (Knuth, TAOCP 6.4).
 __device__ __forceinline__ uint64_t hash_u64(uint64_t k, int log2_slots) {

@@ -1,14 +1,18 @@
-// Fused fragment: fact scan -> optional fact filter -> chain of dimension hash probes ->
-// int64 expression -> hash GROUP BY, in one streaming pass over the fact columns. This is the
-// CS1 map-task pipeline of GHive (TableScan -> Filter -> MapJoin* -> Select -> GroupBy ->
+// Fused fragment: fact scan -> optional fact filter -> chain of dimension probes -> int64
+// expression -> hash GROUP BY, in one streaming pass over the fact columns. This is the CS1
+// map-task pipeline of GHive (TableScan -> Filter -> MapJoin* -> Select -> GroupBy ->
 // ReduceSink), which the reference runs as separate Thrust calls with host round trips
 // (FilterOperator.cu:199-225, SortMergeJoinPredicate.cu:8-461, SelectPredicate.cu:84-409,
 // GroupByPredicate.cu:19-359).
 //
-// Each thread owns 4 consecutive rows per tile (one 16-byte load per referenced column).
-// Dimension tables are probed with the Hive-style min/max pre-check first, most selective join
-// first, and the surviving rows aggregate into an LDS-private group table that is merged into
-// the global table once per workgroup.
+// Each thread owns 4 consecutive rows per tile (one 16-byte load per referenced column). Join
+// and aggregate counts are template parameters, so every per-row array lives in registers.
+// A dimension whose keys span a compact range is probed through its dense companion: one bit
+// test in an LDS copy of its existence bitmap (or in the HBM/L2 bitmap when it does not fit),
+// then one payload load for the survivors. Joins run cheapest-and-most-selective first (rank =
+// probe cost / (1 - selectivity)), with Hive's min/max pre-check in front of every probe. The
+// surviving rows aggregate into an LDS-private group table merged once per workgroup, or
+// straight into the global table when the group bound does not fit LDS.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -27,6 +31,7 @@ namespace ghive {
 constexpr int kFragThreads = 256;
 constexpr int kRowsPerThread = 4;
 constexpr uint64_t kFragTile = (uint64_t)kFragThreads * kRowsPerThread;
+constexpr int kNullI32 = 0x7fffffff;
 
 __device__ __forceinline__ void load4_i32(const int32_t *p, uint64_t r0, uint64_t n, int32_t v[4]) {
   if (r0 + 3 < n) {
@@ -34,18 +39,30 @@ __device__ __forceinline__ void load4_i32(const int32_t *p, uint64_t r0, uint64_
     v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
   } else {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = r0 + q < n ? p[r0 + q] : 0x7fffffff;
+    for (int q = 0; q < 4; ++q) v[q] = r0 + q < n ? p[r0 + q] : kNullI32;
   }
 }
 
-template <bool kLate>
+__device__ __forceinline__ bool in_range(const FragJoinDev &J, int32_t k) {
+  return k != kNullI32 && k >= J.key_min && k <= J.key_max;
+}
+
+template <int NJ, int NA, bool kLate>
 __global__ __launch_bounds__(kFragThreads) void k_fragment(FragDev F, GroupTableDev T, int lcap_log2,
                                                            unsigned long long *joined) {
-  extern __shared__ uint64_t lds[];
-  const bool use_lds = lcap_log2 >= 0;  // uniform: LDS pre-aggregation or global table only
+  extern __shared__ uint64_t lds64[];
+  uint32_t *lbits = reinterpret_cast<uint32_t *>(lds64);
+  uint64_t *ltab = lds64 + F.lds_bitmap_words / 2;
+  const bool use_lds = lcap_log2 >= 0;  // uniform
   const int lstride = use_lds ? (1 << lcap_log2) + 1 : 0;
-  if (use_lds) lds_table_init(lds, T, lcap_log2);
+#pragma unroll
+  for (int j = 0; j < NJ; ++j)
+    if (F.joins[j].mode == kJoinLdsBits)
+      for (uint32_t w = threadIdx.x; w < F.joins[j].nwords; w += blockDim.x)
+        lbits[F.joins[j].lds_word_off + w] = F.joins[j].bitmap[w];
+  if (use_lds) lds_table_init(ltab, T, lcap_log2);
   __syncthreads();
+
   const uint64_t n = F.n;
   const uint64_t ntiles = (n + kFragTile - 1) / kFragTile;
   uint32_t njoined = 0;
@@ -54,12 +71,13 @@ __global__ __launch_bounds__(kFragThreads) void k_fragment(FragDev F, GroupTable
     uint32_t alive = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) alive |= (r0 + q < n ? 1u : 0u) << q;
-    int32_t fk[kMaxFragJoins][4];
-    int32_t av[kMaxFragAggs][4], bv[kMaxFragAggs][4];
-    if (!kLate) {
-      // issue every column load of the tile up front: many bytes in flight per wave
-      for (int j = 0; j < F.njoins; ++j) load4_i32(F.joins[j].fk, r0, n, fk[j]);
-      for (int a = 0; a < F.naggs; ++a) {
+    int32_t fk[NJ > 0 ? NJ : 1][4];
+    int32_t av[NA][4], bv[NA][4];
+    if (!kLate) {  // every column load of the tile issued up front: many bytes in flight
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) load4_i32(F.joins[j].fk, r0, n, fk[j]);
+#pragma unroll
+      for (int a = 0; a < NA; ++a) {
         if (F.aggs[a].expr != GHIVE_EXPR_ONE) load4_i32(F.aggs[a].a, r0, n, av[a]);
         if (F.aggs[a].expr == GHIVE_EXPR_MUL) load4_i32(F.aggs[a].b, r0, n, bv[a]);
       }
@@ -69,22 +87,59 @@ __global__ __launch_bounds__(kFragThreads) void k_fragment(FragDev F, GroupTable
       for (int q = 0; q < 4; ++q)
         if (((alive >> q) & 1u) && !eval_pred(F.filter, r0 + q)) alive &= ~(1u << q);
     }
-    int32_t pay[kMaxFragJoins][4];
-    for (int j = 0; j < F.njoins; ++j) {
+    int32_t pay[NJ > 0 ? NJ : 1][4];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
       const FragJoinDev &J = F.joins[j];
       if (kLate && alive) load4_i32(J.fk, r0, n, fk[j]);
+      if (J.mode == kJoinLdsBits) {
+        const uint32_t *bits = lbits + J.lds_word_off;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (!((alive >> q) & 1u)) continue;
-        const int32_t k = fk[j][q];
-        bool hit = k != 0x7fffffff && k >= J.key_min && k <= J.key_max &&
-                   probe_unique8(J.slots, J.mask, J.log2, k, pay[j][q]);
-        if (!hit) alive &= ~(1u << q);
+        for (int q = 0; q < 4; ++q) {
+          if (!((alive >> q) & 1u)) continue;
+          const int32_t k = fk[j][q];
+          bool hit = false;
+          if (in_range(J, k)) {
+            const uint32_t u = (uint32_t)((int64_t)k - J.key_min);
+            hit = (bits[u >> 5] >> (u & 31)) & 1u;
+          }
+          if (!hit) alive &= ~(1u << q);
+        }
+      } else if (J.mode == kJoinGlobalBits) {
+        uint32_t w[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {  // independent loads, all in flight together
+          w[q] = 0;
+          const int32_t k = fk[j][q];
+          if (((alive >> q) & 1u) && in_range(J, k)) {
+            const uint32_t u = (uint32_t)((int64_t)k - J.key_min);
+            w[q] = J.bitmap[u >> 5] >> (u & 31);
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (!(w[q] & 1u)) alive &= ~(1u << q);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (!((alive >> q) & 1u)) continue;
+          const int32_t k = fk[j][q];
+          int32_t p = 0;
+          bool hit = in_range(J, k) && probe_unique8(J.slots, J.mask, J.log2, k, p);
+          if (!hit) alive &= ~(1u << q);
+          pay[j][q] = p;
+        }
+      }
+      if (J.mode != kJoinHash && J.need_payload) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if ((alive >> q) & 1u) pay[j][q] = J.dense[(uint32_t)((int64_t)fk[j][q] - J.key_min)];
       }
     }
     if (!alive) continue;
     if (kLate) {
-      for (int a = 0; a < F.naggs; ++a) {
+#pragma unroll
+      for (int a = 0; a < NA; ++a) {
         if (F.aggs[a].expr != GHIVE_EXPR_ONE) load4_i32(F.aggs[a].a, r0, n, av[a]);
         if (F.aggs[a].expr == GHIVE_EXPR_MUL) load4_i32(F.aggs[a].b, r0, n, bv[a]);
       }
@@ -95,56 +150,85 @@ __global__ __launch_bounds__(kFragThreads) void k_fragment(FragDev F, GroupTable
       ++njoined;
       uint64_t key = 0;
       for (int k = 0; k < F.nkeys; ++k) {
-        int32_t kv = F.key_src[k] == GHIVE_SRC_JOIN ? pay[F.key_idx[k]][q] : F.key_col[k][r0 + q];
+        int32_t kv = 0;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j)
+          if (F.key_src[k] == GHIVE_SRC_JOIN && F.key_idx[k] == j) kv = pay[j][q];
+        if (F.key_src[k] != GHIVE_SRC_JOIN) kv = F.key_col[k][r0 + q];
         key |= (uint64_t)(uint32_t)kv << (32 * k);
       }
-      const int ls = use_lds ? lds_find_insert(lds, key, lcap_log2) : -1;
+      const int ls = use_lds ? lds_find_insert(ltab, key, lcap_log2) : -1;
       int64_t g = -1;
       if (ls < 0) {
         g = gt_find_insert(T, key);
         if (g < 0) continue;
       }
-      for (int a = 0; a < F.naggs; ++a) {
-        const FragAggDev &E = F.aggs[a];
+#pragma unroll
+      for (int a = 0; a < NA; ++a) {
+        const int expr = F.aggs[a].expr;
         AggVal v;
         v.neg = false;
         v.b = 0;
-        if (E.expr == GHIVE_EXPR_ONE) {
+        if (expr == GHIVE_EXPR_ONE) {
           v.valid = true;
           v.a = 1;
         } else {
-          int32_t x = av[a][q];
-          v.valid = x != 0x7fffffff;
+          const int32_t x = av[a][q];
+          v.valid = x != kNullI32;
           int64_t val = x;
-          if (E.expr == GHIVE_EXPR_MUL) {
-            int32_t y = bv[a][q];
-            v.valid = v.valid && y != 0x7fffffff;
+          if (expr == GHIVE_EXPR_MUL) {
+            const int32_t y = bv[a][q];
+            v.valid = v.valid && y != kNullI32;
             val = (int64_t)((uint64_t)(int64_t)x * (uint64_t)(int64_t)y);
           }
           v.a = (uint64_t)val;
         }
         const AggSpec &A = T.aggs[a];
         if (ls >= 0)
-          acc_apply(A, [&](int w) { return lds + (size_t)(1 + A.word0 + w) * lstride + ls; }, v);
+          acc_apply(A, [&](int w) { return ltab + (size_t)(1 + A.word0 + w) * lstride + ls; }, v);
         else
           acc_apply(A, [&](int w) { return T.acc + (size_t)(A.word0 + w) * T.stride + g; }, v);
       }
     }
   }
   __syncthreads();
-  if (use_lds) lds_table_flush(lds, T, lcap_log2);
-  // rows that survived every join (for reporting)
+  if (use_lds) lds_table_flush(ltab, T, lcap_log2);
   unsigned long long s = njoined;
 #pragma unroll
   for (int d = kWave / 2; d > 0; d >>= 1) s += __shfl_xor(s, d, kWave);
   if (lane_id() == 0 && s) atomicAdd(joined, s);
 }
 
-int fragment_prepare(ghive_ctx *ctx, const ghive_fragment *f, FragDev &F, GroupTableDev &T, KeySpec &K,
-                     uint64_t &group_bound) {
-  memset(&F, 0, sizeof F);
-  memset(&T, 0, sizeof T);
-  memset(&K, 0, sizeof K);
+using FragKernel = void (*)(FragDev, GroupTableDev, int, unsigned long long *);
+
+template <int NJ, int NA>
+static FragKernel kern(bool late) {
+  return late ? k_fragment<NJ, NA, true> : k_fragment<NJ, NA, false>;
+}
+template <int NJ>
+static FragKernel pick_na(int na, bool late) {
+  switch (na) {
+    case 1: return kern<NJ, 1>(late);
+    case 2: return kern<NJ, 2>(late);
+    case 3: return kern<NJ, 3>(late);
+    default: return kern<NJ, 4>(late);
+  }
+}
+static FragKernel pick_kernel(int nj, int na, bool late) {
+  switch (nj) {
+    case 0: return pick_na<0>(na, late);
+    case 1: return pick_na<1>(na, late);
+    case 2: return pick_na<2>(na, late);
+    case 3: return pick_na<3>(na, late);
+    default: return pick_na<4>(na, late);
+  }
+}
+
+int fragment_plan(ghive_ctx *ctx, const ghive_fragment *f, int late, FragPlan &P) {
+  memset(&P, 0, sizeof P);
+  FragDev &F = P.F;
+  GroupTableDev &T = P.T;
+  KeySpec &K = P.K;
   if (!f) return set_error(ctx, GHIVE_ERR_INVALID, "fragment: NULL descriptor");
   GH_TRY(check_table(ctx, &f->fact, "fragment fact"));
   const ghive_table &fact = f->fact;
@@ -164,37 +248,74 @@ int fragment_prepare(ghive_ctx *ctx, const ghive_fragment *f, FragDev &F, GroupT
       return set_error(ctx, GHIVE_ERR_INVALID, "fragment: %s column %u is not 16-byte aligned", what, c);
     return GHIVE_OK;
   };
-  // most selective join first (hit-rate estimate: entries / rows of the dimension)
-  std::vector<uint32_t> order(f->njoins);
-  std::iota(order.begin(), order.end(), 0u);
+  std::vector<int> need_payload(f->njoins, 0);
+  for (uint32_t k = 0; k < f->nkeys; ++k)
+    if (f->keys[k].source == GHIVE_SRC_JOIN && f->keys[k].index < f->njoins) need_payload[f->keys[k].index] = 1;
+  // probe mode of every join: LDS bitmap while the budget lasts (smallest first), else HBM
+  // bitmap for dense tables, else the hash table
+  const uint64_t bits_budget_words = (uint64_t)std::min<int64_t>(ctx->tune.frag_lds_budget, 48 * 1024) / 4;
+  std::vector<int> mode(f->njoins, kJoinHash);
+  std::vector<uint32_t> by_size;
   for (uint32_t j = 0; j < f->njoins; ++j) {
     const ghive_hash_table *h = f->joins[j].table;
     if (!h) return set_error(ctx, GHIVE_ERR_INVALID, "fragment: join %u has no table", j);
     if (h->layout != 8 || !h->unique)
       return set_error(ctx, GHIVE_ERR_UNSUPPORTED,
                        "fragment: join %u needs a unique-key INT table with an INT payload", j);
+    if (need_payload[j] && h->payload_kind == 0)
+      return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "fragment: join %u carries no attribute", j);
     GH_TRY(int_col(f->joins[j].fact_key_col, "join key"));
+    if (h->bitmap && ctx->tune.frag_dense) {
+      mode[j] = kJoinGlobalBits;
+      by_size.push_back(j);
+    }
   }
-  auto rate = [&](uint32_t j) {
+  std::stable_sort(by_size.begin(), by_size.end(),
+                   [&](uint32_t a, uint32_t b) { return f->joins[a].table->range < f->joins[b].table->range; });
+  uint64_t used_words = 0;
+  for (uint32_t j : by_size) {
+    uint64_t w = (f->joins[j].table->range + 31) / 32;
+    if (ctx->tune.frag_lds_bits && used_words + w <= bits_budget_words) {
+      mode[j] = kJoinLdsBits;
+      used_words += w;
+    }
+  }
+  // order: rank = probe cost / (1 - selectivity), selectivity estimated as entries / rows
+  std::vector<uint32_t> order(f->njoins);
+  std::iota(order.begin(), order.end(), 0u);
+  auto rank = [&](uint32_t j) {
     const ghive_hash_table *h = f->joins[j].table;
-    return h->source_rows ? (double)h->entries / (double)h->source_rows : 1.0;
+    double sel = h->source_rows ? (double)h->entries / (double)h->source_rows : 1.0;
+    double cost = mode[j] == kJoinLdsBits ? 1.0 : (mode[j] == kJoinGlobalBits ? 4.0 : 10.0);
+    return cost / std::max(1.0 - sel, 0.01);
   };
-  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return rate(a) < rate(b); });
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return rank(a) < rank(b); });
   std::vector<int> pos_of(f->njoins);
+  uint32_t lds_off = 0;
   F.njoins = (int)f->njoins;
   for (uint32_t i = 0; i < f->njoins; ++i) {
-    const ghive_hash_table *h = f->joins[order[i]].table;
+    const uint32_t j = order[i];
+    const ghive_hash_table *h = f->joins[j].table;
     FragJoinDev &J = F.joins[i];
-    J.fk = static_cast<const int32_t *>(fact.columns[f->joins[order[i]].fact_key_col].data);
+    J.fk = static_cast<const int32_t *>(fact.columns[f->joins[j].fact_key_col].data);
     J.slots = static_cast<const uint64_t *>(h->data);
+    J.bitmap = h->bitmap;
+    J.dense = h->dense;
     J.mask = h->slots - 1;
     J.log2 = h->log2_slots;
-    // an empty table rejects every key through an empty range
-    J.key_min = h->entries ? (int32_t)h->key_min : 1;
+    J.mode = mode[j];
+    J.key_min = h->entries ? (int32_t)h->key_min : 1;  // an empty table rejects every key
     J.key_max = h->entries ? (int32_t)h->key_max : 0;
-    pos_of[order[i]] = (int)i;
+    J.nwords = h->bitmap ? (uint32_t)((h->range + 31) / 32) : 0;
+    J.need_payload = need_payload[j];
+    if (J.mode == kJoinLdsBits) {
+      J.lds_word_off = lds_off;
+      lds_off += J.nwords;
+    }
+    pos_of[j] = (int)i;
   }
-  group_bound = 1;
+  F.lds_bitmap_words = (lds_off + 1) & ~1u;
+  P.group_bound = 1;
   F.nkeys = (int)f->nkeys;
   K.nkeys = (int)f->nkeys;
   for (uint32_t k = 0; k < f->nkeys; ++k) {
@@ -206,14 +327,12 @@ int fragment_prepare(ghive_ctx *ctx, const ghive_fragment *f, FragDev &F, GroupT
       if (fk.index >= f->njoins) return set_error(ctx, GHIVE_ERR_INVALID, "fragment: key %u uses join %u", k, fk.index);
       F.key_idx[k] = (int8_t)pos_of[fk.index];
       const ghive_hash_table *h = f->joins[fk.index].table;
-      if (h->payload_kind == 0)
-        return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "fragment: key %u: join %u carries no attribute", k, fk.index);
       uint64_t range = h->entries ? (uint64_t)(h->pay_max - h->pay_min + 1) : 1;
-      group_bound = group_bound > (1ull << 40) / (range ? range : 1) ? (1ull << 40) : group_bound * range;
+      P.group_bound = P.group_bound > (1ull << 40) / (range ? range : 1) ? (1ull << 40) : P.group_bound * range;
     } else if (fk.source == GHIVE_SRC_FACT) {
       GH_TRY(int_col(fk.index, "group key"));
       F.key_col[k] = static_cast<const int32_t *>(fact.columns[fk.index].data);
-      group_bound = 1ull << 40;
+      P.group_bound = 1ull << 40;
     } else {
       return set_error(ctx, GHIVE_ERR_INVALID, "fragment: key %u source %d", k, fk.source);
     }
@@ -255,59 +374,62 @@ int fragment_prepare(ghive_ctx *ctx, const ghive_fragment *f, FragDev &F, GroupT
     GH_TRY(flatten_pred(ctx, &fact, f->filter, f->filter_nnodes, f->filter_root, F.filter));
     F.has_filter = 1;
   }
+  // LDS group table when the bound fits what the bitmaps leave, at load <= 1/2
+  const size_t bitmap_bytes = (size_t)F.lds_bitmap_words * 4;
+  const int64_t left = ctx->tune.frag_lds_budget - (int64_t)bitmap_bytes;
+  const uint64_t want_slots = next_pow2(2 * (P.group_bound < 32 ? 32 : P.group_bound));
+  P.lcap_log2 = -1;
+  if (left >= 8 * 1024) {
+    int lg = lds_log2_for(T.nwords, (int)left);
+    if ((1ull << lg) >= want_slots) {
+      while (lg > 5 && (1ull << (lg - 1)) >= want_slots) --lg;
+      P.lcap_log2 = lg;
+    }
+  }
+  P.lds_bytes = bitmap_bytes + (P.lcap_log2 >= 0 ? (size_t)((1 << P.lcap_log2) + 1) * (1 + T.nwords) * 8 : 0);
+  P.late = late;
+  FragKernel kfn = pick_kernel(F.njoins, F.naggs, late != 0);
+  P.kernel = reinterpret_cast<const void *>(kfn);
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kFragThreads, P.lds_bytes) != hipSuccess || per_cu < 1) {
+    (void)hipGetLastError();
+    per_cu = 1;
+  }
+  const uint64_t tiles = (F.n + kFragTile - 1) / kFragTile;
+  const uint64_t cap = (uint64_t)ctx->cus * per_cu;
+  P.grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(tiles, cap));
+  if (ctx->tune.frag_grid > 0) P.grid = (uint32_t)ctx->tune.frag_grid;
   return GHIVE_OK;
 }
 
-int fragment_launch(ghive_ctx *ctx, const FragDev &F, const GroupTableDev &T, uint64_t group_bound,
-                    unsigned long long *d_joined, int late, uint32_t grid_override) {
-  // LDS pre-aggregation only when the group bound fits the budget at load <= 1/2; otherwise
-  // every surviving row goes to the global table (few rows survive a selective star join)
-  uint64_t want_slots = next_pow2(2 * (group_bound < 32 ? 32 : group_bound));
-  int lg = lds_log2_for(T.nwords, (int)ctx->tune.frag_lds_budget);
-  if ((1ull << lg) >= want_slots) {
-    while (lg > 5 && (1ull << (lg - 1)) >= want_slots) --lg;
-  } else {
-    lg = -1;
-  }
-  size_t lds_bytes = lg >= 0 ? (size_t)((1 << lg) + 1) * (1 + T.nwords) * 8 : 0;
-  uint64_t tiles = (F.n + kFragTile - 1) / kFragTile;
-  if (!tiles) return GHIVE_OK;
-  int per_cu = 0;
-  auto kern = late ? k_fragment<true> : k_fragment<false>;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kFragThreads, lds_bytes) != hipSuccess || per_cu < 1)
-    per_cu = 1;
-  uint64_t cap = (uint64_t)ctx->cus * per_cu;
-  uint32_t grid = (uint32_t)(tiles < cap ? tiles : cap);
-  if (grid_override) grid = grid_override;
-  else if (ctx->tune.frag_grid > 0) grid = (uint32_t)ctx->tune.frag_grid;
-  if (late)
-    k_fragment<true><<<grid, kFragThreads, lds_bytes, ctx->stream>>>(F, T, lg, d_joined);
-  else
-    k_fragment<false><<<grid, kFragThreads, lds_bytes, ctx->stream>>>(F, T, lg, d_joined);
-  GH_LAUNCH(ctx);
+int fragment_launch(ghive_ctx *ctx, const FragPlan &P, unsigned long long *d_joined) {
+  if (!P.F.n) return GHIVE_OK;
+  FragDev F = P.F;
+  GroupTableDev T = P.T;
+  int lg = P.lcap_log2;
+  unsigned long long *j = d_joined;
+  void *args[] = {&F, &T, &lg, &j};
+  GH_HIP(ctx, hipLaunchKernel(P.kernel, dim3(P.grid), dim3(kFragThreads), args, P.lds_bytes, ctx->stream));
   return GHIVE_OK;
 }
 
 int fragment_run(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_cols, uint64_t capacity,
                  uint64_t *out_groups, int late) {
   if (!out_groups) return set_error(ctx, GHIVE_ERR_INVALID, "fragment: out_groups is NULL");
-  FragDev F;
-  GroupTableDev T;
-  KeySpec K;
-  uint64_t bound = 0;
-  GH_TRY(fragment_prepare(ctx, frag, F, T, K, bound));
+  FragPlan P;
+  GH_TRY(fragment_plan(ctx, frag, late, P));
   if (capacity && !out_cols) return set_error(ctx, GHIVE_ERR_INVALID, "fragment: out_cols is NULL");
   for (uint32_t c = 0; c < frag->nkeys + frag->naggs && capacity; ++c)
     if (!out_cols[c].data) return set_error(ctx, GHIVE_ERR_INVALID, "fragment: output column %u has no buffer", c);
-  uint64_t table_groups = capacity < bound ? capacity : bound;
+  uint64_t table_groups = capacity < P.group_bound ? capacity : P.group_bound;
   DevBuf dkeys, dacc, dcnt, djoined;
-  GH_TRY(group_table_alloc(ctx, T, table_groups ? table_groups : 1, dkeys, dacc, dcnt));
-  T.capacity = capacity;
+  GH_TRY(group_table_alloc(ctx, P.T, table_groups ? table_groups : 1, dkeys, dacc, dcnt));
+  P.T.capacity = capacity;
   GH_TRY(djoined.alloc(ctx, sizeof(unsigned long long)));
   GH_TRY(fill_u64(ctx, djoined.as<uint64_t>(), 0, 1));
-  if (frag->nkeys == 0) GH_TRY(group_table_preinsert(ctx, T, 0ull));  // one group, even over no rows
+  if (frag->nkeys == 0) GH_TRY(group_table_preinsert(ctx, P.T, 0ull));  // one group, even over no rows
   GH_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-  GH_TRY(fragment_launch(ctx, F, T, bound, djoined.as<unsigned long long>(), late, 0));
+  GH_TRY(fragment_launch(ctx, P, djoined.as<unsigned long long>()));
   GH_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
   OutSpec O;
   memset(&O, 0, sizeof O);
@@ -316,7 +438,7 @@ int fragment_run(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_c
     O.type[c] = c < frag->nkeys ? GHIVE_INT : GHIVE_LONG;
     if (out_cols) out_cols[c].type = O.type[c];
   }
-  int st = group_table_emit(ctx, K, T, O, capacity, out_groups);
+  int st = group_table_emit(ctx, P.K, P.T, O, capacity, out_groups);
   GH_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
   GH_HIP(ctx, hipEventSynchronize(ctx->ev[2]));
   float ms_main = 0, ms_fin = 0;
@@ -327,7 +449,7 @@ int fragment_run(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_c
   ctx->times.main_kernel_ms = ms_main;
   ctx->times.finalize_ms = ms_fin;
   ctx->times.total_ms = ms_main + ms_fin;
-  ctx->times.rows_in = F.n;
+  ctx->times.rows_in = P.F.n;
   ctx->times.rows_joined = joined;
   for (uint32_t c = 0; c < frag->nkeys + frag->naggs && out_cols; ++c) out_cols[c].row_num = st == GHIVE_OK ? *out_groups : 0;
   return st;
@@ -345,8 +467,6 @@ int ghive_fragment_execute(ghive_ctx *ctx, const ghive_fragment *frag, ghive_col
   return fragment_run(ctx, frag, out_cols, capacity, out_groups, (int)ctx->tune.frag_late);
 }
 
-// Same operator with late materialisation: a column is read only for lanes that still hold a
-// live row (selection before projection). Exposed for measurement beside the default form.
 int ghive_fragment_execute_late(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_cols,
                                 uint64_t capacity, uint64_t *out_groups) {
   GH_CHECK_CTX(ctx);

@@ -11,12 +11,25 @@ namespace ghive {
 constexpr int kMaxFragJoins = 4;
 constexpr int kMaxFragAggs = 4;
 
+// How a join is probed inside the fused kernel.
+enum FragJoinMode : int32_t {
+  kJoinHash = 0,        // open-addressing L8 table in HBM/L2
+  kJoinGlobalBits = 1,  // dense companion: bitmap + payload array in HBM/L2
+  kJoinLdsBits = 2,     // dense companion with the bitmap copied into LDS per workgroup
+};
+
 struct FragJoinDev {
-  const int32_t *fk;      // fact foreign-key column
-  const uint64_t *slots;  // L8 slot array of the dimension table
+  const int32_t *fk;        // fact foreign-key column
+  const uint64_t *slots;    // L8 slot array
+  const uint32_t *bitmap;   // dense existence bitmap over [key_min, key_max]
+  const int32_t *dense;     // dense payload array
   uint64_t mask;
   int32_t log2;
-  int32_t key_min, key_max;  // inserted key range
+  int32_t mode;
+  int32_t key_min, key_max;  // inserted key range (empty table: min > max)
+  uint32_t lds_word_off;     // kJoinLdsBits: first uint32 word of this bitmap in LDS
+  uint32_t nwords;           // bitmap words
+  int32_t need_payload;      // payload used as a group key
   int32_t pad;
 };
 
@@ -35,19 +48,29 @@ struct FragDev {
   int32_t has_filter;
   int8_t key_src[2];
   int8_t key_idx[2];
-  int32_t pad;
+  uint32_t lds_bitmap_words;  // words of LDS bitmaps (even, so the group table is 8-aligned)
   const int32_t *key_col[2];
   FragJoinDev joins[kMaxFragJoins];
   FragAggDev aggs[kMaxFragAggs];
   DevPred filter;
 };
 
-// Validates the descriptor and fills the device description; group_bound is an upper bound on
-// the number of groups (product of the payload ranges of the join keys).
-int fragment_prepare(ghive_ctx *ctx, const ghive_fragment *f, FragDev &F, GroupTableDev &T, KeySpec &K,
-                     uint64_t &group_bound);
-// Launches the fused kernel on ctx->stream (rows that survive the joins counted in d_joined).
-int fragment_launch(ghive_ctx *ctx, const FragDev &F, const GroupTableDev &T, uint64_t group_bound,
-                    unsigned long long *d_joined, int late, uint32_t grid_override);
+// Validated device description plus launch plan of one fragment.
+struct FragPlan {
+  FragDev F;
+  GroupTableDev T;
+  KeySpec K;
+  uint64_t group_bound;  // product of the payload ranges of the join keys
+  int lcap_log2;         // LDS group table (log2 slots) or -1: global table only
+  size_t lds_bytes;
+  uint32_t grid;
+  int late;
+  const void *kernel;
+};
+
+// Validates the descriptor, chooses the probe mode and order of every join, sizes LDS and grid.
+int fragment_plan(ghive_ctx *ctx, const ghive_fragment *f, int late, FragPlan &P);
+// Launches the fused kernel on ctx->stream (rows surviving every join counted in d_joined).
+int fragment_launch(ghive_ctx *ctx, const FragPlan &P, unsigned long long *d_joined);
 
 }  // namespace ghive

@@ -167,6 +167,30 @@ __global__ __launch_bounds__(kHashThreads) void k_hash_probe(
   }
 }
 
+// bitmap bit and dense payload of every inserted key (unique keys, so no write races on dense)
+__global__ __launch_bounds__(kHashThreads) void k_dense_fill(const void *keys, int key_type, const uint32_t *sel,
+                                                             uint64_t n, const void *pay, int pay_type,
+                                                             int64_t key_min, uint64_t range, uint32_t *bitmap,
+                                                             int32_t *dense) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    uint64_t r = sel ? sel[i] : i;
+    int64_t k;
+    if (!col_load_i64(keys, key_type, r, k)) continue;
+    uint64_t idx = (uint64_t)(k - key_min);
+    if (idx >= range) continue;
+    int64_t p;
+    if (pay) {
+      if (!col_load_i64(pay, pay_type, r, p)) p = 0x7fffffffll;
+    } else {
+      p = (int64_t)r;
+    }
+    atomicOr(&bitmap[idx >> 5], 1u << (idx & 31));
+    dense[idx] = (int32_t)p;
+  }
+}
+
 __global__ void k_seq_i32(int32_t *p, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -260,6 +284,36 @@ int hash_build(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, uin
   h->key_max = (int64_t)host[3];
   h->pay_min = (int64_t)host[4];
   h->pay_max = (int64_t)host[5];
+  // Dense companion of a unique INT table whose keys span a compact range (surrogate keys of a
+  // star schema): an existence bitmap over [key_min, key_max] and a payload array indexed by
+  // key - key_min. A probe is then one bit test (LDS-resident when small) plus, for survivors,
+  // one load -- no probe chain.
+  const uint64_t range = h->entries ? (uint64_t)(h->key_max - h->key_min) + 1 : 0;
+  if (h->layout == 8 && h->unique && h->entries && range <= (1ull << 27) &&
+      range <= 64 * h->entries + 65536) {
+    uint64_t nwords = (range + 31) / 32;
+    h->bitmap = static_cast<uint32_t *>(ctx->arena->alloc(nwords * 4));
+    h->dense = static_cast<int32_t *>(ctx->arena->alloc(range * 4));
+    if (!h->bitmap || !h->dense) {
+      if (h->bitmap) ctx->arena->free(h->bitmap);
+      if (h->dense) ctx->arena->free(h->dense);
+      h->bitmap = nullptr;
+      h->dense = nullptr;
+    } else {
+      h->range = range;
+      st = fill_u32(ctx, h->bitmap, 0u, nwords);
+      if (st == GHIVE_OK) {
+        k_dense_fill<<<grid_for(ctx, n), kHashThreads, 0, ctx->stream>>>(
+            key->data, key->type, sel, n, payload ? payload->data : nullptr, payload ? payload->type : 0,
+            h->key_min, range, h->bitmap, h->dense);
+        if (hipGetLastError() != hipSuccess) st = set_error(ctx, GHIVE_ERR_HIP, "hash_build: dense launch failed");
+      }
+      if (st != GHIVE_OK) {
+        ghive_hash_table_free(ctx, h);
+        return st;
+      }
+    }
+  }
   *out = h;
   return GHIVE_OK;
 }
@@ -303,11 +357,14 @@ int ghive_hash_table_info(const ghive_hash_table *ht, uint64_t *entries, uint64_
 
 void ghive_hash_table_free(ghive_ctx *ctx, ghive_hash_table *ht) {
   if (!ht) return;
-  ghive_ctx *c = ctx ? ctx : ht->owner;
-  if (c && ht->data) {
+  (void)ctx;
+  ghive_ctx *c = ht->owner;  // the table's memory belongs to the context that built it
+  if (c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    c->arena->free(ht->data);
+    if (ht->data) c->arena->free(ht->data);
+    if (ht->bitmap) c->arena->free(ht->bitmap);
+    if (ht->dense) c->arena->free(ht->dense);
   }
   delete ht;
 }

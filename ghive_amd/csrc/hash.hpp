@@ -24,6 +24,10 @@ struct ghive_hash_table {
   int64_t key_min, key_max;
   int64_t pay_min, pay_max;
   void *data;  // device slot array
+  // dense companion (unique INT keys over a compact range), else null
+  uint32_t *bitmap;  // bit (key - key_min) set for every inserted key
+  int32_t *dense;    // payload at (key - key_min)
+  uint64_t range;    // key_max - key_min + 1
   ghive_ctx *owner;
 };
 

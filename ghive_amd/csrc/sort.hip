@@ -23,7 +23,7 @@ struct SortKeySpec {
   int32_t nkeys;
   int32_t type[8];
   int32_t desc[8];
-  int32_t shift[8];  // bit offset of the field inside the 128-bit composite
+  int32_t shift[8];  // bit offset of the field inside the composite (at most 256 bits)
   int32_t width[8];
   const void *data[8];
 };
@@ -47,21 +47,27 @@ __device__ __forceinline__ uint64_t field_bits(int type, const void *p, uint64_t
 
 template <int KW>
 __global__ void k_sort_keys(SortKeySpec S, uint64_t n, uint64_t *keys, uint32_t *idx, unsigned long long *orand) {
-  uint64_t vor[2] = {0, 0}, vand[2] = {~0ull, ~0ull};
+  uint64_t vor[KW], vand[KW];
+#pragma unroll
+  for (int q = 0; q < KW; ++q) {
+    vor[q] = 0;
+    vand[q] = ~0ull;
+  }
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) {
-    uint64_t w[2] = {0, 0};  // w[0] = low 64 bits, w[1] = high 64 bits
+    uint64_t w[KW];  // w[0] = least significant 64 bits
+#pragma unroll
+    for (int q = 0; q < KW; ++q) w[q] = 0;
     for (int k = 0; k < S.nkeys; ++k) {
       uint64_t b = field_bits(S.type[k], S.data[k], i);
       if (S.desc[k]) b = ~b;
       if (S.width[k] < 64) b &= (1ull << S.width[k]) - 1;
-      int sh = S.shift[k];
-      if (sh >= 64) {
-        w[1] |= b << (sh - 64);
-      } else {
-        w[0] |= b << sh;
-        if (sh + S.width[k] > 64) w[1] |= b >> (64 - sh);
+      const int sh = S.shift[k], word = sh / 64, off = sh % 64;
+#pragma unroll
+      for (int q = 0; q < KW; ++q) {
+        if (q == word) w[q] |= b << off;
+        if (q == word + 1 && off + S.width[k] > 64) w[q] |= b >> (64 - off);
       }
     }
 #pragma unroll
@@ -116,13 +122,17 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *
   for (int j = 0; j < kSortPerThread; ++j) {
     const uint64_t e = t0 + (uint64_t)j * kSortThreads + threadIdx.x;
     const bool valid = e < n;
-    uint64_t kv[2] = {0, 0};
+    uint64_t kv[KW];
+#pragma unroll
+    for (int q = 0; q < KW; ++q) kv[q] = 0;
     uint32_t id = 0, digit = 0;
     if (valid) {
 #pragma unroll
       for (int q = 0; q < KW; ++q) kv[q] = keys_in[(uint64_t)q * n + e];
       id = idx_in[e];
-      digit = (uint32_t)(kv[word] >> shift) & 255u;
+#pragma unroll
+      for (int q = 0; q < KW; ++q)
+        if (q == word) digit = (uint32_t)(kv[q] >> shift) & 255u;
     }
     uint64_t peers = ballot(valid);
 #pragma unroll
@@ -205,29 +215,32 @@ int ghive_sort(ghive_ctx *ctx, const ghive_table *table, const uint32_t *key_col
     S.data[k] = c.data;
     bits += S.width[k];
   }
-  if (bits > 128) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "sort: keys are %d bits wide (max 128)", bits);
+  if (bits > 256) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "sort: keys are %d bits wide (max 256)", bits);
   if (n == 0) return GHIVE_OK;
   int top = bits;
   for (uint32_t k = 0; k < nkeys; ++k) {
     top -= S.width[k];
     S.shift[k] = top;
   }
-  const int KW = bits > 64 ? 2 : 1;
+  const int KW = (bits + 63) / 64;
   DevBuf k0, k1, i1, orand;
   GH_TRY(k0.alloc(ctx, 8ull * KW * n));
   GH_TRY(k1.alloc(ctx, 8ull * KW * n));
   GH_TRY(i1.alloc(ctx, 4ull * n));
-  GH_TRY(orand.alloc(ctx, 4 * sizeof(uint64_t)));
-  uint64_t init[4] = {0, ~0ull, 0, ~0ull};
+  GH_TRY(orand.alloc(ctx, 8 * sizeof(uint64_t)));
+  uint64_t init[8] = {0, ~0ull, 0, ~0ull, 0, ~0ull, 0, ~0ull};
   GH_TRY(h2d(ctx, orand.get(), init, sizeof init));
   uint64_t g = (n + 255) / 256, cap = (uint64_t)ctx->cus * 8;
   uint32_t grid = (uint32_t)(g > cap ? cap : g);
-  if (KW == 1)
-    k_sort_keys<1><<<grid, 256, 0, ctx->stream>>>(S, n, k0.as<uint64_t>(), out_perm, orand.as<unsigned long long>());
-  else
-    k_sort_keys<2><<<grid, 256, 0, ctx->stream>>>(S, n, k0.as<uint64_t>(), out_perm, orand.as<unsigned long long>());
+  unsigned long long *oa_d = orand.as<unsigned long long>();
+  switch (KW) {
+    case 1: k_sort_keys<1><<<grid, 256, 0, ctx->stream>>>(S, n, k0.as<uint64_t>(), out_perm, oa_d); break;
+    case 2: k_sort_keys<2><<<grid, 256, 0, ctx->stream>>>(S, n, k0.as<uint64_t>(), out_perm, oa_d); break;
+    case 3: k_sort_keys<3><<<grid, 256, 0, ctx->stream>>>(S, n, k0.as<uint64_t>(), out_perm, oa_d); break;
+    default: k_sort_keys<4><<<grid, 256, 0, ctx->stream>>>(S, n, k0.as<uint64_t>(), out_perm, oa_d); break;
+  }
   GH_LAUNCH(ctx);
-  uint64_t oa[4];
+  uint64_t oa[8];
   GH_TRY(d2h(ctx, oa, orand.get(), sizeof oa));
   std::vector<int> digits;
   for (int q = 0; q < KW; ++q) {
@@ -236,10 +249,14 @@ int ghive_sort(ghive_ctx *ctx, const ghive_table *table, const uint32_t *key_col
       if ((varying >> (8 * b)) & 255u) digits.push_back(8 * q + b);
   }
   uint32_t *res = out_perm;
-  if (KW == 1)
-    GH_TRY(radix_passes<1>(ctx, n, k0.as<uint64_t>(), k1.as<uint64_t>(), out_perm, i1.as<uint32_t>(), digits, &res));
-  else
-    GH_TRY(radix_passes<2>(ctx, n, k0.as<uint64_t>(), k1.as<uint64_t>(), out_perm, i1.as<uint32_t>(), digits, &res));
+  uint64_t *K0 = k0.as<uint64_t>(), *K1 = k1.as<uint64_t>();
+  uint32_t *I1 = i1.as<uint32_t>();
+  switch (KW) {
+    case 1: GH_TRY(radix_passes<1>(ctx, n, K0, K1, out_perm, I1, digits, &res)); break;
+    case 2: GH_TRY(radix_passes<2>(ctx, n, K0, K1, out_perm, I1, digits, &res)); break;
+    case 3: GH_TRY(radix_passes<3>(ctx, n, K0, K1, out_perm, I1, digits, &res)); break;
+    default: GH_TRY(radix_passes<4>(ctx, n, K0, K1, out_perm, I1, digits, &res)); break;
+  }
   if (res != out_perm) GH_HIP(ctx, hipMemcpyAsync(out_perm, res, 4ull * n, hipMemcpyDeviceToDevice, ctx->stream));
   GH_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return GHIVE_OK;

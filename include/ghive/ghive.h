@@ -277,7 +277,8 @@ int ghive_groupby(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel,
 /* SinkOperator ORDER BY (SinkOperator.cu:135-288; orders as SortOrder '+'/'-',
  * GroupByPredicate.hpp:17-20): LSD radix sort; writes the permutation of row ids so that
  * rows are in key order, ties in ascending row id (a total order). Keys: INT, LONG,
- * DOUBLE, FLOAT columns; NULL sorts as its sentinel value (NaN last ascending). */
+ * DOUBLE, FLOAT columns, at most 8 keys and 256 bits in total; NULL sorts as its sentinel
+ * value (NaN last ascending). */
 int ghive_sort(ghive_ctx *ctx, const ghive_table *table, const uint32_t *key_cols,
                const char *orders, uint32_t nkeys, uint32_t *out_perm);
 
@@ -329,7 +330,9 @@ int ghive_fragment_execute_late(ghive_ctx *ctx, const ghive_fragment *frag, ghiv
                                 uint64_t capacity, uint64_t *out_groups);
 
 /* Tunables of a context: "fragment.lds_budget" (bytes, <= 160 KiB), "fragment.late" (0/1),
- * "fragment.grid" (workgroups, 0 = one resident wave), "groupby.lds_budget" (bytes). */
+ * "fragment.grid" (workgroups, 0 = one resident wave), "fragment.lds_bits" (0/1: LDS copies of
+ * small dimension bitmaps), "fragment.dense" (0/1: probe compact-key dimensions through their
+ * bitmap + payload array instead of the hash table), "groupby.lds_budget" (bytes). */
 int ghive_set_option(ghive_ctx *ctx, const char *name, int64_t value);
 int ghive_get_option(ghive_ctx *ctx, const char *name, int64_t *value);
 

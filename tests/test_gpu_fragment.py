@@ -73,16 +73,26 @@ def _star_oracle(orc, T, rows=None):
     return spec.run(nthreads=8, capacity=1 << 14)
 
 
-def test_star_fragment_vs_oracle_20m(ctx, orc):
+@pytest.mark.parametrize("dense,lds_bits", [(1, 1), (1, 0), (0, 0)])
+def test_star_fragment_vs_oracle_20m(ctx, orc, dense, lds_bits):
+    """Every probe mode (LDS bitmap, HBM bitmap, hash table) and both load orders agree with the
+    oracle bit for bit."""
     from ghive_amd import workloads as W
-    T = W.star_tables(ctx, 20_000_000)
+    T = W.star_tables(ctx, 20_000_001)
     hd, hs, hp = W.star_hash_tables(ctx, T)
     frag = W.star_fragment(T.fact, hd, hs, hp)
-    got = _gpu_dict(frag.execute(ctx, capacity=8192), 2)
+    ctx.set_option("fragment.dense", dense)
+    ctx.set_option("fragment.lds_bits", lds_bits)
+    try:
+        got = _gpu_dict(frag.execute(ctx, capacity=8192), 2)
+        joined = ctx.stage_times()["rows_joined"]
+        got_late = _gpu_dict(frag.execute(ctx, capacity=8192, late=True), 2)
+    finally:
+        ctx.set_option("fragment.dense", 1)
+        ctx.set_option("fragment.lds_bits", 1)
     want = _star_oracle(orc, T)
-    assert got == _orc_dict(want)
-    assert ctx.stage_times()["rows_joined"] == want["joined"]
-    assert _gpu_dict(frag.execute(ctx, capacity=8192, late=True), 2) == got
+    assert got == _orc_dict(want) == got_late
+    assert joined == want["joined"]
 
 
 def test_star_fragment_600m_size_independent(ctx):
