@@ -24,6 +24,13 @@ SIGNATURES = {
     "ghive_last_error": (C.c_char_p, [P]),
     "ghive_set_stream": (C.c_int, [P, P]),
     "ghive_get_stream": (P, [P]),
+    "ghive_use_own_stream": (C.c_int, [P]),
+    "ghive_batch_create": (C.c_int, [P, P, P, u32, P, C.c_int, C.POINTER(P)]),
+    "ghive_batch_run": (C.c_int, [P, P, P]),
+    "ghive_batch_results": (C.c_int, [P, P, P, P]),
+    "ghive_batch_info": (C.c_int, [P, C.POINTER(u32), PU64, PU64, C.POINTER(i32), C.POINTER(u32),
+                                   C.POINTER(C.c_float)]),
+    "ghive_batch_free": (None, [P, P]),
     "ghive_sync": (C.c_int, [P]),
     "ghive_device_query": (C.c_int, [P, C.POINTER(abi.DeviceInfo)]),
     "ghive_malloc": (C.c_int, [P, u64, C.POINTER(P)]),

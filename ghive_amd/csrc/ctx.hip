@@ -201,7 +201,13 @@ const char *ghive_last_error(const ghive_ctx *ctx) {
 
 int ghive_set_stream(ghive_ctx *ctx, void *hip_stream) {
   GH_CHECK_CTX(ctx);
-  ctx->stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->own_stream;
+  ctx->stream = static_cast<hipStream_t>(hip_stream);  // NULL: the HIP null stream
+  return GHIVE_OK;
+}
+
+int ghive_use_own_stream(ghive_ctx *ctx) {
+  GH_CHECK_CTX(ctx);
+  ctx->stream = ctx->own_stream;
   return GHIVE_OK;
 }
 

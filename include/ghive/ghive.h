@@ -91,8 +91,10 @@ int ghive_init(int device, uint64_t mem_limit_bytes, ghive_ctx **out);
 void ghive_shutdown(ghive_ctx *ctx);
 const char *ghive_last_error(const ghive_ctx *ctx);
 int ghive_abi_version(void);
-/* Use a caller-owned hipStream_t (NULL = the context's own stream). */
+/* Order all work of the context on a caller-owned hipStream_t (NULL = the HIP null stream).
+ * A new context uses its own non-blocking stream; ghive_use_own_stream goes back to it. */
 int ghive_set_stream(ghive_ctx *ctx, void *hip_stream);
+int ghive_use_own_stream(ghive_ctx *ctx);
 void *ghive_get_stream(ghive_ctx *ctx);
 int ghive_sync(ghive_ctx *ctx);
 
@@ -323,6 +325,50 @@ typedef struct ghive_fragment {
 /* Output: keys (INT each) then aggregates (LONG) into out_cols of `capacity` rows. */
 int ghive_fragment_execute(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_cols,
                            uint64_t capacity, uint64_t *out_groups);
+
+/* ---- batched form: many fragments of one query shape in one persistent kernel ----------- *
+ * No counterpart in the reference (it runs one Tez task per JNI call). Query per item:
+ *   SELECT g, SUM(a*b) FROM fact JOIN dim ON fact.key = dim.key
+ *   WHERE filter_lo <= f <= filter_hi GROUP BY g
+ * where f and g are bit fields of the dimension payload (f = (payload >> filter_shift) &
+ * (2^filter_bits - 1), likewise g with at most 12 bits). The dimension table is shared and
+ * stays resident; every item brings its own fact columns (16-byte aligned INT), rows and filter
+ * constants. Results are dense: sums[item * 2^group_bits + g] and counts[...] (count 0 = no
+ * group). */
+typedef struct ghive_batch ghive_batch;
+typedef struct ghive_batch_item {
+  const int32_t *key;
+  const int32_t *a;
+  const int32_t *b; /* NULL: SUM(a) */
+  uint64_t rows;
+  int32_t filter_lo;
+  int32_t filter_hi;
+} ghive_batch_item;
+typedef struct ghive_batch_spec {
+  const ghive_hash_table *dim;
+  uint32_t filter_shift, filter_bits;
+  uint32_t group_shift, group_bits;
+} ghive_batch_spec;
+/* Autotuned knobs: rows per tile (multiple of 1024; 0 = 8192), workgroups per CU (0 = 4),
+ * probe mode (0 auto, 1 LDS bitmap, 2 HBM bitmap, 3 hash table). */
+typedef struct ghive_batch_tuning {
+  uint32_t tile_rows;
+  uint32_t blocks_per_cu;
+  int32_t probe_mode;
+  int32_t reserved;
+} ghive_batch_tuning;
+/* Uploads the item descriptors; with use_graph the zeroing and the persistent kernel are
+ * captured once in a HIP graph that every run replays. */
+int ghive_batch_create(ghive_ctx *ctx, const ghive_batch_spec *spec, const ghive_batch_item *items,
+                       uint32_t nitems, const ghive_batch_tuning *tuning, int use_graph,
+                       ghive_batch **out);
+/* Runs the batch (spec needed only without a graph) and waits for it. */
+int ghive_batch_run(ghive_ctx *ctx, ghive_batch *bt, const ghive_batch_spec *spec);
+/* Host copies of the dense results (nitems * 2^group_bits each). */
+int ghive_batch_results(ghive_ctx *ctx, ghive_batch *bt, int64_t *sums, int64_t *counts);
+int ghive_batch_info(const ghive_batch *bt, uint32_t *groups_per_item, uint64_t *total_rows,
+                     uint64_t *total_tiles, int32_t *probe_mode, uint32_t *grid, float *last_ms);
+void ghive_batch_free(ghive_ctx *ctx, ghive_batch *bt);
 
 /* Same operator with late materialisation: a fact column is read only for the lanes that
  * still hold a row after the previous join (measured beside the default form). */
