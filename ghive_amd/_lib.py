@@ -90,3 +90,8 @@ def lib():
             fn.argtypes = args
         _lib = L
     return _lib
+
+SIGNATURES_EXTRA = {
+    "ghive_bench_stream": (C.c_int, [P, P, P, u64, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+}
+SIGNATURES.update(SIGNATURES_EXTRA)

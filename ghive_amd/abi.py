@@ -154,3 +154,18 @@ def pred_array(nodes):
     for i, n in enumerate(nodes):
         arr[i] = n
     return arr
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("key", C.c_void_p), ("a", C.c_void_p), ("b", C.c_void_p), ("rows", C.c_uint64),
+                ("filter_lo", C.c_int32), ("filter_hi", C.c_int32)]
+
+
+class BatchSpec(C.Structure):
+    _fields_ = [("dim", C.c_void_p), ("filter_shift", C.c_uint32), ("filter_bits", C.c_uint32),
+                ("group_shift", C.c_uint32), ("group_bits", C.c_uint32)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("tile_rows", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("probe_mode", C.c_int32),
+                ("reserved", C.c_int32)]

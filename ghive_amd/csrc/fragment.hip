@@ -24,205 +24,11 @@
 #include "fragment.hpp"
 #include "groupby.hpp"
 #include "hash.hpp"
+#include "fragment_kernel.hpp"
 #include "pred_host.hpp"
 
 namespace ghive {
 
-constexpr int kFragThreads = 256;
-constexpr int kRowsPerThread = 4;
-constexpr uint64_t kFragTile = (uint64_t)kFragThreads * kRowsPerThread;
-constexpr int kNullI32 = 0x7fffffff;
-
-__device__ __forceinline__ void load4_i32(const int32_t *p, uint64_t r0, uint64_t n, int32_t v[4]) {
-  if (r0 + 3 < n) {
-    int4 x = *reinterpret_cast<const int4 *>(p + r0);
-    v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
-  } else {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = r0 + q < n ? p[r0 + q] : kNullI32;
-  }
-}
-
-__device__ __forceinline__ bool in_range(const FragJoinDev &J, int32_t k) {
-  return k != kNullI32 && k >= J.key_min && k <= J.key_max;
-}
-
-template <int NJ, int NA, bool kLate>
-__global__ __launch_bounds__(kFragThreads) void k_fragment(FragDev F, GroupTableDev T, int lcap_log2,
-                                                           unsigned long long *joined) {
-  extern __shared__ uint64_t lds64[];
-  uint32_t *lbits = reinterpret_cast<uint32_t *>(lds64);
-  uint64_t *ltab = lds64 + F.lds_bitmap_words / 2;
-  const bool use_lds = lcap_log2 >= 0;  // uniform
-  const int lstride = use_lds ? (1 << lcap_log2) + 1 : 0;
-#pragma unroll
-  for (int j = 0; j < NJ; ++j)
-    if (F.joins[j].mode == kJoinLdsBits)
-      for (uint32_t w = threadIdx.x; w < F.joins[j].nwords; w += blockDim.x)
-        lbits[F.joins[j].lds_word_off + w] = F.joins[j].bitmap[w];
-  if (use_lds) lds_table_init(ltab, T, lcap_log2);
-  __syncthreads();
-
-  const uint64_t n = F.n;
-  const uint64_t ntiles = (n + kFragTile - 1) / kFragTile;
-  uint32_t njoined = 0;
-  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const uint64_t r0 = tile * kFragTile + (uint64_t)threadIdx.x * kRowsPerThread;
-    uint32_t alive = 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) alive |= (r0 + q < n ? 1u : 0u) << q;
-    int32_t fk[NJ > 0 ? NJ : 1][4];
-    int32_t av[NA][4], bv[NA][4];
-    if (!kLate) {  // every column load of the tile issued up front: many bytes in flight
-#pragma unroll
-      for (int j = 0; j < NJ; ++j) load4_i32(F.joins[j].fk, r0, n, fk[j]);
-#pragma unroll
-      for (int a = 0; a < NA; ++a) {
-        if (F.aggs[a].expr != GHIVE_EXPR_ONE) load4_i32(F.aggs[a].a, r0, n, av[a]);
-        if (F.aggs[a].expr == GHIVE_EXPR_MUL) load4_i32(F.aggs[a].b, r0, n, bv[a]);
-      }
-    }
-    if (F.has_filter && alive) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if (((alive >> q) & 1u) && !eval_pred(F.filter, r0 + q)) alive &= ~(1u << q);
-    }
-    int32_t pay[NJ > 0 ? NJ : 1][4];
-#pragma unroll
-    for (int j = 0; j < NJ; ++j) {
-      const FragJoinDev &J = F.joins[j];
-      if (kLate && alive) load4_i32(J.fk, r0, n, fk[j]);
-      if (J.mode == kJoinLdsBits) {
-        const uint32_t *bits = lbits + J.lds_word_off;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          if (!((alive >> q) & 1u)) continue;
-          const int32_t k = fk[j][q];
-          bool hit = false;
-          if (in_range(J, k)) {
-            const uint32_t u = (uint32_t)((int64_t)k - J.key_min);
-            hit = (bits[u >> 5] >> (u & 31)) & 1u;
-          }
-          if (!hit) alive &= ~(1u << q);
-        }
-      } else if (J.mode == kJoinGlobalBits) {
-        uint32_t w[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {  // independent loads, all in flight together
-          w[q] = 0;
-          const int32_t k = fk[j][q];
-          if (((alive >> q) & 1u) && in_range(J, k)) {
-            const uint32_t u = (uint32_t)((int64_t)k - J.key_min);
-            w[q] = J.bitmap[u >> 5] >> (u & 31);
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          if (!(w[q] & 1u)) alive &= ~(1u << q);
-      } else {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          if (!((alive >> q) & 1u)) continue;
-          const int32_t k = fk[j][q];
-          int32_t p = 0;
-          bool hit = in_range(J, k) && probe_unique8(J.slots, J.mask, J.log2, k, p);
-          if (!hit) alive &= ~(1u << q);
-          pay[j][q] = p;
-        }
-      }
-      if (J.mode != kJoinHash && J.need_payload) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          if ((alive >> q) & 1u) pay[j][q] = J.dense[(uint32_t)((int64_t)fk[j][q] - J.key_min)];
-      }
-    }
-    if (!alive) continue;
-    if (kLate) {
-#pragma unroll
-      for (int a = 0; a < NA; ++a) {
-        if (F.aggs[a].expr != GHIVE_EXPR_ONE) load4_i32(F.aggs[a].a, r0, n, av[a]);
-        if (F.aggs[a].expr == GHIVE_EXPR_MUL) load4_i32(F.aggs[a].b, r0, n, bv[a]);
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (!((alive >> q) & 1u)) continue;
-      ++njoined;
-      uint64_t key = 0;
-      for (int k = 0; k < F.nkeys; ++k) {
-        int32_t kv = 0;
-#pragma unroll
-        for (int j = 0; j < NJ; ++j)
-          if (F.key_src[k] == GHIVE_SRC_JOIN && F.key_idx[k] == j) kv = pay[j][q];
-        if (F.key_src[k] != GHIVE_SRC_JOIN) kv = F.key_col[k][r0 + q];
-        key |= (uint64_t)(uint32_t)kv << (32 * k);
-      }
-      const int ls = use_lds ? lds_find_insert(ltab, key, lcap_log2) : -1;
-      int64_t g = -1;
-      if (ls < 0) {
-        g = gt_find_insert(T, key);
-        if (g < 0) continue;
-      }
-#pragma unroll
-      for (int a = 0; a < NA; ++a) {
-        const int expr = F.aggs[a].expr;
-        AggVal v;
-        v.neg = false;
-        v.b = 0;
-        if (expr == GHIVE_EXPR_ONE) {
-          v.valid = true;
-          v.a = 1;
-        } else {
-          const int32_t x = av[a][q];
-          v.valid = x != kNullI32;
-          int64_t val = x;
-          if (expr == GHIVE_EXPR_MUL) {
-            const int32_t y = bv[a][q];
-            v.valid = v.valid && y != kNullI32;
-            val = (int64_t)((uint64_t)(int64_t)x * (uint64_t)(int64_t)y);
-          }
-          v.a = (uint64_t)val;
-        }
-        const AggSpec &A = T.aggs[a];
-        if (ls >= 0)
-          acc_apply(A, [&](int w) { return ltab + (size_t)(1 + A.word0 + w) * lstride + ls; }, v);
-        else
-          acc_apply(A, [&](int w) { return T.acc + (size_t)(A.word0 + w) * T.stride + g; }, v);
-      }
-    }
-  }
-  __syncthreads();
-  if (use_lds) lds_table_flush(ltab, T, lcap_log2);
-  unsigned long long s = njoined;
-#pragma unroll
-  for (int d = kWave / 2; d > 0; d >>= 1) s += __shfl_xor(s, d, kWave);
-  if (lane_id() == 0 && s) atomicAdd(joined, s);
-}
-
-using FragKernel = void (*)(FragDev, GroupTableDev, int, unsigned long long *);
-
-template <int NJ, int NA>
-static FragKernel kern(bool late) {
-  return late ? k_fragment<NJ, NA, true> : k_fragment<NJ, NA, false>;
-}
-template <int NJ>
-static FragKernel pick_na(int na, bool late) {
-  switch (na) {
-    case 1: return kern<NJ, 1>(late);
-    case 2: return kern<NJ, 2>(late);
-    case 3: return kern<NJ, 3>(late);
-    default: return kern<NJ, 4>(late);
-  }
-}
-static FragKernel pick_kernel(int nj, int na, bool late) {
-  switch (nj) {
-    case 0: return pick_na<0>(na, late);
-    case 1: return pick_na<1>(na, late);
-    case 2: return pick_na<2>(na, late);
-    case 3: return pick_na<3>(na, late);
-    default: return pick_na<4>(na, late);
-  }
-}
 
 int fragment_plan(ghive_ctx *ctx, const ghive_fragment *f, int late, FragPlan &P) {
   memset(&P, 0, sizeof P);
@@ -388,14 +194,17 @@ int fragment_plan(ghive_ctx *ctx, const ghive_fragment *f, int late, FragPlan &P
   }
   P.lds_bytes = bitmap_bytes + (P.lcap_log2 >= 0 ? (size_t)((1 << P.lcap_log2) + 1) * (1 + T.nwords) * 8 : 0);
   P.late = late;
-  FragKernel kfn = pick_kernel(F.njoins, F.naggs, late != 0);
+  F.late = late ? 1 : 0;
+  F.nt = ctx->tune.frag_nt ? 1 : 0;
+  const int rows = ctx->tune.frag_rows == 8 ? 8 : 4;
+  FragKernel kfn = rows == 8 ? pick_kernel_r8(F.njoins, F.naggs) : pick_kernel_r4(F.njoins, F.naggs);
   P.kernel = reinterpret_cast<const void *>(kfn);
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kFragThreads, P.lds_bytes) != hipSuccess || per_cu < 1) {
     (void)hipGetLastError();
     per_cu = 1;
   }
-  const uint64_t tiles = (F.n + kFragTile - 1) / kFragTile;
+  const uint64_t tiles = (F.n + (uint64_t)kFragThreads * rows - 1) / ((uint64_t)kFragThreads * rows);
   const uint64_t cap = (uint64_t)ctx->cus * per_cu;
   P.grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(tiles, cap));
   if (ctx->tune.frag_grid > 0) P.grid = (uint32_t)ctx->tune.frag_grid;

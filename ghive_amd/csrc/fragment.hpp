@@ -46,6 +46,8 @@ struct FragDev {
   int32_t nkeys;
   int32_t naggs;
   int32_t has_filter;
+  int32_t late;  // read a column only for lanes that still hold a row
+  int32_t nt;    // non-temporal loads of the streamed fact columns
   int8_t key_src[2];
   int8_t key_idx[2];
   uint32_t lds_bitmap_words;  // words of LDS bitmaps (even, so the group table is 8-aligned)

@@ -105,4 +105,52 @@ int exclusive_scan_u32(ghive_ctx *ctx, const uint32_t *in, uint32_t *out, uint64
   return GHIVE_OK;
 }
 
+// ---- STREAM-style bandwidth probes (achievable HBM read / copy rate of the device) --------
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_stream_read(const v4u *p, uint64_t n16, unsigned long long *sink) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  v4u acc = {0u, 0u, 0u, 0u};
+  for (; i + 3 * stride < n16; i += 4 * stride) {
+    v4u a = p[i], b = p[i + stride], c = p[i + 2 * stride], d = p[i + 3 * stride];
+    acc ^= a ^ b ^ c ^ d;
+  }
+  for (; i < n16; i += stride) acc ^= p[i];
+  uint32_t x = acc.x ^ acc.y ^ acc.z ^ acc.w;
+  if (x == 0x9e3779b9u) atomicAdd(sink, 1ull);  // keeps the loads alive; almost never taken
+}
+
+__global__ __launch_bounds__(256) void k_stream_copy(const v4u *src, v4u *dst, uint64_t n16) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n16; i += stride) dst[i] = src[i];
+}
+
 }  // namespace ghive
+
+extern "C" int ghive_bench_stream(ghive_ctx *ctx, void *dst, const void *src, uint64_t bytes, int mode,
+                                  int iters, float *ms_per_iter) {
+  using namespace ghive;
+  GH_CHECK_CTX(ctx);
+  if (!src || (mode == 1 && !dst) || bytes < 16 || (bytes & 15) || iters < 1 || !ms_per_iter ||
+      ((uintptr_t)src & 15) || (dst && ((uintptr_t)dst & 15)))
+    return set_error(ctx, GHIVE_ERR_INVALID, "bench_stream: bad arguments");
+  const uint64_t n16 = bytes / 16;
+  const uint32_t grid = (uint32_t)ctx->cus * 8;
+  GH_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+  for (int it = 0; it < iters; ++it) {
+    if (mode == 0)
+      k_stream_read<<<grid, 256, 0, ctx->stream>>>(static_cast<const v4u *>(src), n16,
+                                                    reinterpret_cast<unsigned long long *>(ctx->d_counters));
+    else
+      k_stream_copy<<<grid, 256, 0, ctx->stream>>>(static_cast<const v4u *>(src), static_cast<v4u *>(dst), n16);
+    GH_LAUNCH(ctx);
+  }
+  GH_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+  GH_HIP(ctx, hipEventSynchronize(ctx->ev[1]));
+  float ms = 0;
+  GH_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+  *ms_per_iter = ms / (float)iters;
+  return GHIVE_OK;
+}

@@ -341,7 +341,10 @@ class Fragment:
         self.f = f
         self.nkeys, self.naggs = len(keys), len(aggs)
 
-    def execute(self, ctx, capacity=4096, late=False, outs=None):
+    def execute(self, ctx, capacity=4096, late=False, outs=None):  # noqa: D102 (see class doc)
+        return self._execute(ctx, capacity, late, outs)
+
+    def _execute(self, ctx, capacity, late, outs):
         dev = self.fact.columns[0].device
         if outs is None:
             outs = ([torch.empty(capacity, dtype=torch.int32, device=dev) for _ in range(self.nkeys)] +
@@ -353,3 +356,68 @@ class Fragment:
         fn = lib().ghive_fragment_execute_late if late else lib().ghive_fragment_execute
         ctx.check(fn(ctx.h, C.byref(self.f), ocols, capacity, C.byref(g)))
         return [o[: g.value] for o in outs]
+
+
+class Batch:
+    """Batched form (ghive_batch_*): many fragments of one query shape in one persistent kernel
+    replayed from a HIP graph. items: list of (key, a, b, filter_lo, filter_hi) where key/a/b
+    are int32 device tensors of one fragment (b may be None)."""
+
+    def __init__(self, ctx, dim, items, filter_shift, filter_bits, group_shift, group_bits,
+                 tile_rows=0, blocks_per_cu=0, probe_mode=0, use_graph=True):
+        self.ctx = ctx
+        self._keep = [dim, items]
+        self.spec = abi.BatchSpec(dim.h.value, filter_shift, filter_bits, group_shift, group_bits)
+        arr = (abi.BatchItem * max(len(items), 1))()
+        for i, (k, a, b, lo, hi) in enumerate(items):
+            arr[i] = abi.BatchItem(k.data_ptr() if k.numel() else None, a.data_ptr() if a.numel() else None,
+                                   b.data_ptr() if b is not None and b.numel() else None, k.numel(), lo, hi)
+        self.n = len(items)
+        tn = abi.BatchTuning(tile_rows, blocks_per_cu, probe_mode, 0)
+        h = C.c_void_p()
+        ctx.check(lib().ghive_batch_create(ctx.h, C.byref(self.spec), arr, len(items), C.byref(tn),
+                                           1 if use_graph else 0, C.byref(h)))
+        self.h = h
+        self.groups = 1 << group_bits
+
+    def run(self):
+        """Runs the batch; returns the device time in ms (HIP events around the launch)."""
+        self.ctx.check(lib().ghive_batch_run(self.ctx.h, self.h, C.byref(self.spec)))
+        return self.info()["last_ms"]
+
+    def results(self):
+        import numpy as np
+        sums = np.empty(self.n * self.groups, dtype=np.int64)
+        counts = np.empty(self.n * self.groups, dtype=np.int64)
+        self.ctx.check(lib().ghive_batch_results(self.ctx.h, self.h, sums.ctypes.data_as(C.c_void_p),
+                                                 counts.ctypes.data_as(C.c_void_p)))
+        return sums.reshape(self.n, self.groups), counts.reshape(self.n, self.groups)
+
+    def info(self):
+        g, rows, tiles, mode, grid, ms = (C.c_uint32(), C.c_uint64(), C.c_uint64(), C.c_int32(),
+                                          C.c_uint32(), C.c_float())
+        self.ctx.check(lib().ghive_batch_info(self.h, C.byref(g), C.byref(rows), C.byref(tiles),
+                                              C.byref(mode), C.byref(grid), C.byref(ms)))
+        return dict(groups=g.value, rows=rows.value, tiles=tiles.value, probe_mode=mode.value,
+                    grid=grid.value, last_ms=ms.value)
+
+    def free(self):
+        if self.h:
+            lib().ghive_batch_free(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def bench_stream(ctx, nbytes, mode=0, iters=10):
+    """Achievable HBM bandwidth (GB/s): mode 0 read, mode 1 copy (counts read + write)."""
+    src = torch.empty(nbytes // 4, dtype=torch.int32, device=f"cuda:{ctx.device}").fill_(1)
+    dst = torch.empty_like(src) if mode == 1 else None
+    ms = C.c_float()
+    ctx.check(lib().ghive_bench_stream(ctx.h, _p(dst), _p(src), nbytes, mode, iters, C.byref(ms)))
+    moved = nbytes * (2 if mode == 1 else 1)
+    return moved / (ms.value * 1e-3) / 1e9, ms.value

@@ -122,3 +122,90 @@ def star_fragment(fact, hd, hs, hp):
 
 
 STAR_REFERENCED_BYTES_PER_ROW = 20  # fk_date, fk_supp, fk_part, price, discount (int32)
+SMALL_REFERENCED_BYTES_PER_ROW = 12  # fk, measure, price (int32)
+
+# ---- batched form: 4096 map-task fragments of the small query shape -------------------------
+BATCH_ATTR_SHIFT, BATCH_ATTR_BITS = 8, 3  # dimension payload = attr << 8 | category
+BATCH_CAT_SHIFT, BATCH_CAT_BITS = 0, 5
+
+
+def lognormal_rows(nfrags, seed, median=256_000, sigma=1.32, lo=1_000, hi=16_000_000):
+    """Fragment row counts: lognormal(median, sigma) clipped to [lo, hi], rounded to multiples of
+    4 rows (16-byte aligned fragment starts in shared column buffers)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    r = np.exp(np.log(median) + sigma * rng.standard_normal(nfrags))
+    r = np.clip(np.round(r / 4) * 4, lo, hi).astype(np.int64)
+    return r
+
+
+@dataclass
+class BatchTables:
+    fk: torch.Tensor       # all fragments' fact columns, concatenated
+    measure: torch.Tensor
+    price: torch.Tensor
+    rows: list             # per fragment
+    starts: list           # per fragment (multiples of 4)
+    attr_const: list       # per-fragment filter constant (dim.attr == c)
+    dim: ops.Table         # key, attr, category, packed payload
+    seed: int
+
+
+def batch_tables(ctx, nfrags=4096, seed=SEED_BASE + 4, dim_rows=200_000):
+    import numpy as np
+    rows = lognormal_rows(nfrags, seed)
+    starts = np.concatenate([[0], np.cumsum(rows)[:-1]]).astype(np.int64)
+    total = int(rows.sum())
+    fk = ctx.gen_uniform_i32(seed, FACT_FK, total, 1, dim_rows)
+    measure = ctx.gen_uniform_i32(seed, FACT_MEASURE, total, 1, 100)
+    price = ctx.gen_uniform_i32(seed, FACT_PRICE, total, 100_000, 10_000_000)
+    consts = np.random.default_rng(seed + 1).integers(0, 7, nfrags).tolist()
+    key = ctx.gen_sequence_i32(dim_rows, 1)
+    attr = ctx.gen_uniform_i32(seed, DIM_ATTR, dim_rows, 0, 6)
+    cat = ctx.gen_uniform_i32(seed, DIM_CATEGORY, dim_rows, 0, 24)
+    packed = ((attr << BATCH_ATTR_SHIFT) | (cat << BATCH_CAT_SHIFT)).contiguous()
+    return BatchTables(fk, measure, price, rows.tolist(), starts.tolist(), consts,
+                       ops.Table([key, attr, cat, packed]), seed)
+
+
+def batch_dim_table(ctx, B, load_factor=0.5):
+    """The resident, unfiltered dimension table shared by every fragment (payload = packed)."""
+    return ops.HashTable(ctx, B.dim.columns[0], payload=B.dim.columns[3], load_factor=load_factor)
+
+
+def batch_items(B):
+    items = []
+    for s, r, c in zip(B.starts, B.rows, B.attr_const):
+        items.append((B.fk[s:s + r], B.measure[s:s + r], B.price[s:s + r], c, c))
+    return items
+
+
+def batch_create(ctx, B, dim_ht, tile_rows=0, blocks_per_cu=0, probe_mode=0, use_graph=True):
+    return ops.Batch(ctx, dim_ht, batch_items(B), BATCH_ATTR_SHIFT, BATCH_ATTR_BITS, BATCH_CAT_SHIFT,
+                     BATCH_CAT_BITS, tile_rows=tile_rows, blocks_per_cu=blocks_per_cu,
+                     probe_mode=probe_mode, use_graph=use_graph)
+
+
+class PlainBatch:
+    """The plain form of the same batch: one ghive_fragment_execute per fragment, with the
+    dimension filtered and its hash table built once per distinct filter constant."""
+
+    def __init__(self, ctx, B, load_factor=0.5):
+        self.ctx, self.B = ctx, B
+        self.tables = {}
+        for c in sorted(set(B.attr_const)):
+            self.tables[c] = small_dim_table(ctx, B.dim, attr_value=c, load_factor=load_factor)
+        dev = B.fk.device
+        self.outs = [torch.empty(64, dtype=torch.int32, device=dev), torch.empty(64, dtype=torch.int64, device=dev)]
+        self.frags = []
+        for s, r, c in zip(B.starts, B.rows, B.attr_const):
+            fact = ops.Table([B.fk[s:s + r], B.measure[s:s + r], B.price[s:s + r]])
+            self.frags.append(small_fragment(fact, self.tables[c]))
+
+    def run(self, collect=False):
+        res = []
+        for f in self.frags:
+            k, v = f.execute(self.ctx, capacity=64, outs=self.outs)
+            if collect:
+                res.append(dict(zip(k.cpu().tolist(), v.cpu().tolist())))
+        return res

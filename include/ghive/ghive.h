@@ -119,6 +119,11 @@ int ghive_free(ghive_ctx *ctx, void *ptr);
 int ghive_memcpy(ghive_ctx *ctx, void *dst, const void *src, uint64_t bytes, int kind);
 int ghive_mem_stats(ghive_ctx *ctx, uint64_t *in_use, uint64_t *peak, uint64_t *limit,
                     uint64_t *reserved);
+/* STREAM-style probe of achievable HBM bandwidth (no counterpart in the reference): mode 0
+ * reads `bytes` from src, mode 1 copies src -> dst; 16-byte aligned, iters launches, mean time
+ * per launch from HIP events. Calibrates roofline fractions and PMC byte counters. */
+int ghive_bench_stream(ghive_ctx *ctx, void *dst, const void *src, uint64_t bytes, int mode,
+                       int iters, float *ms_per_iter);
 
 /* ---- synthetic data: counter-based Philox4x32-10 (Salmon et al., SC'11) -------------- *
  * Value of row r in column c is a pure function of (seed, c, r), so any slice can be     *

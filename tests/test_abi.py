@@ -47,6 +47,8 @@ def test_struct_layouts_match_header(tmp_path):
         "ghive_fragment_join": abi.FragmentJoin, "ghive_fragment_key": abi.FragmentKey,
         "ghive_fragment_agg": abi.FragmentAgg, "ghive_fragment": abi.Fragment,
         "ghive_device_info": abi.DeviceInfo, "ghive_stage_times": abi.StageTimes,
+        "ghive_batch_item": abi.BatchItem, "ghive_batch_spec": abi.BatchSpec,
+        "ghive_batch_tuning": abi.BatchTuning,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ghive/ghive.h"', "int main(void){"]
     for cname, py in structs.items():
