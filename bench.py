@@ -126,6 +126,7 @@ def main():
     ap.add_argument("--rows-per-thread", type=int, default=0, help="fused kernel rows/thread (4 or 8)")
     ap.add_argument("--nontemporal", type=int, default=-1, help="non-temporal fact loads (0/1)")
     ap.add_argument("--grid", type=int, default=0, help="fused kernel workgroups (0 = resident wave)")
+    ap.add_argument("--block", type=int, default=0, help="fused kernel workgroup size (256/512/1024)")
     ap.add_argument("--cpu-sample", type=int, default=600_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 4096-fragment batched form")
@@ -152,6 +153,8 @@ def main():
         ctx.set_option("fragment.nontemporal", args.nontemporal)
     if args.grid:
         ctx.set_option("fragment.grid", args.grid)
+    if args.block:
+        ctx.set_option("fragment.block", args.block)
     variant = ("late" if args.late else "early") + ("/hash" if args.no_dense else
                                                     ("/hbm-bitmap" if args.no_lds_bits else "/lds-bitmap"))
     info = ctx.device_info()

@@ -93,5 +93,7 @@ def lib():
 
 SIGNATURES_EXTRA = {
     "ghive_bench_stream": (C.c_int, [P, P, P, u64, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+    "ghive_gen_rank_keys": (C.c_int, [P, u64, u64, u64, P]),
+    "ghive_gen_zipf_keys": (C.c_int, [P, u64, u32, u64, u64, u64, C.c_double, C.c_double, P]),
 }
 SIGNATURES.update(SIGNATURES_EXTRA)

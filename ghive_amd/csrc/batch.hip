@@ -25,6 +25,9 @@ struct ghive_batch {
   ghive_batch_tuning tune;
   int mode;              // 1 LDS bitmap, 2 HBM bitmap, 3 hash
   void *d_items;         // BatchItemDev[nitems + 1]
+  uint32_t *d_fbits;     // filtered bitmaps, one per distinct [lo, hi]
+  uint32_t npairs;
+  uint32_t nwords;
   unsigned long long *d_acc;  // sums then counts, nitems * groups each
   hipStream_t stream;
   hipEvent_t ev_in, ev_out, t0, t1;
@@ -43,6 +46,8 @@ struct BatchItemDev {
   uint64_t rows;
   uint64_t tile_begin;  // first global tile of this item (item nitems: total tiles)
   int32_t lo, hi;
+  uint32_t fidx;        // which filtered bitmap (distinct [lo, hi] pairs share one)
+  uint32_t pad;
 };
 
 struct BatchDev {
@@ -62,7 +67,31 @@ struct BatchDev {
   uint32_t groups;
   unsigned long long *sums;
   unsigned long long *counts;
+  const uint32_t *fbits;  // modes 1/2: per distinct filter, bitmap of dimension keys that pass it
 };
+
+// Filtered bitmaps: bit u of filter p is set when key (key_min + u) is in the dimension and its
+// filter field lies in [lo_p, hi_p]. One thread per 32-key word.
+__global__ void k_batch_filter_bits(const uint32_t *bitmap, const int32_t *dense, uint64_t range, uint32_t nwords,
+                                    const int32_t *lohi, uint32_t npairs, uint32_t fshift, uint32_t fmask,
+                                    uint32_t *fbits) {
+  const uint64_t total = (uint64_t)nwords * npairs;
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < total; i += stride) {
+    const uint32_t p = (uint32_t)(i / nwords), w = (uint32_t)(i % nwords);
+    const int32_t lo = lohi[2 * p], hi = lohi[2 * p + 1];
+    const uint32_t present = bitmap[w];
+    uint32_t out = 0;
+    for (int b = 0; b < 32; ++b) {
+      const uint64_t u = (uint64_t)w * 32 + b;
+      if (u >= range || !((present >> b) & 1u)) continue;
+      const int32_t f = (int32_t)(((uint32_t)dense[u] >> fshift) & fmask);
+      if (f >= lo && f <= hi) out |= 1u << b;
+    }
+    fbits[(uint64_t)p * nwords + w] = out;
+  }
+}
 
 constexpr int kBatchThreads = 256;
 constexpr int kNull = 0x7fffffff;
@@ -98,8 +127,6 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchDev B) {
   unsigned long long *lcnt = lsum + B.groups;
   uint32_t *lbits = reinterpret_cast<uint32_t *>(lcnt + B.groups);
   for (uint32_t g = threadIdx.x; g < B.groups; g += blockDim.x) lsum[g] = lcnt[g] = 0;
-  if (B.mode == 1)
-    for (uint32_t w = threadIdx.x; w < B.nwords; w += blockDim.x) lbits[w] = B.bitmap[w];
   __syncthreads();
   const uint64_t t_begin = B.total_tiles * blockIdx.x / gridDim.x;
   const uint64_t t_end = B.total_tiles * (blockIdx.x + 1) / gridDim.x;
@@ -114,6 +141,7 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchDev B) {
   uint32_t item = lo;
   BatchItemDev it = B.items[item];
   uint64_t next_begin = B.items[item + 1].tile_begin;
+  uint32_t loaded = 0xffffffffu;  // filtered bitmap currently in LDS (mode 1)
   for (uint64_t t = t_begin; t < t_end; ++t) {
     while (t >= next_begin) {  // crossed into the next item(s)
       batch_flush(B, lsum, lcnt, item);
@@ -121,6 +149,14 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchDev B) {
       it = B.items[item];
       next_begin = B.items[item + 1].tile_begin;
     }
+    if (B.mode == 1 && it.fidx != loaded) {  // uniform: every thread sees the same item
+      __syncthreads();
+      const uint32_t *src = B.fbits + (uint64_t)it.fidx * B.nwords;
+      for (uint32_t w = threadIdx.x; w < B.nwords; w += blockDim.x) lbits[w] = src[w];
+      loaded = it.fidx;
+      __syncthreads();
+    }
+    const uint32_t *gbits = B.mode == 2 ? B.fbits + (uint64_t)it.fidx * B.nwords : nullptr;
     const uint64_t base = (t - it.tile_begin) * B.tile_rows;
     for (uint32_t sub = 0; sub < B.tile_rows; sub += kBatchThreads * 4) {
       const uint64_t r0 = base + sub + (uint64_t)threadIdx.x * 4;
@@ -137,12 +173,28 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchDev B) {
       }
       int32_t pay[4] = {0, 0, 0, 0};
       if (B.mode == 1 || B.mode == 2) {
+        // the filtered bitmap already holds "key present and its filter field in [lo, hi]", so
+        // only the rows that pass read the payload
+        if (B.mode == 1) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          if (!((alive >> q) & 1u)) continue;
-          const uint32_t u = (uint32_t)((int64_t)k[q] - B.key_min);
-          const uint32_t w = B.mode == 1 ? lbits[u >> 5] : B.bitmap[u >> 5];
-          if (!((w >> (u & 31)) & 1u)) alive &= ~(1u << q);
+          for (int q = 0; q < 4; ++q) {
+            if (!((alive >> q) & 1u)) continue;
+            const uint32_t u = (uint32_t)((int64_t)k[q] - B.key_min);
+            if (!((lbits[u >> 5] >> (u & 31)) & 1u)) alive &= ~(1u << q);
+          }
+        } else {
+          uint32_t w[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            w[q] = 0;
+            if ((alive >> q) & 1u) {
+              const uint32_t u = (uint32_t)((int64_t)k[q] - B.key_min);
+              w[q] = gbits[u >> 5] >> (u & 31);
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (!(w[q] & 1u)) alive &= ~(1u << q);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -206,6 +258,7 @@ void ghive_batch_free(ghive_ctx *ctx, ghive_batch *bt) {
   if (bt->graph) (void)hipGraphDestroy(bt->graph);
   if (bt->d_items) c->arena->free(bt->d_items);
   if (bt->d_acc) c->arena->free(bt->d_acc);
+  if (bt->d_fbits) c->arena->free(bt->d_fbits);
   for (hipEvent_t e : {bt->ev_in, bt->ev_out, bt->t0, bt->t1})
     if (e) (void)hipEventDestroy(e);
   if (bt->stream) (void)hipStreamDestroy(bt->stream);
@@ -254,6 +307,7 @@ int ghive_batch_create(ghive_ctx *ctx, const ghive_batch_spec *spec, const ghive
   bt->mode = mode;
   bt->lds_bytes = acc_lds + (mode == 1 ? (size_t)nwords * 4 : 0);
   std::vector<BatchItemDev> hv(nitems + 1);
+  std::vector<std::pair<int32_t, int32_t>> pairs;
   uint64_t tiles = 0, rows = 0;
   for (uint32_t i = 0; i < nitems; ++i) {
     const ghive_batch_item &s = items[i];
@@ -263,13 +317,19 @@ int ghive_batch_create(ghive_ctx *ctx, const ghive_batch_spec *spec, const ghive
       if (p && ((uintptr_t)p & 15))
         return delete bt, set_error(ctx, GHIVE_ERR_INVALID, "batch: item %u column is not 16-byte aligned", i);
     if (s.rows > kMaxRows) return delete bt, set_error(ctx, GHIVE_ERR_UNSUPPORTED, "batch: item %u too large", i);
-    hv[i] = BatchItemDev{s.key, s.a, s.b, s.rows, tiles, s.filter_lo, s.filter_hi};
+    uint32_t fidx = 0;
+    for (; fidx < pairs.size(); ++fidx)
+      if (pairs[fidx].first == s.filter_lo && pairs[fidx].second == s.filter_hi) break;
+    if (fidx == pairs.size()) pairs.push_back({s.filter_lo, s.filter_hi});
+    hv[i] = BatchItemDev{s.key, s.a, s.b, s.rows, tiles, s.filter_lo, s.filter_hi, fidx, 0};
     tiles += (s.rows + tn.tile_rows - 1) / tn.tile_rows;
     rows += s.rows;
   }
-  hv[nitems] = BatchItemDev{nullptr, nullptr, nullptr, 0, tiles, 0, 0};
+  hv[nitems] = BatchItemDev{nullptr, nullptr, nullptr, 0, tiles, 0, 0, 0, 0};
   bt->total_tiles = tiles;
   bt->total_rows = rows;
+  bt->npairs = (uint32_t)pairs.size();
+  bt->nwords = nwords;
   int st = GHIVE_OK;
   bt->d_items = ctx->arena->alloc(sizeof(BatchItemDev) * (nitems + 1));
   bt->d_acc = static_cast<unsigned long long *>(ctx->arena->alloc(16ull * nitems * bt->groups + 16));
@@ -284,6 +344,30 @@ int ghive_batch_create(ghive_ctx *ctx, const ghive_batch_spec *spec, const ghive
   if (st == GHIVE_OK && hipMemcpyAsync(bt->d_items, hv.data(), sizeof(BatchItemDev) * hv.size(),
                                        hipMemcpyHostToDevice, bt->stream) != hipSuccess)
     st = set_error(ctx, GHIVE_ERR_HIP, "batch: upload");
+  if (st == GHIVE_OK && (mode == 1 || mode == 2) && bt->npairs) {
+    // one filtered bitmap per distinct [lo, hi]: bit set = key present and filter field passes
+    bt->d_fbits = static_cast<uint32_t *>(ctx->arena->alloc(4ull * nwords * bt->npairs));
+    void *d_lohi = ctx->arena->alloc(8ull * bt->npairs);
+    std::vector<int32_t> lohi;
+    for (auto &pr : pairs) {
+      lohi.push_back(pr.first);
+      lohi.push_back(pr.second);
+    }
+    if (!bt->d_fbits || !d_lohi) {
+      st = set_error(ctx, GHIVE_ERR_OOM, "batch: filtered bitmaps do not fit");
+    } else if (hipMemcpyAsync(d_lohi, lohi.data(), 8ull * bt->npairs, hipMemcpyHostToDevice, bt->stream) != hipSuccess) {
+      st = set_error(ctx, GHIVE_ERR_HIP, "batch: upload");
+    } else {
+      const uint64_t total = (uint64_t)nwords * bt->npairs;
+      const uint32_t fmask = spec->filter_bits >= 32 ? ~0u : ((1u << spec->filter_bits) - 1);
+      k_batch_filter_bits<<<(uint32_t)std::min<uint64_t>((total + 255) / 256, (uint64_t)ctx->cus * 8), 256, 0,
+                            bt->stream>>>(h->bitmap, h->dense, h->range, nwords, static_cast<const int32_t *>(d_lohi),
+                                          bt->npairs, spec->filter_shift, fmask, bt->d_fbits);
+      if (hipGetLastError() != hipSuccess) st = set_error(ctx, GHIVE_ERR_HIP, "batch: filter bitmaps launch");
+    }
+    if (hipStreamSynchronize(bt->stream) != hipSuccess && st == GHIVE_OK) st = set_error(ctx, GHIVE_ERR_HIP, "batch: sync");
+    if (d_lohi) ctx->arena->free(d_lohi);
+  }
   if (st == GHIVE_OK && hipStreamSynchronize(bt->stream) != hipSuccess) st = set_error(ctx, GHIVE_ERR_HIP, "batch: sync");
   if (st != GHIVE_OK) {
     ghive_batch_free(ctx, bt);
@@ -318,6 +402,7 @@ int ghive_batch_create(ghive_ctx *ctx, const ghive_batch_spec *spec, const ghive
     B.groups = bt->groups;
     B.sums = bt->d_acc;
     B.counts = bt->d_acc + (uint64_t)nitems * bt->groups;
+    B.fbits = bt->d_fbits;
     if (hipStreamBeginCapture(bt->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
       st = set_error(ctx, GHIVE_ERR_HIP, "batch: begin capture");
     if (st == GHIVE_OK) {
@@ -377,6 +462,7 @@ int ghive_batch_run(ghive_ctx *ctx, ghive_batch *bt, const ghive_batch_spec *spe
     B.groups = bt->groups;
     B.sums = bt->d_acc;
     B.counts = bt->d_acc + (uint64_t)bt->nitems * bt->groups;
+    B.fbits = bt->d_fbits;
     GH_TRY(batch_enqueue(ctx, bt, B));
   }
   GH_HIP(ctx, hipEventRecord(bt->t1, bt->stream));

@@ -58,6 +58,7 @@ struct ghive_tuning {
   int64_t frag_dense = 1;               // 1: probe dense dimensions through bitmap + payload array
   int64_t frag_rows = 4;                // fact rows per thread per tile (4 or 8)
   int64_t frag_nt = 0;                  // 1: non-temporal loads of the fact columns
+  int64_t frag_block = 256;             // fused-kernel workgroup size (256, 512 or 1024)
   int64_t gb_lds_budget = 64 * 1024;    // bytes of LDS for the group-by's private table
 };
 

@@ -303,6 +303,10 @@ int ghive_set_option(ghive_ctx *ctx, const char *name, int64_t value) {
     ctx->tune.frag_rows = value;
   } else if (k == "fragment.nontemporal") {
     ctx->tune.frag_nt = value ? 1 : 0;
+  } else if (k == "fragment.block") {
+    if (value != 256 && value != 512 && value != 1024)
+      return set_error(ctx, GHIVE_ERR_INVALID, "option %s: %lld (256, 512 or 1024)", name, (long long)value);
+    ctx->tune.frag_block = value;
   } else if (k == "groupby.lds_budget") {
     if (value < 0 || value > 160 * 1024) return set_error(ctx, GHIVE_ERR_INVALID, "option %s: %lld", name, (long long)value);
     ctx->tune.gb_lds_budget = value;
@@ -322,6 +326,7 @@ int ghive_get_option(ghive_ctx *ctx, const char *name, int64_t *value) {
   else if (k == "fragment.dense") *value = ctx->tune.frag_dense;
   else if (k == "fragment.rows_per_thread") *value = ctx->tune.frag_rows;
   else if (k == "fragment.nontemporal") *value = ctx->tune.frag_nt;
+  else if (k == "fragment.block") *value = ctx->tune.frag_block;
   else if (k == "groupby.lds_budget") *value = ctx->tune.gb_lds_budget;
   else return set_error(ctx, GHIVE_ERR_INVALID, "unknown option '%s'", name);
   return GHIVE_OK;

@@ -199,12 +199,13 @@ int fragment_plan(ghive_ctx *ctx, const ghive_fragment *f, int late, FragPlan &P
   const int rows = ctx->tune.frag_rows == 8 ? 8 : 4;
   FragKernel kfn = rows == 8 ? pick_kernel_r8(F.njoins, F.naggs) : pick_kernel_r4(F.njoins, F.naggs);
   P.kernel = reinterpret_cast<const void *>(kfn);
+  P.threads = (uint32_t)ctx->tune.frag_block;
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kFragThreads, P.lds_bytes) != hipSuccess || per_cu < 1) {
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, (int)P.threads, P.lds_bytes) != hipSuccess || per_cu < 1) {
     (void)hipGetLastError();
     per_cu = 1;
   }
-  const uint64_t tiles = (F.n + (uint64_t)kFragThreads * rows - 1) / ((uint64_t)kFragThreads * rows);
+  const uint64_t tiles = (F.n + (uint64_t)P.threads * rows - 1) / ((uint64_t)P.threads * rows);
   const uint64_t cap = (uint64_t)ctx->cus * per_cu;
   P.grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(tiles, cap));
   if (ctx->tune.frag_grid > 0) P.grid = (uint32_t)ctx->tune.frag_grid;
@@ -218,7 +219,7 @@ int fragment_launch(ghive_ctx *ctx, const FragPlan &P, unsigned long long *d_joi
   int lg = P.lcap_log2;
   unsigned long long *j = d_joined;
   void *args[] = {&F, &T, &lg, &j};
-  GH_HIP(ctx, hipLaunchKernel(P.kernel, dim3(P.grid), dim3(kFragThreads), args, P.lds_bytes, ctx->stream));
+  GH_HIP(ctx, hipLaunchKernel(P.kernel, dim3(P.grid), dim3(P.threads), args, P.lds_bytes, ctx->stream));
   return GHIVE_OK;
 }
 

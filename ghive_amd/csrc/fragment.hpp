@@ -66,6 +66,7 @@ struct FragPlan {
   int lcap_log2;         // LDS group table (log2 slots) or -1: global table only
   size_t lds_bytes;
   uint32_t grid;
+  uint32_t threads;  // workgroup size
   int late;
   const void *kernel;
 };

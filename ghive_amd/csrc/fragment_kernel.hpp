@@ -9,7 +9,8 @@
 
 namespace ghive {
 
-constexpr int kFragThreads = 256;
+constexpr int kFragThreads = 256;     // default workgroup size
+constexpr int kFragMaxThreads = 1024; // launch bound; the size is chosen at launch
 constexpr int kNullI32 = 0x7fffffff;
 
 typedef int v4i __attribute__((ext_vector_type(4)));
@@ -40,7 +41,7 @@ __device__ __forceinline__ bool in_range(const FragJoinDev &J, int32_t k) {
 
 // NJ joins, NA aggregates, R rows per thread (4 or 8). F.late / F.nt are uniform flags.
 template <int NJ, int NA, int R>
-__global__ __launch_bounds__(kFragThreads) void k_fragment(FragDev F, GroupTableDev T, int lcap_log2,
+__global__ __launch_bounds__(kFragMaxThreads) void k_fragment(FragDev F, GroupTableDev T, int lcap_log2,
                                                            unsigned long long *joined) {
   extern __shared__ uint64_t lds64[];
   uint32_t *lbits = reinterpret_cast<uint32_t *>(lds64);
@@ -57,7 +58,7 @@ __global__ __launch_bounds__(kFragThreads) void k_fragment(FragDev F, GroupTable
   __syncthreads();
 
   constexpr uint32_t kAll = (1u << R) - 1u;
-  const uint64_t tile_rows = (uint64_t)kFragThreads * R;
+  const uint64_t tile_rows = (uint64_t)blockDim.x * R;
   const uint64_t n = F.n;
   const uint64_t ntiles = (n + tile_rows - 1) / tile_rows;
   uint32_t njoined = 0;

@@ -59,6 +59,39 @@ __global__ void k_gen_normal(uint64_t seed, uint32_t col, uint64_t row0, uint64_
   }
 }
 
+__global__ void k_gen_rank_keys(uint64_t seed, uint64_t rank0, uint64_t n, int64_t *out) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) out[i] = (int64_t)key62(seed, rank0 + i);
+}
+
+// Probe keys: with probability miss_frac a key of a rank above nranks (never a build key), else
+// the key of a Zipf(s) rank in [1, nranks]. Attempt a of row r draws Philox counter
+// (r_lo, r_hi, column, a); rejection-inversion accepts in ~1.1 attempts.
+__global__ void k_gen_zipf_keys(uint64_t seed, uint32_t col, uint64_t row0, uint64_t n, ZipfParams zp,
+                                uint64_t nranks, uint32_t miss_threshold, int64_t *out) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    const uint64_t r = row0 + i;
+    uint32_t w[4];
+    philox4x32_10((uint32_t)r, (uint32_t)(r >> 32), col, 0u, (uint32_t)seed, (uint32_t)(seed >> 32), w);
+    uint64_t rank;
+    if (w[0] < miss_threshold) {
+      rank = nranks + 1 + (((uint64_t)w[1] << 8) | (w[2] & 0xffu));  // 40-bit offset past the build ranks
+    } else {
+      rank = 0;
+      for (uint32_t a = 0; a < 64 && rank == 0; ++a) {
+        if (a) philox4x32_10((uint32_t)r, (uint32_t)(r >> 32), col, a, (uint32_t)seed, (uint32_t)(seed >> 32), w);
+        const uint64_t bits = ((uint64_t)w[2] << 32) | w[3];
+        rank = zipf_try(zp, (double)(bits >> 11) * (1.0 / 9007199254740992.0));
+      }
+      if (rank == 0) rank = 1;  // 64 rejections in a row (probability ~1e-60)
+    }
+    out[i] = (int64_t)key62(seed, rank);
+  }
+}
+
 template <class T>
 __global__ void k_seq(uint64_t row0, uint64_t n, T start, T *out) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -125,6 +158,31 @@ int ghive_gen_uniform_f32(ghive_ctx *ctx, uint64_t seed, uint32_t column_id, uin
   if (!n) return GHIVE_OK;
   if (!out) return set_error(ctx, GHIVE_ERR_INVALID, "gen: out is NULL");
   k_gen_f32<<<gen_grid(ctx, n), 256, 0, ctx->stream>>>(seed, column_id, row_begin, n, out);
+  GH_LAUNCH(ctx);
+  return GHIVE_OK;
+}
+
+int ghive_gen_rank_keys(ghive_ctx *ctx, uint64_t seed, uint64_t rank_begin, uint64_t n, int64_t *out) {
+  GH_CHECK_CTX(ctx);
+  if (!n) return GHIVE_OK;
+  if (!out) return set_error(ctx, GHIVE_ERR_INVALID, "gen: out is NULL");
+  k_gen_rank_keys<<<gen_grid(ctx, n), 256, 0, ctx->stream>>>(seed, rank_begin, n, out);
+  GH_LAUNCH(ctx);
+  return GHIVE_OK;
+}
+
+int ghive_gen_zipf_keys(ghive_ctx *ctx, uint64_t seed, uint32_t column_id, uint64_t row_begin, uint64_t n,
+                        uint64_t nranks, double theta, double miss_frac, int64_t *out) {
+  GH_CHECK_CTX(ctx);
+  if (nranks < 1 || nranks > (1ull << 40) || !(theta > 0.0) || theta > 10.0 || !(miss_frac >= 0.0) ||
+      miss_frac > 1.0)
+    return set_error(ctx, GHIVE_ERR_INVALID, "gen_zipf: nranks %llu theta %g miss %g", (unsigned long long)nranks,
+                     theta, miss_frac);
+  if (!n) return GHIVE_OK;
+  if (!out) return set_error(ctx, GHIVE_ERR_INVALID, "gen: out is NULL");
+  ZipfParams zp = zipf_params(theta, nranks);
+  const uint32_t thr = miss_frac >= 1.0 ? 0xffffffffu : (uint32_t)(miss_frac * 4294967296.0);
+  k_gen_zipf_keys<<<gen_grid(ctx, n), 256, 0, ctx->stream>>>(seed, column_id, row_begin, n, zp, nranks, thr, out);
   GH_LAUNCH(ctx);
   return GHIVE_OK;
 }

@@ -97,6 +97,16 @@ class Context:
         self.check(lib().ghive_gen_uniform_f32(self.h, seed, col, row_begin, n, _p(out)))
         return out
 
+    def gen_rank_keys(self, seed, n, rank_begin=1):
+        out = torch.empty(n, dtype=torch.int64, device=f"cuda:{self.device}")
+        self.check(lib().ghive_gen_rank_keys(self.h, seed, rank_begin, n, _p(out)))
+        return out
+
+    def gen_zipf_keys(self, seed, col, n, nranks, theta=1.0, miss_frac=0.1, row_begin=0):
+        out = torch.empty(n, dtype=torch.int64, device=f"cuda:{self.device}")
+        self.check(lib().ghive_gen_zipf_keys(self.h, seed, col, row_begin, n, nranks, theta, miss_frac, _p(out)))
+        return out
+
     def gen_normal_f64(self, seed, col, n, row_begin=0):
         out = torch.empty(n, dtype=torch.float64, device=f"cuda:{self.device}")
         self.check(lib().ghive_gen_normal_f64(self.h, seed, col, row_begin, n, _p(out)))

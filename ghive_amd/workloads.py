@@ -124,6 +124,51 @@ def star_fragment(fact, hd, hs, hp):
 STAR_REFERENCED_BYTES_PER_ROW = 20  # fk_date, fk_supp, fk_part, price, discount (int32)
 SMALL_REFERENCED_BYTES_PER_ROW = 12  # fk, measure, price (int32)
 
+# ---- skewed join + sort + high-cardinality group-by -------------------------------------------
+@dataclass
+class SkewTables:
+    build_keys: torch.Tensor   # unique int64 keys of ranks 1..B
+    build_pay: torch.Tensor    # int64 payload = rank
+    probe_keys: torch.Tensor   # Zipf(theta) over the build ranks, miss_frac absent keys
+    tiebreak: torch.Tensor     # float32 U[0,1) per probe row
+    seed: int
+
+
+def skew_tables(ctx, build_rows=64_000_000, probe_rows=1_000_000_000, seed=SEED_BASE + 3, theta=1.0,
+                miss_frac=0.1):
+    bk = ctx.gen_rank_keys(seed, build_rows, rank_begin=1)
+    bp = torch.arange(1, build_rows + 1, dtype=torch.int64, device=bk.device)
+    pk = ctx.gen_zipf_keys(seed, 0, probe_rows, build_rows, theta, miss_frac)
+    tb = ctx.gen_uniform_f32(seed, 1, probe_rows)
+    return SkewTables(bk, bp, pk, tb, seed)
+
+
+def key62_torch(seed, rank):
+    """The generator's rank -> key bijection with torch int64 ops (wraps mod 2^64, masked to 62 bits)."""
+    M = (1 << 62) - 1
+    off = (seed * 0x9E3779B97F4A7C15) & ((1 << 64) - 1)
+    x = (rank + (off & M)) & M
+    x = x ^ (x >> 31)
+    x = (x * _i64(0x7fb5d329728ea185)) & M
+    x = x ^ (x >> 27)
+    x = (x * _i64(0x81dadef4bc2dd44d)) & M
+    x = x ^ (x >> 33)
+    return x
+
+
+def _i64(u):
+    return u - (1 << 64) if u >= (1 << 63) else u
+
+
+def highcard_tables(ctx, rows=1_000_000_000, groups=100_000_000, seed=SEED_BASE + 5):
+    """GROUP BY key COUNT(*), SUM(x): keys uniform over `groups` 62-bit values, x ~ N(0,1)."""
+    ranks = ctx.gen_uniform_i64(seed, 0, rows, 1, groups)
+    keys = key62_torch(seed, ranks)
+    del ranks
+    vals = ctx.gen_normal_f64(seed, 1, rows)
+    return keys, vals
+
+
 # ---- batched form: 4096 map-task fragments of the small query shape -------------------------
 BATCH_ATTR_SHIFT, BATCH_ATTR_BITS = 8, 3  # dimension payload = attr << 8 | category
 BATCH_CAT_SHIFT, BATCH_CAT_BITS = 0, 5

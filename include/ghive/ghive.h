@@ -142,6 +142,14 @@ int ghive_gen_uniform_f32(ghive_ctx *ctx, uint64_t seed, uint32_t column_id, uin
                           uint64_t n, float *out);
 int ghive_gen_normal_f64(ghive_ctx *ctx, uint64_t seed, uint32_t column_id, uint64_t row_begin,
                          uint64_t n, double *out);
+/* Unique 62-bit keys: out[i] = K(seed, rank_begin + i), K a bijection of [0, 2^62) (so keys
+ * never equal LONG_MAX, the NULL sentinel). */
+int ghive_gen_rank_keys(ghive_ctx *ctx, uint64_t seed, uint64_t rank_begin, uint64_t n, int64_t *out);
+/* Skewed probe keys: with probability miss_frac the key of a rank above nranks (absent from a
+ * build side holding ranks 1..nranks), else the key of a Zipf(theta) rank over [1, nranks]
+ * drawn by rejection-inversion (Hoermann & Derflinger 1996). */
+int ghive_gen_zipf_keys(ghive_ctx *ctx, uint64_t seed, uint32_t column_id, uint64_t row_begin, uint64_t n,
+                        uint64_t nranks, double theta, double miss_frac, int64_t *out);
 
 /* ---- filter ------------------------------------------------------------------------- */
 /* PredicateMode (gpu_impl/inc/Operator/FilterPredicate.hpp:17-58), same numbering. */

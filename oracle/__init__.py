@@ -44,6 +44,10 @@ def lib():
             fn = getattr(L, name)
             fn.argtypes = [u64, u32, u64, u64] + extra + [P]
             fn.restype = None
+        L.orc_key62.argtypes = [u64, u64]
+        L.orc_key62.restype = u64
+        L.orc_gen_zipf_keys.argtypes = [u64, u32, u64, u64, u64, C.c_double, C.c_double, P, P]
+        L.orc_gen_zipf_keys.restype = None
         L.orc_hive_long_hash.argtypes = [i64]
         L.orc_hive_long_hash.restype = C.c_int32
         L.orc_filter.argtypes = [P, P, u32, u32, P]
@@ -135,6 +139,18 @@ def gen_normal_f64(seed, col, row_begin, n):
     out = np.empty(n, dtype=np.float64)
     lib().orc_gen_normal_f64(seed, col, row_begin, n, _ptr(out))
     return out
+
+
+def key62(seed, rank):
+    return lib().orc_key62(seed, rank)
+
+
+def gen_zipf_keys(seed, col, row_begin, n, nranks, theta, miss_frac):
+    """(keys int64, ranks uint64) of the skewed probe side."""
+    keys = np.empty(n, dtype=np.int64)
+    ranks = np.empty(n, dtype=np.uint64)
+    lib().orc_gen_zipf_keys(seed, col, row_begin, n, nranks, theta, miss_frac, _ptr(keys), _ptr(ranks))
+    return keys, ranks
 
 
 def hive_long_hash(key):

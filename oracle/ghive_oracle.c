@@ -88,6 +88,66 @@ void orc_gen_normal_f64(uint64_t seed, uint32_t column_id, uint64_t row_begin, u
   }
 }
 
+/* Bijection of [0, 2^62): offset by a seed constant, then xorshift / odd-multiply rounds. */
+uint64_t orc_key62(uint64_t seed, uint64_t rank) {
+  const uint64_t M = (1ull << 62) - 1;
+  uint64_t x = (rank + seed * 0x9E3779B97F4A7C15ull) & M;
+  x ^= x >> 31;
+  x = (x * 0x7fb5d329728ea185ull) & M;
+  x ^= x >> 27;
+  x = (x * 0x81dadef4bc2dd44dull) & M;
+  x ^= x >> 33;
+  return x;
+}
+
+/* Zipf(s) over [1, n] by rejection-inversion (Hoermann & Derflinger, ACM TOMACS 6(3), 1996). */
+static double z_h1(double x) { return fabs(x) > 1e-8 ? log1p(x) / x : 1.0 - x * (0.5 - x * (1.0 / 3.0 - 0.25 * x)); }
+static double z_h2(double x) { return fabs(x) > 1e-8 ? expm1(x) / x : 1.0 + x * 0.5 * (1.0 + x * (1.0 / 3.0) * (1.0 + 0.25 * x)); }
+static double z_H(double s, double x) { double lx = log(x); return z_h2((1.0 - s) * lx) * lx; }
+static double z_h(double s, double x) { return exp(-s * log(x)); }
+static double z_Hinv(double s, double y) {
+  double t = y * (1.0 - s);
+  if (t < -1.0) t = -1.0;
+  return exp(z_h1(t) * y);
+}
+
+void orc_gen_zipf_keys(uint64_t seed, uint32_t column_id, uint64_t row_begin, uint64_t n, uint64_t nranks,
+                       double theta, double miss_frac, int64_t *out_keys, uint64_t *out_ranks) {
+  const double s = theta, N = (double)nranks;
+  const double h_x1 = z_H(s, 1.5) - 1.0, h_n = z_H(s, N + 0.5);
+  const double sx = 2.0 - z_Hinv(s, z_H(s, 2.5) - z_h(s, 2.0));
+  const uint32_t thr = miss_frac >= 1.0 ? 0xffffffffu : (uint32_t)(miss_frac * 4294967296.0);
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint64_t r = row_begin + i;
+    uint32_t w[4];
+    uint32_t ctr[4] = {(uint32_t)r, (uint32_t)(r >> 32), column_id, 0u};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    orc_philox4x32_10(ctr, key, w);
+    uint64_t rank = 0;
+    if (w[0] < thr) {
+      rank = nranks + 1 + (((uint64_t)w[1] << 8) | (w[2] & 0xffu));
+    } else {
+      for (uint32_t a = 0; a < 64 && rank == 0; ++a) {
+        if (a) {
+          ctr[3] = a;
+          orc_philox4x32_10(ctr, key, w);
+        }
+        uint64_t bits = ((uint64_t)w[2] << 32) | w[3];
+        double u = (double)(bits >> 11) * (1.0 / 9007199254740992.0);
+        double uu = h_n + u * (h_x1 - h_n);
+        double x = z_Hinv(s, uu);
+        double kf = floor(x + 0.5);
+        if (kf < 1.0) kf = 1.0;
+        if (kf > N) kf = N;
+        if (kf - x <= sx || uu >= z_H(s, kf + 0.5) - z_h(s, kf)) rank = (uint64_t)kf;
+      }
+      if (rank == 0) rank = 1;
+    }
+    if (out_ranks) out_ranks[i] = rank;
+    out_keys[i] = (int64_t)orc_key62(seed, rank);
+  }
+}
+
 /* ------------------------------------------------------------------------------------ */
 /* Column access and NULL sentinels (gpu_impl/inc/DataFlow/Table.hpp:29-52).               */
 /* ------------------------------------------------------------------------------------ */

@@ -43,6 +43,11 @@ void orc_gen_uniform_f32(uint64_t seed, uint32_t column_id, uint64_t row_begin, 
 void orc_gen_normal_f64(uint64_t seed, uint32_t column_id, uint64_t row_begin, uint64_t n,
                         double *out);
 
+/* Skewed-join workload generators (same definitions as the device generators). */
+uint64_t orc_key62(uint64_t seed, uint64_t rank);
+void orc_gen_zipf_keys(uint64_t seed, uint32_t column_id, uint64_t row_begin, uint64_t n, uint64_t nranks,
+                       double theta, double miss_frac, int64_t *out_keys, uint64_t *out_ranks);
+
 /* HashCodeUtil.calculateLongHashCode (HashCodeUtil.java:36-45). */
 int32_t orc_hive_long_hash(int64_t key);
 

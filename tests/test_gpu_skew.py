@@ -1,0 +1,106 @@
+"""Skewed join, ORDER BY and high-cardinality GROUP BY (the 1B-row probe / 64M-row build
+workload): parity with the oracle at test sizes; at full size, size-independent properties."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import to_dev, to_np
+from ghive_amd import abi
+
+pytestmark = pytest.mark.gpu
+
+
+def test_generators_match_host(ctx, orc):
+    from ghive_amd import workloads as W
+    import torch
+    seed = 0x1234
+    k = to_np(ctx.gen_rank_keys(seed, 100_000, rank_begin=1)).astype(np.uint64)
+    assert all(int(k[i]) == orc.key62(seed, i + 1) for i in range(0, 100_000, 997))
+    ranks = torch.arange(1, 100_001, dtype=torch.int64, device="cuda")
+    assert np.array_equal(to_np(W.key62_torch(seed, ranks)).astype(np.uint64), k)
+    g = to_np(ctx.gen_zipf_keys(seed, 2, 500_000, 1_000_000, 1.0, 0.1, row_begin=7))
+    h, _ = orc.gen_zipf_keys(seed, 2, 7, 500_000, 1_000_000, 1.0, 0.1)
+    # device and host libm may round a rank boundary differently in rare cases
+    assert (g != h).mean() < 1e-4
+
+
+def test_skewed_join_vs_oracle_then_sort(ctx, orc):
+    from ghive_amd import ops, workloads as W
+    import torch
+    S = W.skew_tables(ctx, build_rows=400_000, probe_rows=6_000_000, seed=11)
+    ht = ops.HashTable(ctx, S.build_keys, payload=S.build_pay, load_factor=0.5)
+    rows, pay = ht.probe(S.probe_keys)
+    got = sorted(zip(to_np(rows).astype(np.int64).tolist(), to_np(pay).tolist()))
+    bk, bp, pk = to_np(S.build_keys), to_np(S.build_pay), to_np(S.probe_keys)
+    want_rows, want_pay = orc.mapjoin_inner(bk, pk, payload=bp)
+    assert got == sorted(zip(want_rows.tolist(), want_pay.tolist()))
+    # ORDER BY key ASC, tiebreak ASC over the joined rows
+    r64 = rows.to(torch.int64) & 0xFFFFFFFF
+    jk, jt = S.probe_keys[r64].contiguous(), S.tiebreak[r64].contiguous()
+    perm = to_np(ops.sort(ctx, ops.Table([jk, jt]), [0, 1], ["+", "+"])).astype(np.uint32)
+    assert np.array_equal(perm, orc.sort_perm([to_np(jk), to_np(jt)], [0, 1], ["+", "+"]))
+    ht.free()
+
+
+def test_high_cardinality_double_sum_vs_exact(ctx, orc):
+    from ghive_amd import ops, workloads as W
+    keys, vals = W.highcard_tables(ctx, rows=4_000_000, groups=1_000_000, seed=21)
+    outs = ops.GroupByPredicate([0], [(abi.AGG_CNT, abi.COUNT_STAR), (abi.AGG_SUM, 1)]).gpu_execute(
+        ctx, ops.Table([keys, vals]), capacity=1_100_000)
+    k, c, s = (to_np(o) for o in outs)
+    rep, v, nulls = orc.groupby([to_np(keys), to_np(vals)], [0], [(abi.AGG_CNT, abi.COUNT_STAR), (abi.AGG_SUM, 1)])
+    hk = to_np(keys)[rep]
+    want = {int(a): (int(b), float(d)) for a, b, d in zip(hk, v[0], v[1])}
+    assert len(k) == len(want)
+    worst = 0.0
+    for a, b, d in zip(k.tolist(), c.tolist(), s.tolist()):
+        wc, ws = want[a]
+        assert b == wc
+        if ws != 0.0:
+            worst = max(worst, abs(d - ws) / abs(ws))
+        else:
+            assert abs(d) < 1e-300
+    assert worst <= 1e-12
+
+
+def test_skew_full_size_properties(ctx):
+    """1B Zipf probe rows x 64M unique build keys: every match is a build key with its own rank as
+    payload, the match count equals the non-miss count, and the ORDER BY output is sorted."""
+    import torch
+    from ghive_amd import ops, workloads as W
+    S = W.skew_tables(ctx)
+    ht = ops.HashTable(ctx, S.build_keys, payload=S.build_pay, load_factor=0.5)
+    rows, pay = ht.probe(S.probe_keys, capacity=1_000_000_000)
+    n = S.probe_keys.numel()
+    # a probe key is a build key iff it is the image of a rank in [1, 64M]; misses use larger ranks
+    assert 0.88 * n < rows.numel() < 0.92 * n
+    r64 = rows.to(torch.int64) & 0xFFFFFFFF
+    step = 100_000_000
+    for a in range(0, rows.numel(), step):
+        pk = S.probe_keys[r64[a:a + step]]
+        assert bool((W.key62_torch(S.seed, pay[a:a + step]) == pk).all().item())
+    ht.free()
+    jk = S.probe_keys[r64].contiguous()
+    jt = S.tiebreak[r64].contiguous()
+    del rows, pay, r64
+    perm = ops.sort(ctx, ops.Table([jk, jt]), [0, 1], ["+", "+"]).to(torch.int64) & 0xFFFFFFFF
+    for a in range(0, perm.numel() - 1, step):
+        p = perm[a:a + step + 1]
+        ks, ts = jk[p], jt[p]
+        assert bool(((ks[1:] > ks[:-1]) | ((ks[1:] == ks[:-1]) & (ts[1:] >= ts[:-1]))).all().item())
+
+
+def test_highcard_full_size_properties(ctx):
+    """1B rows, 100M distinct keys: group count = distinct keys, Σ COUNT = rows, Σ SUM equals
+    the sum of all values within 1e-9 relative (the per-group sums are exact)."""
+    import torch
+    from ghive_amd import ops, workloads as W
+    keys, vals = W.highcard_tables(ctx)
+    outs = ops.GroupByPredicate([0], [(abi.AGG_CNT, abi.COUNT_STAR), (abi.AGG_SUM, 1)]).gpu_execute(
+        ctx, ops.Table([keys, vals]), capacity=110_000_000)
+    assert int(outs[1].sum().item()) == keys.numel()
+    ngroups = outs[0].numel()
+    assert 99_000_000 < ngroups <= 100_000_000
+    total = float(vals.sum().item())
+    assert abs(float(outs[2].sum().item()) - total) <= 1e-9 * float(vals.abs().sum().item())

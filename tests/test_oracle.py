@@ -337,6 +337,39 @@ def test_sort_total_order(orc):
     assert np.array_equal(perm, want)
 
 
+# ---------------------------------------------------------------- skewed-join generator ------
+def test_key62_is_injective_and_below_null(orc):
+    ranks = np.arange(1, 200001, dtype=np.uint64)
+    keys = np.array([orc.key62(99, int(r)) for r in ranks[:20000]], dtype=np.uint64)
+    assert len(np.unique(keys)) == len(keys)
+    assert keys.max() < 2**62
+
+
+@pytest.mark.parametrize("theta", [1.0, 0.5, 1.5])
+def test_zipf_rejection_inversion_matches_pmf(orc, theta):
+    n, m = 1000, 400_000
+    keys, ranks = orc.gen_zipf_keys(5, 3, 0, m, n, theta, 0.0)
+    assert ranks.min() >= 1 and ranks.max() <= n
+    pmf = 1.0 / np.arange(1, n + 1) ** theta
+    pmf /= pmf.sum()
+    counts = np.bincount(ranks.astype(np.int64), minlength=n + 1)[1:]
+    for k in (1, 2, 3, 10, 100):  # frequent ranks within 5 standard errors
+        se = np.sqrt(m * pmf[k - 1] * (1 - pmf[k - 1]))
+        assert abs(counts[k - 1] - m * pmf[k - 1]) < 5 * se + 2, (k, counts[k - 1], m * pmf[k - 1])
+    # keys are the rank bijection
+    assert all(int(keys[i]) == orc.key62(5, int(ranks[i])) for i in range(0, m, 9973))
+
+
+def test_zipf_misses_are_outside_the_build_ranks(orc):
+    n, m = 10_000, 200_000
+    keys, ranks = orc.gen_zipf_keys(8, 1, 0, m, n, 1.0, 0.1)
+    miss = ranks > n
+    assert abs(miss.mean() - 0.1) < 0.005
+    build = {orc.key62(8, r) for r in range(1, n + 1)}
+    assert not any(int(k) in build for k in keys[miss][:2000])
+    assert all(int(k) in build for k in keys[~miss][:2000])
+
+
 # ---------------------------------------------------------------- fragment -------------------
 def c1_tables(orc, fact_rows, seed=0x6768697666):
     """C1 shape: fact(fk, measure, price), dim(key, attr, category)."""
