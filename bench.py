@@ -124,7 +124,7 @@ def main():
     ap.add_argument("--no-dense", action="store_true", help="probe every dimension through its hash table")
     ap.add_argument("--no-lds-bits", action="store_true", help="keep dimension bitmaps in HBM/L2")
     ap.add_argument("--rows-per-thread", type=int, default=0, help="fused kernel rows/thread (4 or 8)")
-    ap.add_argument("--nontemporal", type=int, default=-1, help="non-temporal fact loads (0/1)")
+    ap.add_argument("--generic", action="store_true", help="general fused kernel (no dense-probe form)")
     ap.add_argument("--grid", type=int, default=0, help="fused kernel workgroups (0 = resident wave)")
     ap.add_argument("--block", type=int, default=0, help="fused kernel workgroup size (256/512/1024)")
     ap.add_argument("--cpu-sample", type=int, default=600_000_000)
@@ -134,7 +134,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the fragment shards "
-              "by fact row range with replicated dimension tables, see DESIGN.md", file=sys.stderr)
+              "by fact row range with replicated dimension tables", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -149,14 +149,15 @@ def main():
         ctx.set_option("fragment.lds_bits", 0)
     if args.rows_per_thread:
         ctx.set_option("fragment.rows_per_thread", args.rows_per_thread)
-    if args.nontemporal >= 0:
-        ctx.set_option("fragment.nontemporal", args.nontemporal)
     if args.grid:
         ctx.set_option("fragment.grid", args.grid)
     if args.block:
         ctx.set_option("fragment.block", args.block)
+    if args.generic:
+        ctx.set_option("fragment.generic", 1)
     variant = ("late" if args.late else "early") + ("/hash" if args.no_dense else
                                                     ("/hbm-bitmap" if args.no_lds_bits else "/lds-bitmap"))
+    variant += "/generic-kernel" if (args.generic or args.late or args.no_dense) else "/dense-kernel"
     info = ctx.device_info()
     log(f"device {info['name']} {info['arch']} CUs={info['compute_units']}")
     t0 = time.time()

@@ -95,5 +95,11 @@ SIGNATURES_EXTRA = {
     "ghive_bench_stream": (C.c_int, [P, P, P, u64, C.c_int, C.c_int, C.POINTER(C.c_float)]),
     "ghive_gen_rank_keys": (C.c_int, [P, u64, u64, u64, P]),
     "ghive_gen_zipf_keys": (C.c_int, [P, u64, u32, u64, u64, u64, C.c_double, C.c_double, P]),
+    "ghive_server_create": (C.c_int, [C.c_int, C.POINTER(abi.ServerConfig), C.POINTER(P)]),
+    "ghive_server_submit": (C.c_int, [P, C.POINTER(abi.Request), PU64]),
+    "ghive_server_wait": (C.c_int, [P, u64, C.POINTER(abi.Response)]),
+    "ghive_server_get_stats": (C.c_int, [P, C.POINTER(abi.ServerStats)]),
+    "ghive_server_slot_context": (P, [P, u32]),
+    "ghive_server_destroy": (None, [P]),
 }
 SIGNATURES.update(SIGNATURES_EXTRA)

@@ -169,3 +169,32 @@ class BatchSpec(C.Structure):
 class BatchTuning(C.Structure):
     _fields_ = [("tile_rows", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("probe_mode", C.c_int32),
                 ("reserved", C.c_int32)]
+
+
+# ---- serving form (ghive_server_*) ----------------------------------------------------------
+REQ_FRAGMENT, REQ_GROUPBY, REQ_SORT = 1, 2, 3
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("slots", C.c_uint32), ("reserved", C.c_uint32), ("hbm_budget_bytes", C.c_uint64),
+                ("slot_mem_limit_bytes", C.c_uint64)]
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("host_io", C.c_int32), ("fragment", C.POINTER(Fragment)),
+                ("table", C.POINTER(Table)), ("keys", C.POINTER(C.c_uint32)), ("orders", C.c_char_p),
+                ("nkeys", C.c_uint32), ("naggs", C.c_uint32), ("aggs", C.POINTER(Agg)),
+                ("out_cols", C.POINTER(Column)), ("capacity", C.c_uint64), ("out_perm", C.c_void_p),
+                ("footprint_bytes", C.c_uint64)]
+
+
+class Response(C.Structure):
+    _fields_ = [("status", C.c_int32), ("slot", C.c_uint32), ("groups", C.c_uint64),
+                ("submit_us", C.c_double), ("start_us", C.c_double), ("end_us", C.c_double),
+                ("error", C.c_char * 256)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("completed", C.c_uint64), ("rejected", C.c_uint64), ("queued", C.c_uint64),
+                ("inflight", C.c_uint32), ("peak_inflight", C.c_uint32), ("inflight_bytes", C.c_uint64),
+                ("peak_inflight_bytes", C.c_uint64)]

@@ -1,4 +1,4 @@
-"""Builds libghive_mi355x.so in-tree: hipcc, gfx950 only, one object per .hip file.
+"""Builds libghive_mi355x.so in-tree: hipcc, gfx950 only, one object per .hip / .cpp file.
 
     python -m ghive_amd.build          # rebuild everything
 """
@@ -18,7 +18,7 @@ ARCH = "gfx950"
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
     "-Wno-unused-command-line-argument",
-    # packed FP32 instructions stay off ("Build")
+    # packed FP32 instructions stay off: some of their forms have given wrong results on gfx950
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(HERE, "..", "include"),
 ]
@@ -37,7 +37,7 @@ def _compile(src):
 
 def build(verbose=False, jobs=None):
     os.makedirs(OBJ, exist_ok=True)
-    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     jobs = jobs or min(8, os.cpu_count() or 4)
     objs = []
     with cf.ThreadPoolExecutor(jobs) as ex:

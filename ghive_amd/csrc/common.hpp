@@ -57,8 +57,8 @@ struct ghive_tuning {
   int64_t frag_lds_bits = 1;            // 1: copy small dimension bitmaps into LDS
   int64_t frag_dense = 1;               // 1: probe dense dimensions through bitmap + payload array
   int64_t frag_rows = 4;                // fact rows per thread per tile (4 or 8)
-  int64_t frag_nt = 0;                  // 1: non-temporal loads of the fact columns
   int64_t frag_block = 256;             // fused-kernel workgroup size (256, 512 or 1024)
+  int64_t frag_generic = 0;             // 1: always the general fused kernel (no dense-probe form)
   int64_t gb_lds_budget = 64 * 1024;    // bytes of LDS for the group-by's private table
 };
 

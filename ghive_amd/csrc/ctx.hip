@@ -301,8 +301,8 @@ int ghive_set_option(ghive_ctx *ctx, const char *name, int64_t value) {
   } else if (k == "fragment.rows_per_thread") {
     if (value != 4 && value != 8) return set_error(ctx, GHIVE_ERR_INVALID, "option %s: %lld (4 or 8)", name, (long long)value);
     ctx->tune.frag_rows = value;
-  } else if (k == "fragment.nontemporal") {
-    ctx->tune.frag_nt = value ? 1 : 0;
+  } else if (k == "fragment.generic") {
+    ctx->tune.frag_generic = value ? 1 : 0;
   } else if (k == "fragment.block") {
     if (value != 256 && value != 512 && value != 1024)
       return set_error(ctx, GHIVE_ERR_INVALID, "option %s: %lld (256, 512 or 1024)", name, (long long)value);
@@ -325,8 +325,8 @@ int ghive_get_option(ghive_ctx *ctx, const char *name, int64_t *value) {
   else if (k == "fragment.lds_bits") *value = ctx->tune.frag_lds_bits;
   else if (k == "fragment.dense") *value = ctx->tune.frag_dense;
   else if (k == "fragment.rows_per_thread") *value = ctx->tune.frag_rows;
-  else if (k == "fragment.nontemporal") *value = ctx->tune.frag_nt;
   else if (k == "fragment.block") *value = ctx->tune.frag_block;
+  else if (k == "fragment.generic") *value = ctx->tune.frag_generic;
   else if (k == "groupby.lds_budget") *value = ctx->tune.gb_lds_budget;
   else return set_error(ctx, GHIVE_ERR_INVALID, "unknown option '%s'", name);
   return GHIVE_OK;

@@ -71,7 +71,7 @@ int fragment_plan(ghive_ctx *ctx, const ghive_fragment *f, int late, FragPlan &P
     if (need_payload[j] && h->payload_kind == 0)
       return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "fragment: join %u carries no attribute", j);
     GH_TRY(int_col(f->joins[j].fact_key_col, "join key"));
-    if (h->bitmap && ctx->tune.frag_dense) {
+    if (h->bitmap && ctx->tune.frag_dense && h->entries) {  // an empty table is probed as a hash table
       mode[j] = kJoinGlobalBits;
       by_size.push_back(j);
     }
@@ -112,6 +112,9 @@ int fragment_plan(ghive_ctx *ctx, const ghive_fragment *f, int late, FragPlan &P
     J.mode = mode[j];
     J.key_min = h->entries ? (int32_t)h->key_min : 1;  // an empty table rejects every key
     J.key_max = h->entries ? (int32_t)h->key_max : 0;
+    // NULL (INT_MAX) is never inserted, so key_max < INT_MAX and the unsigned test below is exact;
+    // an empty table gets span 2^32-1 and rejects every key in its (empty) hash probe
+    J.span = (uint32_t)J.key_max - (uint32_t)J.key_min;
     J.nwords = h->bitmap ? (uint32_t)((h->range + 31) / 32) : 0;
     J.need_payload = need_payload[j];
     if (J.mode == kJoinLdsBits) {
@@ -176,6 +179,12 @@ int fragment_plan(ghive_ctx *ctx, const ghive_fragment *f, int late, FragPlan &P
     word += agg_words(A.kind);
   }
   T.nwords = word;
+  // unused aggregate inputs read a column the tile loads anyway, so that the dense-probe kernel
+  // issues every load of a tile unconditionally (the values are ignored)
+  for (int a = 0; a < F.naggs && F.njoins > 0; ++a) {
+    if (!F.aggs[a].a) F.aggs[a].a = F.joins[0].fk;
+    if (!F.aggs[a].b) F.aggs[a].b = F.joins[0].fk;
+  }
   if (f->filter_nnodes) {
     GH_TRY(flatten_pred(ctx, &fact, f->filter, f->filter_nnodes, f->filter_root, F.filter));
     F.has_filter = 1;
@@ -195,11 +204,18 @@ int fragment_plan(ghive_ctx *ctx, const ghive_fragment *f, int late, FragPlan &P
   P.lds_bytes = bitmap_bytes + (P.lcap_log2 >= 0 ? (size_t)((1 << P.lcap_log2) + 1) * (1 + T.nwords) * 8 : 0);
   P.late = late;
   F.late = late ? 1 : 0;
-  F.nt = ctx->tune.frag_nt ? 1 : 0;
   const int rows = ctx->tune.frag_rows == 8 ? 8 : 4;
-  FragKernel kfn = rows == 8 ? pick_kernel_r8(F.njoins, F.naggs) : pick_kernel_r4(F.njoins, F.naggs);
+  // the lean dense-probe kernel when the fragment has its shape (no fact filter, early reads, every
+  // join through a bitmap, group keys from join attributes)
+  bool dense_form = !ctx->tune.frag_generic && !F.has_filter && !late && F.njoins >= 1;
+  for (int j = 0; j < F.njoins; ++j) dense_form = dense_form && F.joins[j].mode != kJoinHash;
+  for (int k = 0; k < F.nkeys; ++k) dense_form = dense_form && F.key_src[k] == GHIVE_SRC_JOIN;
+  P.dense_form = dense_form;
+  FragKernel kfn = dense_form ? (rows == 8 ? pick_dense_r8(F.njoins, F.naggs) : pick_dense_r4(F.njoins, F.naggs))
+                              : (rows == 8 ? pick_kernel_r8(F.njoins, F.naggs) : pick_kernel_r4(F.njoins, F.naggs));
   P.kernel = reinterpret_cast<const void *>(kfn);
   P.threads = (uint32_t)ctx->tune.frag_block;
+  P.rows = rows;
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, (int)P.threads, P.lds_bytes) != hipSuccess || per_cu < 1) {
     (void)hipGetLastError();
@@ -232,8 +248,13 @@ int fragment_run(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_c
   for (uint32_t c = 0; c < frag->nkeys + frag->naggs && capacity; ++c)
     if (!out_cols[c].data) return set_error(ctx, GHIVE_ERR_INVALID, "fragment: output column %u has no buffer", c);
   uint64_t table_groups = capacity < P.group_bound ? capacity : P.group_bound;
-  DevBuf dkeys, dacc, dcnt, djoined;
+  DevBuf dkeys, dacc, dcnt, djoined, dstage;
   GH_TRY(group_table_alloc(ctx, P.T, table_groups ? table_groups : 1, dkeys, dacc, dcnt));
+  const uint64_t tile_rows = (uint64_t)P.threads * (uint64_t)P.rows;
+  if (P.F.n % tile_rows) {
+    GH_TRY(dstage.alloc(ctx, (uint64_t)(P.F.njoins + 2 * P.F.naggs) * tile_rows * sizeof(int32_t)));
+    P.F.stage = dstage.as<int32_t>();
+  }
   P.T.capacity = capacity;
   GH_TRY(djoined.alloc(ctx, sizeof(unsigned long long)));
   GH_TRY(fill_u64(ctx, djoined.as<uint64_t>(), 0, 1));

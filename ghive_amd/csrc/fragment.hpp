@@ -30,7 +30,7 @@ struct FragJoinDev {
   uint32_t lds_word_off;     // kJoinLdsBits: first uint32 word of this bitmap in LDS
   uint32_t nwords;           // bitmap words
   int32_t need_payload;      // payload used as a group key
-  int32_t pad;
+  uint32_t span;             // key_max - key_min: key k is in range iff (uint32)(k - key_min) <= span
 };
 
 struct FragAggDev {
@@ -47,11 +47,14 @@ struct FragDev {
   int32_t naggs;
   int32_t has_filter;
   int32_t late;  // read a column only for lanes that still hold a row
-  int32_t nt;    // non-temporal loads of the streamed fact columns
+  int32_t pad;
   int8_t key_src[2];
   int8_t key_idx[2];
   uint32_t lds_bitmap_words;  // words of LDS bitmaps (even, so the group table is 8-aligned)
   const int32_t *key_col[2];
+  // NULL-padded copy of the partial last tile: one tile (blockDim.x * R rows) per streamed column,
+  // joins first (slot j), then aggregate inputs (slots NJ + 2a and NJ + 2a + 1)
+  int32_t *stage;
   FragJoinDev joins[kMaxFragJoins];
   FragAggDev aggs[kMaxFragAggs];
   DevPred filter;
@@ -67,6 +70,8 @@ struct FragPlan {
   size_t lds_bytes;
   uint32_t grid;
   uint32_t threads;  // workgroup size
+  int rows;          // fact rows per lane per tile
+  bool dense_form;   // k_fragment_dense (else the general k_fragment)
   int late;
   const void *kernel;
 };

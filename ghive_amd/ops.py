@@ -120,7 +120,8 @@ def _p(t):
 class Table:
     """Table of device columns (gpu_impl/inc/DataFlow/Table.hpp)."""
 
-    def __init__(self, columns, key_num=0):
+    def __init__(self, columns, key_num=0, host=False):
+        """host=True: columns in host memory, for serving requests with host I/O."""
         self.columns = [c.contiguous() for c in columns]
         n = len(self.columns)
         rows = self.columns[0].numel() if n else 0
@@ -129,8 +130,9 @@ class Table:
                 raise ValueError("ragged table")
             if c.dtype not in _TYPE_OF_DTYPE:
                 raise TypeError(f"unsupported column dtype {c.dtype}")
-            if not c.is_cuda:
-                raise ValueError("columns must be on the GPU")
+            if c.is_cuda == host:
+                raise ValueError("columns must be in host memory" if host else "columns must be on the GPU")
+        self.host = host
         self.row_num = rows
         self.cols = (abi.Column * max(n, 1))()
         for i, c in enumerate(self.columns):
@@ -431,3 +433,111 @@ def bench_stream(ctx, nbytes, mode=0, iters=10):
     ctx.check(lib().ghive_bench_stream(ctx.h, _p(dst), _p(src), nbytes, mode, iters, C.byref(ms)))
     moved = nbytes * (2 if mode == 1 else 1)
     return moved / (ms.value * 1e-3) / 1e9, ms.value
+
+
+class Ticket:
+    """A submitted serving request: keeps its descriptors and output buffers alive until waited."""
+
+    def __init__(self, rid, kind, outs, keep):
+        self.id, self.kind, self.outs, self._keep = rid, kind, outs, keep
+        self.response = None
+
+
+class Server:
+    """Serving form (ghive_server_*): a request loop with `slots` concurrent requests on one
+    device, admission by HBM footprint (inputs + intermediates) against `hbm_budget` bytes, and
+    resident dimension hash tables (HashTable objects built by the caller and shared by every
+    request). Inputs of device requests must be complete when submitted (the slots run on their
+    own streams); host_io requests take host tables and return host tensors."""
+
+    def __init__(self, device=0, slots=4, hbm_budget=64 << 30, slot_mem_limit=16 << 30):
+        cfg = abi.ServerConfig(slots, 0, hbm_budget, slot_mem_limit)
+        h = C.c_void_p()
+        st = lib().ghive_server_create(device, C.byref(cfg), C.byref(h))
+        if st != abi.OK:
+            raise GhiveError(st, f"ghive_server_create failed with status {st}")
+        self.h, self.device, self.slots = h, device, slots
+
+    def set_option(self, name, value):
+        for i in range(self.slots):
+            c = lib().ghive_server_slot_context(self.h, i)
+            st = lib().ghive_set_option(c, name.encode(), value)
+            if st != abi.OK:
+                raise GhiveError(st, lib().ghive_last_error(c).decode())
+
+    def _submit(self, req, kind, outs, keep):
+        if not req.host_io and torch.cuda.is_available():
+            torch.cuda.current_stream(self.device).synchronize()
+        rid = C.c_uint64()
+        st = lib().ghive_server_submit(self.h, C.byref(req), C.byref(rid))
+        if st != abi.OK:
+            raise GhiveError(st, "request rejected: its HBM footprint exceeds the admission budget"
+                             if st == abi.ERR_ADMISSION else f"submit failed with status {st}")
+        return Ticket(rid.value, kind, outs, keep + [req])
+
+    def _outs(self, dtypes, rows, host):
+        if host:
+            return [torch.empty(max(rows, 1), dtype=d, pin_memory=True) for d in dtypes]
+        return [torch.empty(max(rows, 1), dtype=d, device=f"cuda:{self.device}") for d in dtypes]
+
+    def submit_fragment(self, frag, capacity=4096, footprint=0):
+        host = getattr(frag.fact, "host", False)
+        outs = self._outs([torch.int32] * frag.nkeys + [torch.int64] * frag.naggs, capacity, host)
+        ocols = (abi.Column * max(len(outs), 1))()
+        for i, o in enumerate(outs):
+            ocols[i].type = _TYPE_OF_DTYPE[o.dtype]
+            ocols[i].data = o.data_ptr()
+        req = abi.Request(kind=abi.REQ_FRAGMENT, host_io=int(host), fragment=C.pointer(frag.f),
+                          out_cols=ocols, capacity=capacity, footprint_bytes=footprint)
+        return self._submit(req, abi.REQ_FRAGMENT, outs, [frag, ocols])
+
+    def submit_groupby(self, table, keys, aggregation_descs, capacity, footprint=0):
+        gb = GroupByPredicate(keys, aggregation_descs)
+        outs = self._outs([_DTYPE_OF_TYPE[t] for t in gb.output_types(table)], capacity, table.host)
+        ocols = (abi.Column * max(len(outs), 1))()
+        for i, o in enumerate(outs):
+            ocols[i].type = _TYPE_OF_DTYPE[o.dtype]
+            ocols[i].data = o.data_ptr()
+        karr = (C.c_uint32 * max(len(gb.keys), 1))(*gb.keys)
+        aarr = (abi.Agg * max(len(gb.aggregation_descs), 1))(*[abi.Agg(*a) for a in gb.aggregation_descs])
+        req = abi.Request(kind=abi.REQ_GROUPBY, host_io=int(table.host), table=C.pointer(table.table),
+                          keys=karr, nkeys=len(gb.keys), naggs=len(gb.aggregation_descs), aggs=aarr,
+                          out_cols=ocols, capacity=capacity, footprint_bytes=footprint)
+        return self._submit(req, abi.REQ_GROUPBY, outs, [table, ocols, karr, aarr])
+
+    def submit_sort(self, table, key_cols, orders, footprint=0):
+        (perm,) = self._outs([torch.int32], table.row_num, table.host)
+        karr = (C.c_uint32 * max(len(key_cols), 1))(*key_cols)
+        ords = "".join(orders).encode()
+        req = abi.Request(kind=abi.REQ_SORT, host_io=int(table.host), table=C.pointer(table.table),
+                          keys=karr, orders=ords, nkeys=len(key_cols), out_perm=perm.data_ptr(),
+                          footprint_bytes=footprint)
+        return self._submit(req, abi.REQ_SORT, [perm], [table, karr, ords])
+
+    def wait(self, ticket):
+        """Blocks until the request has finished; returns its outputs (group-by / fragment:
+        key and aggregate columns cut to the group count; sort: the permutation)."""
+        r = abi.Response()
+        st = lib().ghive_server_wait(self.h, ticket.id, C.byref(r))
+        if st != abi.OK:
+            raise GhiveError(st, f"wait failed for request {ticket.id}")
+        ticket.response = r
+        if r.status != abi.OK:
+            raise GhiveError(r.status, r.error.decode(errors="replace"))
+        return [o[: r.groups] for o in ticket.outs]
+
+    def stats(self):
+        s = abi.ServerStats()
+        lib().ghive_server_get_stats(self.h, C.byref(s))
+        return {f: getattr(s, f) for f, _ in abi.ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            lib().ghive_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -391,9 +391,72 @@ int ghive_fragment_execute_late(ghive_ctx *ctx, const ghive_fragment *frag, ghiv
 /* Tunables of a context: "fragment.lds_budget" (bytes, <= 160 KiB), "fragment.late" (0/1),
  * "fragment.grid" (workgroups, 0 = one resident wave), "fragment.lds_bits" (0/1: LDS copies of
  * small dimension bitmaps), "fragment.dense" (0/1: probe compact-key dimensions through their
- * bitmap + payload array instead of the hash table), "groupby.lds_budget" (bytes). */
+ * bitmap + payload array instead of the hash table), "fragment.rows_per_thread" (4 or 8),
+ * "fragment.block" (workgroup size 256, 512 or 1024), "fragment.generic" (0/1: always the general
+ * fused kernel, not its dense-probe form), "groupby.lds_budget" (bytes). */
 int ghive_set_option(ghive_ctx *ctx, const char *name, int64_t value);
 int ghive_get_option(ghive_ctx *ctx, const char *name, int64_t *value);
+
+/* ---- serving form: request loop with HBM-footprint admission -------------------------- *
+ * No counterpart in the reference (one Tez task per JNI call, GPUProcess.java:20-143). A
+ * server owns `slots` contexts (each with its own stream and memory manager) on one device and
+ * a dispatcher thread. Requests are admitted in arrival order as soon as a slot is free and
+ * their HBM footprint fits hbm_budget_bytes beside the requests in flight; admitted requests
+ * run concurrently and overlap on the device. Dimension hash tables referenced by fragments
+ * stay resident (built once by the caller, shared read-only by all slots). */
+enum { GHIVE_REQ_FRAGMENT = 1, GHIVE_REQ_GROUPBY = 2, GHIVE_REQ_SORT = 3 };
+typedef struct ghive_server ghive_server;
+typedef struct ghive_server_config {
+  uint32_t slots;                /* concurrent requests on the device, 1..16            */
+  uint32_t reserved;
+  uint64_t hbm_budget_bytes;     /* admission budget over all requests in flight        */
+  uint64_t slot_mem_limit_bytes; /* memory-manager limit of each slot (0 = 90% of free) */
+} ghive_server_config;
+/* One request. FRAGMENT: fragment + out_cols/capacity (as ghive_fragment_execute); GROUPBY:
+ * table + keys/aggs + out_cols/capacity (as ghive_groupby over all rows); SORT: table +
+ * keys/orders + out_perm (as ghive_sort). host_io = 1: the fact table / table columns and the
+ * outputs are host memory; the slot uploads the inputs after admission and downloads the
+ * outputs (pinned host memory lets transfers of different slots overlap). host_io = 0: all
+ * device memory. footprint_bytes = 0 lets the server estimate inputs + intermediates. All
+ * pointed-to descriptors must stay valid until ghive_server_wait returns for the request. */
+typedef struct ghive_request {
+  int32_t kind;
+  int32_t host_io;
+  const ghive_fragment *fragment;
+  const ghive_table *table;
+  const uint32_t *keys;
+  const char *orders;
+  uint32_t nkeys;
+  uint32_t naggs;
+  const ghive_agg *aggs;
+  ghive_column *out_cols;
+  uint64_t capacity;
+  uint32_t *out_perm;
+  uint64_t footprint_bytes;
+} ghive_request;
+typedef struct ghive_response {
+  int32_t status; /* the operator's status code */
+  uint32_t slot;
+  uint64_t groups; /* groups written (FRAGMENT/GROUPBY) or rows sorted */
+  double submit_us, start_us, end_us; /* host steady clock; latency = end - submit */
+  char error[256];
+} ghive_response;
+typedef struct ghive_server_stats {
+  uint64_t completed, rejected, queued;
+  uint32_t inflight, peak_inflight;
+  uint64_t inflight_bytes, peak_inflight_bytes;
+} ghive_server_stats;
+int ghive_server_create(int device, const ghive_server_config *cfg, ghive_server **out);
+/* Queues a request (thread-safe). GHIVE_ERR_ADMISSION when its footprint exceeds the whole
+ * budget, so it could never be admitted. */
+int ghive_server_submit(ghive_server *srv, const ghive_request *req, uint64_t *id);
+/* Blocks until request `id` has finished; each id can be waited for once. */
+int ghive_server_wait(ghive_server *srv, uint64_t id, ghive_response *out);
+int ghive_server_get_stats(ghive_server *srv, ghive_server_stats *out);
+/* Context of a slot (to set options on it before submitting work). */
+ghive_ctx *ghive_server_slot_context(ghive_server *srv, uint32_t slot);
+/* Finishes the requests already admitted, drops queued ones, releases the slots. */
+void ghive_server_destroy(ghive_server *srv);
 
 /* Per-stage timing of the last fragment on this context (HIP events, milliseconds). */
 typedef struct ghive_stage_times {

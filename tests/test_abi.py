@@ -49,6 +49,10 @@ def test_struct_layouts_match_header(tmp_path):
         "ghive_device_info": abi.DeviceInfo, "ghive_stage_times": abi.StageTimes,
         "ghive_batch_item": abi.BatchItem, "ghive_batch_spec": abi.BatchSpec,
         "ghive_batch_tuning": abi.BatchTuning,
+        "ghive_server_config": abi.ServerConfig,
+        "ghive_request": abi.Request,
+        "ghive_response": abi.Response,
+        "ghive_server_stats": abi.ServerStats,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ghive/ghive.h"', "int main(void){"]
     for cname, py in structs.items():

@@ -73,26 +73,85 @@ def _star_oracle(orc, T, rows=None):
     return spec.run(nthreads=8, capacity=1 << 14)
 
 
-@pytest.mark.parametrize("dense,lds_bits", [(1, 1), (1, 0), (0, 0)])
-def test_star_fragment_vs_oracle_20m(ctx, orc, dense, lds_bits):
-    """Every probe mode (LDS bitmap, HBM bitmap, hash table) and both load orders agree with the
-    oracle bit for bit."""
+_DEFAULTS = {"fragment.dense": 1, "fragment.lds_bits": 1, "fragment.generic": 0,
+             "fragment.rows_per_thread": 4, "fragment.block": 256}
+
+
+@pytest.mark.parametrize("opts", [
+    {},                                                        # dense-probe kernel, LDS bitmaps
+    {"fragment.lds_bits": 0},                                  # dense-probe kernel, HBM bitmaps
+    {"fragment.generic": 1},                                   # general kernel, LDS bitmaps
+    {"fragment.generic": 1, "fragment.lds_bits": 0},           # general kernel, HBM bitmaps
+    {"fragment.dense": 0},                                     # hash-table probes
+    {"fragment.rows_per_thread": 8, "fragment.block": 512},
+    {"fragment.block": 1024},
+], ids=["dense-lds", "dense-hbm", "generic-lds", "generic-hbm", "hash", "r8-b512", "b1024"])
+def test_star_fragment_vs_oracle_20m(ctx, orc, opts):
+    """Both fused kernels, every probe mode (LDS bitmap, HBM bitmap, hash table), both load orders
+    and several tile shapes agree with the oracle bit for bit; 20M+1 rows leaves a partial tile."""
     from ghive_amd import workloads as W
     T = W.star_tables(ctx, 20_000_001)
     hd, hs, hp = W.star_hash_tables(ctx, T)
     frag = W.star_fragment(T.fact, hd, hs, hp)
-    ctx.set_option("fragment.dense", dense)
-    ctx.set_option("fragment.lds_bits", lds_bits)
+    for k, v in opts.items():
+        ctx.set_option(k, v)
     try:
         got = _gpu_dict(frag.execute(ctx, capacity=8192), 2)
         joined = ctx.stage_times()["rows_joined"]
         got_late = _gpu_dict(frag.execute(ctx, capacity=8192, late=True), 2)
     finally:
-        ctx.set_option("fragment.dense", 1)
-        ctx.set_option("fragment.lds_bits", 1)
+        for k in opts:
+            ctx.set_option(k, _DEFAULTS[k])
     want = _star_oracle(orc, T)
     assert got == _orc_dict(want) == got_late
     assert joined == want["joined"]
+
+
+@pytest.mark.parametrize("rows", [1, 3, 1023, 1024, 1025, 4096, 262_144, 262_147])
+def test_star_fragment_tile_boundaries(ctx, orc, rows):
+    """Tables smaller than, equal to and just past whole tiles (1024 rows at the default shape):
+    the partial last tile goes through the staging copy, whole tiles do not."""
+    from ghive_amd import ops, workloads as W
+    T = W.star_tables(ctx, 300_000)
+    hd, hs, hp = W.star_hash_tables(ctx, T)
+    part = ops.Table([c[:rows] for c in T.fact.columns])
+    got = _gpu_dict(W.star_fragment(part, hd, hs, hp).execute(ctx, capacity=8192), 2)
+    ctx.set_option("fragment.generic", 1)
+    try:
+        generic = _gpu_dict(W.star_fragment(part, hd, hs, hp).execute(ctx, capacity=8192), 2)
+    finally:
+        ctx.set_option("fragment.generic", 0)
+    assert got == generic == _orc_dict(_star_oracle(orc, T, rows))
+
+
+@pytest.mark.parametrize("generic", [0, 1])
+def test_star_fragment_count_min_max(ctx, orc, generic):
+    """COUNT(*), SUM, MIN and MAX of single columns (unused aggregate inputs) in both kernels."""
+    from ghive_amd import ops, workloads as W
+    T = W.star_tables(ctx, 2_000_003)
+    hd, hs, hp = W.star_hash_tables(ctx, T)
+    aggs = [(abi.AGG_CNT, abi.EXPR_ONE, 0, 0), (abi.AGG_SUM, abi.EXPR_COL, 5, 0),
+            (abi.AGG_MIN, abi.EXPR_COL, 6, 0), (abi.AGG_MAX, abi.EXPR_COL, 4, 0)]
+    frag = ops.Fragment(T.fact, joins=[(W.F_DATE, hd), (W.F_SUPP, hs), (W.F_PART, hp)],
+                        keys=[(abi.SRC_JOIN, 0), (abi.SRC_JOIN, 2)], aggs=aggs)
+    ctx.set_option("fragment.generic", generic)
+    try:
+        got = _gpu_dict(frag.execute(ctx, capacity=8192), 2)
+    finally:
+        ctx.set_option("fragment.generic", 0)
+    fact = [to_np(c) for c in T.fact.columns]
+    d, s_, p = ([to_np(c) for c in t.columns] for t in (T.date, T.supplier, T.part))
+    dims = [
+        dict(columns=d, key_col=0, payload_col=1, fact_key_col=0,
+             filter=([abi.make_pred(abi.FILTER_EQ_RANGE, 1, [1992, 1993], data_type=2)], 0)),
+        dict(columns=s_, key_col=0, payload_col=1, fact_key_col=2,
+             filter=([abi.make_pred(abi.FILTER_EQ, 1, [1], data_type=2)], 0)),
+        dict(columns=p, key_col=0, payload_col=2, fact_key_col=3,
+             filter=([abi.make_pred(abi.FILTER_EQ, 1, [1], data_type=2)], 0)),
+    ]
+    want = orc.FragmentSpec(fact, dims, keys=[(abi.SRC_JOIN, 0), (abi.SRC_JOIN, 2)], aggs=aggs).run(
+        nthreads=8, capacity=1 << 14)
+    assert got == _orc_dict(want)
 
 
 def test_star_fragment_600m_size_independent(ctx):
@@ -148,6 +207,9 @@ def test_fragment_edges(ctx):
                      aggs=[(abi.AGG_CNT, abi.EXPR_ONE, 0, 0), (abi.AGG_SUM, abi.EXPR_COL, 1, 0)])
     outs = f.execute(ctx, capacity=4)
     assert [int(o[0]) for o in outs] == [0, abi.LONG_NULL]
+    # an empty dimension joins nothing (probed through its empty hash table)
+    none = W.small_dim_table(ctx, T.dim, attr_value=99)
+    assert _gpu_dict(W.small_fragment(T.fact, none).execute(ctx, capacity=64), 1) == {}
     # a misaligned column is rejected, not read
     from ghive_amd._lib import GhiveError
     mis = ops.Table([c[1:] for c in T.fact.columns])
