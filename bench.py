@@ -106,6 +106,96 @@ def batched_section(ctx, reps):
     return out
 
 
+def _same_result(kind, a, b):
+    import numpy as np
+    if kind == "sort":
+        return bool(np.array_equal(a[0].cpu().numpy(), b[0].cpu().numpy()))
+    cols_a = [o.cpu().numpy() for o in a]
+    cols_b = [o.cpu().numpy() for o in b]
+    da = {int(cols_a[0][g]): tuple(int(c[g]) for c in cols_a[1:]) for g in range(len(cols_a[0]))}
+    db = {int(cols_b[0][g]): tuple(int(c[g]) for c in cols_b[1:]) for g in range(len(cols_b[0]))}
+    return da == db
+
+
+def serving_section(ctx, rates, seconds, slots):
+    """Serving form: requests of the mix (60% config-1 fragments, 25% group-bys, 15% sorts) arrive
+    as a Poisson process at each rate and go through ghive_server_*; latency = completion - submit
+    (host steady clock), p50/p99 per rate, sustained fact rows/s; the first requests are checked
+    against the plain entry points."""
+    import threading
+    import torch
+    import numpy as np
+    from ghive_amd import workloads as W
+    t0 = time.time()
+    I = W.serving_inputs(ctx)
+    torch.cuda.synchronize()
+    log(f"serving: resident inputs ready in {time.time() - t0:.1f}s")
+    srv = ops_server(slots)
+    check = W.serving_requests(36, 1, I)
+    want = [W.serving_plain(ctx, I, k, p) for k, p in check]
+    torch.cuda.synchronize()
+    tks = [W.serving_submit(srv, I, k, p)[0] for k, p in check]
+    identical = all(_same_result(k, srv.wait(t), w) for (k, _), t, w in zip(check, tks, want))
+    del want, tks
+    per_rate = []
+    for rate in rates:
+        n = max(int(rate * seconds), 1)
+        reqs = W.serving_requests(n, 1000 + rate, I)
+        gaps = np.random.default_rng(rate).exponential(1.0 / rate, n)
+        tickets = [None] * n
+        rows = [0] * n
+
+        def submitter():
+            t_next = time.perf_counter()
+            for i, (k, p) in enumerate(reqs):
+                t_next += gaps[i]
+                d = t_next - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tickets[i], rows[i] = W.serving_submit(srv, I, k, p)
+
+        th = threading.Thread(target=submitter)
+        t_start = time.perf_counter()
+        th.start()
+        lat = {"fragment": [], "groupby": [], "sort": []}
+        first_submit, last_end = None, None
+        for i in range(n):
+            while tickets[i] is None:
+                time.sleep(0.0002)
+            srv.wait(tickets[i])
+            r = tickets[i].response
+            lat[reqs[i][0]].append((r.end_us - r.submit_us) * 1e-3)
+            first_submit = r.submit_us if first_submit is None else min(first_submit, r.submit_us)
+            last_end = r.end_us if last_end is None else max(last_end, r.end_us)
+            tickets[i] = True  # drop the outputs
+        th.join()
+        wall = time.perf_counter() - t_start
+        span_s = (last_end - first_submit) * 1e-6
+        all_lat = sorted(x for v in lat.values() for x in v)
+        st = srv.stats()
+        entry = {"offered_req_per_s": rate, "requests": n, "seconds": wall,
+                 "p50_ms": float(np.percentile(all_lat, 50)), "p99_ms": float(np.percentile(all_lat, 99)),
+                 "served_req_per_s": n / span_s, "rows_per_s": sum(rows) / span_s,
+                 "p99_ms_by_kind": {k: float(np.percentile(v, 99)) for k, v in lat.items() if v},
+                 "p50_ms_by_kind": {k: float(np.percentile(v, 50)) for k, v in lat.items() if v},
+                 "peak_inflight": st["peak_inflight"], "peak_inflight_gb": st["peak_inflight_bytes"] / 1e9}
+        log(f"serving @{rate} req/s: p50 {entry['p50_ms']:.2f} ms p99 {entry['p99_ms']:.2f} ms, "
+            f"{entry['rows_per_s']:.3e} rows/s, peak in flight {st['peak_inflight']}")
+        per_rate.append(entry)
+    srv.close()
+    del I
+    torch.cuda.empty_cache()
+    return {"workload": "Poisson arrivals; 60% config-1 fragments (1k..16M rows, lognormal median 256k), "
+                        "25% group-by SUM/COUNT (1k..16M rows, <= 10M distinct int64 keys), 15% sort "
+                        "(1M..256M int64 keys, lognormal median 8M); dimension tables resident",
+            "slots": slots, "identical_to_plain": identical, "rates": per_rate}
+
+
+def ops_server(slots):
+    from ghive_amd import ops
+    return ops.Server(slots=slots, hbm_budget=96 << 30, slot_mem_limit=32 << 30)
+
+
 def load_traffic(name):
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
@@ -131,6 +221,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 4096-fragment batched form")
     ap.add_argument("--batched-reps", type=int, default=50)
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving form")
+    ap.add_argument("--serving-rates", default="50,200,1000", help="offered request rates (req/s)")
+    ap.add_argument("--serving-seconds", type=float, default=2.0)
+    ap.add_argument("--serving-slots", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the fragment shards "
@@ -211,10 +305,15 @@ def main():
                          f"vectorized map-join + hash group-by, {threads} threads, median of 3"}
 
     batched = None
+    T = None  # the 600M-row tables are not needed past this point
+    torch.cuda.empty_cache()
     if not args.no_batched:
-        del T
-        torch.cuda.empty_cache()
         batched = batched_section(ctx, args.batched_reps)
+
+    serving = None
+    if not args.no_serving:
+        serving = serving_section(ctx, [int(r) for r in args.serving_rates.split(",")], args.serving_seconds,
+                                  args.serving_slots)
 
     line = {
         "metric": METRIC, "value": value, "unit": "fact rows/s", "n_gpus": 1, "steps": args.steps,
@@ -231,6 +330,7 @@ def main():
                      "measured_copy_peak": copy_gbs},
         "cpu_baseline": cpu,
         "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

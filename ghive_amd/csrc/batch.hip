@@ -242,17 +242,9 @@ static int batch_enqueue(ghive_ctx *ctx, ghive_batch *bt, const BatchDev &B) {
   return GHIVE_OK;
 }
 
-}  // namespace ghive
-
-using namespace ghive;
-
-extern "C" {
-
-void ghive_batch_free(ghive_ctx *ctx, ghive_batch *bt) {
-  (void)ctx;
-  if (!bt) return;
+// HIP objects and device memory of a batch (its context still alive).
+void batch_release(ghive_batch *bt) {
   ghive_ctx *c = bt->owner;
-  (void)hipSetDevice(c->device);
   if (bt->stream) (void)hipStreamSynchronize(bt->stream);
   if (bt->exec) (void)hipGraphExecDestroy(bt->exec);
   if (bt->graph) (void)hipGraphDestroy(bt->graph);
@@ -262,6 +254,38 @@ void ghive_batch_free(ghive_ctx *ctx, ghive_batch *bt) {
   for (hipEvent_t e : {bt->ev_in, bt->ev_out, bt->t0, bt->t1})
     if (e) (void)hipEventDestroy(e);
   if (bt->stream) (void)hipStreamDestroy(bt->stream);
+  bt->stream = nullptr;
+  bt->exec = nullptr;
+  bt->graph = nullptr;
+  bt->d_items = nullptr;
+  bt->d_acc = nullptr;
+  bt->d_fbits = nullptr;
+  bt->ev_in = bt->ev_out = bt->t0 = bt->t1 = nullptr;
+}
+
+void orphan_batches(ghive_ctx *ctx) {
+  for (ghive_batch *bt : ctx->batches) {
+    batch_release(bt);
+    bt->owner = nullptr;
+  }
+  ctx->batches.clear();
+}
+
+}  // namespace ghive
+
+using namespace ghive;
+
+extern "C" {
+
+void ghive_batch_free(ghive_ctx *ctx, ghive_batch *bt) {
+  (void)ctx;
+  if (!bt) return;
+  ghive_ctx *c = bt->owner;  // NULL: its context was shut down and released everything already
+  if (c) {
+    (void)hipSetDevice(c->device);
+    ghive::batch_release(bt);
+    c->batches.erase(bt);
+  }
   delete bt;
 }
 
@@ -271,6 +295,7 @@ int ghive_batch_create(ghive_ctx *ctx, const ghive_batch_spec *spec, const ghive
   if (!spec || !out || (nitems && !items)) return set_error(ctx, GHIVE_ERR_INVALID, "batch: NULL argument");
   *out = nullptr;
   const ghive_hash_table *h = spec->dim;
+  if (h && !h->owner) return set_error(ctx, GHIVE_ERR_INVALID, "batch: the dimension table's context was shut down");
   if (!h || h->layout != 8 || !h->unique || h->payload_kind != 1)
     return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "batch: the dimension must be a unique-key INT table with an INT payload");
   if (spec->group_bits < 1 || spec->group_bits > 12 || spec->filter_bits < 1 || spec->filter_bits > 31 ||
@@ -422,6 +447,7 @@ int ghive_batch_create(ghive_ctx *ctx, const ghive_batch_spec *spec, const ghive
   }
   // keep the spec for direct launches
   bt->tune.probe_mode = mode;
+  ctx->batches.insert(bt);
   *out = bt;
   // stash what a direct launch needs in the graph-less case
   if (!use_graph) {
@@ -433,6 +459,7 @@ int ghive_batch_create(ghive_ctx *ctx, const ghive_batch_spec *spec, const ghive
 int ghive_batch_run(ghive_ctx *ctx, ghive_batch *bt, const ghive_batch_spec *spec) {
   GH_CHECK_CTX(ctx);
   if (!bt) return set_error(ctx, GHIVE_ERR_INVALID, "batch_run: NULL batch");
+  if (!bt->owner) return set_error(ctx, GHIVE_ERR_INVALID, "batch_run: the batch's context was shut down");
   GH_HIP(ctx, hipEventRecord(bt->ev_in, ctx->stream));
   GH_HIP(ctx, hipStreamWaitEvent(bt->stream, bt->ev_in, 0));
   GH_HIP(ctx, hipEventRecord(bt->t0, bt->stream));
@@ -476,6 +503,7 @@ int ghive_batch_run(ghive_ctx *ctx, ghive_batch *bt, const ghive_batch_spec *spe
 int ghive_batch_results(ghive_ctx *ctx, ghive_batch *bt, int64_t *sums, int64_t *counts) {
   GH_CHECK_CTX(ctx);
   if (!bt || !sums || !counts) return set_error(ctx, GHIVE_ERR_INVALID, "batch_results: NULL argument");
+  if (!bt->owner) return set_error(ctx, GHIVE_ERR_INVALID, "batch_results: the batch's context was shut down");
   const uint64_t n = (uint64_t)bt->nitems * bt->groups;
   GH_HIP(ctx, hipMemcpyAsync(sums, bt->d_acc, n * 8, hipMemcpyDeviceToHost, bt->stream));
   GH_HIP(ctx, hipMemcpyAsync(counts, bt->d_acc + n, n * 8, hipMemcpyDeviceToHost, bt->stream));

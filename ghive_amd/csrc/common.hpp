@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <set>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -75,11 +76,18 @@ struct ghive_ctx {
   ghive_tuning tune{};
   // reusable small device scratch for counters (8 x uint64) and its host mirror
   uint64_t *d_counters = nullptr;
+  // live objects whose memory comes from this context; ghive_shutdown orphans them so that a
+  // later ghive_hash_table_free / ghive_batch_free only releases the handle
+  std::set<ghive_hash_table *> tables;
+  std::set<ghive_batch *> batches;
 };
 
 namespace ghive {
 
 int set_error(ghive_ctx *ctx, int code, const char *fmt, ...);
+// Called by ghive_shutdown before the context's memory is released (hash.hip, batch.hip).
+void orphan_tables(ghive_ctx *ctx);
+void orphan_batches(ghive_ctx *ctx);
 
 #define GH_CHECK_CTX(ctx)                         \
   do {                                            \

@@ -188,6 +188,8 @@ void ghive_shutdown(ghive_ctx *ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  ghive::orphan_batches(ctx);
+  ghive::orphan_tables(ctx);
   delete ctx->arena;
   for (auto &e : ctx->ev)
     if (e) (void)hipEventDestroy(e);

@@ -230,7 +230,7 @@ int hash_build(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, uin
   if (payload && payload->row_num != key->row_num)
     return set_error(ctx, GHIVE_ERR_INVALID, "hash_build: payload and key row counts differ");
   if (key->row_num > kMaxRows) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "hash_build: too many rows");
-  if (!sel && n != key->row_num)
+  if (!sel && n && n != key->row_num)  // n == 0: an empty build (a selection that kept nothing)
     return set_error(ctx, GHIVE_ERR_INVALID, "hash_build: n (%llu) != rows (%llu) without a selection",
                      (unsigned long long)n, (unsigned long long)key->row_num);
   if (n && !key->data) return set_error(ctx, GHIVE_ERR_INVALID, "hash_build: key has no data");
@@ -314,6 +314,7 @@ int hash_build(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, uin
       }
     }
   }
+  ctx->tables.insert(h);
   *out = h;
   return GHIVE_OK;
 }
@@ -337,6 +338,20 @@ static int probe_run(ghive_ctx *ctx, const ghive_hash_table *h, const ghive_colu
 
 using namespace ghive;
 
+namespace ghive {
+
+void orphan_tables(ghive_ctx *ctx) {
+  for (ghive_hash_table *t : ctx->tables) {
+    t->owner = nullptr;
+    t->data = nullptr;
+    t->bitmap = nullptr;
+    t->dense = nullptr;
+  }
+  ctx->tables.clear();
+}
+
+}  // namespace ghive
+
 extern "C" {
 
 int ghive_hash_build(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, uint64_t n,
@@ -358,13 +373,14 @@ int ghive_hash_table_info(const ghive_hash_table *ht, uint64_t *entries, uint64_
 void ghive_hash_table_free(ghive_ctx *ctx, ghive_hash_table *ht) {
   if (!ht) return;
   (void)ctx;
-  ghive_ctx *c = ht->owner;  // the table's memory belongs to the context that built it
-  if (c) {
+  ghive_ctx *c = ht->owner;  // the table's memory belongs to the context that built it (NULL:
+  if (c) {                   // that context was shut down and released the memory already)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     if (ht->data) c->arena->free(ht->data);
     if (ht->bitmap) c->arena->free(ht->bitmap);
     if (ht->dense) c->arena->free(ht->dense);
+    c->tables.erase(ht);
   }
   delete ht;
 }
@@ -374,6 +390,7 @@ int ghive_hash_probe(ghive_ctx *ctx, const ghive_hash_table *ht, const ghive_col
                      uint64_t capacity, uint64_t *out_count) {
   GH_CHECK_CTX(ctx);
   if (!ht || !key || !out_count) return set_error(ctx, GHIVE_ERR_INVALID, "hash_probe: NULL argument");
+  if (!ht->owner) return set_error(ctx, GHIVE_ERR_INVALID, "hash_probe: the table's context was shut down");
   if (!is_int_type(key->type))
     return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "hash_probe: key type %d (INT/LONG only)", key->type);
   if (!sel && n != key->row_num)

@@ -254,3 +254,82 @@ class PlainBatch:
             if collect:
                 res.append(dict(zip(k.cpu().tolist(), v.cpu().tolist())))
         return res
+
+
+# ---- serving: a request mix over resident inputs ----------------------------------------------
+@dataclass
+class ServingInputs:
+    small: SmallTables          # fact rows sliced by fragment requests (config-1 shape)
+    dim_tables: dict            # attr value -> resident dimension hash table
+    gb_keys: torch.Tensor       # int64 group-by keys, uniform over [0, 10M)
+    gb_vals: torch.Tensor       # int64 values
+    sort_keys: torch.Tensor     # int64 sort keys
+
+
+SERVE_MIX = (("fragment", 0.60), ("groupby", 0.25), ("sort", 0.15))
+SERVE_GB_DISTINCT = 10_000_000
+
+
+def serving_inputs(ctx, frag_rows=16_000_000, gb_rows=16_000_000, sort_rows=256_000_000, seed=SEED_BASE + 5):
+    """Resident inputs of the serving mix: a config-1 fact table with its dimension hash tables
+    (one per filter constant, built once and shared by every request), group-by columns and a
+    sort-key column; requests are prefixes or aligned slices of these."""
+    S = small_tables(ctx, frag_rows, seed=seed)
+    dims = {c: small_dim_table(ctx, S.dim, attr_value=c) for c in range(7)}
+    k = ctx.gen_uniform_i64(seed, 20, gb_rows, 0, SERVE_GB_DISTINCT - 1)
+    v = ctx.gen_uniform_i64(seed, 21, gb_rows, -10**9, 10**9)
+    sk = ctx.gen_uniform_i64(seed, 22, sort_rows, -2**62, 2**62)
+    return ServingInputs(S, dims, k, v, sk)
+
+
+def serving_requests(nreq, seed, I):
+    """nreq requests of the mix (kind, params): fragments of 1k..16M rows (lognormal, median 256k)
+    at an aligned offset with one of the 7 filter constants; group-bys over 1k..16M rows with up
+    to 10M distinct keys; sorts of 1M..256M int64 keys (lognormal, median 8M)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    kinds = rng.choice(len(SERVE_MIX), size=nreq, p=[p for _, p in SERVE_MIX])
+    fact_rows = I.small.fact_rows
+    out = []
+    for i, k in enumerate(kinds):
+        kind = SERVE_MIX[k][0]
+        if kind == "fragment":
+            rows = int(lognormal_rows(1, seed * 7919 + i, median=256_000, hi=fact_rows)[0])
+            start = int(rng.integers(0, (fact_rows - rows) // 4 + 1)) * 4
+            out.append(("fragment", dict(start=start, rows=rows, attr=int(rng.integers(0, 7)))))
+        elif kind == "groupby":
+            rows = int(lognormal_rows(1, seed * 7919 + i, median=1_000_000, hi=I.gb_keys.numel())[0])
+            out.append(("groupby", dict(rows=rows)))
+        else:
+            rows = int(lognormal_rows(1, seed * 7919 + i, median=8_000_000, sigma=1.2, lo=1_000_000,
+                                      hi=I.sort_keys.numel())[0])
+            out.append(("sort", dict(rows=rows)))
+    return out
+
+
+def serving_submit(srv, I, kind, p):
+    """Submits one request of the mix to a Server; returns (ticket, fact rows)."""
+    if kind == "fragment":
+        fact = ops.Table([c[p["start"]: p["start"] + p["rows"]] for c in I.small.fact.columns])
+        return srv.submit_fragment(small_fragment(fact, I.dim_tables[p["attr"]]), capacity=64), p["rows"]
+    if kind == "groupby":
+        n = p["rows"]
+        t = ops.Table([I.gb_keys[:n], I.gb_vals[:n]])
+        aggs = [(abi.AGG_SUM, 1), (abi.AGG_CNT, abi.COUNT_STAR)]
+        return srv.submit_groupby(t, [0], aggs, capacity=min(n, SERVE_GB_DISTINCT)), n
+    n = p["rows"]
+    return srv.submit_sort(ops.Table([I.sort_keys[:n]]), [0], "+"), n
+
+
+def serving_plain(ctx, I, kind, p):
+    """The same request through the plain entry point (for identity checks)."""
+    if kind == "fragment":
+        fact = ops.Table([c[p["start"]: p["start"] + p["rows"]] for c in I.small.fact.columns])
+        return small_fragment(fact, I.dim_tables[p["attr"]]).execute(ctx, capacity=64)
+    if kind == "groupby":
+        n = p["rows"]
+        t = ops.Table([I.gb_keys[:n], I.gb_vals[:n]])
+        aggs = [(abi.AGG_SUM, 1), (abi.AGG_CNT, abi.COUNT_STAR)]
+        return ops.GroupByPredicate([0], aggs).gpu_execute(ctx, t, capacity=min(n, SERVE_GB_DISTINCT))
+    n = p["rows"]
+    return [ops.sort(ctx, ops.Table([I.sort_keys[:n]]), [0], "+")]

@@ -102,3 +102,26 @@ def test_join_nway_vs_oracle(ctx, orc, types):
     got = to_np(jp.process(ctx, [ops.Table([to_dev(t)]) for t in (t0, t1, t2)]))
     want = orc.join([[t0], [t1], [t2]], [0, 0, 0], conds)
     assert sorted(map(tuple, got.tolist())) == sorted(map(tuple, want.tolist()))
+
+
+def test_tables_outlive_their_context(ctx):
+    """A hash table (or batch) whose context was shut down is orphaned, not left dangling: using it
+    is an INVALID error, and freeing it afterwards only releases the handle."""
+    from ghive_amd import ops, workloads as W
+    from ghive_amd._lib import GhiveError
+    T = W.small_tables(ctx, 10_000, seed=21)
+    other = ops.Context(0)
+    ht = W.small_dim_table(other, T.dim)
+    empty = W.small_dim_table(other, T.dim, attr_value=99)  # a selection that keeps nothing
+    assert empty.info()["entries"] == 0
+    other.close()
+    with pytest.raises(GhiveError) as e:
+        W.small_fragment(T.fact, ht).execute(ctx, capacity=64)
+    assert e.value.status == abi.ERR_INVALID and "shut down" in str(e.value)
+    with pytest.raises(GhiveError):
+        ht.probe(T.fact.columns[0])
+    ht.free()
+    empty.free()
+    # the session context is unaffected
+    live = W.small_dim_table(ctx, T.dim)
+    assert len(W.small_fragment(T.fact, live).execute(ctx, capacity=64)[0]) > 0
