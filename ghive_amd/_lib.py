@@ -95,6 +95,8 @@ SIGNATURES_EXTRA = {
     "ghive_bench_stream": (C.c_int, [P, P, P, u64, C.c_int, C.c_int, C.POINTER(C.c_float)]),
     "ghive_gen_rank_keys": (C.c_int, [P, u64, u64, u64, P]),
     "ghive_gen_zipf_keys": (C.c_int, [P, u64, u32, u64, u64, u64, C.c_double, C.c_double, P]),
+    "ghive_join_result": (C.c_int, [P, C.POINTER(abi.Table), u32, C.POINTER(abi.JoinCond), u32, P, P, P, u64,
+                                    C.POINTER(abi.Column), u32, C.POINTER(u32)]),
     "ghive_server_create": (C.c_int, [C.c_int, C.POINTER(abi.ServerConfig), C.POINTER(P)]),
     "ghive_server_submit": (C.c_int, [P, C.POINTER(abi.Request), PU64]),
     "ghive_server_wait": (C.c_int, [P, u64, C.POINTER(abi.Response)]),

@@ -251,13 +251,44 @@ class HashTable:
 
 
 class JoinPredicate:
-    """BaseJoinPredicate (join_cols, join_types as (type, left, right)); hash joins."""
+    """BaseJoinPredicate (join_cols, join_types as (type, left, right), maintain_cols per input
+    table as JoinOperator::parseExtended fills them); hash joins."""
 
-    def __init__(self, join_cols, join_types):
+    def __init__(self, join_cols, join_types, maintain_cols=None):
         self.join_cols = list(join_cols)
         self.join_types = [tuple(t) for t in join_types]
+        self.maintain_cols = [list(m) for m in maintain_cols] if maintain_cols is not None else None
         self.result_offsets = None
         self.cardinality = 0
+
+    def generate_result(self, ctx, tables):
+        """BaseJoinPredicate::generate_result: the maintained columns of every non-semi input,
+        gathered by result_offsets (NULL sentinel where an offset is -1). Call after process()."""
+        if self.result_offsets is None:
+            raise ValueError("generate_result before process")
+        maintain = self.maintain_cols or [[] for _ in tables]
+        tarr = (abi.Table * len(tables))(*[t.table for t in tables])
+        conds = (abi.JoinCond * max(len(self.join_types), 1))(*[abi.JoinCond(*c) for c in self.join_types])
+        rows = [(C.c_uint32 * max(len(m), 1))(*m) for m in maintain]
+        mptr = (C.POINTER(C.c_uint32) * len(tables))(*[C.cast(r, C.POINTER(C.c_uint32)) for r in rows])
+        mcnt = (C.c_uint32 * len(tables))(*[len(m) for m in maintain])
+        semi = {t[2] for t in self.join_types if t[0] == abi.LEFT_SEMI_JOIN}
+        plan = []
+        for ti, t in enumerate(tables):
+            if ti in semi:
+                continue
+            cols = maintain[ti] if maintain[ti] or plan else [0]
+            plan += [t.columns[c] for c in cols]
+        n = self.cardinality
+        outs = [torch.empty(max(n, 1), dtype=c.dtype, device=c.device) for c in plan]
+        ocols = (abi.Column * max(len(outs), 1))()
+        for i, o in enumerate(outs):
+            ocols[i].data = o.data_ptr()
+        nout = C.c_uint32()
+        ctx.check(lib().ghive_join_result(ctx.h, tarr, len(tables), conds, len(self.join_types),
+                                          C.cast(mptr, C.c_void_p), C.cast(mcnt, C.c_void_p),
+                                          _p(self.result_offsets), n, ocols, len(outs), C.byref(nout)))
+        return [o[:n] for o in outs]
 
     def process(self, ctx, tables, capacity=None):
         tarr = (abi.Table * len(tables))(*[t.table for t in tables])

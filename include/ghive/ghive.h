@@ -253,6 +253,20 @@ int ghive_join(ghive_ctx *ctx, const ghive_table *tables, uint32_t ntables,
                const uint32_t *join_cols, const ghive_join_cond *conds, uint32_t nconds,
                int32_t *out_offsets, uint64_t capacity, uint64_t *out_cardinality);
 
+/* BaseJoinPredicate::generate_result (BaseJoinPredicate.cu:4-97): the join's output table from
+ * ghive_join's result_offsets (`cardinality` rows, same tables and conds). For each result table
+ * (the non-semi inputs in table order) its columns maintain_cols[t][0..maintain_counts[t]) are
+ * gathered, in that order; when the first result table maintains nothing its column 0 is kept,
+ * as in the reference. An offset of -1 gives the type's NULL sentinel; a STRING column gathers its
+ * (start,end) pairs into out aux ((-1,-1) for NULL) and shares the input blob (out data = input
+ * data). out_cols: caller device buffers of `cardinality` rows; *out_ncols = the column count
+ * (GHIVE_ERR_CAPACITY when it exceeds out_capacity_cols). */
+int ghive_join_result(ghive_ctx *ctx, const ghive_table *tables, uint32_t ntables,
+                      const ghive_join_cond *conds, uint32_t nconds,
+                      const uint32_t *const *maintain_cols, const uint32_t *maintain_counts,
+                      const int32_t *offsets, uint64_t cardinality, ghive_column *out_cols,
+                      uint32_t out_capacity_cols, uint32_t *out_ncols);
+
 /* ---- group-by ----------------------------------------------------------------------- */
 /* AggregationType (gpu_impl/inc/Operator/AggregationDesc.hpp:6-27), same numbering. */
 enum {

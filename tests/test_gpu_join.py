@@ -125,3 +125,74 @@ def test_tables_outlive_their_context(ctx):
     # the session context is unaffected
     live = W.small_dim_table(ctx, T.dim)
     assert len(W.small_fragment(T.fact, live).execute(ctx, capacity=64)[0]) > 0
+
+
+def _gather_expected(cols, offsets, null):
+    out = np.empty(len(offsets), dtype=cols.dtype)
+    ok = offsets >= 0
+    out[ok] = cols[offsets[ok]]
+    out[~ok] = null
+    return out
+
+
+def test_generate_result_reference_kat(ctx, kats):
+    """generate_result over the reference's own join KAT (LEFT SEMI + LEFT OUTER): the maintained key
+    columns gathered by the KAT's result_offsets, LONG_MAX where the offset is -1."""
+    from ghive_amd import ops
+    for k in kats["join"]:
+        tables = [ops.Table([to_dev(np.array(t, dtype=np.int64))]) for t in k["tables"]]
+        conds = [tuple(c) for c in k["conds"]]
+        jp = ops.JoinPredicate(k["join_cols"], conds, maintain_cols=[[0] for _ in tables])
+        jp.process(ctx, tables)
+        got_cols = [to_np(c) for c in jp.generate_result(ctx, tables)]
+        offs = to_np(jp.result_offsets)
+        semi = {c[2] for c in conds if c[0] == abi.LEFT_SEMI_JOIN}
+        result_tables = [t for t in range(len(tables)) if t not in semi]
+        want_rows = sorted(zip(*[_gather_expected(np.array(k["tables"][t], dtype=np.int64),
+                                                  np.array(k["expect_offsets"], dtype=np.int32).reshape(-1, k["width"])[:, w],
+                                                  LONG_NULL)
+                                 for w, t in enumerate(result_tables)]))
+        got_rows = sorted(zip(*got_cols))
+        assert got_rows == want_rows, k["cite"]
+        # and the gather follows this run's own offsets column by column
+        for w, t in enumerate(result_tables):
+            assert np.array_equal(got_cols[w], _gather_expected(np.array(k["tables"][t], dtype=np.int64),
+                                                               offs[:, w], LONG_NULL))
+
+
+def test_generate_result_types_and_maintained_columns(ctx, orc):
+    """Maintained columns of every type in maintain order, NULL sentinels of each type on the outer
+    side, the first table's column 0 when it maintains nothing, semi inputs emitting nothing."""
+    import torch
+    from ghive_amd import ops
+    rng = np.random.default_rng(41)
+    n0, n1, n2 = 5000, 400, 300
+    k0 = rng.integers(0, 500, n0).astype(np.int64)
+    k1 = rng.integers(0, 500, n1).astype(np.int32)
+    k2 = rng.integers(200, 700, n2).astype(np.int64)
+    t0 = [k0, rng.standard_normal(n0), rng.integers(-9, 9, n0).astype(np.int32)]
+    t1 = [k1, rng.standard_normal(n1).astype(np.float32), rng.integers(0, 1 << 40, n1).astype(np.int64)]
+    t2 = [k2, rng.integers(0, 9, n2).astype(np.int32)]
+    tables = [ops.Table([to_dev(c) for c in t]) for t in (t0, t1, t2)]
+    for conds, maintain in ((((abi.LEFT_OUTER_JOIN, 0, 1), (abi.LEFT_OUTER_JOIN, 0, 2)), [[2, 1], [1, 2, 0], [1]]),
+                            (((abi.INNER_JOIN, 0, 1), (abi.LEFT_SEMI_JOIN, 0, 2)), [[], [2], [1]])):
+        jp = ops.JoinPredicate([0, 0, 0], conds, maintain_cols=maintain)
+        jp.process(ctx, tables)
+        offs = to_np(jp.result_offsets)
+        want_offs = orc.join([[k0], [k1], [k2]], [0, 0, 0], list(conds))
+        assert sorted(map(tuple, offs.tolist())) == sorted(map(tuple, want_offs.tolist()))
+        got = [to_np(c) for c in jp.generate_result(ctx, tables)]
+        semi = {c[2] for c in conds if c[0] == abi.LEFT_SEMI_JOIN}
+        result_tables = [t for t in range(3) if t not in semi]
+        want = []
+        for w, t in enumerate(result_tables):
+            cols = maintain[t] if (maintain[t] or w > 0) else [0]
+            for c in cols:
+                src = (t0, t1, t2)[t][c]
+                null = {np.dtype(np.int64): LONG_NULL, np.dtype(np.int32): 2**31 - 1,
+                        np.dtype(np.float64): np.nan, np.dtype(np.float32): np.float32(np.nan)}[src.dtype]
+                want.append(_gather_expected(src, offs[:, w], null))
+        assert len(got) == len(want)
+        for g, x in zip(got, want):
+            assert g.dtype == x.dtype
+            assert np.array_equal(g, x, equal_nan=g.dtype.kind == "f")
