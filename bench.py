@@ -106,6 +106,75 @@ def batched_section(ctx, reps):
     return out
 
 
+def skew_section(ctx, reps):
+    """Config 3 at full size: 64M-key build, 1e9 Zipf(1.0) probe rows with 10% misses, ORDER BY
+    (key, float tiebreak) over the joined rows, and GROUP BY over 1e9 rows with 100M distinct int64
+    keys (COUNT(*), exact SUM(double)). Wall time around each synchronous call, median of reps;
+    GB/s over the algorithmic bytes of each stage."""
+    import torch
+    import numpy as np
+    from ghive_amd import abi, ops, workloads as W
+
+    def timed(fn):
+        ts, out = [], None
+        for _ in range(reps):
+            out = None
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), out
+
+    S = W.skew_tables(ctx)
+    nb, npr = S.build_keys.numel(), S.probe_keys.numel()
+    t_build, ht = timed(lambda: ops.HashTable(ctx, S.build_keys, payload=S.build_pay, load_factor=0.5))
+    t_probe, (rows, pay) = timed(lambda: ht.probe(S.probe_keys, capacity=npr))
+    matches = rows.numel()
+    ht.free()
+    r64 = rows.to(torch.int64) & 0xFFFFFFFF
+    jk = S.probe_keys[r64].contiguous()
+    jt = S.tiebreak[r64].contiguous()
+    del rows, pay, r64, S
+    t_sort, perm = timed(lambda: ops.sort(ctx, ops.Table([jk, jt]), [0, 1], ["+", "+"]))
+    del perm, jk, jt
+    torch.cuda.empty_cache()
+    keys, vals = W.highcard_tables(ctx)
+    ng = [0]
+
+    def gby():
+        outs = ops.GroupByPredicate([0], [(abi.AGG_CNT, abi.COUNT_STAR), (abi.AGG_SUM, 1)]).gpu_execute(
+            ctx, ops.Table([keys, vals]), capacity=110_000_000)
+        ng[0] = outs[0].numel()
+        return None
+
+    t_gby, _ = timed(gby)
+    nrows = keys.numel()
+    del keys, vals
+    torch.cuda.empty_cache()
+    b_build = nb * 16
+    b_probe = npr * 8 + matches * 12          # probe keys in; (probe row id, payload) out
+    b_sort = matches * 16                      # lower bound: 12 B of keys in, 4 B of permutation out
+    b_gby = nrows * 16 + ng[0] * 24            # key + value in; key, COUNT, SUM out
+    gbs = lambda b, t: b / t / 1e9
+    out = {
+        "workload": "64M unique int64 build keys (int64 payload) x 1e9 probe rows, Zipf(1.0) over build "
+                    "ranks + 10% misses; ORDER BY key, float32 tiebreak over the joined rows; GROUP BY "
+                    "1e9 rows, 100M distinct int64 keys, COUNT(*), SUM(double N(0,1)) exact",
+        "build": {"rows": nb, "ms": 1e3 * t_build, "rows_per_s": nb / t_build, "gbs": gbs(b_build, t_build)},
+        "probe": {"rows": npr, "matches": matches, "ms": 1e3 * t_probe, "rows_per_s": npr / t_probe,
+                  "gbs": gbs(b_probe, t_probe), "frac": gbs(b_probe, t_probe) / HBM_PEAK_GBS},
+        "sort": {"rows": matches, "ms": 1e3 * t_sort, "rows_per_s": matches / t_sort,
+                 "gbs_lower_bound_bytes": gbs(b_sort, t_sort)},
+        "groupby": {"rows": nrows, "groups": ng[0], "ms": 1e3 * t_gby, "rows_per_s": nrows / t_gby,
+                    "gbs": gbs(b_gby, t_gby), "frac": gbs(b_gby, t_gby) / HBM_PEAK_GBS},
+    }
+    log(f"skew: build {1e3 * t_build:.1f} ms, probe {1e3 * t_probe:.1f} ms ({out['probe']['gbs']:.0f} GB/s), "
+        f"sort {1e3 * t_sort:.1f} ms of {matches} rows, group-by {1e3 * t_gby:.1f} ms "
+        f"({out['groupby']['gbs']:.0f} GB/s, {ng[0]} groups)")
+    return out
+
+
 def _same_result(kind, a, b):
     import numpy as np
     if kind == "sort":
@@ -222,9 +291,11 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 4096-fragment batched form")
     ap.add_argument("--batched-reps", type=int, default=50)
     ap.add_argument("--no-serving", action="store_true", help="skip the serving form")
+    ap.add_argument("--no-skew", action="store_true", help="skip the skewed join / sort / group-by stages")
+    ap.add_argument("--skew-reps", type=int, default=3)
     ap.add_argument("--serving-rates", default="50,200,1000", help="offered request rates (req/s)")
     ap.add_argument("--serving-seconds", type=float, default=2.0)
-    ap.add_argument("--serving-slots", type=int, default=4)
+    ap.add_argument("--serving-slots", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the fragment shards "
@@ -310,6 +381,10 @@ def main():
     if not args.no_batched:
         batched = batched_section(ctx, args.batched_reps)
 
+    skew = None
+    if not args.no_skew:
+        skew = skew_section(ctx, args.skew_reps)
+
     serving = None
     if not args.no_serving:
         serving = serving_section(ctx, [int(r) for r in args.serving_rates.split(",")], args.serving_seconds,
@@ -330,6 +405,7 @@ def main():
                      "measured_copy_peak": copy_gbs},
         "cpu_baseline": cpu,
         "batched": batched,
+        "skew": skew,
         "serving": serving,
     }
     print(json.dumps(line), flush=True)

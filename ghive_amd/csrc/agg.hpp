@@ -39,18 +39,30 @@ struct AggSpec {
 
 struct GroupTableDev {
   uint64_t *keys;    // slots + 1
-  uint64_t *acc;     // nwords * stride
+  uint64_t *acc;     // stride records of `rec` words: all accumulator words of a group together
   uint64_t stride;   // slots + 1
   uint64_t mask;     // slots - 1
   int32_t log2;
   int32_t nwords;
   int32_t naggs;
-  int32_t pad;
+  int32_t rec;       // words per record: nwords rounded up to 1, 2, 4 or a multiple of 8, so that a
+                     // record never straddles a 64-byte line and one row's updates share a line
   unsigned long long *nent;      // entries inserted (extra slot included)
   unsigned long long *overflow;  // set when an insert could not be placed
   uint64_t capacity;             // max entries accepted
   AggSpec aggs[kMaxAggs];
 };
+
+// Accumulator word w of group slot g.
+__host__ __device__ __forceinline__ uint64_t *acc_word(const GroupTableDev &T, int w, uint64_t g) {
+  return T.acc + g * (uint64_t)T.rec + (uint64_t)w;
+}
+__host__ __device__ inline int record_words(int nwords) {
+  if (nwords <= 1) return 1;
+  if (nwords <= 2) return 2;
+  if (nwords <= 4) return 4;
+  return (nwords + 7) & ~7;
+}
 
 __device__ __forceinline__ uint64_t init_word(int kind, int w) {
   if (w == 0) {
@@ -292,7 +304,7 @@ __device__ __forceinline__ void lds_table_flush(uint64_t *lds, const GroupTableD
     if (g < 0) continue;
     for (int a = 0; a < T.naggs; ++a) {
       const AggSpec &A = T.aggs[a];
-      acc_merge(A, [&](int j) { return T.acc + (size_t)(A.word0 + j) * T.stride + g; },
+      acc_merge(A, [&](int j) { return acc_word(T, A.word0 + j, (uint64_t)g); },
                 [&](int j) { return lds + (size_t)(1 + A.word0 + j) * lstride + s; });
     }
   }

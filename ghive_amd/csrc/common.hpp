@@ -184,6 +184,8 @@ int h2d(ghive_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 // Launch helpers (defined in kernels_util.hip).
 int fill_u64(ghive_ctx *ctx, uint64_t *p, uint64_t value, uint64_t n);
 int fill_u32(ghive_ctx *ctx, uint32_t *p, uint32_t value, uint64_t n);
+// p[i * step] = value for i < n.
+int fill_u64_strided(ghive_ctx *ctx, uint64_t *p, uint64_t value, uint64_t n, uint64_t step);
 // Exclusive scan of n uint32 values into out (out may alias in); writes the total to
 // *d_total (device) when non-null.
 int exclusive_scan_u32(ghive_ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n,

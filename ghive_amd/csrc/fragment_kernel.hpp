@@ -117,7 +117,7 @@ __device__ __forceinline__ void frag_emit(const FragDev &F, const GroupTableDev 
       if (ls >= 0)
         acc_apply(A, [&](int w) { return B.ltab + (size_t)(1 + A.word0 + w) * B.lstride + ls; }, v);
       else
-        acc_apply(A, [&](int w) { return T.acc + (size_t)(A.word0 + w) * T.stride + g; }, v);
+        acc_apply(A, [&](int w) { return acc_word(T, A.word0 + w, (uint64_t)g); }, v);
     }
   }
 }

@@ -77,18 +77,36 @@ __device__ __forceinline__ AggVal agg_value(const AggSpec &A, uint64_t r) {
   return v;
 }
 
+// A workgroup aggregates into its LDS table first and merges it into the global table at the end.
+// When its rows carry more distinct keys than the LDS table holds (a quarter of the tries miss
+// after 8192 tries), it stops trying the LDS table and updates the global table directly.
 __global__ __launch_bounds__(kGbThreads) void k_groupby(KeySpec K, GroupTableDev T, const uint32_t *sel,
                                                         uint64_t n, int lcap_log2) {
   extern __shared__ uint64_t lds[];
+  __shared__ uint32_t tries, misses, lds_off;
   const int lstride = (1 << lcap_log2) + 1;
   lds_table_init(lds, T, lcap_log2);
+  if (threadIdx.x == 0) {
+    tries = 0;
+    misses = 0;
+    lds_off = 0;
+  }
   __syncthreads();
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) {
     const uint64_t r = sel ? sel[i] : i;
     const uint64_t key = pack_key(K, r);
-    const int ls = lds_find_insert(lds, key, lcap_log2);
+    int ls = -1;
+    if (!lds_off) {
+      ls = lds_find_insert(lds, key, lcap_log2);
+      const uint64_t miss = ballot(ls < 0), all = ballot(true);
+      if (lane_id() == 0) {
+        const uint32_t t = atomicAdd(&tries, (uint32_t)__popcll(all)) + (uint32_t)__popcll(all);
+        const uint32_t m = atomicAdd(&misses, (uint32_t)__popcll(miss)) + (uint32_t)__popcll(miss);
+        if (t >= 8192 && 4u * m > t) lds_off = 1;
+      }
+    }
     if (ls >= 0) {
       for (int a = 0; a < T.naggs; ++a) {
         const AggSpec &A = T.aggs[a];
@@ -101,7 +119,7 @@ __global__ __launch_bounds__(kGbThreads) void k_groupby(KeySpec K, GroupTableDev
       for (int a = 0; a < T.naggs; ++a) {
         const AggSpec &A = T.aggs[a];
         AggVal v = agg_value(A, r);
-        acc_apply(A, [&](int j) { return T.acc + (size_t)(A.word0 + j) * T.stride + g; }, v);
+        acc_apply(A, [&](int j) { return acc_word(T, A.word0 + j, (uint64_t)g); }, v);
       }
     }
   }
@@ -151,9 +169,9 @@ __device__ __forceinline__ void write_null(void *col, int type, uint64_t idx) {
   }
 }
 
-__device__ __forceinline__ void finalize_agg(const AggSpec &A, const uint64_t *acc, uint64_t stride, uint64_t s,
-                                             void *col, uint64_t idx) {
-  auto W = [&](int j) { return acc[(size_t)(A.word0 + j) * stride + s]; };
+__device__ __forceinline__ void finalize_agg(const AggSpec &A, const GroupTableDev &T, uint64_t s, void *col,
+                                             uint64_t idx) {
+  auto W = [&](int j) { return *acc_word(T, A.word0 + j, s); };
   switch (A.kind) {
     case AK_CNT: static_cast<int64_t *>(col)[idx] = (int64_t)W(0); return;
     case AK_SUMI: {
@@ -221,7 +239,7 @@ __global__ void k_groupby_finalize(KeySpec K, GroupTableDev T, OutSpec O, unsign
         default: static_cast<uint32_t *>(O.col[k])[idx] = (uint32_t)bits; break;
       }
     }
-    for (int a = 0; a < T.naggs; ++a) finalize_agg(T.aggs[a], T.acc, T.stride, s, O.col[K.nkeys + a], idx);
+    for (int a = 0; a < T.naggs; ++a) finalize_agg(T.aggs[a], T, s, O.col[K.nkeys + a], idx);
   }
 }
 
@@ -239,7 +257,8 @@ int group_table_alloc(ghive_ctx *ctx, GroupTableDev &T, uint64_t capacity, DevBu
   T.log2 = __builtin_ctzll(slots);
   T.capacity = capacity;
   GH_TRY(keys.alloc(ctx, T.stride * 8));
-  GH_TRY(acc.alloc(ctx, T.stride * 8 * (uint64_t)(T.nwords ? T.nwords : 1)));
+  T.rec = record_words(T.nwords);
+  GH_TRY(acc.alloc(ctx, T.stride * 8 * (uint64_t)T.rec));
   GH_TRY(counters.alloc(ctx, 2 * sizeof(unsigned long long)));
   T.keys = keys.as<uint64_t>();
   T.acc = acc.as<uint64_t>();
@@ -248,15 +267,14 @@ int group_table_alloc(ghive_ctx *ctx, GroupTableDev &T, uint64_t capacity, DevBu
   GH_TRY(fill_u64(ctx, T.keys, kEmptyKey, slots));
   GH_TRY(fill_u64(ctx, T.keys + slots, 0, 1));
   GH_TRY(fill_u64(ctx, reinterpret_cast<uint64_t *>(T.nent), 0, 2));
-  for (int a = 0; a < T.naggs; ++a) {
+  GH_TRY(fill_u64(ctx, T.acc, 0, T.stride * (uint64_t)T.rec));
+  for (int a = 0; a < T.naggs; ++a) {  // MIN / MAX start from their identities
     const AggSpec &A = T.aggs[a];
-    for (int w = 0; w < agg_words(A.kind); ++w) {
-      uint64_t v = 0;
-      if (w == 0 && A.kind == AK_MINI) v = 0x7fffffffffffffffull;
-      if (w == 0 && A.kind == AK_MAXI) v = 0x8000000000000000ull;
-      if (w == 0 && A.kind == AK_MIND) v = ~0ull;
-      GH_TRY(fill_u64(ctx, T.acc + (uint64_t)(A.word0 + w) * T.stride, v, T.stride));
-    }
+    uint64_t v = 0;
+    if (A.kind == AK_MINI) v = 0x7fffffffffffffffull;
+    if (A.kind == AK_MAXI) v = 0x8000000000000000ull;
+    if (A.kind == AK_MIND) v = ~0ull;
+    if (v) GH_TRY(fill_u64_strided(ctx, T.acc + A.word0, v, T.stride, (uint64_t)T.rec));
   }
   return GHIVE_OK;
 }

@@ -167,6 +167,102 @@ __global__ __launch_bounds__(kHashThreads) void k_hash_probe(
   }
 }
 
+// Probe of a unique-key table (or the first match only, LEFT SEMI): at most one output per probe
+// row. A workgroup probes a tile of blockDim.x * R rows (row = tile base + q * blockDim.x + lane,
+// coalesced key loads), counts its matches with wave ballots and one LDS scan, and reserves its
+// output range with ONE atomic per tile -- a per-wave append would put ~n/64 device-scope
+// atomics on a single counter, which serialise across the XCDs.
+constexpr int kProbeThreads = 512;
+template <int R>
+__global__ __launch_bounds__(kProbeThreads) void k_probe_unique(
+    DevHash h, const void *keys, int key_type, const uint32_t *sel, const int32_t *tup, uint64_t n,
+    int emit_index, uint32_t *out_a, int64_t *out_b, uint64_t cap, unsigned long long *counter) {
+  __shared__ uint32_t wave_total[kProbeThreads / kWave];
+  __shared__ unsigned long long tile_base;
+  const uint32_t lane = lane_id(), wave = threadIdx.x / kWave, nwaves = blockDim.x / kWave;
+  const uint64_t tile_rows = (uint64_t)blockDim.x * R;
+  const uint64_t ntiles = (n + tile_rows - 1) / tile_rows;
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    bool match[R];
+    int64_t payload[R];
+    uint64_t row[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const uint64_t i = tile * tile_rows + (uint64_t)q * blockDim.x + threadIdx.x;
+      bool active = i < n;
+      uint64_t r = 0;
+      if (active) {
+        if (tup) {
+          const int32_t t = tup[i];
+          active = t >= 0;
+          r = (uint64_t)(t < 0 ? 0 : t);
+        } else {
+          r = sel ? sel[i] : i;
+        }
+      }
+      int64_t k = 0;
+      bool valid = active && col_load_i64(keys, key_type, r, k) && k >= h.key_min && k <= h.key_max;
+      if (valid && h.layout == 8 && (k < INT32_MIN || k > INT32_MAX)) valid = false;
+      bool hit = false;
+      int64_t p = -1;
+      if (valid) {
+        uint64_t s = home_slot(h, k);
+        for (uint64_t step = 0; step <= h.mask; ++step) {  // a table always has an empty slot
+          if (h.layout == 8) {
+            const uint64_t w = h.slots[s];
+            if ((uint32_t)w == 0x7fffffffu) break;
+            if ((int64_t)(int32_t)(uint32_t)w == k) {
+              hit = true;
+              p = (int64_t)(int32_t)(uint32_t)(w >> 32);
+              break;
+            }
+          } else {
+            const uint64_t kk = h.slots[2 * s];
+            if (kk == kEmpty16) break;
+            if ((int64_t)kk == k) {
+              hit = true;
+              p = (int64_t)h.slots[2 * s + 1];
+              break;
+            }
+          }
+          s = (s + 1) & h.mask;
+        }
+      }
+      match[q] = hit;
+      payload[q] = p;
+      row[q] = emit_index ? i : r;
+    }
+    // offsets: per q a ballot; the tile's matches are laid out q-major, wave-major, lane-minor
+    uint32_t below[R], wave_count = 0;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const uint64_t m = ballot(match[q]);
+      below[q] = wave_count + lanes_below(m);
+      wave_count += (uint32_t)__popcll(m);
+    }
+    if (lane == 0) wave_total[wave] = wave_count;
+    __syncthreads();
+    uint32_t wave_off = 0, tile_count = 0;
+    for (uint32_t w = 0; w < nwaves; ++w) {
+      const uint32_t c = wave_total[w];
+      wave_off += w < wave ? c : 0u;
+      tile_count += c;
+    }
+    if (threadIdx.x == 0) tile_base = tile_count ? atomicAdd(counter, (unsigned long long)tile_count) : 0ull;
+    __syncthreads();
+    const uint64_t base = tile_base + wave_off;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const uint64_t pos = base + below[q];
+      if (match[q] && pos < cap) {
+        out_a[pos] = (uint32_t)row[q];
+        out_b[pos] = payload[q];
+      }
+    }
+    __syncthreads();  // wave_total / tile_base are rewritten by the next tile
+  }
+}
+
 // bitmap bit and dense payload of every inserted key (unique keys, so no write races on dense)
 __global__ __launch_bounds__(kHashThreads) void k_dense_fill(const void *keys, int key_type, const uint32_t *sel,
                                                              uint64_t n, const void *pay, int pay_type,
@@ -324,7 +420,15 @@ static int probe_run(ghive_ctx *ctx, const ghive_hash_table *h, const ghive_colu
                      const uint32_t *sel, const int32_t *tup, uint64_t n, int mode, int emit_index,
                      uint32_t *out_a, int64_t *out_b, uint64_t cap, uint64_t *count) {
   GH_TRY(fill_u64(ctx, ctx->d_counters, 0, 1));
-  if (n) {
+  if (n && mode != 2 && (h->unique || mode == 1)) {  // at most one output per probe row
+    constexpr int R = 8;
+    const uint64_t tiles = (n + (uint64_t)kProbeThreads * R - 1) / ((uint64_t)kProbeThreads * R);
+    const uint64_t cap_grid = (uint64_t)ctx->cus * 8;
+    k_probe_unique<R><<<(uint32_t)(tiles < cap_grid ? tiles : cap_grid), kProbeThreads, 0, ctx->stream>>>(
+        dev_hash(h), key->data, key->type, sel, tup, n, emit_index, out_a, out_b, cap,
+        reinterpret_cast<unsigned long long *>(ctx->d_counters));
+    GH_LAUNCH(ctx);
+  } else if (n) {
     k_hash_probe<<<grid_for(ctx, n), kHashThreads, 0, ctx->stream>>>(
         dev_hash(h), key->data, key->type, sel, tup, n, mode, emit_index, h->unique, out_a, out_b,
         cap, reinterpret_cast<unsigned long long *>(ctx->d_counters));

@@ -13,6 +13,12 @@ __global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n) {
   for (; i < n; i += stride) p[i] = v;
 }
 
+__global__ void k_fill_u64_strided(uint64_t *p, uint64_t v, uint64_t n, uint64_t step) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i * step] = v;
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -28,6 +34,13 @@ static uint32_t fill_grid(ghive_ctx *ctx, uint64_t n) {
 int fill_u64(ghive_ctx *ctx, uint64_t *p, uint64_t value, uint64_t n) {
   if (!n) return GHIVE_OK;
   k_fill_u64<<<fill_grid(ctx, n), 256, 0, ctx->stream>>>(p, value, n);
+  GH_LAUNCH(ctx);
+  return GHIVE_OK;
+}
+
+int fill_u64_strided(ghive_ctx *ctx, uint64_t *p, uint64_t value, uint64_t n, uint64_t step) {
+  if (!n) return GHIVE_OK;
+  k_fill_u64_strided<<<fill_grid(ctx, n), 256, 0, ctx->stream>>>(p, value, n, step);
   GH_LAUNCH(ctx);
   return GHIVE_OK;
 }

@@ -4,7 +4,9 @@
 // the most significant bits; DESC inverts its field; NaN is canonical and sorts after +inf), then
 // sorted 8 bits at a time together with the row id. Digits that are equal in every row are
 // skipped. Each pass: per-tile LDS histogram -> scan of the digit x tile matrix -> stable scatter
-// ranked with wave64 ballots. Equal keys keep ascending row ids, so the result is a total order.
+// ranked with wave64 ballots and staged through LDS in digit order, so that each digit's run is
+// written with consecutive addresses. Equal keys keep ascending row ids, so the result is a total
+// order.
 #include <cstring>
 #include <vector>
 
@@ -92,9 +94,10 @@ __global__ void k_sort_keys(SortKeySpec S, uint64_t n, uint64_t *keys, uint32_t 
   }
 }
 
-// hist[digit * nblocks + block]
+// hist[digit * nblocks + block] (scanned afterwards into global offsets) and the same counts
+// block-major, counts[block * 256 + digit] (a tile's local offsets in the scatter).
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t *keys, uint64_t n, int word, int shift,
-                                                             uint32_t *hist, uint32_t nblocks) {
+                                                             uint32_t *hist, uint32_t *counts, uint32_t nblocks) {
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
@@ -106,38 +109,56 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t *key
   }
   __syncthreads();
   hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+  counts[(uint64_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];
 }
 
+// One stable pass over a tile of kSortTile elements. (1) Every element's position within the tile
+// in digit order: wave64 ballots give the rank among equal digits of a wave, LDS counters carry it
+// across waves and iterations. (2) The tile is rewritten through a 32 KiB LDS buffer, one key
+// word (then the row ids) at a time, in digit order, so each digit's run leaves as consecutive
+// addresses (coalesced stores) at its global offset.
 template <int KW>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *keys_in, const uint32_t *idx_in,
                                                                 uint64_t *keys_out, uint32_t *idx_out, uint64_t n,
                                                                 int word, int shift, const uint32_t *offs,
-                                                                uint32_t nblocks) {
-  __shared__ uint32_t run[256];
+                                                                const uint32_t *counts, uint32_t nblocks) {
+  __shared__ uint64_t stage[kSortTile];
+  __shared__ uint8_t sdigit[kSortTile];
+  __shared__ uint32_t lbase[256], goff[256], run[256];
   __shared__ uint32_t wcnt[kSortWaves][256];
-  const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
-  run[threadIdx.x] = offs[(uint64_t)threadIdx.x * nblocks + blockIdx.x];
+  __shared__ uint32_t scan_tmp[kSortWaves];
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
   const uint64_t t0 = (uint64_t)blockIdx.x * kSortTile;
-  const uint64_t lt = (1ull << lane) - 1ull;
+  const uint32_t tn = (uint32_t)(n - t0 < kSortTile ? n - t0 : kSortTile);
+  uint32_t total;
+  const uint32_t lb = block_exclusive_scan_256(counts[(uint64_t)blockIdx.x * 256 + tid], scan_tmp, total);
+  lbase[tid] = lb;
+  run[tid] = lb;
+  goff[tid] = offs[(uint64_t)tid * nblocks + blockIdx.x];
+  // all loads of the tile up front; past the end a lane re-reads the last element (ignored)
+  uint64_t kv[kSortPerThread][KW];
+  uint32_t id[kSortPerThread], lpos[kSortPerThread];
+#pragma unroll
   for (int j = 0; j < kSortPerThread; ++j) {
-    const uint64_t e = t0 + (uint64_t)j * kSortThreads + threadIdx.x;
-    const bool valid = e < n;
-    uint64_t kv[KW];
+    const uint32_t el = (uint32_t)j * kSortThreads + tid;
+    const uint64_t e = t0 + (el < tn ? el : tn - 1);
 #pragma unroll
-    for (int q = 0; q < KW; ++q) kv[q] = 0;
-    uint32_t id = 0, digit = 0;
-    if (valid) {
+    for (int q = 0; q < KW; ++q) kv[j][q] = keys_in[(uint64_t)q * n + e];
+    id[j] = idx_in[e];
+  }
+  __syncthreads();
+  const uint64_t lt = (1ull << lane) - 1ull;
 #pragma unroll
-      for (int q = 0; q < KW; ++q) kv[q] = keys_in[(uint64_t)q * n + e];
-      id = idx_in[e];
+  for (int j = 0; j < kSortPerThread; ++j) {
+    const bool valid = (uint32_t)j * kSortThreads + tid < tn;
+    uint32_t digit = 0;
 #pragma unroll
-      for (int q = 0; q < KW; ++q)
-        if (q == word) digit = (uint32_t)(kv[q] >> shift) & 255u;
-    }
+    for (int q = 0; q < KW; ++q)
+      if (q == word) digit = (uint32_t)(kv[j][q] >> shift) & 255u;
     uint64_t peers = ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-      uint64_t bb = ballot((digit >> b) & 1u);
+      const uint64_t bb = ballot((digit >> b) & 1u);
       peers &= ((digit >> b) & 1u) ? bb : ~bb;
     }
     const uint32_t rank = (uint32_t)__popcll(peers & lt);
@@ -146,19 +167,36 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *
     __syncthreads();
     if (valid && rank == 0) wcnt[wave][digit] = (uint32_t)__popcll(peers);
     __syncthreads();
+    lpos[j] = 0;
     if (valid) {
       uint32_t pos = run[digit] + rank;
       for (uint32_t w = 0; w < wave; ++w) pos += wcnt[w][digit];
-#pragma unroll
-      for (int q = 0; q < KW; ++q) keys_out[(uint64_t)q * n + pos] = kv[q];
-      idx_out[pos] = id;
+      lpos[j] = pos;
+      sdigit[pos] = (uint8_t)digit;
     }
     __syncthreads();
     uint32_t add = 0;
 #pragma unroll
-    for (int w = 0; w < kSortWaves; ++w) add += wcnt[w][threadIdx.x];
-    run[threadIdx.x] += add;
+    for (int w = 0; w < kSortWaves; ++w) add += wcnt[w][tid];
+    run[tid] += add;
+    __syncthreads();  // every wave has read wcnt and run before the next iteration rewrites them
+  }
+  // write-out, one word at a time through the stage buffer
+#pragma unroll
+  for (int q = 0; q <= KW; ++q) {
     __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kSortPerThread; ++j)
+      if ((uint32_t)j * kSortThreads + tid < tn) stage[lpos[j]] = q < KW ? kv[j][q] : (uint64_t)id[j];
+    __syncthreads();
+    for (uint32_t p = tid; p < tn; p += kSortThreads) {
+      const uint32_t d = sdigit[p];
+      const uint64_t g = (uint64_t)goff[d] + (p - lbase[d]);
+      if (q < KW)
+        keys_out[(uint64_t)q * n + g] = stage[p];
+      else
+        idx_out[g] = (uint32_t)stage[p];
+    }
   }
 }
 
@@ -166,17 +204,20 @@ template <int KW>
 static int radix_passes(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, uint32_t *i0, uint32_t *i1,
                         const std::vector<int> &digits, uint32_t **result) {
   const uint32_t nblocks = ceil_div(n, kSortTile);
-  DevBuf hist;
+  DevBuf hist, counts;
   GH_TRY(hist.alloc(ctx, sizeof(uint32_t) * 256ull * nblocks));
+  GH_TRY(counts.alloc(ctx, sizeof(uint32_t) * 256ull * nblocks));
   uint64_t *kin = k0, *kout = k1;
   uint32_t *iin = i0, *iout = i1;
   for (int d : digits) {
     const int word = d / 8, shift = 8 * (d % 8);
-    k_radix_hist<<<nblocks, kSortThreads, 0, ctx->stream>>>(kin, n, word, shift, hist.as<uint32_t>(), nblocks);
+    k_radix_hist<<<nblocks, kSortThreads, 0, ctx->stream>>>(kin, n, word, shift, hist.as<uint32_t>(),
+                                                            counts.as<uint32_t>(), nblocks);
     GH_LAUNCH(ctx);
     GH_TRY(exclusive_scan_u32(ctx, hist.as<uint32_t>(), hist.as<uint32_t>(), 256ull * nblocks, nullptr));
     k_radix_scatter<KW><<<nblocks, kSortThreads, 0, ctx->stream>>>(kin, iin, kout, iout, n, word, shift,
-                                                                   hist.as<uint32_t>(), nblocks);
+                                                                   hist.as<uint32_t>(), counts.as<uint32_t>(),
+                                                                   nblocks);
     GH_LAUNCH(ctx);
     std::swap(kin, kout);
     std::swap(iin, iout);
