@@ -97,6 +97,9 @@ SIGNATURES_EXTRA = {
     "ghive_gen_zipf_keys": (C.c_int, [P, u64, u32, u64, u64, u64, C.c_double, C.c_double, P]),
     "ghive_join_result": (C.c_int, [P, C.POINTER(abi.Table), u32, C.POINTER(abi.JoinCond), u32, P, P, P, u64,
                                     C.POINTER(abi.Column), u32, C.POINTER(u32)]),
+    "ghive_plan_parse": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, u64, C.POINTER(P)]),
+    "ghive_plan_describe": (C.c_int, [P, C.c_char_p, u64, PU64]),
+    "ghive_plan_free": (None, [P]),
     "ghive_server_create": (C.c_int, [C.c_int, C.POINTER(abi.ServerConfig), C.POINTER(P)]),
     "ghive_server_submit": (C.c_int, [P, C.POINTER(abi.Request), PU64]),
     "ghive_server_wait": (C.c_int, [P, u64, C.POINTER(abi.Response)]),

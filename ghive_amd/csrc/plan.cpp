@@ -1,0 +1,745 @@
+// Query plans: Hive's EXPLAIN (formatted) and EXPLAIN EXTENDED texts of one query, parsed into the
+// operator tree of every Tez vertex with each operator's resolved descriptor -- the host-side
+// front end of the reference (gpu_impl/src/Parser/Parser.cu:58-677) and of its operators'
+// parseExtended (JoinOperator.cu:21-167, GroupByOperator.cu:16-99, SinkOperator.cu:17-96).
+//
+// The concise plan gives the tree (indentation; "<-" marks an input edge), operator ids and the
+// short attributes (Output:[...], predicate:, Conds:, aggregations:/keys:, PartitionCols:). The
+// extended plan gives, per vertex, the operators from the table scan (or the reduce input) up to
+// the vertex's sink, each with its long attributes (expressions:, condition map:, keys:,
+// outputColumnNames:, key expressions:, sort order:); block k counted from the sink belongs to
+// the operator at depth k of the vertex's chain. Column names resolve to positions in the child
+// operator's output columns, as the reference does; a reduce sink lists its key columns first.
+// Differences from the reference, each following Hive: KEY._colN / VALUE._colN take the whole
+// number N (the reference's greedy regex keeps only the last digit); a VALUE column of a
+// group-by over a reduce input is offset by that sink's key count at parse time (the reference
+// reads the key count of the child's runtime result).
+#include <cstring>
+#include <map>
+#include <regex>
+#include <set>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/ghive/ghive.h"
+
+namespace {
+
+struct Op {
+  std::string id;      // FIL_25, RS_30, MAPJOIN_27, TS_0, Union 2, ...
+  std::string kind;    // TableScan Filter Select MapJoin MergeJoin GroupBy ReduceSink FileSink Limit Union PTF
+  std::string vertex;  // the vertex the operator runs in
+  int padding = 0;
+  std::vector<int> children;
+  int parent = -1;
+  std::vector<std::string> outputs;           // output column names (a sink: keys first)
+  std::vector<std::string> original_outputs;  // a sink: before the keys-first reordering
+  std::string table;                          // TableScan: db@table
+  std::string predicate;                      // Filter
+  std::string conds;                          // joins (concise Conds:)
+  std::vector<std::string> agg_strs, key_strs, partition_cols;
+  int limit = -1;
+  std::string extended;  // the linked EXPLAIN EXTENDED block
+  // resolved descriptors
+  std::vector<std::string> expressions;  // Select: one expression per output column
+  std::vector<std::string> join_inputs;  // join: input operator ids in table order
+  std::vector<std::vector<int>> join_cols;
+  std::vector<std::vector<int>> join_types;  // {type (GHIVE join type), left, right}
+  std::vector<std::vector<int>> maintain_cols;
+  std::vector<int> key_offsets;              // GroupBy keys / sink key expressions
+  std::vector<std::vector<int>> agg_descs;   // GroupBy: {AggregationType, column}
+  std::vector<std::string> key_expressions;  // sink
+  std::string sort_orders;                   // sink: '+' / '-' per key
+  std::string gby_mode;
+  bool require_sort = false;                 // the plan's root sink: the final ORDER BY
+};
+
+std::string trim(const std::string &s) {
+  const size_t a = s.find_first_not_of(" \t\r"), b = s.find_last_not_of(" \t\r");
+  return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+}
+
+std::vector<std::string> split_lines(const std::string &s) {
+  std::vector<std::string> out;
+  std::string line;
+  std::istringstream in(s);
+  while (std::getline(in, line)) {
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    out.push_back(line);
+  }
+  return out;
+}
+
+// Splits on `sep` at nesting depth 0 of (), [] and quotes; trims and unquotes each part.
+std::vector<std::string> split_list(const std::string &s, char sep = ',') {
+  std::vector<std::string> out;
+  std::string cur;
+  int depth = 0;
+  bool quoted = false;
+  for (char c : s) {
+    if (c == '"') quoted = !quoted;
+    if (!quoted) {
+      if (c == '(' || c == '[') ++depth;
+      if (c == ')' || c == ']') --depth;
+      if (c == sep && depth == 0) {
+        out.push_back(cur);
+        cur.clear();
+        continue;
+      }
+    }
+    cur += c;
+  }
+  if (!trim(cur).empty() || !out.empty()) out.push_back(cur);
+  for (auto &p : out) {
+    p = trim(p);
+    if (p.size() >= 2 && p.front() == '"' && p.back() == '"') p = p.substr(1, p.size() - 2);
+  }
+  return out;
+}
+
+// "expr (type: t)" -> "expr"
+std::string strip_type(const std::string &s) {
+  static const std::regex re(R"(^(.*) \(type: [^()]*\)$)");
+  std::smatch m;
+  const std::string t = trim(s);
+  return std::regex_match(t, m, re) ? trim(m[1].str()) : t;
+}
+
+int index_of(const std::vector<std::string> &v, const std::string &x) {
+  for (size_t i = 0; i < v.size(); ++i)
+    if (v[i] == x) return (int)i;
+  return -1;
+}
+
+// Minimal JSON reader for {"op": {"tag": ["col", ...], ...}, ...} (Gson of the Java side's
+// Map<String, Map<Byte, List<String>>>).
+struct Json {
+  const char *p;
+  std::string err;
+  void ws() {
+    while (*p == ' ' || *p == '\n' || *p == '\t' || *p == '\r') ++p;
+  }
+  bool str(std::string &out) {
+    ws();
+    if (*p != '"') return false;
+    ++p;
+    out.clear();
+    while (*p && *p != '"') {
+      if (*p == '\\' && p[1]) ++p;
+      out += *p++;
+    }
+    if (*p != '"') return false;
+    ++p;
+    return true;
+  }
+  bool lit(char c) {
+    ws();
+    if (*p != c) return false;
+    ++p;
+    return true;
+  }
+  bool map3(std::map<std::string, std::map<int, std::vector<std::string>>> &out) {
+    if (!lit('{')) return false;
+    if (lit('}')) return true;
+    do {
+      std::string op;
+      if (!str(op) || !lit(':') || !lit('{')) return false;
+      if (!lit('}')) {
+        do {
+          std::string tag;
+          if (!str(tag) || !lit(':') || !lit('[')) return false;
+          std::vector<std::string> cols;
+          if (!lit(']')) {
+            do {
+              std::string c;
+              if (!str(c)) return false;
+              cols.push_back(c);
+            } while (lit(','));
+            if (!lit(']')) return false;
+          }
+          out[op][atoi(tag.c_str())] = cols;
+        } while (lit(','));
+        if (!lit('}')) return false;
+      }
+    } while (lit(','));
+    return lit('}');
+  }
+};
+
+}  // namespace
+
+struct ghive_plan {
+  std::vector<Op> ops;
+  std::map<std::string, int> by_id;
+  std::map<std::string, int> vertex_root;  // vertex -> its sink
+  std::vector<std::string> vertices;       // in order of appearance
+  int root = -1;
+  std::map<std::string, std::map<int, std::vector<std::string>>> maintain;
+};
+
+namespace {
+
+int add_op(ghive_plan &P, Op op) {
+  P.ops.push_back(std::move(op));
+  const int i = (int)P.ops.size() - 1;
+  P.by_id[P.ops[i].id] = i;
+  return i;
+}
+
+bool is_sink(const Op &o) { return o.kind == "ReduceSink" || o.kind == "FileSink" || o.kind == "Union"; }
+
+// --- the concise plan: the operator tree ----------------------------------------------------
+int parse_tree(ghive_plan &P, const std::string &text, std::string &err) {
+  static const std::regex vertex_re(R"(^(Map|Reducer) (\d+)( \[[A-Z_]+\])?( vectorized)?( llap)?$)");
+  static const std::regex id_re(R"(\[([A-Z]+_\d+)\])");
+  static const std::regex union_re(R"(^(Union \d+))");
+  static const std::regex refer_vertex_re(R"(Please refer to the previous ((Map|Reducer) \d+))");
+  static const std::regex refer_op_re(R"(Please refer to the previous .* \[([A-Z]+_\d+)\])");
+  static const std::regex output_re(R"(Output:\[(.*?)\](,|$))");
+  const std::vector<std::string> lines = split_lines(text);
+  size_t i = 0;
+  // the tree starts at the first indented vertex header (after "Stage-N")
+  for (; i < lines.size(); ++i) {
+    const std::string t = trim(lines[i]);
+    if (lines[i].find_first_not_of(' ') > 0 && std::regex_match(t, vertex_re)) break;
+  }
+  if (i == lines.size()) {
+    err = "plan: no vertex in the EXPLAIN text";
+    return GHIVE_ERR_INVALID;
+  }
+  std::vector<int> stack;
+  for (; i < lines.size(); ++i) {
+    const std::string &raw = lines[i];
+    std::string t = trim(raw);
+    if (t.empty()) continue;
+    int padding = (int)raw.find_first_not_of(' ');
+    if (t.rfind("<-", 0) == 0) {
+      padding += 2;
+      t = trim(t.substr(2));
+    }
+    std::smatch m;
+    int node = -1;
+    if (std::regex_search(t, m, refer_op_re)) {
+      auto it = P.by_id.find(m[1].str());
+      if (it == P.by_id.end()) {
+        err = "plan: reference to unknown operator " + m[1].str();
+        return GHIVE_ERR_INVALID;
+      }
+      node = it->second;
+    } else if (std::regex_search(t, m, refer_vertex_re)) {
+      auto it = P.vertex_root.find(m[1].str());
+      if (it == P.vertex_root.end()) {
+        err = "plan: reference to unknown vertex " + m[1].str();
+        return GHIVE_ERR_INVALID;
+      }
+      node = it->second;
+    } else if (std::regex_match(t, m, vertex_re)) {
+      const std::string vertex = m[1].str() + " " + m[2].str();
+      if (i + 1 >= lines.size()) break;
+      const std::string s = trim(lines[++i]);
+      std::smatch ms;
+      if (!std::regex_search(s, ms, id_re)) {
+        err = "plan: vertex " + vertex + " has no sink operator";
+        return GHIVE_ERR_INVALID;
+      }
+      Op o;
+      o.id = ms[1].str();
+      o.kind = o.id.rfind("FS_", 0) == 0 ? "FileSink" : "ReduceSink";
+      o.vertex = vertex;
+      if (i + 1 < lines.size()) {
+        const std::string nx = trim(lines[i + 1]);
+        if (nx.rfind("PartitionCols:", 0) == 0) {
+          o.partition_cols = split_list(nx.substr(14));
+          ++i;
+        }
+      }
+      node = add_op(P, o);
+      P.vertex_root[vertex] = node;
+      P.vertices.push_back(vertex);
+    } else if (std::regex_search(t, m, union_re)) {
+      Op o;
+      o.id = m[1].str();
+      o.kind = "Union";
+      o.outputs = {"_col0"};
+      node = add_op(P, o);
+    } else if (std::regex_search(t, m, id_re)) {
+      Op o;
+      o.id = m[1].str();
+      const std::string attr = i + 1 < lines.size() ? trim(lines[i + 1]) : std::string();
+      std::smatch ma;
+      const std::string pre = o.id.substr(0, o.id.find('_'));
+      if (pre == "TS") {
+        o.kind = "TableScan";
+        static const std::regex ts_re(R"(^(\S+?),)");
+        if (std::regex_search(attr, ma, ts_re)) o.table = ma[1].str();
+        if (std::regex_search(attr, ma, output_re)) o.outputs = split_list(ma[1].str());
+        ++i;
+      } else if (pre == "FIL") {
+        o.kind = "Filter";
+        if (attr.rfind("predicate:", 0) == 0) {
+          o.predicate = trim(attr.substr(10));
+          ++i;
+        }
+      } else if (pre == "SEL") {
+        o.kind = "Select";
+        if (std::regex_search(attr, ma, output_re)) {
+          o.outputs = split_list(ma[1].str());
+          ++i;
+        }
+      } else if (pre == "GBY") {
+        o.kind = "GroupBy";
+        static const std::regex aggs_re(R"(aggregations:\[(.*?)\](,|$))");
+        static const std::regex keys_re(R"(keys:(.*)$)");
+        if (std::regex_search(attr, ma, output_re)) o.outputs = split_list(ma[1].str());
+        if (std::regex_search(attr, ma, aggs_re)) o.agg_strs = split_list(ma[1].str());
+        if (std::regex_search(attr, ma, keys_re)) o.key_strs = split_list(ma[1].str());
+        ++i;
+      } else if (pre == "MAPJOIN" || pre == "MERGEJOIN") {
+        o.kind = pre == "MAPJOIN" ? "MapJoin" : "MergeJoin";
+        static const std::regex conds_re(R"(Conds:(.*?),(HybridGraceHashJoin|Output))");
+        if (std::regex_search(attr, ma, conds_re)) o.conds = ma[1].str();
+        if (std::regex_search(attr, ma, output_re)) o.outputs = split_list(ma[1].str());
+        ++i;
+      } else if (pre == "LIM") {
+        o.kind = "Limit";
+        static const std::regex lim_re(R"(Number of rows:(\d+))");
+        if (std::regex_search(attr, ma, lim_re)) {
+          o.limit = std::stoi(ma[1].str());
+          ++i;
+        }
+      } else if (pre == "PTF") {
+        o.kind = "PTF";
+        ++i;
+      } else if (pre == "RS" || pre == "FS") {  // a sink met without its vertex header
+        o.kind = pre == "FS" ? "FileSink" : "ReduceSink";
+      } else {
+        o.kind = pre;
+      }
+      node = add_op(P, o);
+    } else {
+      continue;  // an attribute line of an operator already handled, or text we do not use
+    }
+    // attach by indentation
+    while (!stack.empty() && P.ops[stack.back()].padding >= padding) stack.pop_back();
+    if (!stack.empty()) {
+      const int parent = stack.back();
+      P.ops[parent].children.push_back(node);
+      if (P.ops[node].parent < 0) P.ops[node].parent = parent;
+    } else if (P.root < 0) {
+      P.root = node;
+    }
+    // a referenced node keeps its first position; only new nodes become the stack top
+    P.ops[node].padding = padding;
+    stack.push_back(node);
+  }
+  if (P.root < 0) {
+    err = "plan: empty operator tree";
+    return GHIVE_ERR_INVALID;
+  }
+  return GHIVE_OK;
+}
+
+// vertex of every operator: the vertex of the nearest sink above it within its own chain
+void assign_vertices(ghive_plan &P, int node, const std::string &vertex, std::set<int> &seen) {
+  if (seen.count(node)) return;
+  seen.insert(node);
+  Op &o = P.ops[node];
+  if (o.vertex.empty() || !is_sink(o)) {
+    if (o.vertex.empty()) o.vertex = vertex;
+  }
+  for (int c : o.children) {
+    const Op &ch = P.ops[c];
+    if (ch.kind == "ReduceSink" || ch.kind == "FileSink")
+      assign_vertices(P, c, ch.vertex, seen);
+    else
+      assign_vertices(P, c, o.vertex, seen);
+  }
+}
+
+// output columns that pass-through operators take from their first child (sinks, filters,
+// limits), bottom-up
+void propagate_outputs(ghive_plan &P, int node, std::set<int> &done) {
+  if (done.count(node)) return;
+  done.insert(node);
+  for (int c : P.ops[node].children) propagate_outputs(P, c, done);
+  Op &o = P.ops[node];
+  if ((o.kind == "ReduceSink" || o.kind == "FileSink" || o.kind == "Filter" || o.kind == "Limit") &&
+      o.outputs.empty() && !o.children.empty())
+    o.outputs = P.ops[o.children[0]].outputs;
+}
+
+// --- the extended plan: per-vertex operator blocks ------------------------------------------
+std::map<std::string, std::vector<std::string>> extended_blocks(const std::string &text) {
+  static const std::regex vertex_re(R"(^(Map|Reducer) \d+$)");
+  static const std::regex op_re(R"(^[a-zA-Z]+( [a-zA-Z]+)? Operator$)");
+  std::map<std::string, std::vector<std::string>> out;
+  const std::vector<std::string> lines = split_lines(text);
+  int vpad = -1;
+  std::string vertex;
+  std::string cur;
+  bool in_op = false;
+  auto flush = [&]() {
+    if (in_op && !vertex.empty()) out[vertex].push_back(cur);
+    cur.clear();
+    in_op = false;
+  };
+  for (const std::string &raw : lines) {
+    const std::string t = trim(raw);
+    const int pad = (int)raw.find_first_not_of(' ');
+    if (t == "Vertices:") {
+      vpad = pad;
+      continue;
+    }
+    if (vpad < 0) continue;
+    if (!t.empty() && pad <= vpad) {  // the Vertices section ended
+      flush();
+      vertex.clear();
+      vpad = -1;
+      continue;
+    }
+    if (pad == vpad + 2 && std::regex_match(t, vertex_re)) {
+      flush();
+      vertex = t;
+      continue;
+    }
+    if (std::regex_match(t, op_re) || t == "TableScan" || t == "Limit") {
+      flush();
+      in_op = true;
+    }
+    if (in_op) cur += raw + "\n";
+  }
+  flush();
+  return out;
+}
+
+void link_extended(ghive_plan &P, const std::map<std::string, std::vector<std::string>> &blocks) {
+  for (const auto &v : P.vertex_root) {
+    auto it = blocks.find(v.first);
+    if (it == blocks.end()) continue;
+    const std::vector<std::string> &b = it->second;
+    // the vertex's chain: from its sink down through the children that are not sinks
+    int node = v.second, depth = 0;
+    while (node >= 0 && depth < (int)b.size()) {
+      P.ops[node].extended = b[b.size() - 1 - depth];
+      int next = -1;
+      for (int c : P.ops[node].children)
+        if (!is_sink(P.ops[c]) && next < 0) next = c;
+      node = next;
+      ++depth;
+    }
+  }
+}
+
+std::vector<std::string> attr_list(const std::string &block, const std::string &name) {
+  for (const std::string &l : split_lines(block)) {
+    const std::string t = trim(l);
+    if (t.rfind(name + ":", 0) == 0) return split_list(t.substr(name.size() + 1));
+  }
+  return {};
+}
+
+std::string attr_value(const std::string &block, const std::string &name) {
+  for (const std::string &l : split_lines(block)) {
+    const std::string t = trim(l);
+    if (t.rfind(name + ":", 0) == 0) return trim(t.substr(name.size() + 1));
+  }
+  return std::string();
+}
+
+// --- parseExtended per operator -------------------------------------------------------------
+void resolve_sink(ghive_plan &P, Op &o) {
+  if (!o.original_outputs.empty() || o.kind == "Union") return;  // done already
+  o.original_outputs = o.outputs;
+  if (o.extended.empty()) return;
+  const std::vector<std::string> child_out = o.children.empty() ? o.outputs : P.ops[o.children[0]].outputs;
+  for (const std::string &e : attr_list(o.extended, "key expressions")) {
+    const std::string x = strip_type(e);
+    o.key_expressions.push_back(x);
+    const int k = index_of(child_out, x);
+    if (k >= 0) o.key_offsets.push_back(k);
+  }
+  const std::string so = attr_value(o.extended, "sort order");
+  o.sort_orders = so;
+  std::vector<std::string> reordered = o.key_expressions;
+  for (const std::string &c : o.outputs)
+    if (index_of(o.key_expressions, c) < 0) reordered.push_back(c);
+  o.outputs = reordered;
+}
+
+int agg_type_of(const std::string &name) {
+  if (name == "sum") return GHIVE_AGG_SUM;
+  if (name == "max") return GHIVE_AGG_MAX;
+  if (name == "min") return GHIVE_AGG_MIN;
+  if (name == "avg") return GHIVE_AGG_AVG;
+  if (name == "count") return GHIVE_AGG_CNT;
+  return GHIVE_AGG_UNKNOWN;
+}
+
+void resolve_groupby(ghive_plan &P, Op &o) {
+  static const std::regex key_re(R"(^KEY\._col(\d+)$)"), value_re(R"(^VALUE\._col(\d+)$)");
+  static const std::regex agg_re(R"(^(\w+)\((.*)\)$)");
+  if (o.children.empty()) return;
+  const Op &child = P.ops[o.children[0]];
+  const bool reduce_input = child.kind == "ReduceSink" || child.kind == "Union";
+  const int child_keys = child.kind == "ReduceSink" ? (int)child.key_expressions.size() : (int)o.key_strs.size();
+  auto col_of = [&](const std::string &name, int value_base) -> int {
+    std::smatch m;
+    if (reduce_input && std::regex_match(name, m, key_re)) return std::stoi(m[1].str());
+    if (reduce_input && std::regex_match(name, m, value_re)) return std::stoi(m[1].str()) + value_base;
+    return index_of(child.outputs, name);
+  };
+  for (const std::string &k : o.key_strs) {
+    const int c = col_of(strip_type(k), (int)o.key_strs.size());
+    if (c >= 0) o.key_offsets.push_back(c);
+  }
+  for (const std::string &a : o.agg_strs) {
+    std::smatch m;
+    if (!std::regex_match(a, m, agg_re)) continue;
+    const std::string arg = trim(m[2].str());
+    int c = arg.empty() ? -1 : col_of(arg, child_keys);
+    if (c < 0) c = 0;  // count() / an argument that is not a column: column 0, as the reference
+    o.agg_descs.push_back({agg_type_of(m[1].str()), c});
+  }
+  o.gby_mode = attr_value(o.extended, "mode");
+}
+
+void resolve_select(Op &o) {
+  if (o.extended.empty()) return;
+  o.expressions.clear();
+  for (const std::string &e : attr_list(o.extended, "expressions")) o.expressions.push_back(strip_type(e));
+  const std::vector<std::string> names = attr_list(o.extended, "outputColumnNames");
+  if (!names.empty()) o.outputs = names;
+}
+
+void resolve_join(ghive_plan &P, Op &o) {
+  static const std::regex name_re(R"([A-Z]+_\d+)");
+  static const std::regex type_re(R"((Inner|Left Outer|Left Semi|Right Outer|Full Outer) Join (\d+) to (\d+))");
+  static const std::regex key_line_re(R"(^(\d+) (.+)$)");
+  // input tables in the order the conditions name them
+  for (auto it = std::sregex_iterator(o.conds.begin(), o.conds.end(), name_re); it != std::sregex_iterator(); ++it) {
+    const std::string n = it->str();
+    if (index_of(o.join_inputs, n) >= 0) break;
+    o.join_inputs.push_back(n);
+  }
+  std::vector<int> inputs;
+  for (const std::string &n : o.join_inputs) {
+    int found = -1;
+    for (int c : o.children)
+      if (P.ops[c].id == n) found = c;
+    inputs.push_back(found);
+  }
+  o.join_cols.assign(inputs.size(), {});
+  o.maintain_cols.assign(inputs.size(), {});
+  bool in_keys = false;
+  for (const std::string &l : split_lines(o.extended)) {
+    const std::string t = trim(l);
+    std::smatch m;
+    if (t.rfind("outputColumnNames:", 0) == 0) {
+      o.outputs = split_list(t.substr(18));
+      break;
+    }
+    if (std::regex_search(t, m, type_re)) {
+      const std::string k = m[1].str();
+      const int type = k == "Inner" ? GHIVE_INNER_JOIN
+                       : k == "Left Outer" ? GHIVE_LEFT_OUTER_JOIN
+                       : k == "Left Semi" ? GHIVE_LEFT_SEMI_JOIN
+                       : k == "Right Outer" ? GHIVE_RIGHT_OUTER_JOIN
+                                            : GHIVE_FULL_OUTER_JOIN;
+      o.join_types.push_back({type, std::stoi(m[2].str()), std::stoi(m[3].str())});
+      continue;
+    }
+    if (t == "keys:") {
+      in_keys = true;
+      continue;
+    }
+    if (in_keys && std::regex_match(t, m, key_line_re)) {
+      const int tbl = std::stoi(m[1].str());
+      if (tbl < 0 || tbl >= (int)inputs.size() || inputs[tbl] < 0) continue;
+      for (const std::string &c : split_list(m[2].str())) {
+        const int k = index_of(P.ops[inputs[tbl]].outputs, strip_type(c));
+        if (k >= 0) o.join_cols[tbl].push_back(k);
+      }
+    }
+  }
+  // maintained columns from the Java side's map (JoinOperator.cu:130-162)
+  auto mit = P.maintain.find(o.id);
+  if (mit == P.maintain.end()) return;
+  static const std::regex value_re(R"(^VALUE\._col(\d+)$)");
+  for (const auto &tag : mit->second) {
+    if (tag.first < 0 || tag.first >= (int)inputs.size() || inputs[tag.first] < 0) continue;
+    const Op &in = P.ops[inputs[tag.first]];
+    for (const std::string &c : tag.second) {
+      int k = -1;
+      std::smatch m;
+      if (o.kind == "MergeJoin" && std::regex_match(c, m, value_re)) {
+        // a reduce-side input: VALUE._colN is the sink's column _col(N+1) (its key is _col0)
+        k = index_of(in.original_outputs, "_col" + std::to_string(std::stoi(m[1].str()) + 1));
+      } else {
+        k = index_of(in.outputs, c);
+      }
+      if (k >= 0) o.maintain_cols[tag.first].push_back(k);
+    }
+  }
+}
+
+void resolve(ghive_plan &P, int node, std::set<int> &done) {
+  if (done.count(node)) return;
+  done.insert(node);
+  for (int c : P.ops[node].children) resolve(P, c, done);
+  Op &o = P.ops[node];
+  if (o.kind == "ReduceSink" || o.kind == "FileSink") resolve_sink(P, o);
+  else if (o.kind == "GroupBy") resolve_groupby(P, o);
+  else if (o.kind == "Select") resolve_select(o);
+  else if (o.kind == "MapJoin" || o.kind == "MergeJoin") resolve_join(P, o);
+  else if (o.kind == "Filter") {
+    const std::string pred = attr_value(o.extended, "predicate");
+    if (!pred.empty() && o.predicate.empty()) o.predicate = strip_type(pred);
+  }
+  // pass-through operators above a resolved child take its (possibly reordered) outputs
+  if ((o.kind == "Filter" || o.kind == "Limit") && !o.children.empty()) o.outputs = P.ops[o.children[0]].outputs;
+}
+
+// --- JSON description ---------------------------------------------------------------------
+std::string jstr(const std::string &s) {
+  std::string r = "\"";
+  for (char c : s) {
+    if (c == '"' || c == '\\') r += '\\';
+    if (c == '\n') {
+      r += "\\n";
+      continue;
+    }
+    r += c;
+  }
+  return r + "\"";
+}
+std::string jstrs(const std::vector<std::string> &v) {
+  std::string r = "[";
+  for (size_t i = 0; i < v.size(); ++i) r += (i ? "," : "") + jstr(v[i]);
+  return r + "]";
+}
+std::string jints(const std::vector<int> &v) {
+  std::string r = "[";
+  for (size_t i = 0; i < v.size(); ++i) r += (i ? "," : "") + std::to_string(v[i]);
+  return r + "]";
+}
+std::string jints2(const std::vector<std::vector<int>> &v) {
+  std::string r = "[";
+  for (size_t i = 0; i < v.size(); ++i) r += (i ? "," : "") + jints(v[i]);
+  return r + "]";
+}
+
+std::string describe(const ghive_plan &P) {
+  std::string r = "{\"root\":" + jstr(P.root >= 0 ? P.ops[P.root].id : "") + ",\"vertices\":{";
+  bool first = true;
+  for (const std::string &v : P.vertices) {
+    r += (first ? "" : ",") + jstr(v) + ":" + jstr(P.ops[P.vertex_root.at(v)].id);
+    first = false;
+  }
+  r += "},\"operators\":{";
+  for (size_t i = 0; i < P.ops.size(); ++i) {
+    const Op &o = P.ops[i];
+    std::vector<std::string> kids;
+    for (int c : o.children) kids.push_back(P.ops[c].id);
+    r += (i ? "," : "") + jstr(o.id) + ":{\"kind\":" + jstr(o.kind) + ",\"vertex\":" + jstr(o.vertex) +
+         ",\"children\":" + jstrs(kids) + ",\"outputs\":" + jstrs(o.outputs);
+    if (o.kind == "TableScan") r += ",\"table\":" + jstr(o.table);
+    if (o.kind == "Filter") r += ",\"predicate\":" + jstr(o.predicate);
+    if (o.kind == "Select") r += ",\"expressions\":" + jstrs(o.expressions);
+    if (o.kind == "Limit") r += ",\"limit\":" + std::to_string(o.limit);
+    if (o.kind == "MapJoin" || o.kind == "MergeJoin")
+      r += ",\"conds\":" + jstr(o.conds) + ",\"join_inputs\":" + jstrs(o.join_inputs) +
+           ",\"join_cols\":" + jints2(o.join_cols) + ",\"join_types\":" + jints2(o.join_types) +
+           ",\"maintain_cols\":" + jints2(o.maintain_cols);
+    if (o.kind == "GroupBy")
+      r += ",\"keys\":" + jints(o.key_offsets) + ",\"aggs\":" + jints2(o.agg_descs) + ",\"mode\":" + jstr(o.gby_mode);
+    if (o.kind == "ReduceSink" || o.kind == "FileSink")
+      r += ",\"key_expressions\":" + jstrs(o.key_expressions) + ",\"key_offsets\":" + jints(o.key_offsets) +
+           ",\"sort_orders\":" + jstr(o.sort_orders) + ",\"partition_cols\":" + jstrs(o.partition_cols) +
+           ",\"original_outputs\":" + jstrs(o.original_outputs) +
+           ",\"require_sort\":" + (o.require_sort ? "true" : "false");
+    r += ",\"linked\":" + std::string(o.extended.empty() ? "false" : "true") + "}";
+  }
+  return r + "}}";
+}
+
+void set_err(char *err, uint64_t cap, const std::string &msg) {
+  if (err && cap) {
+    const size_t n = msg.size() < cap - 1 ? msg.size() : (size_t)cap - 1;
+    memcpy(err, msg.data(), n);
+    err[n] = 0;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ghive_plan_parse(const char *plan, const char *extended, const char *maintain_json, char *err, uint64_t errcap,
+                     ghive_plan **out) {
+  if (!plan || !out) {
+    set_err(err, errcap, "plan: NULL argument");
+    return GHIVE_ERR_INVALID;
+  }
+  *out = nullptr;
+  auto *P = new ghive_plan();
+  std::string e;
+  try {
+    if (maintain_json && *maintain_json) {
+      Json j{maintain_json, {}};
+      if (!j.map3(P->maintain)) {
+        delete P;
+        set_err(err, errcap, "plan: malformed maintain-values JSON");
+        return GHIVE_ERR_INVALID;
+      }
+    }
+    int st = parse_tree(*P, plan, e);
+    if (st != GHIVE_OK) {
+      delete P;
+      set_err(err, errcap, e);
+      return st;
+    }
+    std::set<int> seen;
+    assign_vertices(*P, P->root, P->ops[P->root].vertex, seen);
+    std::set<int> done;
+    propagate_outputs(*P, P->root, done);
+    if (extended) link_extended(*P, extended_blocks(extended));
+    std::set<int> resolved;
+    resolve(*P, P->root, resolved);
+    // the root sink performs the final ORDER BY with the order of the sink that feeds its vertex
+    Op &root = P->ops[P->root];
+    for (int node = P->root; node >= 0;) {
+      int next = -1;
+      for (int c : P->ops[node].children) {
+        if (P->ops[c].kind == "ReduceSink") {
+          root.require_sort = true;
+          root.sort_orders = P->ops[c].sort_orders;
+          next = -1;
+          break;
+        }
+        if (next < 0) next = c;
+      }
+      if (root.require_sort) break;
+      node = next;
+    }
+  } catch (const std::exception &ex) {
+    delete P;
+    set_err(err, errcap, std::string("plan: ") + ex.what());
+    return GHIVE_ERR_INVALID;
+  }
+  *out = P;
+  return GHIVE_OK;
+}
+
+int ghive_plan_describe(const ghive_plan *plan, char *buf, uint64_t cap, uint64_t *len) {
+  if (!plan || !len) return GHIVE_ERR_INVALID;
+  const std::string s = describe(*plan);
+  *len = s.size();
+  if (!buf || cap < s.size() + 1) return GHIVE_ERR_CAPACITY;
+  memcpy(buf, s.c_str(), s.size() + 1);
+  return GHIVE_OK;
+}
+
+void ghive_plan_free(ghive_plan *plan) { delete plan; }
+
+}  // extern "C"

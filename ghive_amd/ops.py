@@ -572,3 +572,53 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class Plan:
+    """A query plan (ghive_plan_*): Hive's EXPLAIN and EXPLAIN EXTENDED texts (and the Java side's
+    maintain-values JSON) parsed into every vertex's operator tree with resolved descriptors.
+    describe() returns {"root", "vertices": {vertex: sink id}, "operators": {id: {...}}}."""
+
+    def __init__(self, plan_text, extended_text=None, maintain_json=None):
+        import json
+        err = C.create_string_buffer(512)
+        h = C.c_void_p()
+        st = lib().ghive_plan_parse(plan_text.encode(), extended_text.encode() if extended_text else None,
+                                    maintain_json.encode() if maintain_json else None, err, len(err), C.byref(h))
+        if st != abi.OK:
+            raise GhiveError(st, err.value.decode(errors="replace"))
+        self.h = h
+        n = C.c_uint64()
+        lib().ghive_plan_describe(self.h, None, 0, C.byref(n))
+        buf = C.create_string_buffer(n.value + 1)
+        st = lib().ghive_plan_describe(self.h, buf, n.value + 1, C.byref(n))
+        if st != abi.OK:
+            raise GhiveError(st, "plan describe failed")
+        self.desc = json.loads(buf.value.decode())
+
+    def describe(self):
+        return self.desc
+
+    def op(self, op_id):
+        return self.desc["operators"][op_id]
+
+    def chain(self, vertex):
+        """Operator ids of a vertex from its sink down to its table scan / reduce input."""
+        out, node = [], self.desc["vertices"][vertex]
+        while node is not None:
+            out.append(node)
+            kids = [c for c in self.op(node)["children"]
+                    if self.op(c)["kind"] not in ("ReduceSink", "FileSink", "Union")]
+            node = kids[0] if kids else None
+        return out
+
+    def close(self):
+        if self.h:
+            lib().ghive_plan_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -472,6 +472,24 @@ ghive_ctx *ghive_server_slot_context(ghive_server *srv, uint32_t slot);
 /* Finishes the requests already admitted, drops queued ones, releases the slots. */
 void ghive_server_destroy(ghive_server *srv);
 
+/* ---- query plans: the operator tree of every vertex ------------------------------------- *
+ * Parser::getOperatorTree + linkOperatorTreeAndExtendedPlan (gpu_impl/src/Parser/Parser.cu:
+ * 58-677) and the operators' parseExtended (JoinOperator.cu:21-167, GroupByOperator.cu:16-99,
+ * SinkOperator.cu:17-96): Hive's EXPLAIN text (`plan`, formatted tree) and EXPLAIN EXTENDED text
+ * (`extended`, may be NULL) of one query, plus the Java side's maintain-values JSON (Gson of
+ * Map<String, Map<Byte, List<String>>>: join operator id -> input tag -> column names; may be
+ * NULL), become the operator tree with each operator's resolved descriptor: filter predicate
+ * text, select expressions, join input order / key columns / join types (BaseJoinType numbering)
+ * / maintained columns, group-by key and aggregate columns (AggregationType numbering), sink key
+ * columns and sort orders. Host only; needs no context. On failure `err` receives the message. */
+typedef struct ghive_plan ghive_plan;
+int ghive_plan_parse(const char *plan, const char *extended, const char *maintain_json, char *err,
+                     uint64_t errcap, ghive_plan **out);
+/* The parsed plan as JSON: {"root": id, "vertices": {vertex: sink id}, "operators": {id: {...}}}.
+ * *len = bytes needed (without the terminator); GHIVE_ERR_CAPACITY when cap is too small. */
+int ghive_plan_describe(const ghive_plan *plan, char *buf, uint64_t cap, uint64_t *len);
+void ghive_plan_free(ghive_plan *plan);
+
 /* Per-stage timing of the last fragment on this context (HIP events, milliseconds). */
 typedef struct ghive_stage_times {
   float total_ms;

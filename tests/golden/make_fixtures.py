@@ -1,5 +1,7 @@
 """Writes tests/golden/reference_kats.json: known-answer vectors transcribed from the reference's
-own tests (inputs and expected outputs only -- data, no source text).
+own tests (inputs and expected outputs only -- data, no source text), and copies a few of the
+reference's query-plan data files (Hive EXPLAIN output, gpu_impl/plan/*.txt) into
+tests/golden/plans/ as inputs of the plan-parser tests.
 
 Run in a container where the reference tree is mounted (it is read, never executed):
     python tests/golden/make_fixtures.py /root/reference
@@ -204,8 +206,24 @@ def hive_groupby_kats():
     return out
 
 
+PLAN_FIXTURES = ["ssb_10_orc_transformed_1_1", "ssb_10_orc_transformed_2_1", "ssb_10_orc_transformed_3_1",
+                 "ssb_10_orc_transformed_4_1", "tpcds_orc_100_transformed_query3"]
+
+
+def copy_plans(ref_root):
+    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "plans")
+    os.makedirs(out, exist_ok=True)
+    for name in PLAN_FIXTURES:
+        for suffix in (".txt", "_extended.txt"):
+            with open(os.path.join(ref_root, "gpu_impl", "plan", name + suffix)) as fh:
+                text = fh.read()
+            with open(os.path.join(out, name + suffix), "w") as fh:
+                fh.write(text)
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    copy_plans(ref)
     data = {
         "filter_parse": filter_parse_kats(),
         "join": join_kats(),
