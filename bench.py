@@ -51,6 +51,36 @@ def cpu_baseline(T, sample_rows, threads):
     return sample_rows / secs[1], secs[1]
 
 
+def cpu_batch_baseline(B, threads, max_rows=200_000_000):
+    """Per-fragment CPU restatement (oracle FragmentSpec, one thread per fragment) over a pool of
+    `threads` workers, on the first fragments of the batch up to max_rows fact rows."""
+    import concurrent.futures as cf
+    import oracle
+    from ghive_amd import abi
+    dim = [c.cpu().numpy() for c in B.dim.columns[:3]]
+    take, total = 0, 0
+    while take < len(B.rows) and total + B.rows[take] <= max_rows:
+        total += B.rows[take]
+        take += 1
+    end = B.starts[take - 1] + B.rows[take - 1] if take else 0
+    fk, me, pr = (c[:end].cpu().numpy() for c in (B.fk, B.measure, B.price))
+    specs = []
+    for f in range(take):
+        s, r, c = B.starts[f], B.rows[f], B.attr_const[f]
+        specs.append(oracle.FragmentSpec(
+            [fk[s:s + r], me[s:s + r], pr[s:s + r]],
+            [dict(columns=dim, key_col=0, payload_col=2, fact_key_col=0,
+                  filter=([abi.make_pred(abi.FILTER_EQ, 1, [c], data_type=2)], 0))],
+            keys=[(abi.SRC_JOIN, 0)], aggs=[(abi.AGG_SUM, abi.EXPR_MUL, 1, 2)]))
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda sp: sp.run(nthreads=1), specs))
+    secs = time.perf_counter() - t0
+    return {"value": total / secs, "unit": "fact rows/s", "cores": threads, "kind": "port",
+            "sample": f"first {take} fragments ({total} fact rows), oracle restatement of Hive 3.1's vectorized "
+                      f"map-join + hash group-by, one thread per fragment on a pool of {threads}"}
+
+
 def batched_section(ctx, reps):
     """4096 lognormal fragments of the small query shape: autotuned persistent kernel replayed from
     a HIP graph vs. 4096 plain per-fragment launches; identical outputs are checked."""
@@ -85,6 +115,8 @@ def batched_section(ctx, reps):
     torch.cuda.synchronize()
     plain_s = time.perf_counter() - t1
     identical = plain.run(collect=True) == got
+    cpu = cpu_batch_baseline(B, min(os.cpu_count() or 1, 16))
+    log(f"batched cpu baseline: {cpu['value']:.3e} rows/s ({cpu['sample']})")
     p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
     out = {
         "workload": "4096 map-task fragments, rows lognormal(median 256k, sigma 1.32) clipped to "
@@ -92,7 +124,7 @@ def batched_section(ctx, reps):
         "fragments": len(B.rows), "rows": total, "p50_ms": p50, "p99_ms": p99, "device_ms": dev_ms,
         "rows_per_s": total / (p50 * 1e-3), "plain_ms": 1e3 * plain_s,
         "plain_rows_per_s": total / plain_s, "speedup_vs_plain": plain_s / (p50 * 1e-3),
-        "identical_to_plain": identical, "tuning": best, "tuning_candidates": len(results),
+        "identical_to_plain": identical, "cpu_baseline": cpu, "tuning": best, "tuning_candidates": len(results),
         "tuning_search_s": search_s,
         "roofline": {"achieved": W.SMALL_REFERENCED_BYTES_PER_ROW * total / (dev_ms * 1e-3) / 1e9,
                      "unit": "GB/s", "peak": HBM_PEAK_GBS},
@@ -104,6 +136,40 @@ def batched_section(ctx, reps):
     ht.free()
     del B, plain
     return out
+
+
+def cpu_skew_baseline(seed):
+    """The oracle's restatement of Hive's CPU operators (one thread each) on bounded samples of
+    the config-3 stages: map join of 10M Zipf(1.0) probe rows (10% misses) into 1M build keys;
+    ORDER BY (key, tiebreak) of 5M joined rows; GROUP BY of 10M rows over 1M distinct keys with
+    COUNT(*) and exact SUM(double)."""
+    import numpy as np
+    import torch
+    import oracle
+    from ghive_amd import abi, workloads as W
+    nb, npr, ns, ng = 1_000_000, 10_000_000, 5_000_000, 10_000_000
+    bk = W.key62_torch(seed, torch.arange(1, nb + 1, dtype=torch.int64)).numpy()
+    bp = np.arange(1, nb + 1, dtype=np.int64)
+    pk, _ = oracle.gen_zipf_keys(seed, 0, 0, npr, nb, 1.0, 0.1)
+    t0 = time.perf_counter()
+    rows, _ = oracle.mapjoin_inner(bk, pk, payload=bp)
+    tj = time.perf_counter() - t0
+    tb = oracle.gen_uniform_f32(seed, 1, 0, npr)
+    sel = rows[:ns].astype(np.int64)
+    jk, jt = np.ascontiguousarray(pk[sel]), np.ascontiguousarray(tb[sel])
+    t0 = time.perf_counter()
+    oracle.sort_perm([jk, jt], [0, 1], "++")
+    ts = time.perf_counter() - t0
+    keys = oracle.gen_uniform_i64(seed, 2, 0, ng, 1, 1_000_000)
+    vals = oracle.gen_normal_f64(seed, 3, 0, ng)
+    t0 = time.perf_counter()
+    oracle.groupby([keys, vals], [0], [(abi.AGG_CNT, abi.COUNT_STAR), (abi.AGG_SUM, 1)])
+    tg = time.perf_counter() - t0
+    return {"probe_rows_per_s": npr / tj, "sort_rows_per_s": len(jk) / ts, "groupby_rows_per_s": ng / tg,
+            "cores": 1, "kind": "port",
+            "sample": "oracle (C restatement of Hive 3.1 VectorMapJoinFastLongHashTable, ORDER BY as a total "
+                      "order, hash group-by), one thread: 10M Zipf probe rows x 1M build keys; 5M-row sort; "
+                      "10M rows / 1M keys group-by"}
 
 
 def skew_section(ctx, reps):
@@ -169,6 +235,8 @@ def skew_section(ctx, reps):
         "groupby": {"rows": nrows, "groups": ng[0], "ms": 1e3 * t_gby, "rows_per_s": nrows / t_gby,
                     "gbs": gbs(b_gby, t_gby), "frac": gbs(b_gby, t_gby) / HBM_PEAK_GBS},
     }
+    out["cpu_baseline"] = cpu_skew_baseline(W.SEED_BASE + 3)
+    log(f"skew cpu baseline: {out['cpu_baseline']}")
     log(f"skew: build {1e3 * t_build:.1f} ms, probe {1e3 * t_probe:.1f} ms ({out['probe']['gbs']:.0f} GB/s), "
         f"sort {1e3 * t_sort:.1f} ms of {matches} rows, group-by {1e3 * t_gby:.1f} ms "
         f"({out['groupby']['gbs']:.0f} GB/s, {ng[0]} groups)")
@@ -184,6 +252,40 @@ def _same_result(kind, a, b):
     da = {int(cols_a[0][g]): tuple(int(c[g]) for c in cols_a[1:]) for g in range(len(cols_a[0]))}
     db = {int(cols_b[0][g]): tuple(int(c[g]) for c in cols_b[1:]) for g in range(len(cols_b[0]))}
     return da == db
+
+
+def cpu_serving_baseline(I, reqs, max_sort_rows=16_000_000):
+    """The same requests run one after another on the CPU restatement (oracle, one thread each):
+    throughput only. Sorts above max_sort_rows are left out of the sample (stated)."""
+    import oracle
+    from ghive_amd import abi
+    fact = [c.cpu().numpy() for c in I.small.fact.columns]
+    dim = [c.cpu().numpy() for c in I.small.dim.columns]
+    gk, gv = I.gb_keys.cpu().numpy(), I.gb_vals.cpu().numpy()
+    sk = I.sort_keys[:max_sort_rows].cpu().numpy()
+    rows, secs, done, skipped = 0, 0.0, 0, 0
+    for kind, p in reqs:
+        if kind == "sort" and p["rows"] > max_sort_rows:
+            skipped += 1
+            continue
+        t0 = time.perf_counter()
+        if kind == "fragment":
+            s, r = p["start"], p["rows"]
+            oracle.FragmentSpec([c[s:s + r] for c in fact],
+                                [dict(columns=dim, key_col=0, payload_col=2, fact_key_col=0,
+                                      filter=([abi.make_pred(abi.FILTER_EQ, 1, [p["attr"]], data_type=2)], 0))],
+                                keys=[(abi.SRC_JOIN, 0)], aggs=[(abi.AGG_SUM, abi.EXPR_MUL, 1, 2)]).run(nthreads=1)
+        elif kind == "groupby":
+            n = p["rows"]
+            oracle.groupby([gk[:n], gv[:n]], [0], [(abi.AGG_SUM, 1), (abi.AGG_CNT, abi.COUNT_STAR)])
+        else:
+            oracle.sort_perm([sk[:p["rows"]]], [0], "+")
+        secs += time.perf_counter() - t0
+        rows += p["rows"]
+        done += 1
+    return {"value": rows / secs if secs else None, "unit": "rows/s", "cores": 1, "kind": "port",
+            "sample": f"{done} requests of the mix run sequentially on the oracle restatement, one thread"
+                      f" ({skipped} sorts above {max_sort_rows} rows left out)"}
 
 
 def serving_section(ctx, rates, seconds, slots):
@@ -252,12 +354,14 @@ def serving_section(ctx, rates, seconds, slots):
             f"{entry['rows_per_s']:.3e} rows/s, peak in flight {st['peak_inflight']}")
         per_rate.append(entry)
     srv.close()
+    cpu = cpu_serving_baseline(I, W.serving_requests(40, 7, I))
+    log(f"serving cpu baseline: {cpu}")
     del I
     torch.cuda.empty_cache()
     return {"workload": "Poisson arrivals; 60% config-1 fragments (1k..16M rows, lognormal median 256k), "
                         "25% group-by SUM/COUNT (1k..16M rows, <= 10M distinct int64 keys), 15% sort "
                         "(1M..256M int64 keys, lognormal median 8M); dimension tables resident",
-            "slots": slots, "identical_to_plain": identical, "rates": per_rate}
+            "slots": slots, "identical_to_plain": identical, "rates": per_rate, "cpu_baseline": cpu}
 
 
 def ops_server(slots):

@@ -309,6 +309,9 @@ int ghive_set_option(ghive_ctx *ctx, const char *name, int64_t value) {
     if (value != 256 && value != 512 && value != 1024)
       return set_error(ctx, GHIVE_ERR_INVALID, "option %s: %lld (256, 512 or 1024)", name, (long long)value);
     ctx->tune.frag_block = value;
+  } else if (k == "groupby.partition") {
+    if (value < 0 || value > 2) return set_error(ctx, GHIVE_ERR_INVALID, "option %s: %lld (0, 1 or 2)", name, (long long)value);
+    ctx->tune.gb_partition = value;
   } else if (k == "groupby.lds_budget") {
     if (value < 0 || value > 160 * 1024) return set_error(ctx, GHIVE_ERR_INVALID, "option %s: %lld", name, (long long)value);
     ctx->tune.gb_lds_budget = value;
@@ -330,6 +333,7 @@ int ghive_get_option(ghive_ctx *ctx, const char *name, int64_t *value) {
   else if (k == "fragment.block") *value = ctx->tune.frag_block;
   else if (k == "fragment.generic") *value = ctx->tune.frag_generic;
   else if (k == "groupby.lds_budget") *value = ctx->tune.gb_lds_budget;
+  else if (k == "groupby.partition") *value = ctx->tune.gb_partition;
   else return set_error(ctx, GHIVE_ERR_INVALID, "unknown option '%s'", name);
   return GHIVE_OK;
 }

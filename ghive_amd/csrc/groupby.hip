@@ -127,6 +127,125 @@ __global__ __launch_bounds__(kGbThreads) void k_groupby(KeySpec K, GroupTableDev
   lds_table_flush(lds, T, lcap_log2);
 }
 
+// ---- high-cardinality form: hash partitions aggregated in LDS --------------------------------
+// When the groups cannot repeat inside a workgroup's share of the rows, every row of the direct
+// form costs device-scope atomics on a table far larger than L2. This form instead (1) writes
+// (mix(key), row id) per row, (2) radix-sorts those pairs by the top bits of mix(key) so that each
+// hash partition is a contiguous range, and (3) aggregates every partition in one workgroup's
+// LDS table, then stores that table with plain stores into the partition's own region of the
+// global table (regions are disjoint, so no atomics), where the usual finalize emits it.
+constexpr int kGbPartThreads = 512;
+
+// splitmix64 finalizer (Steele, Lea, Flood, OOPSLA 2014): every input bit reaches the top bits.
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x ^= x >> 30;
+  x *= 0xbf58476d1ce4e5b9ull;
+  x ^= x >> 27;
+  x *= 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_gb_part_keys(KeySpec K, const uint32_t *sel, uint64_t n, uint64_t *h,
+                                                      uint32_t *ids) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    const uint64_t r = sel ? sel[i] : i;
+    h[i] = mix64(pack_key(K, r));
+    ids[i] = (uint32_t)r;
+  }
+}
+
+// starts[p] = first position of partition p (top pbits of h) in the sorted h; starts[P] = n.
+__global__ void k_gb_part_bounds(const uint64_t *h, uint64_t n, int pbits, uint64_t *starts) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t P = 1ull << pbits;
+  for (; i < n; i += stride) {
+    const uint64_t p = pbits ? h[i] >> (64 - pbits) : 0;
+    const int64_t prev = i ? (int64_t)(pbits ? h[i - 1] >> (64 - pbits) : 0) : -1;
+    for (int64_t q = prev + 1; q <= (int64_t)p; ++q) starts[q] = i;  // empty partitions start here too
+    if (i == n - 1)
+      for (uint64_t q = p + 1; q <= P; ++q) starts[q] = n;
+  }
+}
+
+// One workgroup per partition. The LDS table has lcap slots (unbounded linear probing); a
+// partition with more groups than 7/8 of that sets *fail and the caller uses the direct form.
+__global__ __launch_bounds__(kGbPartThreads) void k_gb_part_agg(KeySpec K, GroupTableDev T, const uint64_t *starts,
+                                                                const uint32_t *ids, int lcap_log2,
+                                                                unsigned long long *fail) {
+  extern __shared__ uint64_t lds[];
+  __shared__ uint32_t used, full;
+  const int lcap = 1 << lcap_log2, lstride = lcap + 1;
+  lds_table_init(lds, T, lcap_log2);
+  if (threadIdx.x == 0) {
+    used = 0;
+    full = 0;
+  }
+  __syncthreads();
+  const uint64_t b = starts[blockIdx.x], e = starts[blockIdx.x + 1];
+  const uint32_t limit = (uint32_t)lcap - (uint32_t)lcap / 8;
+  for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+    if (full) break;
+    const uint64_t r = ids[i];
+    const uint64_t key = pack_key(K, r);
+    if (key == kEmptyKey) {  // the one key equal to the EMPTY pattern lives in the table's extra slot
+      const int64_t g = gt_find_insert(T, key);
+      for (int a = 0; a < T.naggs; ++a) {
+        const AggSpec &A = T.aggs[a];
+        AggVal v = agg_value(A, r);
+        acc_apply(A, [&](int j) { return acc_word(T, A.word0 + j, (uint64_t)g); }, v);
+      }
+      continue;
+    }
+    uint32_t s = (uint32_t)hash_u64(key, lcap_log2);
+    int slot = -1;
+    for (int step = 0; step < lcap; ++step) {
+      const uint64_t k = __atomic_load_n(lds + s, __ATOMIC_RELAXED);
+      if (k == key) {
+        slot = (int)s;
+        break;
+      }
+      if (k == kEmptyKey) {
+        const unsigned long long old = atomicCAS((unsigned long long *)(lds + s), kEmptyKey, key);
+        if (old == kEmptyKey) {
+          if (atomicAdd(&used, 1u) + 1u > limit) full = 1;
+          slot = (int)s;
+          break;
+        }
+        if (old == key) {
+          slot = (int)s;
+          break;
+        }
+      }
+      s = (s + 1) & (uint32_t)(lcap - 1);
+    }
+    if (slot < 0 || full) {
+      full = 1;
+      break;
+    }
+    for (int a = 0; a < T.naggs; ++a) {
+      const AggSpec &A = T.aggs[a];
+      AggVal v = agg_value(A, r);
+      acc_apply(A, [&](int j) { return lds + (size_t)(1 + A.word0 + j) * lstride + slot; }, v);
+    }
+  }
+  __syncthreads();
+  if (full) {
+    if (threadIdx.x == 0) atomicOr(fail, 1ull);
+    return;
+  }
+  // the partition's own region of the global table: plain stores
+  const uint64_t base = (uint64_t)blockIdx.x << lcap_log2;
+  for (int s = threadIdx.x; s < lcap; s += blockDim.x) {
+    const uint64_t k = lds[s];
+    T.keys[base + s] = k;
+    if (k == kEmptyKey) continue;
+    for (int w = 0; w < T.nwords; ++w) *acc_word(T, w, base + s) = lds[(size_t)(1 + w) * lstride + s];
+  }
+}
+
 __global__ void k_gt_preinsert(GroupTableDev T, uint64_t key) {
   if (threadIdx.x == 0 && blockIdx.x == 0) (void)gt_find_insert(T, key);
 }
@@ -279,6 +398,47 @@ int group_table_alloc(ghive_ctx *ctx, GroupTableDev &T, uint64_t capacity, DevBu
   return GHIVE_OK;
 }
 
+// Runs the high-cardinality form into T (P = 2^pbits regions of 2^lg slots, initialised).
+// *failed: a partition held more groups than its LDS table; T is then unusable.
+int groupby_partitioned(ghive_ctx *ctx, const KeySpec &K, const GroupTableDev &T, const uint32_t *sel, uint64_t n,
+                        int lg, int pbits, bool *failed) {
+  *failed = false;
+  DevBuf h0, h1, i0, i1, starts;
+  GH_TRY(h0.alloc(ctx, 8 * n));
+  GH_TRY(h1.alloc(ctx, 8 * n));
+  GH_TRY(i0.alloc(ctx, 4 * n));
+  GH_TRY(i1.alloc(ctx, 4 * n));
+  GH_TRY(starts.alloc(ctx, 8 * ((1ull << pbits) + 1)));
+  {
+    const uint64_t g = (n + 255) / 256, cap = (uint64_t)ctx->cus * 16;
+    k_gb_part_keys<<<(uint32_t)(g > cap ? cap : g), 256, 0, ctx->stream>>>(K, sel, n, h0.as<uint64_t>(),
+                                                                            i0.as<uint32_t>());
+    GH_LAUNCH(ctx);
+  }
+  std::vector<int> digits;  // the 8-bit digits that cover the top pbits bits, least significant first
+  for (int d = pbits ? (64 - pbits) / 8 : 8; d < 8; ++d) digits.push_back(d);
+  uint64_t *hs = h0.as<uint64_t>();
+  uint32_t *ids = i0.as<uint32_t>();
+  GH_TRY(radix_sort_u64_ids(ctx, n, h0.as<uint64_t>(), h1.as<uint64_t>(), i0.as<uint32_t>(), i1.as<uint32_t>(),
+                            digits, &hs, &ids));
+  {
+    const uint64_t g = (n + 255) / 256, cap = (uint64_t)ctx->cus * 16;
+    k_gb_part_bounds<<<(uint32_t)(g > cap ? cap : g), 256, 0, ctx->stream>>>(hs, n, pbits, starts.as<uint64_t>());
+    GH_LAUNCH(ctx);
+  }
+  const size_t lds_bytes = (size_t)((1 << lg) + 1) * (1 + T.nwords) * 8;
+  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_gb_part_agg), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds_bytes);
+  GH_TRY(fill_u64(ctx, ctx->d_counters, 0, 1));
+  k_gb_part_agg<<<(uint32_t)(1ull << pbits), kGbPartThreads, lds_bytes, ctx->stream>>>(
+      K, T, starts.as<uint64_t>(), ids, lg, reinterpret_cast<unsigned long long *>(ctx->d_counters));
+  GH_LAUNCH(ctx);
+  uint64_t f = 0;
+  GH_TRY(d2h(ctx, &f, ctx->d_counters, sizeof f));
+  *failed = f != 0;
+  return GHIVE_OK;
+}
+
 int group_table_preinsert(ghive_ctx *ctx, const GroupTableDev &T, uint64_t key) {
   k_gt_preinsert<<<1, 64, 0, ctx->stream>>>(T, key);
   GH_LAUNCH(ctx);
@@ -408,9 +568,30 @@ int ghive_groupby(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel,
     }
   }
   DevBuf dkeys, dacc, dcnt;
-  GH_TRY(group_table_alloc(ctx, T, capacity, dkeys, dacc, dcnt));
+  // the high-cardinality form: a large group bound over many rows (or forced by the option)
+  const int64_t part_opt = ctx->tune.gb_partition;
+  bool partitioned = false;
+  if (nkeys > 0 && n && part_opt != 0 &&
+      (part_opt == 2 || (n >= (1ull << 22) && capacity >= (1ull << 20)))) {
+    int lg = 11;  // LDS table slots per partition
+    while (lg > 8 && (size_t)((1 << lg) + 1) * (1 + T.nwords) * 8 > 150 * 1024) --lg;
+    int pbits = 0;  // partitions so that the group bound leaves each table at most half full
+    while (pbits < 24 && (capacity << 1) > (1ull << (pbits + lg))) ++pbits;
+    GH_TRY(group_table_alloc(ctx, T, (1ull << (pbits + lg)) / 2, dkeys, dacc, dcnt));
+    T.capacity = capacity;
+    bool failed = false;
+    GH_TRY(groupby_partitioned(ctx, K, T, sel, n, lg, pbits, &failed));
+    if (!failed) {
+      partitioned = true;
+    } else {
+      dkeys.reset();
+      dacc.reset();
+      dcnt.reset();
+    }
+  }
+  if (!partitioned) GH_TRY(group_table_alloc(ctx, T, capacity, dkeys, dacc, dcnt));
   if (nkeys == 0) GH_TRY(group_table_preinsert(ctx, T, 0ull));  // a global aggregate: one group, even over no rows
-  if (n) {
+  if (n && !partitioned) {
     int lg = lds_log2_for(T.nwords, (int)ctx->tune.gb_lds_budget);
     size_t lds_bytes = (size_t)((1 << lg) + 1) * (1 + T.nwords) * 8;
     int per_cu = 0;

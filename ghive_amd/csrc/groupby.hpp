@@ -1,6 +1,8 @@
 // Group-table helpers shared by the group-by operator and the fused fragment.
 #pragma once
 
+#include <vector>
+
 #include "agg.hpp"
 #include "common.hpp"
 
@@ -26,6 +28,10 @@ int group_table_alloc(ghive_ctx *ctx, GroupTableDev &T, uint64_t capacity, DevBu
                       DevBuf &counters);
 // Inserts `key` (so that a keyless aggregate yields its one group even over no rows).
 int group_table_preinsert(ghive_ctx *ctx, const GroupTableDev &T, uint64_t key);
+// Stable LSD radix passes (sort.hip) over 8-bit `digits` of one uint64 word per row with a uint32
+// id; *keys_out / *ids_out point at the buffers holding the result.
+int radix_sort_u64_ids(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, uint32_t *i0, uint32_t *i1,
+                       const std::vector<int> &digits, uint64_t **keys_out, uint32_t **ids_out);
 // Writes every group to the output columns; CAPACITY error when there are more than capacity.
 int group_table_emit(ghive_ctx *ctx, const KeySpec &K, const GroupTableDev &T, const OutSpec &O,
                      uint64_t capacity, uint64_t *out_groups);

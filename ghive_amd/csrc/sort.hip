@@ -202,7 +202,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *
 
 template <int KW>
 static int radix_passes(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, uint32_t *i0, uint32_t *i1,
-                        const std::vector<int> &digits, uint32_t **result) {
+                        const std::vector<int> &digits, uint32_t **result, uint64_t **keys_result = nullptr) {
   const uint32_t nblocks = ceil_div(n, kSortTile);
   DevBuf hist, counts;
   GH_TRY(hist.alloc(ctx, sizeof(uint32_t) * 256ull * nblocks));
@@ -223,7 +223,16 @@ static int radix_passes(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, 
     std::swap(iin, iout);
   }
   *result = iin;
+  if (keys_result) *keys_result = kin;
   return GHIVE_OK;
+}
+
+// Stable LSD passes over the given 8-bit digits of one uint64 word per row, carrying a uint32 id
+// (the group-by's hash partitioning). The ping-pong buffers hold n entries each; the sorted
+// words and ids end in whichever buffer the last pass wrote.
+int radix_sort_u64_ids(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, uint32_t *i0, uint32_t *i1,
+                       const std::vector<int> &digits, uint64_t **keys_out, uint32_t **ids_out) {
+  return radix_passes<1>(ctx, n, k0, k1, i0, i1, digits, ids_out, keys_out);
 }
 
 }  // namespace ghive

@@ -98,9 +98,21 @@ def test_hive_aggregate_kats(ctx, kats):
             assert got == {a: b for a, b in k["expect"]}, k["cite"]
 
 
+@pytest.mark.parametrize("partition", [1, 2], ids=["direct", "partitioned"])
 @pytest.mark.parametrize("card", [7, 3000, 200_000])
 @pytest.mark.parametrize("ktype", ["INT", "LONG", "DOUBLE", "INT2"])
-def test_groupby_all_aggs_vs_oracle(ctx, orc, card, ktype):
+def test_groupby_all_aggs_vs_oracle(ctx, orc, card, ktype, partition):
+    """Every aggregate kind and key type against the oracle, in the direct form and in the
+    high-cardinality form (hash partitions aggregated in LDS); LONG keys include -1, whose bits
+    equal the table's EMPTY pattern."""
+    ctx.set_option("groupby.partition", partition)
+    try:
+        _all_aggs_case(ctx, orc, card, ktype)
+    finally:
+        ctx.set_option("groupby.partition", 1)
+
+
+def _all_aggs_case(ctx, orc, card, ktype):
     rng = np.random.default_rng(card + len(ktype))
     n = 400_000
     if ktype == "INT2":
