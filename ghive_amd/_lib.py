@@ -100,6 +100,13 @@ SIGNATURES_EXTRA = {
     "ghive_plan_parse": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, u64, C.POINTER(P)]),
     "ghive_plan_describe": (C.c_int, [P, C.c_char_p, u64, PU64]),
     "ghive_plan_free": (None, [P]),
+    "ghive_plan_execute": (C.c_int, [P, P, C.c_char_p, C.POINTER(abi.PlanInput), u32, C.c_char_p, u64,
+                                     C.POINTER(P)]),
+    "ghive_result_info": (C.c_int, [P, PU64, C.POINTER(u32)]),
+    "ghive_result_column": (C.c_int, [P, u32, C.POINTER(abi.Column), C.POINTER(C.c_char_p)]),
+    "ghive_result_free": (None, [P, P]),
+    "ghive_select_arith": (C.c_int, [P, C.POINTER(abi.Column), C.POINTER(abi.Column), C.c_char, u64,
+                                     C.POINTER(abi.Column)]),
     "ghive_server_create": (C.c_int, [C.c_int, C.POINTER(abi.ServerConfig), C.POINTER(P)]),
     "ghive_server_submit": (C.c_int, [P, C.POINTER(abi.Request), PU64]),
     "ghive_server_wait": (C.c_int, [P, u64, C.POINTER(abi.Response)]),

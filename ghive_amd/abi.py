@@ -59,6 +59,10 @@ class JoinCond(C.Structure):
     _fields_ = [("type", C.c_int32), ("left", C.c_uint32), ("right", C.c_uint32)]
 
 
+class PlanInput(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("table", C.POINTER(Table)), ("column_names", C.POINTER(C.c_char_p))]
+
+
 class Agg(C.Structure):
     _fields_ = [("type", C.c_int32), ("index", C.c_uint32)]
 

@@ -673,6 +673,392 @@ void set_err(char *err, uint64_t cap, const std::string &msg) {
 
 }  // namespace
 
+namespace {
+
+// ---- execution of one vertex ------------------------------------------------------------------
+// Operator::execute over a vertex's chain (execute() of gpu_impl/src/Operator/), each operator
+// on the library's entry points: TableScan binds an input table by name, Filter parses its
+// predicate text (ghive_parse_predicate) and compacts (ghive_filter_compact), Select aliases
+// columns or computes binary arithmetic (ghive_select_arith), a join runs ghive_join and
+// materialises its maintained columns (ghive_join_result), GroupBy runs ghive_groupby, the sink
+// orders its columns keys first and, as the plan's root, performs the final ORDER BY (ghive_sort
+// + ghive_gather) and LIMIT.
+struct ExecTable {
+  std::vector<std::string> names;
+  std::vector<ghive_column> cols;
+  uint64_t rows = 0;
+  int reduce_keys = -1;  // a reduce input: the number of key columns leading it
+  ghive_table view() const { return ghive_table{cols.data(), (uint32_t)cols.size(), 0, rows}; }
+  int find(const std::string &n) const { return index_of(names, n); }
+};
+
+struct Executor {
+  ghive_ctx *ctx;
+  const ghive_plan &P;
+  const std::string vertex;
+  std::map<std::string, const ghive_plan_input *> inputs;
+  std::vector<void *> owned;  // device buffers made by this execution
+  std::string err;
+
+  int fail(int code, const std::string &m) {
+    err = m;
+    return code;
+  }
+  int alloc_col(int type, uint64_t rows, ghive_column &c) {
+    memset(&c, 0, sizeof c);
+    c.type = type;
+    c.row_num = rows;
+    const uint64_t w = (type == GHIVE_LONG || type == GHIVE_DOUBLE) ? 8 : 4;
+    void *p = nullptr;
+    const int st = ghive_malloc(ctx, (rows ? rows : 1) * w, &p);
+    if (st != GHIVE_OK) return fail(st, std::string("plan: ") + ghive_last_error(ctx));
+    owned.push_back(p);
+    c.data = p;
+    return GHIVE_OK;
+  }
+  int op_error(int st, const Op &o) {
+    return fail(st, "plan: " + o.id + ": " + ghive_last_error(ctx));
+  }
+
+  int from_input(const std::string &name, const std::vector<std::string> &want, ExecTable &t) {
+    auto it = inputs.find(name);
+    if (it == inputs.end()) return fail(GHIVE_ERR_INVALID, "plan: no input named '" + name + "'");
+    const ghive_plan_input &in = *it->second;
+    if (!in.table) return fail(GHIVE_ERR_INVALID, "plan: input '" + name + "' has no table");
+    std::vector<std::string> names;
+    for (uint32_t c = 0; c < in.table->ncols; ++c)
+      names.push_back(in.column_names && in.column_names[c] ? in.column_names[c] : "_col" + std::to_string(c));
+    t.rows = in.table->row_num;
+    const std::vector<std::string> &order = want.empty() ? names : want;
+    for (const std::string &n : order) {
+      const int c = index_of(names, n);
+      if (c < 0) return fail(GHIVE_ERR_INVALID, "plan: input '" + name + "' has no column '" + n + "'");
+      t.names.push_back(n);
+      t.cols.push_back(in.table->columns[c]);
+    }
+    return GHIVE_OK;
+  }
+
+  // one operand of an arithmetic expression: a column of t, or a numeric literal (1 row)
+  int operand(const std::string &e, const ExecTable &t, ghive_column &c) {
+    const int k = t.find(e);
+    if (k >= 0) {
+      c = t.cols[k];
+      return GHIVE_OK;
+    }
+    static const std::regex lit_re(R"(^(-?\d+(\.\d*)?([eE][-+]?\d+)?)([DdLlSsYy]?)$)");
+    std::smatch m;
+    if (!std::regex_match(e, m, lit_re)) return fail(GHIVE_ERR_UNSUPPORTED, "plan: operand '" + e + "'");
+    const std::string suffix = m[4].str();
+    const bool fp = !suffix.empty() ? (suffix == "D" || suffix == "d") : (m[2].matched || m[3].matched);
+    const bool lng = suffix == "L" || suffix == "l";
+    int st = alloc_col(fp ? GHIVE_DOUBLE : (lng ? GHIVE_LONG : GHIVE_INT), 1, c);
+    if (st != GHIVE_OK) return st;
+    if (fp) {
+      const double v = std::stod(m[1].str());
+      return ghive_memcpy(ctx, c.data, &v, 8, 0);
+    }
+    if (lng) {
+      const int64_t v = std::stoll(m[1].str());
+      return ghive_memcpy(ctx, c.data, &v, 8, 0);
+    }
+    const int32_t v = (int32_t)std::stol(m[1].str());
+    return ghive_memcpy(ctx, c.data, &v, 4, 0);
+  }
+
+  // Select expression: a column, KEY.reducesinkkeyN / VALUE._colN of a reduce input, or
+  // (a op b) with op in + - * /, operands columns or literals, nested
+  int expression(const std::string &e0, const ExecTable &t, ghive_column &out) {
+    const std::string e = trim(e0);
+    const int k = t.find(e);
+    if (k >= 0) {
+      out = t.cols[k];
+      return GHIVE_OK;
+    }
+    std::smatch m;
+    static const std::regex key_re(R"(^KEY\.reducesinkkey(\d+)$)"), val_re(R"(^VALUE\._col(\d+)$)");
+    if (std::regex_match(e, m, key_re)) {
+      const int c = std::stoi(m[1].str());
+      if (c >= (int)t.cols.size()) return fail(GHIVE_ERR_INVALID, "plan: " + e + " beyond the reduce input");
+      out = t.cols[c];
+      return GHIVE_OK;
+    }
+    if (e.size() > 2 && e.front() == '(' && e.back() == ')') {
+      const std::string in = e.substr(1, e.size() - 2);
+      int depth = 0;
+      for (size_t i = 0; i < in.size(); ++i) {  // the top-level binary operator " op "
+        if (in[i] == '(') ++depth;
+        if (in[i] == ')') --depth;
+        if (depth == 0 && i + 2 < in.size() && in[i] == ' ' && in[i + 2] == ' ' &&
+            (in[i + 1] == '+' || in[i + 1] == '-' || in[i + 1] == '*' || in[i + 1] == '/')) {
+          ghive_column a, b;
+          int st = sub_operand(in.substr(0, i), t, a);
+          if (st != GHIVE_OK) return st;
+          st = sub_operand(in.substr(i + 3), t, b);
+          if (st != GHIVE_OK) return st;
+          const int otype = (in[i + 1] == '/' || a.type == GHIVE_DOUBLE || b.type == GHIVE_DOUBLE ||
+                             a.type == GHIVE_FLOAT || b.type == GHIVE_FLOAT)
+                                ? GHIVE_DOUBLE
+                                : (a.type == GHIVE_INT && b.type == GHIVE_INT ? GHIVE_INT : GHIVE_LONG);
+          st = alloc_col(otype, t.rows, out);
+          if (st != GHIVE_OK) return st;
+          st = ghive_select_arith(ctx, &a, &b, in[i + 1], t.rows, &out);
+          return st == GHIVE_OK ? GHIVE_OK : fail(st, std::string("plan: ") + ghive_last_error(ctx));
+        }
+      }
+      return expression(in, t, out);  // redundant parentheses
+    }
+    if (std::regex_match(e, m, val_re)) {  // values follow the keys of a reduce input
+      const int c = std::stoi(m[1].str()) + t.reduce_keys;
+      if (t.reduce_keys < 0 || c >= (int)t.cols.size())
+        return fail(GHIVE_ERR_INVALID, "plan: " + e + " does not name a column of the reduce input");
+      out = t.cols[c];
+      return GHIVE_OK;
+    }
+    return fail(GHIVE_ERR_UNSUPPORTED, "plan: expression '" + e + "'");
+  }
+  int sub_operand(const std::string &e, const ExecTable &t, ghive_column &c) {
+    static const std::regex num_re(R"(^-?\d.*$)");
+    const std::string x = trim(e);
+    return std::regex_match(x, num_re) ? operand(x, t, c) : expression(x, t, c);
+  }
+
+  int run(int node, ExecTable &out) {
+    const Op &o = P.ops[node];
+    if ((o.kind == "ReduceSink" || o.kind == "FileSink" || o.kind == "Union") && o.vertex != vertex) {
+      const int st = from_input(o.vertex, {}, out);  // the result of another vertex
+      if (o.kind == "ReduceSink") out.reduce_keys = (int)o.key_expressions.size();
+      return st;
+    }
+    if (o.kind == "TableScan") return from_input(o.table, o.outputs, out);
+    if (o.children.empty()) return fail(GHIVE_ERR_INVALID, "plan: " + o.id + " has no input");
+    if (o.kind == "MapJoin" || o.kind == "MergeJoin") return join(o, out);
+    ExecTable in;
+    int st = run(o.children[0], in);
+    if (st != GHIVE_OK) return st;
+    if (o.kind == "Filter") {
+      std::vector<const char *> names;
+      for (auto &n : in.names) names.push_back(n.c_str());
+      std::vector<ghive_pred> nodes(256);
+      uint32_t nn = 0, root = 0;
+      char perr[256] = {0};
+      st = ghive_parse_predicate(o.predicate.c_str(), names.data(), (uint32_t)names.size(), nodes.data(),
+                                 (uint32_t)nodes.size(), &nn, &root, perr, sizeof perr);
+      if (st != GHIVE_OK) return fail(st, "plan: " + o.id + ": " + perr);
+      out.names = in.names;
+      out.cols.resize(in.cols.size());
+      for (size_t c = 0; c < in.cols.size(); ++c)
+        if ((st = alloc_col(in.cols[c].type, in.rows, out.cols[c])) != GHIVE_OK) return st;
+      ghive_table tv = in.view();
+      uint64_t cnt = 0;
+      st = ghive_filter_compact(ctx, &tv, nodes.data(), nn, root, out.cols.data(), &cnt);
+      if (st != GHIVE_OK) return op_error(st, o);
+      out.rows = cnt;
+      for (auto &c : out.cols) c.row_num = cnt;
+      return GHIVE_OK;
+    }
+    if (o.kind == "Select") {
+      out.rows = in.rows;
+      out.names = o.outputs;
+      const std::vector<std::string> &ex = o.expressions.empty() ? o.outputs : o.expressions;
+      if (ex.size() != o.outputs.size())
+        return fail(GHIVE_ERR_INVALID, "plan: " + o.id + ": expressions and outputs differ in number");
+      for (const std::string &e : ex) {
+        ghive_column c;
+        if ((st = expression(e, in, c)) != GHIVE_OK) return st;
+        out.cols.push_back(c);
+      }
+      return GHIVE_OK;
+    }
+    if (o.kind == "GroupBy") {
+      std::vector<uint32_t> keys(o.key_offsets.begin(), o.key_offsets.end());
+      std::vector<ghive_agg> aggs;
+      for (const auto &a : o.agg_descs) aggs.push_back(ghive_agg{a[0], (uint32_t)a[1]});
+      for (uint32_t k : keys)
+        if (k >= in.cols.size()) return fail(GHIVE_ERR_INVALID, "plan: " + o.id + ": key column beyond the input");
+      if (keys.size() + aggs.size() != o.outputs.size())
+        return fail(GHIVE_ERR_INVALID, "plan: " + o.id + ": keys and aggregates do not match its outputs");
+      const uint64_t cap = in.rows ? in.rows : 1;
+      out.cols.resize(keys.size() + aggs.size());
+      for (size_t k = 0; k < keys.size(); ++k)
+        if ((st = alloc_col(in.cols[keys[k]].type, cap, out.cols[k])) != GHIVE_OK) return st;
+      for (size_t a = 0; a < aggs.size(); ++a) {
+        if (aggs[a].index >= in.cols.size()) return fail(GHIVE_ERR_INVALID, "plan: " + o.id + ": aggregate column");
+        const int ot = ghive_agg_output_type(aggs[a].type, in.cols[aggs[a].index].type);
+        if (ot < 0) return fail(GHIVE_ERR_UNSUPPORTED, "plan: " + o.id + ": aggregate type");
+        if ((st = alloc_col(ot, cap, out.cols[keys.size() + a])) != GHIVE_OK) return st;
+      }
+      ghive_table tv = in.view();
+      uint64_t groups = 0;
+      st = ghive_groupby(ctx, &tv, nullptr, in.rows, keys.data(), (uint32_t)keys.size(), aggs.data(),
+                         (uint32_t)aggs.size(), out.cols.data(), cap, &groups);
+      if (st != GHIVE_OK) return op_error(st, o);
+      out.rows = groups;
+      out.names = o.outputs;
+      return GHIVE_OK;
+    }
+    if (o.kind == "Limit") {
+      out = in;
+      if (o.limit >= 0 && (uint64_t)o.limit < out.rows) out.rows = (uint64_t)o.limit;
+      for (auto &c : out.cols) c.row_num = out.rows;
+      return GHIVE_OK;
+    }
+    if (o.kind == "ReduceSink" || o.kind == "FileSink") {
+      // the vertex's own sink: its columns keys first
+      out.rows = in.rows;
+      for (const std::string &n : o.outputs) {
+        const int c = in.find(n);
+        if (c < 0) return fail(GHIVE_ERR_INVALID, "plan: " + o.id + ": no column '" + n + "' below it");
+        out.names.push_back(n);
+        out.cols.push_back(in.cols[c]);
+      }
+      if (o.outputs.empty()) out = in;
+      return GHIVE_OK;
+    }
+    return fail(GHIVE_ERR_UNSUPPORTED, "plan: " + o.kind + " operator " + o.id);
+  }
+
+  int join(const Op &o, ExecTable &out) {
+    const size_t nt = o.join_inputs.size();
+    if (nt < 2) return fail(GHIVE_ERR_INVALID, "plan: " + o.id + ": fewer than two inputs");
+    std::vector<ExecTable> ts(nt);
+    for (size_t t = 0; t < nt; ++t) {
+      const auto it = P.by_id.find(o.join_inputs[t]);
+      if (it == P.by_id.end()) return fail(GHIVE_ERR_INVALID, "plan: " + o.id + ": unknown input");
+      int st = run(it->second, ts[t]);
+      if (st != GHIVE_OK) return st;
+      if (o.join_cols[t].size() != 1)
+        return fail(GHIVE_ERR_UNSUPPORTED, "plan: " + o.id + ": join on more than one key column");
+    }
+    std::vector<ghive_table> views;
+    for (auto &t : ts) views.push_back(t.view());
+    std::vector<uint32_t> jc;
+    for (size_t t = 0; t < nt; ++t) jc.push_back((uint32_t)o.join_cols[t][0]);
+    std::vector<ghive_join_cond> conds;
+    for (const auto &c : o.join_types) conds.push_back(ghive_join_cond{c[0], (uint32_t)c[1], (uint32_t)c[2]});
+    uint64_t card = 0;
+    int st = ghive_join(ctx, views.data(), (uint32_t)nt, jc.data(), conds.data(), (uint32_t)conds.size(), nullptr, 0,
+                        &card);
+    if (st != GHIVE_OK && st != GHIVE_ERR_CAPACITY) return op_error(st, o);
+    size_t width = nt;
+    for (const auto &c : conds)
+      if (c.type == GHIVE_LEFT_SEMI_JOIN) --width;
+    void *offs = nullptr;
+    if ((st = ghive_malloc(ctx, (card ? card : 1) * width * 4, &offs)) != GHIVE_OK) return op_error(st, o);
+    owned.push_back(offs);
+    uint64_t card2 = 0;
+    st = ghive_join(ctx, views.data(), (uint32_t)nt, jc.data(), conds.data(), (uint32_t)conds.size(),
+                    static_cast<int32_t *>(offs), card, &card2);
+    if (st != GHIVE_OK) return op_error(st, o);
+    // the maintained columns: from the Java side's map, else the join outputs found by name
+    std::vector<std::vector<uint32_t>> keep(nt);
+    bool from_map = false;
+    for (size_t t = 0; t < nt; ++t)
+      for (int c : o.maintain_cols[t]) {
+        keep[t].push_back((uint32_t)c);
+        from_map = true;
+      }
+    if (!from_map) {
+      for (const std::string &n : o.outputs) {
+        bool found = false;
+        for (size_t t = 0; t < nt && !found; ++t) {
+          const int c = ts[t].find(n);
+          if (c >= 0) {
+            keep[t].push_back((uint32_t)c);
+            found = true;
+          }
+        }
+        if (!found)
+          return fail(GHIVE_ERR_UNSUPPORTED, "plan: " + o.id + ": output '" + n +
+                                                 "' is not a column of an input; pass the maintain-values map");
+      }
+    }
+    std::vector<const uint32_t *> kp;
+    std::vector<uint32_t> kc;
+    size_t ncols = 0;
+    for (auto &k : keep) {
+      kp.push_back(k.data());
+      kc.push_back((uint32_t)k.size());
+      ncols += k.size();
+    }
+    if (ncols == 0) ncols = 1;  // the reference keeps column 0 of the first input
+    std::vector<ghive_column> cols(ncols);
+    // output types follow the gathered columns: allocate per the plan's own order
+    {
+      size_t oi = 0;
+      size_t first = 0;
+      while (first < nt && keep[first].empty() && first < nt) ++first;
+      for (size_t t = 0; t < nt; ++t)
+        for (uint32_t c : keep[t])
+          if ((st = alloc_col(ts[t].cols[c].type, card, cols[oi++])) != GHIVE_OK) return st;
+      if (oi == 0 && (st = alloc_col(ts[0].cols[0].type, card, cols[0])) != GHIVE_OK) return st;
+    }
+    uint32_t got = 0;
+    st = ghive_join_result(ctx, views.data(), (uint32_t)nt, conds.data(), (uint32_t)conds.size(), kp.data(),
+                           kc.data(), static_cast<int32_t *>(offs), card, cols.data(), (uint32_t)cols.size(), &got);
+    if (st != GHIVE_OK) return op_error(st, o);
+    out.rows = card;
+    out.cols.assign(cols.begin(), cols.begin() + got);
+    out.names = o.outputs;
+    if (out.names.size() != out.cols.size())
+      return fail(GHIVE_ERR_INVALID, "plan: " + o.id + ": the maintained columns do not match its outputs");
+    return GHIVE_OK;
+  }
+
+  // the plan's root: ORDER BY on the reduce keys (the vertex's Select expressions that read
+  // KEY.reducesinkkeyN, else the leading columns), then LIMIT
+  int finish_root(int root_node, ExecTable &t) {
+    const Op &o = P.ops[root_node];
+    if (!o.require_sort || o.sort_orders.empty() || t.rows == 0) return GHIVE_OK;
+    const size_t nk = o.sort_orders.size();
+    std::vector<uint32_t> keys(nk, UINT32_MAX);
+    static const std::regex key_re(R"(^KEY\.reducesinkkey(\d+)$)");
+    for (int node = root_node; node >= 0;) {
+      const Op &x = P.ops[node];
+      if (x.kind == "Select") {
+        for (size_t k = 0; k < x.expressions.size() && k < x.outputs.size(); ++k) {
+          std::smatch m;
+          if (std::regex_match(x.expressions[k], m, key_re)) {
+            const size_t j = (size_t)std::stoi(m[1].str());
+            const int c = t.find(x.outputs[k]);
+            if (j < nk && c >= 0) keys[j] = (uint32_t)c;
+          }
+        }
+        break;
+      }
+      int next = -1;
+      for (int c : x.children)
+        if (P.ops[c].vertex == vertex && next < 0) next = c;
+      node = next;
+    }
+    for (size_t j = 0; j < nk; ++j)
+      if (keys[j] == UINT32_MAX) keys[j] = (uint32_t)j;
+    void *perm = nullptr;
+    int st = ghive_malloc(ctx, t.rows * 4, &perm);
+    if (st != GHIVE_OK) return fail(st, std::string("plan: ") + ghive_last_error(ctx));
+    owned.push_back(perm);
+    ghive_table tv = t.view();
+    st = ghive_sort(ctx, &tv, keys.data(), o.sort_orders.c_str(), (uint32_t)nk, static_cast<uint32_t *>(perm));
+    if (st != GHIVE_OK) return op_error(st, o);
+    std::vector<ghive_column> sorted(t.cols.size());
+    for (size_t c = 0; c < t.cols.size(); ++c)
+      if ((st = alloc_col(t.cols[c].type, t.rows, sorted[c])) != GHIVE_OK) return st;
+    st = ghive_gather(ctx, &tv, static_cast<uint32_t *>(perm), t.rows, sorted.data());
+    if (st != GHIVE_OK) return op_error(st, o);
+    t.cols = sorted;
+    return GHIVE_OK;
+  }
+};
+
+}  // namespace
+
+struct ghive_result {
+  std::vector<std::string> names;
+  std::vector<ghive_column> cols;
+  uint64_t rows = 0;
+  std::vector<void *> owned;
+};
+
 extern "C" {
 
 int ghive_plan_parse(const char *plan, const char *extended, const char *maintain_json, char *err, uint64_t errcap,
@@ -741,5 +1127,80 @@ int ghive_plan_describe(const ghive_plan *plan, char *buf, uint64_t cap, uint64_
 }
 
 void ghive_plan_free(ghive_plan *plan) { delete plan; }
+
+}  // extern "C"
+
+extern "C" {
+
+int ghive_plan_execute(ghive_ctx *ctx, const ghive_plan *plan, const char *vertex, const ghive_plan_input *inputs,
+                       uint32_t ninputs, char *err, uint64_t errcap, ghive_result **out) {
+  if (!ctx || !plan || !vertex || !out || (ninputs && !inputs)) {
+    set_err(err, errcap, "plan_execute: NULL argument");
+    return GHIVE_ERR_INVALID;
+  }
+  *out = nullptr;
+  auto vit = plan->vertex_root.find(vertex);
+  if (vit == plan->vertex_root.end()) {
+    set_err(err, errcap, std::string("plan_execute: no vertex '") + vertex + "'");
+    return GHIVE_ERR_INVALID;
+  }
+  Executor X{ctx, *plan, vertex, {}, {}, {}};
+  for (uint32_t i = 0; i < ninputs; ++i)
+    if (inputs[i].name) X.inputs[inputs[i].name] = &inputs[i];
+  ExecTable t;
+  int st;
+  try {
+    st = X.run(vit->second, t);
+    if (st == GHIVE_OK && vit->second == plan->root) {
+      st = X.finish_root(vit->second, t);
+      // a LIMIT below the root applies after the ORDER BY
+      for (int node = plan->root; st == GHIVE_OK && node >= 0;) {
+        const Op &o = plan->ops[node];
+        if (o.kind == "Limit" && o.limit >= 0 && (uint64_t)o.limit < t.rows) t.rows = (uint64_t)o.limit;
+        int next = -1;
+        for (int c : o.children)
+          if (plan->ops[c].vertex == vertex && next < 0) next = c;
+        node = next;
+      }
+    }
+  } catch (const std::exception &ex) {
+    st = GHIVE_ERR_INVALID;
+    X.err = std::string("plan_execute: ") + ex.what();
+  }
+  if (st == GHIVE_OK) (void)ghive_sync(ctx);
+  if (st != GHIVE_OK) {
+    for (void *p : X.owned) ghive_free(ctx, p);
+    set_err(err, errcap, X.err);
+    return st;
+  }
+  auto *R = new ghive_result();
+  R->names = t.names;
+  R->cols = t.cols;
+  R->rows = t.rows;
+  for (auto &c : R->cols) c.row_num = t.rows;
+  R->owned = X.owned;  // intermediates included; released together
+  *out = R;
+  return GHIVE_OK;
+}
+
+int ghive_result_info(const ghive_result *r, uint64_t *rows, uint32_t *ncols) {
+  if (!r) return GHIVE_ERR_INVALID;
+  if (rows) *rows = r->rows;
+  if (ncols) *ncols = (uint32_t)r->cols.size();
+  return GHIVE_OK;
+}
+
+int ghive_result_column(const ghive_result *r, uint32_t i, ghive_column *col, const char **name) {
+  if (!r || i >= r->cols.size()) return GHIVE_ERR_INVALID;
+  if (col) *col = r->cols[i];
+  if (name) *name = r->names[i].c_str();
+  return GHIVE_OK;
+}
+
+void ghive_result_free(ghive_ctx *ctx, ghive_result *r) {
+  if (!r) return;
+  for (void *p : r->owned) ghive_free(ctx, p);
+  delete r;
+}
 
 }  // extern "C"

@@ -359,6 +359,30 @@ def sort(ctx, table, key_cols, orders):
     return perm[: table.row_num]
 
 
+def _column_of(t):
+    c = abi.Column()
+    c.type = _TYPE_OF_DTYPE[t.dtype]
+    c.data = t.data_ptr() if t.numel() else None
+    c.row_num = t.numel()
+    return c
+
+
+def select_arith(ctx, a, b, op):
+    """CalculationSelectPredicate::process (gpu_impl/src/Operator/SelectPredicate.cu:84-409):
+    a op b elementwise over device tensors (a 1-element tensor broadcasts). The result dtype is
+    float64 for '/' or a floating operand, int32 for int32 op int32, else int64."""
+    if not (a.is_cuda and b.is_cuda):
+        raise ValueError("operands must be on the GPU")
+    a, b = a.contiguous(), b.contiguous()
+    n = max(a.numel(), b.numel()) if min(a.numel(), b.numel()) == 1 else a.numel()
+    fp = a.dtype in (torch.float32, torch.float64) or b.dtype in (torch.float32, torch.float64)
+    dt = torch.float64 if op == "/" or fp else (torch.int32 if a.dtype == b.dtype == torch.int32 else torch.int64)
+    out = torch.empty(n, dtype=dt, device=a.device)
+    ca, cb, co = _column_of(a), _column_of(b), _column_of(out)
+    ctx.check(lib().ghive_select_arith(ctx.h, C.byref(ca), C.byref(cb), op.encode(), n, C.byref(co)))
+    return out
+
+
 class Fragment:
     """Fused CS1 fragment: fact table, star joins (fact column, HashTable), keys, aggregates."""
 
@@ -611,6 +635,38 @@ class Plan:
                     if self.op(c)["kind"] not in ("ReduceSink", "FileSink", "Union")]
             node = kids[0] if kids else None
         return out
+
+    def execute(self, ctx, vertex, inputs):
+        """Operator::execute over one vertex (ghive_plan_execute). inputs: {name: (Table, column
+        names)} -- a base table by its TableScan name ("db@table") or another vertex's result by the
+        vertex name. Returns (column names, [device tensors]); the tensors are copies owned by torch."""
+        keep, arr = [], (abi.PlanInput * max(len(inputs), 1))()
+        for i, (name, (table, names)) in enumerate(inputs.items()):
+            cn = (C.c_char_p * max(len(names), 1))(*[n.encode() for n in names])
+            keep += [table, cn]
+            arr[i].name = name.encode()
+            arr[i].table = C.pointer(table.table)
+            arr[i].column_names = C.cast(cn, C.POINTER(C.c_char_p))
+        err = C.create_string_buffer(512)
+        r = C.c_void_p()
+        st = lib().ghive_plan_execute(ctx.h, self.h, vertex.encode(), arr, len(inputs), err, len(err), C.byref(r))
+        if st != abi.OK:
+            raise GhiveError(st, err.value.decode(errors="replace"))
+        try:
+            rows, ncols = C.c_uint64(), C.c_uint32()
+            ctx.check(lib().ghive_result_info(r, C.byref(rows), C.byref(ncols)))
+            names, cols = [], []
+            for i in range(ncols.value):
+                c, nm = abi.Column(), C.c_char_p()
+                ctx.check(lib().ghive_result_column(r, i, C.byref(c), C.byref(nm)))
+                t = torch.empty(rows.value, dtype=_DTYPE_OF_TYPE[c.type], device=f"cuda:{ctx.device}")
+                if rows.value:
+                    ctx.check(lib().ghive_memcpy(ctx.h, _p(t), c.data, rows.value * t.element_size(), 2))
+                names.append(nm.value.decode())
+                cols.append(t)
+            return names, cols
+        finally:
+            lib().ghive_result_free(ctx.h, r)
 
     def close(self):
         if self.h:

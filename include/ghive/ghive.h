@@ -302,6 +302,17 @@ int ghive_groupby(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel,
                   const uint32_t *keys, uint32_t nkeys, const ghive_agg *aggs, uint32_t naggs,
                   ghive_column *out_cols, uint64_t capacity, uint64_t *out_groups);
 
+/* ---- projection arithmetic ----------------------------------------------------------- */
+/* CalculationSelectPredicate::process (SelectPredicate.cu:84-409; functors
+ * SelectPredicate.hpp:193-234): out[i] = a[i] op b[i], op one of '+', '-', '*', '/'. A column of
+ * row_num 1 broadcasts (a literal). '/' gives DOUBLE; otherwise INT op INT gives INT (wraps like a
+ * Java int), other integer pairs LONG (wraps), any FLOAT/DOUBLE operand DOUBLE -- the reference
+ * accepts only equal operand types for + - * and these are Hive's widenings. A NULL operand gives
+ * NULL; x / 0 gives NULL (Hive; the reference divides). out->data: device buffer of n values of
+ * the result type (written to out->type). */
+int ghive_select_arith(ghive_ctx *ctx, const ghive_column *a, const ghive_column *b, char op,
+                       uint64_t n, ghive_column *out);
+
 /* ---- sort --------------------------------------------------------------------------- */
 /* SinkOperator ORDER BY (SinkOperator.cu:135-288; orders as SortOrder '+'/'-',
  * GroupByPredicate.hpp:17-20): LSD radix sort; writes the permutation of row ids so that
@@ -489,6 +500,33 @@ int ghive_plan_parse(const char *plan, const char *extended, const char *maintai
  * *len = bytes needed (without the terminator); GHIVE_ERR_CAPACITY when cap is too small. */
 int ghive_plan_describe(const ghive_plan *plan, char *buf, uint64_t cap, uint64_t *len);
 void ghive_plan_free(ghive_plan *plan);
+
+/* One input of a vertex: a base table named as the TableScan names it ("db@table"), or the result
+ * of another vertex named by that vertex ("Map 3"), with its column names (NULL: _col0.._colN-1).
+ * A reduce-side input lists its key columns first, as the sink that produced it. */
+typedef struct ghive_plan_input {
+  const char *name;
+  const ghive_table *table; /* device columns */
+  const char *const *column_names;
+} ghive_plan_input;
+typedef struct ghive_result ghive_result;
+/* Operator::execute over one vertex's operator chain (the execute() of each gpu_impl/src/Operator/ source; the
+ * reference's GPUProcess runs one vertex per call): TableScan binds its input by name and column
+ * names; Filter parses its predicate (ghive_parse_predicate) and compacts (ghive_filter_compact);
+ * Select aliases columns, reads KEY.reducesinkkeyN / VALUE._colN of a reduce input, or computes
+ * (x op y) with ghive_select_arith; MapJoin / MergeJoin run ghive_join + ghive_join_result
+ * (single-column keys; the outputs are the maintained columns, else the join outputs found by
+ * name); GroupBy runs ghive_groupby; the vertex's sink orders its columns keys first; at the
+ * plan's root the final ORDER BY (ghive_sort + ghive_gather) and LIMIT apply. The result holds
+ * device columns owned by it (release with ghive_result_free on the same context); rows of a
+ * group-by or join come in unspecified order unless the root sorts them. */
+int ghive_plan_execute(ghive_ctx *ctx, const ghive_plan *plan, const char *vertex,
+                       const ghive_plan_input *inputs, uint32_t ninputs, char *err, uint64_t errcap,
+                       ghive_result **out);
+int ghive_result_info(const ghive_result *r, uint64_t *rows, uint32_t *ncols);
+/* Column i (device data, owned by the result) and its name. */
+int ghive_result_column(const ghive_result *r, uint32_t i, ghive_column *col, const char **name);
+void ghive_result_free(ghive_ctx *ctx, ghive_result *r);
 
 /* Per-stage timing of the last fragment on this context (HIP events, milliseconds). */
 typedef struct ghive_stage_times {

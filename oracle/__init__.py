@@ -168,6 +168,42 @@ def filter_rows(columns, nodes, root):
     return out[:n].copy()
 
 
+_NULL_OF_NP = {np.dtype(np.int32): 2**31 - 1, np.dtype(np.int64): 2**63 - 1}
+
+
+def select_arith(a, b, op):
+    """Projection arithmetic a op b (CalculationSelectPredicate::process,
+    gpu_impl/src/Operator/SelectPredicate.cu:84-409; functors SelectPredicate.hpp:193-234) with
+    Hive's typing: '/' and any floating operand compute in float64; int32 op int32 stays int32
+    and wraps like a Java int; other integer pairs int64 (wrapping like a Java long). NULL (the
+    type's sentinel, NaN for floats) in either operand gives NULL; x / 0 gives NULL. A 1-element
+    operand broadcasts."""
+    a, b = np.asarray(a), np.asarray(b)
+    n = max(len(a), len(b)) if min(len(a), len(b)) == 1 else len(a)
+    ia = np.zeros(n, dtype=np.int64) if len(a) == 1 else np.arange(n)
+    ib = np.zeros(n, dtype=np.int64) if len(b) == 1 else np.arange(n)
+    a, b = a[ia], b[ib]
+
+    def nulls(x):
+        return np.isnan(x) if x.dtype.kind == "f" else x == _NULL_OF_NP[x.dtype]
+    null = nulls(a) | nulls(b)
+    if op == "/" or a.dtype.kind == "f" or b.dtype.kind == "f":
+        x, y = a.astype(np.float64), b.astype(np.float64)
+        with np.errstate(all="ignore"):
+            r = {"+": np.add, "-": np.subtract, "*": np.multiply, "/": np.divide}[op](x, y)
+        if op == "/":
+            null |= y == 0.0
+        r[null] = np.nan
+        return r
+    dt = np.int32 if a.dtype == b.dtype == np.int32 else np.int64
+    x, y = a.astype(np.int64).astype(np.uint64), b.astype(np.int64).astype(np.uint64)
+    r = {"+": np.add, "-": np.subtract, "*": np.multiply}[op](x, y)  # modulo 2^64
+    r = (r & np.uint64(0xFFFFFFFF)).astype(np.uint32).view(np.int32) if dt == np.int32 else r.view(np.int64)
+    r = r.copy()
+    r[null] = _NULL_OF_NP[np.dtype(dt)]
+    return r
+
+
 def mapjoin_inner(build_keys, probe_keys, build_sel=None, probe_sel=None, payload=None,
                   load_factor=0.75):
     """Hive map-join (inner) restated: (probe row ids, payloads) in Hive emission order."""

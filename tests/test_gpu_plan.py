@@ -1,0 +1,216 @@
+"""GPU parity of projection arithmetic (ghive_select_arith) against the oracle, and of whole SSB
+queries run vertex by vertex from their Hive plans (ghive_plan_execute) against the query's SQL
+restated in numpy. The plans are the reference's own EXPLAIN fixtures (tests/golden/plans/); the
+tables are synthetic, seeded."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import to_dev, to_np
+from ghive_amd import abi
+
+pytestmark = pytest.mark.gpu
+INT_NULL, LONG_NULL = abi.INT_NULL, abi.LONG_NULL
+PLANS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "plans")
+
+
+def _same(got, want):
+    """Bit-exact equality, NaN (DOUBLE NULL) matching NaN."""
+    assert got.dtype == want.dtype and got.shape == want.shape
+    if want.dtype.kind == "f":
+        assert np.array_equal(np.isnan(got), np.isnan(want))
+        m = ~np.isnan(want)
+        assert np.array_equal(got[m].view(np.uint64 if want.dtype == np.float64 else np.uint32),
+                              want[m].view(np.uint64 if want.dtype == np.float64 else np.uint32))
+    else:
+        assert np.array_equal(got, want)
+
+
+def _operand(rng, dtype, n):
+    if dtype == np.float64:
+        x = rng.standard_normal(n) * 1e3
+        x[rng.random(n) < 0.02] = np.nan
+        x[rng.random(n) < 0.05] = 0.0
+    elif dtype == np.float32:
+        x = (rng.standard_normal(n) * 1e2).astype(np.float32)
+        x[rng.random(n) < 0.02] = np.nan
+    elif dtype == np.int32:
+        x = rng.integers(-2**31, 2**31 - 1, n).astype(np.int32)
+        small = rng.random(n) < 0.3  # small values too, so that / gives exact small quotients
+        x[small] = rng.integers(-3, 4, int(small.sum()))
+        x[rng.random(n) < 0.02] = INT_NULL
+        x[rng.random(n) < 0.05] = 0
+    else:
+        x = rng.integers(-2**63, 2**63 - 1, n, dtype=np.int64)
+        x[rng.random(n) < 0.02] = LONG_NULL
+        x[rng.random(n) < 0.05] = 0
+    return x.astype(dtype)
+
+
+@pytest.mark.parametrize("ta,tb", [(np.int32, np.int32), (np.int64, np.int32), (np.int32, np.int64),
+                                   (np.int64, np.int64), (np.float64, np.int64), (np.float32, np.int32),
+                                   (np.float64, np.float64)])
+@pytest.mark.parametrize("op", ["+", "-", "*", "/"])
+def test_select_arith_vs_oracle(ctx, orc, ta, tb, op):
+    from ghive_amd import ops
+    rng = np.random.default_rng(hash((ta.__name__, tb.__name__, op)) % 2**32)
+    n = 300_001
+    a, b = _operand(rng, ta, n), _operand(rng, tb, n)
+    _same(to_np(ops.select_arith(ctx, to_dev(a), to_dev(b), op)), orc.select_arith(a, b, op))
+    # a literal on either side broadcasts
+    for lit in (b[:1], np.array([0], dtype=tb)):
+        _same(to_np(ops.select_arith(ctx, to_dev(a), to_dev(lit), op)), orc.select_arith(a, lit, op))
+        _same(to_np(ops.select_arith(ctx, to_dev(lit), to_dev(a[:1000]), op)), orc.select_arith(lit, a[:1000], op))
+
+
+def test_select_arith_edges(ctx, orc):
+    from ghive_amd import ops
+    from ghive_amd._lib import GhiveError
+    e = to_dev(np.zeros(0, dtype=np.int32))
+    assert ops.select_arith(ctx, e, e, "+").numel() == 0
+    a = to_dev(np.array([2147483000, INT_NULL, 4], dtype=np.int32))
+    b = to_dev(np.array([1000, 1, 0], dtype=np.int32))
+    assert to_np(ops.select_arith(ctx, a, b, "+")).tolist() == [-2147483296, INT_NULL, 4]
+    r = to_np(ops.select_arith(ctx, a, b, "/"))
+    assert r[0] == 2147483.0 and math.isnan(r[1]) and math.isnan(r[2])
+    with pytest.raises(GhiveError):
+        ops.select_arith(ctx, a, b, "%")
+    with pytest.raises(GhiveError):  # ragged operands
+        ops.select_arith(ctx, a, to_dev(np.array([1, 2], dtype=np.int32)), "+")
+
+
+# ---------------------------------------------------------------- whole queries -----------------
+def _read(name):
+    with open(os.path.join(PLANS, name)) as fh:
+        return fh.read()
+
+
+def _plan(q, maintain=None):
+    from ghive_amd import ops
+    return ops.Plan(_read(f"ssb_10_orc_transformed_{q}.txt"), _read(f"ssb_10_orc_transformed_{q}_extended.txt"),
+                    maintain)
+
+
+def _dates():
+    days = np.arange(np.datetime64("1992-01-01"), np.datetime64("1999-01-01"))
+    y = days.astype("datetime64[Y]").astype(np.int64) + 1970
+    m = days.astype("datetime64[M]").astype(np.int64) % 12 + 1
+    d = (days - days.astype("datetime64[M]")).astype(np.int64) + 1
+    return (y * 10000 + m * 100 + d).astype(np.int64), y.astype(np.int32)
+
+
+def _fk(rng, keys, n, miss=0.05, null=0.01):
+    """Foreign keys drawn from `keys`, with some that match nothing and some NULL."""
+    x = keys[rng.integers(0, len(keys), n)].astype(np.int64)
+    x[rng.random(n) < miss] = -7
+    x[rng.random(n) < null] = LONG_NULL
+    return x
+
+
+def _run(plan, ctx, vertex, inputs):
+    from ghive_amd import ops
+    ins = {name: (ops.Table([to_dev(c) for c in cols]), names) for name, (cols, names) in inputs.items()}
+    names, cols = plan.execute(ctx, vertex, ins)
+    return names, [to_np(c) for c in cols]
+
+
+def _concat(parts):
+    return [np.concatenate([p[i] for p in parts]) for i in range(len(parts[0]))]
+
+
+def test_ssb_q1_1_three_vertices(ctx, orc):
+    """SELECT sum(lo_extendedprice*lo_discount) FROM lineorder, dates WHERE lo_orderdate = d_datekey
+    AND d_year = 1993 AND lo_discount BETWEEN 1 AND 3 AND lo_quantity < 25 -- Map 3 (dates), Map 1
+    (lineorder, as two map tasks), Reducer 2 (merges the partial sums)."""
+    rng = np.random.default_rng(101)
+    dk, dy = _dates()
+    n = 400_000
+    od = _fk(rng, dk, n)
+    qty = rng.integers(1, 51, n).astype(np.float64)
+    qty[rng.random(n) < 0.01] = np.nan
+    price = np.round(rng.random(n) * 1e5, 2)
+    disc = rng.integers(0, 11, n).astype(np.float64)
+    disc[rng.random(n) < 0.01] = np.nan
+    p = _plan("1_1")
+    lo_names = ["lo_orderdate", "lo_quantity", "lo_extendedprice", "lo_discount"]
+    n3, c3 = _run(p, ctx, "Map 3", {"ssb_10_orc_transformed@dates": ([dk, dy], ["d_datekey", "d_year"])})
+    assert n3 == ["_col0"] and np.array_equal(np.sort(c3[0]), np.sort(dk[dy == 1993]))
+    parts = []
+    for lo, hi in ((0, n // 3), (n // 3, n)):  # two map tasks over splits of lineorder
+        names, cols = _run(p, ctx, "Map 1", {
+            "ssb_10_orc_transformed@lineorder": ([od[lo:hi], qty[lo:hi], price[lo:hi], disc[lo:hi]], lo_names),
+            "Map 3": (c3, n3)})
+        assert names == ["_col0"] and len(cols[0]) == 1
+        parts.append(cols)
+    names, cols = _run(p, ctx, "Reducer 2", {"Map 1": (_concat(parts), ["_col0"])})
+    assert names == ["_col0"] and len(cols[0]) == 1
+    year93 = np.isin(od, dk[dy == 1993])
+    m = year93 & (disc >= 1) & (disc <= 3) & (qty < 25)
+    rev = price * disc
+    want = math.fsum([math.fsum(rev[lo:hi][m[lo:hi]].tolist()) for lo, hi in ((0, n // 3), (n // 3, n))])
+    assert cols[0][0] == want
+
+
+def test_ssb_q2_1_seven_vertices_with_order_by(ctx, orc):
+    """SELECT sum(lo_revenue), d_year, p_brand1 FROM lineorder, dates, part, supplier WHERE
+    lo_orderdate = d_datekey AND lo_partkey = p_partkey AND lo_suppkey = s_suppkey AND p_category = 1
+    AND s_region = 1 GROUP BY d_year, p_brand1 ORDER BY d_year, p_brand1 -- map joins in Map 1 and
+    Reducer 2, a merge join in Reducer 2 over two shuffled inputs, partial and final group-by, and
+    the final ORDER BY. p_brand1 is INT here: the group-by packs its keys into 64 bits."""
+    rng = np.random.default_rng(202)
+    dk, dy = _dates()
+    ns, npart, n = 2000, 20000, 300_000
+    s_key = rng.permutation(10 * ns)[:ns].astype(np.int64) + 1
+    s_region = rng.integers(0, 5, ns).astype(np.int64)
+    p_key = rng.permutation(10 * npart)[:npart].astype(np.int64) + 1
+    p_cat = rng.integers(0, 5, npart).astype(np.int64)
+    p_brand = rng.integers(0, 40, npart).astype(np.int32) + 1000
+    lo_part, lo_supp, lo_date = _fk(rng, p_key, n), _fk(rng, s_key, n), _fk(rng, dk, n)
+    lo_rev = np.round(rng.random(n) * 1e6, 2)
+    maintain = ('{"MAPJOIN_52":{"0":["_col0","_col2","_col3"],"1":[]},'
+                '"MERGEJOIN_43":{"0":["VALUE._col1","VALUE._col2"],"1":["VALUE._col1"]},'
+                '"MAPJOIN_44":{"0":["_col3","_col8"],"1":["_col1"]}}')
+    p = _plan("2_1", maintain)
+    v5 = _run(p, ctx, "Map 5", {"ssb_10_orc_transformed@supplier": ([s_key, s_region], ["s_suppkey", "s_region"])})
+    v6 = _run(p, ctx, "Map 6", {"ssb_10_orc_transformed@part": ([p_key, p_cat, p_brand],
+                                                                 ["p_partkey", "p_category", "p_brand1"])})
+    v7 = _run(p, ctx, "Map 7", {"ssb_10_orc_transformed@dates": ([dk, dy], ["d_datekey", "d_year"])})
+    assert v6[0] == ["_col0", "_col2"] and v7[0] == ["_col0", "_col1"]
+    lo_names = ["lo_partkey", "lo_suppkey", "lo_orderdate", "lo_revenue"]
+    v1 = _run(p, ctx, "Map 1", {"ssb_10_orc_transformed@lineorder": ([lo_part, lo_supp, lo_date, lo_rev], lo_names),
+                                "Map 5": (v5[1], v5[0])})
+    assert v1[0] == ["_col0", "_col2", "_col3"]
+    v2 = _run(p, ctx, "Reducer 2", {"Map 1": (v1[1], v1[0]), "Map 6": (v6[1], v6[0]), "Map 7": (v7[1], v7[0])})
+    v3 = _run(p, ctx, "Reducer 3", {"Reducer 2": (v2[1], v2[0])})
+    assert v3[0] == ["_col1", "_col2", "_col0"]  # keys first
+    names, (s, year, brand) = _run(p, ctx, "Reducer 4", {"Reducer 3": (v3[1], v3[0])})
+    assert names == ["_col0", "_col1", "_col2"]
+    # the query restated
+    s_ok = dict(zip(s_key.tolist(), (s_region == 1).tolist()))
+    p_of = {k: b for k, c, b in zip(p_key.tolist(), p_cat.tolist(), p_brand.tolist()) if c == 1}
+    y_of = dict(zip(dk.tolist(), dy.tolist()))
+    groups = {}
+    for pk, sk, d, r in zip(lo_part.tolist(), lo_supp.tolist(), lo_date.tolist(), lo_rev.tolist()):
+        if s_ok.get(sk) and pk in p_of and d in y_of:
+            groups.setdefault((y_of[d], p_of[pk]), []).append(r)
+    want = sorted(groups)
+    assert list(zip(year.tolist(), brand.tolist())) == want
+    assert year.dtype == np.int32 and brand.dtype == np.int32
+    # one partial sum per group from the map side, summed exactly in the reducer
+    assert s.tolist() == [math.fsum(groups[k]) for k in want]
+
+
+def test_plan_execute_errors(ctx):
+    from ghive_amd._lib import GhiveError
+    p = _plan("1_1")
+    with pytest.raises(GhiveError) as e:
+        _run(p, ctx, "Map 9", {})
+    assert e.value.status == abi.ERR_INVALID and "Map 9" in str(e.value)
+    with pytest.raises(GhiveError) as e:  # the dates table is not given
+        _run(p, ctx, "Map 3", {})
+    assert "ssb_10_orc_transformed@dates" in str(e.value)
+    with pytest.raises(GhiveError) as e:  # a column missing from the input
+        _run(p, ctx, "Map 3", {"ssb_10_orc_transformed@dates": ([np.zeros(3, np.int64)], ["d_datekey"])})
+    assert "d_year" in str(e.value)

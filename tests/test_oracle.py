@@ -321,6 +321,47 @@ def test_exact_sum_against_fsum(orc):
     assert orc.exact_sum(np.array([0.1] * 10)) == math.fsum([0.1] * 10)
 
 
+# ---------------------------------------------------------------- projection arithmetic -------
+def _java_int(v):
+    return (v + 2**31) % 2**32 - 2**31
+
+
+def _java_long(v):
+    return (v + 2**63) % 2**64 - 2**63
+
+
+def test_select_arith_known_answers(orc):
+    i32, i64 = np.int32, np.int64
+    a = np.array([2147483000, -5, 7, INT_NULL, 0, -2147483648], dtype=i32)
+    b = np.array([1000, 3, -2, 1, 0, -1], dtype=i32)
+    # INT op INT wraps like a Java int; NULL in, NULL out
+    for op, f in (("+", lambda x, y: x + y), ("-", lambda x, y: x - y), ("*", lambda x, y: x * y)):
+        r = orc.select_arith(a, b, op)
+        assert r.dtype == i32
+        want = [INT_NULL if x == INT_NULL else _java_int(f(int(x), int(y))) for x, y in zip(a, b)]
+        assert r.tolist() == want
+    assert orc.select_arith(a, b, "+")[0] == -2147483296
+    # '/' is DOUBLE; x / 0 is NULL (NaN)
+    r = orc.select_arith(a, b, "/")
+    assert r.dtype == np.float64
+    assert r[1] == -5 / 3 and r[2] == -3.5 and math.isnan(r[3]) and math.isnan(r[4])
+    assert r[5] == 2147483648.0
+    # INT op LONG widens to LONG and wraps like a Java long
+    la = np.array([2**62, -(2**63), 5, LONG_NULL], dtype=i64)
+    lb = np.array([4, 5, 6, 7], dtype=i32)
+    r = orc.select_arith(la, lb, "*")
+    assert r.dtype == i64
+    assert r.tolist() == [_java_long(2**64), _java_long(-(2**63) * 5), 30, LONG_NULL]
+    assert orc.select_arith(la, lb, "-").tolist() == [2**62 - 4, _java_long(-(2**63) - 5), -1, LONG_NULL]
+    # a floating operand makes DOUBLE; NaN is NULL; a 1-row operand broadcasts
+    d = np.array([1.5, np.nan, -2.0], dtype=np.float64)
+    r = orc.select_arith(d, np.array([2], dtype=i32), "*")
+    assert r[0] == 3.0 and math.isnan(r[1]) and r[2] == -4.0
+    r = orc.select_arith(np.array([10.0], dtype=np.float32), np.array([4, 0, INT_NULL], dtype=i32), "/")
+    assert r[0] == 2.5 and math.isnan(r[1]) and math.isnan(r[2])
+    assert len(orc.select_arith(np.zeros(0, i32), np.zeros(0, i32), "+")) == 0
+
+
 # ---------------------------------------------------------------- sort -----------------------
 def test_sort_total_order(orc):
     rng = np.random.default_rng(4)
