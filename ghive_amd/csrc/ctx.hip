@@ -117,6 +117,8 @@ int check_table(ghive_ctx *ctx, const ghive_table *t, const char *what) {
                        (unsigned long long)c.row_num, (unsigned long long)t->row_num);
     if (t->row_num && !c.data)
       return set_error(ctx, GHIVE_ERR_INVALID, "%s: column %u has no data", what, i);
+    if (c.type == GHIVE_STRING && t->row_num && !c.aux)
+      return set_error(ctx, GHIVE_ERR_INVALID, "%s: STRING column %u has no (start,end) pairs", what, i);
   }
   return GHIVE_OK;
 }

@@ -246,6 +246,140 @@ __global__ __launch_bounds__(kGbPartThreads) void k_gb_part_agg(KeySpec K, Group
   }
 }
 
+// ---- wide keys: composite keys wider than 64 bits, more than kMaxKeys columns, or STRING -------
+// A dictionary gives every row a dense group code; the 64-bit group-by then aggregates on the
+// code, and each group's key values are gathered from its representative row. A dictionary slot
+// holds (32 hash bits << 32 | representative row id): the first row of a key to claim a slot
+// represents it, and a row matches a slot when the hash bits agree and its key columns equal the
+// representative's. Slots never change once claimed and the input columns are immutable, so a
+// reader never waits for a writer. Row ids stay below 2^32 - 1, so no slot word equals EMPTY.
+constexpr int kMaxWideKeys = 8;
+
+struct WideKeySpec {
+  int32_t nkeys;
+  int32_t type[kMaxWideKeys];
+  const void *data[kMaxWideKeys];    // values; STRING: the character blob
+  const int32_t *aux[kMaxWideKeys];  // STRING: (start,end) per row, (-1,-1) = NULL
+};
+
+struct WideOut {
+  void *col[kMaxWideKeys];
+  int32_t *aux[kMaxWideKeys];
+};
+
+// Key bits as the 64-bit form compares them: every NaN is the one NULL.
+__device__ __forceinline__ uint64_t wide_bits(int type, const void *p, uint64_t r) {
+  switch (type) {
+    case GHIVE_INT: return (uint32_t) static_cast<const int32_t *>(p)[r];
+    case GHIVE_LONG: return (uint64_t) static_cast<const int64_t *>(p)[r];
+    case GHIVE_DOUBLE: {
+      const double d = static_cast<const double *>(p)[r];
+      return __builtin_isnan(d) ? 0x7ff8000000000000ull : (uint64_t)__double_as_longlong(d);
+    }
+    default: {
+      const float f = static_cast<const float *>(p)[r];
+      return __builtin_isnan(f) ? 0x7fc00000u : (uint32_t)__float_as_uint(f);
+    }
+  }
+}
+
+__device__ uint64_t wide_hash(const WideKeySpec &K, uint64_t r) {
+  uint64_t h = 0x243f6a8885a308d3ull;
+  for (int k = 0; k < K.nkeys; ++k) {
+    if (K.type[k] == GHIVE_STRING) {
+      const int32_t s = K.aux[k][2 * r], e = K.aux[k][2 * r + 1];
+      uint64_t x = 0x9e3779b97f4a7c15ull;  // NULL
+      if (s >= 0) {
+        const uint8_t *b = static_cast<const uint8_t *>(K.data[k]);
+        x = 0xcbf29ce484222325ull;  // FNV-1a over the bytes, then the length
+        for (int32_t i = s; i < e; ++i) x = (x ^ b[i]) * 0x100000001b3ull;
+        x ^= (uint64_t)(uint32_t)(e - s);
+      }
+      h = mix64(h ^ x);
+    } else {
+      h = mix64(h ^ wide_bits(K.type[k], K.data[k], r));
+    }
+  }
+  return h;
+}
+
+__device__ bool wide_equal(const WideKeySpec &K, uint64_t r1, uint64_t r2) {
+  for (int k = 0; k < K.nkeys; ++k) {
+    if (K.type[k] == GHIVE_STRING) {
+      const int32_t s1 = K.aux[k][2 * r1], e1 = K.aux[k][2 * r1 + 1];
+      const int32_t s2 = K.aux[k][2 * r2], e2 = K.aux[k][2 * r2 + 1];
+      if (s1 < 0 || s2 < 0) {
+        if ((s1 < 0) != (s2 < 0)) return false;
+        continue;
+      }
+      if (e1 - s1 != e2 - s2) return false;
+      const uint8_t *b = static_cast<const uint8_t *>(K.data[k]);
+      for (int32_t i = 0; i < e1 - s1; ++i)
+        if (b[s1 + i] != b[s2 + i]) return false;
+    } else if (wide_bits(K.type[k], K.data[k], r1) != wide_bits(K.type[k], K.data[k], r2)) {
+      return false;
+    }
+  }
+  return true;
+}
+
+// codes[r] = dictionary slot of row r's key. cnt[0] counts keys, cnt[1] is set on overflow.
+__global__ __launch_bounds__(256) void k_gbw_codes(WideKeySpec K, uint64_t *dict, uint64_t mask, uint64_t cap,
+                                                   unsigned long long *cnt, const uint32_t *sel, uint64_t n,
+                                                   uint32_t *codes) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    const uint64_t r = sel ? sel[i] : i;
+    const uint64_t h = wide_hash(K, r);
+    const uint64_t tag = h >> 32, mine = (tag << 32) | r;
+    uint64_t s = h & mask;
+    bool found = false;
+    for (uint64_t step = 0; step <= mask; ++step) {
+      uint64_t w = __atomic_load_n(dict + s, __ATOMIC_RELAXED);
+      if (w == kEmptyKey) {
+        const unsigned long long old = atomicCAS((unsigned long long *)(dict + s), kEmptyKey, mine);
+        if (old == kEmptyKey) {
+          if (atomicAdd(cnt, 1ull) >= cap) atomicOr(cnt + 1, 1ull);
+          found = true;
+          break;
+        }
+        w = old;
+      }
+      if ((w >> 32) == tag && wide_equal(K, w & 0xffffffffull, r)) {
+        found = true;
+        break;
+      }
+      s = (s + 1) & mask;
+    }
+    if (!found) atomicOr(cnt + 1, 1ull);
+    codes[r] = (uint32_t)s;
+  }
+}
+
+// Key columns of group g from the representative row of its code.
+__global__ __launch_bounds__(256) void k_gbw_emit_keys(WideKeySpec K, const uint64_t *dict, const int32_t *gcode,
+                                                       uint64_t groups, WideOut O) {
+  uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; g < groups; g += stride) {
+    const uint64_t r = dict[(uint32_t)gcode[g]] & 0xffffffffull;
+    for (int k = 0; k < K.nkeys; ++k) {
+      switch (K.type[k]) {
+        case GHIVE_LONG:
+        case GHIVE_DOUBLE:
+          static_cast<uint64_t *>(O.col[k])[g] = static_cast<const uint64_t *>(K.data[k])[r];
+          break;
+        case GHIVE_STRING:
+          O.aux[k][2 * g] = K.aux[k][2 * r];
+          O.aux[k][2 * g + 1] = K.aux[k][2 * r + 1];
+          break;
+        default: static_cast<uint32_t *>(O.col[k])[g] = static_cast<const uint32_t *>(K.data[k])[r]; break;
+      }
+    }
+  }
+}
+
 __global__ void k_gt_preinsert(GroupTableDev T, uint64_t key) {
   if (threadIdx.x == 0 && blockIdx.x == 0) (void)gt_find_insert(T, key);
 }
@@ -478,26 +612,39 @@ int ghive_groupby(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel,
   if (!out_groups) return set_error(ctx, GHIVE_ERR_INVALID, "groupby: out_groups is NULL");
   if (!sel && n != table->row_num)
     return set_error(ctx, GHIVE_ERR_INVALID, "groupby: n != rows without a selection");
-  if (nkeys > (uint32_t)kMaxKeys) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "groupby: %u keys (max %d)", nkeys, kMaxKeys);
+  if (nkeys > (uint32_t)kMaxWideKeys)
+    return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "groupby: %u keys (max %d)", nkeys, kMaxWideKeys);
   if (naggs > (uint32_t)kMaxAggs) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "groupby: %u aggregates (max %d)", naggs, kMaxAggs);
   if ((nkeys && !keys) || (naggs && !aggs)) return set_error(ctx, GHIVE_ERR_INVALID, "groupby: NULL keys/aggs");
   if (nkeys + naggs && !out_cols) return set_error(ctx, GHIVE_ERR_INVALID, "groupby: out_cols is NULL");
+  // keys that pack into 64 bits aggregate on the packed key; wider ones through the dictionary
   KeySpec K;
   memset(&K, 0, sizeof K);
-  K.nkeys = (int)nkeys;
-  int shift = 0;
+  WideKeySpec WK;
+  memset(&WK, 0, sizeof WK);
+  WK.nkeys = (int)nkeys;
+  bool wide = nkeys > (uint32_t)kMaxKeys;
+  int bits = 0;
   for (uint32_t k = 0; k < nkeys; ++k) {
     if (keys[k] >= table->ncols) return set_error(ctx, GHIVE_ERR_INVALID, "groupby: key column %u", keys[k]);
     const ghive_column &c = table->columns[keys[k]];
-    if (!is_numeric_type(c.type))
-      return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "groupby: STRING keys are not supported");
-    K.type[k] = c.type;
-    K.data[k] = c.data;
-    K.shift[k] = shift;
-    shift += 8 * type_width(c.type);
+    WK.type[k] = c.type;
+    WK.data[k] = c.data;
+    WK.aux[k] = static_cast<const int32_t *>(c.aux);
+    if (c.type == GHIVE_STRING) wide = true;
+    else bits += 8 * type_width(c.type);
   }
-  if (shift > 64)
-    return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "groupby: keys are %d bits wide (max 64)", shift);
+  if (bits > 64) wide = true;
+  if (!wide) {
+    K.nkeys = (int)nkeys;
+    int shift = 0;
+    for (uint32_t k = 0; k < nkeys; ++k) {
+      K.type[k] = WK.type[k];
+      K.data[k] = WK.data[k];
+      K.shift[k] = shift;
+      shift += 8 * type_width(WK.type[k]);
+    }
+  }
   GroupTableDev T;
   memset(&T, 0, sizeof T);
   T.naggs = (int)naggs;
@@ -538,8 +685,16 @@ int ghive_groupby(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel,
     o.type = A.out_type;
   }
   for (uint32_t k = 0; k < nkeys; ++k) {
-    if (capacity && !out_cols[k].data) return set_error(ctx, GHIVE_ERR_INVALID, "groupby: output column %u has no buffer", k);
-    out_cols[k].type = K.type[k];
+    ghive_column &o = out_cols[k];
+    if (WK.type[k] == GHIVE_STRING) {  // (start,end) pairs into the caller's aux; the blob is the input's
+      if (capacity && !o.aux)
+        return set_error(ctx, GHIVE_ERR_INVALID, "groupby: STRING output column %u has no aux buffer", k);
+      o.data = const_cast<void *>(WK.data[k]);
+      o.char_size = table->columns[keys[k]].char_size;
+    } else if (capacity && !o.data) {
+      return set_error(ctx, GHIVE_ERR_INVALID, "groupby: output column %u has no buffer", k);
+    }
+    o.type = WK.type[k];
   }
   T.nwords = word;
   // exact double sums: pick the fixed-point LSB from max |x| and the row count
@@ -566,6 +721,38 @@ int ghive_groupby(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel,
       A.scale_e = std::ilogb(mx) + 1 + lg_n - 126;
       if (A.scale_e < -1074) A.scale_e = -1074;
     }
+  }
+  DevBuf wdict, wcodes, wgcode;
+  uint64_t *dict = nullptr;
+  if (wide) {
+    if (capacity > (1ull << 31))
+      return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "groupby: wide keys with a group bound above 2^31");
+    const uint64_t slots = next_pow2(capacity * 2 < 64 ? 64 : capacity * 2);
+    GH_TRY(wdict.alloc(ctx, slots * 8));
+    GH_TRY(wcodes.alloc(ctx, (table->row_num ? table->row_num : 1) * 4));
+    GH_TRY(wgcode.alloc(ctx, (capacity ? capacity : 1) * 4));
+    dict = wdict.as<uint64_t>();
+    GH_TRY(fill_u64(ctx, dict, kEmptyKey, slots));
+    GH_TRY(fill_u64(ctx, ctx->d_counters, 0, 2));
+    if (n) {
+      const uint64_t g = (n + 255) / 256, cap = (uint64_t)ctx->cus * 16;
+      k_gbw_codes<<<(uint32_t)(g > cap ? cap : g), 256, 0, ctx->stream>>>(
+          WK, dict, slots - 1, capacity, reinterpret_cast<unsigned long long *>(ctx->d_counters), sel, n,
+          wcodes.as<uint32_t>());
+      GH_LAUNCH(ctx);
+    }
+    uint64_t c[2] = {0, 0};
+    GH_TRY(d2h(ctx, c, ctx->d_counters, sizeof c));
+    if (c[1] || c[0] > capacity) {
+      *out_groups = c[0] > capacity ? c[0] : capacity + 1;
+      for (uint32_t k = 0; k < nkeys + naggs; ++k) out_cols[k].row_num = 0;
+      return set_error(ctx, GHIVE_ERR_CAPACITY, "group-by: more than %llu groups (capacity)",
+                       (unsigned long long)capacity);
+    }
+    K.nkeys = 1;  // aggregate on the dictionary code
+    K.type[0] = GHIVE_INT;
+    K.shift[0] = 0;
+    K.data[0] = wcodes.as<uint32_t>();
   }
   DevBuf dkeys, dacc, dcnt;
   // the high-cardinality form: a large group bound over many rows (or forced by the option)
@@ -603,11 +790,32 @@ int ghive_groupby(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel,
   }
   OutSpec O;
   memset(&O, 0, sizeof O);
-  for (uint32_t c = 0; c < nkeys + naggs; ++c) {
-    O.col[c] = out_cols[c].data;
-    O.type[c] = out_cols[c].type;
+  if (wide) {
+    O.col[0] = wgcode.as<int32_t>();
+    O.type[0] = GHIVE_INT;
+    for (uint32_t a = 0; a < naggs; ++a) {
+      O.col[1 + a] = out_cols[nkeys + a].data;
+      O.type[1 + a] = out_cols[nkeys + a].type;
+    }
+  } else {
+    for (uint32_t c = 0; c < nkeys + naggs; ++c) {
+      O.col[c] = out_cols[c].data;
+      O.type[c] = out_cols[c].type;
+    }
   }
   int st = group_table_emit(ctx, K, T, O, capacity, out_groups);
+  if (st == GHIVE_OK && wide && *out_groups) {
+    WideOut WO;
+    memset(&WO, 0, sizeof WO);
+    for (uint32_t k = 0; k < nkeys; ++k) {
+      WO.col[k] = out_cols[k].data;
+      WO.aux[k] = static_cast<int32_t *>(out_cols[k].aux);
+    }
+    const uint64_t g = (*out_groups + 255) / 256, cap = (uint64_t)ctx->cus * 16;
+    k_gbw_emit_keys<<<(uint32_t)(g > cap ? cap : g), 256, 0, ctx->stream>>>(WK, dict, wgcode.as<int32_t>(),
+                                                                             *out_groups, WO);
+    GH_LAUNCH(ctx);
+  }
   for (uint32_t c = 0; c < nkeys + naggs; ++c) out_cols[c].row_num = st == GHIVE_OK ? *out_groups : 0;
   return st;
 }

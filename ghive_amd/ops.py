@@ -117,8 +117,48 @@ def _p(t):
     return C.c_void_p(t.data_ptr()) if t is not None and t.numel() else None
 
 
+class StringColumn:
+    """STRING column (gpu_impl/inc/DataFlow/Column.hpp:13-35, ghive/GTable.java:231-259): a UTF-8
+    character blob (uint8 tensor) and int32 (start,end) pairs per row, (-1,-1) = NULL."""
+
+    def __init__(self, blob, idx):
+        self.blob = blob.contiguous()
+        self.idx = idx.contiguous().view(-1)
+        if self.blob.dtype != torch.uint8 or self.idx.dtype != torch.int32 or self.idx.numel() % 2:
+            raise TypeError("STRING column: uint8 blob and int32 (start,end) pairs")
+        if self.blob.numel() == 0:  # the library wants a blob address even when every string is empty
+            self.blob = torch.zeros(1, dtype=torch.uint8, device=self.blob.device)
+        self.is_cuda = self.blob.is_cuda
+
+    @staticmethod
+    def from_list(values, device="cuda"):
+        """values: str / bytes / None (NULL)."""
+        blob, idx, o = bytearray(), [], 0
+        for v in values:
+            if v is None:
+                idx += [-1, -1]
+                continue
+            b = v.encode() if isinstance(v, str) else bytes(v)
+            blob += b
+            idx += [o, o + len(b)]
+            o += len(b)
+        return StringColumn(torch.tensor(list(blob), dtype=torch.uint8, device=device),
+                            torch.tensor(idx, dtype=torch.int32, device=device))
+
+    def numel(self):
+        return self.idx.numel() // 2
+
+    def contiguous(self):
+        return self
+
+    def to_list(self):
+        """Host list of bytes (None for NULL)."""
+        blob, idx = bytes(self.blob.cpu().numpy().tobytes()), self.idx.cpu().numpy()
+        return [None if idx[2 * i] < 0 else blob[idx[2 * i]: idx[2 * i + 1]] for i in range(len(idx) // 2)]
+
+
 class Table:
-    """Table of device columns (gpu_impl/inc/DataFlow/Table.hpp)."""
+    """Table of device columns (gpu_impl/inc/DataFlow/Table.hpp): torch tensors or StringColumns."""
 
     def __init__(self, columns, key_num=0, host=False):
         """host=True: columns in host memory, for serving requests with host I/O."""
@@ -128,7 +168,7 @@ class Table:
         for c in self.columns:
             if c.numel() != rows:
                 raise ValueError("ragged table")
-            if c.dtype not in _TYPE_OF_DTYPE:
+            if not isinstance(c, StringColumn) and c.dtype not in _TYPE_OF_DTYPE:
                 raise TypeError(f"unsupported column dtype {c.dtype}")
             if c.is_cuda == host:
                 raise ValueError("columns must be in host memory" if host else "columns must be on the GPU")
@@ -136,8 +176,14 @@ class Table:
         self.row_num = rows
         self.cols = (abi.Column * max(n, 1))()
         for i, c in enumerate(self.columns):
-            self.cols[i].type = _TYPE_OF_DTYPE[c.dtype]
-            self.cols[i].data = c.data_ptr() if c.numel() else None
+            if isinstance(c, StringColumn):
+                self.cols[i].type = abi.STRING
+                self.cols[i].data = c.blob.data_ptr()
+                self.cols[i].aux = c.idx.data_ptr() if rows else None
+                self.cols[i].char_size = c.blob.numel()
+            else:
+                self.cols[i].type = _TYPE_OF_DTYPE[c.dtype]
+                self.cols[i].data = c.data_ptr() if c.numel() else None
             self.cols[i].row_num = rows
         self.table = abi.Table(self.cols, n, key_num, rows)
 
@@ -328,18 +374,24 @@ class GroupByPredicate:
         return types
 
     def gpu_execute(self, ctx, table, sel=None, capacity=None):
-        """List of output tensors: keys then aggregates, one row per group (unordered)."""
+        """List of outputs: keys then aggregates, one row per group (unordered). A STRING key comes
+        back as a StringColumn sharing the input's blob."""
         n = sel.numel() if sel is not None else table.row_num
         cap = capacity if capacity is not None else max(min(n, 1 << 20), 1)
         types = self.output_types(table)
-        dev = table.columns[0].device if table.columns else torch.device("cuda")
+        dev = table.columns[0].blob.device if table.columns and isinstance(table.columns[0], StringColumn) \
+            else (table.columns[0].device if table.columns else torch.device("cuda"))
         karr = (C.c_uint32 * max(len(self.keys), 1))(*self.keys)
         aarr = (abi.Agg * max(len(self.aggregation_descs), 1))(*[abi.Agg(*a) for a in self.aggregation_descs])
         while True:
-            outs = [torch.empty(max(cap, 1), dtype=_DTYPE_OF_TYPE[t], device=dev) for t in types]
+            outs = [torch.empty(2 * max(cap, 1), dtype=torch.int32, device=dev) if t == abi.STRING
+                    else torch.empty(max(cap, 1), dtype=_DTYPE_OF_TYPE[t], device=dev) for t in types]
             ocols = (abi.Column * max(len(outs), 1))()
             for i, o in enumerate(outs):
-                ocols[i].data = o.data_ptr()
+                if types[i] == abi.STRING:
+                    ocols[i].aux = o.data_ptr()
+                else:
+                    ocols[i].data = o.data_ptr()
             g = C.c_uint64()
             st = lib().ghive_groupby(ctx.h, table.ref(), _p(sel), n, C.cast(karr, C.c_void_p), len(self.keys),
                                      aarr, len(self.aggregation_descs), ocols, cap, C.byref(g))
@@ -347,7 +399,13 @@ class GroupByPredicate:
                 cap = max(n, 1)
                 continue
             ctx.check(st)
-            return [o[: g.value] for o in outs]
+            res = []
+            for i, o in enumerate(outs):
+                if types[i] == abi.STRING:
+                    res.append(StringColumn(table.columns[self.keys[i]].blob, o[: 2 * g.value]))
+                else:
+                    res.append(o[: g.value])
+            return res
 
 
 def sort(ctx, table, key_cols, orders):

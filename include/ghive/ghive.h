@@ -291,8 +291,11 @@ typedef struct ghive_agg {
 int ghive_agg_output_type(int32_t agg_type, int32_t input_type);
 
 /* GroupByPredicate::gpu_execute (GroupByPredicate.cu:19-359) as a hash aggregation. Rows
- * sel[0..n) (sel == NULL: all rows). Keys: INT/LONG/DOUBLE/FLOAT columns, total width
- * <= 8 bytes; NULL keys form one group. Output: keys then aggregates into out_cols
+ * sel[0..n) (sel == NULL: all rows). Keys: up to 8 columns of any type, STRING included
+ * (string keys as GroupByPredicate.hpp:69-96); keys that pack into 64 bits aggregate on the
+ * packed key, wider ones through a key dictionary (group bound <= 2^31). NULL keys form one
+ * group. A STRING key's output gathers (start,end) pairs into the caller's aux buffer (2 x
+ * capacity int32) and shares the input blob (out data = input data). Output: keys then aggregates into out_cols
  * (types as ghive_agg_output_type; device buffers of `capacity` rows), groups in
  * unspecified order. An aggregate with no non-NULL input is NULL (Hive semantics).
  * SUM(LONG) wraps modulo 2^64 like a Java long. SUM/AVG(DOUBLE) are accumulated exactly

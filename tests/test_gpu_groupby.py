@@ -14,19 +14,41 @@ INT_NULL, LONG_NULL = abi.INT_NULL, abi.LONG_NULL
 REL = 1e-12
 
 
+def _dev(c):
+    """numpy column, or a ('string', blob, idx) fixture column -> device column."""
+    from ghive_amd import ops
+    if isinstance(c, tuple) and c[0] == "string":
+        import torch
+        blob = torch.frombuffer(bytearray(c[1]), dtype=torch.uint8) if len(c[1]) else torch.zeros(0, dtype=torch.uint8)
+        return ops.StringColumn(blob.cuda(), to_dev(np.asarray(c[2], dtype=np.int32)))
+    return to_dev(c)
+
+
+def _host(o):
+    """Device output -> numpy array, or a list of str / None for a STRING column."""
+    from ghive_amd import ops
+    if isinstance(o, ops.StringColumn):
+        return [None if v is None else v.decode() for v in o.to_list()]
+    return to_np(o)
+
+
 def _run(ctx, cols, keys, aggs, sel=None, capacity=None):
     from ghive_amd import ops
-    t = ops.Table([to_dev(c) for c in cols])
+    t = ops.Table([_dev(c) for c in cols])
     outs = ops.GroupByPredicate(keys, aggs).gpu_execute(
         ctx, t, sel=None if sel is None else to_dev(sel.astype(np.int32)), capacity=capacity)
-    return [to_np(o) for o in outs]
+    return [_host(o) for o in outs]
+
+
+def _item(x):
+    return x.item() if hasattr(x, "item") else x
 
 
 def _as_dict(outs, nkeys):
     res = {}
     for g in range(len(outs[0]) if outs else 0):
-        key = tuple(o[g].item() for o in outs[:nkeys])
-        res[key] = [o[g].item() for o in outs[nkeys:]]
+        key = tuple(_item(o[g]) for o in outs[:nkeys])
+        res[key] = [_item(o[g]) for o in outs[nkeys:]]
     return res
 
 
@@ -42,11 +64,34 @@ def _same(a, b):
     return a == b
 
 
+def _oracle_key(col, r):
+    if isinstance(col, tuple) and col[0] == "string":
+        s, e = col[2][2 * r], col[2][2 * r + 1]
+        return None if s < 0 else col[1][s:e].decode()
+    return col[r].item()
+
+
+def _strings(rng, n, vocab, null_frac=0.02):
+    """A ('string', blob, idx) column drawn from `vocab` distinct strings (empty ones included)."""
+    words = [bytes(rng.integers(97, 123, int(rng.integers(0, 12))).astype(np.uint8)) for _ in range(vocab)]
+    words[0] = b""
+    pick = rng.integers(0, vocab, n)
+    blob, idx, o = bytearray(), np.empty(2 * n, dtype=np.int32), 0
+    for i, w in enumerate(pick):
+        if rng.random() < null_frac:
+            idx[2 * i], idx[2 * i + 1] = -1, -1
+            continue
+        blob += words[w]
+        idx[2 * i], idx[2 * i + 1] = o, o + len(words[w])
+        o += len(words[w])
+    return ("string", bytes(blob), idx)
+
+
 def _oracle_dict(orc, cols, keys, aggs, sel=None):
     rep, vals, nulls = orc.groupby(cols, keys, aggs, sel=sel)
     res = {}
     for g, r in enumerate(rep):
-        key = tuple(cols[k][r].item() for k in keys)
+        key = tuple(_oracle_key(cols[k], r) for k in keys)
         key = tuple(float("nan") if isinstance(x, float) and math.isnan(x) else x for x in key)
         res[key] = [None if nulls[a][g] else vals[a][g].item() for a in range(len(aggs))]
     return res
@@ -63,10 +108,9 @@ def _compare(got, want):
             assert _same(a, b), (k, g[k], w[k])
 
 
-def test_gpu_impl_kats_numeric_keys(ctx, kats):
+def test_gpu_impl_kats(ctx, kats):
+    """Every group-by KAT of GroupByPredicateTest.cu, STRING keys included (dictionary path)."""
     for k in kats["groupby_gpu_impl"]:
-        if any(c["type"] == "STRING" for c in k["columns"]):
-            continue  # STRING keys are out of scope on the device
         cols = [np_col(c) for c in k["columns"]]
         outs = _run(ctx, cols, k["keys"], [tuple(a) for a in k["aggs"]])
         got = _as_dict(outs, len(k["keys"]))
@@ -144,6 +188,71 @@ def _all_aggs_case(ctx, orc, card, ktype):
     aggs2 = [(abi.AGG_MIN, b + 3), (abi.AGG_AVG, b + 2), (abi.AGG_SUM, b + 3)]
     outs = _run(ctx, cols, keys, aggs2)
     _compare(_as_dict(outs, len(keys)), _oracle_dict(orc, cols, keys, aggs2))
+
+
+@pytest.mark.parametrize("partition", [1, 2], ids=["direct", "partitioned"])
+@pytest.mark.parametrize("kset", ["INT+LONG", "LONG3", "DOUBLE+LONG", "INT5", "STRING", "STRING+INT"])
+def test_groupby_wide_keys_vs_oracle(ctx, orc, kset, partition):
+    """Keys wider than 64 bits, more than four key columns, and STRING keys (the dictionary path)
+    against the oracle, with NULLs in every key column, in both aggregation forms."""
+    rng = np.random.default_rng(len(kset) * 7 + partition)
+    n = 300_000
+
+    def ints(dt, lo, hi, null):
+        x = rng.integers(lo, hi, n).astype(dt)
+        x[rng.random(n) < 0.01] = null
+        return x
+    if kset == "INT+LONG":
+        kcols = [ints(np.int32, 1992, 1999, INT_NULL), ints(np.int64, -(2**40), -(2**40) + 300, LONG_NULL)]
+    elif kset == "LONG3":
+        kcols = [ints(np.int64, 0, 9, LONG_NULL), ints(np.int64, 2**50, 2**50 + 30, LONG_NULL),
+                 ints(np.int64, -1, 2, LONG_NULL)]
+    elif kset == "DOUBLE+LONG":
+        d = rng.integers(0, 50, n) / 8.0
+        d[rng.random(n) < 0.01] = np.nan
+        kcols = [d, ints(np.int64, 0, 40, LONG_NULL)]
+    elif kset == "INT5":
+        kcols = [ints(np.int32, 0, 3, INT_NULL) for _ in range(5)]
+    elif kset == "STRING":
+        kcols = [_strings(rng, n, 5000)]
+    else:
+        kcols = [_strings(rng, n, 300), ints(np.int32, 0, 20, INT_NULL)]
+    v_long = rng.integers(-2**62, 2**62, n, dtype=np.int64)
+    v_long[::37] = LONG_NULL
+    v_dbl = rng.standard_normal(n) * 10.0 ** rng.integers(-3, 4, n)
+    v_dbl[::41] = np.nan
+    cols = kcols + [v_long, v_dbl]
+    b = len(kcols)
+    keys = list(range(b))
+    aggs = [(abi.AGG_SUM, b), (abi.AGG_MIN, b), (abi.AGG_CNT, abi.COUNT_STAR), (abi.AGG_SUM, b + 1),
+            (abi.AGG_MAX, b + 1), (abi.AGG_AVG, b)]
+    ctx.set_option("groupby.partition", partition)
+    try:
+        outs = _run(ctx, cols, keys, aggs)
+        _compare(_as_dict(outs, len(keys)), _oracle_dict(orc, cols, keys, aggs))
+        sel = np.nonzero(rng.random(n) < 0.4)[0].astype(np.uint32)
+        outs = _run(ctx, cols, keys, aggs, sel=sel)
+        _compare(_as_dict(outs, len(keys)), _oracle_dict(orc, cols, keys, aggs, sel=sel))
+    finally:
+        ctx.set_option("groupby.partition", 1)
+
+
+def test_groupby_wide_keys_edges(ctx, orc):
+    from ghive_amd._lib import GhiveError
+    rng = np.random.default_rng(5)
+    k1 = np.arange(1000, dtype=np.int64)
+    k2 = np.arange(1000, dtype=np.int64) % 7
+    with pytest.raises(GhiveError) as e:  # 1000 groups, room for 100
+        _run(ctx, [k1, k2, k1], [0, 1], [(abi.AGG_SUM, 2)], capacity=100)
+    assert e.value.status == abi.ERR_CAPACITY
+    outs = _run(ctx, [k1[:0], k2[:0], k1[:0]], [0, 1], [(abi.AGG_SUM, 2)])  # no rows, no groups
+    assert len(outs[0]) == 0
+    # every string NULL or empty, with a high-cardinality second key in the partitioned form
+    s = ("string", b"", np.array([-1, -1, 0, 0] * 50_000, dtype=np.int32))
+    k = rng.integers(0, 60_000, 100_000).astype(np.int64)
+    cols = [s, k, k]
+    outs = _run(ctx, cols, [0, 1], [(abi.AGG_CNT, abi.COUNT_STAR), (abi.AGG_SUM, 2)])
+    _compare(_as_dict(outs, 2), _oracle_dict(orc, cols, [0, 1], [(abi.AGG_CNT, abi.COUNT_STAR), (abi.AGG_SUM, 2)]))
 
 
 def test_groupby_with_selection_and_empty(ctx, orc):

@@ -158,7 +158,7 @@ def test_ssb_q2_1_seven_vertices_with_order_by(ctx, orc):
     lo_orderdate = d_datekey AND lo_partkey = p_partkey AND lo_suppkey = s_suppkey AND p_category = 1
     AND s_region = 1 GROUP BY d_year, p_brand1 ORDER BY d_year, p_brand1 -- map joins in Map 1 and
     Reducer 2, a merge join in Reducer 2 over two shuffled inputs, partial and final group-by, and
-    the final ORDER BY. p_brand1 is INT here: the group-by packs its keys into 64 bits."""
+    the final ORDER BY. Column types are the plan's: d_year int, p_brand1 bigint (96-bit group keys)."""
     rng = np.random.default_rng(202)
     dk, dy = _dates()
     ns, npart, n = 2000, 20000, 300_000
@@ -166,7 +166,7 @@ def test_ssb_q2_1_seven_vertices_with_order_by(ctx, orc):
     s_region = rng.integers(0, 5, ns).astype(np.int64)
     p_key = rng.permutation(10 * npart)[:npart].astype(np.int64) + 1
     p_cat = rng.integers(0, 5, npart).astype(np.int64)
-    p_brand = rng.integers(0, 40, npart).astype(np.int32) + 1000
+    p_brand = rng.integers(0, 40, npart).astype(np.int64) + 1000
     lo_part, lo_supp, lo_date = _fk(rng, p_key, n), _fk(rng, s_key, n), _fk(rng, dk, n)
     lo_rev = np.round(rng.random(n) * 1e6, 2)
     maintain = ('{"MAPJOIN_52":{"0":["_col0","_col2","_col3"],"1":[]},'
@@ -197,7 +197,7 @@ def test_ssb_q2_1_seven_vertices_with_order_by(ctx, orc):
             groups.setdefault((y_of[d], p_of[pk]), []).append(r)
     want = sorted(groups)
     assert list(zip(year.tolist(), brand.tolist())) == want
-    assert year.dtype == np.int32 and brand.dtype == np.int32
+    assert year.dtype == np.int32 and brand.dtype == np.int64
     # one partial sum per group from the map side, summed exactly in the reducer
     assert s.tolist() == [math.fsum(groups[k]) for k in want]
 
