@@ -48,11 +48,15 @@ class Table(C.Structure):
                 ("row_num", C.c_uint64)]
 
 
+PRED_STR_MAX = 64
+
+
 class Pred(C.Structure):
     _fields_ = [("mode", C.c_int32), ("pred_col_num", C.c_int32), ("filter_col", C.c_uint32 * 2),
                 ("param_num", C.c_int32), ("data_type", C.c_int32),
                 ("long_params", C.c_int64 * 2), ("int_params", C.c_int32 * 2),
-                ("double_params", C.c_double * 2), ("children", C.c_int32 * 2)]
+                ("double_params", C.c_double * 2), ("children", C.c_int32 * 2),
+                ("string_sizes", C.c_int32 * 2), ("string_params", (C.c_char * PRED_STR_MAX) * 2)]
 
 
 class JoinCond(C.Structure):
@@ -141,10 +145,19 @@ def make_pred(mode, col=0, params=(), data_type=None, col2=None, children=(-1, -
         p.data_type = 4
         return p
     if data_type is None:
-        data_type = 1 if any(isinstance(v, float) for v in params) else 0
+        if any(isinstance(v, (str, bytes)) for v in params):
+            data_type = 3
+        else:
+            data_type = 1 if any(isinstance(v, float) for v in params) else 0
     p.data_type = data_type
     for i, v in enumerate(params):
-        if data_type == 1:
+        if data_type == 3:
+            b = v.encode() if isinstance(v, str) else bytes(v)
+            if len(b) > PRED_STR_MAX:
+                raise ValueError(f"string literal longer than {PRED_STR_MAX} bytes")
+            p.string_sizes[i] = len(b)
+            C.memmove(C.addressof(p.string_params[i]), b, len(b))
+        elif data_type == 1:
             p.double_params[i] = float(v)
         elif data_type == 2:
             p.int_params[i] = int(v)

@@ -77,6 +77,7 @@ __device__ __forceinline__ bool col_load_f64(const void *p, int type, uint64_t r
 // ---- flattened predicate program ----------------------------------------------------------
 constexpr int kMaxPredNodes = 32;
 constexpr int kMaxPredCols = 8;
+constexpr int kPredLitBytes = 256;  // string literals of one predicate
 
 enum DevPredOp : int8_t {
   kLeafInt = 0,   // column (as int64) OP literal(s)
@@ -86,6 +87,9 @@ enum DevPredOp : int8_t {
   kNotNull = 4,
   kAnd = 5,
   kOr = 6,
+  kLeafStr = 7,     // STRING column OP string literal(s)
+  kColColStr = 8,   // STRING column OP STRING column
+  kNotNullStr = 9,  // STRING column is not NULL
 };
 
 struct DevPredNode {
@@ -93,8 +97,8 @@ struct DevPredNode {
   int8_t mode;
   int8_t a;  // column slot, or child node index for AND/OR
   int8_t b;
-  int32_t pad;
-  int64_t ia, ib;
+  int32_t pad;     // kLeafStr: number of literals
+  int64_t ia, ib;  // kLeafStr: literal k at lit[ia/ib & 0xffffffff], length ia/ib >> 32
   double da, db;
 };
 
@@ -103,9 +107,20 @@ struct DevPred {
   int32_t nnodes;
   int32_t ncols;
   const void *col_data[kMaxPredCols];
+  const int32_t *col_aux[kMaxPredCols];  // STRING: (start,end) pairs
   int32_t col_type[kMaxPredCols];
   DevPredNode nodes[kMaxPredNodes];
+  uint8_t lit[kPredLitBytes];
 };
+
+// Hive's string order (StringExpr.compare, storage-api .../vector/expressions/StringExpr.java:37-47):
+// unsigned bytes, then length. Returns <0, 0, >0.
+__device__ __forceinline__ int str_compare(const uint8_t *a, int la, const uint8_t *b, int lb) {
+  const int m = la < lb ? la : lb;
+  for (int i = 0; i < m; ++i)
+    if (a[i] != b[i]) return (int)a[i] - (int)b[i];
+  return la - lb;
+}
 
 template <class T>
 __device__ __forceinline__ bool cmp_mode(int mode, T x, T a, T b) {
@@ -159,6 +174,31 @@ __device__ __forceinline__ bool eval_pred(const DevPred &P, uint64_t r) {
         v = col_load_f64(P.col_data[n.a], P.col_type[n.a], r, x);
         break;
       }
+      case kLeafStr: {
+        const int32_t s = P.col_aux[n.a][2 * r], e = P.col_aux[n.a][2 * r + 1];
+        if (s < 0) {
+          v = false;
+          break;
+        }
+        const uint8_t *x = static_cast<const uint8_t *>(P.col_data[n.a]) + s;
+        const int c0 = str_compare(x, e - s, P.lit + (uint32_t)n.ia, (int)(n.ia >> 32));
+        const int c1 = n.pad == 2 ? str_compare(x, e - s, P.lit + (uint32_t)n.ib, (int)(n.ib >> 32)) : 0;
+        v = cmp_mode<int>(n.mode, 0, -c0, -c1);  // 0 OP -c  <=>  string OP literal
+        break;
+      }
+      case kColColStr: {
+        const int32_t s0 = P.col_aux[n.a][2 * r], e0 = P.col_aux[n.a][2 * r + 1];
+        const int32_t s1 = P.col_aux[n.b][2 * r], e1 = P.col_aux[n.b][2 * r + 1];
+        if (s0 < 0 || s1 < 0) {
+          v = false;
+          break;
+        }
+        const int c = str_compare(static_cast<const uint8_t *>(P.col_data[n.a]) + s0, e0 - s0,
+                                  static_cast<const uint8_t *>(P.col_data[n.b]) + s1, e1 - s1);
+        v = cmp_mode<int>(n.mode, 0, -c, 0);
+        break;
+      }
+      case kNotNullStr: v = P.col_aux[n.a][2 * r] >= 0; break;
       case kAnd: v = ((bits >> n.a) & 1u) && ((bits >> n.b) & 1u); break;
       default: v = ((bits >> n.a) & 1u) || ((bits >> n.b) & 1u); break;
     }

@@ -63,6 +63,10 @@ __global__ __launch_bounds__(kFilterThreads) void k_filter_write(const uint64_t 
   }
 }
 
+struct StrPair {  // a STRING row's (start,end); 4-byte aligned like the int32 pairs it reads
+  int32_t start, end;
+};
+
 template <class T>
 __global__ void k_gather(const T *in, const uint32_t *sel, uint64_t n, T *out) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -97,14 +101,21 @@ int gather_columns(ghive_ctx *ctx, const ghive_table *t, const uint32_t *sel, ui
   for (uint32_t c = 0; c < t->ncols; ++c) {
     const ghive_column &in = t->columns[c];
     ghive_column &o = out_cols[c];
-    if (!is_numeric_type(in.type))
-      return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "gather: STRING columns are not supported");
     o.type = in.type;
     o.row_num = n;
-    if (n == 0) continue;
-    if (!o.data) return set_error(ctx, GHIVE_ERR_INVALID, "gather: output column %u has no buffer", c);
     uint64_t g = (n + 255) / 256, cap = (uint64_t)ctx->cus * 16;
     uint32_t grid = (uint32_t)(g > cap ? cap : g);
+    if (in.type == GHIVE_STRING) {  // (start,end) pairs into the caller's aux; the blob is shared
+      o.data = in.data;
+      o.char_size = in.char_size;
+      if (n == 0) continue;
+      if (!o.aux) return set_error(ctx, GHIVE_ERR_INVALID, "gather: STRING output column %u has no aux buffer", c);
+      k_gather<StrPair><<<grid, 256, 0, ctx->stream>>>((const StrPair *)in.aux, sel, n, (StrPair *)o.aux);
+      GH_LAUNCH(ctx);
+      continue;
+    }
+    if (n == 0) continue;
+    if (!o.data) return set_error(ctx, GHIVE_ERR_INVALID, "gather: output column %u has no buffer", c);
     if (type_width(in.type) == 8)
       k_gather<uint64_t><<<grid, 256, 0, ctx->stream>>>((const uint64_t *)in.data, sel, n, (uint64_t *)o.data);
     else

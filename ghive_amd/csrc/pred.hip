@@ -23,7 +23,7 @@ struct Parser {
 
   static bool is_op(const std::string &t) { return t == "and" || t == "or"; }
 
-  bool parse_literal(const std::string &s, int &dtype, int64_t &l, double &d, int32_t &i) {
+  bool parse_literal(const std::string &s, int &dtype, int64_t &l, double &d, int32_t &i, std::string &str) {
     auto all = [&](size_t b, size_t e, bool dot) {
       if (b >= e) return false;
       bool digit = false;
@@ -55,8 +55,9 @@ struct Parser {
       }
       return true;
     }
-    if (s.size() >= 2 && s.front() == '\'' && s.back() == '\'') {
+    if (s.size() >= 2 && s.front() == '\'' && s.back() == '\'') {  // '...' as the regex '(.*)'
       dtype = 3;
+      str = s.substr(1, s.size() - 2);
       return true;
     }
     return false;
@@ -77,17 +78,30 @@ struct Parser {
     std::string col, v1, v2;
     static const char *kNotNull = " is not null";
     size_t L = strlen(kNotNull);
+    // positions of `op` outside '...' literals
+    auto unquoted = [&](const char *op, bool last, size_t from) -> size_t {
+      size_t found = std::string::npos;
+      const size_t L2 = strlen(op);
+      bool q = false;
+      for (size_t k = 0; k < s.size(); ++k) {
+        if (s[k] == '\'') q = !q;
+        if (q || k < from || s.compare(k, L2, op) != 0) continue;
+        found = k;
+        if (!last) break;
+      }
+      return found;
+    };
     auto split2 = [&](const char *op) -> bool {  // greedy: the last occurrence, like (.*) OP (.*)
-      size_t at = s.rfind(op);
+      size_t at = unquoted(op, true, 0);
       if (at == std::string::npos || at == 0) return false;
       col = s.substr(0, at);
       v1 = s.substr(at + strlen(op));
       return !v1.empty();
     };
     auto between = [&](const char *kw) -> bool {
-      size_t at = s.find(kw);
+      size_t at = unquoted(kw, false, 0);
       if (at == std::string::npos || at == 0) return false;
-      size_t an = s.find(" AND ", at + strlen(kw));
+      size_t an = unquoted(" AND ", false, at + strlen(kw));
       if (an == std::string::npos) return false;
       col = s.substr(0, at);
       v1 = s.substr(at + strlen(kw), an - at - strlen(kw));
@@ -138,9 +152,18 @@ struct Parser {
         int64_t l = 0;
         double d = 0;
         int32_t iv = 0;
-        if (!parse_literal(*vs[k], dt, l, d, iv)) {
+        std::string str;
+        if (!parse_literal(*vs[k], dt, l, d, iv, str)) {
           err = "unsupported literal '" + *vs[k] + "'";
           return -1;
+        }
+        if (dt == 3) {
+          if (str.size() > (size_t)GHIVE_PRED_STR_MAX) {
+            err = "string literal longer than " + std::to_string(GHIVE_PRED_STR_MAX) + " bytes: " + *vs[k];
+            return -1;
+          }
+          p.string_sizes[k] = (int32_t)str.size();
+          memcpy(p.string_params[k], str.data(), str.size());
         }
         if (first >= 0 && dt != first) {
           // mixed int/long literals of one BETWEEN: widen both to long
@@ -241,6 +264,10 @@ void dump(const ghive_pred *nodes, uint32_t nnodes, uint32_t idx, int level, std
       if (p.data_type == 0) out += std::to_string((long long)p.long_params[i]) + " ";
       else if (p.data_type == 1) out += std::to_string(p.double_params[i]) + " ";
       else if (p.data_type == 2) out += std::to_string(p.int_params[i]) + " ";
+      else if (p.data_type == 3) {
+        const int len = p.string_sizes[i] < 0 ? 0 : (p.string_sizes[i] > GHIVE_PRED_STR_MAX ? GHIVE_PRED_STR_MAX : p.string_sizes[i]);
+        out += std::string(p.string_params[i], (size_t)len) + " ";
+      }
     }
   }
   out += ";\n";
@@ -298,6 +325,7 @@ int flatten_pred(ghive_ctx *ctx, const ghive_table *t, const ghive_pred *nodes, 
     slot_col.push_back(col);
     return (int)slot_col.size() - 1;
   };
+  int lit_used = 0;
   for (size_t i = 0; i < order.size(); ++i) {
     const ghive_pred &p = nodes[order[i]];
     DevPredNode &n = out.nodes[i];
@@ -314,25 +342,24 @@ int flatten_pred(ghive_ctx *ctx, const ghive_table *t, const ghive_pred *nodes, 
     if (p.filter_col[0] >= t->ncols)
       return set_error(ctx, GHIVE_ERR_INVALID, "predicate: column %u of %u", p.filter_col[0], t->ncols);
     const ghive_column &c0 = t->columns[p.filter_col[0]];
-    if (!is_numeric_type(c0.type))
-      return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "predicate: STRING column filters are not supported");
+    const bool str0 = c0.type == GHIVE_STRING;
     int s0 = slot(p.filter_col[0]);
     if (s0 < 0) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "predicate: more than %d columns", kMaxPredCols);
     n.a = (int8_t)s0;
     if (p.mode == GHIVE_FILTER_NOT_NULL) {
-      n.op = kNotNull;
+      n.op = str0 ? kNotNullStr : kNotNull;
       continue;
     }
     if (p.pred_col_num == 2) {
       if (p.filter_col[1] >= t->ncols)
         return set_error(ctx, GHIVE_ERR_INVALID, "predicate: column %u of %u", p.filter_col[1], t->ncols);
       const ghive_column &c1 = t->columns[p.filter_col[1]];
-      if (!is_numeric_type(c1.type))
-        return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "predicate: STRING column filters are not supported");
+      if (str0 != (c1.type == GHIVE_STRING))
+        return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "predicate: a STRING column against a numeric column");
       int s1 = slot(p.filter_col[1]);
       if (s1 < 0) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "predicate: more than %d columns", kMaxPredCols);
       n.b = (int8_t)s1;
-      n.op = (is_int_type(c0.type) && is_int_type(c1.type)) ? kColColInt : kColColDbl;
+      n.op = str0 ? kColColStr : (is_int_type(c0.type) && is_int_type(c1.type)) ? kColColInt : kColColDbl;
       if (p.mode > GHIVE_FILTER_NOT_EQ || p.mode == GHIVE_FILTER_RANGE || p.mode == GHIVE_FILTER_EQ_RANGE)
         return set_error(ctx, GHIVE_ERR_INVALID, "predicate: mode %d needs literals", p.mode);
       continue;
@@ -340,7 +367,31 @@ int flatten_pred(ghive_ctx *ctx, const ghive_table *t, const ghive_pred *nodes, 
     if (p.pred_col_num != 1)
       return set_error(ctx, GHIVE_ERR_INVALID, "predicate: pred_col_num %d", p.pred_col_num);
     int dt = p.data_type;
-    if (dt == 3) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "predicate: string literals are not supported");
+    if (str0 || dt == 3) {
+      if (!str0 || dt != 3)
+        return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "predicate: a %s column against a %s literal",
+                         str0 ? "STRING" : "numeric", dt == 3 ? "string" : "numeric");
+      const bool two = p.mode == GHIVE_FILTER_RANGE || p.mode == GHIVE_FILTER_EQ_RANGE ||
+                       p.mode == GHIVE_FILTER_NOT_RANGE || p.mode == GHIVE_FILTER_NOT_EQ_RANGE;
+      if (p.param_num != (two ? 2 : 1))
+        return set_error(ctx, GHIVE_ERR_INVALID, "predicate: mode %d with %d literals", p.mode, p.param_num);
+      int64_t lit[2] = {0, 0};
+      for (int k = 0; k < p.param_num; ++k) {
+        const int len = p.string_sizes[k];
+        if (len < 0 || len > GHIVE_PRED_STR_MAX)
+          return set_error(ctx, GHIVE_ERR_INVALID, "predicate: string literal of %d bytes", len);
+        if (lit_used + len > kPredLitBytes)
+          return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "predicate: string literals above %d bytes", kPredLitBytes);
+        memcpy(out.lit + lit_used, p.string_params[k], (size_t)len);
+        lit[k] = ((int64_t)len << 32) | (int64_t)lit_used;
+        lit_used += len;
+      }
+      n.op = kLeafStr;
+      n.ia = lit[0];
+      n.ib = lit[1];
+      n.pad = p.param_num;
+      continue;
+    }
     if (dt < 0 || dt > 2) return set_error(ctx, GHIVE_ERR_INVALID, "predicate: data_type %d", dt);
     if (is_int_type(c0.type) && dt != 1) {
       n.op = kLeafInt;
@@ -357,6 +408,7 @@ int flatten_pred(ghive_ctx *ctx, const ghive_table *t, const ghive_pred *nodes, 
   out.ncols = (int32_t)slot_col.size();
   for (size_t k = 0; k < slot_col.size(); ++k) {
     out.col_data[k] = t->columns[slot_col[k]].data;
+    out.col_aux[k] = static_cast<const int32_t *>(t->columns[slot_col[k]].aux);
     out.col_type[k] = t->columns[slot_col[k]].type;
   }
   return GHIVE_OK;
@@ -381,6 +433,13 @@ int ghive_parse_predicate(const char *text, const char *const *col_names, uint32
   P.ncols = ncols;
   std::string cur;
   for (const char *c = text; *c; ++c) {
+    if (*c == '\'') {  // a '...' literal is one token, spaces and parentheses included
+      const char *e = strchr(c + 1, '\'');
+      if (!e) return fail("unterminated string literal");
+      cur.append(c, (size_t)(e - c + 1));
+      c = e;
+      continue;
+    }
     if (*c == ' ' || *c == '(' || *c == ')') {
       if (!cur.empty()) P.tok.push_back(cur);
       cur.clear();

@@ -129,6 +129,7 @@ class StringColumn:
         if self.blob.numel() == 0:  # the library wants a blob address even when every string is empty
             self.blob = torch.zeros(1, dtype=torch.uint8, device=self.blob.device)
         self.is_cuda = self.blob.is_cuda
+        self.device = self.blob.device
 
     @staticmethod
     def from_list(values, device="cuda"):
@@ -238,14 +239,19 @@ class FilterOperator:
     def clean_data(self, ctx, table, predicate=None):
         """Table compacted to the passing rows (the input is not modified)."""
         pred = predicate or self.generatePredicate()
-        outs = [torch.empty_like(c) for c in table.columns]
+        outs = [torch.empty_like(c.idx) if isinstance(c, StringColumn) else torch.empty_like(c)
+                for c in table.columns]
         ocols = (abi.Column * max(len(outs), 1))()
         for i, o in enumerate(outs):
-            ocols[i].data = o.data_ptr() if o.numel() else None
+            if isinstance(table.columns[i], StringColumn):
+                ocols[i].aux = o.data_ptr() if o.numel() else None
+            else:
+                ocols[i].data = o.data_ptr() if o.numel() else None
         cnt = C.c_uint64()
         ctx.check(lib().ghive_filter_compact(ctx.h, table.ref(), pred._arr, len(pred.nodes), pred.root,
                                              ocols, C.byref(cnt)))
-        return Table([o[: cnt.value] for o in outs])
+        return Table([StringColumn(c.blob, o[: 2 * cnt.value]) if isinstance(c, StringColumn) else o[: cnt.value]
+                      for c, o in zip(table.columns, outs)])
 
 
 class HashTable:

@@ -60,7 +60,7 @@ enum {
   GHIVE_LONG = 0,   /* int64                                                       */
   GHIVE_DOUBLE = 1, /* IEEE binary64                                               */
   GHIVE_INT = 2,    /* int32                                                       */
-  GHIVE_STRING = 3, /* UTF-8 blob + int32 (start,end) per row; not computed on here */
+  GHIVE_STRING = 3, /* UTF-8 blob + int32 (start,end) per row                      */
   GHIVE_FLOAT = 4   /* IEEE binary32 -- extension: the reference has no FLOAT type  */
 };
 
@@ -173,7 +173,10 @@ enum {
  * pred_col_num 0 = AND/OR over children[0..1]; 1 = column vs literal(s); 2 = column vs
  * column (filter_col[0] OP filter_col[1]). data_type: 0 long, 1 double, 2 int, 3 string,
  * 4 none. The comparison is done on values (an INT column against a long or double
- * literal compares the converted values); a NULL operand makes the node false. */
+ * literal compares the converted values); a NULL operand makes the node false. A STRING
+ * column compares with string literals (data_type 3) or another STRING column in Hive's order:
+ * unsigned bytes, then length (StringExpr.compare); all modes but RANGE / NOT_EQ_RANGE. */
+#define GHIVE_PRED_STR_MAX 64
 typedef struct ghive_pred {
   int32_t mode;
   int32_t pred_col_num;
@@ -183,7 +186,9 @@ typedef struct ghive_pred {
   int64_t long_params[2];
   int32_t int_params[2];
   double double_params[2];
-  int32_t children[2]; /* node indexes (AND/OR), else -1 */
+  int32_t children[2];                          /* node indexes (AND/OR), else -1 */
+  int32_t string_sizes[2];                      /* data_type 3: literal lengths in bytes */
+  char string_params[2][GHIVE_PRED_STR_MAX];    /* data_type 3: literal bytes (stringFilterParams) */
 } ghive_pred;
 
 /* FilterOperator::generatePredicate (FilterOperator.cu:227-487): parses Hive's EXPLAIN
@@ -201,11 +206,12 @@ int ghive_filter(ghive_ctx *ctx, const ghive_table *table, const ghive_pred *nod
                  uint32_t nnodes, uint32_t root, uint32_t *out_sel, uint64_t *out_count);
 /* clean_data compaction form: every column of `table` gathered by the passing rows into
  * out_cols[i].data (device buffers of capacity table->row_num, caller-owned). Unlike the
- * reference the input is not modified in place. STRING columns: UNSUPPORTED. */
+ * reference the input is not modified in place. A STRING column gathers its (start,end) pairs
+ * into out_cols[i].aux (2 x row_num int32) and shares the input blob (out data = input data). */
 int ghive_filter_compact(ghive_ctx *ctx, const ghive_table *table, const ghive_pred *nodes,
                          uint32_t nnodes, uint32_t root, ghive_column *out_cols,
                          uint64_t *out_count);
-/* Gather rows `sel[0..n)` of every column into out_cols (device). */
+/* Gather rows `sel[0..n)` of every column into out_cols (device); STRING as ghive_filter_compact. */
 int ghive_gather(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel, uint64_t n,
                  ghive_column *out_cols);
 

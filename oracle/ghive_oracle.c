@@ -220,15 +220,55 @@ static int cmp_mode_d(int mode, double x, double a, double b) {
   }
 }
 
+/* StringExpr.compare (hive storage-api .../vector/expressions/StringExpr.java:37-47): unsigned
+ * bytes, then length. */
+static int str_cmp(const uint8_t *a, int la, const uint8_t *b, int lb) {
+  for (int i = 0; i < la && i < lb; ++i)
+    if (a[i] != b[i]) return (int)a[i] - (int)b[i];
+  return la - lb;
+}
+
+/* A STRING row against the node: string literals (FilterStringGroupColumnCompareStringGroupScalarBase.txt,
+ * FilterStringColumnBetween.txt:79-99 -- BETWEEN inclusive, NOT BETWEEN its negation) or
+ * another STRING column. */
+static int str_leaf_pass(const ghive_table *t, const ghive_pred *p, uint64_t r) {
+  const ghive_column *c = &t->columns[p->filter_col[0]];
+  const int32_t *ix = (const int32_t *)c->aux;
+  const uint8_t *x = (const uint8_t *)c->data + ix[2 * r];
+  const int lx = ix[2 * r + 1] - ix[2 * r];
+  if (p->pred_col_num == 2) {
+    const ghive_column *d = &t->columns[p->filter_col[1]];
+    const int32_t *iy = (const int32_t *)d->aux;
+    const int cy = str_cmp(x, lx, (const uint8_t *)d->data + iy[2 * r], iy[2 * r + 1] - iy[2 * r]);
+    return cmp_mode_i(p->mode, cy, 0, 0);
+  }
+  const int c0 = str_cmp(x, lx, (const uint8_t *)p->string_params[0], p->string_sizes[0]);
+  const int c1 = p->param_num == 2 ? str_cmp(x, lx, (const uint8_t *)p->string_params[1], p->string_sizes[1]) : 0;
+  switch (p->mode) {
+    case GHIVE_FILTER_EQ: return c0 == 0;
+    case GHIVE_FILTER_LE: return c0 <= 0;
+    case GHIVE_FILTER_GE: return c0 >= 0;
+    case GHIVE_FILTER_LT: return c0 < 0;
+    case GHIVE_FILTER_GT: return c0 > 0;
+    case GHIVE_FILTER_RANGE: return c0 > 0 && c1 < 0;
+    case GHIVE_FILTER_EQ_RANGE: return c0 >= 0 && c1 <= 0;
+    case GHIVE_FILTER_NOT_EQ: return c0 != 0;
+    case GHIVE_FILTER_NOT_RANGE: return !(c0 >= 0 && c1 <= 0);
+    case GHIVE_FILTER_NOT_EQ_RANGE: return !(c0 > 0 && c1 < 0);
+    default: return 0;
+  }
+}
+
 /* One row against a leaf node; value semantics (an int literal on a LONG column is widened,
  * FilterPredicate.cu:484-487; a double literal compares the converted value, as Hive's
  * FilterLongColCompareDoubleScalar does). */
 static int leaf_pass(const ghive_table *t, const ghive_pred *p, uint64_t r) {
   const ghive_column *c = &t->columns[p->filter_col[0]];
   if (col_null(c, r)) return 0;
+  if (p->pred_col_num == 2 && col_null(&t->columns[p->filter_col[1]], r)) return 0;
+  if (c->type == GHIVE_STRING) return p->mode == GHIVE_FILTER_NOT_NULL ? 1 : str_leaf_pass(t, p, r);
   if (p->pred_col_num == 2) {
     const ghive_column *d = &t->columns[p->filter_col[1]];
-    if (col_null(d, r)) return 0;
     if (col_is_int(c) && col_is_int(d))
       return cmp_mode_i(p->mode, col_i64(c, r), col_i64(d, r), 0);
     return cmp_mode_d(p->mode, col_f64(c, r), col_f64(d, r), 0.0);
@@ -257,12 +297,17 @@ static int pred_supported(const ghive_table *t, const ghive_pred *nodes, uint32_
            pred_supported(t, nodes, nnodes, (uint32_t)p->children[1], depth + 1);
   if (p->mode < 0 || p->mode > GHIVE_FILTER_NOT_NULL) return 0;
   if (p->filter_col[0] >= t->ncols) return 0;
-  if (t->columns[p->filter_col[0]].type == GHIVE_STRING) return 0;
+  const int str0 = t->columns[p->filter_col[0]].type == GHIVE_STRING;
   if (p->pred_col_num == 2) {
     if (p->filter_col[1] >= t->ncols) return 0;
-    if (t->columns[p->filter_col[1]].type == GHIVE_STRING) return 0;
-  } else if (p->mode != GHIVE_FILTER_NOT_NULL && (p->data_type < 0 || p->data_type > 2)) {
-    return 0;
+    if ((t->columns[p->filter_col[1]].type == GHIVE_STRING) != str0) return 0;
+  } else if (p->mode != GHIVE_FILTER_NOT_NULL) {
+    if (str0 != (p->data_type == 3)) return 0;
+    if (p->data_type < 0 || p->data_type > 3) return 0;
+    if (str0 && (p->param_num < 1 || p->param_num > 2 || p->string_sizes[0] < 0 ||
+                 p->string_sizes[0] > GHIVE_PRED_STR_MAX || p->string_sizes[1] < 0 ||
+                 p->string_sizes[1] > GHIVE_PRED_STR_MAX))
+      return 0;
   }
   return 1;
 }

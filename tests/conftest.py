@@ -57,3 +57,26 @@ def to_dev(a):
 
 def to_np(t):
     return t.cpu().numpy()
+
+
+def str_col(values):
+    """bytes / None values -> a ('string', blob, idx) host column (the oracle's form)."""
+    blob, idx, o = bytearray(), [], 0
+    for v in values:
+        if v is None:
+            idx += [-1, -1]
+            continue
+        blob += v
+        idx += [o, o + len(v)]
+        o += len(v)
+    return ("string", bytes(blob), np.array(idx, dtype=np.int32))
+
+
+def to_dev_col(c):
+    """numpy column or ('string', blob, idx) -> a device column (tensor or StringColumn)."""
+    if isinstance(c, tuple) and c[0] == "string":
+        import torch
+        from ghive_amd import ops
+        blob = torch.frombuffer(bytearray(c[1]), dtype=torch.uint8) if len(c[1]) else torch.zeros(0, dtype=torch.uint8)
+        return ops.StringColumn(blob.cuda(), to_dev(np.asarray(c[2], dtype=np.int32)))
+    return to_dev(c)

@@ -101,6 +101,34 @@ def test_parse_predicate_forms():
     assert p.nodes[inner.children[0]].filter_col[0] == 2 and p.nodes[inner.children[1]].filter_col[0] == 1
 
 
+def _lit(n, k):
+    return bytes(n.string_params[k][: n.string_sizes[k]])
+
+
+def test_parse_predicate_string_literals():
+    """'...' literals (FilterOperator.cu:251-255: the text between the quotes) are one token, spaces,
+    parentheses, '=' and ' AND ' inside included; the tree dump prints them as the reference does."""
+    from ghive_amd import ops
+    cols = ["p_brand1", "c_region", "s_nation", "x"]
+    P = lambda s: ops.FilterOperator(s, cols).generatePredicate()
+    p = P("(p_brand1 BETWEEN 'MFGR#2221' AND 'MFGR#2228')")
+    n = p.nodes[p.root]
+    assert n.mode == abi.FILTER_EQ_RANGE and n.data_type == 3 and n.param_num == 2
+    assert _lit(n, 0) == b"MFGR#2221" and _lit(n, 1) == b"MFGR#2228"
+    p = P("((c_region = 'AMERICA') and (s_nation = 'UNITED KI1'))")
+    top = p.nodes[p.root]
+    assert _lit(p.nodes[top.children[0]], 0) == b"UNITED KI1"
+    assert _lit(p.nodes[top.children[1]], 0) == b"AMERICA"
+    n = P("x = 'a (b) = c AND d'").nodes[0]
+    assert n.mode == abi.FILTER_EQ and n.filter_col[0] == 3 and _lit(n, 0) == b"a (b) = c AND d"
+    n = P("x <> ''").nodes[0]
+    assert n.mode == abi.FILTER_NOT_EQ and n.string_sizes[0] == 0
+    assert P("(c_region = 'ASIA')").toString() == "predicate: 0; filterCol: 1;paramNum: 1;params: ASIA ;\n"
+    for bad in ["x = 'abc", "x = '" + "y" * 65 + "'", "x BETWEEN 'a' AND 3"]:
+        with pytest.raises(Exception):
+            P(bad)
+
+
 @pytest.mark.parametrize("text", ["(a = 1", "z = 1", "a ~ 2", "a = xyz", "(a = 1) b"])
 def test_parse_predicate_errors(text):
     from ghive_amd import ops

@@ -2,7 +2,7 @@
 import numpy as np
 import pytest
 
-from conftest import to_dev, to_np
+from conftest import str_col, to_dev, to_dev_col, to_np
 from ghive_amd import abi
 
 pytestmark = pytest.mark.gpu
@@ -138,3 +138,72 @@ def test_filter_rejects_bad_descriptors(ctx):
     cyc = [abi.make_pred(abi.FILTER_AND, children=(1, 0)), abi.make_pred(abi.FILTER_EQ, 0, [1])]
     with pytest.raises(GhiveError):
         ops.FilterPredicate(cyc, 0).process(ctx, t)
+
+
+# ---------------------------------------------------------------- STRING columns --------------
+def _rand_strings(rng, n, null=0.08):
+    alphabet = [b"a", b"b", b"#", b"1", b"\x7f", b"\x80", b"\xff"]
+    return [None if rng.random() < null else b"".join(alphabet[i] for i in rng.integers(0, 7, rng.integers(0, 5)))
+            for _ in range(n)]
+
+
+def test_string_filter_every_mode_vs_oracle(ctx, orc):
+    """STRING column against string literals in every mode, and against another STRING column, in
+    Hive's order (unsigned bytes, then length), NULLs and empty strings included."""
+    from ghive_amd import ops
+    rng = np.random.default_rng(31)
+    lo, hi = b"a\x80", b"b#"
+    for n in (0, 1, 65, 4097, 120_001):
+        vals, vals2 = _rand_strings(rng, n), _rand_strings(rng, n)
+        cols = [str_col(vals), str_col(vals2)]
+        t = ops.Table([to_dev_col(c) for c in cols])
+        for mode in range(11):
+            two = mode in (abi.FILTER_RANGE, abi.FILTER_EQ_RANGE, abi.FILTER_NOT_RANGE, abi.FILTER_NOT_EQ_RANGE)
+            params = () if mode == abi.FILTER_NOT_NULL else ([lo, hi] if two else [lo])
+            node = abi.make_pred(mode, 0, params)
+            got = to_np(ops.FilterPredicate([node], 0).process(ctx, t)).astype(np.uint32)
+            assert np.array_equal(got, orc.filter_rows(cols, [node], 0)), (n, mode)
+        for mode in (abi.FILTER_EQ, abi.FILTER_LT, abi.FILTER_GE, abi.FILTER_NOT_EQ):
+            node = abi.make_pred(mode, 0, col2=1)
+            got = to_np(ops.FilterPredicate([node], 0).process(ctx, t)).astype(np.uint32)
+            assert np.array_equal(got, orc.filter_rows(cols, [node], 0)), (n, "col-col", mode)
+
+
+def test_string_filter_reference_kat_on_gpu(ctx):
+    """FilterPredicateTest.cu:99-204: clean_data of (INT, STRING, DOUBLE) rows by
+    brand BETWEEN 'MFGR#1221' AND 'MFGR#1223' keeps the six rows its expectation lists."""
+    from ghive_amd import ops
+    brands = [b"MFGR#1221", b"MFGR#1223", b"MFGR#1238", b"MFGR#1221", b"MFGR#1238", b"MFGR#122",
+              b"MFGR#1232", b"MFGR#1228", b"MFGR#123", b"MFGR#1224", b"MFGR#1248", b"MFGR#12223",
+              b"MFGR#1237", b"MFGR#1222", b"MFGR#1230", b"MFGR#1221", b"MFGR#1234", b"MFGR#1220",
+              b"MFGR#1239", b"MFGR#12234"]
+    years = np.array([1998] * 13 + [1997] * 7, dtype=np.int32)
+    price = np.array([30000.0, 80000.0, 75000.0, 27000.0, 14660.0, 23333.0, 77000.0, 38438.0, 10395.0,
+                      22422.0, 73300.0, 27999.0, 69897.0, 12941.0, 20719.0, 75246.5, 13932.4, 83299.1,
+                      49278.3, 46527.6])
+    t = ops.Table([to_dev(years), to_dev_col(str_col(brands)), to_dev(price)])
+    op = ops.FilterOperator("b BETWEEN 'MFGR#1221' AND 'MFGR#1223'", ["y", "b", "p"])
+    out = op.clean_data(ctx, t)
+    rows = list(zip(to_np(out.columns[0]).tolist(), out.columns[1].to_list(), to_np(out.columns[2]).tolist()))
+    assert rows == [(1998, b"MFGR#1221", 30000.0), (1998, b"MFGR#1223", 80000.0), (1998, b"MFGR#1221", 27000.0),
+                    (1998, b"MFGR#12223", 27999.0), (1997, b"MFGR#1222", 12941.0), (1997, b"MFGR#1221", 75246.5)]
+
+
+def test_clean_data_with_string_columns_vs_oracle(ctx, orc):
+    """Predicate text mixing string and numeric leaves; every column compacted, STRING ones as
+    (start,end) pairs into the shared blob."""
+    from ghive_amd import ops
+    rng = np.random.default_rng(8)
+    n = 300_000
+    region = [[b"AMERICA", b"ASIA", b"EUROPE", b"AFRICA", None][i] for i in rng.integers(0, 5, n)]
+    nation = _rand_strings(rng, n)
+    qty = rng.integers(0, 50, n).astype(np.int64)
+    cols = [str_col(region), str_col(nation), qty]
+    op = ops.FilterOperator("((c_region = 'AMERICA') or ((c_nation >= 'b') and (q < 10L)))", ["c_region", "c_nation", "q"])
+    out = op.clean_data(ctx, ops.Table([to_dev_col(c) for c in cols]))
+    pred = op.generatePredicate()
+    sel = orc.filter_rows(cols, pred.nodes, pred.root)
+    assert out.row_num == len(sel) > 0
+    assert out.columns[0].to_list() == [region[i] for i in sel]
+    assert out.columns[1].to_list() == [nation[i] for i in sel]
+    assert np.array_equal(to_np(out.columns[2]), qty[sel])

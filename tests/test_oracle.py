@@ -321,6 +321,60 @@ def test_exact_sum_against_fsum(orc):
     assert orc.exact_sum(np.array([0.1] * 10)) == math.fsum([0.1] * 10)
 
 
+# ---------------------------------------------------------------- STRING filters -------------
+def _str_col(values):
+    blob, idx, o = bytearray(), [], 0
+    for v in values:
+        if v is None:
+            idx += [-1, -1]
+            continue
+        blob += v
+        idx += [o, o + len(v)]
+        o += len(v)
+    return ("string", bytes(blob), np.array(idx, dtype=np.int32))
+
+
+def test_string_filter_reference_kat(orc):
+    """FilterPredicateTest.cu:99-204: BETWEEN 'MFGR#1221' AND 'MFGR#1223' over 20 brands keeps the
+    six rows its (commented) expectation lists."""
+    brands = [b"MFGR#1221", b"MFGR#1223", b"MFGR#1238", b"MFGR#1221", b"MFGR#1238", b"MFGR#122",
+              b"MFGR#1232", b"MFGR#1228", b"MFGR#123", b"MFGR#1224", b"MFGR#1248", b"MFGR#12223",
+              b"MFGR#1237", b"MFGR#1222", b"MFGR#1230", b"MFGR#1221", b"MFGR#1234", b"MFGR#1220",
+              b"MFGR#1239", b"MFGR#12234"]
+    node = abi.make_pred(abi.FILTER_EQ_RANGE, 0, [b"MFGR#1221", b"MFGR#1223"])
+    sel = orc.filter_rows([_str_col(brands)], [node], 0)
+    assert [brands[i] for i in sel] == [b"MFGR#1221", b"MFGR#1223", b"MFGR#1221", b"MFGR#12223",
+                                        b"MFGR#1222", b"MFGR#1221"]
+
+
+def test_string_filter_brute_force(orc):
+    """Every mode against Python's bytes order, which is Hive's (unsigned bytes, then length):
+    prefixes, empty strings, bytes above 0x7f, NULLs, and STRING column against STRING column."""
+    rng = np.random.default_rng(17)
+    alphabet = [b"a", b"b", b"\x7f", b"\x80", b"\xff"]
+    vals = [None if rng.random() < 0.1 else b"".join(alphabet[i] for i in rng.integers(0, 5, rng.integers(0, 4)))
+            for _ in range(3000)]
+    vals2 = [None if rng.random() < 0.1 else b"".join(alphabet[i] for i in rng.integers(0, 5, rng.integers(0, 4)))
+             for _ in range(3000)]
+    col, col2 = _str_col(vals), _str_col(vals2)
+    lo, hi = b"a\x80", b"\x7f"
+    lo, hi = min(lo, hi), max(lo, hi)
+    ref = {abi.FILTER_EQ: lambda v: v == lo, abi.FILTER_LE: lambda v: v <= lo, abi.FILTER_GE: lambda v: v >= lo,
+           abi.FILTER_LT: lambda v: v < lo, abi.FILTER_GT: lambda v: v > lo, abi.FILTER_NOT_EQ: lambda v: v != lo,
+           abi.FILTER_RANGE: lambda v: lo < v < hi, abi.FILTER_EQ_RANGE: lambda v: lo <= v <= hi,
+           abi.FILTER_NOT_RANGE: lambda v: not (lo <= v <= hi), abi.FILTER_NOT_EQ_RANGE: lambda v: not (lo < v < hi)}
+    for mode, f in ref.items():
+        two = mode in (abi.FILTER_RANGE, abi.FILTER_EQ_RANGE, abi.FILTER_NOT_RANGE, abi.FILTER_NOT_EQ_RANGE)
+        node = abi.make_pred(mode, 0, [lo, hi] if two else [lo])
+        want = [i for i, v in enumerate(vals) if v is not None and f(v)]
+        assert orc.filter_rows([col], [node], 0).tolist() == want, mode
+    node = abi.make_pred(abi.FILTER_NOT_NULL, 0)
+    assert orc.filter_rows([col], [node], 0).tolist() == [i for i, v in enumerate(vals) if v is not None]
+    node = abi.make_pred(abi.FILTER_LT, 0, col2=1)
+    want = [i for i, (u, v) in enumerate(zip(vals, vals2)) if u is not None and v is not None and u < v]
+    assert orc.filter_rows([col, col2], [node], 0).tolist() == want
+
+
 # ---------------------------------------------------------------- projection arithmetic -------
 def _java_int(v):
     return (v + 2**31) % 2**32 - 2**31
