@@ -253,15 +253,6 @@ __global__ __launch_bounds__(kGbPartThreads) void k_gb_part_agg(KeySpec K, Group
 // represents it, and a row matches a slot when the hash bits agree and its key columns equal the
 // representative's. Slots never change once claimed and the input columns are immutable, so a
 // reader never waits for a writer. Row ids stay below 2^32 - 1, so no slot word equals EMPTY.
-constexpr int kMaxWideKeys = 8;
-
-struct WideKeySpec {
-  int32_t nkeys;
-  int32_t type[kMaxWideKeys];
-  const void *data[kMaxWideKeys];    // values; STRING: the character blob
-  const int32_t *aux[kMaxWideKeys];  // STRING: (start,end) per row, (-1,-1) = NULL
-};
-
 struct WideOut {
   void *col[kMaxWideKeys];
   int32_t *aux[kMaxWideKeys];
@@ -378,6 +369,32 @@ __global__ __launch_bounds__(256) void k_gbw_emit_keys(WideKeySpec K, const uint
       }
     }
   }
+}
+
+int key_dictionary(ghive_ctx *ctx, const WideKeySpec &K, uint64_t rows, const uint32_t *sel, uint64_t n,
+                   uint64_t capacity, DevBuf &dict, DevBuf &codes, uint64_t *slots, uint64_t *count) {
+  if (capacity > (1ull << 31))
+    return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "key dictionary: more than 2^31 distinct keys");
+  *slots = next_pow2(capacity * 2 < 64 ? 64 : capacity * 2);
+  *count = 0;
+  GH_TRY(dict.alloc(ctx, *slots * 8));
+  GH_TRY(codes.alloc(ctx, (rows ? rows : 1) * 4));
+  GH_TRY(fill_u64(ctx, dict.as<uint64_t>(), kEmptyKey, *slots));
+  GH_TRY(fill_u64(ctx, ctx->d_counters, 0, 2));
+  if (n) {
+    const uint64_t g = (n + 255) / 256, cap = (uint64_t)ctx->cus * 16;
+    k_gbw_codes<<<(uint32_t)(g > cap ? cap : g), 256, 0, ctx->stream>>>(
+        K, dict.as<uint64_t>(), *slots - 1, capacity, reinterpret_cast<unsigned long long *>(ctx->d_counters), sel,
+        n, codes.as<uint32_t>());
+    GH_LAUNCH(ctx);
+  }
+  uint64_t c[2] = {0, 0};
+  GH_TRY(d2h(ctx, c, ctx->d_counters, sizeof c));
+  *count = c[0];
+  if (c[1] || c[0] > capacity)
+    return set_error(ctx, GHIVE_ERR_CAPACITY, "key dictionary: more than %llu distinct keys",
+                     (unsigned long long)capacity);
+  return GHIVE_OK;
 }
 
 __global__ void k_gt_preinsert(GroupTableDev T, uint64_t key) {
@@ -727,28 +744,17 @@ int ghive_groupby(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel,
   if (wide) {
     if (capacity > (1ull << 31))
       return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "groupby: wide keys with a group bound above 2^31");
-    const uint64_t slots = next_pow2(capacity * 2 < 64 ? 64 : capacity * 2);
-    GH_TRY(wdict.alloc(ctx, slots * 8));
-    GH_TRY(wcodes.alloc(ctx, (table->row_num ? table->row_num : 1) * 4));
+    uint64_t slots = 0, count = 0;
     GH_TRY(wgcode.alloc(ctx, (capacity ? capacity : 1) * 4));
-    dict = wdict.as<uint64_t>();
-    GH_TRY(fill_u64(ctx, dict, kEmptyKey, slots));
-    GH_TRY(fill_u64(ctx, ctx->d_counters, 0, 2));
-    if (n) {
-      const uint64_t g = (n + 255) / 256, cap = (uint64_t)ctx->cus * 16;
-      k_gbw_codes<<<(uint32_t)(g > cap ? cap : g), 256, 0, ctx->stream>>>(
-          WK, dict, slots - 1, capacity, reinterpret_cast<unsigned long long *>(ctx->d_counters), sel, n,
-          wcodes.as<uint32_t>());
-      GH_LAUNCH(ctx);
-    }
-    uint64_t c[2] = {0, 0};
-    GH_TRY(d2h(ctx, c, ctx->d_counters, sizeof c));
-    if (c[1] || c[0] > capacity) {
-      *out_groups = c[0] > capacity ? c[0] : capacity + 1;
+    const int st = key_dictionary(ctx, WK, table->row_num, sel, n, capacity, wdict, wcodes, &slots, &count);
+    if (st == GHIVE_ERR_CAPACITY) {
+      *out_groups = count > capacity ? count : capacity + 1;
       for (uint32_t k = 0; k < nkeys + naggs; ++k) out_cols[k].row_num = 0;
       return set_error(ctx, GHIVE_ERR_CAPACITY, "group-by: more than %llu groups (capacity)",
                        (unsigned long long)capacity);
     }
+    GH_TRY(st);
+    dict = wdict.as<uint64_t>();
     K.nkeys = 1;  // aggregate on the dictionary code
     K.type[0] = GHIVE_INT;
     K.shift[0] = 0;

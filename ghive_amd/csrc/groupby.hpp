@@ -16,6 +16,15 @@ struct KeySpec {
   const void *data[kMaxKeys];
 };
 
+// Key columns of any type, STRING included, compared by value (the dictionary form of the group-by).
+constexpr int kMaxWideKeys = 8;
+struct WideKeySpec {
+  int32_t nkeys;
+  int32_t type[kMaxWideKeys];
+  const void *data[kMaxWideKeys];    // values; STRING: the character blob
+  const int32_t *aux[kMaxWideKeys];  // STRING: (start,end) per row, (-1,-1) = NULL
+};
+
 struct OutSpec {
   void *col[kMaxKeys + kMaxAggs];
   int32_t type[kMaxKeys + kMaxAggs];
@@ -32,6 +41,12 @@ int group_table_preinsert(ghive_ctx *ctx, const GroupTableDev &T, uint64_t key);
 // id; *keys_out / *ids_out point at the buffers holding the result.
 int radix_sort_u64_ids(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, uint32_t *i0, uint32_t *i1,
                        const std::vector<int> &digits, uint64_t **keys_out, uint32_t **ids_out);
+// Key dictionary of rows sel[0..n) (sel NULL: rows 0..n): codes[r] (rows entries) = the slot of
+// row r's key; dict[slot] = (32 hash bits << 32) | the key's representative row, or all ones when
+// empty; *slots = dictionary size (a power of two >= 2 x capacity); *count = distinct keys.
+// CAPACITY when there are more than `capacity` distinct keys.
+int key_dictionary(ghive_ctx *ctx, const WideKeySpec &K, uint64_t rows, const uint32_t *sel, uint64_t n,
+                   uint64_t capacity, DevBuf &dict, DevBuf &codes, uint64_t *slots, uint64_t *count);
 // Writes every group to the output columns; CAPACITY error when there are more than capacity.
 int group_table_emit(ghive_ctx *ctx, const KeySpec &K, const GroupTableDev &T, const OutSpec &O,
                      uint64_t capacity, uint64_t *out_groups);

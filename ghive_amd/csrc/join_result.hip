@@ -22,13 +22,17 @@ __global__ __launch_bounds__(kThreads) void k_gather_offsets(const T *in, const 
 }
 
 // STRING: the (start,end) pair of each row; (-1,-1) for NULL.
-__global__ __launch_bounds__(kThreads) void k_gather_pairs(const int2 *in, const int32_t *offsets, uint32_t width,
-                                                           uint32_t w, uint64_t n, int2 *out) {
+struct StrPair {  // 4-byte aligned, like the int32 pairs it copies
+  int32_t start, end;
+};
+
+__global__ __launch_bounds__(kThreads) void k_gather_pairs(const StrPair *in, const int32_t *offsets, uint32_t width,
+                                                           uint32_t w, uint64_t n, StrPair *out) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) {
     const int32_t r = offsets[i * width + w];
-    out[i] = r < 0 ? make_int2(-1, -1) : in[r];
+    out[i] = r < 0 ? StrPair{-1, -1} : in[r];
   }
 }
 
@@ -90,7 +94,7 @@ int ghive_join_result(ghive_ctx *ctx, const ghive_table *tables, uint32_t ntable
   for (size_t o = 0; o < plan.size(); ++o) {
     const ghive_column &in = tables[plan[o].table].columns[plan[o].col];
     ghive_column &out = out_cols[o];
-    if (cardinality && (!out.data || (in.type == GHIVE_STRING && !out.aux)))
+    if (cardinality && (in.type == GHIVE_STRING ? !out.aux : !out.data))
       return set_error(ctx, GHIVE_ERR_INVALID, "join_result: output column %zu has no buffer", o);
     if (cardinality && tables[plan[o].table].row_num && !in.data)
       return set_error(ctx, GHIVE_ERR_INVALID, "join_result: table %u column %u has no data", plan[o].table,
@@ -127,8 +131,8 @@ int ghive_join_result(ghive_ctx *ctx, const ghive_table *tables, uint32_t ntable
         break;
       case GHIVE_STRING:  // the blob is shared; only the (start,end) pairs are gathered
         out.data = in.data;
-        k_gather_pairs<<<g, kThreads, 0, ctx->stream>>>(static_cast<const int2 *>(in.aux), offsets, width, w,
-                                                        cardinality, static_cast<int2 *>(out.aux));
+        k_gather_pairs<<<g, kThreads, 0, ctx->stream>>>(static_cast<const StrPair *>(in.aux), offsets, width, w,
+                                                        cardinality, static_cast<StrPair *>(out.aux));
         break;
       default:
         return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "join_result: column type %d", in.type);

@@ -14,6 +14,7 @@
 // number N (the reference's greedy regex keeps only the last digit); a VALUE column of a
 // group-by over a reduce input is offset by that sink's key count at parse time (the reference
 // reads the key count of the child's runtime result).
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <regex>
@@ -704,18 +705,21 @@ struct Executor {
     err = m;
     return code;
   }
+  // A STRING column gets its (start,end) pairs; the operator that fills it points data at the blob.
   int alloc_col(int type, uint64_t rows, ghive_column &c) {
     memset(&c, 0, sizeof c);
     c.type = type;
     c.row_num = rows;
-    const uint64_t w = (type == GHIVE_LONG || type == GHIVE_DOUBLE) ? 8 : 4;
+    const uint64_t w = (type == GHIVE_LONG || type == GHIVE_DOUBLE || type == GHIVE_STRING) ? 8 : 4;
     void *p = nullptr;
     const int st = ghive_malloc(ctx, (rows ? rows : 1) * w, &p);
     if (st != GHIVE_OK) return fail(st, std::string("plan: ") + ghive_last_error(ctx));
     owned.push_back(p);
-    c.data = p;
+    if (type == GHIVE_STRING) c.aux = p;
+    else c.data = p;
     return GHIVE_OK;
   }
+  int op_fail(int st) { return fail(st, std::string("plan: ") + ghive_last_error(ctx)); }
   int op_error(int st, const Op &o) {
     return fail(st, "plan: " + o.id + ": " + ghive_last_error(ctx));
   }
@@ -807,6 +811,24 @@ struct Executor {
         }
       }
       return expression(in, t, out);  // redundant parentheses
+    }
+    if (e.size() >= 2 && e.front() == '\'' && e.back() == '\'') {  // a string constant, on every row
+      const std::string lit = e.substr(1, e.size() - 2);
+      int st = alloc_col(GHIVE_STRING, t.rows, out);
+      if (st != GHIVE_OK) return st;
+      void *blob = nullptr;
+      if ((st = ghive_malloc(ctx, lit.size() + 1, &blob)) != GHIVE_OK) return op_fail(st);
+      owned.push_back(blob);
+      if ((st = ghive_memcpy(ctx, blob, lit.c_str(), lit.size() + 1, 0)) != GHIVE_OK) return op_fail(st);
+      std::vector<int32_t> pairs(2 * t.rows);
+      for (uint64_t r = 0; r < t.rows; ++r) {
+        pairs[2 * r] = 0;
+        pairs[2 * r + 1] = (int32_t)lit.size();
+      }
+      if (t.rows && (st = ghive_memcpy(ctx, out.aux, pairs.data(), 8 * t.rows, 0)) != GHIVE_OK) return op_fail(st);
+      out.data = blob;
+      out.char_size = lit.size();
+      return GHIVE_OK;
     }
     if (std::regex_match(e, m, val_re)) {  // values follow the keys of a reduce input
       const int c = std::stoi(m[1].str()) + t.reduce_keys;
@@ -959,18 +981,40 @@ struct Executor {
         from_map = true;
       }
     if (!from_map) {
+      // Without the Java side's map: Hive names a join's outputs _colN by position in the
+      // concatenation of its inputs' columns, so input 0's columns keep their names and input 1's
+      // come renumbered after them. Input 0 maps by name; the rest map to input 1 by the offset that
+      // puts the last of them on input 1's last non-key column (exact when that column is kept, as
+      // in every SSB plan).
+      static const std::regex col_re(R"(^_col(\d+)$)");
+      std::vector<std::pair<long, std::string>> rest;
       for (const std::string &n : o.outputs) {
-        bool found = false;
-        for (size_t t = 0; t < nt && !found; ++t) {
-          const int c = ts[t].find(n);
-          if (c >= 0) {
-            keep[t].push_back((uint32_t)c);
-            found = true;
-          }
+        const int c = ts[0].find(n);
+        if (c >= 0) {
+          keep[0].push_back((uint32_t)c);
+          continue;
         }
-        if (!found)
+        std::smatch m;
+        if (nt != 2 || !std::regex_match(n, m, col_re))
           return fail(GHIVE_ERR_UNSUPPORTED, "plan: " + o.id + ": output '" + n +
-                                                 "' is not a column of an input; pass the maintain-values map");
+                                                 "' is not a column of input 0; pass the maintain-values map");
+        rest.push_back({std::stol(m[1].str()), n});
+      }
+      if (!rest.empty()) {
+        long jmax = -1;
+        for (size_t c = 0; c < ts[1].names.size(); ++c) {
+          std::smatch m;
+          if ((int)c != o.join_cols[1][0] && std::regex_match(ts[1].names[c], m, col_re))
+            jmax = std::max(jmax, std::stol(m[1].str()));
+        }
+        const long offset = rest.back().first - jmax;
+        for (const auto &r : rest) {
+          const int c = jmax < 0 ? -1 : ts[1].find("_col" + std::to_string(r.first - offset));
+          if (c < 0)
+            return fail(GHIVE_ERR_UNSUPPORTED, "plan: " + o.id + ": cannot place output '" + r.second +
+                                                   "' in input 1; pass the maintain-values map");
+          keep[1].push_back((uint32_t)c);
+        }
       }
     }
     std::vector<const uint32_t *> kp;

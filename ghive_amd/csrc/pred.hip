@@ -135,6 +135,12 @@ struct Parser {
       err = "unsupported predicate: '" + s + "'";
       return -1;
     }
+    // UDFToDouble(x), UDFToLong(x), ...: a cast of a numeric column, which the comparison of
+    // values already performs (Hive writes it around a column compared with another type's literal)
+    for (const char *cast : {"UDFToDouble(", "UDFToFloat(", "UDFToLong(", "UDFToInteger("}) {
+      const size_t L3 = strlen(cast);
+      if (col.size() > L3 + 1 && col.compare(0, L3, cast) == 0 && col.back() == ')') col = col.substr(L3, col.size() - L3 - 1);
+    }
     int c = col_index(col);
     if (c < 0) {
       err = "column '" + col + "' is not an output column of the operator";
@@ -438,6 +444,16 @@ int ghive_parse_predicate(const char *text, const char *const *col_names, uint32
       if (!e) return fail("unterminated string literal");
       cur.append(c, (size_t)(e - c + 1));
       c = e;
+      continue;
+    }
+    if (*c == '(' && !cur.empty()) {  // f(x): a function call is part of its token
+      int depth = 0;
+      for (; *c; ++c) {
+        cur += *c;
+        if (*c == '(') ++depth;
+        if (*c == ')' && --depth == 0) break;
+      }
+      if (!*c) return fail("missing ')'");
       continue;
     }
     if (*c == ' ' || *c == '(' || *c == ')') {

@@ -12,6 +12,7 @@
 
 #include "common.hpp"
 #include "device.hpp"
+#include "groupby.hpp"
 #include "scan.hpp"
 
 namespace ghive {
@@ -235,6 +236,140 @@ int radix_sort_u64_ids(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, u
   return radix_passes<1>(ctx, n, k0, k1, i0, i1, digits, ids_out, keys_out);
 }
 
+// ---- STRING keys: dense ranks in Hive's string order -------------------------------------------
+// A STRING key sorts by the rank of its value among the column's distinct values. The key
+// dictionary of the group-by names each distinct value once (a representative row); the distinct
+// values are sorted by stable LSD passes over one 64-bit word each: first the length, then the
+// 8-byte big-endian chunks from the last to the first (zero padded), which is the order of
+// unsigned bytes, then length (StringExpr.compare). NULL takes all-ones words: it ranks last.
+constexpr uint64_t kDictEmpty = ~0ull;
+
+// The used dictionary slots, compacted: dslot[i] = slot, drep[i] = representative row.
+__global__ void k_dict_list(const uint64_t *dict, uint64_t slots, unsigned long long *cnt, uint32_t *dslot,
+                            uint32_t *drep) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t iters = (slots + stride - 1) / stride;  // the same trip count in every lane
+  for (uint64_t it = 0; it < iters; ++it) {
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x + it * stride;
+    const uint64_t w = s < slots ? dict[s] : kDictEmpty;
+    const bool used = w != kDictEmpty;
+    const uint64_t idx = wave_append(cnt, used);
+    if (used) {
+      dslot[idx] = (uint32_t)s;
+      drep[idx] = (uint32_t)w;
+    }
+  }
+}
+
+__global__ void k_str_maxlen(const int32_t *aux, const uint32_t *drep, uint64_t d, unsigned int *out) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  unsigned int m = 0;
+  for (; i < d; i += stride) {
+    const uint64_t r = drep[i];
+    const int32_t s = aux[2 * r], e = aux[2 * r + 1];
+    if (s >= 0 && e > s && (unsigned)(e - s) > m) m = (unsigned)(e - s);
+  }
+  if (m) atomicMax(out, m);
+}
+
+// key[i] = word `word` of the i-th value in the current order (word -1: the length).
+__global__ void k_str_word(const uint8_t *blob, const int32_t *aux, const uint32_t *drep, const uint32_t *ids,
+                           uint64_t d, int word, uint64_t *key) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < d; i += stride) {
+    const uint64_t r = drep[ids[i]];
+    const int32_t s = aux[2 * r], e = aux[2 * r + 1];
+    uint64_t w;
+    if (s < 0) {
+      w = ~0ull;
+    } else if (word < 0) {
+      w = (uint64_t)(uint32_t)(e - s);
+    } else {
+      w = 0;
+      const int64_t p0 = (int64_t)s + 8 * (int64_t)word;
+      for (int b = 0; b < 8; ++b) w = (w << 8) | (p0 + b < e ? blob[p0 + b] : 0u);
+    }
+    key[i] = w;
+  }
+}
+
+__global__ void k_rank_of_slot(const uint32_t *dslot, const uint32_t *ids, uint64_t d, uint32_t *rank_of_slot) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < d; i += stride) rank_of_slot[dslot[ids[i]]] = (uint32_t)i;
+}
+
+__global__ void k_row_rank(const uint32_t *codes, uint64_t n, const uint32_t *rank_of_slot, int32_t *rank) {
+  uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; r < n; r += stride) rank[r] = (int32_t)rank_of_slot[codes[r]];
+}
+
+__global__ void k_iota_u32(uint32_t *p, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = (uint32_t)i;
+}
+
+// rank[r] (n rows) = rank of row r's value of STRING column c among its distinct values.
+int string_ranks(ghive_ctx *ctx, const ghive_column &c, uint64_t n, int32_t *rank) {
+  WideKeySpec K;
+  memset(&K, 0, sizeof K);
+  K.nkeys = 1;
+  K.type[0] = GHIVE_STRING;
+  K.data[0] = c.data;
+  K.aux[0] = static_cast<const int32_t *>(c.aux);
+  DevBuf dict, codes;
+  uint64_t slots = 0, d = 0;
+  GH_TRY(key_dictionary(ctx, K, n, nullptr, n, n, dict, codes, &slots, &d));
+  DevBuf dslot, drep, k0, k1, i0, i1, ros;
+  GH_TRY(dslot.alloc(ctx, 4 * d));
+  GH_TRY(drep.alloc(ctx, 4 * d));
+  GH_TRY(k0.alloc(ctx, 8 * d));
+  GH_TRY(k1.alloc(ctx, 8 * d));
+  GH_TRY(i0.alloc(ctx, 4 * d));
+  GH_TRY(i1.alloc(ctx, 4 * d));
+  GH_TRY(ros.alloc(ctx, 4 * slots));
+  auto grid_for = [&](uint64_t m) {
+    const uint64_t g = (m + 255) / 256, cap = (uint64_t)ctx->cus * 16;
+    return (uint32_t)(g > cap ? cap : (g ? g : 1));
+  };
+  GH_TRY(fill_u64(ctx, ctx->d_counters, 0, 2));
+  k_dict_list<<<grid_for(slots), 256, 0, ctx->stream>>>(dict.as<uint64_t>(), slots,
+                                                       reinterpret_cast<unsigned long long *>(ctx->d_counters),
+                                                       dslot.as<uint32_t>(), drep.as<uint32_t>());
+  GH_LAUNCH(ctx);
+  unsigned int *d_max = reinterpret_cast<unsigned int *>(ctx->d_counters + 1);
+  k_str_maxlen<<<grid_for(d), 256, 0, ctx->stream>>>(K.aux[0], drep.as<uint32_t>(), d, d_max);
+  GH_LAUNCH(ctx);
+  unsigned int maxlen = 0;
+  GH_TRY(d2h(ctx, &maxlen, d_max, sizeof maxlen));
+  k_iota_u32<<<grid_for(d), 256, 0, ctx->stream>>>(i0.as<uint32_t>(), d);
+  GH_LAUNCH(ctx);
+  std::vector<int> digits;
+  for (int b = 0; b < 8; ++b) digits.push_back(b);
+  uint32_t *ids = i0.as<uint32_t>(), *other = i1.as<uint32_t>();
+  const int nwords = (int)((maxlen + 7) / 8);
+  for (int word = -1; word < nwords; ++word) {  // least significant first: the length, then chunks last to first
+    const int w = word < 0 ? -1 : nwords - 1 - word;
+    k_str_word<<<grid_for(d), 256, 0, ctx->stream>>>(static_cast<const uint8_t *>(c.data), K.aux[0],
+                                                     drep.as<uint32_t>(), ids, d, w, k0.as<uint64_t>());
+    GH_LAUNCH(ctx);
+    uint64_t *kout = nullptr;
+    uint32_t *iout = nullptr;
+    GH_TRY(radix_sort_u64_ids(ctx, d, k0.as<uint64_t>(), k1.as<uint64_t>(), ids, other, digits, &kout, &iout));
+    other = iout == ids ? other : ids;
+    ids = iout;
+  }
+  k_rank_of_slot<<<grid_for(d), 256, 0, ctx->stream>>>(dslot.as<uint32_t>(), ids, d, ros.as<uint32_t>());
+  GH_LAUNCH(ctx);
+  k_row_rank<<<grid_for(n), 256, 0, ctx->stream>>>(codes.as<uint32_t>(), n, ros.as<uint32_t>(), rank);
+  GH_LAUNCH(ctx);
+  return GHIVE_OK;
+}
+
 }  // namespace ghive
 
 using namespace ghive;
@@ -253,16 +388,25 @@ int ghive_sort(ghive_ctx *ctx, const ghive_table *table, const uint32_t *key_col
   memset(&S, 0, sizeof S);
   S.nkeys = (int)nkeys;
   int bits = 0;
+  DevBuf ranks[8];  // STRING keys sort by the rank of their value
   for (uint32_t k = 0; k < nkeys; ++k) {
     if (key_cols[k] >= table->ncols) return set_error(ctx, GHIVE_ERR_INVALID, "sort: key column %u", key_cols[k]);
     const ghive_column &c = table->columns[key_cols[k]];
-    if (!is_numeric_type(c.type)) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "sort: STRING keys are not supported");
     char o = orders ? orders[k] : '+';
     if (o != '+' && o != '-') return set_error(ctx, GHIVE_ERR_INVALID, "sort: order '%c' (use '+' or '-')", o);
-    S.type[k] = c.type;
     S.desc[k] = o == '-';
-    S.width[k] = 8 * type_width(c.type);
-    S.data[k] = c.data;
+    if (c.type == GHIVE_STRING) {
+      if (n > (1ull << 31)) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "sort: STRING keys over 2^31 rows");
+      GH_TRY(ranks[k].alloc(ctx, 4 * (n ? n : 1)));
+      if (n) GH_TRY(string_ranks(ctx, c, n, ranks[k].as<int32_t>()));
+      S.type[k] = GHIVE_INT;
+      S.width[k] = 32;
+      S.data[k] = ranks[k].get();
+    } else {
+      S.type[k] = c.type;
+      S.width[k] = 8 * type_width(c.type);
+      S.data[k] = c.data;
+    }
     bits += S.width[k];
   }
   if (bits > 256) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "sort: keys are %d bits wide (max 256)", bits);

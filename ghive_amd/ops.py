@@ -723,9 +723,19 @@ class Plan:
             for i in range(ncols.value):
                 c, nm = abi.Column(), C.c_char_p()
                 ctx.check(lib().ghive_result_column(r, i, C.byref(c), C.byref(nm)))
-                t = torch.empty(rows.value, dtype=_DTYPE_OF_TYPE[c.type], device=f"cuda:{ctx.device}")
-                if rows.value:
-                    ctx.check(lib().ghive_memcpy(ctx.h, _p(t), c.data, rows.value * t.element_size(), 2))
+                dev = f"cuda:{ctx.device}"
+                if c.type == abi.STRING:  # copy the (start,end) pairs and the blob they point into
+                    idx = torch.empty(2 * rows.value, dtype=torch.int32, device=dev)
+                    blob = torch.empty(max(c.char_size, 1), dtype=torch.uint8, device=dev)
+                    if rows.value:
+                        ctx.check(lib().ghive_memcpy(ctx.h, _p(idx), c.aux, 8 * rows.value, 2))
+                    if c.char_size:
+                        ctx.check(lib().ghive_memcpy(ctx.h, _p(blob), c.data, c.char_size, 2))
+                    t = StringColumn(blob, idx)
+                else:
+                    t = torch.empty(rows.value, dtype=_DTYPE_OF_TYPE[c.type], device=dev)
+                    if rows.value:
+                        ctx.check(lib().ghive_memcpy(ctx.h, _p(t), c.data, rows.value * t.element_size(), 2))
                 names.append(nm.value.decode())
                 cols.append(t)
             return names, cols

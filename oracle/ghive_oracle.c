@@ -905,7 +905,14 @@ static int sort_cmp(const void *pa, const void *pb, void *pc) {
     const ghive_column *col = &c->t->columns[c->keys[k]];
     int asc = c->orders[k] != '-';
     int r = 0;
-    if (col_is_int(col)) {
+    if (col->type == GHIVE_STRING) { /* Hive's string order; NULL sorts last ascending */
+      const int32_t *ix = (const int32_t *)col->aux;
+      int nx = ix[2 * i] < 0, ny = ix[2 * j] < 0;
+      if (nx || ny) r = nx == ny ? 0 : (nx ? 1 : -1);
+      else r = str_cmp((const uint8_t *)col->data + ix[2 * i], ix[2 * i + 1] - ix[2 * i],
+                       (const uint8_t *)col->data + ix[2 * j], ix[2 * j + 1] - ix[2 * j]);
+      r = r < 0 ? -1 : (r > 0 ? 1 : 0);
+    } else if (col_is_int(col)) {
       int64_t x = col_i64(col, i), y = col_i64(col, j);
       r = x < y ? -1 : (x > y ? 1 : 0);
     } else {
@@ -923,7 +930,6 @@ int orc_sort(const ghive_table *table, const uint32_t *key_cols, const char *ord
              uint32_t nkeys, uint32_t *out_perm) {
   for (uint32_t k = 0; k < nkeys; ++k) {
     if (key_cols[k] >= table->ncols) return -2;
-    if (table->columns[key_cols[k]].type == GHIVE_STRING) return -2;
   }
   for (uint64_t i = 0; i < table->row_num; ++i) out_perm[i] = (uint32_t)i;
   orc_sort_ctx c = {table, key_cols, orders, nkeys};

@@ -3,7 +3,7 @@
 import numpy as np
 import pytest
 
-from conftest import to_dev, to_np
+from conftest import str_col, to_dev, to_dev_col, to_np
 from ghive_amd import abi
 
 pytestmark = pytest.mark.gpu
@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 
 def _check(ctx, orc, cols, keys, orders):
     from ghive_amd import ops
-    perm = to_np(ops.sort(ctx, ops.Table([to_dev(c) for c in cols]), keys, orders)).astype(np.uint32)
+    perm = to_np(ops.sort(ctx, ops.Table([to_dev_col(c) for c in cols]), keys, orders)).astype(np.uint32)
     want = orc.sort_perm(cols, keys, orders)
     assert np.array_equal(perm, want)
 
@@ -57,6 +57,22 @@ def test_sort_composite_mixed_orders(ctx, orc):
     c = (rng.integers(0, 4, n) / 2).astype(np.float64)
     _check(ctx, orc, [a, b, c], [0, 1, 2], ["-", "+", "-"])
     _check(ctx, orc, [a, b, c], [2, 0], ["+", "+"])
+
+
+@pytest.mark.parametrize("n", [1, 4097, 200_003])
+def test_sort_string_keys(ctx, orc, n):
+    """STRING keys (ranked among their distinct values, then sorted as INT) against the oracle:
+    shared prefixes, empty strings, NUL and high bytes, NULLs, long strings spanning several
+    8-byte chunks, both orders, with a numeric tiebreak key."""
+    rng = np.random.default_rng(n)
+    alphabet = [b"a", b"b", b"\x00", b"\x80", b"\xff", b"MFGR#1", b"UNITED KI"]
+    vals = [None if rng.random() < 0.03 else b"".join(alphabet[i] for i in rng.integers(0, 7, rng.integers(0, 6)))
+            for _ in range(n)]
+    k2 = rng.integers(0, 4, n).astype(np.int64)
+    col = str_col(vals)
+    for orders in (["+", "+"], ["-", "+"], ["+", "-"]):
+        _check(ctx, orc, [col, k2], [0, 1], orders)
+    _check(ctx, orc, [k2, col], [0, 1], ["+", "-"])
 
 
 def test_sort_large_sorted_and_permutation(ctx):

@@ -432,6 +432,38 @@ def test_sort_total_order(orc):
     assert np.array_equal(perm, want)
 
 
+def test_sort_string_keys_against_python_order(orc):
+    """STRING keys sort in Python's bytes order (unsigned bytes, then length = StringExpr.compare),
+    NULL last ascending, ties by the next key and then by row id."""
+    rng = np.random.default_rng(14)
+    alphabet = [b"a", b"b", b"\x00", b"\x80", b"\xff"]
+    n = 4000
+    vals = [None if rng.random() < 0.05 else b"".join(alphabet[i] for i in rng.integers(0, 5, rng.integers(0, 4)))
+            for _ in range(n)]
+    k2 = rng.integers(0, 3, n).astype(np.int64)
+    col = _str_col(vals)
+    for orders in (["+", "+"], ["-", "+"], ["+", "-"]):
+        perm = orc.sort_perm([col, k2], [0, 1], orders)
+        s0 = 1 if orders[0] == "+" else -1
+        s1 = 1 if orders[1] == "+" else -1
+
+        def key(i):
+            v = vals[i]
+            v = (1, b"") if v is None else (0, v)
+            return v, k2[i] * s1, i
+        want = sorted(range(n), key=key)
+        if s0 < 0:  # descending on the string (NULL first), ascending ties
+            from functools import cmp_to_key
+
+            def cmp(i, j):
+                a, b = key(i)[0], key(j)[0]
+                if a != b:
+                    return -1 if a > b else 1
+                return -1 if (key(i)[1:] < key(j)[1:]) else (1 if key(i)[1:] > key(j)[1:] else 0)
+            want = sorted(range(n), key=cmp_to_key(cmp))
+        assert perm.tolist() == want, orders
+
+
 # ---------------------------------------------------------------- skewed-join generator ------
 def test_key62_is_injective_and_below_null(orc):
     ranks = np.arange(1, 200001, dtype=np.uint64)
