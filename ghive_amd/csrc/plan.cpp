@@ -724,14 +724,22 @@ struct Executor {
     return fail(st, "plan: " + o.id + ": " + ghive_last_error(ctx));
   }
 
-  int from_input(const std::string &name, const std::vector<std::string> &want, ExecTable &t) {
+  // Binds input `name` (else `alt`, when given). Without column names an input's columns are
+  // positional: `positional` names them when it has one name per column (a TableScan's outputs, a
+  // reduce sink's keys-first outputs -- the order GTables delivers them in), else _col0.._colN-1.
+  int from_input(const std::string &name, const std::string &alt, const std::vector<std::string> &want,
+                 const std::vector<std::string> &positional, ExecTable &t) {
     auto it = inputs.find(name);
+    if (it == inputs.end() && !alt.empty()) it = inputs.find(alt);
     if (it == inputs.end()) return fail(GHIVE_ERR_INVALID, "plan: no input named '" + name + "'");
     const ghive_plan_input &in = *it->second;
     if (!in.table) return fail(GHIVE_ERR_INVALID, "plan: input '" + name + "' has no table");
     std::vector<std::string> names;
-    for (uint32_t c = 0; c < in.table->ncols; ++c)
-      names.push_back(in.column_names && in.column_names[c] ? in.column_names[c] : "_col" + std::to_string(c));
+    for (uint32_t c = 0; c < in.table->ncols; ++c) {
+      if (in.column_names && in.column_names[c]) names.push_back(in.column_names[c]);
+      else if (!in.column_names && positional.size() == in.table->ncols) names.push_back(positional[c]);
+      else names.push_back("_col" + std::to_string(c));
+    }
     t.rows = in.table->row_num;
     const std::vector<std::string> &order = want.empty() ? names : want;
     for (const std::string &n : order) {
@@ -848,11 +856,12 @@ struct Executor {
   int run(int node, ExecTable &out) {
     const Op &o = P.ops[node];
     if ((o.kind == "ReduceSink" || o.kind == "FileSink" || o.kind == "Union") && o.vertex != vertex) {
-      const int st = from_input(o.vertex, {}, out);  // the result of another vertex
+      const int st = from_input(o.vertex, "", {}, o.outputs, out);  // the result of another vertex
       if (o.kind == "ReduceSink") out.reduce_keys = (int)o.key_expressions.size();
       return st;
     }
-    if (o.kind == "TableScan") return from_input(o.table, o.outputs, out);
+    // a table scan reads "db@table", or -- as GTables names it -- this vertex's own input
+    if (o.kind == "TableScan") return from_input(o.table, vertex, o.outputs, o.outputs, out);
     if (o.children.empty()) return fail(GHIVE_ERR_INVALID, "plan: " + o.id + " has no input");
     if (o.kind == "MapJoin" || o.kind == "MergeJoin") return join(o, out);
     ExecTable in;

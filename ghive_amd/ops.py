@@ -702,15 +702,18 @@ class Plan:
 
     def execute(self, ctx, vertex, inputs):
         """Operator::execute over one vertex (ghive_plan_execute). inputs: {name: (Table, column
-        names)} -- a base table by its TableScan name ("db@table") or another vertex's result by the
-        vertex name. Returns (column names, [device tensors]); the tensors are copies owned by torch."""
+        names or None)} -- a base table by its TableScan name ("db@table") or by the vertex run, or
+        another vertex's result by that vertex's name; None = positional columns. Returns (column
+        names, [device tensors / StringColumns]); the columns are copies owned by torch."""
         keep, arr = [], (abi.PlanInput * max(len(inputs), 1))()
         for i, (name, (table, names)) in enumerate(inputs.items()):
-            cn = (C.c_char_p * max(len(names), 1))(*[n.encode() for n in names])
-            keep += [table, cn]
+            keep.append(table)
             arr[i].name = name.encode()
             arr[i].table = C.pointer(table.table)
-            arr[i].column_names = C.cast(cn, C.POINTER(C.c_char_p))
+            if names is not None:  # else positional, as GTables delivers the columns
+                cn = (C.c_char_p * max(len(names), 1))(*[n.encode() for n in names])
+                keep.append(cn)
+                arr[i].column_names = C.cast(cn, C.POINTER(C.c_char_p))
         err = C.create_string_buffer(512)
         r = C.c_void_p()
         st = lib().ghive_plan_execute(ctx.h, self.h, vertex.encode(), arr, len(inputs), err, len(err), C.byref(r))

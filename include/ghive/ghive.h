@@ -510,9 +510,11 @@ int ghive_plan_parse(const char *plan, const char *extended, const char *maintai
 int ghive_plan_describe(const ghive_plan *plan, char *buf, uint64_t cap, uint64_t *len);
 void ghive_plan_free(ghive_plan *plan);
 
-/* One input of a vertex: a base table named as the TableScan names it ("db@table"), or the result
- * of another vertex named by that vertex ("Map 3"), with its column names (NULL: _col0.._colN-1).
- * A reduce-side input lists its key columns first, as the sink that produced it. */
+/* One input of a vertex: a base table named as the TableScan names it ("db@table") or by the
+ * vertex being run (GTables names the scanned table so, ghive_JNIInterface.cu:200-204), or the
+ * result of another vertex named by that vertex ("Map 3"). column_names NULL: the columns are
+ * positional, in the order GTables delivers them -- the TableScan's output columns, or the
+ * producing sink's outputs with its key columns first. */
 typedef struct ghive_plan_input {
   const char *name;
   const ghive_table *table; /* device columns */

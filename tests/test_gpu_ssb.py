@@ -99,8 +99,10 @@ def _dev_table(cols):
     return ops.Table(dev), names
 
 
-def _execute(ctx, q, tables):
-    """Runs every vertex of the plan once its input vertices have run; returns the root's rows."""
+def _execute(ctx, q, tables, gtables=False):
+    """Runs every vertex of the plan once its input vertices have run; returns the root's rows.
+    gtables: bind inputs as the JNI GTables delivers them -- the scanned table under the vertex's
+    own name with the TableScan's columns in order, every input without column names."""
     from ghive_amd import ops
     with open(os.path.join(PLANS, f"ssb_2_orc_{q}.txt")) as fh:
         text = fh.read()
@@ -121,10 +123,16 @@ def _execute(ctx, q, tables):
         ready = [v for v in deps if v not in done and deps[v] <= done.keys()]
         assert ready, "cyclic vertex dependencies"
         for v in ready:
-            inputs = dict(base)
+            inputs = {} if gtables else dict(base)
+            if gtables:
+                for o in opsd.values():
+                    if o["kind"] == "TableScan" and o["vertex"] == v:
+                        cols = tables[o["table"][len(DB):]]
+                        dev, _ = _dev_table({c: cols[c] for c in o["outputs"]})
+                        inputs[v] = (dev, None)
             for dv in deps[v]:
                 names, cols = done[dv]
-                inputs[dv] = (ops.Table(cols), names)
+                inputs[dv] = (ops.Table(cols), None if gtables else names)
             done[v] = plan.execute(ctx, v, inputs)
     root_vertex = opsd[d["root"]]["vertex"]
     names, cols = done[root_vertex]
@@ -272,3 +280,10 @@ def test_ssb_query_end_to_end(ctx, ssb_tables, q):
     if q.startswith("3_") or q.startswith("4_") or q.startswith("2_"):
         assert len(want) >= 2, q  # the synthetic data keeps several groups in every query
     assert [tuple(r) for r in got] == want, q
+
+
+@pytest.mark.parametrize("q", ["1_3", "2_3", "3_4", "4_1"])
+def test_ssb_query_with_gtables_binding(ctx, ssb_tables, q):
+    """The same queries with inputs bound as GTables delivers them: positional columns, the
+    scanned table under the vertex's own name."""
+    assert [tuple(r) for r in _execute(ctx, q, ssb_tables, gtables=True)] == _expected(q, ssb_tables)
