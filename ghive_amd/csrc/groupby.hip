@@ -37,7 +37,41 @@ __device__ __forceinline__ uint64_t pack_key(const KeySpec &K, uint64_t r) {
   return key;
 }
 
-__device__ __forceinline__ AggVal agg_value(const AggSpec &A, uint64_t r) {
+// Raw bits of row r of a column (4-byte types zero-extended).
+__device__ __forceinline__ uint64_t col_bits(const void *p, int type, uint64_t r) {
+  return (type == GHIVE_LONG || type == GHIVE_DOUBLE) ? static_cast<const uint64_t *>(p)[r]
+                                                      : (uint64_t) static_cast<const uint32_t *>(p)[r];
+}
+
+// A value from its raw bits, as col_load_i64 / col_load_f64 read it; false when NULL.
+__device__ __forceinline__ bool bits_i64(uint64_t bits, int type, int64_t &v) {
+  if (type == GHIVE_INT) {
+    const int32_t x = (int32_t)(uint32_t)bits;
+    v = x;
+    return x != 0x7fffffff;
+  }
+  v = (int64_t)bits;
+  return v != 0x7fffffffffffffffll;
+}
+__device__ __forceinline__ bool bits_f64(uint64_t bits, int type, double &v) {
+  switch (type) {
+    case GHIVE_DOUBLE: v = __longlong_as_double((long long)bits); return !__builtin_isnan(v);
+    case GHIVE_FLOAT: v = (double)__uint_as_float((uint32_t)bits); return !__builtin_isnan(v);
+    case GHIVE_INT: {
+      const int32_t x = (int32_t)(uint32_t)bits;
+      v = (double)x;
+      return x != 0x7fffffff;
+    }
+    default: {
+      const int64_t x = (int64_t)bits;
+      v = (double)x;
+      return x != 0x7fffffffffffffffll;
+    }
+  }
+}
+
+// One row's contribution to aggregate A, from the raw bits of its input column.
+__device__ __forceinline__ AggVal agg_value_bits(const AggSpec &A, uint64_t bits) {
   AggVal v;
   v.valid = false;
   v.neg = false;
@@ -49,32 +83,36 @@ __device__ __forceinline__ AggVal agg_value(const AggSpec &A, uint64_t r) {
   switch (A.kind) {
     case AK_CNT: {
       double d;
-      v.valid = col_load_f64(A.data, A.in_type, r, d);
+      v.valid = bits_f64(bits, A.in_type, d);
       break;
     }
     case AK_SUMI:
     case AK_MINI:
     case AK_MAXI: {
       int64_t x;
-      v.valid = col_load_i64(A.data, A.in_type, r, x);
+      v.valid = bits_i64(bits, A.in_type, x);
       v.a = (uint64_t)x;
       break;
     }
     case AK_MIND:
     case AK_MAXD: {
       double d;
-      v.valid = col_load_f64(A.data, A.in_type, r, d);
+      v.valid = bits_f64(bits, A.in_type, d);
       v.a = dbl_order(d);
       break;
     }
     default: {
       double d;
-      v.valid = col_load_f64(A.data, A.in_type, r, d);
+      v.valid = bits_f64(bits, A.in_type, d);
       if (v.valid) v.neg = to_fixed(d, A.scale_e, v.a, v.b);
       break;
     }
   }
   return v;
+}
+
+__device__ __forceinline__ AggVal agg_value(const AggSpec &A, uint64_t r) {
+  return agg_value_bits(A, A.col < 0 ? 0 : col_bits(A.data, A.in_type, r));
 }
 
 // A workgroup aggregates into its LDS table first and merges it into the global table at the end.
@@ -145,14 +183,25 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
   return x ^ (x >> 31);
 }
 
+// Inverse of mix64: the key comes back from its hash, so the passes need not carry it.
+__device__ __forceinline__ uint64_t unmix64(uint64_t x) {
+  x ^= (x >> 31) ^ (x >> 62);
+  x *= 0x319642b2d24d8ec3ull;  // inverse of 0x94d049bb133111eb mod 2^64
+  x ^= (x >> 27) ^ (x >> 54);
+  x *= 0x96de1b173f119089ull;  // inverse of 0xbf58476d1ce4e5b9 mod 2^64
+  return x ^ (x >> 30) ^ (x >> 60);
+}
+
+// h[i] = mix64(key of row r); v[i] = the raw bits of the carried value column (when vdata).
 __global__ __launch_bounds__(256) void k_gb_part_keys(KeySpec K, const uint32_t *sel, uint64_t n, uint64_t *h,
-                                                      uint32_t *ids) {
+                                                      uint32_t *ids, const void *vdata, int vtype, uint64_t *v) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) {
     const uint64_t r = sel ? sel[i] : i;
     h[i] = mix64(pack_key(K, r));
     ids[i] = (uint32_t)r;
+    if (vdata) v[i] = col_bits(vdata, vtype, r);
   }
 }
 
@@ -167,82 +216,6 @@ __global__ void k_gb_part_bounds(const uint64_t *h, uint64_t n, int pbits, uint6
     for (int64_t q = prev + 1; q <= (int64_t)p; ++q) starts[q] = i;  // empty partitions start here too
     if (i == n - 1)
       for (uint64_t q = p + 1; q <= P; ++q) starts[q] = n;
-  }
-}
-
-// One workgroup per partition. The LDS table has lcap slots (unbounded linear probing); a
-// partition with more groups than 7/8 of that sets *fail and the caller uses the direct form.
-__global__ __launch_bounds__(kGbPartThreads) void k_gb_part_agg(KeySpec K, GroupTableDev T, const uint64_t *starts,
-                                                                const uint32_t *ids, int lcap_log2,
-                                                                unsigned long long *fail) {
-  extern __shared__ uint64_t lds[];
-  __shared__ uint32_t used, full;
-  const int lcap = 1 << lcap_log2, lstride = lcap + 1;
-  lds_table_init(lds, T, lcap_log2);
-  if (threadIdx.x == 0) {
-    used = 0;
-    full = 0;
-  }
-  __syncthreads();
-  const uint64_t b = starts[blockIdx.x], e = starts[blockIdx.x + 1];
-  const uint32_t limit = (uint32_t)lcap - (uint32_t)lcap / 8;
-  for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
-    if (full) break;
-    const uint64_t r = ids[i];
-    const uint64_t key = pack_key(K, r);
-    if (key == kEmptyKey) {  // the one key equal to the EMPTY pattern lives in the table's extra slot
-      const int64_t g = gt_find_insert(T, key);
-      for (int a = 0; a < T.naggs; ++a) {
-        const AggSpec &A = T.aggs[a];
-        AggVal v = agg_value(A, r);
-        acc_apply(A, [&](int j) { return acc_word(T, A.word0 + j, (uint64_t)g); }, v);
-      }
-      continue;
-    }
-    uint32_t s = (uint32_t)hash_u64(key, lcap_log2);
-    int slot = -1;
-    for (int step = 0; step < lcap; ++step) {
-      const uint64_t k = __atomic_load_n(lds + s, __ATOMIC_RELAXED);
-      if (k == key) {
-        slot = (int)s;
-        break;
-      }
-      if (k == kEmptyKey) {
-        const unsigned long long old = atomicCAS((unsigned long long *)(lds + s), kEmptyKey, key);
-        if (old == kEmptyKey) {
-          if (atomicAdd(&used, 1u) + 1u > limit) full = 1;
-          slot = (int)s;
-          break;
-        }
-        if (old == key) {
-          slot = (int)s;
-          break;
-        }
-      }
-      s = (s + 1) & (uint32_t)(lcap - 1);
-    }
-    if (slot < 0 || full) {
-      full = 1;
-      break;
-    }
-    for (int a = 0; a < T.naggs; ++a) {
-      const AggSpec &A = T.aggs[a];
-      AggVal v = agg_value(A, r);
-      acc_apply(A, [&](int j) { return lds + (size_t)(1 + A.word0 + j) * lstride + slot; }, v);
-    }
-  }
-  __syncthreads();
-  if (full) {
-    if (threadIdx.x == 0) atomicOr(fail, 1ull);
-    return;
-  }
-  // the partition's own region of the global table: plain stores
-  const uint64_t base = (uint64_t)blockIdx.x << lcap_log2;
-  for (int s = threadIdx.x; s < lcap; s += blockDim.x) {
-    const uint64_t k = lds[s];
-    T.keys[base + s] = k;
-    if (k == kEmptyKey) continue;
-    for (int w = 0; w < T.nwords; ++w) *acc_word(T, w, base + s) = lds[(size_t)(1 + w) * lstride + s];
   }
 }
 
@@ -439,9 +412,9 @@ __device__ __forceinline__ void write_null(void *col, int type, uint64_t idx) {
   }
 }
 
-__device__ __forceinline__ void finalize_agg(const AggSpec &A, const GroupTableDev &T, uint64_t s, void *col,
-                                             uint64_t idx) {
-  auto W = [&](int j) { return *acc_word(T, A.word0 + j, s); };
+// Output value of aggregate A from its accumulator words W(j), at row idx of col.
+template <class Words>
+__device__ __forceinline__ void finalize_words(const AggSpec &A, Words W, void *col, uint64_t idx) {
   switch (A.kind) {
     case AK_CNT: static_cast<int64_t *>(col)[idx] = (int64_t)W(0); return;
     case AK_SUMI: {
@@ -485,6 +458,24 @@ __device__ __forceinline__ void finalize_agg(const AggSpec &A, const GroupTableD
   }
 }
 
+__device__ __forceinline__ void finalize_agg(const AggSpec &A, const GroupTableDev &T, uint64_t s, void *col,
+                                             uint64_t idx) {
+  finalize_words(A, [&](int j) { return *acc_word(T, A.word0 + j, s); }, col, idx);
+}
+
+// The key columns of a packed key at output row idx.
+__device__ __forceinline__ void emit_key(const KeySpec &K, const OutSpec &O, uint64_t key, uint64_t idx) {
+  for (int k = 0; k < K.nkeys; ++k) {
+    const uint64_t bits = key >> K.shift[k];
+    switch (K.type[k]) {
+      case GHIVE_INT: static_cast<int32_t *>(O.col[k])[idx] = (int32_t)(uint32_t)bits; break;
+      case GHIVE_LONG: static_cast<int64_t *>(O.col[k])[idx] = (int64_t)bits; break;
+      case GHIVE_DOUBLE: static_cast<uint64_t *>(O.col[k])[idx] = bits; break;
+      default: static_cast<uint32_t *>(O.col[k])[idx] = (uint32_t)bits; break;
+    }
+  }
+}
+
 __global__ void k_groupby_finalize(KeySpec K, GroupTableDev T, OutSpec O, unsigned long long *counter, uint64_t cap) {
   const uint64_t total = T.stride;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -500,16 +491,103 @@ __global__ void k_groupby_finalize(KeySpec K, GroupTableDev T, OutSpec O, unsign
     }
     uint64_t idx = wave_append(counter, used);
     if (!used || idx >= cap) continue;
-    for (int k = 0; k < K.nkeys; ++k) {
-      uint64_t bits = key >> K.shift[k];
-      switch (K.type[k]) {
-        case GHIVE_INT: static_cast<int32_t *>(O.col[k])[idx] = (int32_t)(uint32_t)bits; break;
-        case GHIVE_LONG: static_cast<int64_t *>(O.col[k])[idx] = (int64_t)bits; break;
-        case GHIVE_DOUBLE: static_cast<uint64_t *>(O.col[k])[idx] = bits; break;
-        default: static_cast<uint32_t *>(O.col[k])[idx] = (uint32_t)bits; break;
-      }
-    }
+    emit_key(K, O, key, idx);
     for (int a = 0; a < T.naggs; ++a) finalize_agg(T.aggs[a], T, s, O.col[K.nkeys + a], idx);
+  }
+}
+
+// ---- high-cardinality form, aggregation: one workgroup per hash partition --------------------
+// The rows of partition p are [starts[p], starts[p+1]) of the sorted pairs: the key is unmix64 of
+// the carried hash, the aggregate input the carried value bits (VAL 1), the input columns at the
+// row id (VAL 2), or nothing (VAL 0: COUNT(*) only). The partition's groups are aggregated in an
+// LDS table of lcap slots (unbounded linear probing); more than 7/8 of that sets *fail and the
+// caller uses the direct form. Then the workgroup reserves its range of output rows with one
+// atomic and writes its groups there: no global table is built. The one key equal to the EMPTY
+// pattern is aggregated in the small global table T instead, emitted afterwards.
+template <int VAL>
+__global__ __launch_bounds__(kGbPartThreads) void k_gb_part_emit(KeySpec K, GroupTableDev T, const uint64_t *starts,
+                                                                 const uint64_t *hs, const uint64_t *vs,
+                                                                 const uint32_t *ids, int lcap_log2,
+                                                                 unsigned long long *fail,
+                                                                 unsigned long long *out_count, OutSpec O,
+                                                                 uint64_t cap) {
+  extern __shared__ uint64_t lds[];
+  __shared__ uint32_t used, full, wpos;
+  __shared__ unsigned long long base;
+  const int lcap = 1 << lcap_log2, lstride = lcap + 1;
+  lds_table_init(lds, T, lcap_log2);
+  if (threadIdx.x == 0) {
+    used = 0;
+    full = 0;
+    wpos = 0;
+  }
+  __syncthreads();
+  const uint64_t b = starts[blockIdx.x], e = starts[blockIdx.x + 1];
+  const uint32_t limit = (uint32_t)lcap - (uint32_t)lcap / 8;
+  for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+    if (full) break;
+    const uint64_t key = unmix64(hs[i]);
+    const uint64_t vbits = VAL == 1 ? vs[i] : 0;
+    const uint64_t r = VAL == 2 ? ids[i] : 0;
+    if (key == kEmptyKey) {
+      const int64_t g = gt_find_insert(T, key);
+      if (g < 0) continue;
+      for (int a = 0; a < T.naggs; ++a) {
+        const AggSpec &A = T.aggs[a];
+        const AggVal v = VAL == 2 ? agg_value(A, r) : agg_value_bits(A, vbits);
+        acc_apply(A, [&](int j) { return acc_word(T, A.word0 + j, (uint64_t)g); }, v);
+      }
+      continue;
+    }
+    uint32_t s = (uint32_t)hash_u64(key, lcap_log2);
+    int slot = -1;
+    for (int step = 0; step < lcap; ++step) {
+      const uint64_t k = __atomic_load_n(lds + s, __ATOMIC_RELAXED);
+      if (k == key) {
+        slot = (int)s;
+        break;
+      }
+      if (k == kEmptyKey) {
+        const unsigned long long old = atomicCAS((unsigned long long *)(lds + s), kEmptyKey, key);
+        if (old == kEmptyKey) {
+          if (atomicAdd(&used, 1u) + 1u > limit) full = 1;
+          slot = (int)s;
+          break;
+        }
+        if (old == key) {
+          slot = (int)s;
+          break;
+        }
+      }
+      s = (s + 1) & (uint32_t)(lcap - 1);
+    }
+    if (slot < 0 || full) {
+      full = 1;
+      break;
+    }
+    for (int a = 0; a < T.naggs; ++a) {
+      const AggSpec &A = T.aggs[a];
+      const AggVal v = VAL == 2 ? agg_value(A, r) : agg_value_bits(A, vbits);
+      acc_apply(A, [&](int j) { return lds + (size_t)(1 + A.word0 + j) * lstride + slot; }, v);
+    }
+  }
+  __syncthreads();
+  if (full) {
+    if (threadIdx.x == 0) atomicOr(fail, 1ull);
+    return;
+  }
+  if (threadIdx.x == 0) base = atomicAdd(out_count, (unsigned long long)used);
+  __syncthreads();
+  for (int s = threadIdx.x; s < lcap; s += blockDim.x) {
+    const uint64_t k = lds[s];
+    if (k == kEmptyKey) continue;
+    const uint64_t idx = base + atomicAdd(&wpos, 1u);
+    if (idx >= cap) continue;
+    emit_key(K, O, k, idx);
+    for (int a = 0; a < T.naggs; ++a) {
+      const AggSpec &A = T.aggs[a];
+      finalize_words(A, [&](int j) { return lds[(size_t)(1 + A.word0 + j) * lstride + s]; }, O.col[K.nkeys + a], idx);
+    }
   }
 }
 
@@ -549,44 +627,106 @@ int group_table_alloc(ghive_ctx *ctx, GroupTableDev &T, uint64_t capacity, DevBu
   return GHIVE_OK;
 }
 
-// Runs the high-cardinality form into T (P = 2^pbits regions of 2^lg slots, initialised).
-// *failed: a partition held more groups than its LDS table; T is then unusable.
-int groupby_partitioned(ghive_ctx *ctx, const KeySpec &K, const GroupTableDev &T, const uint32_t *sel, uint64_t n,
-                        int lg, int pbits, bool *failed) {
+// Runs the high-cardinality form: 2^pbits hash partitions, each aggregated in an LDS table of 2^lg
+// slots and written straight to the output columns O (rows in unspecified order). T carries the
+// aggregate specs. *failed: a partition held more groups than its LDS table (nothing usable was
+// written; use the direct form). *groups = groups found (CAPACITY when above capacity).
+int groupby_partitioned(ghive_ctx *ctx, const KeySpec &K, const GroupTableDev &Tspec, const uint32_t *sel,
+                        uint64_t n, int lg, int pbits, const OutSpec &O, uint64_t capacity, bool *failed,
+                        uint64_t *groups) {
   *failed = false;
-  DevBuf h0, h1, i0, i1, starts;
-  GH_TRY(h0.alloc(ctx, 8 * n));
-  GH_TRY(h1.alloc(ctx, 8 * n));
+  *groups = 0;
+  // the aggregate inputs: none (COUNT(*) only), one column carried through the passes, or several
+  // (read at the row id)
+  int vcol = -1, val = 0;
+  for (int a = 0; a < Tspec.naggs; ++a) {
+    const int c = Tspec.aggs[a].col;
+    if (c < 0) continue;
+    if (vcol < 0) {
+      vcol = c;
+      val = 1;
+    } else if (c != vcol) {
+      val = 2;
+    }
+  }
+  const void *vdata = nullptr;
+  int vtype = 0;
+  for (int a = 0; a < Tspec.naggs && val == 1; ++a)
+    if (Tspec.aggs[a].col == vcol) {
+      vdata = Tspec.aggs[a].data;
+      vtype = Tspec.aggs[a].in_type;
+    }
+  const int KW = val == 1 ? 2 : 1;
+  DevBuf k0, k1, i0, i1, starts, sk, sa, sc;
+  GH_TRY(k0.alloc(ctx, 8ull * KW * n));
+  GH_TRY(k1.alloc(ctx, 8ull * KW * n));
   GH_TRY(i0.alloc(ctx, 4 * n));
   GH_TRY(i1.alloc(ctx, 4 * n));
   GH_TRY(starts.alloc(ctx, 8 * ((1ull << pbits) + 1)));
   {
     const uint64_t g = (n + 255) / 256, cap = (uint64_t)ctx->cus * 16;
-    k_gb_part_keys<<<(uint32_t)(g > cap ? cap : g), 256, 0, ctx->stream>>>(K, sel, n, h0.as<uint64_t>(),
-                                                                            i0.as<uint32_t>());
+    k_gb_part_keys<<<(uint32_t)(g > cap ? cap : g), 256, 0, ctx->stream>>>(
+        K, sel, n, k0.as<uint64_t>(), i0.as<uint32_t>(), vdata, vtype, val == 1 ? k0.as<uint64_t>() + n : nullptr);
     GH_LAUNCH(ctx);
   }
-  std::vector<int> digits;  // the 8-bit digits that cover the top pbits bits, least significant first
+  std::vector<int> digits;  // the 8-bit digits of the hash (word 0) that cover its top pbits bits
   for (int d = pbits ? (64 - pbits) / 8 : 8; d < 8; ++d) digits.push_back(d);
-  uint64_t *hs = h0.as<uint64_t>();
+  uint64_t *ks = k0.as<uint64_t>();
   uint32_t *ids = i0.as<uint32_t>();
-  GH_TRY(radix_sort_u64_ids(ctx, n, h0.as<uint64_t>(), h1.as<uint64_t>(), i0.as<uint32_t>(), i1.as<uint32_t>(),
-                            digits, &hs, &ids));
+  if (KW == 2)
+    GH_TRY(radix_sort_u64x2_ids(ctx, n, k0.as<uint64_t>(), k1.as<uint64_t>(), i0.as<uint32_t>(), i1.as<uint32_t>(),
+                                digits, &ks, &ids));
+  else
+    GH_TRY(radix_sort_u64_ids(ctx, n, k0.as<uint64_t>(), k1.as<uint64_t>(), i0.as<uint32_t>(), i1.as<uint32_t>(),
+                              digits, &ks, &ids));
   {
     const uint64_t g = (n + 255) / 256, cap = (uint64_t)ctx->cus * 16;
-    k_gb_part_bounds<<<(uint32_t)(g > cap ? cap : g), 256, 0, ctx->stream>>>(hs, n, pbits, starts.as<uint64_t>());
+    k_gb_part_bounds<<<(uint32_t)(g > cap ? cap : g), 256, 0, ctx->stream>>>(ks, n, pbits, starts.as<uint64_t>());
     GH_LAUNCH(ctx);
   }
+  // the small global table that takes the one key equal to the EMPTY pattern
+  GroupTableDev T = Tspec;
+  GH_TRY(group_table_alloc(ctx, T, 1, sk, sa, sc));
   const size_t lds_bytes = (size_t)((1 << lg) + 1) * (1 + T.nwords) * 8;
-  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_gb_part_agg), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds_bytes);
-  GH_TRY(fill_u64(ctx, ctx->d_counters, 0, 1));
-  k_gb_part_agg<<<(uint32_t)(1ull << pbits), kGbPartThreads, lds_bytes, ctx->stream>>>(
-      K, T, starts.as<uint64_t>(), ids, lg, reinterpret_cast<unsigned long long *>(ctx->d_counters));
+  unsigned long long *fail = reinterpret_cast<unsigned long long *>(ctx->d_counters);
+  unsigned long long *count = fail + 1;
+  GH_TRY(fill_u64(ctx, ctx->d_counters, 0, 2));
+  const uint32_t P = (uint32_t)(1ull << pbits);
+  const uint64_t *vs = KW == 2 ? ks + n : nullptr;
+  switch (val) {
+    case 0:
+      (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_gb_part_emit<0>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+      k_gb_part_emit<0><<<P, kGbPartThreads, lds_bytes, ctx->stream>>>(K, T, starts.as<uint64_t>(), ks, vs, ids, lg,
+                                                                       fail, count, O, capacity);
+      break;
+    case 1:
+      (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_gb_part_emit<1>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+      k_gb_part_emit<1><<<P, kGbPartThreads, lds_bytes, ctx->stream>>>(K, T, starts.as<uint64_t>(), ks, vs, ids, lg,
+                                                                       fail, count, O, capacity);
+      break;
+    default:
+      (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_gb_part_emit<2>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+      k_gb_part_emit<2><<<P, kGbPartThreads, lds_bytes, ctx->stream>>>(K, T, starts.as<uint64_t>(), ks, vs, ids, lg,
+                                                                       fail, count, O, capacity);
+      break;
+  }
   GH_LAUNCH(ctx);
   uint64_t f = 0;
-  GH_TRY(d2h(ctx, &f, ctx->d_counters, sizeof f));
-  *failed = f != 0;
+  GH_TRY(d2h(ctx, &f, fail, sizeof f));
+  if (f) {
+    *failed = true;
+    return GHIVE_OK;
+  }
+  // the EMPTY-pattern key's group, if any, after the partitions' groups (same counter)
+  k_groupby_finalize<<<1, 256, 0, ctx->stream>>>(K, T, O, count, capacity);
+  GH_LAUNCH(ctx);
+  GH_TRY(d2h(ctx, groups, count, sizeof(uint64_t)));
+  if (*groups > capacity)
+    return set_error(ctx, GHIVE_ERR_CAPACITY, "group-by: more than %llu groups (capacity)",
+                     (unsigned long long)capacity);
   return GHIVE_OK;
 }
 
@@ -760,40 +900,6 @@ int ghive_groupby(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel,
     K.shift[0] = 0;
     K.data[0] = wcodes.as<uint32_t>();
   }
-  DevBuf dkeys, dacc, dcnt;
-  // the high-cardinality form: a large group bound over many rows (or forced by the option)
-  const int64_t part_opt = ctx->tune.gb_partition;
-  bool partitioned = false;
-  if (nkeys > 0 && n && part_opt != 0 &&
-      (part_opt == 2 || (n >= (1ull << 22) && capacity >= (1ull << 20)))) {
-    int lg = 11;  // LDS table slots per partition
-    while (lg > 8 && (size_t)((1 << lg) + 1) * (1 + T.nwords) * 8 > 150 * 1024) --lg;
-    int pbits = 0;  // partitions so that the group bound leaves each table at most half full
-    while (pbits < 24 && (capacity << 1) > (1ull << (pbits + lg))) ++pbits;
-    GH_TRY(group_table_alloc(ctx, T, (1ull << (pbits + lg)) / 2, dkeys, dacc, dcnt));
-    T.capacity = capacity;
-    bool failed = false;
-    GH_TRY(groupby_partitioned(ctx, K, T, sel, n, lg, pbits, &failed));
-    if (!failed) {
-      partitioned = true;
-    } else {
-      dkeys.reset();
-      dacc.reset();
-      dcnt.reset();
-    }
-  }
-  if (!partitioned) GH_TRY(group_table_alloc(ctx, T, capacity, dkeys, dacc, dcnt));
-  if (nkeys == 0) GH_TRY(group_table_preinsert(ctx, T, 0ull));  // a global aggregate: one group, even over no rows
-  if (n && !partitioned) {
-    int lg = lds_log2_for(T.nwords, (int)ctx->tune.gb_lds_budget);
-    size_t lds_bytes = (size_t)((1 << lg) + 1) * (1 + T.nwords) * 8;
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_groupby, kGbThreads, lds_bytes) != hipSuccess || per_cu < 1)
-      per_cu = 1;
-    uint64_t want = (n + kGbThreads - 1) / kGbThreads, cap = (uint64_t)ctx->cus * per_cu;
-    k_groupby<<<(uint32_t)(want > cap ? cap : want), kGbThreads, lds_bytes, ctx->stream>>>(K, T, sel, n, lg);
-    GH_LAUNCH(ctx);
-  }
   OutSpec O;
   memset(&O, 0, sizeof O);
   if (wide) {
@@ -809,7 +915,35 @@ int ghive_groupby(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel,
       O.type[c] = out_cols[c].type;
     }
   }
-  int st = group_table_emit(ctx, K, T, O, capacity, out_groups);
+  DevBuf dkeys, dacc, dcnt;
+  // the high-cardinality form: a large group bound over many rows (or forced by the option)
+  const int64_t part_opt = ctx->tune.gb_partition;
+  bool partitioned = false;
+  int st = GHIVE_OK;
+  if (nkeys > 0 && n && part_opt != 0 &&
+      (part_opt == 2 || (n >= (1ull << 22) && capacity >= (1ull << 20)))) {
+    int lg = 11;  // LDS table slots per partition
+    while (lg > 8 && (size_t)((1 << lg) + 1) * (1 + T.nwords) * 8 > 150 * 1024) --lg;
+    int pbits = 0;  // partitions so that the group bound leaves each table at most half full
+    while (pbits < 24 && (capacity << 1) > (1ull << (pbits + lg))) ++pbits;
+    bool failed = false;
+    st = groupby_partitioned(ctx, K, T, sel, n, lg, pbits, O, capacity, &failed, out_groups);
+    if (st != GHIVE_OK && st != GHIVE_ERR_CAPACITY) return st;
+    partitioned = !failed;
+  }
+  if (!partitioned) GH_TRY(group_table_alloc(ctx, T, capacity, dkeys, dacc, dcnt));
+  if (nkeys == 0) GH_TRY(group_table_preinsert(ctx, T, 0ull));  // a global aggregate: one group, even over no rows
+  if (n && !partitioned) {
+    int lg = lds_log2_for(T.nwords, (int)ctx->tune.gb_lds_budget);
+    size_t lds_bytes = (size_t)((1 << lg) + 1) * (1 + T.nwords) * 8;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_groupby, kGbThreads, lds_bytes) != hipSuccess || per_cu < 1)
+      per_cu = 1;
+    uint64_t want = (n + kGbThreads - 1) / kGbThreads, cap = (uint64_t)ctx->cus * per_cu;
+    k_groupby<<<(uint32_t)(want > cap ? cap : want), kGbThreads, lds_bytes, ctx->stream>>>(K, T, sel, n, lg);
+    GH_LAUNCH(ctx);
+  }
+  if (!partitioned) st = group_table_emit(ctx, K, T, O, capacity, out_groups);
   if (st == GHIVE_OK && wide && *out_groups) {
     WideOut WO;
     memset(&WO, 0, sizeof WO);

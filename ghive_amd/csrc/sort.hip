@@ -236,6 +236,11 @@ int radix_sort_u64_ids(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, u
   return radix_passes<1>(ctx, n, k0, k1, i0, i1, digits, ids_out, keys_out);
 }
 
+int radix_sort_u64x2_ids(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, uint32_t *i0, uint32_t *i1,
+                         const std::vector<int> &digits, uint64_t **keys_out, uint32_t **ids_out) {
+  return radix_passes<2>(ctx, n, k0, k1, i0, i1, digits, ids_out, keys_out);
+}
+
 // ---- STRING keys: dense ranks in Hive's string order -------------------------------------------
 // A STRING key sorts by the rank of its value among the column's distinct values. The key
 // dictionary of the group-by names each distinct value once (a representative row); the distinct

@@ -255,6 +255,39 @@ def test_groupby_wide_keys_edges(ctx, orc):
     _compare(_as_dict(outs, 2), _oracle_dict(orc, cols, [0, 1], [(abi.AGG_CNT, abi.COUNT_STAR), (abi.AGG_SUM, 2)]))
 
 
+@pytest.mark.parametrize("aggset", ["count_star", "one_column", "two_columns"])
+def test_groupby_partitioned_value_forms(ctx, orc, aggset):
+    """The high-cardinality form reads aggregate inputs three ways: none (COUNT(*) only), one column
+    carried through the partition passes, several columns read at the row id. LONG keys include -1,
+    whose bits equal the EMPTY pattern (aggregated in the small global table), and NULL."""
+    rng = np.random.default_rng(len(aggset))
+    n = 500_000
+    k = rng.integers(-1000, 60_000, n).astype(np.int64)
+    k[::97] = -1
+    k[::89] = LONG_NULL
+    x = rng.standard_normal(n) * 10.0 ** rng.integers(-3, 4, n)
+    x[::41] = np.nan
+    y = rng.integers(-10**12, 10**12, n).astype(np.int64)
+    aggs = {"count_star": [(abi.AGG_CNT, abi.COUNT_STAR)],
+            "one_column": [(abi.AGG_CNT, abi.COUNT_STAR), (abi.AGG_SUM, 1), (abi.AGG_MIN, 1), (abi.AGG_AVG, 1),
+                           (abi.AGG_CNT, 1)],
+            "two_columns": [(abi.AGG_SUM, 1), (abi.AGG_MAX, 2), (abi.AGG_CNT, abi.COUNT_STAR)]}[aggset]
+    cols = [k, x, y]
+    ctx.set_option("groupby.partition", 2)
+    try:
+        outs = _run(ctx, cols, [0], aggs)
+        _compare(_as_dict(outs, 1), _oracle_dict(orc, cols, [0], aggs))
+        sel = np.nonzero(rng.random(n) < 0.5)[0].astype(np.uint32)
+        outs = _run(ctx, cols, [0], aggs, sel=sel)
+        _compare(_as_dict(outs, 1), _oracle_dict(orc, cols, [0], aggs, sel=sel))
+        from ghive_amd._lib import GhiveError
+        with pytest.raises(GhiveError) as e:  # more groups than the bound
+            _run(ctx, cols, [0], aggs, capacity=1000)
+        assert e.value.status == abi.ERR_CAPACITY
+    finally:
+        ctx.set_option("groupby.partition", 1)
+
+
 def test_groupby_with_selection_and_empty(ctx, orc):
     rng = np.random.default_rng(2)
     n = 100_000
