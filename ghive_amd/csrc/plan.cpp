@@ -838,6 +838,19 @@ struct Executor {
       out.char_size = lit.size();
       return GHIVE_OK;
     }
+    static const std::regex num_re(R"(^-?\d+(\.\d*)?([eE][-+]?\d+)?[DdLlSsYy]?$)");
+    if (std::regex_match(e, num_re)) {  // a numeric constant, on every row
+      ghive_column one;
+      int st = operand(e, t, one);
+      if (st != GHIVE_OK) return st;
+      if ((st = alloc_col(one.type, t.rows, out)) != GHIVE_OK) return st;
+      const size_t w = (one.type == GHIVE_LONG || one.type == GHIVE_DOUBLE) ? 8 : 4;
+      std::vector<uint8_t> v(w), host(w * t.rows);
+      if ((st = ghive_memcpy(ctx, v.data(), one.data, w, 1)) != GHIVE_OK) return op_fail(st);
+      for (uint64_t r = 0; r < t.rows; ++r) memcpy(host.data() + r * w, v.data(), w);
+      if (t.rows && (st = ghive_memcpy(ctx, out.data, host.data(), host.size(), 0)) != GHIVE_OK) return op_fail(st);
+      return GHIVE_OK;
+    }
     if (std::regex_match(e, m, val_re)) {  // values follow the keys of a reduce input
       const int c = std::stoi(m[1].str()) + t.reduce_keys;
       if (t.reduce_keys < 0 || c >= (int)t.cols.size())

@@ -1,10 +1,13 @@
-"""Every SSB query, Q1.1 to Q4.3, from the reference's plan fixtures for the non-transformed schema
-(STRING region / nation / city / category / brand attributes; tests/golden/plans/ssb_2_orc_*), run
-vertex by vertex through ghive_plan_execute on synthetic seeded tables, against the query's SQL
-restated in numpy. Each query's predicates, grouping and ordering are the ones its plan carries
-(Q1.3 and Q1.4 filter on d_yearmonthnum there). The join outputs are placed without a
-maintain-values map. Sums are compared exactly: one partial per group reaches the final group-by,
-so each result is the correctly rounded sum of the group's values."""
+"""Every SSB query, Q1.1 to Q4.3, from the reference's plan fixtures, run vertex by vertex through
+ghive_plan_execute on synthetic seeded tables, against the query's SQL restated in numpy. Two
+schemas: the non-transformed one (STRING region / nation / city / category / brand / yearmonth
+attributes; tests/golden/plans/ssb_2_orc_*) and the transformed one (the same attributes
+dictionary-encoded as BIGINT ranks of their sorted domains -- 'AMERICA' = 1, 'UNITED STATES' = 23,
+'UNITED KI1' = 221, 'MFGR#2239' = 272, 'Dec1997' = 19, as the plans' literals show;
+tests/golden/plans/ssb_10_orc_transformed_*). Each query's predicates, grouping and ordering are the
+ones its plan carries (the non-transformed Q1.3 and Q1.4 filter on d_yearmonthnum). The join outputs
+are placed without a maintain-values map. Sums are compared exactly: one partial per group reaches
+the final group-by, so each result is the correctly rounded sum of the group's values."""
 import math
 import os
 
@@ -17,7 +20,7 @@ from ghive_amd import abi
 pytestmark = pytest.mark.gpu
 LONG_NULL = abi.LONG_NULL
 PLANS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "plans")
-DB = "ssb_2_orc@"
+FORMS = {"strings": "ssb_2_orc", "codes": "ssb_10_orc_transformed"}
 
 NATIONS = {"AFRICA": ["ALGERIA", "ETHIOPIA", "KENYA", "MOROCCO", "MOZAMBIQUE"],
            "AMERICA": ["ARGENTINA", "BRAZIL", "CANADA", "PERU", "UNITED STATES"],
@@ -26,6 +29,17 @@ NATIONS = {"AFRICA": ["ALGERIA", "ETHIOPIA", "KENYA", "MOROCCO", "MOZAMBIQUE"],
            "MIDDLE EAST": ["EGYPT", "IRAN", "IRAQ", "JORDAN", "SAUDI ARABIA"]}
 ALL_NATIONS = [(r, n) for r, ns in NATIONS.items() for n in ns]
 MONTHS = ["Jan", "Feb", "Mar", "Apr", "May", "Jun", "Jul", "Aug", "Sep", "Oct", "Nov", "Dec"]
+# the sorted domains whose ranks are the transformed schema's codes
+DOMAINS = {
+    "region": sorted(NATIONS),
+    "nation": sorted(n for _, n in ALL_NATIONS),
+    "city": sorted(n[:9].ljust(9) + str(d) for _, n in ALL_NATIONS for d in range(10)),
+    "mfgr": sorted(f"MFGR#{m}" for m in range(1, 6)),
+    "category": sorted(f"MFGR#{m}{c}" for m in range(1, 6) for c in range(1, 6)),
+    "brand1": sorted(f"MFGR#{m}{c}{b}" for m in range(1, 6) for c in range(1, 6) for b in range(1, 41)),
+    "yearmonth": sorted(m + str(y) for m in MONTHS for y in range(1992, 1999)),
+}
+CODE = {k: {v: i for i, v in enumerate(vs)} for k, vs in DOMAINS.items()}
 
 
 def _geo(rng, n):
@@ -51,8 +65,9 @@ def _tables(seed=7, n=400_000):
     y = (days.astype("datetime64[Y]").astype(np.int64) + 1970)
     m = days.astype("datetime64[M]").astype(np.int64) % 12 + 1
     d = (days - days.astype("datetime64[M]")).astype(np.int64) + 1
+    doy = (days - days.astype("datetime64[Y]")).astype(np.int64)
     dates = dict(d_datekey=(y * 10000 + m * 100 + d).astype(np.int64), d_year=y.astype(np.int32),
-                 d_yearmonthnum=(y * 100 + m).astype(np.int32),
+                 d_yearmonthnum=(y * 100 + m).astype(np.int32), d_weeknuminyear=(doy // 7 + 1).astype(np.int32),
                  d_yearmonth=[MONTHS[mm - 1] + str(yy) for yy, mm in zip(y.tolist(), m.tolist())])
     S, Cn, P = 2000, 3000, 20000
     sg, cg = _geo(rng, S), _geo(rng, Cn)
@@ -62,9 +77,10 @@ def _tables(seed=7, n=400_000):
                     c_nation=[g[1] for g in cg], c_city=[g[2] for g in cg])
     mf = rng.integers(1, 6, P)
     cat = mf * 10 + rng.integers(1, 6, P)
-    brand = cat * 100 + rng.integers(1, 41, P)
+    bn = rng.integers(1, 41, P)
     part = dict(p_partkey=np.arange(1, P + 1, dtype=np.int64), p_mfgr=[f"MFGR#{v}" for v in mf.tolist()],
-                p_category=[f"MFGR#{v}" for v in cat.tolist()], p_brand1=[f"MFGR#{v}" for v in brand.tolist()])
+                p_category=[f"MFGR#{v}" for v in cat.tolist()],
+                p_brand1=[f"MFGR#{c}{b}" for c, b in zip(cat.tolist(), bn.tolist())])
 
     def fk(keys, null=0.005, miss=0.01):
         x = keys[rng.integers(0, len(keys), n)].copy()
@@ -78,6 +94,9 @@ def _tables(seed=7, n=400_000):
     ym_jan94 = dates["d_datekey"][dates["d_yearmonthnum"] == 199401]
     sel = rng.random(n) < 0.05
     od[sel] = ym_jan94[rng.integers(0, len(ym_jan94), int(sel.sum()))]
+    wk6_94 = dates["d_datekey"][(dates["d_weeknuminyear"] == 6) & (dates["d_year"] == 1994)]
+    sel = rng.random(n) < 0.03
+    od[sel] = wk6_94[rng.integers(0, len(wk6_94), int(sel.sum()))]
 
     def dbl(lo, hi, nan=0.002, integral=False):
         x = rng.integers(lo, hi + 1, n).astype(np.float64) if integral else np.round(rng.uniform(lo, hi, n), 2)
@@ -90,6 +109,20 @@ def _tables(seed=7, n=400_000):
     return dict(dates=dates, supplier=supplier, customer=customer, part=part, lineorder=lineorder)
 
 
+def _domain_of(name):
+    attr = name.split("_", 1)[1]
+    return attr if attr in DOMAINS else None
+
+
+def _encoded(tables):
+    """The transformed schema: every STRING attribute replaced by its BIGINT code."""
+    out = {}
+    for t, cols in tables.items():
+        out[t] = {name: (np.array([CODE[_domain_of(name)][v] for v in col], dtype=np.int64)
+                         if isinstance(col, list) else col) for name, col in cols.items()}
+    return out
+
+
 def _dev_table(cols):
     from ghive_amd import ops
     names, dev = [], []
@@ -99,14 +132,15 @@ def _dev_table(cols):
     return ops.Table(dev), names
 
 
-def _execute(ctx, q, tables, gtables=False):
+def _execute(ctx, q, tables, gtables=False, form="strings"):
     """Runs every vertex of the plan once its input vertices have run; returns the root's rows.
     gtables: bind inputs as the JNI GTables delivers them -- the scanned table under the vertex's
     own name with the TableScan's columns in order, every input without column names."""
     from ghive_amd import ops
-    with open(os.path.join(PLANS, f"ssb_2_orc_{q}.txt")) as fh:
+    db = FORMS[form] + "@"
+    with open(os.path.join(PLANS, f"{FORMS[form]}_{q}.txt")) as fh:
         text = fh.read()
-    with open(os.path.join(PLANS, f"ssb_2_orc_{q}_extended.txt")) as fh:
+    with open(os.path.join(PLANS, f"{FORMS[form]}_{q}_extended.txt")) as fh:
         ext = fh.read()
     plan = ops.Plan(text, ext)
     d = plan.describe()
@@ -117,7 +151,7 @@ def _execute(ctx, q, tables, gtables=False):
             co = opsd[c]
             if co["kind"] in ("ReduceSink", "FileSink", "Union") and co["vertex"] != o["vertex"]:
                 deps[o["vertex"]].add(co["vertex"])
-    base = {DB + name: _dev_table(cols) for name, cols in tables.items()}
+    base = {db + name: _dev_table(cols) for name, cols in tables.items()}
     done = {}
     while len(done) < len(deps):
         ready = [v for v in deps if v not in done and deps[v] <= done.keys()]
@@ -127,7 +161,7 @@ def _execute(ctx, q, tables, gtables=False):
             if gtables:
                 for o in opsd.values():
                     if o["kind"] == "TableScan" and o["vertex"] == v:
-                        cols = tables[o["table"][len(DB):]]
+                        cols = tables[o["table"][len(db):]]
                         dev, _ = _dev_table({c: cols[c] for c in o["outputs"]})
                         inputs[v] = (dev, None)
             for dv in deps[v]:
@@ -204,7 +238,29 @@ def _present(*cols):
     return m
 
 
-def _expected(q, t):
+def _coded(rows):
+    """Result rows of the string schema in the transformed one: every string as its code."""
+    out = []
+    for r in rows:
+        row = []
+        for v in r:
+            if isinstance(v, str):
+                v = next(CODE[d][v] for d in ("region", "nation", "city", "mfgr", "category", "brand1", "yearmonth")
+                         if v in CODE[d])
+            row.append(v)
+        out.append(tuple(row))
+    return out
+
+
+def _expected(q, t, form="strings"):
+    if form == "codes":
+        if q == "1_3":  # the transformed plan's Q1.3 is the SSB definition
+            J = _joined(t)
+            with np.errstate(invalid="ignore"):
+                disc, qty = J["lo_discount"], J["lo_quantity"]
+                return _q1(J, _eq(J["d_weeknuminyear"], 6) & _eq(J["d_year"], 1994),
+                           (disc >= 5) & (disc <= 7) & (qty >= 26) & (qty <= 35))
+        return _coded(_expected(q, t))
     J = _joined(t)
     lo = J
     with np.errstate(invalid="ignore"):
@@ -287,3 +343,14 @@ def test_ssb_query_with_gtables_binding(ctx, ssb_tables, q):
     """The same queries with inputs bound as GTables delivers them: positional columns, the
     scanned table under the vertex's own name."""
     assert [tuple(r) for r in _execute(ctx, q, ssb_tables, gtables=True)] == _expected(q, ssb_tables)
+
+
+@pytest.mark.parametrize("q", ["1_1", "1_2", "1_3", "2_1", "2_2", "2_3", "3_1", "3_2", "3_3", "3_4", "4_1", "4_2",
+                               "4_3"])
+def test_ssb_transformed_query_end_to_end(ctx, ssb_tables, q):
+    """The transformed schema's plans (BIGINT codes, UDFToDouble on p_brand1, a BIGINT constant in
+    Q2.3's projection) over the same tables encoded."""
+    got = _execute(ctx, q, _encoded(ssb_tables), form="codes")
+    want = _expected(q, ssb_tables, form="codes")
+    assert len(want) > 0
+    assert [tuple(r) for r in got] == want, q
