@@ -105,6 +105,10 @@ SIGNATURES_EXTRA = {
     "ghive_result_info": (C.c_int, [P, PU64, C.POINTER(u32)]),
     "ghive_result_column": (C.c_int, [P, u32, C.POINTER(abi.Column), C.POINTER(C.c_char_p)]),
     "ghive_result_free": (None, [P, P]),
+    "ghive_gtables_stage": (C.c_int, [P, C.POINTER(abi.GTables), C.POINTER(P)]),
+    "ghive_staged_inputs": (C.c_int, [P, C.POINTER(C.POINTER(abi.PlanInput)), C.POINTER(u32)]),
+    "ghive_staged_free": (None, [P, P]),
+    "ghive_gpuresult_sequence": (C.c_int, [C.POINTER(i32), u32, C.POINTER(i32), C.POINTER(u32)]),
     "ghive_select_arith": (C.c_int, [P, C.POINTER(abi.Column), C.POINTER(abi.Column), C.c_char, u64,
                                      C.POINTER(abi.Column)]),
     "ghive_server_create": (C.c_int, [C.c_int, C.POINTER(abi.ServerConfig), C.POINTER(P)]),

@@ -67,6 +67,17 @@ class PlanInput(C.Structure):
     _fields_ = [("name", C.c_char_p), ("table", C.POINTER(Table)), ("column_names", C.POINTER(C.c_char_p))]
 
 
+class GTables(C.Structure):
+    _fields_ = [("ntables", C.c_uint32), ("types", C.POINTER(C.POINTER(C.c_int32))),
+                ("ncols", C.POINTER(C.c_uint32)), ("row_cnt", C.POINTER(C.c_int32)),
+                ("key_cnt", C.POINTER(C.c_int32)), ("vertex_name", C.POINTER(C.c_char_p)),
+                ("long_cols", C.POINTER(C.c_void_p)), ("nlong", C.c_uint32),
+                ("double_cols", C.POINTER(C.c_void_p)), ("ndouble", C.c_uint32),
+                ("int_cols", C.POINTER(C.c_void_p)), ("nint", C.c_uint32),
+                ("string_cols", C.POINTER(C.c_void_p)), ("string_idx_cols", C.POINTER(C.c_void_p)),
+                ("nstring", C.c_uint32)]
+
+
 class Agg(C.Structure):
     _fields_ = [("type", C.c_int32), ("index", C.c_uint32)]
 

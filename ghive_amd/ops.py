@@ -662,6 +662,88 @@ class Server:
             pass
 
 
+class GTablesStaged:
+    """GTables (ghive/GTables.java) staged through ghive_gtables_stage. tables: a list of
+    (vertex name or None, [columns]) where a column is a numpy array (int64 LONG, float64 DOUBLE,
+    int32 INT) or a (bytes blob, int32 (start,end) array) STRING column; the buffers of each type
+    are concatenated across tables in order, as the Java side hands them over."""
+
+    def __init__(self, ctx, tables, key_cnt=None):
+        import numpy as np
+        self.ctx, self.h = ctx, C.c_void_p()
+        keep = []
+        typecode = {np.dtype(np.int64): 0, np.dtype(np.float64): 1, np.dtype(np.int32): 2}
+        bufs = {0: [], 1: [], 2: [], 3: [], "idx": []}
+        types, ncols, rows, names = [], [], [], []
+        for name, cols in tables:
+            tt = []
+            n = None
+            for c in cols:
+                if isinstance(c, tuple):
+                    blob, idx = c
+                    idx = np.ascontiguousarray(idx, dtype=np.int32)
+                    b = C.create_string_buffer(bytes(blob), max(len(blob), 1))
+                    keep += [b, idx]
+                    bufs[3].append(C.cast(b, C.c_void_p))
+                    bufs["idx"].append(idx.ctypes.data)
+                    tt.append(3)
+                    n = len(idx) // 2
+                else:
+                    a = np.ascontiguousarray(c)
+                    keep.append(a)
+                    code = typecode[a.dtype]
+                    bufs[code].append(a.ctypes.data)
+                    tt.append(code)
+                    n = len(a)
+            arr = (C.c_int32 * max(len(tt), 1))(*tt)
+            keep.append(arr)
+            types.append(C.cast(arr, C.POINTER(C.c_int32)))
+            ncols.append(len(tt))
+            rows.append(n or 0)
+            names.append(None if name is None else name.encode())
+        g = abi.GTables()
+        g.ntables = len(tables)
+        g.types = (C.POINTER(C.c_int32) * max(len(types), 1))(*types)
+        g.ncols = (C.c_uint32 * max(len(ncols), 1))(*ncols)
+        g.row_cnt = (C.c_int32 * max(len(rows), 1))(*rows)
+        if key_cnt is not None:
+            g.key_cnt = (C.c_int32 * max(len(key_cnt), 1))(*key_cnt)
+        g.vertex_name = (C.c_char_p * max(len(names), 1))(*names)
+        for code, fld, cnt in ((0, "long_cols", "nlong"), (1, "double_cols", "ndouble"), (2, "int_cols", "nint"),
+                               (3, "string_cols", "nstring")):
+            setattr(g, fld, (C.c_void_p * max(len(bufs[code]), 1))(*bufs[code]))
+            setattr(g, cnt, len(bufs[code]))
+        g.string_idx_cols = (C.c_void_p * max(len(bufs["idx"]), 1))(*bufs["idx"])
+        ctx.check(lib().ghive_gtables_stage(ctx.h, C.byref(g), C.byref(self.h)))
+
+    def inputs(self):
+        p, n = C.POINTER(abi.PlanInput)(), C.c_uint32()
+        lib().ghive_staged_inputs(self.h, C.byref(p), C.byref(n))
+        return p, n.value
+
+    def close(self):
+        if self.h:
+            lib().ghive_staged_free(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gpuresult_sequence(types):
+    """generate_tbl_result's numbering of result columns: (sequence, counts per LONG/DOUBLE/INT/STRING)."""
+    t = (C.c_int32 * max(len(types), 1))(*types)
+    out = (C.c_int32 * max(len(types), 1))()
+    counts = (C.c_uint32 * 4)()
+    st = lib().ghive_gpuresult_sequence(t, len(types), out, counts)
+    if st != abi.OK:
+        raise GhiveError(st, "gpuresult_sequence: unknown column type")
+    return list(out)[: len(types)], list(counts)
+
+
 class Plan:
     """A query plan (ghive_plan_*): Hive's EXPLAIN and EXPLAIN EXTENDED texts (and the Java side's
     maintain-values JSON) parsed into every vertex's operator tree with resolved descriptors.
@@ -705,18 +787,22 @@ class Plan:
         names or None)} -- a base table by its TableScan name ("db@table") or by the vertex run, or
         another vertex's result by that vertex's name; None = positional columns. Returns (column
         names, [device tensors / StringColumns]); the columns are copies owned by torch."""
-        keep, arr = [], (abi.PlanInput * max(len(inputs), 1))()
-        for i, (name, (table, names)) in enumerate(inputs.items()):
-            keep.append(table)
-            arr[i].name = name.encode()
-            arr[i].table = C.pointer(table.table)
-            if names is not None:  # else positional, as GTables delivers the columns
-                cn = (C.c_char_p * max(len(names), 1))(*[n.encode() for n in names])
-                keep.append(cn)
-                arr[i].column_names = C.cast(cn, C.POINTER(C.c_char_p))
+        if isinstance(inputs, GTablesStaged):  # tables staged from GTables buffers
+            arr, count = inputs.inputs()
+            keep = [inputs]
+        else:
+            keep, arr, count = [], (abi.PlanInput * max(len(inputs), 1))(), len(inputs)
+            for i, (name, (table, names)) in enumerate(inputs.items()):
+                keep.append(table)
+                arr[i].name = name.encode()
+                arr[i].table = C.pointer(table.table)
+                if names is not None:  # else positional, as GTables delivers the columns
+                    cn = (C.c_char_p * max(len(names), 1))(*[n.encode() for n in names])
+                    keep.append(cn)
+                    arr[i].column_names = C.cast(cn, C.POINTER(C.c_char_p))
         err = C.create_string_buffer(512)
         r = C.c_void_p()
-        st = lib().ghive_plan_execute(ctx.h, self.h, vertex.encode(), arr, len(inputs), err, len(err), C.byref(r))
+        st = lib().ghive_plan_execute(ctx.h, self.h, vertex.encode(), arr, count, err, len(err), C.byref(r))
         if st != abi.OK:
             raise GhiveError(st, err.value.decode(errors="replace"))
         try:

@@ -534,6 +534,44 @@ typedef struct ghive_result ghive_result;
 int ghive_plan_execute(ghive_ctx *ctx, const ghive_plan *plan, const char *vertex,
                        const ghive_plan_input *inputs, uint32_t ninputs, char *err, uint64_t errcap,
                        ghive_result **out);
+/* ---- the JNI boundary's marshalling, without a JVM ---------------------------------------- *
+ * GTables (ghive/GTables.java:13-31) as the addresses of its direct ByteBuffers, in GTables
+ * order: per table its column type codes (0 LONG, 1 DOUBLE, 2 INT, 3 STRING; other codes are
+ * skipped), row and key counts and vertex name; the column buffers of each type concatenated
+ * across tables and consumed in table and column order (ghive_JNIInterface.cu:103-200). A STRING
+ * column is a UTF-8 buffer plus an int32 (start,end) buffer; the blob's size is the largest value
+ * of its pairs, as the reference computes it. Buffers are host memory (little-endian). */
+typedef struct ghive_gtables {
+  uint32_t ntables;
+  const int32_t *const *types;      /* types[t][0..ncols[t]) */
+  const uint32_t *ncols;
+  const int32_t *row_cnt;           /* rowCnt */
+  const int32_t *key_cnt;           /* keyCnt (may be NULL) */
+  const char *const *vertex_name;   /* vertexName; a NULL entry means a missing input */
+  const void *const *long_cols;
+  uint32_t nlong;
+  const void *const *double_cols;
+  uint32_t ndouble;
+  const void *const *int_cols;
+  uint32_t nint;
+  const void *const *string_cols;
+  const void *const *string_idx_cols;
+  uint32_t nstring;
+} ghive_gtables;
+typedef struct ghive_staged ghive_staged;
+/* ghive_JNIInterface.cu:88-204: copies every table to device memory (two pinned buffers, so the
+ * host copy of one chunk overlaps the transfer of the previous one) and describes each as a plan
+ * input named by its vertex name, columns positional. A NULL vertex name (the reference's missing
+ * input) or too few buffers of a type: GHIVE_ERR_INVALID. */
+int ghive_gtables_stage(ghive_ctx *ctx, const ghive_gtables *g, ghive_staged **out);
+/* The staged tables as ghive_plan_execute inputs (valid until ghive_staged_free). */
+int ghive_staged_inputs(const ghive_staged *s, const ghive_plan_input **inputs, uint32_t *n);
+void ghive_staged_free(ghive_ctx *ctx, ghive_staged *s);
+/* generate_tbl_result numbering (ghive_JNIInterface.cu:223-342): GPUResult holds the LONG columns,
+ * then DOUBLE, INT and STRING, each group in column order; sequence[c] = the position of column c
+ * in that concatenation, counts[4] = the columns of each group. FLOAT is numbered as DOUBLE. Host
+ * only. */
+int ghive_gpuresult_sequence(const int32_t *types, uint32_t ncols, int32_t *sequence, uint32_t counts[4]);
 int ghive_result_info(const ghive_result *r, uint64_t *rows, uint32_t *ncols);
 /* Column i (device data, owned by the result) and its name. */
 int ghive_result_column(const ghive_result *r, uint32_t i, ghive_column *col, const char **name);

@@ -48,7 +48,7 @@ def test_struct_layouts_match_header(tmp_path):
         "ghive_fragment_agg": abi.FragmentAgg, "ghive_fragment": abi.Fragment,
         "ghive_device_info": abi.DeviceInfo, "ghive_stage_times": abi.StageTimes,
         "ghive_batch_item": abi.BatchItem, "ghive_batch_spec": abi.BatchSpec,
-        "ghive_batch_tuning": abi.BatchTuning,
+        "ghive_batch_tuning": abi.BatchTuning, "ghive_plan_input": abi.PlanInput, "ghive_gtables": abi.GTables,
         "ghive_server_config": abi.ServerConfig,
         "ghive_request": abi.Request,
         "ghive_response": abi.Response,
@@ -135,6 +135,27 @@ def test_parse_predicate_errors(text):
     from ghive_amd._lib import GhiveError
     with pytest.raises(GhiveError):
         ops.FilterOperator(text, ["a", "b"]).generatePredicate()
+
+
+def test_gpuresult_sequence_numbering(L):
+    """generate_tbl_result (ghive_JNIInterface.cu:223-342): GPUResult holds LONG columns first, then
+    DOUBLE, INT, STRING, each in column order; sequence[c] is column c's place in that order."""
+    import ctypes as C
+    LONG, DOUBLE, INT, STRING, FLOAT = abi.LONG, abi.DOUBLE, abi.INT, abi.STRING, abi.FLOAT
+
+    def seq(types):
+        t = (C.c_int32 * max(len(types), 1))(*types)
+        out = (C.c_int32 * max(len(types), 1))()
+        counts = (C.c_uint32 * 4)()
+        assert L.ghive_gpuresult_sequence(t, len(types), out, counts) == abi.OK
+        return list(out)[: len(types)], list(counts)
+    # 2 LONG (0, 1), 1 DOUBLE (2), 1 INT (3), 1 STRING (4)
+    assert seq([INT, LONG, STRING, DOUBLE, LONG]) == ([3, 0, 4, 2, 1], [2, 1, 1, 1])
+    assert seq([STRING, STRING, INT]) == ([1, 2, 0], [0, 0, 1, 2])
+    assert seq([FLOAT, DOUBLE, LONG]) == ([1, 2, 0], [1, 2, 0, 0])  # FLOAT travels as double
+    assert seq([]) == ([], [0, 0, 0, 0])
+    bad = (C.c_int32 * 1)(9)
+    assert L.ghive_gpuresult_sequence(bad, 1, (C.c_int32 * 1)(), (C.c_uint32 * 4)()) == abi.ERR_INVALID
 
 
 def test_agg_output_types(L):

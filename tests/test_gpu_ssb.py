@@ -354,3 +354,97 @@ def test_ssb_transformed_query_end_to_end(ctx, ssb_tables, q):
     want = _expected(q, ssb_tables, form="codes")
     assert len(want) > 0
     assert [tuple(r) for r in got] == want, q
+
+
+def _host_col(c):
+    from ghive_amd import ops
+    if isinstance(c, ops.StringColumn):
+        return (c.blob.cpu().numpy().tobytes(), c.idx.cpu().numpy())
+    return c.cpu().numpy()
+
+
+def _gpuresult_rows(names, cols):
+    """The result as the JNI shim hands it to GPUResult: typed column groups plus sequence[], then
+    read back in column order through sequence[]."""
+    from ghive_amd import ops
+    types = [abi.STRING if isinstance(c, ops.StringColumn) else
+             {np.dtype(np.int64): abi.LONG, np.dtype(np.float64): abi.DOUBLE,
+              np.dtype(np.int32): abi.INT}[np.dtype(str(c.dtype).replace("torch.", ""))] for c in cols]
+    seq, counts = ops.gpuresult_sequence(types)
+    groups = []
+    for c in cols:
+        if isinstance(c, ops.StringColumn):
+            groups.append([None if v is None else v.decode() for v in c.to_list()])
+        else:
+            groups.append(c.cpu().numpy().tolist())
+    concatenated = [None] * len(cols)  # long[][] ++ double[][] ++ int[][] ++ String[][]
+    for i, pos in enumerate(seq):
+        concatenated[pos] = groups[i]
+    assert sum(counts) == len(cols)
+    return list(zip(*[concatenated[seq[i]] for i in range(len(cols))]))
+
+
+@pytest.mark.parametrize("q", ["1_1", "2_2", "3_3", "4_2"])
+def test_ssb_query_through_gtables_marshal(ctx, ssb_tables, q):
+    """Each vertex receives its inputs as GTables delivers them -- host buffers of each type
+    concatenated across tables, staged by ghive_gtables_stage -- and each result travels on as host
+    buffers, as through GPUResult and the Tez shuffle; the root's rows are read back through the
+    GPUResult column numbering."""
+    from ghive_amd import ops
+    with open(os.path.join(PLANS, f"ssb_2_orc_{q}.txt")) as fh:
+        text = fh.read()
+    with open(os.path.join(PLANS, f"ssb_2_orc_{q}_extended.txt")) as fh:
+        ext = fh.read()
+    plan = ops.Plan(text, ext)
+    d = plan.describe()
+    opsd = d["operators"]
+    deps = {v: set() for v in d["vertices"]}
+    for o in opsd.values():
+        for c in o["children"]:
+            co = opsd[c]
+            if co["kind"] in ("ReduceSink", "FileSink") and co["vertex"] != o["vertex"]:
+                deps[o["vertex"]].add(co["vertex"])
+    host_results, done = {}, {}
+    while len(done) < len(deps):
+        for v in [v for v in deps if v not in done and deps[v] <= done.keys()]:
+            tables = []
+            for o in opsd.values():
+                if o["kind"] == "TableScan" and o["vertex"] == v:
+                    src = ssb_tables[o["table"].split("@")[1]]
+                    cols = []
+                    for name in o["outputs"]:
+                        x = src[name]
+                        if isinstance(x, list):
+                            blob, idx, off = bytearray(), [], 0
+                            for s in x:
+                                b = s.encode()
+                                blob += b
+                                idx += [off, off + len(b)]
+                                off += len(b)
+                            x = (bytes(blob), np.array(idx, dtype=np.int32))
+                        cols.append(x)
+                    tables.append((v, cols))
+            for dv in sorted(deps[v]):
+                tables.append((dv, host_results[dv]))
+            staged = ops.GTablesStaged(ctx, tables)
+            try:
+                done[v] = plan.execute(ctx, v, staged)
+            finally:
+                staged.close()
+            host_results[v] = [_host_col(c) for c in done[v][1]]
+    names, cols = done[opsd[d["root"]]["vertex"]]
+    assert [tuple(r) for r in _gpuresult_rows(names, cols)] == _expected(q, ssb_tables)
+
+
+def test_gtables_missing_input_and_short_buffers(ctx):
+    from ghive_amd import ops
+    from ghive_amd._lib import GhiveError
+    with pytest.raises(GhiveError) as e:  # a NULL vertex name: the reference returns nullptr
+        ops.GTablesStaged(ctx, [("Map 1", [np.arange(4, dtype=np.int64)]), (None, [np.arange(3, dtype=np.int32)])])
+    assert e.value.status == abi.ERR_INVALID and "missing input" in str(e.value)
+    staged = ops.GTablesStaged(ctx, [("Map 1", [np.arange(5, dtype=np.int64), (b"abcde", np.array([0, 2, -1, -1, 2, 2, 2, 5, 5, 5], dtype=np.int32))])])
+    p, n = staged.inputs()
+    assert n == 1 and p[0].name == b"Map 1" and p[0].table.contents.row_num == 5
+    col = p[0].table.contents.columns[1]
+    assert col.type == abi.STRING and col.char_size == 5
+    staged.close()
