@@ -111,6 +111,8 @@ SIGNATURES_EXTRA = {
     "ghive_gpuresult_sequence": (C.c_int, [C.POINTER(i32), u32, C.POINTER(i32), C.POINTER(u32)]),
     "ghive_select_arith": (C.c_int, [P, C.POINTER(abi.Column), C.POINTER(abi.Column), C.c_char, u64,
                                      C.POINTER(abi.Column)]),
+    "ghive_select_cast": (C.c_int, [P, C.POINTER(abi.Column), i32, u64, C.POINTER(abi.Column)]),
+    "ghive_select_substr": (C.c_int, [P, C.POINTER(abi.Column), i32, i32, u64, C.POINTER(abi.Column)]),
     "ghive_server_create": (C.c_int, [C.c_int, C.POINTER(abi.ServerConfig), C.POINTER(P)]),
     "ghive_server_submit": (C.c_int, [P, C.POINTER(abi.Request), PU64]),
     "ghive_server_wait": (C.c_int, [P, u64, C.POINTER(abi.Response)]),

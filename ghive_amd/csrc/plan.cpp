@@ -15,6 +15,7 @@
 // group-by over a reduce input is offset by that sink's key count at parse time (the reference
 // reads the key count of the child's runtime result).
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <map>
 #include <regex>
@@ -819,6 +820,38 @@ struct Executor {
         }
       }
       return expression(in, t, out);  // redundant parentheses
+    }
+    static const std::regex call_re(R"(^(\w+)\((.*)\)$)");
+    if (std::regex_match(e, m, call_re)) {  // UDFToInteger / UDFToLong / UDFToDouble / UDFToFloat, substr
+      const std::string fn = m[1].str();
+      const std::vector<std::string> args = split_list(m[2].str());
+      int to = -1;
+      if (fn == "UDFToInteger") to = GHIVE_INT;
+      if (fn == "UDFToLong") to = GHIVE_LONG;
+      if (fn == "UDFToDouble") to = GHIVE_DOUBLE;
+      if (fn == "UDFToFloat") to = GHIVE_FLOAT;
+      if (to >= 0 && args.size() == 1) {
+        ghive_column x;
+        int st = expression(args[0], t, x);
+        if (st != GHIVE_OK) return st;
+        if ((st = alloc_col(to, t.rows, out)) != GHIVE_OK) return st;
+        st = ghive_select_cast(ctx, &x, to, t.rows, &out);
+        return st == GHIVE_OK ? GHIVE_OK : op_fail(st);
+      }
+      if ((fn == "substr" || fn == "substring") && (args.size() == 2 || args.size() == 3)) {
+        ghive_column x;
+        int st = expression(args[0], t, x);
+        if (st != GHIVE_OK) return st;
+        static const std::regex int_re(R"(^-?\d+$)");
+        if (!std::regex_match(args[1], int_re) || (args.size() == 3 && !std::regex_match(args[2], int_re)))
+          return fail(GHIVE_ERR_UNSUPPORTED, "plan: substr with non-constant bounds: " + e);
+        const int32_t pos = (int32_t)std::stol(args[1]);
+        const int32_t len = args.size() == 3 ? (int32_t)std::stol(args[2]) : INT32_MAX;
+        if ((st = alloc_col(GHIVE_STRING, t.rows, out)) != GHIVE_OK) return st;
+        st = ghive_select_substr(ctx, &x, pos, len, t.rows, &out);
+        return st == GHIVE_OK ? GHIVE_OK : op_fail(st);
+      }
+      return fail(GHIVE_ERR_UNSUPPORTED, "plan: function " + fn);
     }
     if (e.size() >= 2 && e.front() == '\'' && e.back() == '\'') {  // a string constant, on every row
       const std::string lit = e.substr(1, e.size() - 2);

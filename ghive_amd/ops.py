@@ -447,6 +447,39 @@ def select_arith(ctx, a, b, op):
     return out
 
 
+def _any_column_of(c):
+    if isinstance(c, StringColumn):
+        col = abi.Column()
+        col.type = abi.STRING
+        col.data = c.blob.data_ptr()
+        col.aux = c.idx.data_ptr() if c.numel() else None
+        col.row_num = c.numel()
+        col.char_size = c.blob.numel()
+        return col
+    return _column_of(c)
+
+
+def select_cast(ctx, x, to_type):
+    """CAST of a device column (tensor or StringColumn) to INT / LONG / DOUBLE / FLOAT, as Hive's
+    UDFToInteger / UDFToLong / UDFToDouble (string_convert_int_functor, SelectPredicate.hpp:22-66)."""
+    n = x.numel()
+    dev = x.device
+    out = torch.empty(max(n, 1), dtype=_DTYPE_OF_TYPE[to_type], device=dev)
+    ci, co = _any_column_of(x), _column_of(out)
+    ctx.check(lib().ghive_select_cast(ctx.h, C.byref(ci), to_type, n, C.byref(co)))
+    return out[:n]
+
+
+def select_substr(ctx, s, pos, length):
+    """SUBSTR(s, pos, length) of a StringColumn as Hive's UDFSubstr; the result shares s's blob."""
+    n = s.numel()
+    pairs = torch.empty(max(2 * n, 2), dtype=torch.int32, device=s.device)
+    ci, co = _any_column_of(s), abi.Column()
+    co.aux = pairs.data_ptr()
+    ctx.check(lib().ghive_select_substr(ctx.h, C.byref(ci), pos, length, n, C.byref(co)))
+    return StringColumn(s.blob, pairs[: 2 * n])
+
+
 class Fragment:
     """Fused CS1 fragment: fact table, star joins (fact column, HashTable), keys, aggregates."""
 

@@ -322,6 +322,23 @@ int ghive_groupby(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel,
 int ghive_select_arith(ghive_ctx *ctx, const ghive_column *a, const ghive_column *b, char op,
                        uint64_t n, ghive_column *out);
 
+/* CAST (FunctorSelectPredicate with string_convert_int_functor, SelectPredicate.hpp:22-66, as
+ * Hive's UDFToInteger / UDFToLong of Text: ql/.../udf/UDFToInteger.java:169-186 with
+ * serde2/lazy/LazyInteger.java parseInt(trim)): a STRING to INT or LONG -- ASCII whitespace
+ * trimmed, an optional sign, decimal digits, a '.' fraction truncated; malformed, empty or
+ * overflowing text is NULL (the reference traps on a non-digit). A numeric column to any numeric
+ * type as Java converts (narrowing integers keep the low bits, floating to integer truncates and
+ * saturates). NULL stays NULL. out->data: device buffer of n values of to_type. */
+int ghive_select_cast(ghive_ctx *ctx, const ghive_column *in, int32_t to_type, uint64_t n,
+                      ghive_column *out);
+/* SUBSTR(str, pos, len) (substr_functor, SelectPredicate.hpp:68-113, as Hive's UDFSubstr,
+ * ql/.../udf/UDFSubstr.java:61-105): positions in UTF-16 units of the UTF-8 text, pos 1-based
+ * (negative: from the end, 0: from the start), len <= 0 or |pos| > length gives ''. The result
+ * shares the input blob: out->aux receives n (start,end) pairs (caller buffer), out->data is set to
+ * the input's. NULL stays NULL. */
+int ghive_select_substr(ghive_ctx *ctx, const ghive_column *str, int32_t pos, int32_t len, uint64_t n,
+                        ghive_column *out);
+
 /* ---- sort --------------------------------------------------------------------------- */
 /* SinkOperator ORDER BY (SinkOperator.cu:135-288; orders as SortOrder '+'/'-',
  * GroupByPredicate.hpp:17-20): LSD radix sort; writes the permutation of row ids so that

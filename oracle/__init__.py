@@ -204,6 +204,72 @@ def select_arith(a, b, op):
     return r
 
 
+_JAVA_ASCII_SPACE = set(b" \t\n\x0b\x0c\r\x1c\x1d\x1e\x1f")
+
+
+def cast_string_to_int(values, bits=32):
+    """CAST(string AS INT / BIGINT) as Hive 3.1: UDFToInteger.evaluate(Text)
+    (ql/src/java/org/apache/hadoop/hive/ql/udf/UDFToInteger.java:169-186) = LazyUtils.isNumberMaybe
+    (serde/.../lazy/LazyUtils.java:92-110) then LazyInteger.parseInt(bytes, 0, len, 10, trim=true)
+    (serde/.../lazy/LazyInteger.java). values: bytes or None; returns int or None (NULL)."""
+    lo, hi = -(1 << (bits - 1)), (1 << (bits - 1)) - 1
+    out = []
+    for v in values:
+        out.append(None if v is None else _parse_java_int(bytes(v), lo, hi))
+    return out
+
+
+def _parse_java_int(b, lo, hi):
+    n = len(b)
+    isdig = lambda c: 48 <= c <= 57  # noqa: E731
+    if n == 0 or (n == 1 and not isdig(b[0])) or (n == 2 and not (isdig(b[0]) or isdig(b[1]))) or \
+            (n == 4 and b[0] in b"Nn"):
+        return None
+    s, e = 0, n
+    while s < e and b[s] in _JAVA_ASCII_SPACE:
+        s += 1
+    while e > s and b[e - 1] in _JAVA_ASCII_SPACE:
+        e -= 1
+    if s == e:
+        return None
+    neg = b[s] == 0x2D
+    i = s
+    if neg or b[s] == 0x2B:
+        i += 1
+        if e - s == 1:
+            return None
+    whole = b[i:e].split(b".", 1)
+    if not whole[0].isdigit() and whole[0] != b"":
+        return None
+    if len(whole) == 2 and whole[1] and not whole[1].isdigit():
+        return None
+    if len(whole) == 1 and whole[0] == b"":
+        return None
+    v = int(whole[0]) if whole[0] else 0
+    v = -v if neg else v
+    return v if lo <= v <= hi else None
+
+
+def substr(values, pos, length):
+    """SUBSTR(str, pos, len) as Hive 3.1's UDFSubstr.evaluate(Text, IntWritable, IntWritable)
+    (ql/src/java/org/apache/hadoop/hive/ql/udf/UDFSubstr.java:61-105): Java String positions (UTF-16
+    units), makeIndex's rules. values: bytes or None; returns bytes or None."""
+    out = []
+    for v in values:
+        if v is None:
+            out.append(None)
+            continue
+        u = bytes(v).decode("utf-8", errors="replace").encode("utf-16-le", errors="surrogatepass")
+        L = len(u) // 2
+        if length <= 0 or abs(pos) > L:
+            out.append(b"")
+            continue
+        start = pos - 1 if pos > 0 else (L + pos if pos < 0 else 0)
+        end = L if L - start < length else start + length
+        out.append(u[2 * start: 2 * end].decode("utf-16-le", errors="surrogatepass").encode("utf-8"))
+    return out
+
+
 def mapjoin_inner(build_keys, probe_keys, build_sel=None, probe_sel=None, payload=None,
                   load_factor=0.75):
     """Hive map-join (inner) restated: (probe row ids, payloads) in Hive emission order."""

@@ -214,3 +214,127 @@ def test_plan_execute_errors(ctx):
     with pytest.raises(GhiveError) as e:  # a column missing from the input
         _run(p, ctx, "Map 3", {"ssb_10_orc_transformed@dates": ([np.zeros(3, np.int64)], ["d_datekey"])})
     assert "d_year" in str(e.value)
+
+
+# ---------------------------------------------------------------- string functions ---------------
+def _rand_texts(rng, n):
+    pool = [b"123", b" 42 ", b"-7", b"+8", b"1.9", b"-1.9", b".5", b"abc", b"", b"-", b"2147483648",
+            b"-2147483648", b"12a", b"1.2.3", b"N123", b"x", b" 5", b"\t12\n", b"9223372036854775807",
+            b"9223372036854775808", "héllo wörld".encode(), "中文字".encode()]
+    out = []
+    for _ in range(n):
+        u = rng.random()
+        if u < 0.05:
+            out.append(None)
+        elif u < 0.5:
+            out.append(pool[rng.integers(0, len(pool))])
+        else:
+            digits = b"".join(bytes([48 + int(d)]) for d in rng.integers(0, 10, rng.integers(1, 21)))
+            sign = [b"", b"-", b"+"][rng.integers(0, 3)]
+            frac = b"" if rng.random() < 0.7 else b"." + digits[:3]
+            out.append(sign + digits + frac)
+    return out
+
+
+def _dev_strings(vals):
+    from ghive_amd import ops
+    return ops.StringColumn.from_list(vals)
+
+
+@pytest.mark.parametrize("to_type", [abi.INT, abi.LONG])
+def test_cast_string_to_integer_vs_oracle(ctx, orc, to_type):
+    from ghive_amd import ops
+    rng = np.random.default_rng(to_type)
+    vals = _rand_texts(rng, 50_000)
+    got = to_np(ops.select_cast(ctx, _dev_strings(vals), to_type)).tolist()
+    bits = 32 if to_type == abi.INT else 64
+    null = INT_NULL if to_type == abi.INT else LONG_NULL
+    want = [null if v is None else v for v in orc.cast_string_to_int(vals, bits=bits)]
+    assert got == want
+
+
+def test_cast_numeric_like_java(ctx):
+    from ghive_amd import ops
+    i64 = np.array([5, -(2**40) + 7, 2**31 + 3, LONG_NULL, -1], dtype=np.int64)
+    assert to_np(ops.select_cast(ctx, to_dev(i64), abi.INT)).tolist() == [5, 7, -(2**31) + 3, INT_NULL, -1]
+    i32 = np.array([5, -3, INT_NULL], dtype=np.int32)
+    assert to_np(ops.select_cast(ctx, to_dev(i32), abi.LONG)).tolist() == [5, -3, LONG_NULL]
+    d = np.array([1.9, -1.9, 3e10, -3e10, np.nan, 1e300], dtype=np.float64)
+    assert to_np(ops.select_cast(ctx, to_dev(d), abi.INT)).tolist() == [1, -1, 2**31 - 1, -(2**31), INT_NULL,
+                                                                       2**31 - 1]
+    assert to_np(ops.select_cast(ctx, to_dev(d), abi.LONG)).tolist()[:4] == [1, -1, 30000000000, -30000000000]
+    r = to_np(ops.select_cast(ctx, to_dev(i32), abi.DOUBLE))
+    assert r[0] == 5.0 and r[1] == -3.0 and math.isnan(r[2])
+
+
+@pytest.mark.parametrize("pos,length", [(1, 3), (5, 2**31 - 1), (-4, 2**31 - 1), (0, 2), (3, 1), (-3, 2),
+                                        (2, 0), (30, 5), (-30, 1)])
+def test_substr_vs_oracle(ctx, orc, pos, length):
+    from ghive_amd import ops
+    rng = np.random.default_rng(abs(pos) * 31 + length % 97)
+    words = [b"Facebook", b"abc", b"", "héllo".encode(), "中文字符".encode(),
+             b"MFGR#1221", None]
+    vals = [words[i] for i in rng.integers(0, len(words), 20_000)]
+    out = ops.select_substr(ctx, _dev_strings(vals), pos, length)
+    assert out.to_list() == orc.substr(vals, pos, length)
+
+
+def test_substr_reference_functor_case(ctx):
+    """SelectPredicateTest.cu substrTest: substr_functor(0, 3) (the first three bytes) of the 20
+    strings, then FILTER_EQ "122" -- as Hive's SUBSTR(s, 1, 3)."""
+    from ghive_amd import ops
+    strs = [b"1221", b"1223", b"1238", b"1221", b"1238", b"122", b"1232", b"1228", b"123", b"1224", b"1248",
+            b"12223", b"1237", b"1222", b"1230", b"1221", b"1234", b"1220", b"1239", b"12234"]
+    sub = ops.select_substr(ctx, _dev_strings(strs), 1, 3)
+    assert sub.to_list() == [s[:3] for s in strs]
+    t = ops.Table([sub])
+    sel = to_np(ops.FilterPredicate([abi.make_pred(abi.FILTER_EQ, 0, [b"122"])], 0).process(ctx, t)).tolist()
+    assert sel == [i for i, s in enumerate(strs) if s[:3] == b"122"]
+    # string_convert_int_functor over the same strings (stringConversionTest), as CAST(s AS INT)
+    assert to_np(ops.select_cast(ctx, _dev_strings(strs), abi.INT)).tolist() == [int(s) for s in strs]
+
+
+_TINY_PLAN = """Plan optimized by CBO.
+
+Stage-0
+  Fetch Operator
+    limit:-1
+    Stage-1
+      Map 1
+      File Output Operator [FS_2]
+        Select Operator [SEL_1] (rows=10 width=100)
+          Output:["_col0","_col1","_col2"]
+          TableScan [TS_0] (rows=10 width=100)
+            db@t,t,Tbl:COMPLETE,Col:NONE,Output:["s","n"]
+"""
+_TINY_EXTENDED = """STAGE PLANS:
+  Stage: Stage-1
+    Tez
+      Vertices:
+        Map 1 
+            Map Operator Tree:
+                TableScan
+                  alias: t
+                  Select Operator
+                    expressions: substr(s, 2, 3) (type: string), UDFToInteger(s) (type: int), UDFToDouble(n) (type: double)
+                    outputColumnNames: _col0, _col1, _col2
+                    File Output Operator
+                      compressed: false
+"""
+
+
+def test_plan_projection_functions(ctx, orc):
+    """A projection calling substr, UDFToInteger and UDFToDouble (a hand-written single-vertex plan
+    in EXPLAIN / EXPLAIN EXTENDED form) against the oracle's SUBSTR and CAST."""
+    from ghive_amd import ops
+    rng = np.random.default_rng(77)
+    vals = _rand_texts(rng, 5000)
+    n = rng.integers(-10**6, 10**6, 5000).astype(np.int64)
+    n[::13] = LONG_NULL
+    plan = ops.Plan(_TINY_PLAN, _TINY_EXTENDED)
+    names, cols = plan.execute(ctx, "Map 1", {"db@t": (ops.Table([_dev_strings(vals), to_dev(n)]), ["s", "n"])})
+    assert names == ["_col0", "_col1", "_col2"]
+    assert cols[0].to_list() == orc.substr(vals, 2, 3)
+    assert to_np(cols[1]).tolist() == [INT_NULL if v is None else v for v in orc.cast_string_to_int(vals)]
+    d = to_np(cols[2])
+    assert np.array_equal(np.isnan(d), n == LONG_NULL) and np.array_equal(d[n != LONG_NULL], n[n != LONG_NULL])

@@ -375,6 +375,32 @@ def test_string_filter_brute_force(orc):
     assert orc.filter_rows([col, col2], [node], 0).tolist() == want
 
 
+def test_cast_string_to_int_known_answers(orc):
+    """Hive 3.1 CAST(string AS INT) (UDFToInteger of Text -> LazyInteger.parseInt with trim)."""
+    cases = {b"123": 123, b" 42 ": 42, b"-7": -7, b"+8": 8, b"1.9": 1, b"-1.9": -1, b"1.": 1, b".5": 0,
+             b"abc": None, b"": None, b"-": None, b"2147483648": None, b"-2147483648": -2147483648,
+             b"12a": None, b"1.2.3": None, b"1e5": None, b"N123": None, b"x": None, b"7": 7, b" 5": 5,
+             b"\t12\n": 12, b"1 2": None, b"0x1F": None, b"--1": None, b"1.x": None, b"  ": None}
+    got = orc.cast_string_to_int(list(cases))
+    assert dict(zip(cases, got)) == cases
+    assert orc.cast_string_to_int([b"9223372036854775807", b"9223372036854775808"], bits=64) == [2**63 - 1, None]
+    assert orc.cast_string_to_int([None]) == [None]
+
+
+def test_substr_known_answers(orc):
+    """Hive's documented SUBSTR behaviour (UDFSubstr.makeIndex): 1-based, negative from the end."""
+    s = [b"Facebook"]
+    assert orc.substr(s, 5, 2**31 - 1) == [b"book"]
+    assert orc.substr(s, 5, 1) == [b"b"]
+    assert orc.substr(s, -4, 2**31 - 1) == [b"book"]
+    assert orc.substr(s, 0, 2) == [b"Fa"]
+    assert orc.substr([b"abc"], 4, 5) == [b""] and orc.substr([b"abc"], 3, 5) == [b"c"]
+    assert orc.substr([b"abc"], -3, 2) == [b"ab"] and orc.substr([b"abc"], -4, 1) == [b""]
+    assert orc.substr([b"abc"], 1, 0) == [b""] and orc.substr([None], 1, 1) == [None]
+    assert orc.substr(["h\u00e9llo".encode()], 2, 2) == ["\u00e9l".encode()]  # a 2-byte character is one unit
+    assert orc.substr(["a\U0001F600b".encode()], 2, 2) == ["\U0001F600".encode()]  # a 4-byte one is two
+
+
 # ---------------------------------------------------------------- projection arithmetic -------
 def _java_int(v):
     return (v + 2**31) % 2**32 - 2**31
