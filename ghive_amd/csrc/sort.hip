@@ -48,8 +48,11 @@ __device__ __forceinline__ uint64_t field_bits(int type, const void *p, uint64_t
   }
 }
 
+// embed_id: the row id fills the low 32 bits of word 0 (the fields start at bit 32), so the passes
+// carry no separate id array; otherwise idx[i] = i.
 template <int KW>
-__global__ void k_sort_keys(SortKeySpec S, uint64_t n, uint64_t *keys, uint32_t *idx, unsigned long long *orand) {
+__global__ void k_sort_keys(SortKeySpec S, uint64_t n, uint64_t *keys, uint32_t *idx, unsigned long long *orand,
+                            int embed_id) {
   uint64_t vor[KW], vand[KW];
 #pragma unroll
   for (int q = 0; q < KW; ++q) {
@@ -73,13 +76,14 @@ __global__ void k_sort_keys(SortKeySpec S, uint64_t n, uint64_t *keys, uint32_t 
         if (q == word + 1 && off + S.width[k] > 64) w[q] |= b >> (64 - off);
       }
     }
+    if (embed_id) w[0] |= (uint32_t)i;
 #pragma unroll
     for (int q = 0; q < KW; ++q) {
       keys[(uint64_t)q * n + i] = w[q];
       vor[q] |= w[q];
       vand[q] &= w[q];
     }
-    idx[i] = (uint32_t)i;
+    if (!embed_id) idx[i] = (uint32_t)i;
   }
 #pragma unroll
   for (int q = 0; q < KW; ++q) {
@@ -118,7 +122,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t *key
 // across waves and iterations. (2) The tile is rewritten through a 32 KiB LDS buffer, one key
 // word (then the row ids) at a time, in digit order, so each digit's run leaves as consecutive
 // addresses (coalesced stores) at its global offset.
-template <int KW>
+template <int KW, bool IDS>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *keys_in, const uint32_t *idx_in,
                                                                 uint64_t *keys_out, uint32_t *idx_out, uint64_t n,
                                                                 int word, int shift, const uint32_t *offs,
@@ -145,7 +149,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *
     const uint64_t e = t0 + (el < tn ? el : tn - 1);
 #pragma unroll
     for (int q = 0; q < KW; ++q) kv[j][q] = keys_in[(uint64_t)q * n + e];
-    id[j] = idx_in[e];
+    id[j] = IDS ? idx_in[e] : 0;
   }
   __syncthreads();
   const uint64_t lt = (1ull << lane) - 1ull;
@@ -182,9 +186,9 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *
     run[tid] += add;
     __syncthreads();  // every wave has read wcnt and run before the next iteration rewrites them
   }
-  // write-out, one word at a time through the stage buffer
+  // write-out, one word at a time through the stage buffer (then the ids, when carried)
 #pragma unroll
-  for (int q = 0; q <= KW; ++q) {
+  for (int q = 0; q < (IDS ? KW + 1 : KW); ++q) {
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kSortPerThread; ++j)
@@ -201,9 +205,11 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *
   }
 }
 
+// ids: carry the uint32 id arrays i0/i1 (false: the ids live inside the keys; i0/i1 unused).
 template <int KW>
 static int radix_passes(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, uint32_t *i0, uint32_t *i1,
-                        const std::vector<int> &digits, uint32_t **result, uint64_t **keys_result = nullptr) {
+                        const std::vector<int> &digits, uint32_t **result, uint64_t **keys_result = nullptr,
+                        bool ids = true) {
   const uint32_t nblocks = ceil_div(n, kSortTile);
   DevBuf hist, counts;
   GH_TRY(hist.alloc(ctx, sizeof(uint32_t) * 256ull * nblocks));
@@ -216,9 +222,14 @@ static int radix_passes(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, 
                                                             counts.as<uint32_t>(), nblocks);
     GH_LAUNCH(ctx);
     GH_TRY(exclusive_scan_u32(ctx, hist.as<uint32_t>(), hist.as<uint32_t>(), 256ull * nblocks, nullptr));
-    k_radix_scatter<KW><<<nblocks, kSortThreads, 0, ctx->stream>>>(kin, iin, kout, iout, n, word, shift,
-                                                                   hist.as<uint32_t>(), counts.as<uint32_t>(),
-                                                                   nblocks);
+    if (ids)
+      k_radix_scatter<KW, true><<<nblocks, kSortThreads, 0, ctx->stream>>>(kin, iin, kout, iout, n, word, shift,
+                                                                         hist.as<uint32_t>(), counts.as<uint32_t>(),
+                                                                         nblocks);
+    else
+      k_radix_scatter<KW, false><<<nblocks, kSortThreads, 0, ctx->stream>>>(kin, iin, kout, iout, n, word, shift,
+                                                                          hist.as<uint32_t>(),
+                                                                          counts.as<uint32_t>(), nblocks);
     GH_LAUNCH(ctx);
     std::swap(kin, kout);
     std::swap(iin, iout);
@@ -310,6 +321,12 @@ __global__ void k_row_rank(const uint32_t *codes, uint64_t n, const uint32_t *ra
   uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (; r < n; r += stride) rank[r] = (int32_t)rank_of_slot[codes[r]];
+}
+
+__global__ void k_low32(const uint64_t *w, uint64_t n, uint32_t *out) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) out[i] = (uint32_t)w[i];
 }
 
 __global__ void k_iota_u32(uint32_t *p, uint64_t n) {
@@ -416,12 +433,15 @@ int ghive_sort(ghive_ctx *ctx, const ghive_table *table, const uint32_t *key_col
   }
   if (bits > 256) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "sort: keys are %d bits wide (max 256)", bits);
   if (n == 0) return GHIVE_OK;
-  int top = bits;
+  // when the last key word has 32 spare bits the row id rides in them (below every field): the
+  // passes then move no separate id array, and equal keys stay in row-id order as before
+  const int KW = (bits + 63) / 64;
+  const bool embed = KW * 64 - bits >= 32;
+  int top = bits + (embed ? 32 : 0);
   for (uint32_t k = 0; k < nkeys; ++k) {
     top -= S.width[k];
     S.shift[k] = top;
   }
-  const int KW = (bits + 63) / 64;
   DevBuf k0, k1, i1, orand;
   GH_TRY(k0.alloc(ctx, 8ull * KW * n));
   GH_TRY(k1.alloc(ctx, 8ull * KW * n));
@@ -433,10 +453,10 @@ int ghive_sort(ghive_ctx *ctx, const ghive_table *table, const uint32_t *key_col
   uint32_t grid = (uint32_t)(g > cap ? cap : g);
   unsigned long long *oa_d = orand.as<unsigned long long>();
   switch (KW) {
-    case 1: k_sort_keys<1><<<grid, 256, 0, ctx->stream>>>(S, n, k0.as<uint64_t>(), out_perm, oa_d); break;
-    case 2: k_sort_keys<2><<<grid, 256, 0, ctx->stream>>>(S, n, k0.as<uint64_t>(), out_perm, oa_d); break;
-    case 3: k_sort_keys<3><<<grid, 256, 0, ctx->stream>>>(S, n, k0.as<uint64_t>(), out_perm, oa_d); break;
-    default: k_sort_keys<4><<<grid, 256, 0, ctx->stream>>>(S, n, k0.as<uint64_t>(), out_perm, oa_d); break;
+    case 1: k_sort_keys<1><<<grid, 256, 0, ctx->stream>>>(S, n, k0.as<uint64_t>(), out_perm, oa_d, embed); break;
+    case 2: k_sort_keys<2><<<grid, 256, 0, ctx->stream>>>(S, n, k0.as<uint64_t>(), out_perm, oa_d, embed); break;
+    case 3: k_sort_keys<3><<<grid, 256, 0, ctx->stream>>>(S, n, k0.as<uint64_t>(), out_perm, oa_d, embed); break;
+    default: k_sort_keys<4><<<grid, 256, 0, ctx->stream>>>(S, n, k0.as<uint64_t>(), out_perm, oa_d, embed); break;
   }
   GH_LAUNCH(ctx);
   uint64_t oa[8];
@@ -445,18 +465,23 @@ int ghive_sort(ghive_ctx *ctx, const ghive_table *table, const uint32_t *key_col
   for (int q = 0; q < KW; ++q) {
     uint64_t varying = oa[2 * q] ^ oa[2 * q + 1];
     for (int b = 0; b < 8; ++b)
-      if ((varying >> (8 * b)) & 255u) digits.push_back(8 * q + b);
+      if (((varying >> (8 * b)) & 255u) && !(embed && q == 0 && b < 4)) digits.push_back(8 * q + b);
   }
   uint32_t *res = out_perm;
-  uint64_t *K0 = k0.as<uint64_t>(), *K1 = k1.as<uint64_t>();
+  uint64_t *K0 = k0.as<uint64_t>(), *K1 = k1.as<uint64_t>(), *kres = K0;
   uint32_t *I1 = i1.as<uint32_t>();
   switch (KW) {
-    case 1: GH_TRY(radix_passes<1>(ctx, n, K0, K1, out_perm, I1, digits, &res)); break;
-    case 2: GH_TRY(radix_passes<2>(ctx, n, K0, K1, out_perm, I1, digits, &res)); break;
-    case 3: GH_TRY(radix_passes<3>(ctx, n, K0, K1, out_perm, I1, digits, &res)); break;
-    default: GH_TRY(radix_passes<4>(ctx, n, K0, K1, out_perm, I1, digits, &res)); break;
+    case 1: GH_TRY(radix_passes<1>(ctx, n, K0, K1, out_perm, I1, digits, &res, &kres, !embed)); break;
+    case 2: GH_TRY(radix_passes<2>(ctx, n, K0, K1, out_perm, I1, digits, &res, &kres, !embed)); break;
+    case 3: GH_TRY(radix_passes<3>(ctx, n, K0, K1, out_perm, I1, digits, &res, &kres, !embed)); break;
+    default: GH_TRY(radix_passes<4>(ctx, n, K0, K1, out_perm, I1, digits, &res, &kres, !embed)); break;
   }
-  if (res != out_perm) GH_HIP(ctx, hipMemcpyAsync(out_perm, res, 4ull * n, hipMemcpyDeviceToDevice, ctx->stream));
+  if (embed) {  // the permutation is the low 32 bits of word 0 in sorted order
+    k_low32<<<grid, 256, 0, ctx->stream>>>(kres, n, out_perm);
+    GH_LAUNCH(ctx);
+  } else if (res != out_perm) {
+    GH_HIP(ctx, hipMemcpyAsync(out_perm, res, 4ull * n, hipMemcpyDeviceToDevice, ctx->stream));
+  }
   GH_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return GHIVE_OK;
 }
