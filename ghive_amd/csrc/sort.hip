@@ -129,33 +129,39 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *
                                                                 const uint32_t *counts, uint32_t nblocks) {
   __shared__ uint64_t stage[kSortTile];
   __shared__ uint8_t sdigit[kSortTile];
-  __shared__ uint32_t lbase[256], goff[256], run[256];
+  __shared__ uint32_t lbase[256], goff[256];
   __shared__ uint32_t wcnt[kSortWaves][256];
   __shared__ uint32_t scan_tmp[kSortWaves];
+  constexpr uint32_t kSeg = kSortPerThread * kWave;  // each wave owns one contiguous segment of the tile
   const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
   const uint64_t t0 = (uint64_t)blockIdx.x * kSortTile;
   const uint32_t tn = (uint32_t)(n - t0 < kSortTile ? n - t0 : kSortTile);
   uint32_t total;
   const uint32_t lb = block_exclusive_scan_256(counts[(uint64_t)blockIdx.x * 256 + tid], scan_tmp, total);
   lbase[tid] = lb;
-  run[tid] = lb;
   goff[tid] = offs[(uint64_t)tid * nblocks + blockIdx.x];
-  // all loads of the tile up front; past the end a lane re-reads the last element (ignored)
+#pragma unroll
+  for (int q = 0; q < 256 / kWave; ++q) wcnt[wave][lane + q * kWave] = 0;
+  // all loads of the tile up front, element wave * kSeg + j * 64 + lane (coalesced per j); past the
+  // end a lane re-reads the last element (ignored)
   uint64_t kv[kSortPerThread][KW];
-  uint32_t id[kSortPerThread], lpos[kSortPerThread];
+  uint32_t id[kSortPerThread], lpos[kSortPerThread], dg[kSortPerThread];
 #pragma unroll
   for (int j = 0; j < kSortPerThread; ++j) {
-    const uint32_t el = (uint32_t)j * kSortThreads + tid;
+    const uint32_t el = wave * kSeg + (uint32_t)j * kWave + lane;
     const uint64_t e = t0 + (el < tn ? el : tn - 1);
 #pragma unroll
     for (int q = 0; q < KW; ++q) kv[j][q] = keys_in[(uint64_t)q * n + e];
     id[j] = IDS ? idx_in[e] : 0;
   }
   __syncthreads();
+  // Rank every element among the equal digits of its wave's segment, in element order: ballots give
+  // the rank inside one load of 64, the wave's own LDS row carries the count across its loads (one
+  // wave reads and then updates its row: no barrier between loads).
   const uint64_t lt = (1ull << lane) - 1ull;
 #pragma unroll
   for (int j = 0; j < kSortPerThread; ++j) {
-    const bool valid = (uint32_t)j * kSortThreads + tid < tn;
+    const bool valid = wave * kSeg + (uint32_t)j * kWave + lane < tn;
     uint32_t digit = 0;
 #pragma unroll
     for (int q = 0; q < KW; ++q)
@@ -166,25 +172,32 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *
       const uint64_t bb = ballot((digit >> b) & 1u);
       peers &= ((digit >> b) & 1u) ? bb : ~bb;
     }
+    const uint32_t before = wcnt[wave][digit];
     const uint32_t rank = (uint32_t)__popcll(peers & lt);
+    __builtin_amdgcn_wave_barrier();
+    if (valid && rank == 0) wcnt[wave][digit] = before + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+    lpos[j] = before + rank;
+    dg[j] = digit;
+  }
+  __syncthreads();
+  {  // exclusive prefix of each digit's counts over the waves (segments are in wave order)
+    uint32_t acc = 0;
 #pragma unroll
-    for (int q = 0; q < 256 / kWave; ++q) wcnt[wave][lane + q * kWave] = 0;
-    __syncthreads();
-    if (valid && rank == 0) wcnt[wave][digit] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    lpos[j] = 0;
-    if (valid) {
-      uint32_t pos = run[digit] + rank;
-      for (uint32_t w = 0; w < wave; ++w) pos += wcnt[w][digit];
-      lpos[j] = pos;
-      sdigit[pos] = (uint8_t)digit;
+    for (int w = 0; w < kSortWaves; ++w) {
+      const uint32_t c = wcnt[w][tid];
+      wcnt[w][tid] = acc;
+      acc += c;
     }
-    __syncthreads();
-    uint32_t add = 0;
+  }
+  __syncthreads();
 #pragma unroll
-    for (int w = 0; w < kSortWaves; ++w) add += wcnt[w][tid];
-    run[tid] += add;
-    __syncthreads();  // every wave has read wcnt and run before the next iteration rewrites them
+  for (int j = 0; j < kSortPerThread; ++j) {
+    if (wave * kSeg + (uint32_t)j * kWave + lane < tn) {
+      const uint32_t pos = lbase[dg[j]] + wcnt[wave][dg[j]] + lpos[j];
+      lpos[j] = pos;
+      sdigit[pos] = (uint8_t)dg[j];
+    }
   }
   // write-out, one word at a time through the stage buffer (then the ids, when carried)
 #pragma unroll
@@ -192,7 +205,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kSortPerThread; ++j)
-      if ((uint32_t)j * kSortThreads + tid < tn) stage[lpos[j]] = q < KW ? kv[j][q] : (uint64_t)id[j];
+      if (wave * kSeg + (uint32_t)j * kWave + lane < tn) stage[lpos[j]] = q < KW ? kv[j][q] : (uint64_t)id[j];
     __syncthreads();
     for (uint32_t p = tid; p < tn; p += kSortThreads) {
       const uint32_t d = sdigit[p];
