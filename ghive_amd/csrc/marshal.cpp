@@ -28,7 +28,7 @@ struct ghive_staged {
 
 namespace {
 
-constexpr uint64_t kChunk = 32ull << 20;  // bytes per pinned staging buffer
+constexpr uint64_t kChunk = 8ull << 20;  // bytes per pinned staging buffer
 
 struct PinnedPipe {
   void *buf[2] = {nullptr, nullptr};
@@ -159,6 +159,7 @@ int ghive_gtables_stage(ghive_ctx *ctx, const ghive_gtables *g, ghive_staged **o
   }
   if (st == GHIVE_OK) st = ghive_sync(ctx);
   if (st != GHIVE_OK) {
+    (void)hipStreamSynchronize(ctx->stream);  // no copy may still write into buffers given back
     release(ctx, S);
     return st;
   }
