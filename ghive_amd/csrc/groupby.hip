@@ -200,7 +200,7 @@ __global__ __launch_bounds__(256) void k_gb_part_keys(KeySpec K, const uint32_t 
   for (; i < n; i += stride) {
     const uint64_t r = sel ? sel[i] : i;
     h[i] = mix64(pack_key(K, r));
-    ids[i] = (uint32_t)r;
+    if (ids) ids[i] = (uint32_t)r;
     if (vdata) v[i] = col_bits(vdata, vtype, r);
   }
 }
@@ -666,19 +666,21 @@ int groupby_partitioned(ghive_ctx *ctx, const KeySpec &K, const GroupTableDev &T
   {
     const uint64_t g = (n + 255) / 256, cap = (uint64_t)ctx->cus * 16;
     k_gb_part_keys<<<(uint32_t)(g > cap ? cap : g), 256, 0, ctx->stream>>>(
-        K, sel, n, k0.as<uint64_t>(), i0.as<uint32_t>(), vdata, vtype, val == 1 ? k0.as<uint64_t>() + n : nullptr);
+        K, sel, n, k0.as<uint64_t>(), val == 2 ? i0.as<uint32_t>() : nullptr, vdata, vtype,
+        val == 1 ? k0.as<uint64_t>() + n : nullptr);
     GH_LAUNCH(ctx);
   }
   std::vector<int> digits;  // the 8-bit digits of the hash (word 0) that cover its top pbits bits
   for (int d = pbits ? (64 - pbits) / 8 : 8; d < 8; ++d) digits.push_back(d);
   uint64_t *ks = k0.as<uint64_t>();
   uint32_t *ids = i0.as<uint32_t>();
+  const bool need_ids = val == 2;  // the row ids travel only when the aggregation reads columns at them
   if (KW == 2)
     GH_TRY(radix_sort_u64x2_ids(ctx, n, k0.as<uint64_t>(), k1.as<uint64_t>(), i0.as<uint32_t>(), i1.as<uint32_t>(),
-                                digits, &ks, &ids));
+                                digits, &ks, &ids, need_ids));
   else
     GH_TRY(radix_sort_u64_ids(ctx, n, k0.as<uint64_t>(), k1.as<uint64_t>(), i0.as<uint32_t>(), i1.as<uint32_t>(),
-                              digits, &ks, &ids));
+                              digits, &ks, &ids, need_ids));
   {
     const uint64_t g = (n + 255) / 256, cap = (uint64_t)ctx->cus * 16;
     k_gb_part_bounds<<<(uint32_t)(g > cap ? cap : g), 256, 0, ctx->stream>>>(ks, n, pbits, starts.as<uint64_t>());

@@ -40,11 +40,11 @@ int group_table_preinsert(ghive_ctx *ctx, const GroupTableDev &T, uint64_t key);
 // Stable LSD radix passes (sort.hip) over 8-bit `digits` of one uint64 word per row with a uint32
 // id; *keys_out / *ids_out point at the buffers holding the result.
 int radix_sort_u64_ids(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, uint32_t *i0, uint32_t *i1,
-                       const std::vector<int> &digits, uint64_t **keys_out, uint32_t **ids_out);
+                       const std::vector<int> &digits, uint64_t **keys_out, uint32_t **ids_out, bool ids = true);
 // The same over two uint64 words per row (word-major: word q of row i at q * n + i); the digits
 // index word 0 (0..7) and word 1 (8..15); both words travel with the id.
 int radix_sort_u64x2_ids(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, uint32_t *i0, uint32_t *i1,
-                         const std::vector<int> &digits, uint64_t **keys_out, uint32_t **ids_out);
+                         const std::vector<int> &digits, uint64_t **keys_out, uint32_t **ids_out, bool ids = true);
 // Key dictionary of rows sel[0..n) (sel NULL: rows 0..n): codes[r] (rows entries) = the slot of
 // row r's key; dict[slot] = (32 hash bits << 32) | the key's representative row, or all ones when
 // empty; *slots = dictionary size (a power of two >= 2 x capacity); *count = distinct keys.

@@ -256,13 +256,13 @@ static int radix_passes(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, 
 // (the group-by's hash partitioning). The ping-pong buffers hold n entries each; the sorted
 // words and ids end in whichever buffer the last pass wrote.
 int radix_sort_u64_ids(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, uint32_t *i0, uint32_t *i1,
-                       const std::vector<int> &digits, uint64_t **keys_out, uint32_t **ids_out) {
-  return radix_passes<1>(ctx, n, k0, k1, i0, i1, digits, ids_out, keys_out);
+                       const std::vector<int> &digits, uint64_t **keys_out, uint32_t **ids_out, bool ids) {
+  return radix_passes<1>(ctx, n, k0, k1, i0, i1, digits, ids_out, keys_out, ids);
 }
 
 int radix_sort_u64x2_ids(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, uint32_t *i0, uint32_t *i1,
-                         const std::vector<int> &digits, uint64_t **keys_out, uint32_t **ids_out) {
-  return radix_passes<2>(ctx, n, k0, k1, i0, i1, digits, ids_out, keys_out);
+                         const std::vector<int> &digits, uint64_t **keys_out, uint32_t **ids_out, bool ids) {
+  return radix_passes<2>(ctx, n, k0, k1, i0, i1, digits, ids_out, keys_out, ids);
 }
 
 // ---- STRING keys: dense ranks in Hive's string order -------------------------------------------
