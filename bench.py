@@ -388,6 +388,7 @@ def main():
     ap.add_argument("--no-lds-bits", action="store_true", help="keep dimension bitmaps in HBM/L2")
     ap.add_argument("--rows-per-thread", type=int, default=0, help="fused kernel rows/thread (4 or 8)")
     ap.add_argument("--generic", action="store_true", help="general fused kernel (no dense-probe form)")
+    ap.add_argument("--no-pipe", action="store_true", help="dense-probe kernel without the pipelined tile loop")
     ap.add_argument("--grid", type=int, default=0, help="fused kernel workgroups (0 = resident wave)")
     ap.add_argument("--block", type=int, default=0, help="fused kernel workgroup size (256/512/1024)")
     ap.add_argument("--cpu-sample", type=int, default=600_000_000)
@@ -424,9 +425,16 @@ def main():
         ctx.set_option("fragment.block", args.block)
     if args.generic:
         ctx.set_option("fragment.generic", 1)
+    if args.no_pipe:
+        ctx.set_option("fragment.pipe", 0)
     variant = ("late" if args.late else "early") + ("/hash" if args.no_dense else
                                                     ("/hbm-bitmap" if args.no_lds_bits else "/lds-bitmap"))
-    variant += "/generic-kernel" if (args.generic or args.late or args.no_dense) else "/dense-kernel"
+    if args.generic or args.late or args.no_dense:
+        variant += "/generic-kernel"
+    elif args.no_pipe or args.rows_per_thread == 8:
+        variant += "/dense-kernel"
+    else:
+        variant += "/pipe-kernel"
     info = ctx.device_info()
     log(f"device {info['name']} {info['arch']} CUs={info['compute_units']}")
     t0 = time.time()

@@ -60,6 +60,7 @@ struct ghive_tuning {
   int64_t frag_rows = 4;                // fact rows per thread per tile (4 or 8)
   int64_t frag_block = 256;             // fused-kernel workgroup size (256, 512 or 1024)
   int64_t frag_generic = 0;             // 1: always the general fused kernel (no dense-probe form)
+  int64_t frag_pipe = 1;                // 1: pipelined star kernel where group keys fit direct slots
   int64_t gb_lds_budget = 64 * 1024;    // bytes of LDS for the group-by's private table
   int64_t gb_partition = 1;             // high-cardinality form: 0 never, 1 by size, 2 always
 };

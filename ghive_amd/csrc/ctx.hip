@@ -307,6 +307,8 @@ int ghive_set_option(ghive_ctx *ctx, const char *name, int64_t value) {
     ctx->tune.frag_rows = value;
   } else if (k == "fragment.generic") {
     ctx->tune.frag_generic = value ? 1 : 0;
+  } else if (k == "fragment.pipe") {
+    ctx->tune.frag_pipe = value ? 1 : 0;
   } else if (k == "fragment.block") {
     if (value != 256 && value != 512 && value != 1024)
       return set_error(ctx, GHIVE_ERR_INVALID, "option %s: %lld (256, 512 or 1024)", name, (long long)value);
@@ -334,6 +336,7 @@ int ghive_get_option(ghive_ctx *ctx, const char *name, int64_t *value) {
   else if (k == "fragment.rows_per_thread") *value = ctx->tune.frag_rows;
   else if (k == "fragment.block") *value = ctx->tune.frag_block;
   else if (k == "fragment.generic") *value = ctx->tune.frag_generic;
+  else if (k == "fragment.pipe") *value = ctx->tune.frag_pipe;
   else if (k == "groupby.lds_budget") *value = ctx->tune.gb_lds_budget;
   else if (k == "groupby.partition") *value = ctx->tune.gb_partition;
   else return set_error(ctx, GHIVE_ERR_INVALID, "unknown option '%s'", name);

@@ -212,10 +212,72 @@ int fragment_plan(ghive_ctx *ctx, const ghive_fragment *f, int late, FragPlan &P
   for (int j = 0; j < F.njoins; ++j) dense_form = dense_form && F.joins[j].mode != kJoinHash;
   for (int k = 0; k < F.nkeys; ++k) dense_form = dense_form && F.key_src[k] == GHIVE_SRC_JOIN;
   P.dense_form = dense_form;
-  FragKernel kfn = dense_form ? (rows == 8 ? pick_dense_r8(F.njoins, F.naggs) : pick_dense_r4(F.njoins, F.naggs))
+  // direct group slots (pipelined kernel): every key is a join attribute, the product of the
+  // attributes' ranges is small, and no key packs to the all-ones pattern that marks an empty slot
+  bool pipe_form = dense_form && ctx->tune.frag_pipe && rows == 4;
+  if (pipe_form) {
+    uint64_t size = 1;
+    bool both_hold_minus1 = F.nkeys == 2;
+    for (int k = F.nkeys - 1; k >= 0; --k) {
+      const ghive_hash_table *h = f->joins[f->keys[k].index].table;
+      const uint64_t range = (uint64_t)(h->pay_max - h->pay_min) + 1;
+      F.dmin[k] = (int32_t)h->pay_min;
+      F.dmul[k] = (uint32_t)size;
+      both_hold_minus1 = both_hold_minus1 && h->pay_min <= -1 && h->pay_max >= -1;
+      if (range > kMaxDirectGroups || size * range > kMaxDirectGroups) {
+        pipe_form = false;
+        break;
+      }
+      size *= range;
+    }
+    if (both_hold_minus1) pipe_form = false;
+    if (P.lcap_log2 >= 0) pipe_form = false;  // group domains that fit LDS aggregate there
+    F.dsize = (uint32_t)size;
+  }
+  F.direct = pipe_form ? 1 : 0;
+  P.pipe_form = pipe_form;
+  int nh = 0;
+  if (pipe_form) {  // the pipelined kernel probes the LDS bitmaps first: order the joins so
+    FragJoinDev sorted[kMaxFragJoins];
+    int newpos[kMaxFragJoins], c = 0;
+    for (int pass = 0; pass < 2; ++pass)
+      for (int j = 0; j < F.njoins; ++j)
+        if ((F.joins[j].mode == kJoinLdsBits) == (pass == 0)) {
+          newpos[j] = c;
+          sorted[c++] = F.joins[j];
+        }
+    for (int j = 0; j < F.njoins; ++j) {
+      F.joins[j] = sorted[j];
+      nh += sorted[j].mode != kJoinLdsBits;
+    }
+    for (int k = 0; k < F.nkeys; ++k) F.key_idx[k] = (int8_t)newpos[F.key_idx[k]];
+  }
+  if (pipe_form) {
+    // per-wave survivor queues in the LDS the bitmaps leave over at their occupancy (LDS is
+    // allocated in 512-byte granules); at least 16 entries each
+    const size_t gran = 512, lds_max = 160 * 1024, waves = kFragThreads / 64;
+    const size_t entry = 4 * (2 + 2 * (size_t)F.naggs), base = P.lds_bytes;
+    const size_t base_up = std::max<size_t>((base + gran - 1) / gran * gran, gran);
+    const size_t blocks = std::min<size_t>(8, lds_max / base_up);
+    const size_t budget = lds_max / blocks / gran * gran;
+    size_t q = budget > base ? (budget - base) / (waves * entry) : 0;
+    q = std::min<size_t>(std::max<size_t>(q, 16), 512);
+    F.queue_cap = (uint32_t)q;
+    P.lds_bytes = base + waves * q * entry;
+  }
+  auto pick_pipe = [](int nj, int nhb, int na) {
+    switch (nj) {
+      case 1: return pick_pipe_j1(nhb, na);
+      case 2: return pick_pipe_j2(nhb, na);
+      case 3: return pick_pipe_j3(nhb, na);
+      default: return pick_pipe_j4(nhb, na);
+    }
+  };
+  FragKernel kfn = pipe_form ? pick_pipe(F.njoins, nh, F.naggs)
+                 : dense_form ? (rows == 8 ? pick_dense_r8(F.njoins, F.naggs) : pick_dense_r4(F.njoins, F.naggs))
                               : (rows == 8 ? pick_kernel_r8(F.njoins, F.naggs) : pick_kernel_r4(F.njoins, F.naggs));
   P.kernel = reinterpret_cast<const void *>(kfn);
-  P.threads = (uint32_t)ctx->tune.frag_block;
+  P.threads = pipe_form ? (uint32_t)kFragThreads : (uint32_t)ctx->tune.frag_block;
   P.rows = rows;
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, (int)P.threads, P.lds_bytes) != hipSuccess || per_cu < 1) {
@@ -249,6 +311,7 @@ int fragment_run(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_c
   for (uint32_t c = 0; c < frag->nkeys + frag->naggs && capacity; ++c)
     if (!out_cols[c].data) return set_error(ctx, GHIVE_ERR_INVALID, "fragment: output column %u has no buffer", c);
   uint64_t table_groups = capacity < P.group_bound ? capacity : P.group_bound;
+  if (P.pipe_form) table_groups = P.F.dsize;  // the slots are addressed by key, whatever the capacity
   DevBuf dkeys, dacc, dcnt, djoined, dstage;
   GH_TRY(group_table_alloc(ctx, P.T, table_groups ? table_groups : 1, dkeys, dacc, dcnt));
   const uint64_t tile_rows = (uint64_t)P.threads * (uint64_t)P.rows;
@@ -259,7 +322,10 @@ int fragment_run(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_c
   P.T.capacity = capacity;
   GH_TRY(djoined.alloc(ctx, sizeof(unsigned long long)));
   GH_TRY(fill_u64(ctx, djoined.as<uint64_t>(), 0, 1));
-  if (frag->nkeys == 0) GH_TRY(group_table_preinsert(ctx, P.T, 0ull));  // one group, even over no rows
+  if (frag->nkeys == 0) {  // one group, even over no rows
+    if (P.pipe_form) GH_TRY(fill_u64(ctx, P.T.keys, 0ull, 1));  // direct slot 0 holds key 0
+    else GH_TRY(group_table_preinsert(ctx, P.T, 0ull));
+  }
   GH_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
   GH_TRY(fragment_launch(ctx, P, djoined.as<unsigned long long>()));
   GH_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));

@@ -10,6 +10,7 @@ namespace ghive {
 
 constexpr int kMaxFragJoins = 4;
 constexpr int kMaxFragAggs = 4;
+constexpr uint32_t kMaxDirectGroups = 1u << 20;  // largest group-key domain given direct slots
 
 // How a join is probed inside the fused kernel.
 enum FragJoinMode : int32_t {
@@ -51,6 +52,14 @@ struct FragDev {
   int8_t key_src[2];
   int8_t key_idx[2];
   uint32_t lds_bitmap_words;  // words of LDS bitmaps (even, so the group table is 8-aligned)
+  // direct group slots (join-attribute keys of small ranges): the group of key values v[k] sits
+  // at slot sum_k (v[k] - dmin[k]) * dmul[k], no hashing; dsize slots are used
+  int32_t direct;
+  uint32_t dsize;
+  int32_t dmin[2];
+  uint32_t dmul[2];
+  uint32_t queue_cap;  // entries of each wave's LDS queue of survivors (global direct table)
+  uint32_t pad2;
   const int32_t *key_col[2];
   // NULL-padded copy of the partial last tile: one tile (blockDim.x * R rows) per streamed column,
   // joins first (slot j), then aggregate inputs (slots NJ + 2a and NJ + 2a + 1)
@@ -71,7 +80,8 @@ struct FragPlan {
   uint32_t grid;
   uint32_t threads;  // workgroup size
   int rows;          // fact rows per lane per tile
-  bool dense_form;   // k_fragment_dense (else the general k_fragment)
+  bool dense_form;   // k_fragment_dense or k_fragment_pipe (else the general k_fragment)
+  bool pipe_form;    // k_fragment_pipe (direct group slots, next tile's loads in flight)
   int late;
   const void *kernel;
 };

@@ -1,5 +1,6 @@
-// The fused fragment kernel template (see fragment.hip for the design). Instantiated in
-// fragment_r4.hip and fragment_r8.hip so that the instantiations compile in parallel.
+// The fused fragment kernel templates (see fragment.hip for the design). Instantiated in
+// fragment_r4.hip, fragment_r8.hip, fragment_dense_r4.hip, fragment_dense_r8.hip and
+// fragment_pipe.hip so that the instantiations compile in parallel.
 #pragma once
 
 #include "agg.hpp"
@@ -348,6 +349,350 @@ __global__ __launch_bounds__(kFragMaxThreads) void k_fragment_dense(FragDev F, G
   if (lane_id() == 0 && s) atomicAdd(joined, s);
 }
 
+// ---- the pipelined star-shape kernel ---------------------------------------------------------
+// Same shape as k_fragment_dense (every join through a bitmap, group keys from join attributes),
+// for group domains too large for a workgroup's LDS table but small enough to be addressed
+// directly (FragDev::direct: no hashing, no returning atomic). Vector memory operations complete
+// in issue order, so a wave that waits for one load also waits for every older one. One tile step
+// is ordered so that the next tile's column loads are in flight while the current tile waits for
+// everything else it needs:
+//   1. wait for the current tile's columns (and for any global updates of the previous step);
+//      apply the wave's survivor queue if it is half full (see pipe_emit);
+//   2. LDS bitmap probes;
+//   3. HBM bitmap words, loaded only by lanes whose rows are still alive;
+//   4. the next tile's columns, into the other register set;
+//   5. HBM bitmap hits: waits for 3 only;
+//   6. survivors into the wave's LDS queue.
+// Global writes are issued only in steps 1 and 6, and step 1 waits for all memory operations:
+// with writes pending as well as loads, loads can no longer be counted in order, so step 5 would
+// wait for the next tile's columns too.
+constexpr int kPipeRows = 4;
+// s_waitcnt immediate of "vmcnt(0)" on gfx9-family chips (expcnt and lgkmcnt left at their maxima);
+// OR-ing a count below 16 into the low bits gives "vmcnt(count)"
+constexpr int kWaitVmcnt0 = 0x0f70;
+static_assert(kMaxFragJoins + 2 * kMaxFragAggs < 16, "the prefetch's load count must fit vmcnt's low bits");
+
+template <int NJ, int NA>
+struct PipeTile {
+  int32_t c[NJ + 2 * NA][kPipeRows];  // joins' fact keys, then each aggregate's two inputs
+};
+
+template <int NJ, int NA>
+__device__ __forceinline__ void pipe_load(const int32_t *const (&col)[NJ + 2 * NA], uint64_t off, PipeTile<NJ, NA> &t) {
+#pragma unroll
+  for (int i = 0; i < NJ + 2 * NA; ++i) load_rows<kPipeRows>(col[i] + off, t.c[i]);
+}
+
+// Steps 2-3. Joins 0 .. NJ-NH-1 are probed in LDS, joins NJ-NH .. NJ-1 through HBM bitmaps (the
+// plan orders them so). Returns the rows alive after the LDS probes whose keys are in range of
+// every HBM bitmap; sh[q] packs the bit positions (5 bits per HBM join) that step 5 tests.
+template <int NJ, int NH, int NA>
+__device__ __forceinline__ uint32_t pipe_probe_issue(const FragDev &F, const uint32_t *lbits, const PipeTile<NJ, NA> &t,
+                                                     uint32_t (&w)[NH > 0 ? NH : 1][kPipeRows],
+                                                     uint32_t (&sh)[kPipeRows]) {
+  constexpr int R = kPipeRows, NL = NJ - NH;
+  uint32_t alive = (1u << R) - 1;
+#pragma unroll
+  for (int j = 0; j < NL; ++j) {
+    const FragJoinDev &J = F.joins[j];
+    const uint32_t kmin = (uint32_t)J.key_min, span = J.span;
+    const uint32_t *bits = lbits + J.lds_word_off;
+    uint32_t u[R], x[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      u[q] = (uint32_t)t.c[j][q] - kmin;
+      x[q] = bits[(u[q] < span ? u[q] : span) >> 5];
+    }
+    uint32_t hit = 0;
+#pragma unroll
+    for (int q = 0; q < R; ++q) hit |= ((uint32_t)(u[q] <= span) & (x[q] >> (u[q] & 31u))) << q;
+    alive &= hit;
+  }
+#pragma unroll
+  for (int q = 0; q < R; ++q) sh[q] = 0;
+#pragma unroll
+  for (int h = 0; h < NH; ++h) {  // out-of-range keys of the HBM bitmaps leave first
+    const FragJoinDev &J = F.joins[NL + h];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const uint32_t u = (uint32_t)t.c[NL + h][q] - (uint32_t)J.key_min;
+      alive &= ~((uint32_t)(u > J.span) << q);
+      sh[q] |= (u & 31u) << (5 * h);
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < NH; ++h) {
+    const FragJoinDev &J = F.joins[NL + h];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      w[h][q] = 0;
+      if ((alive >> q) & 1u)  // exec-masked: a dead row issues no request
+        w[h][q] = J.bitmap[((uint32_t)t.c[NL + h][q] - (uint32_t)J.key_min) >> 5];
+    }
+  }
+  return alive;
+}
+
+// Step 5: the HBM bitmap tests.
+template <int NH>
+__device__ __forceinline__ uint32_t pipe_probe_finish(uint32_t alive, const uint32_t (&w)[NH > 0 ? NH : 1][kPipeRows],
+                                                      const uint32_t (&sh)[kPipeRows]) {
+#pragma unroll
+  for (int h = 0; h < NH; ++h) {
+    uint32_t hit = 0;
+#pragma unroll
+    for (int q = 0; q < kPipeRows; ++q) hit |= ((w[h][q] >> ((sh[q] >> (5 * h)) & 31u)) & 1u) << q;
+    alive &= hit;
+  }
+  return alive;
+}
+
+// Step 6. Each wave appends its survivors -- the offsets of their keys in the key joins' dense
+// attribute arrays and their raw aggregate inputs -- to a small queue of its own in LDS. The queue
+// is applied to the global direct table, one entry per lane, at the start of a tile step once it
+// is half full (and once at the end): the attribute loads (one per survivor, not one per row
+// alive after the LDS probes) and the fire-and-forget atomics then make a wave wait a few times
+// in its life rather than on every tile, and run with every lane busy. A tile with more
+// survivors than the queue has room for sends them straight to the global table.
+struct PipeState {
+  const FragDev *F;
+  const GroupTableDev *T;
+  uint32_t *qoff;       // this wave's queue: 2 rows of qcap key offsets, then
+  int32_t *qin;         // 2 * NA rows of qcap aggregate inputs
+  uint32_t qcap;
+  uint32_t qn;          // entries queued (wave-uniform)
+  uint32_t njoined;
+  const int32_t *kdense[2];  // dense attribute array of each group key's join
+  int32_t kmin[2];           // that join's smallest key
+};
+
+__device__ __forceinline__ void pipe_apply_global(int kind, uint64_t *w0, uint64_t *w1, int64_t val) {
+  typedef __attribute__((address_space(1))) unsigned long long g_u64;
+  typedef __attribute__((address_space(1))) long long g_i64;
+  g_u64 *a0 = (g_u64 *)w0, *a1 = (g_u64 *)w1;
+  if (kind == AK_CNT) {
+    __hip_atomic_fetch_add(a0, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  if (kind == AK_SUMI) __hip_atomic_fetch_add(a0, (unsigned long long)val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else if (kind == AK_MINI) __hip_atomic_fetch_min((g_i64 *)w0, (long long)val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else __hip_atomic_fetch_max((g_i64 *)w0, (long long)val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_add(a1, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// value and validity of an aggregate for the inputs (x, y)
+__device__ __forceinline__ bool pipe_agg_value(int expr, int32_t x, int32_t y, int64_t &val) {
+  if (expr == GHIVE_EXPR_ONE) {
+    val = 1;
+    return true;
+  }
+  val = x;
+  if (expr != GHIVE_EXPR_MUL) return x != kNullI32;
+  val = (int64_t)((uint64_t)(int64_t)x * (uint64_t)(int64_t)y);
+  return x != kNullI32 && y != kNullI32;
+}
+
+// one surviving row into the global table: key attribute offsets off[k], aggregate inputs x, y
+template <int NA>
+__device__ __forceinline__ void pipe_global_row(const PipeState &S, const uint32_t (&off)[2], const int32_t (&x)[NA],
+                                                const int32_t (&y)[NA]) {
+  const FragDev &F = *S.F;
+  const GroupTableDev &T = *S.T;
+  int32_t v[2] = {0, 0};
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+    if (k < F.nkeys) v[k] = S.kdense[k][off[k]];
+  uint32_t g = 0;
+  uint64_t key = 0;
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+    if (k < F.nkeys) {
+      g += (uint32_t)(v[k] - F.dmin[k]) * F.dmul[k];
+      key |= (uint64_t)(uint32_t)v[k] << (32 * k);
+    }
+  *(__attribute__((address_space(1))) uint64_t *)(T.keys + g) = key;  // same key for every writer
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    int64_t val;
+    if (pipe_agg_value(F.aggs[a].expr, x[a], y[a], val))
+      pipe_apply_global(T.aggs[a].kind, acc_word(T, T.aggs[a].word0, (uint64_t)g),
+                        acc_word(T, T.aggs[a].word0 + 1, (uint64_t)g), val);
+  }
+}
+
+template <int NA>
+__device__ __forceinline__ void pipe_queue_flush(PipeState &S) {
+  for (uint32_t i = lane_id(); i < S.qn; i += kWave) {
+    uint32_t off[2];
+    int32_t x[NA], y[NA];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) off[k] = S.qoff[(size_t)k * S.qcap + i];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      x[a] = S.qin[(size_t)(2 * a) * S.qcap + i];
+      y[a] = S.qin[(size_t)(2 * a + 1) * S.qcap + i];
+    }
+    pipe_global_row<NA>(S, off, x, y);
+  }
+  S.qn = 0;
+}
+
+// offset of row q's key k in its join's dense attribute array
+template <int NJ, int NA>
+__device__ __forceinline__ uint32_t pipe_key_off(const PipeState &S, const PipeTile<NJ, NA> &t, int k, int q) {
+  int32_t fk = 0;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j)
+    if (S.F->key_idx[k] == j) fk = t.c[j][q];
+  return (uint32_t)fk - (uint32_t)S.kmin[k];
+}
+
+template <int NJ, int NA>
+__device__ __forceinline__ void pipe_emit(PipeState &S, uint32_t mask, const PipeTile<NJ, NA> &t) {
+  const FragDev &F = *S.F;
+  // this lane's survivors (0..4) and the number queued by the lanes below it: a prefix sum of the
+  // 3-bit counts from one ballot per bit
+  const uint32_t c = (uint32_t)__builtin_popcount(mask);
+  uint32_t total = 0, below = 0;
+#pragma unroll
+  for (int bit = 0; bit < 3; ++bit) {
+    const uint64_t b = __ballot((c >> bit) & 1u);
+    total += (uint32_t)__popcll(b) << bit;
+    below += __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0u)) << bit;
+  }
+  S.njoined += c;
+  if (S.qn + total > S.qcap) {  // more rows than the queue has room for: straight to the global table
+#pragma unroll
+    for (int q = 0; q < kPipeRows; ++q) {
+      if (!((mask >> q) & 1u)) continue;
+      uint32_t off[2] = {0, 0};
+      int32_t x[NA], y[NA];
+#pragma unroll
+      for (int k = 0; k < 2; ++k)
+        if (k < F.nkeys) off[k] = pipe_key_off<NJ, NA>(S, t, k, q);
+#pragma unroll
+      for (int a = 0; a < NA; ++a) {
+        x[a] = t.c[NJ + 2 * a][q];
+        y[a] = t.c[NJ + 2 * a + 1][q];
+      }
+      pipe_global_row<NA>(S, off, x, y);
+    }
+    return;
+  }
+  uint32_t i = S.qn + below;
+#pragma unroll
+  for (int q = 0; q < kPipeRows; ++q) {
+    if (!((mask >> q) & 1u)) continue;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) S.qoff[(size_t)k * S.qcap + i] = k < F.nkeys ? pipe_key_off<NJ, NA>(S, t, k, q) : 0u;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      S.qin[(size_t)(2 * a) * S.qcap + i] = t.c[NJ + 2 * a][q];
+      S.qin[(size_t)(2 * a + 1) * S.qcap + i] = t.c[NJ + 2 * a + 1][q];
+    }
+    ++i;
+  }
+  S.qn += total;
+}
+
+// One tile step: `cur` holds the tile, `nxt` receives the tile at element offset next_off (the
+// tile itself when it is the workgroup's last); prefetch = false for the staged partial tile.
+template <int NJ, int NH, int NA>
+__device__ __forceinline__ void pipe_step(PipeState &S, const uint32_t *lbits, const int32_t *const (&col)[NJ + 2 * NA],
+                                          uint64_t next_off, const PipeTile<NJ, NA> &cur, PipeTile<NJ, NA> &nxt,
+                                          bool prefetch) {
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+  // the queue is applied here, where the other register set is free, once it is half full
+  if (S.qn > S.qcap / 2) pipe_queue_flush<NA>(S);
+  uint32_t w[NH > 0 ? NH : 1][kPipeRows], sh[kPipeRows];
+  uint32_t alive = pipe_probe_issue<NJ, NH, NA>(*S.F, lbits, cur, w, sh);
+  if (prefetch) {
+    pipe_load<NJ, NA>(col, next_off, nxt);
+    // everything older than the next tile's loads: the bitmap words and payloads of this tile
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0 | (NJ + 2 * NA));
+  } else {
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+  }
+  alive = pipe_probe_finish<NH>(alive, w, sh);
+  if (__ballot(alive != 0)) pipe_emit<NJ, NA>(S, alive, cur);
+}
+
+// NJ joins of which the last NH probe HBM bitmaps, NA aggregates.
+template <int NJ, int NH, int NA>
+__global__ __launch_bounds__(kFragThreads) void k_fragment_pipe(FragDev F, GroupTableDev T, int lcap_log2,
+                                                             unsigned long long *joined) {
+  constexpr int R = kPipeRows;
+  extern __shared__ uint64_t lds64[];
+  uint32_t *lbits = reinterpret_cast<uint32_t *>(lds64);
+  (void)lcap_log2;  // no LDS group table: the direct slots live in the global table
+  PipeState S;
+  S.F = &F;
+  S.T = &T;
+  // queues after the bitmaps, one per wave: 2 rows of qcap key offsets, 2 * NA rows of inputs
+  S.qcap = F.queue_cap;
+  S.qoff = reinterpret_cast<uint32_t *>(lds64 + F.lds_bitmap_words / 2) +
+           (size_t)(threadIdx.x / kWave) * F.queue_cap * (2 + 2 * NA);
+  S.qin = reinterpret_cast<int32_t *>(S.qoff + 2 * (size_t)F.queue_cap);
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const FragJoinDev &J = F.joins[k < F.nkeys ? F.key_idx[k] : 0];
+    S.kdense[k] = J.dense;
+    S.kmin[k] = J.key_min;
+  }
+  S.qn = 0;
+  S.njoined = 0;
+#pragma unroll
+  for (int j = 0; j < NJ; ++j)
+    if (F.joins[j].mode == kJoinLdsBits)
+      for (uint32_t i = threadIdx.x; i < F.joins[j].nwords; i += blockDim.x)
+        lbits[F.joins[j].lds_word_off + i] = F.joins[j].bitmap[i];
+  __syncthreads();
+
+  const int32_t *col[NJ + 2 * NA];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) col[j] = F.joins[j].fk;
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {  // both inputs always valid here (the plan fills unused ones)
+    col[NJ + 2 * a] = F.aggs[a].a;
+    col[NJ + 2 * a + 1] = F.aggs[a].b;
+  }
+  const uint64_t tile_rows = (uint64_t)blockDim.x * R;
+  const uint64_t n = F.n;
+  const uint64_t ntiles = (n + tile_rows - 1) / tile_rows;
+  const uint64_t tile_begin = ntiles * blockIdx.x / gridDim.x;
+  const uint64_t tile_end = ntiles * (blockIdx.x + 1) / gridDim.x;
+  const uint64_t full_tiles = n / tile_rows;
+  const uint64_t full_end = tile_end < full_tiles ? tile_end : full_tiles;
+  const uint64_t lane_row = (uint64_t)threadIdx.x * R;
+  PipeTile<NJ, NA> ta, tb;
+  if (tile_begin < full_end) {
+    // two steps per iteration with the register sets' roles swapped, so that no tile is copied
+    pipe_load<NJ, NA>(col, tile_begin * tile_rows + lane_row, ta);
+    for (uint64_t tile = tile_begin;; tile += 2) {
+      const uint64_t n1 = tile + 1 < full_end ? tile + 1 : tile;
+      pipe_step<NJ, NH, NA>(S, lbits, col, n1 * tile_rows + lane_row, ta, tb, true);
+      if (tile + 1 >= full_end) break;
+      const uint64_t n2 = tile + 2 < full_end ? tile + 2 : tile + 1;
+      pipe_step<NJ, NH, NA>(S, lbits, col, n2 * tile_rows + lane_row, tb, ta, true);
+      if (tile + 2 >= full_end) break;
+    }
+  }
+  if (tile_end > full_end) {  // uniform: this workgroup owns the table's partial last tile
+    FragBlock B;
+    B.tile_rows = tile_rows;
+    frag_stage_tail<NJ, NA>(F, B, full_tiles * tile_rows);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NJ + 2 * NA; ++i) load_rows<R>(F.stage + (uint64_t)i * tile_rows + lane_row, ta.c[i]);
+    pipe_step<NJ, NH, NA>(S, lbits, col, 0, ta, tb, false);
+  }
+  pipe_queue_flush<NA>(S);
+  unsigned long long s = S.njoined;
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) s += __shfl_xor(s, d, kWave);
+  if (lane_id() == 0 && s) atomicAdd(joined, s);
+}
+
 using FragKernel = void (*)(FragDev, GroupTableDev, int, unsigned long long *);
 
 FragKernel pick_kernel_r4(int nj, int na);
@@ -355,5 +700,10 @@ FragKernel pick_kernel_r8(int nj, int na);
 // k_fragment_dense instantiations (1 <= nj)
 FragKernel pick_dense_r4(int nj, int na);
 FragKernel pick_dense_r8(int nj, int na);
+// k_fragment_pipe instantiations (1 <= nj, 0 <= nh <= nj); one file per join count
+FragKernel pick_pipe_j1(int nh, int na);
+FragKernel pick_pipe_j2(int nh, int na);
+FragKernel pick_pipe_j3(int nh, int na);
+FragKernel pick_pipe_j4(int nh, int na);
 
 }  // namespace ghive

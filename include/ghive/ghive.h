@@ -444,7 +444,8 @@ int ghive_fragment_execute_late(ghive_ctx *ctx, const ghive_fragment *frag, ghiv
  * small dimension bitmaps), "fragment.dense" (0/1: probe compact-key dimensions through their
  * bitmap + payload array instead of the hash table), "fragment.rows_per_thread" (4 or 8),
  * "fragment.block" (workgroup size 256, 512 or 1024), "fragment.generic" (0/1: always the general
- * fused kernel, not its dense-probe form), "groupby.lds_budget" (bytes). */
+ * fused kernel, not its dense-probe form), "fragment.pipe" (0/1: the pipelined dense-probe kernel
+ * with direct group slots where the group keys' ranges allow it), "groupby.lds_budget" (bytes). */
 int ghive_set_option(ghive_ctx *ctx, const char *name, int64_t value);
 int ghive_get_option(ghive_ctx *ctx, const char *name, int64_t *value);
 

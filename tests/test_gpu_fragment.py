@@ -74,20 +74,23 @@ def _star_oracle(orc, T, rows=None):
 
 
 _DEFAULTS = {"fragment.dense": 1, "fragment.lds_bits": 1, "fragment.generic": 0,
-             "fragment.rows_per_thread": 4, "fragment.block": 256}
+             "fragment.rows_per_thread": 4, "fragment.block": 256, "fragment.pipe": 1}
 
 
 @pytest.mark.parametrize("opts", [
-    {},                                                        # dense-probe kernel, LDS bitmaps
-    {"fragment.lds_bits": 0},                                  # dense-probe kernel, HBM bitmaps
+    {},                                                        # pipelined kernel, LDS bitmaps
+    {"fragment.lds_bits": 0},                                  # pipelined kernel, HBM bitmaps
+    {"fragment.pipe": 0},                                      # dense-probe kernel, LDS bitmaps
+    {"fragment.pipe": 0, "fragment.lds_bits": 0},              # dense-probe kernel, HBM bitmaps
     {"fragment.generic": 1},                                   # general kernel, LDS bitmaps
     {"fragment.generic": 1, "fragment.lds_bits": 0},           # general kernel, HBM bitmaps
     {"fragment.dense": 0},                                     # hash-table probes
     {"fragment.rows_per_thread": 8, "fragment.block": 512},
-    {"fragment.block": 1024},
-], ids=["dense-lds", "dense-hbm", "generic-lds", "generic-hbm", "hash", "r8-b512", "b1024"])
+    {"fragment.block": 1024, "fragment.pipe": 0},
+], ids=["pipe-lds", "pipe-hbm", "dense-lds", "dense-hbm", "generic-lds", "generic-hbm", "hash", "r8-b512",
+        "b1024"])
 def test_star_fragment_vs_oracle_20m(ctx, orc, opts):
-    """Both fused kernels, every probe mode (LDS bitmap, HBM bitmap, hash table), both load orders
+    """Every fused kernel, every probe mode (LDS bitmap, HBM bitmap, hash table), both load orders
     and several tile shapes agree with the oracle bit for bit; 20M+1 rows leaves a partial tile."""
     from ghive_amd import workloads as W
     T = W.star_tables(ctx, 20_000_001)
@@ -107,26 +110,46 @@ def test_star_fragment_vs_oracle_20m(ctx, orc, opts):
     assert joined == want["joined"]
 
 
-@pytest.mark.parametrize("rows", [1, 3, 1023, 1024, 1025, 4096, 262_144, 262_147])
+@pytest.mark.parametrize("rows", [1, 3, 1023, 1024, 1025, 2048, 3072, 4096, 262_144, 262_147])
 def test_star_fragment_tile_boundaries(ctx, orc, rows):
     """Tables smaller than, equal to and just past whole tiles (1024 rows at the default shape):
-    the partial last tile goes through the staging copy, whole tiles do not."""
+    the partial last tile goes through the staging copy, whole tiles do not; odd and even numbers
+    of whole tiles per workgroup end the pipelined loop in either register set."""
     from ghive_amd import ops, workloads as W
     T = W.star_tables(ctx, 300_000)
     hd, hs, hp = W.star_hash_tables(ctx, T)
     part = ops.Table([c[:rows] for c in T.fact.columns])
     got = _gpu_dict(W.star_fragment(part, hd, hs, hp).execute(ctx, capacity=8192), 2)
-    ctx.set_option("fragment.generic", 1)
+    res = {}
+    for opt in ("fragment.generic", "fragment.pipe"):
+        ctx.set_option(opt, 1 - _DEFAULTS[opt])
+        try:
+            res[opt] = _gpu_dict(W.star_fragment(part, hd, hs, hp).execute(ctx, capacity=8192), 2)
+        finally:
+            ctx.set_option(opt, _DEFAULTS[opt])
+    assert got == res["fragment.generic"] == res["fragment.pipe"] == _orc_dict(_star_oracle(orc, T, rows))
+
+
+@pytest.mark.parametrize("grid", [1, 3, 7])
+def test_star_fragment_few_workgroups(ctx, orc, grid):
+    """A handful of workgroups, each streaming hundreds of tiles through the pipelined loop."""
+    from ghive_amd import workloads as W
+    T = W.star_tables(ctx, 1_000_037)
+    hd, hs, hp = W.star_hash_tables(ctx, T)
+    ctx.set_option("fragment.grid", grid)
     try:
-        generic = _gpu_dict(W.star_fragment(part, hd, hs, hp).execute(ctx, capacity=8192), 2)
+        got = _gpu_dict(W.star_fragment(T.fact, hd, hs, hp).execute(ctx, capacity=8192), 2)
+        joined = ctx.stage_times()["rows_joined"]
     finally:
-        ctx.set_option("fragment.generic", 0)
-    assert got == generic == _orc_dict(_star_oracle(orc, T, rows))
+        ctx.set_option("fragment.grid", 0)
+    want = _star_oracle(orc, T)
+    assert got == _orc_dict(want)
+    assert joined == want["joined"]
 
 
-@pytest.mark.parametrize("generic", [0, 1])
-def test_star_fragment_count_min_max(ctx, orc, generic):
-    """COUNT(*), SUM, MIN and MAX of single columns (unused aggregate inputs) in both kernels."""
+@pytest.mark.parametrize("generic,pipe", [(0, 1), (0, 0), (1, 0)])
+def test_star_fragment_count_min_max(ctx, orc, generic, pipe):
+    """COUNT(*), SUM, MIN and MAX of single columns (unused aggregate inputs) in every kernel."""
     from ghive_amd import ops, workloads as W
     T = W.star_tables(ctx, 2_000_003)
     hd, hs, hp = W.star_hash_tables(ctx, T)
@@ -135,10 +158,12 @@ def test_star_fragment_count_min_max(ctx, orc, generic):
     frag = ops.Fragment(T.fact, joins=[(W.F_DATE, hd), (W.F_SUPP, hs), (W.F_PART, hp)],
                         keys=[(abi.SRC_JOIN, 0), (abi.SRC_JOIN, 2)], aggs=aggs)
     ctx.set_option("fragment.generic", generic)
+    ctx.set_option("fragment.pipe", pipe)
     try:
         got = _gpu_dict(frag.execute(ctx, capacity=8192), 2)
     finally:
         ctx.set_option("fragment.generic", 0)
+        ctx.set_option("fragment.pipe", 1)
     fact = [to_np(c) for c in T.fact.columns]
     d, s_, p = ([to_np(c) for c in t.columns] for t in (T.date, T.supplier, T.part))
     dims = [
@@ -215,3 +240,135 @@ def test_fragment_edges(ctx):
     mis = ops.Table([c[1:] for c in T.fact.columns])
     with pytest.raises(GhiveError):
         W.small_fragment(mis, ht).execute(ctx, capacity=64)
+
+
+def _orc_dict_nulls(res):
+    return {tuple(int(x) for x in k): tuple(abi.LONG_NULL if nl else int(x) for x, nl in zip(v, nn))
+            for k, v, nn in zip(res["keys"], res["aggs"], res["nulls"])}
+
+
+@pytest.mark.parametrize("pipe", [1, 0])
+def test_small_fragment_lds_slots_all_aggregates(ctx, orc, pipe):
+    """25 groups (the workgroup-private LDS table, so the dense-probe kernel whatever the pipelined
+    kernel's setting): COUNT(*), SUM, MIN and MAX of a product."""
+    from ghive_amd import ops, workloads as W
+    T = W.small_tables(ctx, 3_000_017, seed=7)
+    ht = W.small_dim_table(ctx, T.dim)
+    aggs = [(abi.AGG_CNT, abi.EXPR_ONE, 0, 0), (abi.AGG_SUM, abi.EXPR_COL, 2, 0),
+            (abi.AGG_MIN, abi.EXPR_COL, 1, 0), (abi.AGG_MAX, abi.EXPR_MUL, 1, 2)]
+    frag = ops.Fragment(T.fact, joins=[(0, ht)], keys=[(abi.SRC_JOIN, 0)], aggs=aggs)
+    ctx.set_option("fragment.pipe", pipe)
+    try:
+        got = _gpu_dict(frag.execute(ctx, capacity=64), 1)
+        joined = ctx.stage_times()["rows_joined"]
+    finally:
+        ctx.set_option("fragment.pipe", 1)
+    fact = [to_np(c) for c in T.fact.columns]
+    dim = [to_np(c) for c in T.dim.columns]
+    spec = orc.FragmentSpec(fact, [dict(columns=dim, key_col=0, payload_col=2, fact_key_col=0,
+                                        filter=([abi.make_pred(abi.FILTER_EQ, 1, [3], data_type=2)], 0))],
+                            keys=[(abi.SRC_JOIN, 0)], aggs=aggs)
+    want = spec.run(nthreads=8)
+    assert got == _orc_dict_nulls(want)
+    assert joined == want["joined"]
+    assert len(got) == 25
+
+
+@pytest.mark.parametrize("pipe", [1, 0])
+@pytest.mark.parametrize("neg", ["one", "both"])
+def test_fragment_negative_keys_and_nulls(ctx, orc, pipe, neg):
+    """Group keys from two joins with negative attribute values and a domain too large for the
+    LDS table (direct slots offset by each key's minimum; when both ranges hold -1 a key could pack
+    to the empty-slot pattern and the hashed table is used), NULL foreign keys and NULL aggregate
+    inputs, keys outside the dimension ranges. Most rows survive, so tiles overflow the pipelined
+    kernel's survivor queues and go straight to the global table."""
+    import torch
+    from ghive_amd import ops
+    rng = np.random.default_rng(11)
+    n = 700_001
+    ka = np.arange(1, 301, dtype=np.int32)
+    pa = rng.integers(-3, 41, 300).astype(np.int32)
+    kb = np.arange(10, 210, dtype=np.int32)
+    pb = rng.integers(-1 if neg == "both" else 0, 31, 200).astype(np.int32)
+    cols = [rng.integers(0, 320, n), rng.integers(0, 225, n), rng.integers(-50, 51, n), rng.integers(1, 1001, n)]
+    cols = [c.astype(np.int32) for c in cols]
+    for c in cols:
+        c[rng.random(n) < 0.01] = abi.INT_NULL
+    dev = f"cuda:{ctx.device}"
+    tcols = [torch.from_numpy(c).to(dev) for c in cols]
+    fact = ops.Table(tcols)
+    dka, dpa, dkb, dpb = (torch.from_numpy(x).to(dev) for x in (ka, pa, kb, pb))
+    ha = ops.HashTable(ctx, dka, payload=dpa)
+    hb = ops.HashTable(ctx, dkb, payload=dpb)
+    aggs = [(abi.AGG_SUM, abi.EXPR_MUL, 2, 3), (abi.AGG_CNT, abi.EXPR_ONE, 0, 0),
+            (abi.AGG_MIN, abi.EXPR_COL, 2, 0), (abi.AGG_MAX, abi.EXPR_COL, 3, 0)]
+    frag = ops.Fragment(fact, joins=[(0, ha), (1, hb)], keys=[(abi.SRC_JOIN, 0), (abi.SRC_JOIN, 1)], aggs=aggs)
+    ctx.set_option("fragment.pipe", pipe)
+    try:
+        got = _gpu_dict(frag.execute(ctx, capacity=4096), 2)
+        joined = ctx.stage_times()["rows_joined"]
+    finally:
+        ctx.set_option("fragment.pipe", 1)
+    dims = [dict(columns=[ka, pa], key_col=0, payload_col=1, fact_key_col=0),
+            dict(columns=[kb, pb], key_col=0, payload_col=1, fact_key_col=1)]
+    want = orc.FragmentSpec(cols, dims, keys=[(abi.SRC_JOIN, 0), (abi.SRC_JOIN, 1)], aggs=aggs).run(nthreads=8)
+    assert got == _orc_dict_nulls(want)
+    assert joined == want["joined"]
+    assert any(k[0] < 0 for k in got)
+
+
+@pytest.mark.parametrize("pipe", [1, 0])
+def test_fragment_keyless_star(ctx, orc, pipe):
+    """No group keys: one group (key 0) holds the global aggregates."""
+    from ghive_amd import ops, workloads as W
+    T = W.star_tables(ctx, 1_500_001)
+    hd, hs, hp = W.star_hash_tables(ctx, T)
+    aggs = [(abi.AGG_SUM, abi.EXPR_MUL, 5, 6), (abi.AGG_CNT, abi.EXPR_ONE, 0, 0)]
+    frag = ops.Fragment(T.fact, joins=[(W.F_DATE, hd), (W.F_SUPP, hs), (W.F_PART, hp)], keys=[], aggs=aggs)
+    ctx.set_option("fragment.pipe", pipe)
+    try:
+        outs = frag.execute(ctx, capacity=4)
+    finally:
+        ctx.set_option("fragment.pipe", 1)
+    got = tuple(int(to_np(o)[0]) for o in outs)
+    fact = [to_np(c) for c in T.fact.columns]
+    d, s_, p = ([to_np(c) for c in t.columns] for t in (T.date, T.supplier, T.part))
+    dims = [
+        dict(columns=d, key_col=0, payload_col=1, fact_key_col=0,
+             filter=([abi.make_pred(abi.FILTER_EQ_RANGE, 1, [1992, 1993], data_type=2)], 0)),
+        dict(columns=s_, key_col=0, payload_col=1, fact_key_col=2,
+             filter=([abi.make_pred(abi.FILTER_EQ, 1, [1], data_type=2)], 0)),
+        dict(columns=p, key_col=0, payload_col=2, fact_key_col=3,
+             filter=([abi.make_pred(abi.FILTER_EQ, 1, [1], data_type=2)], 0)),
+    ]
+    want = orc.FragmentSpec(fact, dims, keys=[], aggs=aggs).run(nthreads=8)
+    assert len(outs[0]) == 1
+    assert got == tuple(int(x) for x in want["aggs"][0])
+
+
+@pytest.mark.parametrize("naggs", [1, 2, 4])
+def test_star_fragment_queue_sizes(ctx, orc, naggs):
+    """The pipelined kernel's per-wave survivor queues shrink as aggregates are added (the LDS the
+    bitmaps leave is fixed); every size gives the oracle's groups."""
+    from ghive_amd import ops, workloads as W
+    T = W.star_tables(ctx, 3_000_011)
+    hd, hs, hp = W.star_hash_tables(ctx, T)
+    pool = [(abi.AGG_SUM, abi.EXPR_MUL, 5, 6), (abi.AGG_CNT, abi.EXPR_ONE, 0, 0),
+            (abi.AGG_MIN, abi.EXPR_COL, 4, 0), (abi.AGG_MAX, abi.EXPR_MUL, 4, 6)]
+    aggs = pool[:naggs]
+    frag = ops.Fragment(T.fact, joins=[(W.F_DATE, hd), (W.F_SUPP, hs), (W.F_PART, hp)],
+                        keys=[(abi.SRC_JOIN, 0), (abi.SRC_JOIN, 2)], aggs=aggs)
+    got = _gpu_dict(frag.execute(ctx, capacity=8192), 2)
+    fact = [to_np(c) for c in T.fact.columns]
+    d, s_, p = ([to_np(c) for c in t.columns] for t in (T.date, T.supplier, T.part))
+    dims = [
+        dict(columns=d, key_col=0, payload_col=1, fact_key_col=0,
+             filter=([abi.make_pred(abi.FILTER_EQ_RANGE, 1, [1992, 1993], data_type=2)], 0)),
+        dict(columns=s_, key_col=0, payload_col=1, fact_key_col=2,
+             filter=([abi.make_pred(abi.FILTER_EQ, 1, [1], data_type=2)], 0)),
+        dict(columns=p, key_col=0, payload_col=2, fact_key_col=3,
+             filter=([abi.make_pred(abi.FILTER_EQ, 1, [1], data_type=2)], 0)),
+    ]
+    want = orc.FragmentSpec(fact, dims, keys=[(abi.SRC_JOIN, 0), (abi.SRC_JOIN, 2)], aggs=aggs).run(
+        nthreads=8, capacity=1 << 14)
+    assert got == _orc_dict_nulls(want)
