@@ -12,7 +12,9 @@
 // then one payload load for the survivors. Joins run cheapest-and-most-selective first (rank =
 // probe cost / (1 - selectivity)), with Hive's min/max pre-check in front of every probe. The
 // surviving rows aggregate into an LDS-private group table merged once per workgroup, or
-// straight into the global table when the group bound does not fit LDS.
+// straight into the global table when the group bound does not fit LDS. For the star shape with
+// such a group bound (and at most 2^20 group-key combinations) the pipelined kernel
+// (k_fragment_pipe) addresses the groups directly and keeps the next tile's loads in flight.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
