@@ -78,6 +78,9 @@ struct ghive_ctx {
   ghive_tuning tune{};
   // reusable small device scratch for counters (8 x uint64) and its host mirror
   uint64_t *d_counters = nullptr;
+  // pinned host words that small device-to-host read-backs land in, so that several of them
+  // are issued asynchronously and waited for once
+  uint64_t *h_words = nullptr;
   // live objects whose memory comes from this context; ghive_shutdown orphans them so that a
   // later ghive_hash_table_free / ghive_batch_free only releases the handle
   std::set<ghive_hash_table *> tables;
@@ -182,6 +185,9 @@ constexpr uint64_t kMaxRows = 0xFFFFFFFEull;  // row ids are uint32
 int check_table(ghive_ctx *ctx, const ghive_table *t, const char *what);
 // Synchronous device->host copy on the context stream.
 int d2h(ghive_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+// Reads up to kMaxReadWords device words (src[i] -> dst[i]) with one wait on the stream.
+constexpr int kMaxReadWords = 16;
+int d2h_words(ghive_ctx *ctx, int n, uint64_t *const *dst, const uint64_t *const *src);
 int h2d(ghive_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 // Launch helpers (defined in kernels_util.hip).
 int fill_u64(ghive_ctx *ctx, uint64_t *p, uint64_t value, uint64_t n);

@@ -130,6 +130,15 @@ int d2h(ghive_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
   return GHIVE_OK;
 }
 
+int d2h_words(ghive_ctx *ctx, int n, uint64_t *const *dst, const uint64_t *const *src) {
+  if (n < 0 || n > kMaxReadWords) return set_error(ctx, GHIVE_ERR_INVALID, "d2h_words: %d words", n);
+  for (int i = 0; i < n; ++i)
+    GH_HIP(ctx, hipMemcpyAsync(ctx->h_words + i, src[i], sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+  GH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < n; ++i) *dst[i] = ctx->h_words[i];
+  return GHIVE_OK;
+}
+
 int h2d(ghive_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
   if (!bytes) return GHIVE_OK;
   GH_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
@@ -177,7 +186,8 @@ int ghive_init(int device, uint64_t mem_limit_bytes, ghive_ctx **out) {
   uint64_t limit = mem_limit_bytes ? mem_limit_bytes : (uint64_t)(0.9 * (double)free_b);
   ctx->arena = new DeviceArena(limit);
   ctx->d_counters = static_cast<uint64_t *>(ctx->arena->alloc(64 * sizeof(uint64_t)));
-  if (!ctx->d_counters) {
+  if (!ctx->d_counters || hipHostMalloc((void **)&ctx->h_words, kMaxReadWords * sizeof(uint64_t),
+                                        hipHostMallocDefault) != hipSuccess) {
     delete ctx->arena;
     delete ctx;
     return GHIVE_ERR_OOM;
@@ -193,6 +203,7 @@ void ghive_shutdown(ghive_ctx *ctx) {
   ghive::orphan_batches(ctx);
   ghive::orphan_tables(ctx);
   delete ctx->arena;
+  if (ctx->h_words) (void)hipHostFree(ctx->h_words);
   for (auto &e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);

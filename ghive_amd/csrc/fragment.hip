@@ -338,14 +338,12 @@ int fragment_run(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_c
     O.type[c] = c < frag->nkeys ? GHIVE_INT : GHIVE_LONG;
     if (out_cols) out_cols[c].type = O.type[c];
   }
-  int st = group_table_emit(ctx, P.K, P.T, O, capacity, out_groups);
-  GH_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-  GH_HIP(ctx, hipEventSynchronize(ctx->ev[2]));
+  uint64_t joined = 0;
+  int st = group_table_emit(ctx, P.K, P.T, O, capacity, out_groups, ctx->ev[2], djoined.as<uint64_t>(), &joined);
+  if (st != GHIVE_OK && st != GHIVE_ERR_CAPACITY) return st;
   float ms_main = 0, ms_fin = 0;
   (void)hipEventElapsedTime(&ms_main, ctx->ev[0], ctx->ev[1]);
   (void)hipEventElapsedTime(&ms_fin, ctx->ev[1], ctx->ev[2]);
-  uint64_t joined = 0;
-  GH_TRY(d2h(ctx, &joined, djoined.get(), sizeof joined));
   ctx->times.main_kernel_ms = ms_main;
   ctx->times.finalize_ms = ms_fin;
   ctx->times.total_ms = ms_main + ms_fin;

@@ -739,15 +739,19 @@ int group_table_preinsert(ghive_ctx *ctx, const GroupTableDev &T, uint64_t key) 
 }
 
 int group_table_emit(ghive_ctx *ctx, const KeySpec &K, const GroupTableDev &T, const OutSpec &O,
-                     uint64_t capacity, uint64_t *out_groups) {
+                     uint64_t capacity, uint64_t *out_groups, hipEvent_t after, const uint64_t *extra_src,
+                     uint64_t *extra_dst) {
   GH_TRY(fill_u64(ctx, ctx->d_counters, 0, 1));
   uint64_t g = (T.stride + 255) / 256, cap = (uint64_t)ctx->cus * 16;
   k_groupby_finalize<<<(uint32_t)(g > cap ? cap : g), 256, 0, ctx->stream>>>(
       K, T, O, reinterpret_cast<unsigned long long *>(ctx->d_counters), capacity);
   GH_LAUNCH(ctx);
-  uint64_t host[2] = {0, 0};
-  GH_TRY(d2h(ctx, &host[0], ctx->d_counters, sizeof(uint64_t)));
-  GH_TRY(d2h(ctx, &host[1], T.overflow, sizeof(uint64_t)));
+  if (after) GH_HIP(ctx, hipEventRecord(after, ctx->stream));
+  uint64_t host[3] = {0, 0, 0};
+  uint64_t *dst[3] = {&host[0], &host[1], &host[2]};
+  const uint64_t *src[3] = {ctx->d_counters, reinterpret_cast<const uint64_t *>(T.overflow), extra_src};
+  GH_TRY(d2h_words(ctx, extra_src ? 3 : 2, dst, src));
+  if (extra_src && extra_dst) *extra_dst = host[2];
   *out_groups = host[0];
   if (host[1] || host[0] > capacity) {
     if (*out_groups <= capacity) *out_groups = capacity + 1;

@@ -52,7 +52,10 @@ int radix_sort_u64x2_ids(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1,
 int key_dictionary(ghive_ctx *ctx, const WideKeySpec &K, uint64_t rows, const uint32_t *sel, uint64_t n,
                    uint64_t capacity, DevBuf &dict, DevBuf &codes, uint64_t *slots, uint64_t *count);
 // Writes every group to the output columns; CAPACITY error when there are more than capacity.
+// `after` (optional) is recorded on the stream after the write-out kernel; extra_src / extra_dst
+// (optional) is one more device word read back with the group count.
 int group_table_emit(ghive_ctx *ctx, const KeySpec &K, const GroupTableDev &T, const OutSpec &O,
-                     uint64_t capacity, uint64_t *out_groups);
+                     uint64_t capacity, uint64_t *out_groups, hipEvent_t after = nullptr,
+                     const uint64_t *extra_src = nullptr, uint64_t *extra_dst = nullptr);
 
 }  // namespace ghive

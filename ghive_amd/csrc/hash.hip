@@ -30,11 +30,13 @@ __device__ __forceinline__ long long wave_max_i64(long long v) {
   return v;
 }
 
-// stats: [0] entries, [1] duplicate-key flag; mm: [0] key min, [1] key max, [2] payload min,
-// [3] payload max.
+// Build statistics per workgroup, part[6 * blockIdx.x + i]: [0] entries, [1] duplicate-key flag,
+// [2] key min, [3] key max, [4] payload min, [5] payload max (k_build_stats combines them; no
+// atomics on shared counters, which serialise across the XCDs).
+constexpr int kStatWords = 6;
 __global__ __launch_bounds__(kHashThreads) void k_hash_build(
     const void *keys, int key_type, const uint32_t *sel, uint64_t n, const void *pay, int pay_type,
-    int layout, uint64_t *slots, uint64_t mask, int log2, unsigned long long *stats, long long *mm) {
+    int layout, uint64_t *slots, uint64_t mask, int log2, unsigned long long *part) {
   long long kmin = LLONG_MAX, kmax = LLONG_MIN, pmin = LLONG_MAX, pmax = LLONG_MIN;
   uint32_t inserted = 0, dup = 0;
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -80,17 +82,62 @@ __global__ __launch_bounds__(kHashThreads) void k_hash_build(
   kmax = wave_max_i64(kmax);
   pmin = wave_min_i64(pmin);
   pmax = wave_max_i64(pmax);
-  unsigned long long ins = wave_sum_u64(inserted);
-  uint64_t anydup = ballot(dup != 0);
+  const unsigned long long ins = wave_sum_u64(inserted);
+  const bool anydup = ballot(dup != 0) != 0;
+  __shared__ long long red[kHashThreads / kWave][kStatWords];
+  const int wv = threadIdx.x / kWave;
   if (lane_id() == 0) {
-    if (ins) {
-      atomicAdd(&stats[0], ins);
-      atomicMin(&mm[0], kmin);
-      atomicMax(&mm[1], kmax);
-      atomicMin(&mm[2], pmin);
-      atomicMax(&mm[3], pmax);
+    red[wv][0] = (long long)ins;
+    red[wv][1] = anydup;
+    red[wv][2] = kmin;
+    red[wv][3] = kmax;
+    red[wv][4] = pmin;
+    red[wv][5] = pmax;
+  }
+  __syncthreads();
+  if (threadIdx.x < kStatWords) {
+    const int w = threadIdx.x;
+    long long v = red[0][w];
+    for (int j = 1; j < (int)(blockDim.x / kWave); ++j) {
+      const long long o = red[j][w];
+      v = w < 2 ? v + o : ((w & 1) == 0 ? (o < v ? o : v) : (o > v ? o : v));  // sums, mins, maxes
     }
-    if (anydup) atomicOr(&stats[1], 1ull);
+    part[(uint64_t)blockIdx.x * kStatWords + w] = (unsigned long long)v;
+  }
+}
+
+// Combines the per-workgroup build statistics into stats[0..5] (one workgroup).
+__global__ __launch_bounds__(kHashThreads) void k_build_stats(const unsigned long long *part, uint32_t nparts,
+                                                             unsigned long long *stats) {
+  long long v[kStatWords] = {0, 0, LLONG_MAX, LLONG_MIN, LLONG_MAX, LLONG_MIN};
+  for (uint32_t b = threadIdx.x; b < nparts; b += blockDim.x) {
+    const unsigned long long *p = part + (uint64_t)b * kStatWords;
+    v[0] += (long long)p[0];
+    v[1] |= (long long)p[1];
+    v[2] = (long long)p[2] < v[2] ? (long long)p[2] : v[2];
+    v[3] = (long long)p[3] > v[3] ? (long long)p[3] : v[3];
+    v[4] = (long long)p[4] < v[4] ? (long long)p[4] : v[4];
+    v[5] = (long long)p[5] > v[5] ? (long long)p[5] : v[5];
+  }
+  v[0] = (long long)wave_sum_u64((unsigned long long)v[0]);
+  v[1] = ballot(v[1] != 0) != 0;
+  v[2] = wave_min_i64(v[2]);
+  v[3] = wave_max_i64(v[3]);
+  v[4] = wave_min_i64(v[4]);
+  v[5] = wave_max_i64(v[5]);
+  __shared__ long long red[kHashThreads / kWave][kStatWords];
+  const int wv = threadIdx.x / kWave;
+  if (lane_id() == 0)
+    for (int w = 0; w < kStatWords; ++w) red[wv][w] = v[w];
+  __syncthreads();
+  if (threadIdx.x < kStatWords) {
+    const int w = threadIdx.x;
+    long long r = red[0][w];
+    for (int j = 1; j < (int)(blockDim.x / kWave); ++j) {
+      const long long o = red[j][w];
+      r = w < 2 ? (w == 0 ? r + o : (r | o)) : ((w & 1) == 0 ? (o < r ? o : r) : (o > r ? o : r));
+    }
+    stats[w] = (unsigned long long)r;
   }
 }
 
@@ -354,21 +401,23 @@ int hash_build(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, uin
     // {key = LONG_MAX, payload = 0}: fill keys and payloads as one u64 array then fix payloads
     st = fill_u64(ctx, static_cast<uint64_t *>(h->data), kEmpty16, words);
   }
-  DevBuf stats;
-  if (st == GHIVE_OK) st = stats.alloc(ctx, 8 * sizeof(uint64_t));
-  long long init_mm[4] = {LLONG_MAX, LLONG_MIN, LLONG_MAX, LLONG_MIN};
-  unsigned long long init_st[2] = {0, 0};
-  if (st == GHIVE_OK) st = h2d(ctx, stats.as<char>(), init_st, sizeof init_st);
-  if (st == GHIVE_OK) st = h2d(ctx, stats.as<char>() + 16, init_mm, sizeof init_mm);
+  // statistics: per-workgroup partials combined by one small kernel, then one copy to the host
+  uint64_t host[kStatWords] = {0, 0, (uint64_t)LLONG_MAX, (uint64_t)LLONG_MIN, (uint64_t)LLONG_MAX,
+                               (uint64_t)LLONG_MIN};
   if (st == GHIVE_OK && n) {
-    k_hash_build<<<grid_for(ctx, n), kHashThreads, 0, ctx->stream>>>(
-        key->data, key->type, sel, n, payload ? payload->data : nullptr, payload ? payload->type : 0,
-        h->layout, static_cast<uint64_t *>(h->data), h->slots - 1, h->log2_slots,
-        stats.as<unsigned long long>(), reinterpret_cast<long long *>(stats.as<char>() + 16));
-    if (hipGetLastError() != hipSuccess) st = set_error(ctx, GHIVE_ERR_HIP, "hash_build: launch failed");
+    const uint32_t grid = grid_for(ctx, n);
+    DevBuf part;
+    st = part.alloc(ctx, ((uint64_t)grid + 1) * kStatWords * sizeof(uint64_t));
+    unsigned long long *stats = st == GHIVE_OK ? part.as<unsigned long long>() + (uint64_t)grid * kStatWords : nullptr;
+    if (st == GHIVE_OK) {
+      k_hash_build<<<grid, kHashThreads, 0, ctx->stream>>>(
+          key->data, key->type, sel, n, payload ? payload->data : nullptr, payload ? payload->type : 0,
+          h->layout, static_cast<uint64_t *>(h->data), h->slots - 1, h->log2_slots, part.as<unsigned long long>());
+      k_build_stats<<<1, kHashThreads, 0, ctx->stream>>>(part.as<unsigned long long>(), grid, stats);
+      if (hipGetLastError() != hipSuccess) st = set_error(ctx, GHIVE_ERR_HIP, "hash_build: launch failed");
+    }
+    if (st == GHIVE_OK) st = d2h(ctx, host, stats, sizeof host);
   }
-  uint64_t host[6];
-  if (st == GHIVE_OK) st = d2h(ctx, host, stats.get(), sizeof host);
   if (st != GHIVE_OK) {
     ctx->arena->free(h->data);
     delete h;
