@@ -139,72 +139,117 @@ __device__ __forceinline__ bool cmp_mode(int mode, T x, T a, T b) {
   }
 }
 
+// Value of node n for row r, given the values of the nodes before it (bit i of bits).
+__device__ __forceinline__ bool eval_pred_node(const DevPred &P, const DevPredNode &n, uint32_t bits, uint64_t r) {
+  bool v;
+  switch (n.op) {
+    case kLeafInt: {
+      int64_t x;
+      v = col_load_i64(P.col_data[n.a], P.col_type[n.a], r, x) && cmp_mode<int64_t>(n.mode, x, n.ia, n.ib);
+      break;
+    }
+    case kLeafDbl: {
+      double x;
+      v = col_load_f64(P.col_data[n.a], P.col_type[n.a], r, x) && cmp_mode<double>(n.mode, x, n.da, n.db);
+      break;
+    }
+    case kColColInt: {
+      int64_t x, y;
+      bool ok = col_load_i64(P.col_data[n.a], P.col_type[n.a], r, x);
+      ok = col_load_i64(P.col_data[n.b], P.col_type[n.b], r, y) && ok;
+      v = ok && cmp_mode<int64_t>(n.mode, x, y, 0);
+      break;
+    }
+    case kColColDbl: {
+      double x, y;
+      bool ok = col_load_f64(P.col_data[n.a], P.col_type[n.a], r, x);
+      ok = col_load_f64(P.col_data[n.b], P.col_type[n.b], r, y) && ok;
+      v = ok && cmp_mode<double>(n.mode, x, y, 0.0);
+      break;
+    }
+    case kNotNull: {
+      double x;
+      v = col_load_f64(P.col_data[n.a], P.col_type[n.a], r, x);
+      break;
+    }
+    case kLeafStr: {
+      const int32_t s = P.col_aux[n.a][2 * r], e = P.col_aux[n.a][2 * r + 1];
+      if (s < 0) {
+        v = false;
+        break;
+      }
+      const uint8_t *x = static_cast<const uint8_t *>(P.col_data[n.a]) + s;
+      const int c0 = str_compare(x, e - s, P.lit + (uint32_t)n.ia, (int)(n.ia >> 32));
+      const int c1 = n.pad == 2 ? str_compare(x, e - s, P.lit + (uint32_t)n.ib, (int)(n.ib >> 32)) : 0;
+      v = cmp_mode<int>(n.mode, 0, -c0, -c1);  // 0 OP -c  <=>  string OP literal
+      break;
+    }
+    case kColColStr: {
+      const int32_t s0 = P.col_aux[n.a][2 * r], e0 = P.col_aux[n.a][2 * r + 1];
+      const int32_t s1 = P.col_aux[n.b][2 * r], e1 = P.col_aux[n.b][2 * r + 1];
+      if (s0 < 0 || s1 < 0) {
+        v = false;
+        break;
+      }
+      const int c = str_compare(static_cast<const uint8_t *>(P.col_data[n.a]) + s0, e0 - s0,
+                                static_cast<const uint8_t *>(P.col_data[n.b]) + s1, e1 - s1);
+      v = cmp_mode<int>(n.mode, 0, -c, 0);
+      break;
+    }
+    case kNotNullStr: v = P.col_aux[n.a][2 * r] >= 0; break;
+    case kAnd: v = ((bits >> n.a) & 1u) && ((bits >> n.b) & 1u); break;
+    default: v = ((bits >> n.a) & 1u) || ((bits >> n.b) & 1u); break;
+  }
+  return v;
+}
+
 __device__ __forceinline__ bool eval_pred(const DevPred &P, uint64_t r) {
   uint32_t bits = 0;
-  for (int i = 0; i < P.nnodes; ++i) {
-    const DevPredNode &n = P.nodes[i];
-    bool v;
-    switch (n.op) {
-      case kLeafInt: {
-        int64_t x;
-        v = col_load_i64(P.col_data[n.a], P.col_type[n.a], r, x) && cmp_mode<int64_t>(n.mode, x, n.ia, n.ib);
-        break;
-      }
-      case kLeafDbl: {
-        double x;
-        v = col_load_f64(P.col_data[n.a], P.col_type[n.a], r, x) && cmp_mode<double>(n.mode, x, n.da, n.db);
-        break;
-      }
-      case kColColInt: {
-        int64_t x, y;
-        bool ok = col_load_i64(P.col_data[n.a], P.col_type[n.a], r, x);
-        ok = col_load_i64(P.col_data[n.b], P.col_type[n.b], r, y) && ok;
-        v = ok && cmp_mode<int64_t>(n.mode, x, y, 0);
-        break;
-      }
-      case kColColDbl: {
-        double x, y;
-        bool ok = col_load_f64(P.col_data[n.a], P.col_type[n.a], r, x);
-        ok = col_load_f64(P.col_data[n.b], P.col_type[n.b], r, y) && ok;
-        v = ok && cmp_mode<double>(n.mode, x, y, 0.0);
-        break;
-      }
-      case kNotNull: {
-        double x;
-        v = col_load_f64(P.col_data[n.a], P.col_type[n.a], r, x);
-        break;
-      }
-      case kLeafStr: {
-        const int32_t s = P.col_aux[n.a][2 * r], e = P.col_aux[n.a][2 * r + 1];
-        if (s < 0) {
-          v = false;
-          break;
-        }
-        const uint8_t *x = static_cast<const uint8_t *>(P.col_data[n.a]) + s;
-        const int c0 = str_compare(x, e - s, P.lit + (uint32_t)n.ia, (int)(n.ia >> 32));
-        const int c1 = n.pad == 2 ? str_compare(x, e - s, P.lit + (uint32_t)n.ib, (int)(n.ib >> 32)) : 0;
-        v = cmp_mode<int>(n.mode, 0, -c0, -c1);  // 0 OP -c  <=>  string OP literal
-        break;
-      }
-      case kColColStr: {
-        const int32_t s0 = P.col_aux[n.a][2 * r], e0 = P.col_aux[n.a][2 * r + 1];
-        const int32_t s1 = P.col_aux[n.b][2 * r], e1 = P.col_aux[n.b][2 * r + 1];
-        if (s0 < 0 || s1 < 0) {
-          v = false;
-          break;
-        }
-        const int c = str_compare(static_cast<const uint8_t *>(P.col_data[n.a]) + s0, e0 - s0,
-                                  static_cast<const uint8_t *>(P.col_data[n.b]) + s1, e1 - s1);
-        v = cmp_mode<int>(n.mode, 0, -c, 0);
-        break;
-      }
-      case kNotNullStr: v = P.col_aux[n.a][2 * r] >= 0; break;
-      case kAnd: v = ((bits >> n.a) & 1u) && ((bits >> n.b) & 1u); break;
-      default: v = ((bits >> n.a) & 1u) || ((bits >> n.b) & 1u); break;
-    }
-    bits |= (uint32_t)v << i;
-  }
+  for (int i = 0; i < P.nnodes; ++i) bits |= (uint32_t)eval_pred_node(P, P.nodes[i], bits, r) << i;
   return (bits >> (P.nnodes - 1)) & 1u;
+}
+
+// The predicate over R rows r0 + q * step (q < R; rows >= n fail): bit q of the result. Nodes are
+// evaluated one at a time over all R rows, so that a leaf's R column loads are independent and
+// issue together, and a node's fields are read once for the R rows.
+template <int R>
+__device__ __forceinline__ uint32_t eval_pred_rows(const DevPred &P, uint64_t r0, uint64_t step, uint64_t n) {
+  uint32_t bits[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) bits[q] = 0;
+  for (int i = 0; i < P.nnodes; ++i) {
+    const DevPredNode &nd = P.nodes[i];
+    const int op = nd.op;
+    if (op == kLeafInt) {
+      int64_t x[R];
+      bool ok[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const uint64_t r = r0 + (uint64_t)q * step;
+        x[q] = 0;
+        ok[q] = r < n && col_load_i64(P.col_data[nd.a], P.col_type[nd.a], r, x[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < R; ++q) bits[q] |= (uint32_t)(ok[q] && cmp_mode<int64_t>(nd.mode, x[q], nd.ia, nd.ib)) << i;
+    } else if (op == kAnd || op == kOr) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const uint32_t a = (bits[q] >> nd.a) & 1u, b = (bits[q] >> nd.b) & 1u;
+        bits[q] |= (op == kAnd ? (a & b) : (a | b)) << i;
+      }
+    } else {  // the other leaf kinds, row by row
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const uint64_t r = r0 + (uint64_t)q * step;
+        if (r >= n) continue;
+        bits[q] |= (uint32_t)eval_pred_node(P, nd, bits[q], r) << i;
+      }
+    }
+  }
+  uint32_t pass = 0;
+#pragma unroll
+  for (int q = 0; q < R; ++q) pass |= ((bits[q] >> (P.nnodes - 1)) & 1u) << q;
+  return pass;
 }
 
 // ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) -------------------------------------

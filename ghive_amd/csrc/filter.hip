@@ -14,15 +14,18 @@
 
 namespace ghive {
 
+typedef int v4i __attribute__((ext_vector_type(4)));
 constexpr int kFilterThreads = 256;
 constexpr int kFilterRowsPerThread = 16;
 constexpr uint64_t kFilterTile = (uint64_t)kFilterThreads * kFilterRowsPerThread;  // 4096
 constexpr uint64_t kWordsPerTile = kFilterTile / 64;                               // 64
 
-__global__ __launch_bounds__(kFilterThreads) void k_filter_mask(DevPred P, uint64_t n,
+// Tiles first_tile + blockIdx.x: one bit per row in masks (row order), passing rows per tile.
+__global__ __launch_bounds__(kFilterThreads) void k_filter_mask(DevPred P, uint64_t n, uint64_t first_tile,
                                                                 uint64_t *masks, uint32_t *tile_counts) {
   __shared__ uint32_t lds[kFilterThreads / kWave];
-  const uint64_t tile0 = (uint64_t)blockIdx.x * kFilterTile;
+  const uint64_t tile = first_tile + blockIdx.x;
+  const uint64_t tile0 = tile * kFilterTile;
   const uint32_t wave = threadIdx.x / kWave;
   uint32_t cnt = 0;
 #pragma unroll 4
@@ -33,6 +36,62 @@ __global__ __launch_bounds__(kFilterThreads) void k_filter_mask(DevPred P, uint6
     uint64_t word = (tile0 >> 6) + (uint64_t)j * (kFilterThreads / kWave) + wave;
     if (lane_id() == 0 && (word << 6) < n) masks[word] = m;
     cnt += pass;
+  }
+  uint32_t total;
+  (void)block_exclusive_scan_256(cnt, lds, total);
+  if (threadIdx.x == 0) tile_counts[tile] = total;
+}
+
+// The same masks for whole tiles when every leaf compares a 4-byte INT column with literals
+// (AND / OR inside): each lane reads 4 consecutive rows with one 16-byte load per leaf column and
+// the 4 row groups of its tile together, then evaluates the nodes on the 16 values in registers.
+// A lane's 4 result bits are a nibble of the row-order mask word that 16 consecutive lanes share.
+constexpr int kFilterGroups = kFilterTile / (kFilterThreads * 4);  // 4 groups of 1024 rows
+__global__ __launch_bounds__(kFilterThreads) void k_filter_mask_i32(DevPred P, uint64_t *masks, uint32_t *tile_counts) {
+  __shared__ uint32_t lds[kFilterThreads / kWave];
+  const uint64_t tile0 = (uint64_t)blockIdx.x * kFilterTile;
+  uint32_t bits[kFilterGroups][4];
+#pragma unroll
+  for (int g = 0; g < kFilterGroups; ++g)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bits[g][q] = 0;
+  for (int i = 0; i < P.nnodes; ++i) {
+    const DevPredNode &nd = P.nodes[i];
+    if (nd.op == kLeafInt) {
+      const int32_t *col = static_cast<const int32_t *>(P.col_data[nd.a]) + tile0 + 4 * threadIdx.x;
+      v4i x[kFilterGroups];
+#pragma unroll
+      for (int g = 0; g < kFilterGroups; ++g) x[g] = *reinterpret_cast<const v4i *>(col + (uint64_t)g * kFilterThreads * 4);
+#pragma unroll
+      for (int g = 0; g < kFilterGroups; ++g) {
+        const int32_t v[4] = {x[g].x, x[g].y, x[g].z, x[g].w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          bits[g][q] |= (uint32_t)(!is_null_i32(v[q]) && cmp_mode<int64_t>(nd.mode, v[q], nd.ia, nd.ib)) << i;
+      }
+    } else {  // AND / OR
+      const bool is_and = nd.op == kAnd;
+#pragma unroll
+      for (int g = 0; g < kFilterGroups; ++g)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t a = (bits[g][q] >> nd.a) & 1u, b = (bits[g][q] >> nd.b) & 1u;
+          bits[g][q] |= (is_and ? (a & b) : (a | b)) << i;
+        }
+    }
+  }
+  const int top = P.nnodes - 1;
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int g = 0; g < kFilterGroups; ++g) {
+    uint32_t nib = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) nib |= ((bits[g][q] >> top) & 1u) << q;
+    cnt += (uint32_t)__builtin_popcount(nib);
+    uint64_t word = (uint64_t)nib << (4 * (threadIdx.x & 15));
+#pragma unroll
+    for (int d = 1; d < 16; d <<= 1) word |= __shfl_xor(word, d, kWave);
+    if ((threadIdx.x & 15) == 0) masks[(tile0 >> 6) + (uint64_t)g * (kFilterThreads / 16) + threadIdx.x / 16] = word;
   }
   uint32_t total;
   (void)block_exclusive_scan_256(cnt, lds, total);
@@ -86,8 +145,25 @@ int filter_select(ghive_ctx *ctx, const ghive_table *t, const ghive_pred *nodes,
   DevBuf masks, counts;
   GH_TRY(masks.alloc(ctx, sizeof(uint64_t) * nwords));
   GH_TRY(counts.alloc(ctx, sizeof(uint32_t) * ntiles));
-  k_filter_mask<<<ntiles, kFilterThreads, 0, ctx->stream>>>(P, n, masks.as<uint64_t>(), counts.as<uint32_t>());
-  GH_LAUNCH(ctx);
+  // whole tiles through the INT form when every leaf compares an aligned INT column, the rest
+  // (and every other predicate) through the general one
+  bool int_form = P.nnodes > 0;
+  for (int i = 0; i < P.nnodes; ++i) {
+    const DevPredNode &nd = P.nodes[i];
+    if (nd.op == kAnd || nd.op == kOr) continue;
+    int_form = int_form && nd.op == kLeafInt && P.col_type[nd.a] == GHIVE_INT &&
+               ((uintptr_t)P.col_data[nd.a] & 15) == 0;
+  }
+  const uint64_t full = int_form ? n / kFilterTile : 0;
+  if (full) {
+    k_filter_mask_i32<<<(uint32_t)full, kFilterThreads, 0, ctx->stream>>>(P, masks.as<uint64_t>(), counts.as<uint32_t>());
+    GH_LAUNCH(ctx);
+  }
+  if (full < ntiles) {
+    k_filter_mask<<<(uint32_t)(ntiles - full), kFilterThreads, 0, ctx->stream>>>(P, n, full, masks.as<uint64_t>(),
+                                                                              counts.as<uint32_t>());
+    GH_LAUNCH(ctx);
+  }
   GH_TRY(exclusive_scan_u32(ctx, counts.as<uint32_t>(), counts.as<uint32_t>(), ntiles, ctx->d_counters));
   k_filter_write<<<ntiles, kFilterThreads, 0, ctx->stream>>>(masks.as<uint64_t>(), nwords,
                                                              counts.as<uint32_t>(), out_sel);

@@ -80,6 +80,31 @@ def test_filter_tree_and_columns_vs_oracle(ctx, orc):
     assert np.array_equal(got, orc.filter_rows([a, b, c], nodes, 0))
 
 
+@pytest.mark.parametrize("n", [4095, 4096, 12288, 12289, 1_000_003])
+def test_filter_int_tree_whole_tiles_vs_oracle(ctx, orc, n):
+    """AND/OR trees over INT columns only: whole 4096-row tiles go through the 16-byte-load form,
+    the partial tile through the general one; a column slice that is not 16-byte aligned sends
+    everything through the general form. Both give the oracle's rows."""
+    from ghive_amd import ops
+    rng = np.random.default_rng(n)
+    a = rng.integers(0, 10, n + 1).astype(np.int32)
+    b = rng.integers(-5, 5, n + 1).astype(np.int32)
+    a[::61] = INT_NULL
+    b[::53] = INT_NULL
+    nodes = [
+        abi.make_pred(abi.FILTER_OR, children=(1, 4)),
+        abi.make_pred(abi.FILTER_AND, children=(2, 3)),
+        abi.make_pred(abi.FILTER_EQ_RANGE, 0, [2, 6], data_type=2),
+        abi.make_pred(abi.FILTER_NOT_EQ, 1, [0], data_type=2),
+        abi.make_pred(abi.FILTER_NOT_EQ_RANGE, 1, [-4, 3], data_type=2),
+    ]
+    da, db = to_dev(a), to_dev(b)
+    for sl in (slice(0, n), slice(1, n + 1)):
+        t = ops.Table([da[sl], db[sl]])
+        got = to_np(ops.FilterPredicate(nodes, 0).process(ctx, t)).astype(np.uint32)
+        assert np.array_equal(got, orc.filter_rows([a[sl], b[sl]], nodes, 0)), (n, sl)
+
+
 def test_filter_hive_kats_on_gpu(ctx, kats):
     from ghive_amd import ops
     for k in kats["hive_filter"]:
