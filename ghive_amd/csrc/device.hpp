@@ -209,49 +209,6 @@ __device__ __forceinline__ bool eval_pred(const DevPred &P, uint64_t r) {
   return (bits >> (P.nnodes - 1)) & 1u;
 }
 
-// The predicate over R rows r0 + q * step (q < R; rows >= n fail): bit q of the result. Nodes are
-// evaluated one at a time over all R rows, so that a leaf's R column loads are independent and
-// issue together, and a node's fields are read once for the R rows.
-template <int R>
-__device__ __forceinline__ uint32_t eval_pred_rows(const DevPred &P, uint64_t r0, uint64_t step, uint64_t n) {
-  uint32_t bits[R];
-#pragma unroll
-  for (int q = 0; q < R; ++q) bits[q] = 0;
-  for (int i = 0; i < P.nnodes; ++i) {
-    const DevPredNode &nd = P.nodes[i];
-    const int op = nd.op;
-    if (op == kLeafInt) {
-      int64_t x[R];
-      bool ok[R];
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const uint64_t r = r0 + (uint64_t)q * step;
-        x[q] = 0;
-        ok[q] = r < n && col_load_i64(P.col_data[nd.a], P.col_type[nd.a], r, x[q]);
-      }
-#pragma unroll
-      for (int q = 0; q < R; ++q) bits[q] |= (uint32_t)(ok[q] && cmp_mode<int64_t>(nd.mode, x[q], nd.ia, nd.ib)) << i;
-    } else if (op == kAnd || op == kOr) {
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const uint32_t a = (bits[q] >> nd.a) & 1u, b = (bits[q] >> nd.b) & 1u;
-        bits[q] |= (op == kAnd ? (a & b) : (a | b)) << i;
-      }
-    } else {  // the other leaf kinds, row by row
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const uint64_t r = r0 + (uint64_t)q * step;
-        if (r >= n) continue;
-        bits[q] |= (uint32_t)eval_pred_node(P, nd, bits[q], r) << i;
-      }
-    }
-  }
-  uint32_t pass = 0;
-#pragma unroll
-  for (int q = 0; q < R; ++q) pass |= ((bits[q] >> (P.nnodes - 1)) & 1u) << q;
-  return pass;
-}
-
 // ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) -------------------------------------
 __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                               uint32_t k0, uint32_t k1, uint32_t out[4]) {
