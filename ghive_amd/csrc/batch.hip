@@ -121,6 +121,80 @@ __device__ __forceinline__ void batch_flush(const BatchDev &B, unsigned long lon
   __syncthreads();
 }
 
+// s_waitcnt immediate of "vmcnt(0)" on gfx9-family chips (expcnt and lgkmcnt at their maxima);
+// OR-ing a count below 16 into the low bits gives "vmcnt(count)"
+constexpr int kWaitVmcnt0 = 0x0f70;
+
+struct BatchRows {
+  int32_t k[4], a[4], b[4];
+};
+
+__device__ __forceinline__ void batch_load(const int32_t *kp, const int32_t *ap, const int32_t *bp, uint64_t r0,
+                                           BatchRows &x) {
+  const int4 k = *reinterpret_cast<const int4 *>(kp + r0);
+  const int4 a = *reinterpret_cast<const int4 *>(ap + r0);
+  const int4 b = *reinterpret_cast<const int4 *>(bp + r0);
+  x.k[0] = k.x; x.k[1] = k.y; x.k[2] = k.z; x.k[3] = k.w;
+  x.a[0] = a.x; x.a[1] = a.y; x.a[2] = a.z; x.a[3] = a.w;
+  x.b[0] = b.x; x.b[1] = b.y; x.b[2] = b.z; x.b[3] = b.w;
+}
+
+// One whole sub-tile (kBatchThreads * 4 rows) of the LDS-bitmap mode, with the next sub-tile's
+// three column loads in flight: LDS bitmap test, then the attribute loads of the passing rows,
+// then the next loads, then a wait for everything older than those (vector memory completes in
+// order) and the LDS accumulation. No global write happens here, so the waits count loads only.
+__device__ __forceinline__ void batch_sub(const BatchDev &B, const BatchItemDev &it, const uint32_t *lbits,
+                                          unsigned long long *lsum, unsigned long long *lcnt, const int32_t *bp,
+                                          const BatchRows &cur, BatchRows &nxt, uint64_t next_r0) {
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+  uint32_t alive = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int32_t k = cur.k[q];
+    bool ok = k != kNull && k >= B.key_min && k <= B.key_max;
+    if (ok) {
+      const uint32_t u = (uint32_t)((int64_t)k - B.key_min);
+      ok = (lbits[u >> 5] >> (u & 31)) & 1u;
+    }
+    alive |= (uint32_t)ok << q;
+  }
+  int32_t pay[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if ((alive >> q) & 1u) pay[q] = B.dense[(uint32_t)((int64_t)cur.k[q] - B.key_min)];
+  batch_load(it.key, it.a, bp, next_r0, nxt);
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0 | 3);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (!((alive >> q) & 1u)) continue;
+    const int32_t f = (int32_t)(((uint32_t)pay[q] >> B.fshift) & B.fmask);
+    if (f < it.lo || f > it.hi) continue;
+    if (cur.a[q] == kNull || (it.b && cur.b[q] == kNull)) continue;
+    const uint32_t g = ((uint32_t)pay[q] >> B.gshift) & B.gmask;
+    const int64_t v = it.b ? (int64_t)((uint64_t)(int64_t)cur.a[q] * (uint64_t)(int64_t)cur.b[q]) : (int64_t)cur.a[q];
+    atomicAdd(&lsum[g], (unsigned long long)v);
+    atomicAdd(&lcnt[g], 1ull);
+  }
+}
+
+// Sub-tiles 0 .. nfull-1 of the tile at row `base` of item `it`, all whole.
+__device__ __forceinline__ void batch_tile_pipelined(const BatchDev &B, const BatchItemDev &it, const uint32_t *lbits,
+                                                     unsigned long long *lsum, unsigned long long *lcnt, uint64_t base,
+                                                     uint32_t nfull) {
+  constexpr uint32_t kSub = kBatchThreads * 4;
+  const int32_t *bp = it.b ? it.b : it.a;  // always three loads per sub-tile (b unused without it)
+  const uint64_t lane_r = base + (uint64_t)threadIdx.x * 4;
+  BatchRows x0, x1;
+  batch_load(it.key, it.a, bp, lane_r, x0);
+  for (uint32_t s = 0;; s += 2) {
+    batch_sub(B, it, lbits, lsum, lcnt, bp, x0, x1, lane_r + (uint64_t)(s + 1 < nfull ? s + 1 : s) * kSub);
+    if (s + 1 >= nfull) break;
+    batch_sub(B, it, lbits, lsum, lcnt, bp, x1, x0, lane_r + (uint64_t)(s + 2 < nfull ? s + 2 : s + 1) * kSub);
+    if (s + 2 >= nfull) break;
+  }
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);  // the last (repeated) loads land before the registers are reused
+}
+
 __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchDev B) {
   extern __shared__ uint64_t lds64[];
   unsigned long long *lsum = reinterpret_cast<unsigned long long *>(lds64);
@@ -158,7 +232,14 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(BatchDev B) {
     }
     const uint32_t *gbits = B.mode == 2 ? B.fbits + (uint64_t)it.fidx * B.nwords : nullptr;
     const uint64_t base = (t - it.tile_begin) * B.tile_rows;
-    for (uint32_t sub = 0; sub < B.tile_rows; sub += kBatchThreads * 4) {
+    uint32_t sub0 = 0;  // first sub-tile left to the row-by-row loop below
+    if (B.mode == 1) {  // whole sub-tiles of the LDS-bitmap mode: software-pipelined
+      const uint64_t lim = it.rows < base + B.tile_rows ? it.rows : base + B.tile_rows;
+      const uint32_t nfull = lim > base ? (uint32_t)((lim - base) / (kBatchThreads * 4)) : 0u;
+      if (nfull) batch_tile_pipelined(B, it, lbits, lsum, lcnt, base, nfull);
+      sub0 = nfull * kBatchThreads * 4;
+    }
+    for (uint32_t sub = sub0; sub < B.tile_rows; sub += kBatchThreads * 4) {
       const uint64_t r0 = base + sub + (uint64_t)threadIdx.x * 4;
       if (r0 >= it.rows) break;
       int32_t k[4], a[4], b[4];
