@@ -93,12 +93,14 @@ __global__ void k_batch_filter_bits(const uint32_t *bitmap, const int32_t *dense
   }
 }
 
+typedef int v4i __attribute__((ext_vector_type(4)));
 constexpr int kBatchThreads = 256;
 constexpr int kNull = 0x7fffffff;
 
+// Fact columns are read once: non-temporal loads keep the dimension bitmap and attributes cached.
 __device__ __forceinline__ void bload4(const int32_t *p, uint64_t r0, uint64_t n, int32_t v[4]) {
   if (r0 + 3 < n) {
-    int4 x = *reinterpret_cast<const int4 *>(p + r0);
+    const v4i x = __builtin_nontemporal_load(reinterpret_cast<const v4i *>(p + r0));
     v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
   } else {
 #pragma unroll
@@ -131,9 +133,9 @@ struct BatchRows {
 
 __device__ __forceinline__ void batch_load(const int32_t *kp, const int32_t *ap, const int32_t *bp, uint64_t r0,
                                            BatchRows &x) {
-  const int4 k = *reinterpret_cast<const int4 *>(kp + r0);
-  const int4 a = *reinterpret_cast<const int4 *>(ap + r0);
-  const int4 b = *reinterpret_cast<const int4 *>(bp + r0);
+  const v4i k = __builtin_nontemporal_load(reinterpret_cast<const v4i *>(kp + r0));
+  const v4i a = __builtin_nontemporal_load(reinterpret_cast<const v4i *>(ap + r0));
+  const v4i b = __builtin_nontemporal_load(reinterpret_cast<const v4i *>(bp + r0));
   x.k[0] = k.x; x.k[1] = k.y; x.k[2] = k.z; x.k[3] = k.w;
   x.a[0] = a.x; x.a[1] = a.y; x.a[2] = a.z; x.a[3] = a.w;
   x.b[0] = b.x; x.b[1] = b.y; x.b[2] = b.z; x.b[3] = b.w;

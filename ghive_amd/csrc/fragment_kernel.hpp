@@ -17,12 +17,13 @@ constexpr int kNullI32 = 0x7fffffff;
 typedef int v4i __attribute__((ext_vector_type(4)));
 
 // R consecutive int32 values at p: 16-byte loads with no per-lane guard, so the loads of all
-// columns of a tile issue back to back.
+// columns of a tile issue back to back. Fact columns are read once: non-temporal loads, so that
+// they do not displace the dimension bitmaps and group tables from the caches.
 template <int R>
 __device__ __forceinline__ void load_rows(const int32_t *p, int32_t *v) {
 #pragma unroll
   for (int c = 0; c < R / 4; ++c) {
-    const v4i x = reinterpret_cast<const v4i *>(p)[c];
+    const v4i x = __builtin_nontemporal_load(reinterpret_cast<const v4i *>(p) + c);
     v[4 * c + 0] = x.x;
     v[4 * c + 1] = x.y;
     v[4 * c + 2] = x.z;
