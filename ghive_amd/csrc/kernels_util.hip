@@ -121,23 +121,31 @@ int exclusive_scan_u32(ghive_ctx *ctx, const uint32_t *in, uint32_t *out, uint64
 // ---- STREAM-style bandwidth probes (achievable HBM read / copy rate of the device) --------
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
+// Each workgroup streams one contiguous range (consecutive DRAM pages), four 16-byte
+// non-temporal loads per thread in flight -- the access pattern of the fused kernels.
 __global__ __launch_bounds__(256) void k_stream_read(const v4u *p, uint64_t n16, unsigned long long *sink) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t b0 = n16 * blockIdx.x / gridDim.x, b1 = n16 * (blockIdx.x + 1) / gridDim.x;
   v4u acc = {0u, 0u, 0u, 0u};
-  for (; i + 3 * stride < n16; i += 4 * stride) {
-    v4u a = p[i], b = p[i + stride], c = p[i + 2 * stride], d = p[i + 3 * stride];
+  uint64_t i = b0 + threadIdx.x;
+  for (; i + 3 * blockDim.x < b1; i += 4 * blockDim.x) {
+    const v4u a = __builtin_nontemporal_load(p + i), b = __builtin_nontemporal_load(p + i + blockDim.x);
+    const v4u c = __builtin_nontemporal_load(p + i + 2 * blockDim.x), d = __builtin_nontemporal_load(p + i + 3 * blockDim.x);
     acc ^= a ^ b ^ c ^ d;
   }
-  for (; i < n16; i += stride) acc ^= p[i];
+  for (; i < b1; i += blockDim.x) acc ^= __builtin_nontemporal_load(p + i);
   uint32_t x = acc.x ^ acc.y ^ acc.z ^ acc.w;
   if (x == 0x9e3779b9u) atomicAdd(sink, 1ull);  // keeps the loads alive; almost never taken
 }
 
 __global__ __launch_bounds__(256) void k_stream_copy(const v4u *src, v4u *dst, uint64_t n16) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (; i < n16; i += stride) dst[i] = src[i];
+  const uint64_t b0 = n16 * blockIdx.x / gridDim.x, b1 = n16 * (blockIdx.x + 1) / gridDim.x;
+  uint64_t i = b0 + threadIdx.x;
+  for (; i + blockDim.x < b1; i += 2 * blockDim.x) {
+    const v4u a = __builtin_nontemporal_load(src + i), b = __builtin_nontemporal_load(src + i + blockDim.x);
+    __builtin_nontemporal_store(a, dst + i);
+    __builtin_nontemporal_store(b, dst + i + blockDim.x);
+  }
+  for (; i < b1; i += blockDim.x) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
 }  // namespace ghive
