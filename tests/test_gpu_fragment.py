@@ -372,3 +372,21 @@ def test_star_fragment_queue_sizes(ctx, orc, naggs):
     want = orc.FragmentSpec(fact, dims, keys=[(abi.SRC_JOIN, 0), (abi.SRC_JOIN, 2)], aggs=aggs).run(
         nthreads=8, capacity=1 << 14)
     assert got == _orc_dict_nulls(want)
+
+
+@pytest.mark.parametrize("nbytes", [16, 4096 + 48, 3 * (1 << 20) + 16 * 257])
+def test_bandwidth_probe_copies_and_reads(ctx, nbytes):
+    """The bandwidth probe behind roofline peaks: the copy form copies every 16-byte word of odd-
+    sized ranges (workgroup ranges of unequal length, partial unrolled steps), the read form runs."""
+    import ctypes as C
+    import torch
+    from ghive_amd._lib import lib
+    src = torch.randint(-2**31, 2**31 - 1, (nbytes // 4,), dtype=torch.int32, device=f"cuda:{ctx.device}")
+    dst = torch.zeros_like(src)
+    ms = C.c_float()
+    ctx.check(lib().ghive_bench_stream(ctx.h, C.c_void_p(dst.data_ptr()), C.c_void_p(src.data_ptr()), nbytes, 1, 2,
+                                       C.byref(ms)))
+    torch.cuda.synchronize()
+    assert torch.equal(dst, src)
+    ctx.check(lib().ghive_bench_stream(ctx.h, None, C.c_void_p(src.data_ptr()), nbytes, 0, 2, C.byref(ms)))
+    assert ms.value > 0
