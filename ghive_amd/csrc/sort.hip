@@ -18,7 +18,7 @@
 namespace ghive {
 
 constexpr int kSortThreads = 256;
-constexpr int kSortPerThread = 16;
+constexpr int kSortPerThread = 32;
 constexpr uint64_t kSortTile = (uint64_t)kSortThreads * kSortPerThread;
 constexpr int kSortWaves = kSortThreads / kWave;
 
