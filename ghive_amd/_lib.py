@@ -47,11 +47,14 @@ SIGNATURES = {
                                         u32, C.POINTER(u32), C.POINTER(u32), C.c_char_p, C.c_size_t]),
     "ghive_predicate_to_string": (C.c_size_t, [C.POINTER(abi.Pred), u32, u32, C.c_char_p, C.c_size_t]),
     "ghive_filter": (C.c_int, [P, C.POINTER(abi.Table), C.POINTER(abi.Pred), u32, u32, P, PU64]),
+    "ghive_filter_async": (C.c_int, [P, C.POINTER(abi.Table), C.POINTER(abi.Pred), u32, u32, P, P]),
     "ghive_filter_compact": (C.c_int, [P, C.POINTER(abi.Table), C.POINTER(abi.Pred), u32, u32,
                                        C.POINTER(abi.Column), PU64]),
     "ghive_gather": (C.c_int, [P, C.POINTER(abi.Table), P, u64, C.POINTER(abi.Column)]),
     "ghive_hash_build": (C.c_int, [P, C.POINTER(abi.Column), P, u64, C.POINTER(abi.Column), C.c_float,
                                    C.POINTER(P)]),
+    "ghive_hash_build_async": (C.c_int, [P, C.POINTER(abi.Column), P, P, u64, C.POINTER(abi.Column),
+                                         C.c_float, C.POINTER(P)]),
     "ghive_hash_table_info": (C.c_int, [P, PU64, PU64, PU64, C.POINTER(i32)]),
     "ghive_hash_table_free": (None, [P, P]),
     "ghive_hash_probe": (C.c_int, [P, P, C.POINTER(abi.Column), P, u64, P, P, u64, PU64]),

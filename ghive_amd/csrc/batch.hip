@@ -379,6 +379,7 @@ int ghive_batch_create(ghive_ctx *ctx, const ghive_batch_spec *spec, const ghive
   *out = nullptr;
   const ghive_hash_table *h = spec->dim;
   if (h && !h->owner) return set_error(ctx, GHIVE_ERR_INVALID, "batch: the dimension table's context was shut down");
+  if (h) GH_TRY(hash_table_ready(ctx, h));
   if (!h || h->layout != 8 || !h->unique || h->payload_kind != 1)
     return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "batch: the dimension must be a unique-key INT table with an INT payload");
   if (spec->group_bits < 1 || spec->group_bits > 12 || spec->filter_bits < 1 || spec->filter_bits > 31 ||

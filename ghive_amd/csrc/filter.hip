@@ -133,13 +133,18 @@ __global__ void k_gather(const T *in, const uint32_t *sel, uint64_t n, T *out) {
   for (; i < n; i += stride) out[i] = in[sel[i]];
 }
 
-int filter_select(ghive_ctx *ctx, const ghive_table *t, const ghive_pred *nodes, uint32_t nnodes,
-                  uint32_t root, uint32_t *out_sel, uint64_t *out_count) {
+// out_count (host, after a wait) or, when count_dev is set, the count left on the device with no
+// wait (an asynchronous consumer reads it there).
+static int filter_run(ghive_ctx *ctx, const ghive_table *t, const ghive_pred *nodes, uint32_t nnodes,
+                      uint32_t root, uint32_t *out_sel, uint64_t *out_count, uint64_t *count_dev) {
   DevPred P;
   GH_TRY(flatten_pred(ctx, t, nodes, nnodes, root, P));
   const uint64_t n = t->row_num;
-  *out_count = 0;
-  if (n == 0) return GHIVE_OK;
+  if (out_count) *out_count = 0;
+  if (n == 0) {
+    if (count_dev) GH_TRY(fill_u64(ctx, count_dev, 0, 1));
+    return GHIVE_OK;
+  }
   const uint32_t ntiles = ceil_div(n, kFilterTile);
   const uint64_t nwords = (n + 63) / 64;
   DevBuf masks, counts;
@@ -168,8 +173,17 @@ int filter_select(ghive_ctx *ctx, const ghive_table *t, const ghive_pred *nodes,
   k_filter_write<<<ntiles, kFilterThreads, 0, ctx->stream>>>(masks.as<uint64_t>(), nwords,
                                                              counts.as<uint32_t>(), out_sel);
   GH_LAUNCH(ctx);
+  if (count_dev) {
+    GH_HIP(ctx, hipMemcpyAsync(count_dev, ctx->d_counters, sizeof(uint64_t), hipMemcpyDeviceToDevice, ctx->stream));
+    return GHIVE_OK;
+  }
   GH_TRY(d2h(ctx, out_count, ctx->d_counters, sizeof(uint64_t)));
   return GHIVE_OK;
+}
+
+int filter_select(ghive_ctx *ctx, const ghive_table *t, const ghive_pred *nodes, uint32_t nnodes,
+                  uint32_t root, uint32_t *out_sel, uint64_t *out_count) {
+  return filter_run(ctx, t, nodes, nnodes, root, out_sel, out_count, nullptr);
 }
 
 int gather_columns(ghive_ctx *ctx, const ghive_table *t, const uint32_t *sel, uint64_t n,
@@ -214,6 +228,15 @@ int ghive_filter(ghive_ctx *ctx, const ghive_table *table, const ghive_pred *nod
   if (!out_count) return set_error(ctx, GHIVE_ERR_INVALID, "filter: out_count is NULL");
   if (table->row_num && !out_sel) return set_error(ctx, GHIVE_ERR_INVALID, "filter: out_sel is NULL");
   return filter_select(ctx, table, nodes, nnodes, root, out_sel, out_count);
+}
+
+int ghive_filter_async(ghive_ctx *ctx, const ghive_table *table, const ghive_pred *nodes, uint32_t nnodes,
+                       uint32_t root, uint32_t *out_sel, uint64_t *count_dev) {
+  GH_CHECK_CTX(ctx);
+  GH_TRY(check_table(ctx, table, "filter_async"));
+  if (!count_dev) return set_error(ctx, GHIVE_ERR_INVALID, "filter_async: count_dev is NULL");
+  if (table->row_num && !out_sel) return set_error(ctx, GHIVE_ERR_INVALID, "filter_async: out_sel is NULL");
+  return filter_run(ctx, table, nodes, nnodes, root, out_sel, nullptr, count_dev);
 }
 
 int ghive_filter_compact(ghive_ctx *ctx, const ghive_table *table, const ghive_pred *nodes,

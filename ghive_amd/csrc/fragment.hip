@@ -68,6 +68,7 @@ int fragment_plan(ghive_ctx *ctx, const ghive_fragment *f, int late, FragPlan &P
     const ghive_hash_table *h = f->joins[j].table;
     if (!h) return set_error(ctx, GHIVE_ERR_INVALID, "fragment: join %u has no table", j);
     if (!h->owner) return set_error(ctx, GHIVE_ERR_INVALID, "fragment: join %u's table: its context was shut down", j);
+    GH_TRY(hash_table_ready(ctx, h));
     if (h->layout != 8 || !h->unique)
       return set_error(ctx, GHIVE_ERR_UNSUPPORTED,
                        "fragment: join %u needs a unique-key INT table with an INT payload", j);

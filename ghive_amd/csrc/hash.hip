@@ -34,11 +34,13 @@ __device__ __forceinline__ long long wave_max_i64(long long v) {
 // [2] key min, [3] key max, [4] payload min, [5] payload max (k_build_stats combines them; no
 // atomics on shared counters, which serialise across the XCDs).
 constexpr int kStatWords = 6;
+// n_dev (optional): the row count on the device (a filter's count), at most n.
 __global__ __launch_bounds__(kHashThreads) void k_hash_build(
-    const void *keys, int key_type, const uint32_t *sel, uint64_t n, const void *pay, int pay_type,
-    int layout, uint64_t *slots, uint64_t mask, int log2, unsigned long long *part) {
+    const void *keys, int key_type, const uint32_t *sel, uint64_t n, const uint64_t *n_dev, const void *pay,
+    int pay_type, int layout, uint64_t *slots, uint64_t mask, int log2, unsigned long long *part) {
   long long kmin = LLONG_MAX, kmax = LLONG_MIN, pmin = LLONG_MAX, pmax = LLONG_MIN;
   uint32_t inserted = 0, dup = 0;
+  if (n_dev && *n_dev < n) n = *n_dev;
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) {
@@ -312,9 +314,10 @@ __global__ __launch_bounds__(kProbeThreads) void k_probe_unique(
 
 // bitmap bit and dense payload of every inserted key (unique keys, so no write races on dense)
 __global__ __launch_bounds__(kHashThreads) void k_dense_fill(const void *keys, int key_type, const uint32_t *sel,
-                                                             uint64_t n, const void *pay, int pay_type,
-                                                             int64_t key_min, uint64_t range, uint32_t *bitmap,
-                                                             int32_t *dense) {
+                                                             uint64_t n, const uint64_t *n_dev, const void *pay,
+                                                             int pay_type, int64_t key_min, uint64_t range,
+                                                             uint32_t *bitmap, int32_t *dense) {
+  if (n_dev && *n_dev < n) n = *n_dev;
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) {
@@ -362,8 +365,45 @@ static uint32_t grid_for(ghive_ctx *ctx, uint64_t n, uint32_t threads = kHashThr
   return (uint32_t)(g < 1 ? 1 : (g > cap ? cap : g));
 }
 
-int hash_build(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, uint64_t n,
-               const ghive_column *payload, float load_factor, ghive_hash_table **out) {
+// Host-side statistics of a build (host[kStatWords] as k_build_stats writes them), then the dense
+// companion of a unique INT table whose keys span a compact range (surrogate keys of a star
+// schema): an existence bitmap over [key_min, key_max] and a payload array indexed by
+// key - key_min. A probe is then one bit test (LDS-resident when small) plus, for survivors, one
+// load -- no probe chain. Rows: sel[0 .. min(n, *n_dev)) (n_dev optional).
+static int finish_build(ghive_ctx *ctx, ghive_hash_table *h, const uint64_t (&host)[kStatWords], const void *key,
+                        const uint32_t *sel, uint64_t n, const uint64_t *n_dev, const void *pay, int pay_type) {
+  h->entries = host[0];
+  h->unique = host[1] == 0;
+  h->key_min = (int64_t)host[2];
+  h->key_max = (int64_t)host[3];
+  h->pay_min = (int64_t)host[4];
+  h->pay_max = (int64_t)host[5];
+  const uint64_t range = h->entries ? (uint64_t)(h->key_max - h->key_min) + 1 : 0;
+  if (!(h->layout == 8 && h->unique && h->entries && range <= (1ull << 27) && range <= 64 * h->entries + 65536))
+    return GHIVE_OK;
+  const uint64_t nwords = (range + 31) / 32;
+  h->bitmap = static_cast<uint32_t *>(ctx->arena->alloc(nwords * 4));
+  h->dense = static_cast<int32_t *>(ctx->arena->alloc(range * 4));
+  if (!h->bitmap || !h->dense) {  // no room: the table is probed through its slots
+    if (h->bitmap) ctx->arena->free(h->bitmap);
+    if (h->dense) ctx->arena->free(h->dense);
+    h->bitmap = nullptr;
+    h->dense = nullptr;
+    return GHIVE_OK;
+  }
+  h->range = range;
+  GH_TRY(fill_u32(ctx, h->bitmap, 0u, nwords));
+  k_dense_fill<<<grid_for(ctx, n), kHashThreads, 0, ctx->stream>>>(key, h->key_type, sel, n, n_dev, pay, pay_type,
+                                                                   h->key_min, range, h->bitmap, h->dense);
+  if (hipGetLastError() != hipSuccess) return set_error(ctx, GHIVE_ERR_HIP, "hash_build: dense launch failed");
+  return GHIVE_OK;
+}
+
+// Rows sel[0 .. n) (n_dev: the count on the device, at most n). Synchronous unless `async`:
+// then the statistics stay on the device and the table is finished on its first use.
+static int hash_build_core(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, uint64_t n,
+                           const uint64_t *n_dev, const ghive_column *payload, float load_factor, bool async,
+                           ghive_hash_table **out) {
   if (!key || !out) return set_error(ctx, GHIVE_ERR_INVALID, "hash_build: NULL argument");
   *out = nullptr;
   if (!is_int_type(key->type))
@@ -401,6 +441,8 @@ int hash_build(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, uin
     // {key = LONG_MAX, payload = 0}: fill keys and payloads as one u64 array then fix payloads
     st = fill_u64(ctx, static_cast<uint64_t *>(h->data), kEmpty16, words);
   }
+  const void *pay = payload ? payload->data : nullptr;
+  const int pay_type = payload ? payload->type : 0;
   // statistics: per-workgroup partials combined by one small kernel, then one copy to the host
   uint64_t host[kStatWords] = {0, 0, (uint64_t)LLONG_MAX, (uint64_t)LLONG_MIN, (uint64_t)LLONG_MAX,
                                (uint64_t)LLONG_MIN};
@@ -410,57 +452,65 @@ int hash_build(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, uin
     st = part.alloc(ctx, ((uint64_t)grid + 1) * kStatWords * sizeof(uint64_t));
     unsigned long long *stats = st == GHIVE_OK ? part.as<unsigned long long>() + (uint64_t)grid * kStatWords : nullptr;
     if (st == GHIVE_OK) {
-      k_hash_build<<<grid, kHashThreads, 0, ctx->stream>>>(
-          key->data, key->type, sel, n, payload ? payload->data : nullptr, payload ? payload->type : 0,
-          h->layout, static_cast<uint64_t *>(h->data), h->slots - 1, h->log2_slots, part.as<unsigned long long>());
+      k_hash_build<<<grid, kHashThreads, 0, ctx->stream>>>(key->data, key->type, sel, n, n_dev, pay, pay_type, h->layout,
+                                                           static_cast<uint64_t *>(h->data), h->slots - 1, h->log2_slots,
+                                                           part.as<unsigned long long>());
       k_build_stats<<<1, kHashThreads, 0, ctx->stream>>>(part.as<unsigned long long>(), grid, stats);
       if (hipGetLastError() != hipSuccess) st = set_error(ctx, GHIVE_ERR_HIP, "hash_build: launch failed");
+    }
+    if (st == GHIVE_OK && async) {
+      h->pending = true;
+      h->stats_dev = stats;
+      h->stats_buf = part.release();
+      h->in_key = key->data;
+      h->in_pay = pay;
+      h->in_pay_type = pay_type;
+      h->in_sel = sel;
+      h->in_n_dev = n_dev;
+      h->in_n_max = n;
+      ctx->tables.insert(h);
+      *out = h;
+      return GHIVE_OK;
     }
     if (st == GHIVE_OK) st = d2h(ctx, host, stats, sizeof host);
   }
   if (st != GHIVE_OK) {
+    (void)hipStreamSynchronize(ctx->stream);  // no kernel may still write the memory given back
     ctx->arena->free(h->data);
     delete h;
     return st;
   }
-  h->entries = host[0];
-  h->unique = host[1] == 0;
-  h->key_min = (int64_t)host[2];
-  h->key_max = (int64_t)host[3];
-  h->pay_min = (int64_t)host[4];
-  h->pay_max = (int64_t)host[5];
-  // Dense companion of a unique INT table whose keys span a compact range (surrogate keys of a
-  // star schema): an existence bitmap over [key_min, key_max] and a payload array indexed by
-  // key - key_min. A probe is then one bit test (LDS-resident when small) plus, for survivors,
-  // one load -- no probe chain.
-  const uint64_t range = h->entries ? (uint64_t)(h->key_max - h->key_min) + 1 : 0;
-  if (h->layout == 8 && h->unique && h->entries && range <= (1ull << 27) &&
-      range <= 64 * h->entries + 65536) {
-    uint64_t nwords = (range + 31) / 32;
-    h->bitmap = static_cast<uint32_t *>(ctx->arena->alloc(nwords * 4));
-    h->dense = static_cast<int32_t *>(ctx->arena->alloc(range * 4));
-    if (!h->bitmap || !h->dense) {
-      if (h->bitmap) ctx->arena->free(h->bitmap);
-      if (h->dense) ctx->arena->free(h->dense);
-      h->bitmap = nullptr;
-      h->dense = nullptr;
-    } else {
-      h->range = range;
-      st = fill_u32(ctx, h->bitmap, 0u, nwords);
-      if (st == GHIVE_OK) {
-        k_dense_fill<<<grid_for(ctx, n), kHashThreads, 0, ctx->stream>>>(
-            key->data, key->type, sel, n, payload ? payload->data : nullptr, payload ? payload->type : 0,
-            h->key_min, range, h->bitmap, h->dense);
-        if (hipGetLastError() != hipSuccess) st = set_error(ctx, GHIVE_ERR_HIP, "hash_build: dense launch failed");
-      }
-      if (st != GHIVE_OK) {
-        ghive_hash_table_free(ctx, h);
-        return st;
-      }
-    }
+  st = finish_build(ctx, h, host, key->data, sel, n, n_dev, pay, pay_type);
+  if (st != GHIVE_OK) {
+    ghive_hash_table_free(ctx, h);
+    return st;
   }
   ctx->tables.insert(h);
   *out = h;
+  return GHIVE_OK;
+}
+
+int hash_build(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, uint64_t n,
+               const ghive_column *payload, float load_factor, ghive_hash_table **out) {
+  return hash_build_core(ctx, key, sel, n, nullptr, payload, load_factor, false, out);
+}
+
+int hash_table_ready(ghive_ctx *caller, const ghive_hash_table *hc) {
+  auto *h = const_cast<ghive_hash_table *>(hc);  // finishing is a deferred part of the build
+  if (!h || !h->pending.load()) return GHIVE_OK;
+  std::lock_guard<std::mutex> lock(h->ready_mu);
+  if (!h->pending.load()) return GHIVE_OK;
+  ghive_ctx *ctx = h->owner;
+  if (!ctx) return GHIVE_ERR_INVALID;
+  uint64_t host[kStatWords];
+  GH_TRY(d2h(ctx, host, h->stats_dev, sizeof host));
+  ctx->arena->free(h->stats_buf);
+  h->stats_buf = nullptr;
+  h->stats_dev = nullptr;
+  GH_TRY(finish_build(ctx, h, host, h->in_key, h->in_sel, h->in_n_max, h->in_n_dev, h->in_pay, h->in_pay_type));
+  // another context's stream does not follow the owner's: let the dense companion land first
+  if (caller != ctx) GH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  h->pending.store(false);
   return GHIVE_OK;
 }
 
@@ -513,9 +563,19 @@ int ghive_hash_build(ghive_ctx *ctx, const ghive_column *key, const uint32_t *se
   return hash_build(ctx, key, sel, n, payload, load_factor, out);
 }
 
+int ghive_hash_build_async(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, const uint64_t *n_dev,
+                           uint64_t n_max, const ghive_column *payload, float load_factor, ghive_hash_table **out) {
+  GH_CHECK_CTX(ctx);
+  if (!n_dev) return set_error(ctx, GHIVE_ERR_INVALID, "hash_build_async: n_dev is NULL");
+  if (!sel && n_max != key->row_num)
+    return set_error(ctx, GHIVE_ERR_INVALID, "hash_build_async: n_max != rows without a selection");
+  return hash_build_core(ctx, key, sel, n_max, n_dev, payload, load_factor, true, out);
+}
+
 int ghive_hash_table_info(const ghive_hash_table *ht, uint64_t *entries, uint64_t *slots,
                           uint64_t *bytes, int32_t *unique_keys) {
   if (!ht) return GHIVE_ERR_INVALID;
+  if (ht->owner) GH_TRY(hash_table_ready(ht->owner, ht));
   if (entries) *entries = ht->entries;
   if (slots) *slots = ht->slots;
   if (bytes) *bytes = ht->slots * (ht->layout == 8 ? 8 : 16);
@@ -533,6 +593,7 @@ void ghive_hash_table_free(ghive_ctx *ctx, ghive_hash_table *ht) {
     if (ht->data) c->arena->free(ht->data);
     if (ht->bitmap) c->arena->free(ht->bitmap);
     if (ht->dense) c->arena->free(ht->dense);
+    if (ht->stats_buf) c->arena->free(ht->stats_buf);
     c->tables.erase(ht);
   }
   delete ht;
@@ -544,6 +605,7 @@ int ghive_hash_probe(ghive_ctx *ctx, const ghive_hash_table *ht, const ghive_col
   GH_CHECK_CTX(ctx);
   if (!ht || !key || !out_count) return set_error(ctx, GHIVE_ERR_INVALID, "hash_probe: NULL argument");
   if (!ht->owner) return set_error(ctx, GHIVE_ERR_INVALID, "hash_probe: the table's context was shut down");
+  GH_TRY(hash_table_ready(ctx, ht));
   if (!is_int_type(key->type))
     return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "hash_probe: key type %d (INT/LONG only)", key->type);
   if (!sel && n != key->row_num)

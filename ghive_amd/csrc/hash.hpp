@@ -9,6 +9,9 @@
 // so no real key can be confused with an empty slot.
 #pragma once
 
+#include <atomic>
+#include <mutex>
+
 #include "common.hpp"
 #include "device.hpp"
 
@@ -29,9 +32,25 @@ struct ghive_hash_table {
   int32_t *dense;    // payload at (key - key_min)
   uint64_t range;    // key_max - key_min + 1
   ghive_ctx *owner;
+  // Asynchronous build: the statistics above are still on the device (read, and the dense
+  // companion built, on the table's first use); the build inputs are kept for that.
+  std::atomic<bool> pending{false};
+  void *stats_buf = nullptr;              // arena block: per-workgroup partials, then the statistics
+  const unsigned long long *stats_dev = nullptr;
+  const void *in_key = nullptr, *in_pay = nullptr;
+  int in_pay_type = 0;
+  const uint32_t *in_sel = nullptr;
+  const uint64_t *in_n_dev = nullptr;
+  uint64_t in_n_max = 0;
+  std::mutex ready_mu;
 };
 
 namespace ghive {
+
+// Reads the statistics of an asynchronously built table and builds its dense companion, once;
+// every consumer of a table calls it first (caller: the consumer's context, whose stream must
+// see the finished table). A no-op for tables built synchronously.
+int hash_table_ready(ghive_ctx *caller, const ghive_hash_table *h);
 
 constexpr uint64_t kEmpty8 = 0x000000007fffffffull;
 constexpr uint64_t kEmpty16 = 0x7fffffffffffffffull;

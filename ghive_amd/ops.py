@@ -216,6 +216,16 @@ class FilterPredicate:
                                      _p(out), C.byref(cnt)))
         return out[: cnt.value]
 
+    def process_async(self, ctx, table):
+        """(ids, count): the passing rows' ids in the first `count` entries of a row_num-long
+        tensor and their count as a 1-element device tensor, with no wait on the host."""
+        dev = table.columns[0].device if table.columns else "cuda"
+        out = torch.empty(max(table.row_num, 1), dtype=torch.int32, device=dev)
+        cnt = torch.empty(1, dtype=torch.int64, device=dev)
+        ctx.check(lib().ghive_filter_async(ctx.h, table.ref(), self._arr, len(self.nodes), self.root,
+                                           _p(out), _p(cnt)))
+        return out, cnt
+
 
 class FilterOperator:
     """FilterOperator: EXPLAIN predicate text + the operator's output columns."""
@@ -257,15 +267,22 @@ class FilterOperator:
 class HashTable:
     """Device hash table of a build side (unique or duplicate keys)."""
 
-    def __init__(self, ctx, key, sel=None, payload=None, load_factor=0.5):
+    def __init__(self, ctx, key, sel=None, payload=None, load_factor=0.5, count=None):
+        """count: a 1-element device tensor holding how many entries of sel to insert (from
+        FilterPredicate.process_async): the build then leaves its statistics on the device until
+        the table's first use, so that filter, build and the consumer wait once."""
         self.ctx = ctx
         kt = Table([key] + ([payload] if payload is not None else []))
-        self._keep = (kt, key, payload, sel)
+        self._keep = (kt, key, payload, sel, count)
         n = sel.numel() if sel is not None else key.numel()
         h = C.c_void_p()
         pcol = C.byref(kt.cols[1]) if payload is not None else None
-        ctx.check(lib().ghive_hash_build(ctx.h, C.byref(kt.cols[0]), _p(sel), n, pcol, load_factor,
-                                         C.byref(h)))
+        if count is not None:
+            ctx.check(lib().ghive_hash_build_async(ctx.h, C.byref(kt.cols[0]), _p(sel), _p(count), n, pcol,
+                                                   load_factor, C.byref(h)))
+        else:
+            ctx.check(lib().ghive_hash_build(ctx.h, C.byref(kt.cols[0]), _p(sel), n, pcol, load_factor,
+                                             C.byref(h)))
         self.h = h
 
     def info(self):

@@ -102,17 +102,26 @@ def star_tables(ctx, fact_rows=600_000_000, seed=SEED_BASE + 2):
     return StarTables(star_fact(ctx, fact_rows, seed), date, supplier, part, customer, seed, fact_rows)
 
 
-def star_hash_tables(ctx, T, year_lo=1992, year_hi=1993, region=1, category=1, load_factor=0.5):
-    """Filtered dimension hash tables: date (-> year), supplier (existence), part (-> brand)."""
+def star_hash_tables(ctx, T, year_lo=1992, year_hi=1993, region=1, category=1, load_factor=0.5,
+                     wait_each=False):
+    """Filtered dimension hash tables: date (-> year), supplier (existence), part (-> brand).
+    By default each filter hands its count to the build on the device (no host round trip until
+    the tables are first used); wait_each=True reads every count on the host first."""
     P = ops.FilterPredicate
-    dsel = P([abi.make_pred(abi.FILTER_EQ_RANGE, 1, [year_lo, year_hi], data_type=2)], 0).process(ctx, T.date)
-    ssel = P([abi.make_pred(abi.FILTER_EQ, 1, [region], data_type=2)], 0).process(ctx, T.supplier)
-    psel = P([abi.make_pred(abi.FILTER_EQ, 1, [category], data_type=2)], 0).process(ctx, T.part)
-    hd = ops.HashTable(ctx, T.date.columns[0], sel=dsel, payload=T.date.columns[1], load_factor=load_factor)
-    hs = ops.HashTable(ctx, T.supplier.columns[0], sel=ssel, payload=T.supplier.columns[1],
-                       load_factor=load_factor)
-    hp = ops.HashTable(ctx, T.part.columns[0], sel=psel, payload=T.part.columns[2], load_factor=load_factor)
-    return hd, hs, hp
+    preds = [(T.date, [abi.make_pred(abi.FILTER_EQ_RANGE, 1, [year_lo, year_hi], data_type=2)], 1),
+             (T.supplier, [abi.make_pred(abi.FILTER_EQ, 1, [region], data_type=2)], 1),
+             (T.part, [abi.make_pred(abi.FILTER_EQ, 1, [category], data_type=2)], 2)]
+    tables = []
+    for dim, nodes, pay_col in preds:
+        if wait_each:
+            sel = P(nodes, 0).process(ctx, dim)
+            tables.append(ops.HashTable(ctx, dim.columns[0], sel=sel, payload=dim.columns[pay_col],
+                                        load_factor=load_factor))
+        else:
+            sel, cnt = P(nodes, 0).process_async(ctx, dim)
+            tables.append(ops.HashTable(ctx, dim.columns[0], sel=sel, payload=dim.columns[pay_col],
+                                        load_factor=load_factor, count=cnt))
+    return tuple(tables)
 
 
 def star_fragment(fact, hd, hs, hp):

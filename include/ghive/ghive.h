@@ -204,6 +204,12 @@ size_t ghive_predicate_to_string(const ghive_pred *nodes, uint32_t nnodes, uint3
  * the rows that pass, in ascending order, to out_sel (device, capacity row_num). */
 int ghive_filter(ghive_ctx *ctx, const ghive_table *table, const ghive_pred *nodes,
                  uint32_t nnodes, uint32_t root, uint32_t *out_sel, uint64_t *out_count);
+/* ghive_filter with no wait on the host (no counterpart in the reference, whose operators each
+ * return to the host): the passing rows' ids go to out_sel as above and their count to
+ * *count_dev (device memory, 8 bytes), ready for a consumer on the same stream such as
+ * ghive_hash_build_async. */
+int ghive_filter_async(ghive_ctx *ctx, const ghive_table *table, const ghive_pred *nodes,
+                       uint32_t nnodes, uint32_t root, uint32_t *out_sel, uint64_t *count_dev);
 /* clean_data compaction form: every column of `table` gathered by the passing rows into
  * out_cols[i].data (device buffers of capacity table->row_num, caller-owned). Unlike the
  * reference the input is not modified in place. A STRING column gathers its (start,end) pairs
@@ -234,6 +240,15 @@ typedef struct ghive_hash_table ghive_hash_table;
  * the build row id when payload == NULL. load_factor in (0, 0.95]; 0 = 0.5. */
 int ghive_hash_build(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, uint64_t n,
                      const ghive_column *payload, float load_factor, ghive_hash_table **out);
+/* ghive_hash_build over rows sel[0 .. *n_dev) where the count is on the device (at most n_max,
+ * e.g. from ghive_filter_async), with no wait on the host: the table is sized for n_max and its
+ * statistics (entry count, key range, dense companion) are read when it is first used (probe,
+ * info, fragment, batch), so that a chain filter -> build -> fused fragment waits once. key,
+ * payload, sel and n_dev must stay valid until that first use. No counterpart in the reference
+ * (HashTableLoader builds from rows already on the host). */
+int ghive_hash_build_async(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel,
+                           const uint64_t *n_dev, uint64_t n_max, const ghive_column *payload,
+                           float load_factor, ghive_hash_table **out);
 int ghive_hash_table_info(const ghive_hash_table *ht, uint64_t *entries, uint64_t *slots,
                           uint64_t *bytes, int32_t *unique_keys);
 void ghive_hash_table_free(ghive_ctx *ctx, ghive_hash_table *ht);

@@ -390,3 +390,43 @@ def test_bandwidth_probe_copies_and_reads(ctx, nbytes):
     assert torch.equal(dst, src)
     ctx.check(lib().ghive_bench_stream(ctx.h, None, C.c_void_p(src.data_ptr()), nbytes, 0, 2, C.byref(ms)))
     assert ms.value > 0
+
+
+def test_async_filter_and_build_match_the_waiting_forms(ctx, orc):
+    """Filters that leave their count on the device and builds that read their statistics on
+    first use give the same tables and fragment results as the forms that wait after each step;
+    an empty selection builds an empty table."""
+    import torch
+    from ghive_amd import ops, workloads as W
+    T = W.star_tables(ctx, 2_000_003)
+    sync_tabs = W.star_hash_tables(ctx, T, wait_each=True)
+    async_tabs = W.star_hash_tables(ctx, T)
+    for a, b in zip(sync_tabs, async_tabs):
+        ia, ib = a.info(), b.info()
+        assert (ia["entries"], ia["unique"]) == (ib["entries"], ib["unique"])
+    got_sync = _gpu_dict(W.star_fragment(T.fact, *sync_tabs).execute(ctx, capacity=8192), 2)
+    got_async = _gpu_dict(W.star_fragment(T.fact, *W.star_hash_tables(ctx, T)).execute(ctx, capacity=8192), 2)
+    assert got_sync == got_async == _orc_dict(_star_oracle(orc, T))
+    # the filter's device count equals the waiting filter's count
+    pred = ops.FilterPredicate([abi.make_pred(abi.FILTER_EQ, 1, [1], data_type=2)], 0)
+    sel_a, cnt = pred.process_async(ctx, T.part)
+    sel_s = pred.process(ctx, T.part)
+    assert int(cnt.item()) == sel_s.numel()
+    assert torch.equal(sel_a[: sel_s.numel()], sel_s)
+    # probe through an asynchronously built table
+    sel_d, cnt_d = ops.FilterPredicate([abi.make_pred(abi.FILTER_EQ_RANGE, 1, [1992, 1993], data_type=2)],
+                                       0).process_async(ctx, T.date)
+    hd = ops.HashTable(ctx, T.date.columns[0], sel=sel_d, payload=T.date.columns[1], count=cnt_d)
+    rows, pay = hd.probe(T.fact.columns[0][:100_000])
+    hs = ops.HashTable(ctx, T.date.columns[0], sel=sel_d[: int(cnt_d.item())], payload=T.date.columns[1])
+    rows2, pay2 = hs.probe(T.fact.columns[0][:100_000])
+    key = lambda r, p: sorted(zip(r.cpu().tolist(), p.cpu().tolist()))
+    assert key(rows, pay) == key(rows2, pay2)
+    # nothing selected
+    sel_e, cnt_e = ops.FilterPredicate([abi.make_pred(abi.FILTER_EQ, 1, [99], data_type=2)], 0).process_async(ctx, T.part)
+    he = ops.HashTable(ctx, T.part.columns[0], sel=sel_e, payload=T.part.columns[2], count=cnt_e)
+    assert he.info()["entries"] == 0
+    frag = W.star_fragment(T.fact, async_tabs[0], async_tabs[1], he)
+    assert _gpu_dict(frag.execute(ctx, capacity=8192), 2) == {}
+    for h in (*sync_tabs, *async_tabs, hd, hs, he):
+        h.free()
