@@ -1,6 +1,6 @@
 """Autotuner of the batched form: tile size (rows per tile), workgroups per CU, and the probe
-structure of the shared dimension (LDS bitmap, HBM bitmap, or the open-addressing hash table at
-load factor 0.5 / 0.7 / 0.9). Every candidate is timed on the actual batch (HIP graph replay,
+structure of the shared dimension (LDS bitmap with or without ranked group codes, HBM bitmap, or
+the open-addressing hash table at load factor 0.5 / 0.7 / 0.9). Every candidate is timed on the actual batch (HIP graph replay,
 best of `reps`), the fastest wins, and the whole search is reported (time spent and per-candidate
 results) and cached in a JSON file keyed by the batch signature.
 """
@@ -9,11 +9,12 @@ import os
 import time
 
 from . import workloads as W
+from ._lib import GhiveError
 
 TILE_ROWS = (2048, 4096, 8192, 16384, 32768)
 BLOCKS_PER_CU = (2, 4, 8)
-# (probe_mode, load_factor): 1 LDS bitmap, 2 HBM bitmap, 3 hash table
-PROBES = ((1, 0.5), (2, 0.5), (3, 0.5), (3, 0.7), (3, 0.9))
+# (probe_mode, load_factor): 1 LDS bitmap, 2 HBM bitmap, 3 hash table, 4 LDS bitmap + ranked group codes
+PROBES = ((4, 0.5), (1, 0.5), (2, 0.5), (3, 0.5), (3, 0.7), (3, 0.9))
 
 
 def signature(ctx, B):
@@ -39,7 +40,10 @@ def tune_batch(ctx, B, tile_rows=TILE_ROWS, blocks_per_cu=BLOCKS_PER_CU, probes=
         ht = tables[lf]
         for tr in tile_rows:
             for bpc in blocks_per_cu:
-                bt = W.batch_create(ctx, B, ht, tile_rows=tr, blocks_per_cu=bpc, probe_mode=mode)
+                try:
+                    bt = W.batch_create(ctx, B, ht, tile_rows=tr, blocks_per_cu=bpc, probe_mode=mode)
+                except GhiveError:  # a probe mode this shape does not support (e.g. tables too large for LDS)
+                    continue
                 ms = min(bt.run() for _ in range(reps))
                 info = bt.info()
                 bt.free()

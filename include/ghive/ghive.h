@@ -429,7 +429,9 @@ typedef struct ghive_batch_spec {
   uint32_t group_shift, group_bits;
 } ghive_batch_spec;
 /* Autotuned knobs: rows per tile (multiple of 1024; 0 = 8192), workgroups per CU (0 = 4),
- * probe mode (0 auto, 1 LDS bitmap, 2 HBM bitmap, 3 hash table). */
+ * probe mode (0 auto, 1 LDS bitmap, 2 HBM bitmap, 3 hash table, 4 LDS bitmap with the group
+ * field of every passing key ranked beside it, for group_bits <= 8: no per-row read of the
+ * dimension payload). */
 typedef struct ghive_batch_tuning {
   uint32_t tile_rows;
   uint32_t blocks_per_cu;

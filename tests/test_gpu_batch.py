@@ -30,7 +30,7 @@ def _batch_dicts(sums, counts):
     return [{g: int(sums[f, g]) for g in np.nonzero(counts[f])[0].tolist()} for f in range(sums.shape[0])]
 
 
-@pytest.mark.parametrize("probe_mode,lf", [(1, 0.5), (2, 0.5), (3, 0.5), (3, 0.9)])
+@pytest.mark.parametrize("probe_mode,lf", [(1, 0.5), (2, 0.5), (3, 0.5), (3, 0.9), (4, 0.5)])
 @pytest.mark.parametrize("tile_rows,use_graph", [(1024, True), (8192, True), (4096, False)])
 def test_batch_equals_plain_per_fragment(ctx, probe_mode, lf, tile_rows, use_graph):
     from ghive_amd import workloads as W
@@ -46,6 +46,26 @@ def test_batch_equals_plain_per_fragment(ctx, probe_mode, lf, tile_rows, use_gra
     plain = W.PlainBatch(ctx, B).run(collect=True)
     assert got == plain
     assert got[0] == {} and len(got) == len(B.rows)
+    bt.free()
+    ht.free()
+
+
+def test_batch_ranked_codes_mode(ctx):
+    """Probe mode 4 (filtered bitmap with ranked group codes in LDS) is the automatic choice for
+    this shape; fragments whose filter passes no dimension key, and several filters, agree with
+    the plain per-fragment operator."""
+    from ghive_amd import workloads as W
+    B = _small_batch(ctx, nfrags=40, seed=99)
+    for f in (3, 4, 17):
+        B.attr_const[f] = 99  # no dimension row has attr 99
+    ht = W.batch_dim_table(ctx, B)
+    bt = W.batch_create(ctx, B, ht, tile_rows=2048, blocks_per_cu=2)
+    assert bt.info()["probe_mode"] == 4
+    bt.run()
+    got = _batch_dicts(*bt.results())
+    assert got == W.PlainBatch(ctx, B).run(collect=True)
+    assert got[3] == {} and got[4] == {} and got[17] == {}
+    assert sum(1 for g in got if g) > 30
     bt.free()
     ht.free()
 
