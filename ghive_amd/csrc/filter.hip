@@ -169,14 +169,13 @@ static int filter_run(ghive_ctx *ctx, const ghive_table *t, const ghive_pred *no
                                                                               counts.as<uint32_t>());
     GH_LAUNCH(ctx);
   }
-  GH_TRY(exclusive_scan_u32(ctx, counts.as<uint32_t>(), counts.as<uint32_t>(), ntiles, ctx->d_counters));
+  // the asynchronous form takes the count where the scan leaves its total, with no copy after
+  GH_TRY(exclusive_scan_u32(ctx, counts.as<uint32_t>(), counts.as<uint32_t>(), ntiles,
+                            count_dev ? count_dev : ctx->d_counters));
   k_filter_write<<<ntiles, kFilterThreads, 0, ctx->stream>>>(masks.as<uint64_t>(), nwords,
                                                              counts.as<uint32_t>(), out_sel);
   GH_LAUNCH(ctx);
-  if (count_dev) {
-    GH_HIP(ctx, hipMemcpyAsync(count_dev, ctx->d_counters, sizeof(uint64_t), hipMemcpyDeviceToDevice, ctx->stream));
-    return GHIVE_OK;
-  }
+  if (count_dev) return GHIVE_OK;
   GH_TRY(d2h(ctx, out_count, ctx->d_counters, sizeof(uint64_t)));
   return GHIVE_OK;
 }
