@@ -442,23 +442,43 @@ def main():
     torch.cuda.synchronize()
     log(f"generated {args.rows} fact rows in {time.time() - t0:.1f}s")
 
+    host = {"filters_builds": [], "fragment": [], "free": []}
+
     def step():
+        t_a = time.perf_counter()
         hd, hs, hp = W.star_hash_tables(ctx, T)
+        t_b = time.perf_counter()
         out = W.star_fragment(T.fact, hd, hs, hp).execute(ctx, capacity=8192)
+        t_c = time.perf_counter()
         for h in (hd, hs, hp):
             h.free()
+        t_d = time.perf_counter()
+        for k, v in zip(host, (t_b - t_a, t_c - t_b, t_d - t_c)):
+            host[k].append(1e3 * v)
         return out
 
     for _ in range(args.warmup):
         out = step()
     torch.cuda.synchronize()
-    kernel_ms = []
+    for v in host.values():
+        v.clear()
+    kernel_ms, fin_ms = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         out = step()
-        kernel_ms.append(ctx.stage_times()["main_kernel_ms"])
+        stt = ctx.stage_times()
+        kernel_ms.append(stt["main_kernel_ms"])
+        fin_ms.append(stt["finalize_ms"])
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    breakdown = {"host_ms_median": {k: float(np.median(v)) for k, v in host.items()},
+                 "device_ms_mean": {"fused_kernel": float(np.mean(kernel_ms)),
+                                    "group_write_out": float(np.mean(fin_ms))},
+                 "note": "host_ms: wall time of each call of the step (filters + builds are submitted without "
+                         "a host wait; the fragment call holds the step's host waits: one read-back of the "
+                         "three builds' statistics and one of the group count); device_ms: HIP events on "
+                         "the context stream"}
+    log(f"step breakdown: {breakdown}")
     st = ctx.stage_times()
     groups = int(out[0].numel())
     ms_per_step = 1e3 * elapsed / args.steps
@@ -510,7 +530,7 @@ def main():
                                "supplier(200k) x part(1.4M), GROUP BY year, brand SUM(price*discount)",
                    "fact_rows": args.rows, "groups": groups, "rows_joined": st["rows_joined"],
                    "kernel_variant": variant,
-                   "fused_kernel_ms": k_ms},
+                   "fused_kernel_ms": k_ms, "step_breakdown": breakdown},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "measured_read_peak": read_gbs, "frac_of_measured_read_peak": achieved / read_gbs,

@@ -690,8 +690,17 @@ int ghive_batch_create(ghive_ctx *ctx, const ghive_batch_spec *spec, const ghive
     } else if (st == GHIVE_OK && acc_lds + bt->fcopy > 160 * 1024) {
       st = set_error(ctx, GHIVE_ERR_UNSUPPORTED, "batch: ranked filter tables do not fit LDS");
     }
-    if (mode == 4) bt->lds_bytes = acc_lds + bt->fcopy;
-    else bt->lds_bytes = acc_lds + (mode == 1 ? (size_t)nwords * 4 : 0);
+    if (mode == 4) {
+      bt->lds_bytes = acc_lds + bt->fcopy;
+    } else {
+      bt->lds_bytes = acc_lds + (mode == 1 ? (size_t)nwords * 4 : 0);
+      // modes 1 and 2 never read the ranked tables: give their memory back now
+      if (hipStreamSynchronize(bt->stream) != hipSuccess && st == GHIVE_OK)
+        st = set_error(ctx, GHIVE_ERR_HIP, "batch: sync");
+      if (bt->d_franks) ctx->arena->free(bt->d_franks);
+      bt->d_franks = nullptr;
+      bt->fstride = bt->fcopy = bt->nwe = 0;
+    }
   }
   bt->mode = mode;
   if (st == GHIVE_OK && hipStreamSynchronize(bt->stream) != hipSuccess) st = set_error(ctx, GHIVE_ERR_HIP, "batch: sync");

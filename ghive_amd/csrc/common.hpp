@@ -186,7 +186,7 @@ int check_table(ghive_ctx *ctx, const ghive_table *t, const char *what);
 // Synchronous device->host copy on the context stream.
 int d2h(ghive_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 // Reads up to kMaxReadWords device words (src[i] -> dst[i]) with one wait on the stream.
-constexpr int kMaxReadWords = 16;
+constexpr int kMaxReadWords = 64;
 int d2h_words(ghive_ctx *ctx, int n, uint64_t *const *dst, const uint64_t *const *src);
 int h2d(ghive_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 // Launch helpers (defined in kernels_util.hip).

@@ -214,14 +214,18 @@ const char *ghive_last_error(const ghive_ctx *ctx) {
   return ctx ? ctx->last_error.c_str() : "no context";
 }
 
+// Memory the context's manager hands out again is used in the order of the context's stream, so
+// the old stream's work is waited for before the stream changes.
 int ghive_set_stream(ghive_ctx *ctx, void *hip_stream) {
   GH_CHECK_CTX(ctx);
+  if (ctx->stream != static_cast<hipStream_t>(hip_stream)) GH_HIP(ctx, hipStreamSynchronize(ctx->stream));
   ctx->stream = static_cast<hipStream_t>(hip_stream);  // NULL: the HIP null stream
   return GHIVE_OK;
 }
 
 int ghive_use_own_stream(ghive_ctx *ctx) {
   GH_CHECK_CTX(ctx);
+  if (ctx->stream != ctx->own_stream) GH_HIP(ctx, hipStreamSynchronize(ctx->stream));
   ctx->stream = ctx->own_stream;
   return GHIVE_OK;
 }

@@ -64,11 +64,16 @@ int fragment_plan(ghive_ctx *ctx, const ghive_fragment *f, int late, FragPlan &P
   const uint64_t bits_budget_words = (uint64_t)std::min<int64_t>(ctx->tune.frag_lds_budget, 48 * 1024) / 4;
   std::vector<int> mode(f->njoins, kJoinHash);
   std::vector<uint32_t> by_size;
+  std::vector<const ghive_hash_table *> tables(f->njoins);
   for (uint32_t j = 0; j < f->njoins; ++j) {
     const ghive_hash_table *h = f->joins[j].table;
     if (!h) return set_error(ctx, GHIVE_ERR_INVALID, "fragment: join %u has no table", j);
     if (!h->owner) return set_error(ctx, GHIVE_ERR_INVALID, "fragment: join %u's table: its context was shut down", j);
-    GH_TRY(hash_table_ready(ctx, h));
+    tables[j] = h;
+  }
+  GH_TRY(hash_tables_ready(ctx, tables.data(), (int)f->njoins));  // one wait for every pending build
+  for (uint32_t j = 0; j < f->njoins; ++j) {
+    const ghive_hash_table *h = f->joins[j].table;
     if (h->layout != 8 || !h->unique)
       return set_error(ctx, GHIVE_ERR_UNSUPPORTED,
                        "fragment: join %u needs a unique-key INT table with an INT payload", j);

@@ -495,24 +495,106 @@ int hash_build(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, uin
   return hash_build_core(ctx, key, sel, n, nullptr, payload, load_factor, false, out);
 }
 
-int hash_table_ready(ghive_ctx *caller, const ghive_hash_table *hc) {
-  auto *h = const_cast<ghive_hash_table *>(hc);  // finishing is a deferred part of the build
-  if (!h || !h->pending.load()) return GHIVE_OK;
-  std::lock_guard<std::mutex> lock(h->ready_mu);
-  if (!h->pending.load()) return GHIVE_OK;
-  ghive_ctx *ctx = h->owner;
-  if (!ctx) return GHIVE_ERR_INVALID;
-  uint64_t host[kStatWords];
-  GH_TRY(d2h(ctx, host, h->stats_dev, sizeof host));
-  ctx->arena->free(h->stats_buf);
+// A table whose deferred finish failed keeps the status and message; consumers get them back.
+static int fail_table(ghive_ctx *owner, ghive_hash_table *h, int st) {
+  h->failed = st;
+  h->fail_msg = owner->last_error;
+  if (h->stats_buf) {
+    (void)hipStreamSynchronize(owner->stream);  // no kernel may still write the memory given back
+    owner->arena->free(h->stats_buf);
+  }
   h->stats_buf = nullptr;
   h->stats_dev = nullptr;
-  GH_TRY(finish_build(ctx, h, host, h->in_key, h->in_sel, h->in_n_max, h->in_n_dev, h->in_pay, h->in_pay_type));
-  // another context's stream does not follow the owner's: let the dense companion land first
-  if (caller != ctx) GH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  h->pending.store(false);
+  return st;
+}
+
+static int table_failed(ghive_ctx *caller, const ghive_hash_table *h) {
+  return set_error(caller, h->failed, "%s", h->fail_msg.c_str());
+}
+
+// A table built on another context: its owner's stream is waited for, so that the statistics and
+// then the dense companion are complete before the caller's stream reads them.
+static int ready_foreign(ghive_ctx *caller, ghive_hash_table *h) {
+  std::lock_guard<std::mutex> lock(h->ready_mu);
+  if (!h->pending.load()) return h->failed ? table_failed(caller, h) : GHIVE_OK;
+  ghive_ctx *ctx = h->owner;
+  if (!ctx) return set_error(caller, GHIVE_ERR_INVALID, "hash table: the context that built it was shut down");
+  uint64_t host[kStatWords];
+  int st = d2h(ctx, host, h->stats_dev, sizeof host);
+  if (st == GHIVE_OK) {
+    ctx->arena->free(h->stats_buf);
+    h->stats_buf = nullptr;
+    h->stats_dev = nullptr;
+    st = finish_build(ctx, h, host, h->in_key, h->in_sel, h->in_n_max, h->in_n_dev, h->in_pay, h->in_pay_type);
+  }
+  if (st == GHIVE_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
+    st = set_error(ctx, GHIVE_ERR_HIP, "hash table: waiting for the building context failed");
+  if (st != GHIVE_OK) {
+    fail_table(ctx, h, st);
+    return table_failed(caller, h);
+  }
   h->pending.store(false);
   return GHIVE_OK;
 }
+
+int hash_tables_ready(ghive_ctx *caller, const ghive_hash_table *const *hs, int n) {
+  std::vector<ghive_hash_table *> mine;  // pending tables the caller built
+  for (int i = 0; i < n; ++i) {
+    auto *h = const_cast<ghive_hash_table *>(hs[i]);  // finishing is a deferred part of the build
+    if (!h) continue;
+    if (!h->pending.load()) {
+      if (h->failed) return table_failed(caller, h);
+      continue;
+    }
+    if (h->owner != caller) {
+      GH_TRY(ready_foreign(caller, h));
+      continue;
+    }
+    if (std::find(mine.begin(), mine.end(), h) == mine.end()) mine.push_back(h);
+  }
+  std::sort(mine.begin(), mine.end());  // one locking order for every caller
+  constexpr size_t kBatch = kMaxReadWords / kStatWords;
+  for (size_t b = 0; b < mine.size(); b += kBatch) {
+    const size_t e = std::min(mine.size(), b + kBatch);
+    std::vector<std::unique_lock<std::mutex>> locks;
+    std::vector<ghive_hash_table *> todo;
+    for (size_t i = b; i < e; ++i) {
+      locks.emplace_back(mine[i]->ready_mu);
+      if (mine[i]->pending.load()) todo.push_back(mine[i]);
+    }
+    int st = GHIVE_OK;
+    for (size_t i = 0; i < todo.size() && st == GHIVE_OK; ++i)
+      if (hipMemcpyAsync(caller->h_words + kStatWords * i, todo[i]->stats_dev, kStatWords * sizeof(uint64_t),
+                         hipMemcpyDeviceToHost, caller->stream) != hipSuccess)
+        st = set_error(caller, GHIVE_ERR_HIP, "hash table: statistics read-back failed");
+    if (st == GHIVE_OK && hipStreamSynchronize(caller->stream) != hipSuccess)
+      st = set_error(caller, GHIVE_ERR_HIP, "hash table: statistics read-back failed");
+    for (size_t i = 0; i < todo.size(); ++i) {
+      ghive_hash_table *h = todo[i];
+      if (st != GHIVE_OK) {
+        fail_table(caller, h, st);
+        continue;
+      }
+      uint64_t host[kStatWords];
+      for (int w = 0; w < kStatWords; ++w) host[w] = caller->h_words[kStatWords * i + w];
+      caller->arena->free(h->stats_buf);  // the read-back above has completed
+      h->stats_buf = nullptr;
+      h->stats_dev = nullptr;
+      const int s = finish_build(caller, h, host, h->in_key, h->in_sel, h->in_n_max, h->in_n_dev, h->in_pay,
+                                 h->in_pay_type);
+      if (s != GHIVE_OK) {
+        st = fail_table(caller, h, s);
+        continue;
+      }
+      h->pending.store(false);
+    }
+    if (st != GHIVE_OK) return st;
+  }
+  return GHIVE_OK;
+}
+
+int hash_table_ready(ghive_ctx *caller, const ghive_hash_table *h) { return hash_tables_ready(caller, &h, 1); }
 
 // Probe into caller buffers; on overflow returns CAPACITY with the needed count in *count.
 static int probe_run(ghive_ctx *ctx, const ghive_hash_table *h, const ghive_column *key,
@@ -566,6 +648,7 @@ int ghive_hash_build(ghive_ctx *ctx, const ghive_column *key, const uint32_t *se
 int ghive_hash_build_async(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, const uint64_t *n_dev,
                            uint64_t n_max, const ghive_column *payload, float load_factor, ghive_hash_table **out) {
   GH_CHECK_CTX(ctx);
+  if (!key || !out) return set_error(ctx, GHIVE_ERR_INVALID, "hash_build_async: NULL argument");
   if (!n_dev) return set_error(ctx, GHIVE_ERR_INVALID, "hash_build_async: n_dev is NULL");
   if (!sel && n_max != key->row_num)
     return set_error(ctx, GHIVE_ERR_INVALID, "hash_build_async: n_max != rows without a selection");
@@ -588,8 +671,11 @@ void ghive_hash_table_free(ghive_ctx *ctx, ghive_hash_table *ht) {
   (void)ctx;
   ghive_ctx *c = ht->owner;  // the table's memory belongs to the context that built it (NULL:
   if (c) {                   // that context was shut down and released the memory already)
+    // Stream-ordered release, no wait: the memory goes back to the owner's memory manager, whose
+    // next user is work on the same stream, ordered after every kernel that read the table
+    // (ghive_set_stream waits for the old stream before switching). Work of other contexts that
+    // reads the table must have finished, as the API requires.
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
     if (ht->data) c->arena->free(ht->data);
     if (ht->bitmap) c->arena->free(ht->bitmap);
     if (ht->dense) c->arena->free(ht->dense);

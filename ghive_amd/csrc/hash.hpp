@@ -43,6 +43,9 @@ struct ghive_hash_table {
   const uint64_t *in_n_dev = nullptr;
   uint64_t in_n_max = 0;
   std::mutex ready_mu;
+  // finishing an asynchronous build failed: its status and message, returned to every consumer
+  int failed = 0;
+  std::string fail_msg;
 };
 
 namespace ghive {
@@ -51,6 +54,9 @@ namespace ghive {
 // every consumer of a table calls it first (caller: the consumer's context, whose stream must
 // see the finished table). A no-op for tables built synchronously.
 int hash_table_ready(ghive_ctx *caller, const ghive_hash_table *h);
+// The same for n tables at once: the statistics of the pending tables the caller owns are read
+// back with one wait on its stream.
+int hash_tables_ready(ghive_ctx *caller, const ghive_hash_table *const *tables, int n);
 
 constexpr uint64_t kEmpty8 = 0x000000007fffffffull;
 constexpr uint64_t kEmpty16 = 0x7fffffffffffffffull;

@@ -976,7 +976,14 @@ struct Executor {
     }
     if (o.kind == "Limit") {
       out = in;
-      if (o.limit >= 0 && (uint64_t)o.limit < out.rows) out.rows = (uint64_t)o.limit;
+      // In the plan's root vertex with a final ORDER BY, LIMIT keeps the first rows of the sorted
+      // result: ghive_plan_execute applies it after the sort, so nothing is cut here. (The
+      // reference's LimitOperator.cu:11 truncates before the sink sorts, which keeps an arbitrary
+      // subset of unsorted input such as group-by output.)
+      const auto vr = P.vertex_root.find(vertex);
+      const bool after_sort = vr != P.vertex_root.end() && vr->second == P.root && P.ops[P.root].require_sort &&
+                              !P.ops[P.root].sort_orders.empty();
+      if (!after_sort && o.limit >= 0 && (uint64_t)o.limit < out.rows) out.rows = (uint64_t)o.limit;
       for (auto &c : out.cols) c.row_num = out.rows;
       return GHIVE_OK;
     }

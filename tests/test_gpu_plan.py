@@ -338,3 +338,71 @@ def test_plan_projection_functions(ctx, orc):
     assert to_np(cols[1]).tolist() == [INT_NULL if v is None else v for v in orc.cast_string_to_int(vals)]
     d = to_np(cols[2])
     assert np.array_equal(np.isnan(d), n == LONG_NULL) and np.array_equal(d[n != LONG_NULL], n[n != LONG_NULL])
+
+
+_LIMIT_PLAN = """Plan optimized by CBO.
+
+Vertex dependency in root stage
+Reducer 2 <- Map 1 (SIMPLE_EDGE)
+
+Stage-0
+  Fetch Operator
+    limit:10
+    Stage-1
+      Reducer 2
+      File Output Operator [FS_6]
+        Limit [LIM_5] (rows=10 width=16)
+          Number of rows:10
+          Select Operator [SEL_4] (rows=1000 width=16)
+            Output:["_col0","_col1"]
+          <-Map 1 [SIMPLE_EDGE]
+            SHUFFLE [RS_3]
+              Select Operator [SEL_2] (rows=1000 width=16)
+                Output:["_col0","_col1"]
+                TableScan [TS_0] (rows=1000 width=16)
+                  db@t,t,Tbl:COMPLETE,Col:NONE,Output:["k","v"]
+"""
+_LIMIT_EXTENDED = """STAGE PLANS:
+  Stage: Stage-1
+    Tez
+      Vertices:
+        Map 1 
+            Map Operator Tree:
+                TableScan
+                  alias: t
+                  Select Operator
+                    expressions: k (type: bigint), v (type: bigint)
+                    outputColumnNames: _col0, _col1
+                    Reduce Output Operator
+                      key expressions: _col0 (type: bigint)
+                      null sort order: a
+                      sort order: +
+                      tag: -1
+                      value expressions: _col1 (type: bigint)
+        Reducer 2 
+            Reduce Operator Tree:
+              Select Operator
+                expressions: KEY.reducesinkkey0 (type: bigint), VALUE._col0 (type: bigint)
+                outputColumnNames: _col0, _col1
+                Limit
+                  Number of rows: 10
+                  File Output Operator
+                    compressed: false
+"""
+
+
+def test_plan_limit_applies_after_order_by(ctx):
+    """ORDER BY k LIMIT 10 over a reduce input in hash order (a hand-written two-vertex plan in
+    EXPLAIN / EXPLAIN EXTENDED form): the ten smallest keys, not a sort of ten arbitrary rows."""
+    from ghive_amd import ops
+    rng = np.random.default_rng(91)
+    k = rng.permutation(1000).astype(np.int64) * 3 - 500
+    v = k * 7 + 1
+    plan = ops.Plan(_LIMIT_PLAN, _LIMIT_EXTENDED)
+    # the map side passes its rows through (keys first)
+    names, cols = plan.execute(ctx, "Map 1", {"db@t": (ops.Table([to_dev(k), to_dev(v)]), ["k", "v"])})
+    assert names == ["_col0", "_col1"]
+    names, (rk, rv) = plan.execute(ctx, "Reducer 2", {"Map 1": (ops.Table(cols), None)})
+    order = np.lexsort((np.arange(len(k)), k))[:10]
+    assert to_np(rk).tolist() == k[order].tolist()
+    assert to_np(rv).tolist() == v[order].tolist()
