@@ -70,6 +70,18 @@ SIGNATURES = {
     "ghive_set_option": (C.c_int, [P, C.c_char_p, i64]),
     "ghive_get_option": (C.c_int, [P, C.c_char_p, C.POINTER(i64)]),
     "ghive_last_stage_times": (C.c_int, [P, C.POINTER(abi.StageTimes)]),
+    "ghive_last_plan_info": (C.c_int, [P, C.POINTER(abi.PlanExecInfo)]),
+    "ghive_gpu_process": (C.c_int, [P, P, C.POINTER(abi.GTables), C.c_char_p, C.c_char_p, C.c_uint64,
+                                    C.POINTER(P)]),
+    "ghive_gpu_result_info": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                        C.POINTER(C.POINTER(C.c_int32))]),
+    "ghive_gpu_result_long": (C.POINTER(C.c_int64), [P, C.c_uint32]),
+    "ghive_gpu_result_double": (C.POINTER(C.c_double), [P, C.c_uint32]),
+    "ghive_gpu_result_int": (C.POINTER(C.c_int32), [P, C.c_uint32]),
+    "ghive_gpu_result_string": (C.c_int, [P, C.c_uint32, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64),
+                                          C.POINTER(C.POINTER(C.c_int32))]),
+    "ghive_gpu_result_free": (None, [P]),
+    "ghive_last_process_times": (C.c_int, [P, C.POINTER(abi.ProcessTimes)]),
 }
 
 

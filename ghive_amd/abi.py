@@ -113,6 +113,18 @@ class StageTimes(C.Structure):
                 ("finalize_ms", C.c_float), ("rows_in", C.c_uint64), ("rows_joined", C.c_uint64)]
 
 
+class PlanExecInfo(C.Structure):
+    _fields_ = [("fused", C.c_int32), ("joins", C.c_uint32), ("narrowed_columns", C.c_uint32),
+                ("has_filter", C.c_int32), ("fused_kernel_ms", C.c_float), ("reserved", C.c_uint32),
+                ("not_fused", C.c_char * 160)]
+
+
+class ProcessTimes(C.Structure):
+    _fields_ = [("stage_ms", C.c_double), ("execute_ms", C.c_double), ("result_ms", C.c_double),
+                ("total_ms", C.c_double), ("h2d_bytes", C.c_uint64), ("narrowed_columns", C.c_uint32),
+                ("fused", C.c_int32)]
+
+
 # Oracle-side fragment description (oracle/ghive_oracle.h), kept here so both sides share layouts.
 class OrcDim(C.Structure):
     _fields_ = [("table", Table), ("filter", C.POINTER(Pred)), ("filter_nnodes", C.c_uint32),

@@ -63,6 +63,7 @@ struct ghive_tuning {
   int64_t frag_pipe = 1;                // 1: pipelined star kernel where group keys fit direct slots
   int64_t gb_lds_budget = 64 * 1024;    // bytes of LDS for the group-by's private table
   int64_t gb_partition = 1;             // high-cardinality form: 0 never, 1 by size, 2 always
+  int64_t plan_fuse = 1;                // 1: plan vertices of the map-join shape run as one fragment
 };
 
 struct ghive_ctx {
@@ -75,6 +76,8 @@ struct ghive_ctx {
   int cus = 256;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   ghive_stage_times times{};
+  ghive_plan_exec_info plan_info{};        // what the last ghive_plan_execute did
+  ghive_process_times process_times{};     // the last ghive_gpu_process
   ghive_tuning tune{};
   // reusable small device scratch for counters (8 x uint64) and its host mirror
   uint64_t *d_counters = nullptr;
@@ -194,6 +197,13 @@ int fill_u64(ghive_ctx *ctx, uint64_t *p, uint64_t value, uint64_t n);
 int fill_u32(ghive_ctx *ctx, uint32_t *p, uint32_t value, uint64_t n);
 // p[i * step] = value for i < n.
 int fill_u64_strided(ghive_ctx *ctx, uint64_t *p, uint64_t value, uint64_t n, uint64_t step);
+// ghive_hash_build without a host wait: the statistics are read when the table is first used
+// (hash.hip).
+int hash_build_deferred(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, uint64_t n,
+                        const ghive_column *payload, float load_factor, ghive_hash_table **out);
+// 32-bit copy of a LONG column (NULL kept); *bad_dev (device, zeroed by the caller) becomes nonzero
+// when some value does not fit.
+int narrow_long(ghive_ctx *ctx, const int64_t *in, uint64_t n, int32_t *out, uint64_t *bad_dev);
 // Exclusive scan of n uint32 values into out (out may alias in); writes the total to
 // *d_total (device) when non-null.
 int exclusive_scan_u32(ghive_ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n,

@@ -158,6 +158,7 @@ int fragment_plan(ghive_ctx *ctx, const ghive_fragment *f, int late, FragPlan &P
   F.naggs = (int)f->naggs;
   T.naggs = (int)f->naggs;
   int word = 0;
+  bool any_dbl = false;
   for (uint32_t a = 0; a < f->naggs; ++a) {
     const ghive_fragment_agg &E = f->aggs[a];
     FragAggDev &D = F.aggs[a];
@@ -165,27 +166,53 @@ int fragment_plan(ghive_ctx *ctx, const ghive_fragment *f, int late, FragPlan &P
     D.expr = E.expr;
     if (E.expr != GHIVE_EXPR_COL && E.expr != GHIVE_EXPR_MUL && E.expr != GHIVE_EXPR_ONE)
       return set_error(ctx, GHIVE_ERR_INVALID, "fragment: aggregate %u expression %d", a, E.expr);
-    if (E.expr != GHIVE_EXPR_ONE) {
-      GH_TRY(int_col(E.a, "aggregate"));
+    // DOUBLE inputs (both, for a product) go to the general kernel, which reads them per surviving
+    // row; INT inputs stream with the tile
+    const bool dbl = E.expr != GHIVE_EXPR_ONE && E.a < fact.ncols && fact.columns[E.a].type == GHIVE_DOUBLE;
+    if (dbl) {
+      for (int i = 0; i < (E.expr == GHIVE_EXPR_MUL ? 2 : 1); ++i) {
+        const uint32_t c = i ? E.b : E.a;
+        if (c >= fact.ncols) return set_error(ctx, GHIVE_ERR_INVALID, "fragment: aggregate column %u of %u", c, fact.ncols);
+        if (fact.columns[c].type != GHIVE_DOUBLE)
+          return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "fragment: aggregate %u mixes DOUBLE and non-DOUBLE inputs", a);
+        if (((uintptr_t)fact.columns[c].data & 7) != 0)
+          return set_error(ctx, GHIVE_ERR_INVALID, "fragment: DOUBLE column %u is not 8-byte aligned", c);
+      }
+      D.dbl = 1;
       D.a = static_cast<const int32_t *>(fact.columns[E.a].data);
-    }
-    if (E.expr == GHIVE_EXPR_MUL) {
-      GH_TRY(int_col(E.b, "aggregate"));
-      D.b = static_cast<const int32_t *>(fact.columns[E.b].data);
+      D.b = static_cast<const int32_t *>(fact.columns[E.expr == GHIVE_EXPR_MUL ? E.b : E.a].data);
+      any_dbl = true;
+    } else {
+      if (E.expr != GHIVE_EXPR_ONE) {
+        GH_TRY(int_col(E.a, "aggregate"));
+        D.a = static_cast<const int32_t *>(fact.columns[E.a].data);
+      }
+      if (E.expr == GHIVE_EXPR_MUL) {
+        GH_TRY(int_col(E.b, "aggregate"));
+        D.b = static_cast<const int32_t *>(fact.columns[E.b].data);
+      }
     }
     switch (E.type) {
-      case GHIVE_AGG_SUM: A.kind = AK_SUMI; break;
+      case GHIVE_AGG_SUM: A.kind = dbl ? AK_SUMD : AK_SUMI; break;
       case GHIVE_AGG_CNT: A.kind = AK_CNT; break;
-      case GHIVE_AGG_MIN: A.kind = AK_MINI; break;
-      case GHIVE_AGG_MAX: A.kind = AK_MAXI; break;
+      case GHIVE_AGG_MIN: A.kind = dbl ? AK_MIND : AK_MINI; break;
+      case GHIVE_AGG_MAX: A.kind = dbl ? AK_MAXD : AK_MAXI; break;
       default:
         return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "fragment: aggregate %u type %d (SUM/COUNT/MIN/MAX)", a, E.type);
     }
-    A.in_type = GHIVE_LONG;
-    A.out_type = GHIVE_LONG;
+    A.in_type = dbl ? GHIVE_DOUBLE : GHIVE_LONG;
+    A.out_type = (dbl && A.kind != AK_CNT) ? GHIVE_DOUBLE : GHIVE_LONG;
     A.col = -1;
     A.word0 = (int16_t)word;
     word += agg_words(A.kind);
+    if (A.kind == AK_SUMD) {
+      // exact fixed-point sum: the LSB from a bound of |product| (the product of the columns'
+      // largest magnitudes; rounding is monotonic) and the row count
+      double ma = 0, mb = 1;
+      GH_TRY(column_absmax(ctx, fact.columns[E.a].data, GHIVE_DOUBLE, nullptr, F.n, &ma));
+      if (E.expr == GHIVE_EXPR_MUL) GH_TRY(column_absmax(ctx, fact.columns[E.b].data, GHIVE_DOUBLE, nullptr, F.n, &mb));
+      GH_TRY(sum_scale(ctx, ma * mb, F.n, &A.scale_e));
+    }
   }
   T.nwords = word;
   // unused aggregate inputs read a column the tile loads anyway, so that the dense-probe kernel
@@ -216,7 +243,7 @@ int fragment_plan(ghive_ctx *ctx, const ghive_fragment *f, int late, FragPlan &P
   const int rows = ctx->tune.frag_rows == 8 ? 8 : 4;
   // the lean dense-probe kernel when the fragment has its shape (no fact filter, early reads, every
   // join through a bitmap, group keys from join attributes)
-  bool dense_form = !ctx->tune.frag_generic && !F.has_filter && !late && F.njoins >= 1;
+  bool dense_form = !ctx->tune.frag_generic && !F.has_filter && !late && F.njoins >= 1 && !any_dbl;
   for (int j = 0; j < F.njoins; ++j) dense_form = dense_form && F.joins[j].mode != kJoinHash;
   for (int k = 0; k < F.nkeys; ++k) dense_form = dense_form && F.key_src[k] == GHIVE_SRC_JOIN;
   P.dense_form = dense_form;
@@ -341,7 +368,7 @@ int fragment_run(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_c
   memset(&O, 0, sizeof O);
   for (uint32_t c = 0; c < frag->nkeys + frag->naggs; ++c) {
     O.col[c] = out_cols ? out_cols[c].data : nullptr;
-    O.type[c] = c < frag->nkeys ? GHIVE_INT : GHIVE_LONG;
+    O.type[c] = c < frag->nkeys ? GHIVE_INT : P.T.aggs[c - frag->nkeys].out_type;
     if (out_cols) out_cols[c].type = O.type[c];
   }
   uint64_t joined = 0;

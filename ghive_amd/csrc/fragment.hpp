@@ -36,7 +36,7 @@ struct FragJoinDev {
 
 struct FragAggDev {
   int32_t expr;
-  int32_t pad;
+  int32_t dbl;  // 1: a and b are DOUBLE columns (general kernel: read per surviving row)
   const int32_t *a;
   const int32_t *b;
 };

@@ -54,6 +54,7 @@ __device__ __forceinline__ void frag_stage_tail(const FragDev &F, const FragBloc
   for (int j = 0; j < NJ; ++j) stage(F.joins[j].fk, j);
 #pragma unroll
   for (int a = 0; a < NA; ++a) {
+    if (F.aggs[a].dbl) continue;  // read per surviving row, from the column itself
     if (F.aggs[a].a) stage(F.aggs[a].a, NJ + 2 * a);
     if (F.aggs[a].b) stage(F.aggs[a].b, NJ + 2 * a + 1);
   }
@@ -101,7 +102,22 @@ __device__ __forceinline__ void frag_emit(const FragDev &F, const GroupTableDev 
       AggVal v;
       v.neg = false;
       v.b = 0;
-      if (expr == GHIVE_EXPR_ONE) {
+      if (F.aggs[a].dbl) {
+        // Hive's double arithmetic: the product rounded to a double, then the aggregate over it
+        const double x = reinterpret_cast<const double *>(F.aggs[a].a)[r0 + q];
+        const double y = expr == GHIVE_EXPR_MUL ? reinterpret_cast<const double *>(F.aggs[a].b)[r0 + q] : 1.0;
+        const double p = expr == GHIVE_EXPR_MUL ? x * y : x;
+        v.valid = !__builtin_isnan(x) && !__builtin_isnan(y);
+        const AggSpec &A = T.aggs[a];
+        if (A.kind == AK_SUMD) {
+          uint64_t lo, hi;
+          v.neg = to_fixed(p, A.scale_e, lo, hi);
+          v.a = lo;
+          v.b = hi;
+        } else {
+          v.a = dbl_order(p);
+        }
+      } else if (expr == GHIVE_EXPR_ONE) {
         v.valid = true;
         v.a = 1;
       } else {
@@ -144,6 +160,7 @@ __device__ __forceinline__ void frag_tile(const FragDev &F, const GroupTableDev 
     for (int j = 0; j < NJ; ++j) load_rows<R>(at(F.joins[j].fk, j), fk[j]);
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
+      if (F.aggs[a].dbl) continue;  // DOUBLE inputs: read for the surviving rows only
       if (F.aggs[a].expr != GHIVE_EXPR_ONE) load_rows<R>(at(F.aggs[a].a, NJ + 2 * a), av[a]);
       if (F.aggs[a].expr == GHIVE_EXPR_MUL) load_rows<R>(at(F.aggs[a].b, NJ + 2 * a + 1), bv[a]);
     }
@@ -206,6 +223,7 @@ __device__ __forceinline__ void frag_tile(const FragDev &F, const GroupTableDev 
   if (late) {
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
+      if (F.aggs[a].dbl) continue;  // DOUBLE inputs: read for the surviving rows only
       if (F.aggs[a].expr != GHIVE_EXPR_ONE) load_rows<R>(at(F.aggs[a].a, NJ + 2 * a), av[a]);
       if (F.aggs[a].expr == GHIVE_EXPR_MUL) load_rows<R>(at(F.aggs[a].b, NJ + 2 * a + 1), bv[a]);
     }

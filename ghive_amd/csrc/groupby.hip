@@ -394,6 +394,34 @@ __global__ void k_absmax(const void *data, int type, const uint32_t *sel, uint64
   if (lane_id() == 0 && m) atomicMax(out, (unsigned long long)m);
 }
 
+int column_absmax(ghive_ctx *ctx, const void *data, int type, const uint32_t *sel, uint64_t n, double *out) {
+  uint64_t m = 0;
+  if (n) {
+    GH_TRY(fill_u64(ctx, ctx->d_counters, 0, 1));
+    const uint64_t g = (n + 255) / 256, cap = (uint64_t)ctx->cus * 8;
+    k_absmax<<<(uint32_t)(g > cap ? cap : g), 256, 0, ctx->stream>>>(data, type, sel, n,
+                                                                     reinterpret_cast<unsigned long long *>(ctx->d_counters));
+    GH_LAUNCH(ctx);
+    GH_TRY(d2h(ctx, &m, ctx->d_counters, sizeof m));
+  }
+  memcpy(out, &m, sizeof *out);
+  return GHIVE_OK;
+}
+
+int sum_scale(ghive_ctx *ctx, double mx, uint64_t n, int32_t *scale_e) {
+  if (std::isinf(mx)) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "SUM over a double whose magnitude overflows");
+  if (!(mx > 0.0)) {
+    *scale_e = -1074;
+    return GHIVE_OK;
+  }
+  int lg_n = 0;
+  while ((1ull << lg_n) < n + 1 && lg_n < 63) ++lg_n;
+  // 2^(E + 126) > n * max |x|: the positive and the negative 128-bit halves never overflow
+  const int e = std::ilogb(mx) + 1 + lg_n - 126;
+  *scale_e = e < -1074 ? -1074 : e;
+  return GHIVE_OK;
+}
+
 __device__ __forceinline__ void write_typed(void *col, int type, uint64_t idx, int64_t iv, double dv, bool is_int) {
   switch (type) {
     case GHIVE_LONG: static_cast<int64_t *>(col)[idx] = is_int ? iv : (int64_t)dv; break;
@@ -864,26 +892,9 @@ int ghive_groupby(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel,
   for (uint32_t a = 0; a < naggs; ++a) {
     AggSpec &A = T.aggs[a];
     if (A.kind != AK_SUMD) continue;
-    uint64_t m = 0;
-    if (n) {
-      GH_TRY(fill_u64(ctx, ctx->d_counters, 0, 1));
-      uint64_t g = (n + 255) / 256, cap = (uint64_t)ctx->cus * 8;
-      k_absmax<<<(uint32_t)(g > cap ? cap : g), 256, 0, ctx->stream>>>(A.data, A.in_type, sel, n,
-                                                                       reinterpret_cast<unsigned long long *>(ctx->d_counters));
-      GH_LAUNCH(ctx);
-      GH_TRY(d2h(ctx, &m, ctx->d_counters, sizeof m));
-    }
-    double mx;
-    memcpy(&mx, &m, sizeof mx);
-    if (!(mx > 0.0) || !std::isfinite(mx)) {
-      if (std::isinf(mx)) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "groupby: SUM over an infinite double");
-      A.scale_e = -1074;
-    } else {
-      int lg_n = 0;
-      while ((1ull << lg_n) < n + 1 && lg_n < 63) ++lg_n;
-      A.scale_e = std::ilogb(mx) + 1 + lg_n - 126;
-      if (A.scale_e < -1074) A.scale_e = -1074;
-    }
+    double mx = 0;
+    GH_TRY(column_absmax(ctx, A.data, A.in_type, sel, n, &mx));
+    GH_TRY(sum_scale(ctx, mx, n, &A.scale_e));
   }
   DevBuf wdict, wcodes, wgcode;
   uint64_t *dict = nullptr;

@@ -54,6 +54,11 @@ int key_dictionary(ghive_ctx *ctx, const WideKeySpec &K, uint64_t rows, const ui
 // Writes every group to the output columns; CAPACITY error when there are more than capacity.
 // `after` (optional) is recorded on the stream after the write-out kernel; extra_src / extra_dst
 // (optional) is one more device word read back with the group count.
+// Largest |x| over rows sel[0..n) (sel NULL: 0..n) of a numeric column, NULLs skipped (0 if none).
+int column_absmax(ghive_ctx *ctx, const void *data, int type, const uint32_t *sel, uint64_t n, double *out);
+// LSB exponent of an exact fixed-point double sum of n values of magnitude <= mx;
+// GHIVE_ERR_UNSUPPORTED when mx is infinite.
+int sum_scale(ghive_ctx *ctx, double mx, uint64_t n, int32_t *scale_e);
 int group_table_emit(ghive_ctx *ctx, const KeySpec &K, const GroupTableDev &T, const OutSpec &O,
                      uint64_t capacity, uint64_t *out_groups, hipEvent_t after = nullptr,
                      const uint64_t *extra_src = nullptr, uint64_t *extra_dst = nullptr);

@@ -495,6 +495,11 @@ int hash_build(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, uin
   return hash_build_core(ctx, key, sel, n, nullptr, payload, load_factor, false, out);
 }
 
+int hash_build_deferred(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, uint64_t n,
+                        const ghive_column *payload, float load_factor, ghive_hash_table **out) {
+  return hash_build_core(ctx, key, sel, n, nullptr, payload, load_factor, true, out);
+}
+
 // A table whose deferred finish failed keeps the status and message; consumers get them back.
 static int fail_table(ghive_ctx *owner, ghive_hash_table *h, int st) {
   h->failed = st;

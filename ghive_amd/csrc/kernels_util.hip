@@ -52,6 +52,29 @@ int fill_u32(ghive_ctx *ctx, uint32_t *p, uint32_t value, uint64_t n) {
   return GHIVE_OK;
 }
 
+// LONG -> 32-bit copy for kernels that stream int32 columns: out[i] = in[i], LONG NULL -> INT NULL.
+// *bad becomes nonzero when a non-NULL value lies outside [INT_MIN, INT_MAX - 1] (INT_MAX is the
+// INT NULL sentinel), i.e. when the copy would not hold the same values.
+__global__ void k_narrow_i64(const int64_t *in, uint64_t n, int32_t *out, unsigned long long *bad) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  bool b = false;
+  for (; i < n; i += stride) {
+    const int64_t v = in[i];
+    const bool null = v == 0x7fffffffffffffffll;
+    b |= !null && (v < -2147483648ll || v > 2147483646ll);
+    out[i] = null ? 0x7fffffff : (int32_t)v;
+  }
+  if (__ballot(b) && lane_id() == 0) atomicOr(bad, 1ull);
+}
+
+int narrow_long(ghive_ctx *ctx, const int64_t *in, uint64_t n, int32_t *out, uint64_t *bad_dev) {
+  if (!n) return GHIVE_OK;
+  k_narrow_i64<<<fill_grid(ctx, n), 256, 0, ctx->stream>>>(in, n, out, reinterpret_cast<unsigned long long *>(bad_dev));
+  GH_LAUNCH(ctx);
+  return GHIVE_OK;
+}
+
 constexpr int kScanThreads = 256;
 constexpr int kScanPerThread = 16;
 constexpr uint64_t kScanChunk = (uint64_t)kScanThreads * kScanPerThread;

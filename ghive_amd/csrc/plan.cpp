@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <regex>
 #include <set>
@@ -25,6 +26,8 @@
 #include <vector>
 
 #include "../../include/ghive/ghive.h"
+#include "common.hpp"
+#include "plan_internal.hpp"
 
 namespace {
 
@@ -631,6 +634,9 @@ std::string jints2(const std::vector<std::vector<int>> &v) {
   return r + "]";
 }
 
+// Whether GroupBy `node` runs as one fused fragment (the plan alone decides the shape).
+bool fusable(const ghive_plan &P, int node, std::string &why);
+
 std::string describe(const ghive_plan &P) {
   std::string r = "{\"root\":" + jstr(P.root >= 0 ? P.ops[P.root].id : "") + ",\"vertices\":{";
   bool first = true;
@@ -653,8 +659,12 @@ std::string describe(const ghive_plan &P) {
       r += ",\"conds\":" + jstr(o.conds) + ",\"join_inputs\":" + jstrs(o.join_inputs) +
            ",\"join_cols\":" + jints2(o.join_cols) + ",\"join_types\":" + jints2(o.join_types) +
            ",\"maintain_cols\":" + jints2(o.maintain_cols);
-    if (o.kind == "GroupBy")
-      r += ",\"keys\":" + jints(o.key_offsets) + ",\"aggs\":" + jints2(o.agg_descs) + ",\"mode\":" + jstr(o.gby_mode);
+    if (o.kind == "GroupBy") {
+      std::string why;
+      const bool f = fusable(P, (int)i, why);
+      r += ",\"keys\":" + jints(o.key_offsets) + ",\"aggs\":" + jints2(o.agg_descs) + ",\"mode\":" + jstr(o.gby_mode) +
+           ",\"fused_form\":" + (f ? "true" : "false") + ",\"not_fused\":" + jstr(f ? "" : why);
+    }
     if (o.kind == "ReduceSink" || o.kind == "FileSink")
       r += ",\"key_expressions\":" + jstrs(o.key_expressions) + ",\"key_offsets\":" + jints(o.key_offsets) +
            ",\"sort_orders\":" + jstr(o.sort_orders) + ",\"partition_cols\":" + jstrs(o.partition_cols) +
@@ -685,9 +695,256 @@ namespace {
 // materialises its maintained columns (ghive_join_result), GroupBy runs ghive_groupby, the sink
 // orders its columns keys first and, as the plan's root, performs the final ORDER BY (ghive_sort
 // + ghive_gather) and LIMIT.
+// The columns each input of a join keeps, in output order (indexes into in_names[t]): from the
+// Java side's maintain-values map, else by Hive's naming -- a join's outputs are numbered _colN by
+// position in the concatenation of its inputs' columns, so input 0's columns keep their names and
+// input 1's come renumbered after them. Input 0 maps by name; the rest map to input 1 by the offset
+// that puts the last of them on input 1's last non-key column (exact when that column is kept, as
+// in every SSB plan).
+int join_keep(const Op &o, const std::vector<std::vector<std::string>> &in_names,
+              std::vector<std::vector<uint32_t>> &keep, std::string &err) {
+  const size_t nt = in_names.size();
+  keep.assign(nt, {});
+  bool from_map = false;
+  for (size_t t = 0; t < nt && t < o.maintain_cols.size(); ++t)
+    for (int c : o.maintain_cols[t]) {
+      keep[t].push_back((uint32_t)c);
+      from_map = true;
+    }
+  if (from_map) return GHIVE_OK;
+  static const std::regex col_re(R"(^_col(\d+)$)");
+  std::vector<std::pair<long, std::string>> rest;
+  for (const std::string &n : o.outputs) {
+    const int c = index_of(in_names[0], n);
+    if (c >= 0) {
+      keep[0].push_back((uint32_t)c);
+      continue;
+    }
+    std::smatch m;
+    if (nt != 2 || !std::regex_match(n, m, col_re)) {
+      err = "plan: " + o.id + ": output '" + n + "' is not a column of input 0; pass the maintain-values map";
+      return GHIVE_ERR_UNSUPPORTED;
+    }
+    rest.push_back({std::stol(m[1].str()), n});
+  }
+  if (rest.empty()) return GHIVE_OK;
+  long jmax = -1;
+  for (size_t c = 0; c < in_names[1].size(); ++c) {
+    std::smatch m;
+    if (o.join_cols.size() > 1 && !o.join_cols[1].empty() && (int)c != o.join_cols[1][0] &&
+        std::regex_match(in_names[1][c], m, col_re))
+      jmax = std::max(jmax, std::stol(m[1].str()));
+  }
+  const long offset = rest.back().first - jmax;
+  for (const auto &r : rest) {
+    const int c = jmax < 0 ? -1 : index_of(in_names[1], "_col" + std::to_string(r.first - offset));
+    if (c < 0) {
+      err = "plan: " + o.id + ": cannot place output '" + r.second + "' in input 1; pass the maintain-values map";
+      return GHIVE_ERR_UNSUPPORTED;
+    }
+    keep[1].push_back((uint32_t)c);
+  }
+  return GHIVE_OK;
+}
+
+// ---- the fused form of a map vertex ------------------------------------------------------------
+// A map vertex of the CS1 shape (TableScan -> [Filter] -> [Select] -> MapJoin+ -> [Select] ->
+// GroupBy -> sink) needs no intermediate table: every column the group-by reads is a fact column,
+// a product of two, or an attribute of a joined dimension, so the whole chain is one
+// ghive_fragment_execute. The recognizer follows the column names through the chain symbolically.
+struct Sym {
+  enum Kind { kNone, kFact, kDim, kMul } kind = kNone;
+  int col = -1;        // kFact: fact column (position in the scan's outputs); kDim: dimension column
+  int join = -1;       // kDim: the join (0 = nearest the scan)
+  int a = -1, b = -1;  // kMul: fact columns
+};
+
+struct FuseJoin {
+  const Op *op = nullptr;
+  const Op *dim = nullptr;  // the broadcast sink of the dimension's vertex
+  int fact_col = -1;        // probed fact column
+  int dim_key = -1;         // key column of the dimension table
+  int dim_pay = -1;         // the attribute the join carries (a group key), -1: none
+};
+
+struct FuseAgg {
+  int type = 0;  // GHIVE_AGG_*
+  Sym arg;       // kNone: count()
+};
+
+struct FuseShape {
+  const Op *scan = nullptr;
+  const Op *filter = nullptr;
+  std::vector<std::string> filter_names;  // the filter's input columns
+  std::vector<Sym> filter_syms;
+  std::vector<FuseJoin> joins;
+  std::vector<Sym> keys;
+  std::vector<FuseAgg> aggs;
+};
+
+// "(x * y)" -> x, y
+bool split_product(const std::string &e0, std::string &x, std::string &y) {
+  const std::string e = trim(e0);
+  if (e.size() < 7 || e.front() != '(' || e.back() != ')') return false;
+  const std::string in = e.substr(1, e.size() - 2);
+  const size_t p = in.find(" * ");
+  if (p == std::string::npos || in.find(" * ", p + 1) != std::string::npos || in.find('(') != std::string::npos)
+    return false;
+  x = trim(in.substr(0, p));
+  y = trim(in.substr(p + 3));
+  return !x.empty() && !y.empty();
+}
+
+// Whether the GroupBy `gby` of `vertex` has the fused shape; on success S describes it, else
+// `why` says what is missing. Needs the plan only (no data).
+bool fuse_shape(const ghive_plan &P, const std::string &vertex, int gby, FuseShape &S, std::string &why) {
+  const Op &g = P.ops[gby];
+  if (g.kind != "GroupBy" || g.vertex != vertex) return why = "not a group-by of this vertex", false;
+  if (!g.gby_mode.empty() && g.gby_mode != "hash") return why = "group-by mode " + g.gby_mode, false;
+  if (g.children.empty()) return why = "group-by without input", false;
+  std::vector<int> chain;  // from the group-by's input down to the scan
+  for (int node = g.children[0]; node >= 0;) {
+    const Op &o = P.ops[node];
+    if (o.vertex != vertex) return why = "the group-by reads another vertex's output", false;
+    chain.push_back(node);
+    if (o.kind == "TableScan") break;
+    if (o.kind == "MapJoin") {
+      if (o.join_inputs.empty()) return why = o.id + " has no inputs", false;
+      const auto it = P.by_id.find(o.join_inputs[0]);
+      node = it == P.by_id.end() ? -1 : it->second;
+    } else if (o.kind == "Select" || o.kind == "Filter") {
+      node = o.children.empty() ? -1 : o.children[0];
+    } else {
+      return why = o.kind + " " + o.id + " in the chain", false;
+    }
+    if (node < 0) return why = "broken operator chain at " + o.id, false;
+  }
+  if (chain.empty() || P.ops[chain.back()].kind != "TableScan") return why = "no table scan below the group-by", false;
+  std::vector<std::string> names;
+  std::vector<Sym> syms;
+  bool joined = false;
+  for (auto it = chain.rbegin(); it != chain.rend(); ++it) {
+    const Op &o = P.ops[*it];
+    if (o.kind == "TableScan") {
+      S.scan = &o;
+      names = o.outputs;
+      syms.assign(names.size(), Sym());
+      for (size_t c = 0; c < names.size(); ++c) {
+        syms[c].kind = Sym::kFact;
+        syms[c].col = (int)c;
+      }
+    } else if (o.kind == "Filter") {
+      if (joined || S.filter) return why = "a filter above a join", false;
+      S.filter = &o;
+      S.filter_names = names;
+      S.filter_syms = syms;
+    } else if (o.kind == "Select") {
+      const std::vector<std::string> &ex = o.expressions.empty() ? o.outputs : o.expressions;
+      if (ex.size() != o.outputs.size()) return why = o.id + ": expressions and outputs differ", false;
+      std::vector<Sym> ns(ex.size());
+      for (size_t i = 0; i < ex.size(); ++i) {
+        const int k = index_of(names, trim(ex[i]));
+        std::string x, y;
+        if (k >= 0) {
+          ns[i] = syms[k];
+        } else if (split_product(ex[i], x, y)) {
+          const int kx = index_of(names, x), ky = index_of(names, y);
+          if (kx >= 0 && ky >= 0 && syms[kx].kind == Sym::kFact && syms[ky].kind == Sym::kFact) {
+            ns[i].kind = Sym::kMul;
+            ns[i].a = syms[kx].col;
+            ns[i].b = syms[ky].col;
+          }
+        }
+      }
+      names = o.outputs;
+      syms = ns;
+    } else {  // MapJoin
+      if (o.join_inputs.size() != 2) return why = o.id + ": not a two-input join", false;
+      if (o.join_types.size() != 1 || o.join_types[0][0] != GHIVE_INNER_JOIN || o.join_types[0][1] != 0 ||
+          o.join_types[0][2] != 1)
+        return why = o.id + ": not an inner join of input 0 to input 1", false;
+      const auto dit = P.by_id.find(o.join_inputs[1]);
+      if (dit == P.by_id.end()) return why = o.id + ": unknown input", false;
+      const Op &dim = P.ops[dit->second];
+      if (dim.kind != "ReduceSink" || dim.vertex == vertex)
+        return why = o.id + ": input 1 is not another vertex's broadcast", false;
+      if (o.join_cols.size() != 2 || o.join_cols[0].size() != 1 || o.join_cols[1].size() != 1)
+        return why = o.id + ": not a single-column key", false;
+      const int pk = o.join_cols[0][0];
+      if (pk < 0 || pk >= (int)syms.size() || syms[pk].kind != Sym::kFact)
+        return why = o.id + ": the probe key is not a fact column", false;
+      FuseJoin J;
+      J.op = &o;
+      J.dim = &dim;
+      J.fact_col = syms[pk].col;
+      J.dim_key = o.join_cols[1][0];
+      std::vector<std::vector<uint32_t>> keep;
+      std::string e;
+      if (join_keep(o, {names, dim.outputs}, keep, e) != GHIVE_OK) return why = e, false;
+      if (keep[0].size() + keep[1].size() != o.outputs.size()) return why = o.id + ": outputs not placed", false;
+      std::vector<Sym> ns;
+      for (uint32_t c : keep[0]) ns.push_back(c < syms.size() ? syms[c] : Sym());
+      for (uint32_t c : keep[1]) {
+        Sym d;
+        d.kind = Sym::kDim;
+        d.join = (int)S.joins.size();
+        d.col = (int)c;
+        ns.push_back(d);
+      }
+      S.joins.push_back(J);
+      names = o.outputs;
+      syms = ns;
+      joined = true;
+    }
+  }
+  if (S.joins.empty()) return why = "no map join below the group-by", false;
+  if (g.key_offsets.size() != g.key_strs.size()) return why = "unresolved group-by keys", false;
+  if (g.key_offsets.size() > 2) return why = "more than 2 group-by keys", false;
+  for (int k : g.key_offsets) {
+    if (k < 0 || k >= (int)syms.size()) return why = "group-by key beyond its input", false;
+    const Sym &s = syms[k];
+    if (s.kind == Sym::kDim) {
+      int &pay = S.joins[s.join].dim_pay;
+      if (pay >= 0 && pay != s.col) return why = "two group-by keys from one dimension", false;
+      pay = s.col;
+    } else if (s.kind != Sym::kFact) {
+      return why = "a group-by key that is neither a fact column nor a dimension attribute", false;
+    }
+    S.keys.push_back(s);
+  }
+  static const std::regex agg_re(R"(^(\w+)\((.*)\)$)");
+  if (g.agg_descs.size() != g.agg_strs.size()) return why = "unresolved aggregates", false;
+  for (size_t a = 0; a < g.agg_strs.size(); ++a) {
+    std::smatch m;
+    if (!std::regex_match(g.agg_strs[a], m, agg_re)) return why = "aggregate " + g.agg_strs[a], false;
+    FuseAgg A;
+    A.type = g.agg_descs[a][0];
+    if (A.type != GHIVE_AGG_SUM && A.type != GHIVE_AGG_CNT && A.type != GHIVE_AGG_MIN && A.type != GHIVE_AGG_MAX)
+      return why = "aggregate " + g.agg_strs[a], false;
+    if (!trim(m[2].str()).empty()) {
+      const int c = g.agg_descs[a][1];
+      if (c < 0 || c >= (int)syms.size() || (syms[c].kind != Sym::kFact && syms[c].kind != Sym::kMul))
+        return why = "aggregate over " + g.agg_strs[a] + ": not a fact column or a product of two", false;
+      A.arg = syms[c];
+    } else if (A.type != GHIVE_AGG_CNT) {
+      return why = "aggregate " + g.agg_strs[a], false;
+    }
+    S.aggs.push_back(A);
+  }
+  if (S.aggs.empty()) return why = "a group-by without aggregates", false;
+  if (S.keys.size() + S.aggs.size() != g.outputs.size()) return why = "keys and aggregates do not match the outputs", false;
+  return true;
+}
+
+bool fusable(const ghive_plan &P, int node, std::string &why) {
+  FuseShape S;
+  return fuse_shape(P, P.ops[node].vertex, node, S, why);
+}
+
 struct ExecTable {
   std::vector<std::string> names;
   std::vector<ghive_column> cols;
+  std::vector<int> logical;  // from_input: the column's type in the plan (LONG for a 32-bit staged copy)
   uint64_t rows = 0;
   int reduce_keys = -1;  // a reduce input: the number of key columns leading it
   ghive_table view() const { return ghive_table{cols.data(), (uint32_t)cols.size(), 0, rows}; }
@@ -701,6 +958,8 @@ struct Executor {
   std::map<std::string, const ghive_plan_input *> inputs;
   std::vector<void *> owned;  // device buffers made by this execution
   std::string err;
+  // input name -> per column: 1 when the staged column holds LONG values as 32-bit integers
+  const std::map<std::string, std::vector<char>> *narrowed = nullptr;
 
   int fail(int code, const std::string &m) {
     err = m;
@@ -728,11 +987,18 @@ struct Executor {
   // Binds input `name` (else `alt`, when given). Without column names an input's columns are
   // positional: `positional` names them when it has one name per column (a TableScan's outputs, a
   // reduce sink's keys-first outputs -- the order GTables delivers them in), else _col0.._colN-1.
+  // A LONG column staged as 32-bit integers reaches the fused kernel as it is (keep_narrow) and every
+  // other operator as LONG again.
   int from_input(const std::string &name, const std::string &alt, const std::vector<std::string> &want,
-                 const std::vector<std::string> &positional, ExecTable &t) {
+                 const std::vector<std::string> &positional, ExecTable &t, bool keep_narrow = false) {
     auto it = inputs.find(name);
     if (it == inputs.end() && !alt.empty()) it = inputs.find(alt);
     if (it == inputs.end()) return fail(GHIVE_ERR_INVALID, "plan: no input named '" + name + "'");
+    const std::vector<char> *nflags = nullptr;
+    if (narrowed) {
+      const auto nit = narrowed->find(it->first);
+      if (nit != narrowed->end()) nflags = &nit->second;
+    }
     const ghive_plan_input &in = *it->second;
     if (!in.table) return fail(GHIVE_ERR_INVALID, "plan: input '" + name + "' has no table");
     std::vector<std::string> names;
@@ -747,7 +1013,18 @@ struct Executor {
       const int c = index_of(names, n);
       if (c < 0) return fail(GHIVE_ERR_INVALID, "plan: input '" + name + "' has no column '" + n + "'");
       t.names.push_back(n);
-      t.cols.push_back(in.table->columns[c]);
+      ghive_column col = in.table->columns[c];
+      const bool narrow = nflags && c < (int)nflags->size() && (*nflags)[c] && col.type == GHIVE_INT;
+      t.logical.push_back(narrow ? GHIVE_LONG : col.type);
+      if (narrow && !keep_narrow) {
+        ghive_column wide;
+        int st = alloc_col(GHIVE_LONG, t.rows, wide);
+        if (st != GHIVE_OK) return st;
+        if ((st = ghive_select_cast(ctx, &col, GHIVE_LONG, t.rows, &wide)) != GHIVE_OK) return op_fail(st);
+        wide.row_num = t.rows;
+        col = wide;
+      }
+      t.cols.push_back(col);
     }
     return GHIVE_OK;
   }
@@ -899,8 +1176,291 @@ struct Executor {
     return std::regex_match(x, num_re) ? operand(x, t, c) : expression(x, t, c);
   }
 
+  // Releases the buffers this execution made after position `mark` (a fused attempt given up).
+  void release_from(size_t mark) {
+    for (size_t i = mark; i < owned.size(); ++i) ghive_free(ctx, owned[i]);
+    owned.resize(mark);
+  }
+
+  // GroupBy `node` and the chain below it (shape S) as one ghive_fragment_execute. `done` stays
+  // false, with the reason in `why`, when the data rule the fused form out (column types, values
+  // beyond 32 bits, dimension keys that are not unique); the caller then runs the operator chain.
+  int fused(int node, const FuseShape &S, ExecTable &out, bool &done, std::string &why) {
+    done = false;
+    const Op &g = P.ops[node];
+    const size_t mark = owned.size();
+    std::vector<ghive_hash_table *> tables;
+    auto give_up = [&](const std::string &w) {
+      for (ghive_hash_table *h : tables) ghive_hash_table_free(ctx, h);
+      release_from(mark);
+      why = w;
+      return GHIVE_OK;
+    };
+    ExecTable fact;
+    int st = from_input(S.scan->table, vertex, S.scan->outputs, S.scan->outputs, fact, true);
+    if (st != GHIVE_OK) return st;
+    std::vector<ExecTable> dims(S.joins.size());
+    for (size_t j = 0; j < S.joins.size(); ++j)
+      if ((st = from_input(S.joins[j].dim->vertex, "", {}, S.joins[j].dim->outputs, dims[j], true)) != GHIVE_OK)
+        return st;
+    // 32-bit copies of the LONG columns the kernel streams (checked: every value must fit)
+    struct Narrow {
+      const int64_t *src;
+      uint64_t n;
+      int32_t *dst;
+    };
+    std::vector<Narrow> narrows;
+    std::string bad;
+    auto as_i32 = [&](const ghive_column &c, uint64_t rows, ghive_column &v) -> int {
+      if (c.type == GHIVE_INT && ((uintptr_t)c.data & 15) == 0) {
+        v = c;
+        return GHIVE_OK;
+      }
+      if (c.type != GHIVE_INT && c.type != GHIVE_LONG) {
+        bad = "a joined or aggregated column is neither INT nor LONG";
+        return GHIVE_OK;
+      }
+      ghive_column n;
+      int s = alloc_col(GHIVE_INT, rows, n);
+      if (s != GHIVE_OK) return s;
+      if (c.type == GHIVE_INT) {  // an unaligned INT column: a copy the kernel can stream
+        if (rows && (s = ghive_memcpy(ctx, n.data, c.data, 4 * rows, 2)) != GHIVE_OK) return op_fail(s);
+      } else {
+        narrows.push_back({static_cast<const int64_t *>(c.data), rows, static_cast<int32_t *>(n.data)});
+      }
+      v = n;
+      return GHIVE_OK;
+    };
+    std::vector<ghive_column> fcols = fact.cols;  // the scan's columns, then the 32-bit copies
+    std::map<int, uint32_t> i32_of;               // scan column -> its INT column in fcols
+    auto fact_i32 = [&](int c, uint32_t &idx) -> int {
+      auto it = i32_of.find(c);
+      if (it != i32_of.end()) {
+        idx = it->second;
+        return GHIVE_OK;
+      }
+      ghive_column v;
+      const int s = as_i32(fact.cols[c], fact.rows, v);
+      if (s != GHIVE_OK || !bad.empty()) return s;
+      if (v.data == fact.cols[c].data) {
+        idx = (uint32_t)c;
+      } else {
+        idx = (uint32_t)fcols.size();
+        fcols.push_back(v);
+      }
+      i32_of[c] = idx;
+      return GHIVE_OK;
+    };
+    std::vector<ghive_fragment_join> fj(S.joins.size());
+    std::vector<ghive_column> dkey(S.joins.size()), dpay(S.joins.size());
+    for (size_t j = 0; j < S.joins.size() && bad.empty(); ++j) {
+      const FuseJoin &J = S.joins[j];
+      if (J.dim_key >= (int)dims[j].cols.size() || J.dim_pay >= (int)dims[j].cols.size())
+        return give_up("a dimension input has too few columns");
+      if ((st = fact_i32(J.fact_col, fj[j].fact_key_col)) != GHIVE_OK) return st;
+      if ((st = as_i32(dims[j].cols[J.dim_key], dims[j].rows, dkey[j])) != GHIVE_OK) return st;
+      if (J.dim_pay >= 0 && (st = as_i32(dims[j].cols[J.dim_pay], dims[j].rows, dpay[j])) != GHIVE_OK) return st;
+    }
+    // aggregates: (type, expression, inputs) and the output type the operator chain would give
+    std::vector<ghive_fragment_agg> fa(S.aggs.size());
+    std::vector<int> agg_type(S.aggs.size());
+    for (size_t a = 0; a < S.aggs.size() && bad.empty(); ++a) {
+      const FuseAgg &A = S.aggs[a];
+      ghive_fragment_agg &E = fa[a];
+      E.type = A.type;
+      if (A.arg.kind == Sym::kNone) {
+        E.expr = GHIVE_EXPR_ONE;
+        agg_type[a] = GHIVE_LONG;
+        continue;
+      }
+      const int ta = fact.logical[A.arg.kind == Sym::kMul ? A.arg.a : A.arg.col];
+      const int tb = A.arg.kind == Sym::kMul ? fact.logical[A.arg.b] : ta;
+      E.expr = A.arg.kind == Sym::kMul ? GHIVE_EXPR_MUL : GHIVE_EXPR_COL;
+      if (ta == GHIVE_DOUBLE && tb == GHIVE_DOUBLE) {
+        E.a = (uint32_t)(A.arg.kind == Sym::kMul ? A.arg.a : A.arg.col);
+        E.b = (uint32_t)(A.arg.kind == Sym::kMul ? A.arg.b : A.arg.col);
+        agg_type[a] = A.type == GHIVE_AGG_CNT ? GHIVE_LONG : GHIVE_DOUBLE;
+        continue;
+      }
+      if (!ghive::is_int_type(ta) || !ghive::is_int_type(tb)) return give_up("an aggregate over a column that is neither INT, LONG nor DOUBLE");
+      // a product of two INT columns wraps at 32 bits in Hive; the fused kernel multiplies in 64
+      if (A.arg.kind == Sym::kMul && ta == GHIVE_INT && tb == GHIVE_INT) return give_up("INT * INT product");
+      if (A.arg.kind == Sym::kMul) {
+        if ((st = fact_i32(A.arg.a, E.a)) != GHIVE_OK || (st = fact_i32(A.arg.b, E.b)) != GHIVE_OK) return st;
+      } else if ((st = fact_i32(A.arg.col, E.a)) != GHIVE_OK) {
+        return st;
+      }
+      agg_type[a] = (A.type == GHIVE_AGG_MIN || A.type == GHIVE_AGG_MAX) && A.arg.kind == Sym::kFact ? ta : GHIVE_LONG;
+    }
+    std::vector<ghive_fragment_key> fk(S.keys.size());
+    std::vector<int> key_type(S.keys.size());
+    for (size_t k = 0; k < S.keys.size() && bad.empty(); ++k) {
+      const Sym &s = S.keys[k];
+      if (s.kind == Sym::kDim) {
+        fk[k] = ghive_fragment_key{GHIVE_SRC_JOIN, (uint32_t)s.join};
+        key_type[k] = dims[s.join].logical[s.col];
+      } else {
+        fk[k].source = GHIVE_SRC_FACT;
+        if ((st = fact_i32(s.col, fk[k].index)) != GHIVE_OK) return st;
+        key_type[k] = fact.logical[s.col];
+      }
+    }
+    if (!bad.empty()) return give_up(bad);
+    if (!narrows.empty()) {  // all range checks read back with one wait
+      void *flags = nullptr;
+      if ((st = ghive_malloc(ctx, 8 * narrows.size(), &flags)) != GHIVE_OK) return op_fail(st);
+      owned.push_back(flags);
+      uint64_t *fl = static_cast<uint64_t *>(flags);
+      if ((st = ghive::fill_u64(ctx, fl, 0, narrows.size())) != GHIVE_OK) return op_fail(st);
+      for (size_t i = 0; i < narrows.size(); ++i)
+        if ((st = ghive::narrow_long(ctx, narrows[i].src, narrows[i].n, narrows[i].dst, fl + i)) != GHIVE_OK)
+          return op_fail(st);
+      std::vector<uint64_t> host(narrows.size());
+      if ((st = ghive_memcpy(ctx, host.data(), flags, 8 * narrows.size(), 1)) != GHIVE_OK) return op_fail(st);
+      for (uint64_t f : host)
+        if (f) return give_up("a LONG column holds values beyond 32 bits");
+    }
+    for (size_t j = 0; j < S.joins.size(); ++j) {
+      ghive_hash_table *h = nullptr;
+      st = ghive::hash_build_deferred(ctx, &dkey[j], nullptr, dims[j].rows, S.joins[j].dim_pay >= 0 ? &dpay[j] : nullptr,
+                                      0.5f, &h);
+      if (st != GHIVE_OK) {
+        if (st == GHIVE_ERR_UNSUPPORTED) return give_up(ghive_last_error(ctx));
+        for (ghive_hash_table *t : tables) ghive_hash_table_free(ctx, t);
+        return op_fail(st);
+      }
+      tables.push_back(h);
+      fj[j].table = h;
+    }
+    // the fact predicate, on the scan's columns; IS NOT NULL of a probed key is implied by the join
+    std::vector<ghive_pred> pred;
+    uint32_t proot = 0;
+    if (S.filter && !S.filter->predicate.empty()) {
+      std::vector<const char *> cn;
+      for (auto &n : S.filter_names) cn.push_back(n.c_str());
+      std::vector<ghive_pred> nodes(256);
+      uint32_t nn = 0, root = 0;
+      char perr[256] = {0};
+      st = ghive_parse_predicate(S.filter->predicate.c_str(), cn.data(), (uint32_t)cn.size(), nodes.data(),
+                                 (uint32_t)nodes.size(), &nn, &root, perr, sizeof perr);
+      if (st != GHIVE_OK) {
+        for (ghive_hash_table *t : tables) ghive_hash_table_free(ctx, t);
+        return fail(st, "plan: " + S.filter->id + ": " + perr);
+      }
+      std::set<uint32_t> probed;
+      for (const FuseJoin &J : S.joins) probed.insert((uint32_t)J.fact_col);
+      for (uint32_t i = 0; i < nn; ++i)
+        for (int c = 0; c < nodes[i].pred_col_num && c < 2; ++c) {
+          const uint32_t x = nodes[i].filter_col[c];
+          if (x >= S.filter_syms.size() || S.filter_syms[x].kind != Sym::kFact)
+            return give_up("the filter reads a computed column");
+          nodes[i].filter_col[c] = (uint32_t)S.filter_syms[x].col;
+        }
+      std::vector<int> seen(nn, 0);
+      std::function<int(uint32_t)> keep = [&](uint32_t i) -> int {  // new index; -1: always true
+        if (i >= nn || seen[i]++ > 0) return -2;                       // malformed or shared node
+        ghive_pred p = nodes[i];
+        if (p.pred_col_num == 0 && (p.mode == GHIVE_FILTER_AND || p.mode == GHIVE_FILTER_OR)) {
+          const int a = keep((uint32_t)p.children[0]), b = keep((uint32_t)p.children[1]);
+          if (a == -2 || b == -2) return -2;
+          if (p.mode == GHIVE_FILTER_AND) {
+            if (a < 0) return b;
+            if (b < 0) return a;
+          } else if (a < 0 || b < 0) {
+            return -1;
+          }
+          p.children[0] = a;
+          p.children[1] = b;
+        } else if (p.mode == GHIVE_FILTER_NOT_NULL && p.pred_col_num == 1 && probed.count(p.filter_col[0])) {
+          return -1;
+        }
+        pred.push_back(p);
+        return (int)pred.size() - 1;
+      };
+      const int r = keep(root);
+      if (r == -2) return give_up("an unexpected predicate tree");
+      if (r >= 0) proot = (uint32_t)r;
+      else pred.clear();
+    }
+    ghive_fragment f;
+    memset(&f, 0, sizeof f);
+    f.fact = ghive_table{fcols.data(), (uint32_t)fcols.size(), 0, fact.rows};
+    f.filter = pred.empty() ? nullptr : pred.data();
+    f.filter_nnodes = (uint32_t)pred.size();
+    f.filter_root = proot;
+    f.joins = fj.data();
+    f.njoins = (uint32_t)fj.size();
+    f.keys = fk.data();
+    f.nkeys = (uint32_t)fk.size();
+    f.aggs = fa.data();
+    f.naggs = (uint32_t)fa.size();
+    const size_t nout = fk.size() + fa.size();
+    uint64_t cap = std::max<uint64_t>(1, std::min<uint64_t>(fact.rows, 1u << 16)), groups = 0;
+    std::vector<ghive_column> oc(nout);
+    for (int attempt = 0;; ++attempt) {
+      for (size_t c = 0; c < nout; ++c) {
+        if ((st = alloc_col(c < fk.size() ? GHIVE_INT : GHIVE_LONG, cap, oc[c])) != GHIVE_OK) {
+          for (ghive_hash_table *t : tables) ghive_hash_table_free(ctx, t);
+          return st;
+        }
+      }
+      st = ghive_fragment_execute(ctx, &f, oc.data(), cap, &groups);
+      if (st == GHIVE_ERR_CAPACITY && attempt == 0 && groups > cap) {
+        cap = groups;
+        continue;
+      }
+      break;
+    }
+    ghive_stage_times times{};
+    (void)ghive_last_stage_times(ctx, &times);
+    for (ghive_hash_table *t : tables) ghive_hash_table_free(ctx, t);
+    tables.clear();
+    if (st == GHIVE_ERR_UNSUPPORTED) return give_up(ghive_last_error(ctx));
+    if (st != GHIVE_OK) return op_error(st, g);
+    // the operator chain's output types: LONG group keys, INT MIN/MAX of an INT column
+    out.cols.clear();
+    for (size_t c = 0; c < nout; ++c) {
+      ghive_column x = oc[c];
+      x.row_num = groups;
+      const int want = c < fk.size() ? key_type[c] : agg_type[c - fk.size()];
+      if (want != x.type) {
+        ghive_column y;
+        if ((st = alloc_col(want, groups, y)) != GHIVE_OK) return st;
+        if ((st = ghive_select_cast(ctx, &x, want, groups, &y)) != GHIVE_OK) return op_error(st, g);
+        y.row_num = groups;
+        x = y;
+      }
+      out.cols.push_back(x);
+    }
+    out.rows = groups;
+    out.names = g.outputs;
+    ghive_plan_exec_info &I = ctx->plan_info;
+    I.fused = 1;
+    I.joins = (uint32_t)S.joins.size();
+    uint32_t staged32 = 0;  // columns that arrived as 32-bit copies from the staging
+    for (size_t c = 0; c < fact.cols.size(); ++c) staged32 += fact.logical[c] == GHIVE_LONG && fact.cols[c].type == GHIVE_INT;
+    for (const ExecTable &d : dims)
+      for (size_t c = 0; c < d.cols.size(); ++c) staged32 += d.logical[c] == GHIVE_LONG && d.cols[c].type == GHIVE_INT;
+    I.narrowed_columns = (uint32_t)narrows.size() + staged32;
+    I.has_filter = pred.empty() ? 0 : 1;
+    I.fused_kernel_ms = times.main_kernel_ms;
+    I.not_fused[0] = 0;
+    done = true;
+    return GHIVE_OK;
+  }
+
   int run(int node, ExecTable &out) {
     const Op &o = P.ops[node];
+    if (o.kind == "GroupBy" && o.vertex == vertex && ctx->tune.plan_fuse) {
+      FuseShape S;
+      std::string why;
+      if (fuse_shape(P, vertex, node, S, why)) {
+        bool done = false;
+        const int st = fused(node, S, out, done, why);
+        if (st != GHIVE_OK || done) return st;
+      }
+      snprintf(ctx->plan_info.not_fused, sizeof ctx->plan_info.not_fused, "%s: %s", o.id.c_str(), why.c_str());
+    }
     if ((o.kind == "ReduceSink" || o.kind == "FileSink" || o.kind == "Union") && o.vertex != vertex) {
       const int st = from_input(o.vertex, "", {}, o.outputs, out);  // the result of another vertex
       if (o.kind == "ReduceSink") out.reduce_keys = (int)o.key_expressions.size();
@@ -1035,50 +1595,10 @@ struct Executor {
                     static_cast<int32_t *>(offs), card, &card2);
     if (st != GHIVE_OK) return op_error(st, o);
     // the maintained columns: from the Java side's map, else the join outputs found by name
-    std::vector<std::vector<uint32_t>> keep(nt);
-    bool from_map = false;
-    for (size_t t = 0; t < nt; ++t)
-      for (int c : o.maintain_cols[t]) {
-        keep[t].push_back((uint32_t)c);
-        from_map = true;
-      }
-    if (!from_map) {
-      // Without the Java side's map: Hive names a join's outputs _colN by position in the
-      // concatenation of its inputs' columns, so input 0's columns keep their names and input 1's
-      // come renumbered after them. Input 0 maps by name; the rest map to input 1 by the offset that
-      // puts the last of them on input 1's last non-key column (exact when that column is kept, as
-      // in every SSB plan).
-      static const std::regex col_re(R"(^_col(\d+)$)");
-      std::vector<std::pair<long, std::string>> rest;
-      for (const std::string &n : o.outputs) {
-        const int c = ts[0].find(n);
-        if (c >= 0) {
-          keep[0].push_back((uint32_t)c);
-          continue;
-        }
-        std::smatch m;
-        if (nt != 2 || !std::regex_match(n, m, col_re))
-          return fail(GHIVE_ERR_UNSUPPORTED, "plan: " + o.id + ": output '" + n +
-                                                 "' is not a column of input 0; pass the maintain-values map");
-        rest.push_back({std::stol(m[1].str()), n});
-      }
-      if (!rest.empty()) {
-        long jmax = -1;
-        for (size_t c = 0; c < ts[1].names.size(); ++c) {
-          std::smatch m;
-          if ((int)c != o.join_cols[1][0] && std::regex_match(ts[1].names[c], m, col_re))
-            jmax = std::max(jmax, std::stol(m[1].str()));
-        }
-        const long offset = rest.back().first - jmax;
-        for (const auto &r : rest) {
-          const int c = jmax < 0 ? -1 : ts[1].find("_col" + std::to_string(r.first - offset));
-          if (c < 0)
-            return fail(GHIVE_ERR_UNSUPPORTED, "plan: " + o.id + ": cannot place output '" + r.second +
-                                                   "' in input 1; pass the maintain-values map");
-          keep[1].push_back((uint32_t)c);
-        }
-      }
-    }
+    std::vector<std::vector<std::string>> in_names;
+    for (auto &t : ts) in_names.push_back(t.names);
+    std::vector<std::vector<uint32_t>> keep;
+    if ((st = join_keep(o, in_names, keep, err)) != GHIVE_OK) return st;
     std::vector<const uint32_t *> kp;
     std::vector<uint32_t> kc;
     size_t ncols = 0;
@@ -1240,6 +1760,47 @@ extern "C" {
 
 int ghive_plan_execute(ghive_ctx *ctx, const ghive_plan *plan, const char *vertex, const ghive_plan_input *inputs,
                        uint32_t ninputs, char *err, uint64_t errcap, ghive_result **out) {
+  return ghive::plan_execute_staged(ctx, plan, vertex, inputs, ninputs, nullptr, err, errcap, out);
+}
+
+}  // extern "C"
+
+namespace ghive {
+
+void plan_narrow_candidates(const ghive_plan *plan, const char *vertex,
+                            std::map<std::string, std::set<uint32_t>> &out) {
+  out.clear();
+  if (!plan || !vertex) return;
+  for (size_t i = 0; i < plan->ops.size(); ++i) {
+    const Op &o = plan->ops[i];
+    if (o.kind != "GroupBy" || o.vertex != vertex) continue;
+    FuseShape S;
+    std::string why;
+    if (!fuse_shape(*plan, vertex, (int)i, S, why)) continue;
+    // the scanned table arrives named by the vertex being run (GTables) or by its table name
+    std::set<uint32_t> fact;
+    for (const FuseJoin &J : S.joins) fact.insert((uint32_t)J.fact_col);
+    for (const Sym &k : S.keys)
+      if (k.kind == Sym::kFact) fact.insert((uint32_t)k.col);
+    for (const FuseAgg &A : S.aggs) {
+      if (A.arg.kind == Sym::kFact) fact.insert((uint32_t)A.arg.col);
+      if (A.arg.kind == Sym::kMul) {
+        fact.insert((uint32_t)A.arg.a);
+        fact.insert((uint32_t)A.arg.b);
+      }
+    }
+    out[vertex].insert(fact.begin(), fact.end());
+    out[S.scan->table].insert(fact.begin(), fact.end());
+    for (const FuseJoin &J : S.joins) {
+      out[J.dim->vertex].insert((uint32_t)J.dim_key);
+      if (J.dim_pay >= 0) out[J.dim->vertex].insert((uint32_t)J.dim_pay);
+    }
+  }
+}
+
+int plan_execute_staged(ghive_ctx *ctx, const ghive_plan *plan, const char *vertex, const ghive_plan_input *inputs,
+                        uint32_t ninputs, const std::map<std::string, std::vector<char>> *narrowed, char *err,
+                        uint64_t errcap, ghive_result **out) {
   if (!ctx || !plan || !vertex || !out || (ninputs && !inputs)) {
     set_err(err, errcap, "plan_execute: NULL argument");
     return GHIVE_ERR_INVALID;
@@ -1250,7 +1811,9 @@ int ghive_plan_execute(ghive_ctx *ctx, const ghive_plan *plan, const char *verte
     set_err(err, errcap, std::string("plan_execute: no vertex '") + vertex + "'");
     return GHIVE_ERR_INVALID;
   }
+  memset(&ctx->plan_info, 0, sizeof ctx->plan_info);
   Executor X{ctx, *plan, vertex, {}, {}, {}};
+  X.narrowed = narrowed;
   for (uint32_t i = 0; i < ninputs; ++i)
     if (inputs[i].name) X.inputs[inputs[i].name] = &inputs[i];
   ExecTable t;
@@ -1288,6 +1851,10 @@ int ghive_plan_execute(ghive_ctx *ctx, const ghive_plan *plan, const char *verte
   *out = R;
   return GHIVE_OK;
 }
+
+}  // namespace ghive
+
+extern "C" {
 
 int ghive_result_info(const ghive_result *r, uint64_t *rows, uint32_t *ncols) {
   if (!r) return GHIVE_ERR_INVALID;

@@ -76,6 +76,14 @@ class Context:
         return dict(total_ms=t.total_ms, main_kernel_ms=t.main_kernel_ms, finalize_ms=t.finalize_ms,
                     rows_in=t.rows_in, rows_joined=t.rows_joined)
 
+    def plan_info(self):
+        """What the last Plan.execute on this context did (ghive_last_plan_info)."""
+        i = abi.PlanExecInfo()
+        self.check(lib().ghive_last_plan_info(self.h, C.byref(i)))
+        return dict(fused=bool(i.fused), joins=i.joins, narrowed_columns=i.narrowed_columns,
+                    has_filter=bool(i.has_filter), fused_kernel_ms=i.fused_kernel_ms,
+                    not_fused=i.not_fused.decode(errors="replace"))
+
     # ---- synthetic data (Philox4x32-10) -----------------------------------------------------
     def gen_uniform_i32(self, seed, col, n, lo, hi, row_begin=0, out=None):
         out = torch.empty(n, dtype=torch.int32, device=f"cuda:{self.device}") if out is None else out
@@ -536,7 +544,9 @@ class Fragment:
         g = C.c_uint64()
         fn = lib().ghive_fragment_execute_late if late else lib().ghive_fragment_execute
         ctx.check(fn(ctx.h, C.byref(self.f), ocols, capacity, C.byref(g)))
-        return [o[: g.value] for o in outs]
+        # aggregates over DOUBLE inputs come back as DOUBLE in the 8-byte buffers
+        return [(o.view(torch.float64) if ocols[i].type == abi.DOUBLE and o.dtype == torch.int64 else o)[: g.value]
+                for i, o in enumerate(outs)]
 
 
 class Batch:
@@ -712,6 +722,58 @@ class Server:
             pass
 
 
+def gtables_struct(tables, key_cnt=None):
+    """ghive_gtables for a list of (vertex name or None, [columns]) -- a column is a numpy array
+    (int64 LONG, float64 DOUBLE, int32 INT) or a (bytes blob, int32 (start,end) array) STRING column --
+    with the buffers of each type concatenated across tables in order, as the Java side hands them
+    over. Returns (struct, objects that must stay alive while it is used)."""
+    import numpy as np
+    keep = []
+    typecode = {np.dtype(np.int64): 0, np.dtype(np.float64): 1, np.dtype(np.int32): 2}
+    bufs = {0: [], 1: [], 2: [], 3: [], "idx": []}
+    types, ncols, rows, names = [], [], [], []
+    for name, cols in tables:
+        tt = []
+        n = None
+        for c in cols:
+            if isinstance(c, tuple):
+                blob, idx = c
+                idx = np.ascontiguousarray(idx, dtype=np.int32)
+                b = C.create_string_buffer(bytes(blob), max(len(blob), 1))
+                keep += [b, idx]
+                bufs[3].append(C.cast(b, C.c_void_p))
+                bufs["idx"].append(idx.ctypes.data)
+                tt.append(3)
+                n = len(idx) // 2
+            else:
+                a = np.ascontiguousarray(c)
+                keep.append(a)
+                code = typecode[a.dtype]
+                bufs[code].append(a.ctypes.data)
+                tt.append(code)
+                n = len(a)
+        arr = (C.c_int32 * max(len(tt), 1))(*tt)
+        keep.append(arr)
+        types.append(C.cast(arr, C.POINTER(C.c_int32)))
+        ncols.append(len(tt))
+        rows.append(n or 0)
+        names.append(None if name is None else name.encode())
+    g = abi.GTables()
+    g.ntables = len(tables)
+    g.types = (C.POINTER(C.c_int32) * max(len(types), 1))(*types)
+    g.ncols = (C.c_uint32 * max(len(ncols), 1))(*ncols)
+    g.row_cnt = (C.c_int32 * max(len(rows), 1))(*rows)
+    if key_cnt is not None:
+        g.key_cnt = (C.c_int32 * max(len(key_cnt), 1))(*key_cnt)
+    g.vertex_name = (C.c_char_p * max(len(names), 1))(*names)
+    for code, fld, cnt in ((0, "long_cols", "nlong"), (1, "double_cols", "ndouble"), (2, "int_cols", "nint"),
+                           (3, "string_cols", "nstring")):
+        setattr(g, fld, (C.c_void_p * max(len(bufs[code]), 1))(*bufs[code]))
+        setattr(g, cnt, len(bufs[code]))
+    g.string_idx_cols = (C.c_void_p * max(len(bufs["idx"]), 1))(*bufs["idx"])
+    return g, keep
+
+
 class GTablesStaged:
     """GTables (ghive/GTables.java) staged through ghive_gtables_stage. tables: a list of
     (vertex name or None, [columns]) where a column is a numpy array (int64 LONG, float64 DOUBLE,
@@ -719,51 +781,8 @@ class GTablesStaged:
     are concatenated across tables in order, as the Java side hands them over."""
 
     def __init__(self, ctx, tables, key_cnt=None):
-        import numpy as np
         self.ctx, self.h = ctx, C.c_void_p()
-        keep = []
-        typecode = {np.dtype(np.int64): 0, np.dtype(np.float64): 1, np.dtype(np.int32): 2}
-        bufs = {0: [], 1: [], 2: [], 3: [], "idx": []}
-        types, ncols, rows, names = [], [], [], []
-        for name, cols in tables:
-            tt = []
-            n = None
-            for c in cols:
-                if isinstance(c, tuple):
-                    blob, idx = c
-                    idx = np.ascontiguousarray(idx, dtype=np.int32)
-                    b = C.create_string_buffer(bytes(blob), max(len(blob), 1))
-                    keep += [b, idx]
-                    bufs[3].append(C.cast(b, C.c_void_p))
-                    bufs["idx"].append(idx.ctypes.data)
-                    tt.append(3)
-                    n = len(idx) // 2
-                else:
-                    a = np.ascontiguousarray(c)
-                    keep.append(a)
-                    code = typecode[a.dtype]
-                    bufs[code].append(a.ctypes.data)
-                    tt.append(code)
-                    n = len(a)
-            arr = (C.c_int32 * max(len(tt), 1))(*tt)
-            keep.append(arr)
-            types.append(C.cast(arr, C.POINTER(C.c_int32)))
-            ncols.append(len(tt))
-            rows.append(n or 0)
-            names.append(None if name is None else name.encode())
-        g = abi.GTables()
-        g.ntables = len(tables)
-        g.types = (C.POINTER(C.c_int32) * max(len(types), 1))(*types)
-        g.ncols = (C.c_uint32 * max(len(ncols), 1))(*ncols)
-        g.row_cnt = (C.c_int32 * max(len(rows), 1))(*rows)
-        if key_cnt is not None:
-            g.key_cnt = (C.c_int32 * max(len(key_cnt), 1))(*key_cnt)
-        g.vertex_name = (C.c_char_p * max(len(names), 1))(*names)
-        for code, fld, cnt in ((0, "long_cols", "nlong"), (1, "double_cols", "ndouble"), (2, "int_cols", "nint"),
-                               (3, "string_cols", "nstring")):
-            setattr(g, fld, (C.c_void_p * max(len(bufs[code]), 1))(*bufs[code]))
-            setattr(g, cnt, len(bufs[code]))
-        g.string_idx_cols = (C.c_void_p * max(len(bufs["idx"]), 1))(*bufs["idx"])
+        g, self._keep = gtables_struct(tables, key_cnt)
         ctx.check(lib().ghive_gtables_stage(ctx.h, C.byref(g), C.byref(self.h)))
 
     def inputs(self):
@@ -891,3 +910,53 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def gpu_process(ctx, plan, tables, this_vertex, key_cnt=None):
+    """GPUProcess in one call (ghive_gpu_process): GTables host buffers in (as gtables_struct takes
+    them), the vertex run, the GPUResult out as (column arrays in output order, sequence, counts), or
+    None for a missing input (a table named None). STRING columns come back as (blob, pairs)."""
+    import numpy as np
+    g, keep = gtables_struct(tables, key_cnt)
+    err = C.create_string_buffer(512)
+    h = C.c_void_p()
+    st = lib().ghive_gpu_process(ctx.h, plan.h, C.byref(g), this_vertex.encode(), err, len(err), C.byref(h))
+    if st != abi.OK:
+        raise GhiveError(st, err.value.decode(errors="replace") or lib().ghive_last_error(ctx.h).decode())
+    if not h.value:
+        return None
+    try:
+        rows, ncols = C.c_uint64(), C.c_uint32()
+        counts = (C.c_uint32 * 4)()
+        seq = C.POINTER(C.c_int32)()
+        ctx.check(lib().ghive_gpu_result_info(h, C.byref(rows), C.byref(ncols), counts, C.byref(seq)))
+        n = rows.value
+
+        def arr(ptr, count, dtype):  # an empty result column has no buffer
+            return np.ctypeslib.as_array(ptr, (count,)).copy() if count else np.zeros(0, dtype=dtype)
+
+        groups = []
+        for i in range(counts[0]):
+            groups.append(arr(lib().ghive_gpu_result_long(h, i), n, np.int64))
+        for i in range(counts[1]):
+            groups.append(arr(lib().ghive_gpu_result_double(h, i), n, np.float64))
+        for i in range(counts[2]):
+            groups.append(arr(lib().ghive_gpu_result_int(h, i), n, np.int32))
+        for i in range(counts[3]):
+            blob, nb, pairs = C.c_char_p(), C.c_uint64(), C.POINTER(C.c_int32)()
+            ctx.check(lib().ghive_gpu_result_string(h, i, C.byref(blob), C.byref(nb), C.byref(pairs)))
+            b = C.string_at(blob, nb.value) if nb.value else b""
+            groups.append((b, arr(pairs, 2 * n, np.int32)))
+        sequence = [seq[c] for c in range(ncols.value)]
+        cols = [groups[q] for q in sequence]
+        return cols, sequence, list(counts)
+    finally:
+        lib().ghive_gpu_result_free(h)
+        del keep
+
+
+def process_times(ctx):
+    """Wall-clock split of the last gpu_process (ghive_last_process_times)."""
+    t = abi.ProcessTimes()
+    ctx.check(lib().ghive_last_process_times(ctx.h, C.byref(t)))
+    return {f: getattr(t, f) for f, _ in abi.ProcessTimes._fields_}

@@ -379,8 +379,13 @@ typedef struct ghive_fragment_key {
   uint32_t index;
 } ghive_fragment_key;
 
-/* Aggregate over a fact expression: EXPR_COL a; EXPR_MUL a*b (int64 product of INT/LONG
- * columns, as SelectPredicate's LONG multiply); EXPR_ONE counts rows. */
+/* Aggregate over a fact expression: EXPR_COL a; EXPR_MUL a*b; EXPR_ONE counts rows. Inputs are
+ * INT columns (16-byte aligned; the product is the int64 product, as SelectPredicate's LONG
+ * multiply) or DOUBLE columns (both, for a product: the product rounded to a double, as Hive's
+ * double multiply; read only for the rows that survive the joins). SUM over DOUBLE inputs is an
+ * exact fixed-point sum of the values (LSB chosen from the columns' largest magnitudes and the row
+ * count) rounded once to DOUBLE; MIN / MAX over DOUBLE give DOUBLE; every other aggregate gives
+ * LONG. */
 typedef struct ghive_fragment_agg {
   int32_t type; /* GHIVE_AGG_SUM / CNT / MIN / MAX */
   int32_t expr;
@@ -401,7 +406,8 @@ typedef struct ghive_fragment {
   uint32_t naggs;
 } ghive_fragment;
 
-/* Output: keys (INT each) then aggregates (LONG) into out_cols of `capacity` rows. */
+/* Output: keys (INT each) then aggregates (LONG, or DOUBLE as above) into out_cols of `capacity`
+ * rows (8-byte buffers for the aggregates); out_cols[i].type is set. */
 int ghive_fragment_execute(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_cols,
                            uint64_t capacity, uint64_t *out_groups);
 
@@ -569,6 +575,24 @@ typedef struct ghive_result ghive_result;
 int ghive_plan_execute(ghive_ctx *ctx, const ghive_plan *plan, const char *vertex,
                        const ghive_plan_input *inputs, uint32_t ninputs, char *err, uint64_t errcap,
                        ghive_result **out);
+/* What the last ghive_plan_execute on this context did with its vertex's group-by. A map vertex
+ * of the CS1 shape -- TableScan -> [Filter] -> [Select] -> MapJoin+ (inner, one broadcast
+ * dimension each, single-column keys) -> [Select] -> GroupBy (hash mode; keys: dimension attributes
+ * or fact columns, at most 2; SUM / COUNT / MIN / MAX of a fact column or of a product of two) ->
+ * ReduceSink -- runs as one ghive_fragment_execute over the fact columns instead of the operator
+ * chain (option "plan.fuse", default 1). LONG columns the fused kernel streams are given to it as
+ * 32-bit copies when every value fits; IS NOT NULL conjuncts on probed keys are dropped (an inner
+ * join drops NULL keys anyway). The result is the same as the operator chain's. */
+typedef struct ghive_plan_exec_info {
+  int32_t fused;             /* 1: the vertex's group-by ran as one fused fragment              */
+  uint32_t joins;            /* map joins folded into it                                       */
+  uint32_t narrowed_columns; /* LONG columns it read as 32-bit copies                          */
+  int32_t has_filter;        /* 1: a fact predicate was evaluated inside the fused kernel      */
+  float fused_kernel_ms;     /* the fused kernel (HIP events)                                  */
+  uint32_t reserved;
+  char not_fused[160];       /* why a vertex with a group-by ran as the operator chain         */
+} ghive_plan_exec_info;
+int ghive_last_plan_info(ghive_ctx *ctx, ghive_plan_exec_info *out);
 /* ---- the JNI boundary's marshalling, without a JVM ---------------------------------------- *
  * GTables (ghive/GTables.java:13-31) as the addresses of its direct ByteBuffers, in GTables
  * order: per table its column type codes (0 LONG, 1 DOUBLE, 2 INT, 3 STRING; other codes are
@@ -607,6 +631,42 @@ void ghive_staged_free(ghive_ctx *ctx, ghive_staged *s);
  * in that concatenation, counts[4] = the columns of each group. FLOAT is numbered as DOUBLE. Host
  * only. */
 int ghive_gpuresult_sequence(const int32_t *types, uint32_t ncols, int32_t *sequence, uint32_t counts[4]);
+
+/* GPUProcess in one call (Java_..._JNIInterface_GPUProcess, gpu_impl/src/ghive_JNIInterface.cu:13-221,
+ * with generate_tbl_result :223-342): GTables in, the vertex `this_vertex` of the plan run, a GPUResult
+ * out -- so that a JNI shim is a pure translation of Java arrays. The table whose vertex name is
+ * this_vertex is the vertex's own scan input. Staging sends the LONG columns a fused form of the
+ * vertex streams as 32-bit integers when every value fits (half the bytes over PCIe and through HBM;
+ * they are LONG again to every other operator). A table with a NULL vertex name is a missing input:
+ * GHIVE_OK with *out = NULL and nothing run, as the reference returns nullptr. The result is host
+ * memory grouped as GPUResult's constructor takes it: counts[4] columns of LONG, DOUBLE (FLOAT
+ * widened), INT and STRING, each group in column order, and sequence[c] = the position of output
+ * column c in that concatenation; a STRING column is a UTF-8 blob plus int32 (start,end) per row.
+ * Everything staged on the device is released before the call returns. */
+typedef struct ghive_gpu_result ghive_gpu_result;
+int ghive_gpu_process(ghive_ctx *ctx, const ghive_plan *plan, const ghive_gtables *g, const char *this_vertex,
+                      char *err, uint64_t errcap, ghive_gpu_result **out);
+int ghive_gpu_result_info(const ghive_gpu_result *r, uint64_t *rows, uint32_t *ncols, uint32_t counts[4],
+                          const int32_t **sequence);
+/* Column i of a group (i < counts[group]); NULL when out of range. */
+const int64_t *ghive_gpu_result_long(const ghive_gpu_result *r, uint32_t i);
+const double *ghive_gpu_result_double(const ghive_gpu_result *r, uint32_t i);
+const int32_t *ghive_gpu_result_int(const ghive_gpu_result *r, uint32_t i);
+int ghive_gpu_result_string(const ghive_gpu_result *r, uint32_t i, const char **blob, uint64_t *blob_bytes,
+                            const int32_t **pairs);
+void ghive_gpu_result_free(ghive_gpu_result *r);
+/* Wall-clock split of the last ghive_gpu_process on this context (host steady clock, ms). */
+typedef struct ghive_process_times {
+  double stage_ms;           /* GTables buffers -> device columns (host copies + PCIe)  */
+  double execute_ms;         /* the vertex's operators on the device                    */
+  double result_ms;          /* device result -> host GPUResult arrays                  */
+  double total_ms;
+  uint64_t h2d_bytes;        /* bytes sent to the device                                */
+  uint32_t narrowed_columns; /* LONG columns sent as 32-bit integers                    */
+  int32_t fused;             /* the vertex ran as one fused fragment                    */
+} ghive_process_times;
+int ghive_last_process_times(ghive_ctx *ctx, ghive_process_times *out);
+
 int ghive_result_info(const ghive_result *r, uint64_t *rows, uint32_t *ncols);
 /* Column i (device data, owned by the result) and its name. */
 int ghive_result_column(const ghive_result *r, uint32_t i, ghive_column *col, const char **name);

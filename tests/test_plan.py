@@ -135,3 +135,23 @@ def test_plan_errors(text, maintain):
     with pytest.raises(GhiveError) as e:
         ops.Plan(text, None, maintain)
     assert e.value.status == abi.ERR_INVALID
+
+
+def test_fused_form_recognised_from_the_plan():
+    """The map vertex of the CS1 shape (scan, filter, map joins, group-by, sink) has a fused form;
+    reduce vertices and chains through a merge join do not (host-only: the plan decides the shape)."""
+    from ghive_amd import ops
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+    def gbys(d, a, b):
+        p = ops.Plan(open(os.path.join(golden, d, a)).read(), open(os.path.join(golden, d, b)).read())
+        return {k: (o["vertex"], o["fused_form"], o["not_fused"]) for k, o in p.describe()["operators"].items()
+                if o["kind"] == "GroupBy"}
+
+    g = gbys("plans_authored", "star3.txt", "star3_extended.txt")
+    assert g["GBY_42"][:2] == ("Map 1", True) and g["GBY_44"][1] is False and "mergepartial" in g["GBY_44"][2]
+    for schema in ("ssb_10_orc_transformed", "ssb_2_orc"):
+        g = gbys("plans", f"{schema}_1_1.txt", f"{schema}_1_1_extended.txt")
+        assert g["GBY_29"][:2] == ("Map 1", True)
+    g = gbys("plans", "ssb_10_orc_transformed_2_1.txt", "ssb_10_orc_transformed_2_1_extended.txt")
+    assert g["GBY_22"][1] is False and "MERGEJOIN_43" in g["GBY_22"][2]
