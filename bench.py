@@ -101,10 +101,13 @@ def batched_section(ctx, reps):
                         probe_mode=best["probe_mode"])
     bt.run()
     lat = []
+    gc.collect()
+    gc.disable()  # as for the plain steps: no collection of the harness's heap inside a timed run
     for _ in range(reps):
         t1 = time.perf_counter()
         bt.run()
         lat.append(1e3 * (time.perf_counter() - t1))
+    gc.enable()
     dev_ms = bt.info()["last_ms"]
     sums, counts = bt.results()
     got = [{g: int(sums[f, g]) for g in np.nonzero(counts[f])[0].tolist()} for f in range(len(B.rows))]
