@@ -65,6 +65,7 @@ SIGNATURES = {
                                 C.POINTER(abi.Column), u64, PU64]),
     "ghive_sort": (C.c_int, [P, C.POINTER(abi.Table), P, C.c_char_p, u32, P]),
     "ghive_fragment_execute": (C.c_int, [P, C.POINTER(abi.Fragment), C.POINTER(abi.Column), u64, PU64]),
+    "ghive_fragment_execute_noalloc": (C.c_int, [P, C.POINTER(abi.Fragment), C.POINTER(abi.Column), u64, PU64]),
     "ghive_fragment_execute_late": (C.c_int, [P, C.POINTER(abi.Fragment), C.POINTER(abi.Column), u64,
                                               PU64]),
     "ghive_set_option": (C.c_int, [P, C.c_char_p, i64]),

@@ -78,6 +78,8 @@ struct ghive_ctx {
   ghive_stage_times times{};
   ghive_plan_exec_info plan_info{};        // what the last ghive_plan_execute did
   ghive_process_times process_times{};     // the last ghive_gpu_process
+  // the device address of this device's static scratch array (ctx.hip), looked up at init
+  void *scratch = nullptr;
   ghive_tuning tune{};
   // reusable small device scratch for counters (8 x uint64) and its host mirror
   uint64_t *d_counters = nullptr;
@@ -91,6 +93,10 @@ struct ghive_ctx {
 };
 
 namespace ghive {
+
+// Bytes of the static device array that ghive_fragment_execute_noalloc carves its group table,
+// counters and staging from (one array per device, shared by every call on that device).
+constexpr uint64_t kNoAllocScratchBytes = 8ull << 20;
 
 int set_error(ghive_ctx *ctx, int code, const char *fmt, ...);
 // Called by ghive_shutdown before the context's memory is released (hash.hip, batch.hip).

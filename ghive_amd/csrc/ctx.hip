@@ -154,6 +154,32 @@ extern "C" {
 
 int ghive_abi_version(void) { return GHIVE_ABI_VERSION; }
 
+// The scratch memory of ghive_fragment_execute_noalloc: one array per device for the life of the
+// process, so that the call needs no allocation. In host code `scratch` names a host-side stand-in,
+// not device memory: only the address hipGetSymbolAddress returns for the current device is used.
+static __device__ __align__(256) __attribute__((used)) uint8_t scratch[ghive::kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch_ptr;  // device -> device address of its copy of `scratch`
+
+// Looked up once per device, with that device current.
+int scratch_address(int device, void **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it == g_scratch_ptr.end()) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return GHIVE_ERR_HIP;
+    }
+    it = g_scratch_ptr.emplace(device, p).first;
+  }
+  *out = it->second;
+  return GHIVE_OK;
+}
+}  // namespace
+
 int ghive_init(int device, uint64_t mem_limit_bytes, ghive_ctx **out) {
   if (!out) return GHIVE_ERR_INVALID;
   *out = nullptr;
@@ -184,6 +210,10 @@ int ghive_init(int device, uint64_t mem_limit_bytes, ghive_ctx **out) {
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 1ull << 34;
   uint64_t limit = mem_limit_bytes ? mem_limit_bytes : (uint64_t)(0.9 * (double)free_b);
+  if (scratch_address(device, &ctx->scratch) != GHIVE_OK) {
+    delete ctx;
+    return GHIVE_ERR_HIP;
+  }
   ctx->arena = new DeviceArena(limit);
   ctx->d_counters = static_cast<uint64_t *>(ctx->arena->alloc(64 * sizeof(uint64_t)));
   if (!ctx->d_counters || hipHostMalloc((void **)&ctx->h_words, kMaxReadWords * sizeof(uint64_t),

@@ -337,8 +337,39 @@ int fragment_launch(ghive_ctx *ctx, const FragPlan &P, unsigned long long *d_joi
   return GHIVE_OK;
 }
 
+namespace {
+
+// Device memory of one fragment run: from the context's memory manager, or carved in order out of
+// a caller-provided region (the library's static scratch array, for the entry point that
+// allocates nothing).
+struct FragMem {
+  ghive_ctx *ctx;
+  char *base = nullptr;  // non-null: carve from [base, base + cap)
+  uint64_t cap = 0, used = 0;
+  std::vector<DevBuf> bufs;
+  int get(uint64_t bytes, void **p) {
+    if (base) {
+      const uint64_t off = (used + 255) & ~255ull;
+      if (off + bytes > cap)
+        return set_error(ctx, GHIVE_ERR_UNSUPPORTED,
+                         "fragment (no allocation): needs more than the %llu bytes of its scratch array "
+                         "(group table of %llu slots); use ghive_fragment_execute",
+                         (unsigned long long)cap, (unsigned long long)bytes);
+      *p = base + off;
+      used = off + bytes;
+      return GHIVE_OK;
+    }
+    bufs.emplace_back();
+    GH_TRY(bufs.back().alloc(ctx, bytes));
+    *p = bufs.back().get();
+    return GHIVE_OK;
+  }
+};
+
+}  // namespace
+
 int fragment_run(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_cols, uint64_t capacity,
-                 uint64_t *out_groups, int late) {
+                 uint64_t *out_groups, int late, FragMem &mem) {
   if (!out_groups) return set_error(ctx, GHIVE_ERR_INVALID, "fragment: out_groups is NULL");
   FragPlan P;
   GH_TRY(fragment_plan(ctx, frag, late, P));
@@ -347,22 +378,29 @@ int fragment_run(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_c
     if (!out_cols[c].data) return set_error(ctx, GHIVE_ERR_INVALID, "fragment: output column %u has no buffer", c);
   uint64_t table_groups = capacity < P.group_bound ? capacity : P.group_bound;
   if (P.pipe_form) table_groups = P.F.dsize;  // the slots are addressed by key, whatever the capacity
-  DevBuf dkeys, dacc, dcnt, djoined, dstage;
-  GH_TRY(group_table_alloc(ctx, P.T, table_groups ? table_groups : 1, dkeys, dacc, dcnt));
+  uint64_t key_bytes = 0, acc_bytes = 0;
+  group_table_size(P.T, table_groups ? table_groups : 1, &key_bytes, &acc_bytes);
+  void *dkeys = nullptr, *dacc = nullptr, *dcnt = nullptr, *djoined = nullptr;
+  GH_TRY(mem.get(key_bytes, &dkeys));
+  GH_TRY(mem.get(acc_bytes, &dacc));
+  GH_TRY(mem.get(3 * sizeof(unsigned long long), &dcnt));  // entries, overflow, rows joined
+  djoined = static_cast<unsigned long long *>(dcnt) + 2;
+  GH_TRY(group_table_init(ctx, P.T, static_cast<uint64_t *>(dkeys), static_cast<uint64_t *>(dacc),
+                          static_cast<unsigned long long *>(dcnt)));
   const uint64_t tile_rows = (uint64_t)P.threads * (uint64_t)P.rows;
   if (P.F.n % tile_rows) {
-    GH_TRY(dstage.alloc(ctx, (uint64_t)(P.F.njoins + 2 * P.F.naggs) * tile_rows * sizeof(int32_t)));
-    P.F.stage = dstage.as<int32_t>();
+    void *stage = nullptr;
+    GH_TRY(mem.get((uint64_t)(P.F.njoins + 2 * P.F.naggs) * tile_rows * sizeof(int32_t), &stage));
+    P.F.stage = static_cast<int32_t *>(stage);
   }
   P.T.capacity = capacity;
-  GH_TRY(djoined.alloc(ctx, sizeof(unsigned long long)));
-  GH_TRY(fill_u64(ctx, djoined.as<uint64_t>(), 0, 1));
+  GH_TRY(fill_u64(ctx, static_cast<uint64_t *>(djoined), 0, 1));
   if (frag->nkeys == 0) {  // one group, even over no rows
     if (P.pipe_form) GH_TRY(fill_u64(ctx, P.T.keys, 0ull, 1));  // direct slot 0 holds key 0
     else GH_TRY(group_table_preinsert(ctx, P.T, 0ull));
   }
   GH_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-  GH_TRY(fragment_launch(ctx, P, djoined.as<unsigned long long>()));
+  GH_TRY(fragment_launch(ctx, P, static_cast<unsigned long long *>(djoined)));
   GH_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
   OutSpec O;
   memset(&O, 0, sizeof O);
@@ -372,7 +410,8 @@ int fragment_run(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_c
     if (out_cols) out_cols[c].type = O.type[c];
   }
   uint64_t joined = 0;
-  int st = group_table_emit(ctx, P.K, P.T, O, capacity, out_groups, ctx->ev[2], djoined.as<uint64_t>(), &joined);
+  int st = group_table_emit(ctx, P.K, P.T, O, capacity, out_groups, ctx->ev[2], static_cast<uint64_t *>(djoined),
+                            &joined);
   if (st != GHIVE_OK && st != GHIVE_ERR_CAPACITY) return st;
   float ms_main = 0, ms_fin = 0;
   (void)hipEventElapsedTime(&ms_main, ctx->ev[0], ctx->ev[1]);
@@ -395,13 +434,33 @@ extern "C" {
 int ghive_fragment_execute(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_cols,
                            uint64_t capacity, uint64_t *out_groups) {
   GH_CHECK_CTX(ctx);
-  return fragment_run(ctx, frag, out_cols, capacity, out_groups, (int)ctx->tune.frag_late);
+  FragMem mem{ctx};
+  return fragment_run(ctx, frag, out_cols, capacity, out_groups, (int)ctx->tune.frag_late, mem);
 }
 
 int ghive_fragment_execute_late(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_cols,
                                 uint64_t capacity, uint64_t *out_groups) {
   GH_CHECK_CTX(ctx);
-  return fragment_run(ctx, frag, out_cols, capacity, out_groups, /*late=*/1);
+  FragMem mem{ctx};
+  return fragment_run(ctx, frag, out_cols, capacity, out_groups, /*late=*/1, mem);
+}
+
+int ghive_fragment_execute_noalloc(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_cols,
+                                   uint64_t capacity, uint64_t *out_groups) {
+  GH_CHECK_CTX(ctx);
+  if (!ctx->scratch) return set_error(ctx, GHIVE_ERR_HIP, "fragment (no allocation): no scratch array on this device");
+  if (!frag) return set_error(ctx, GHIVE_ERR_INVALID, "fragment: NULL descriptor");
+  // finishing a deferred build allocates its dense companion: the tables must be finished already
+  for (uint32_t j = 0; j < frag->njoins && frag->joins; ++j) {
+    const ghive_hash_table *h = frag->joins[j].table;
+    if (h && h->pending.load())
+      return set_error(ctx, GHIVE_ERR_INVALID, "fragment (no allocation): join %u's table is still being built "
+                       "(build it with ghive_hash_build, or use it once first)", j);
+  }
+  FragMem mem{ctx};
+  mem.base = static_cast<char *>(ctx->scratch);
+  mem.cap = kNoAllocScratchBytes;
+  return fragment_run(ctx, frag, out_cols, capacity, out_groups, (int)ctx->tune.frag_late, mem);
 }
 
 }  // extern "C"

@@ -625,20 +625,32 @@ int lds_log2_for(int nwords, int budget_bytes) {
   return lg;
 }
 
-int group_table_alloc(ghive_ctx *ctx, GroupTableDev &T, uint64_t capacity, DevBuf &keys, DevBuf &acc,
-                      DevBuf &counters) {
-  uint64_t slots = next_pow2(capacity * 2 < 64 ? 64 : capacity * 2);
+void group_table_size(GroupTableDev &T, uint64_t capacity, uint64_t *key_bytes, uint64_t *acc_bytes) {
+  const uint64_t slots = next_pow2(capacity * 2 < 64 ? 64 : capacity * 2);
   T.stride = slots + 1;
   T.mask = slots - 1;
   T.log2 = __builtin_ctzll(slots);
   T.capacity = capacity;
-  GH_TRY(keys.alloc(ctx, T.stride * 8));
   T.rec = record_words(T.nwords);
-  GH_TRY(acc.alloc(ctx, T.stride * 8 * (uint64_t)T.rec));
+  *key_bytes = T.stride * 8;
+  *acc_bytes = T.stride * 8 * (uint64_t)T.rec;
+}
+
+int group_table_alloc(ghive_ctx *ctx, GroupTableDev &T, uint64_t capacity, DevBuf &keys, DevBuf &acc,
+                      DevBuf &counters) {
+  uint64_t kb = 0, ab = 0;
+  group_table_size(T, capacity, &kb, &ab);
+  GH_TRY(keys.alloc(ctx, kb));
+  GH_TRY(acc.alloc(ctx, ab));
   GH_TRY(counters.alloc(ctx, 2 * sizeof(unsigned long long)));
-  T.keys = keys.as<uint64_t>();
-  T.acc = acc.as<uint64_t>();
-  T.nent = counters.as<unsigned long long>();
+  return group_table_init(ctx, T, keys.as<uint64_t>(), acc.as<uint64_t>(), counters.as<unsigned long long>());
+}
+
+int group_table_init(ghive_ctx *ctx, GroupTableDev &T, uint64_t *keys, uint64_t *acc, unsigned long long *counters) {
+  const uint64_t slots = T.stride - 1;
+  T.keys = keys;
+  T.acc = acc;
+  T.nent = counters;
   T.overflow = T.nent + 1;
   GH_TRY(fill_u64(ctx, T.keys, kEmptyKey, slots));
   GH_TRY(fill_u64(ctx, T.keys + slots, 0, 1));

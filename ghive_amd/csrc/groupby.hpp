@@ -32,6 +32,11 @@ struct OutSpec {
 
 // Largest power-of-two LDS table (log2) whose keys + words fit in budget_bytes.
 int lds_log2_for(int nwords, int budget_bytes);
+// Sizes the global table for `capacity` groups (T.aggs/naggs/nwords set): fills T's shape and the
+// bytes of its key and accumulator arrays (the two counters take 16 more bytes).
+void group_table_size(GroupTableDev &T, uint64_t capacity, uint64_t *key_bytes, uint64_t *acc_bytes);
+// Initialises a table sized by group_table_size in caller memory (fill kernels only).
+int group_table_init(ghive_ctx *ctx, GroupTableDev &T, uint64_t *keys, uint64_t *acc, unsigned long long *counters);
 // Allocates and initialises the global table for `capacity` groups (T.aggs/naggs/nwords set).
 int group_table_alloc(ghive_ctx *ctx, GroupTableDev &T, uint64_t capacity, DevBuf &keys, DevBuf &acc,
                       DevBuf &counters);

@@ -530,10 +530,10 @@ class Fragment:
         self.f = f
         self.nkeys, self.naggs = len(keys), len(aggs)
 
-    def execute(self, ctx, capacity=4096, late=False, outs=None):  # noqa: D102 (see class doc)
-        return self._execute(ctx, capacity, late, outs)
+    def execute(self, ctx, capacity=4096, late=False, outs=None, noalloc=False):  # noqa: D102 (see class doc)
+        return self._execute(ctx, capacity, late, outs, noalloc)
 
-    def _execute(self, ctx, capacity, late, outs):
+    def _execute(self, ctx, capacity, late, outs, noalloc=False):
         dev = self.fact.columns[0].device
         if outs is None:
             outs = ([torch.empty(capacity, dtype=torch.int32, device=dev) for _ in range(self.nkeys)] +
@@ -542,7 +542,8 @@ class Fragment:
         for i, o in enumerate(outs):
             ocols[i].data = o.data_ptr()
         g = C.c_uint64()
-        fn = lib().ghive_fragment_execute_late if late else lib().ghive_fragment_execute
+        fn = (lib().ghive_fragment_execute_noalloc if noalloc else
+              lib().ghive_fragment_execute_late if late else lib().ghive_fragment_execute)
         ctx.check(fn(ctx.h, C.byref(self.f), ocols, capacity, C.byref(g)))
         # aggregates over DOUBLE inputs come back as DOUBLE in the 8-byte buffers
         return [(o.view(torch.float64) if ocols[i].type == abi.DOUBLE and o.dtype == torch.int64 else o)[: g.value]

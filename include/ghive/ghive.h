@@ -411,6 +411,18 @@ typedef struct ghive_fragment {
 int ghive_fragment_execute(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_cols,
                            uint64_t capacity, uint64_t *out_groups);
 
+/* ghive_fragment_execute that allocates nothing. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The group table, its counters and the partial-tile staging
+ * come from a static array of GHIVE_NOALLOC_SCRATCH_BYTES on each device (held until the process
+ * ends); a fragment whose group table does not fit is rejected with GHIVE_ERR_UNSUPPORTED. The join
+ * tables must be finished (ghive_hash_build, or an asynchronous build already used once). Results
+ * equal ghive_fragment_execute's. Every call on a device shares that device's array and
+ * re-initialises it: calls that overlap on one device race. */
+#define GHIVE_NOALLOC_SCRATCH_BYTES (8u << 20)
+int ghive_fragment_execute_noalloc(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_cols,
+                                   uint64_t capacity, uint64_t *out_groups);
+
 /* ---- batched form: many fragments of one query shape in one persistent kernel ----------- *
  * No counterpart in the reference (it runs one Tez task per JNI call). Query per item:
  *   SELECT g, SUM(a*b) FROM fact JOIN dim ON fact.key = dim.key
