@@ -9,9 +9,9 @@
 // Accumulators per aggregate kind:
 //   CNT  : count                               SUMI : int64 sum (wraps like a Java long), count
 //   MINI/MAXI : int64 value, count             MIND/MAXD : order-preserving double bits, count
-//   SUMD : exact fixed-point sum of doubles, LSB 2^E, as two unsigned 128-bit halves
-//          (positives, negatives) -- integer addition, so the result does not depend on the
-//          order rows arrive in -- plus count.
+//   SUMD : exact fixed-point sum of doubles, LSB 2^E, as one two's-complement 128-bit integer
+//          (E is chosen so that |sum| < 2^126: no overflow) -- integer addition, so the result
+//          does not depend on the order rows arrive in -- plus count.
 #pragma once
 
 #include "device.hpp"
@@ -23,7 +23,7 @@ constexpr int kMaxAggs = 8;
 constexpr int kMaxKeys = 4;
 constexpr uint64_t kEmptyKey = ~0ull;
 
-__host__ __device__ inline int agg_words(int kind) { return kind == AK_CNT ? 1 : (kind == AK_SUMD ? 5 : 2); }
+__host__ __device__ inline int agg_words(int kind) { return kind == AK_CNT ? 1 : (kind == AK_SUMD ? 3 : 2); }
 
 struct AggSpec {
   int8_t kind;
@@ -181,10 +181,14 @@ __device__ __forceinline__ void acc_apply(const AggSpec &A, Word word, const Agg
       atomicMax((unsigned long long *)word(0), (unsigned long long)v.a);
       atomicAdd((unsigned long long *)word(1), 1ull);
       break;
-    default: {  // AK_SUMD
-      int base = v.neg ? 2 : 0;
-      add128(word(base), word(base + 1), v.a, v.b);
-      atomicAdd((unsigned long long *)word(4), 1ull);
+    default: {  // AK_SUMD: the signed value in two's complement, added modulo 2^128
+      uint64_t lo = v.a, hi = v.b;
+      if (v.neg) {
+        lo = ~lo + 1ull;
+        hi = ~hi + (lo == 0 ? 1ull : 0ull);
+      }
+      add128(word(0), word(1), lo, hi);
+      atomicAdd((unsigned long long *)word(2), 1ull);
       break;
     }
   }
@@ -221,11 +225,10 @@ __device__ __forceinline__ void acc_merge(const AggSpec &A, Dst dst, Src src) {
       break;
     }
     default: {
-      uint64_t c = *src(4);
+      uint64_t c = *src(2);
       if (!c) break;
       add128(dst(0), dst(1), *src(0), *src(1));
-      add128(dst(2), dst(3), *src(2), *src(3));
-      atomicAdd((unsigned long long *)dst(4), (unsigned long long)c);
+      atomicAdd((unsigned long long *)dst(2), (unsigned long long)c);
       break;
     }
   }

@@ -1,8 +1,10 @@
 // Hash group-by (GroupByPredicate::gpu_execute, gpu_impl/src/Operator/GroupByPredicate.cu:19-359,
 // which sorts row ids with a comparator and runs reduce_by_key; here one pass aggregates into
 // LDS-private tables per workgroup that spill to and merge into one global table).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "agg.hpp"
 #include "common.hpp"
@@ -172,7 +174,8 @@ __global__ __launch_bounds__(kGbThreads) void k_groupby(KeySpec K, GroupTableDev
 // hash partition is a contiguous range, and (3) aggregates every partition in one workgroup's
 // LDS table, then stores that table with plain stores into the partition's own region of the
 // global table (regions are disjoint, so no atomics), where the usual finalize emits it.
-constexpr int kGbPartThreads = 512;
+constexpr int kGbPartThreads = 1024;  // one partition per workgroup; its LDS table allows one per CU
+constexpr int kEmitRows = 8;          // rows per thread per step of the partition emit
 
 // splitmix64 finalizer (Steele, Lea, Flood, OOPSLA 2014): every input bit reaches the top bits.
 __device__ __forceinline__ uint64_t mix64(uint64_t x) {
@@ -465,17 +468,13 @@ __device__ __forceinline__ void finalize_words(const AggSpec &A, Words W, void *
       return;
     }
     default: {
-      uint64_t c = W(4);
+      uint64_t c = W(2);
       if (!c) return write_null(col, A.out_type, idx);
-      uint64_t plo = W(0), phi = W(1), nlo = W(2), nhi = W(3);
-      bool neg = nhi > phi || (nhi == phi && nlo > plo);
-      uint64_t lo, hi;
-      if (!neg) {
-        lo = plo - nlo;
-        hi = phi - nhi - (plo < nlo);
-      } else {
-        lo = nlo - plo;
-        hi = nhi - phi - (nlo < plo);
+      uint64_t lo = W(0), hi = W(1);  // two's complement
+      const bool neg = hi >> 63;
+      if (neg) {
+        lo = ~lo + 1ull;
+        hi = ~hi + (lo == 0 ? 1ull : 0ull);
       }
       double d = fixed_to_double(lo, hi, A.scale_e);
       if (neg) d = -d;
@@ -535,87 +534,120 @@ __global__ void k_groupby_finalize(KeySpec K, GroupTableDev T, OutSpec O, unsign
 template <int VAL>
 __global__ __launch_bounds__(kGbPartThreads) void k_gb_part_emit(KeySpec K, GroupTableDev T, const uint64_t *starts,
                                                                  const uint64_t *hs, const uint64_t *vs,
-                                                                 const uint32_t *ids, int lcap_log2,
+                                                                 const uint32_t *ids, uint32_t lcap, uint32_t nparts,
                                                                  unsigned long long *fail,
                                                                  unsigned long long *out_count, OutSpec O,
                                                                  uint64_t cap) {
+  // One resident workgroup per CU takes partitions blockIdx.x, blockIdx.x + gridDim.x, ... in turn
+  // (its table fills most of the CU's LDS; a new workgroup per partition would leave the CU idle
+  // between them). The partition's table: keys[lcap] (EMPTY = all ones), then the accumulator
+  // words, lcap + 1 apart (slot lcap is unused: the one key equal to EMPTY goes to the small
+  // global table).
   extern __shared__ uint64_t lds[];
   __shared__ uint32_t used, full, wpos;
   __shared__ unsigned long long base;
-  const int lcap = 1 << lcap_log2, lstride = lcap + 1;
-  lds_table_init(lds, T, lcap_log2);
-  if (threadIdx.x == 0) {
-    used = 0;
-    full = 0;
-    wpos = 0;
-  }
-  __syncthreads();
-  const uint64_t b = starts[blockIdx.x], e = starts[blockIdx.x + 1];
-  const uint32_t limit = (uint32_t)lcap - (uint32_t)lcap / 8;
-  for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
-    if (full) break;
-    const uint64_t key = unmix64(hs[i]);
-    const uint64_t vbits = VAL == 1 ? vs[i] : 0;
-    const uint64_t r = VAL == 2 ? ids[i] : 0;
-    if (key == kEmptyKey) {
-      const int64_t g = gt_find_insert(T, key);
-      if (g < 0) continue;
+  const uint32_t lstride = lcap + 1;
+  const uint32_t limit = lcap - lcap / 8;
+  for (uint32_t part = blockIdx.x; part < nparts; part += gridDim.x) {
+    for (uint32_t s = threadIdx.x; s < lstride; s += blockDim.x) lds[s] = kEmptyKey;
+    for (int a = 0; a < T.naggs; ++a) {
+      const AggSpec &A = T.aggs[a];
+      for (int w = 0; w < agg_words(A.kind); ++w) {
+        const uint64_t v = init_word(A.kind, w);
+        uint64_t *p = lds + (size_t)(1 + A.word0 + w) * lstride;
+        for (uint32_t s = threadIdx.x; s < lstride; s += blockDim.x) p[s] = v;
+      }
+    }
+    if (threadIdx.x == 0) {
+      used = 0;
+      full = 0;
+      wpos = 0;
+    }
+    __syncthreads();
+    const uint64_t b = starts[part], e = starts[part + 1];
+    // kEmitRows rows a thread per step, all their loads issued first: the loads in flight then
+    // cover HBM latency (one row a thread leaves a CU a few KiB in flight)
+    for (uint64_t i0 = b; i0 < e && !full; i0 += (uint64_t)blockDim.x * kEmitRows) {
+      uint64_t hv[kEmitRows], vv[kEmitRows], rv[kEmitRows];
+#pragma unroll
+      for (int q = 0; q < kEmitRows; ++q) {
+        const uint64_t i = i0 + (uint64_t)q * blockDim.x + threadIdx.x;
+        const uint64_t c = i < e ? i : e - 1;  // past the end: the last row again (not used)
+        hv[q] = hs[c];
+        vv[q] = VAL == 1 ? vs[c] : 0;
+        rv[q] = VAL == 2 ? ids[c] : 0;
+      }
+#pragma unroll
+      for (int q = 0; q < kEmitRows; ++q) {
+        if (i0 + (uint64_t)q * blockDim.x + threadIdx.x >= e || full) break;
+        // the table holds h = mix64(key) (a bijection): no unmix per row, and h's low bits give the
+        // slot (its top bits are the partition's, equal for every row here)
+        const uint64_t h = hv[q];
+        const uint64_t vbits = vv[q];
+        const uint64_t r = rv[q];
+        if (h == kEmptyKey) {  // the one key whose hash is the EMPTY pattern: the small global table
+          const int64_t g = gt_find_insert(T, unmix64(h));
+          if (g < 0) continue;
+          for (int a = 0; a < T.naggs; ++a) {
+            const AggSpec &A = T.aggs[a];
+            const AggVal v = VAL == 2 ? agg_value(A, r) : agg_value_bits(A, vbits);
+            acc_apply(A, [&](int j) { return acc_word(T, A.word0 + j, (uint64_t)g); }, v);
+          }
+          continue;
+        }
+        // slot: the low 32 bits of h scaled to [0, lcap), then linear probing
+        uint32_t s = (uint32_t)(((h & 0xffffffffull) * (uint64_t)lcap) >> 32);
+        int slot = -1;
+        for (uint32_t step = 0; step < lcap; ++step) {
+          const uint64_t k = __atomic_load_n(lds + s, __ATOMIC_RELAXED);
+          if (k == h) {
+            slot = (int)s;
+            break;
+          }
+          if (k == kEmptyKey) {
+            const unsigned long long old = atomicCAS((unsigned long long *)(lds + s), kEmptyKey, h);
+            if (old == kEmptyKey) {
+              if (atomicAdd(&used, 1u) + 1u > limit) full = 1;
+              slot = (int)s;
+              break;
+            }
+            if (old == h) {
+              slot = (int)s;
+              break;
+            }
+          }
+          s = s + 1 == lcap ? 0 : s + 1;
+        }
+        if (slot < 0) {
+          full = 1;
+          break;
+        }
+        for (int a = 0; a < T.naggs; ++a) {
+          const AggSpec &A = T.aggs[a];
+          const AggVal v = VAL == 2 ? agg_value(A, r) : agg_value_bits(A, vbits);
+          acc_apply(A, [&](int j) { return lds + (size_t)(1 + A.word0 + j) * lstride + slot; }, v);
+        }
+      }
+    }
+    __syncthreads();
+    if (full) {  // uniform: the whole result is discarded (the direct form runs instead)
+      if (threadIdx.x == 0) atomicOr(fail, 1ull);
+      return;
+    }
+    if (threadIdx.x == 0) base = atomicAdd(out_count, (unsigned long long)used);
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < lcap; s += blockDim.x) {
+      const uint64_t h = lds[s];
+      if (h == kEmptyKey) continue;
+      const uint64_t idx = base + atomicAdd(&wpos, 1u);
+      if (idx >= cap) continue;
+      emit_key(K, O, unmix64(h), idx);
       for (int a = 0; a < T.naggs; ++a) {
         const AggSpec &A = T.aggs[a];
-        const AggVal v = VAL == 2 ? agg_value(A, r) : agg_value_bits(A, vbits);
-        acc_apply(A, [&](int j) { return acc_word(T, A.word0 + j, (uint64_t)g); }, v);
+        finalize_words(A, [&](int j) { return lds[(size_t)(1 + A.word0 + j) * lstride + s]; }, O.col[K.nkeys + a], idx);
       }
-      continue;
     }
-    uint32_t s = (uint32_t)hash_u64(key, lcap_log2);
-    int slot = -1;
-    for (int step = 0; step < lcap; ++step) {
-      const uint64_t k = __atomic_load_n(lds + s, __ATOMIC_RELAXED);
-      if (k == key) {
-        slot = (int)s;
-        break;
-      }
-      if (k == kEmptyKey) {
-        const unsigned long long old = atomicCAS((unsigned long long *)(lds + s), kEmptyKey, key);
-        if (old == kEmptyKey) {
-          if (atomicAdd(&used, 1u) + 1u > limit) full = 1;
-          slot = (int)s;
-          break;
-        }
-        if (old == key) {
-          slot = (int)s;
-          break;
-        }
-      }
-      s = (s + 1) & (uint32_t)(lcap - 1);
-    }
-    if (slot < 0 || full) {
-      full = 1;
-      break;
-    }
-    for (int a = 0; a < T.naggs; ++a) {
-      const AggSpec &A = T.aggs[a];
-      const AggVal v = VAL == 2 ? agg_value(A, r) : agg_value_bits(A, vbits);
-      acc_apply(A, [&](int j) { return lds + (size_t)(1 + A.word0 + j) * lstride + slot; }, v);
-    }
-  }
-  __syncthreads();
-  if (full) {
-    if (threadIdx.x == 0) atomicOr(fail, 1ull);
-    return;
-  }
-  if (threadIdx.x == 0) base = atomicAdd(out_count, (unsigned long long)used);
-  __syncthreads();
-  for (int s = threadIdx.x; s < lcap; s += blockDim.x) {
-    const uint64_t k = lds[s];
-    if (k == kEmptyKey) continue;
-    const uint64_t idx = base + atomicAdd(&wpos, 1u);
-    if (idx >= cap) continue;
-    emit_key(K, O, k, idx);
-    for (int a = 0; a < T.naggs; ++a) {
-      const AggSpec &A = T.aggs[a];
-      finalize_words(A, [&](int j) { return lds[(size_t)(1 + A.word0 + j) * lstride + s]; }, O.col[K.nkeys + a], idx);
-    }
+    __syncthreads();  // the table is reset for the next partition only after every slot was read
   }
 }
 
@@ -667,12 +699,254 @@ int group_table_init(ghive_ctx *ctx, GroupTableDev &T, uint64_t *keys, uint64_t 
   return GHIVE_OK;
 }
 
+// ---- partitioning of the high-cardinality form ---------------------------------------------------
+// Stable LSD passes over the partition bits of h = mix64(key) (its top pbits bits), DB = 8 or 9
+// bits a pass, carrying the aggregated value's bits beside h (VAL). The first pass reads the key
+// and value columns and computes h itself, so no separate hashing pass writes (h, value) first.
+// Per pass: a per-tile digit histogram, one scan of the digit x tile matrix, then the scatter: every
+// element ranked among the equal digits of its wave with ballots (stable), the tile staged through
+// LDS in digit order and each digit's run written to consecutive addresses. A 9-bit pass uses
+// 4096-element tiles (52 KB of LDS, three workgroups per CU), an 8-bit pass 8192.
+template <int DB>
+struct Gbp {
+  static constexpr int kBins = 1 << DB;
+  static constexpr int kPer = DB == 9 ? 16 : 32;
+  static constexpr uint32_t kTile = 256u * kPer;
+  typedef typename std::conditional<DB == 9, uint16_t, uint8_t>::type Digit;
+};
+
+struct GbpSrc {
+  KeySpec K;              // the first pass: key columns,
+  const uint32_t *sel;    // the rows (NULL: all),
+  const void *vdata;      // the value column
+  int vtype;
+  const uint64_t *h;      // later passes: the previous pass's output
+  const uint64_t *v;
+};
+
+// Where a pass reads: kSrcPass the previous pass's (h, v); kSrcCols the key and value columns at the
+// selected rows (h computed); kSrcCol64 one 8-byte key column and an 8-byte value column over all
+// rows, read as plain words so that every load of a tile issues before any hash is computed.
+enum GbpSrcKind { kSrcPass = 0, kSrcCols = 1, kSrcCol64 = 2 };
+
+template <int SRC>
+__device__ __forceinline__ uint64_t gbp_hash(const GbpSrc &S, uint64_t e) {
+  if (SRC == kSrcPass) return S.h[e];
+  if (SRC == kSrcCol64) return mix64(static_cast<const uint64_t *>(S.K.data[0])[e]);
+  return mix64(pack_key(S.K, S.sel ? S.sel[e] : e));
+}
+
+template <int DB, int SRC>
+__global__ __launch_bounds__(256) void k_gbp_hist(GbpSrc S, uint64_t n, int shift, uint32_t *hist, uint32_t *counts,
+                                                  uint32_t nblocks) {
+  constexpr int B = Gbp<DB>::kBins;
+  __shared__ uint32_t hc[B];
+  for (int b = threadIdx.x; b < B; b += 256) hc[b] = 0;
+  __syncthreads();
+  const uint64_t t0 = (uint64_t)blockIdx.x * Gbp<DB>::kTile;
+#pragma unroll 4
+  for (int j = 0; j < Gbp<DB>::kPer; ++j) {
+    const uint64_t e = t0 + (uint64_t)j * 256 + threadIdx.x;
+    if (e < n) atomicAdd(&hc[(gbp_hash<SRC>(S, e) >> shift) & (B - 1)], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < B; b += 256) {
+    hist[(uint64_t)b * nblocks + blockIdx.x] = hc[b];
+    counts[(uint64_t)blockIdx.x * B + b] = hc[b];
+  }
+}
+
+template <int DB, int SRC, bool VAL>
+__global__ __launch_bounds__(256) void k_gbp_scatter(GbpSrc S, uint64_t n, int shift, const uint32_t *offs,
+                                                     const uint32_t *counts, uint32_t nblocks, uint64_t *h_out,
+                                                     uint64_t *v_out) {
+  constexpr int B = Gbp<DB>::kBins, kPer = Gbp<DB>::kPer, kWaves = 4, kBpt = B / 256;  // bins per thread
+  constexpr uint32_t kTile = Gbp<DB>::kTile, kSeg = (uint32_t)kPer * kWave;
+  typedef typename Gbp<DB>::Digit Digit;
+  __shared__ uint64_t stage[kTile];
+  __shared__ Digit sdigit[kTile];
+  __shared__ uint32_t lbase[B], goff[B];
+  __shared__ uint32_t wcnt[kWaves][B];
+  __shared__ uint32_t scan_tmp[kWaves];
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
+  const uint64_t t0 = (uint64_t)blockIdx.x * kTile;
+  const uint32_t tn = (uint32_t)(n - t0 < kTile ? n - t0 : kTile);
+  {  // this tile's first position of every digit, and the digit's global offset
+    uint32_t c[kBpt], s = 0, total;
+#pragma unroll
+    for (int q = 0; q < kBpt; ++q) {
+      c[q] = counts[(uint64_t)blockIdx.x * B + tid * kBpt + q];
+      s += c[q];
+    }
+    uint32_t lb = block_exclusive_scan_256(s, scan_tmp, total);
+#pragma unroll
+    for (int q = 0; q < kBpt; ++q) {
+      lbase[tid * kBpt + q] = lb;
+      lb += c[q];
+      goff[tid * kBpt + q] = offs[(uint64_t)(tid * kBpt + q) * nblocks + blockIdx.x];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < B / kWave; ++q) wcnt[wave][lane + q * kWave] = 0;
+  // every load of the tile first: element wave * kSeg + j * 64 + lane; past the end a lane
+  // re-reads the last element (ignored)
+  uint64_t hv[kPer], vv[kPer];
+  uint32_t lpos[kPer], dg[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const uint32_t el = wave * kSeg + (uint32_t)j * kWave + lane;
+    const uint64_t e = t0 + (el < tn ? el : tn - 1);
+    if (SRC == kSrcCols) {
+      const uint64_t r = S.sel ? S.sel[e] : e;
+      hv[j] = mix64(pack_key(S.K, r));
+      vv[j] = VAL ? col_bits(S.vdata, S.vtype, r) : 0;
+    } else if (SRC == kSrcCol64) {  // raw key words now, hashed below once every load is in flight
+      hv[j] = static_cast<const uint64_t *>(S.K.data[0])[e];
+      vv[j] = VAL ? static_cast<const uint64_t *>(S.vdata)[e] : 0;
+    } else {
+      hv[j] = S.h[e];
+      vv[j] = VAL ? S.v[e] : 0;
+    }
+  }
+  if (SRC == kSrcCol64) {
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) hv[j] = mix64(hv[j]);
+  }
+  __syncthreads();
+  const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const bool valid = wave * kSeg + (uint32_t)j * kWave + lane < tn;
+    const uint32_t digit = (uint32_t)(hv[j] >> shift) & (uint32_t)(B - 1);
+    uint64_t peers = ballot(valid);
+#pragma unroll
+    for (int b = 0; b < DB; ++b) {
+      const uint64_t bb = ballot((digit >> b) & 1u);
+      peers &= ((digit >> b) & 1u) ? bb : ~bb;
+    }
+    const uint32_t before = wcnt[wave][digit];
+    const uint32_t rank = (uint32_t)__popcll(peers & lt);
+    __builtin_amdgcn_wave_barrier();
+    if (valid && rank == 0) wcnt[wave][digit] = before + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+    lpos[j] = before + rank;
+    dg[j] = digit;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kBpt; ++q) {  // exclusive prefix of each digit's counts over the waves
+    const uint32_t d = tid * kBpt + q;
+    uint32_t acc = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      const uint32_t c = wcnt[w][d];
+      wcnt[w][d] = acc;
+      acc += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    if (wave * kSeg + (uint32_t)j * kWave + lane < tn) {
+      const uint32_t pos = lbase[dg[j]] + wcnt[wave][dg[j]] + lpos[j];
+      lpos[j] = pos;
+      sdigit[pos] = (Digit)dg[j];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < (VAL ? 2 : 1); ++q) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kPer; ++j)
+      if (wave * kSeg + (uint32_t)j * kWave + lane < tn) stage[lpos[j]] = q ? vv[j] : hv[j];
+    __syncthreads();
+    uint64_t *out = q ? v_out : h_out;
+    for (uint32_t p = tid; p < tn; p += 256) {
+      const uint32_t d = sdigit[p];
+      out[(uint64_t)goff[d] + (p - lbase[d])] = stage[p];
+    }
+  }
+}
+
+// The stable passes over the top pbits bits of h, low digits first (at most 9 bits a pass, the
+// first pass as wide as the rest allow). h/v hold n words each; the partitioned (h, v) end in the
+// buffers *h_res / *v_res.
+static int gbp_partition(ghive_ctx *ctx, const GbpSrc &src, uint64_t n, int pbits, bool val, uint64_t *h0,
+                         uint64_t *v0, uint64_t *h1, uint64_t *v1, uint64_t **h_res, uint64_t **v_res) {
+  std::vector<int> widths;  // digit widths, low digit first (one 8-bit pass at least)
+  for (int left = pbits < 8 ? 8 : pbits; left > 0;) {
+    const int passes = (left + 8) / 9;
+    const int w = (left + passes - 1) / passes;
+    widths.push_back(w > 9 ? 9 : w);
+    left -= widths.back();
+  }
+  const uint32_t nb_max = ceil_div(n, Gbp<9>::kTile);  // the smaller tile
+  DevBuf hist, counts;
+  GH_TRY(hist.alloc(ctx, sizeof(uint32_t) * 512ull * nb_max));
+  GH_TRY(counts.alloc(ctx, sizeof(uint32_t) * 512ull * nb_max));
+  GbpSrc S = src;
+  // one LONG key column (a DOUBLE key's NaNs need canonical bits), an 8-byte value column (or none),
+  // every row: the plain-word first pass
+  const bool fast = !src.sel && src.K.nkeys == 1 && src.K.shift[0] == 0 && src.K.type[0] == GHIVE_LONG &&
+                    (!val || src.vtype == GHIVE_LONG || src.vtype == GHIVE_DOUBLE);
+  uint64_t *hin = nullptr, *vin = nullptr, *hout = h0, *vout = v0;
+  int lo = 64 - (pbits < 8 ? 8 : pbits);
+  for (size_t p = 0; p < widths.size(); ++p) {
+    const bool first = p == 0;
+    const int db = widths[p] > 8 ? 9 : 8;  // a narrower digit runs as an 8-bit one over its bits
+    const int shift = lo;
+    // a digit narrower than DB bits must not reach past bit 63: shift so that it ends at 63 when it
+    // is the last one, else take DB bits starting at lo (the extra high bits are the next digit's,
+    // which the later, stable passes order anyway)
+    const int sh = shift + db > 64 ? 64 - db : shift;
+    const uint32_t tile = db == 9 ? Gbp<9>::kTile : Gbp<8>::kTile;
+    const uint32_t nb = ceil_div(n, tile);
+    const uint32_t bins = db == 9 ? 512 : 256;
+    if (!first) {
+      S.h = hin;
+      S.v = vin;
+    }
+    const int src = !first ? kSrcPass : (fast ? kSrcCol64 : kSrcCols);
+#define GBP_HIST(D, F) k_gbp_hist<D, F><<<nb, 256, 0, ctx->stream>>>(S, n, sh, hist.as<uint32_t>(), counts.as<uint32_t>(), nb)
+    if (db == 9) {
+      if (src == kSrcCols) GBP_HIST(9, kSrcCols); else if (src == kSrcCol64) GBP_HIST(9, kSrcCol64); else GBP_HIST(9, kSrcPass);
+    } else {
+      if (src == kSrcCols) GBP_HIST(8, kSrcCols); else if (src == kSrcCol64) GBP_HIST(8, kSrcCol64); else GBP_HIST(8, kSrcPass);
+    }
+#undef GBP_HIST
+    GH_LAUNCH(ctx);
+    GH_TRY(exclusive_scan_u32(ctx, hist.as<uint32_t>(), hist.as<uint32_t>(), (uint64_t)bins * nb, nullptr));
+#define GBP_SCATTER(D, F, V)                                                                                   \
+  k_gbp_scatter<D, F, V><<<nb, 256, 0, ctx->stream>>>(S, n, sh, hist.as<uint32_t>(), counts.as<uint32_t>(), nb, \
+                                                      hout, vout)
+    if (db == 9) {
+      if (src == kSrcCols) { if (val) GBP_SCATTER(9, kSrcCols, true); else GBP_SCATTER(9, kSrcCols, false); }
+      else if (src == kSrcCol64) { if (val) GBP_SCATTER(9, kSrcCol64, true); else GBP_SCATTER(9, kSrcCol64, false); }
+      else { if (val) GBP_SCATTER(9, kSrcPass, true); else GBP_SCATTER(9, kSrcPass, false); }
+    } else {
+      if (src == kSrcCols) { if (val) GBP_SCATTER(8, kSrcCols, true); else GBP_SCATTER(8, kSrcCols, false); }
+      else if (src == kSrcCol64) { if (val) GBP_SCATTER(8, kSrcCol64, true); else GBP_SCATTER(8, kSrcCol64, false); }
+      else { if (val) GBP_SCATTER(8, kSrcPass, true); else GBP_SCATTER(8, kSrcPass, false); }
+    }
+#undef GBP_SCATTER
+    GH_LAUNCH(ctx);
+    lo += widths[p];
+    hin = hout;
+    vin = vout;
+    hout = hin == h0 ? h1 : h0;
+    vout = vin == v0 ? v1 : v0;
+  }
+  *h_res = hin;
+  *v_res = vin;
+  return GHIVE_OK;
+}
+
 // Runs the high-cardinality form: 2^pbits hash partitions, each aggregated in an LDS table of 2^lg
 // slots and written straight to the output columns O (rows in unspecified order). T carries the
 // aggregate specs. *failed: a partition held more groups than its LDS table (nothing usable was
 // written; use the direct form). *groups = groups found (CAPACITY when above capacity).
 int groupby_partitioned(ghive_ctx *ctx, const KeySpec &K, const GroupTableDev &Tspec, const uint32_t *sel,
-                        uint64_t n, int lg, int pbits, const OutSpec &O, uint64_t capacity, bool *failed,
+                        uint64_t n, uint32_t lcap, int pbits, const OutSpec &O, uint64_t capacity, bool *failed,
                         uint64_t *groups) {
   *failed = false;
   *groups = 0;
@@ -700,27 +974,33 @@ int groupby_partitioned(ghive_ctx *ctx, const KeySpec &K, const GroupTableDev &T
   DevBuf k0, k1, i0, i1, starts, sk, sa, sc;
   GH_TRY(k0.alloc(ctx, 8ull * KW * n));
   GH_TRY(k1.alloc(ctx, 8ull * KW * n));
-  GH_TRY(i0.alloc(ctx, 4 * n));
-  GH_TRY(i1.alloc(ctx, 4 * n));
-  GH_TRY(starts.alloc(ctx, 8 * ((1ull << pbits) + 1)));
-  {
-    const uint64_t g = (n + 255) / 256, cap = (uint64_t)ctx->cus * 16;
-    k_gb_part_keys<<<(uint32_t)(g > cap ? cap : g), 256, 0, ctx->stream>>>(
-        K, sel, n, k0.as<uint64_t>(), val == 2 ? i0.as<uint32_t>() : nullptr, vdata, vtype,
-        val == 1 ? k0.as<uint64_t>() + n : nullptr);
-    GH_LAUNCH(ctx);
+  if (val == 2) {
+    GH_TRY(i0.alloc(ctx, 4 * n));
+    GH_TRY(i1.alloc(ctx, 4 * n));
   }
-  std::vector<int> digits;  // the 8-bit digits of the hash (word 0) that cover its top pbits bits
-  for (int d = pbits ? (64 - pbits) / 8 : 8; d < 8; ++d) digits.push_back(d);
-  uint64_t *ks = k0.as<uint64_t>();
+  GH_TRY(starts.alloc(ctx, 8 * ((1ull << pbits) + 1)));
+  uint64_t *ks = nullptr, *vs_part = nullptr;
   uint32_t *ids = i0.as<uint32_t>();
-  const bool need_ids = val == 2;  // the row ids travel only when the aggregation reads columns at them
-  if (KW == 2)
-    GH_TRY(radix_sort_u64x2_ids(ctx, n, k0.as<uint64_t>(), k1.as<uint64_t>(), i0.as<uint32_t>(), i1.as<uint32_t>(),
-                                digits, &ks, &ids, need_ids));
-  else
+  if (val != 2) {  // hashed inside the first partition pass; the value word travels beside the hash
+    GbpSrc S;
+    memset(&S, 0, sizeof S);
+    S.K = K;
+    S.sel = sel;
+    S.vdata = vdata;
+    S.vtype = vtype;
+    GH_TRY(gbp_partition(ctx, S, n, pbits, val == 1, k0.as<uint64_t>(), val == 1 ? k0.as<uint64_t>() + n : nullptr,
+                         k1.as<uint64_t>(), val == 1 ? k1.as<uint64_t>() + n : nullptr, &ks, &vs_part));
+  } else {  // several value columns: (hash, row id) pairs, the values read at the row id
+    const uint64_t g = (n + 255) / 256, cap = (uint64_t)ctx->cus * 16;
+    k_gb_part_keys<<<(uint32_t)(g > cap ? cap : g), 256, 0, ctx->stream>>>(K, sel, n, k0.as<uint64_t>(),
+                                                                           i0.as<uint32_t>(), nullptr, 0, nullptr);
+    GH_LAUNCH(ctx);
+    std::vector<int> digits;  // the 8-bit digits of the hash that cover its top pbits bits
+    for (int d = pbits ? (64 - pbits) / 8 : 8; d < 8; ++d) digits.push_back(d);
+    ks = k0.as<uint64_t>();
     GH_TRY(radix_sort_u64_ids(ctx, n, k0.as<uint64_t>(), k1.as<uint64_t>(), i0.as<uint32_t>(), i1.as<uint32_t>(),
-                              digits, &ks, &ids, need_ids));
+                              digits, &ks, &ids, true));
+  }
   {
     const uint64_t g = (n + 255) / 256, cap = (uint64_t)ctx->cus * 16;
     k_gb_part_bounds<<<(uint32_t)(g > cap ? cap : g), 256, 0, ctx->stream>>>(ks, n, pbits, starts.as<uint64_t>());
@@ -729,30 +1009,31 @@ int groupby_partitioned(ghive_ctx *ctx, const KeySpec &K, const GroupTableDev &T
   // the small global table that takes the one key equal to the EMPTY pattern
   GroupTableDev T = Tspec;
   GH_TRY(group_table_alloc(ctx, T, 1, sk, sa, sc));
-  const size_t lds_bytes = (size_t)((1 << lg) + 1) * (1 + T.nwords) * 8;
+  const size_t lds_bytes = (size_t)(lcap + 1) * (1 + T.nwords) * 8;
   unsigned long long *fail = reinterpret_cast<unsigned long long *>(ctx->d_counters);
   unsigned long long *count = fail + 1;
   GH_TRY(fill_u64(ctx, ctx->d_counters, 0, 2));
   const uint32_t P = (uint32_t)(1ull << pbits);
-  const uint64_t *vs = KW == 2 ? ks + n : nullptr;
+  const uint32_t egrid = std::min<uint32_t>(P, (uint32_t)ctx->cus);  // one resident workgroup per CU
+  const uint64_t *vs = vs_part;
   switch (val) {
     case 0:
       (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_gb_part_emit<0>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-      k_gb_part_emit<0><<<P, kGbPartThreads, lds_bytes, ctx->stream>>>(K, T, starts.as<uint64_t>(), ks, vs, ids, lg,
-                                                                       fail, count, O, capacity);
+      k_gb_part_emit<0><<<egrid, kGbPartThreads, lds_bytes, ctx->stream>>>(K, T, starts.as<uint64_t>(), ks, vs, ids, lcap, P,
+                                                                           fail, count, O, capacity);
       break;
     case 1:
       (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_gb_part_emit<1>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-      k_gb_part_emit<1><<<P, kGbPartThreads, lds_bytes, ctx->stream>>>(K, T, starts.as<uint64_t>(), ks, vs, ids, lg,
-                                                                       fail, count, O, capacity);
+      k_gb_part_emit<1><<<egrid, kGbPartThreads, lds_bytes, ctx->stream>>>(K, T, starts.as<uint64_t>(), ks, vs, ids, lcap, P,
+                                                                           fail, count, O, capacity);
       break;
     default:
       (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_gb_part_emit<2>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-      k_gb_part_emit<2><<<P, kGbPartThreads, lds_bytes, ctx->stream>>>(K, T, starts.as<uint64_t>(), ks, vs, ids, lg,
-                                                                       fail, count, O, capacity);
+      k_gb_part_emit<2><<<egrid, kGbPartThreads, lds_bytes, ctx->stream>>>(K, T, starts.as<uint64_t>(), ks, vs, ids, lcap, P,
+                                                                           fail, count, O, capacity);
       break;
   }
   GH_LAUNCH(ctx);
@@ -951,12 +1232,15 @@ int ghive_groupby(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel,
   int st = GHIVE_OK;
   if (nkeys > 0 && n && part_opt != 0 &&
       (part_opt == 2 || (n >= (1ull << 22) && capacity >= (1ull << 20)))) {
-    int lg = 11;  // LDS table slots per partition
-    while (lg > 8 && (size_t)((1 << lg) + 1) * (1 + T.nwords) * 8 > 150 * 1024) --lg;
-    int pbits = 0;  // partitions so that the group bound leaves each table at most half full
-    while (pbits < 24 && (capacity << 1) > (1ull << (pbits + lg))) ++pbits;
+    // LDS table slots per partition: a multiple of 512 that fits 150 KiB; partitions so that the
+    // group bound fills each table to at most 60% on average (the emit falls back to the direct form
+    // if a partition ever exceeds 7/8)
+    uint32_t lcap = 4096;
+    while (lcap > 512 && (size_t)(lcap + 1) * (1 + T.nwords) * 8 > 150 * 1024) lcap -= 512;
+    int pbits = 0;
+    while (pbits < 24 && (double)capacity > 0.6 * (double)lcap * (double)(1ull << pbits)) ++pbits;
     bool failed = false;
-    st = groupby_partitioned(ctx, K, T, sel, n, lg, pbits, O, capacity, &failed, out_groups);
+    st = groupby_partitioned(ctx, K, T, sel, n, lcap, pbits, O, capacity, &failed, out_groups);
     if (st != GHIVE_OK && st != GHIVE_ERR_CAPACITY) return st;
     partitioned = !failed;
   }
