@@ -122,7 +122,9 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t *key
 // across waves and iterations. (2) The tile is rewritten through a 32 KiB LDS buffer, one key
 // word (then the row ids) at a time, in digit order, so each digit's run leaves as consecutive
 // addresses (coalesced stores) at its global offset.
-template <int KW, bool IDS>
+// DROP0: the last pass over word 0, whose low 32 bits hold the row id: the later passes need only
+// the id, so word 0 leaves as a uint32 id array (idx_out) and words 1.. as a (KW - 1)-word array.
+template <int KW, bool IDS, bool DROP0 = false>
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *keys_in, const uint32_t *idx_in,
                                                                 uint64_t *keys_out, uint32_t *idx_out, uint64_t n,
                                                                 int word, int shift, const uint32_t *offs,
@@ -210,8 +212,10 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *
     for (uint32_t p = tid; p < tn; p += kSortThreads) {
       const uint32_t d = sdigit[p];
       const uint64_t g = (uint64_t)goff[d] + (p - lbase[d]);
-      if (q < KW)
-        keys_out[(uint64_t)q * n + g] = stage[p];
+      if (DROP0 && q == 0)
+        idx_out[g] = (uint32_t)stage[p];
+      else if (q < KW)
+        keys_out[(uint64_t)(DROP0 ? q - 1 : q) * n + g] = stage[p];
       else
         idx_out[g] = (uint32_t)stage[p];
     }
@@ -219,10 +223,12 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *
 }
 
 // ids: carry the uint32 id arrays i0/i1 (false: the ids live inside the keys; i0/i1 unused).
+// drop0_last: the last pass writes word 0 as the uint32 id array (k_radix_scatter DROP0; needs
+// !ids, KW >= 2): its outputs are then *result (ids) and *keys_result (KW - 1 words).
 template <int KW>
 static int radix_passes(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, uint32_t *i0, uint32_t *i1,
                         const std::vector<int> &digits, uint32_t **result, uint64_t **keys_result = nullptr,
-                        bool ids = true) {
+                        bool ids = true, bool drop0_last = false) {
   const uint32_t nblocks = ceil_div(n, kSortTile);
   DevBuf hist, counts;
   GH_TRY(hist.alloc(ctx, sizeof(uint32_t) * 256ull * nblocks));
@@ -235,7 +241,10 @@ static int radix_passes(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, 
                                                             counts.as<uint32_t>(), nblocks);
     GH_LAUNCH(ctx);
     GH_TRY(exclusive_scan_u32(ctx, hist.as<uint32_t>(), hist.as<uint32_t>(), 256ull * nblocks, nullptr));
-    if (ids)
+    if (drop0_last && KW >= 2 && d == digits.back())
+      k_radix_scatter<KW, false, true><<<nblocks, kSortThreads, 0, ctx->stream>>>(
+          kin, iin, kout, iout, n, word, shift, hist.as<uint32_t>(), counts.as<uint32_t>(), nblocks);
+    else if (ids)
       k_radix_scatter<KW, true><<<nblocks, kSortThreads, 0, ctx->stream>>>(kin, iin, kout, iout, n, word, shift,
                                                                          hist.as<uint32_t>(), counts.as<uint32_t>(),
                                                                          nblocks);
@@ -483,6 +492,33 @@ int ghive_sort(ghive_ctx *ctx, const ghive_table *table, const uint32_t *key_col
   uint32_t *res = out_perm;
   uint64_t *K0 = k0.as<uint64_t>(), *K1 = k1.as<uint64_t>(), *kres = K0;
   uint32_t *I1 = i1.as<uint32_t>();
+  // With the row id inside word 0 and more words above it, word 0 is needed only until its last
+  // digit: that pass writes the ids out as their own array, and the passes over the higher words
+  // move (KW - 1) key words plus a 4-byte id instead of KW words.
+  std::vector<int> d0, dhi;
+  for (int d : digits) (d < 8 ? d0 : dhi).push_back(d);
+  if (embed && KW >= 2 && !d0.empty() && !dhi.empty()) {
+    uint32_t *ids = nullptr;
+    uint64_t *keys = nullptr;
+    switch (KW) {
+      case 2: GH_TRY(radix_passes<2>(ctx, n, K0, K1, out_perm, I1, d0, &ids, &keys, false, true)); break;
+      case 3: GH_TRY(radix_passes<3>(ctx, n, K0, K1, out_perm, I1, d0, &ids, &keys, false, true)); break;
+      default: GH_TRY(radix_passes<4>(ctx, n, K0, K1, out_perm, I1, d0, &ids, &keys, false, true)); break;
+    }
+    // the higher words now live at `keys` (KW - 1 words) with the ids beside them; the free key
+    // buffer and the other id buffer take the next ping-pong
+    uint64_t *other = keys == K0 ? K1 : K0;
+    uint32_t *iother = ids == out_perm ? I1 : out_perm;
+    for (int &d : dhi) d -= 8;
+    switch (KW) {
+      case 2: GH_TRY(radix_passes<1>(ctx, n, keys, other, ids, iother, dhi, &res, &kres, true)); break;
+      case 3: GH_TRY(radix_passes<2>(ctx, n, keys, other, ids, iother, dhi, &res, &kres, true)); break;
+      default: GH_TRY(radix_passes<3>(ctx, n, keys, other, ids, iother, dhi, &res, &kres, true)); break;
+    }
+    if (res != out_perm) GH_HIP(ctx, hipMemcpyAsync(out_perm, res, 4ull * n, hipMemcpyDeviceToDevice, ctx->stream));
+    GH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GHIVE_OK;
+  }
   switch (KW) {
     case 1: GH_TRY(radix_passes<1>(ctx, n, K0, K1, out_perm, I1, digits, &res, &kres, !embed)); break;
     case 2: GH_TRY(radix_passes<2>(ctx, n, K0, K1, out_perm, I1, digits, &res, &kres, !embed)); break;
