@@ -100,3 +100,35 @@ def test_batch_rejects_bad_descriptors(ctx):
     x = torch.zeros(9, dtype=torch.int32, device="cuda")
     with pytest.raises(GhiveError):  # misaligned column
         ops.Batch(ctx, ht, [(x[1:], x[1:], None, 0, 0)], 8, 3, 0, 5)
+
+
+def test_full_4096_fragment_batch(ctx, orc):
+    """The batched workload at full size: 4096 fragments with lognormal row counts (median 256k,
+    clipped to 1k..16M, ~2.4e9 rows) in one persistent kernel replayed from a HIP graph, against the
+    4096 plain per-fragment launches over the same columns, fragment by fragment, and the oracle on
+    the smallest fragments and a few random ones."""
+    import torch
+    from ghive_amd import workloads as W
+    B = W.batch_tables(ctx, nfrags=4096)
+    assert len(B.rows) == 4096 and sum(B.rows) > 2_000_000_000
+    ht = W.batch_dim_table(ctx, B)
+    bt = W.batch_create(ctx, B, ht)
+    bt.run()
+    got = _batch_dicts(*bt.results())
+    bt.free()
+    plain = W.PlainBatch(ctx, B).run(collect=True)
+    assert got == plain
+    dim = [to_np(c) for c in B.dim.columns[:3]]
+    order = np.argsort(B.rows)
+    sample = list(order[:6]) + list(np.random.default_rng(1).choice(4096, 4, replace=False))
+    for f in sample:
+        s, r, c = B.starts[f], B.rows[f], B.attr_const[f]
+        fact = [to_np(t[s:s + r]) for t in (B.fk, B.measure, B.price)]
+        spec = orc.FragmentSpec(fact, [dict(columns=dim, key_col=0, payload_col=2, fact_key_col=0,
+                                            filter=([abi.make_pred(abi.FILTER_EQ, 1, [c], data_type=2)], 0))],
+                                keys=[(abi.SRC_JOIN, 0)], aggs=[(abi.AGG_SUM, abi.EXPR_MUL, 1, 2)])
+        res = spec.run(nthreads=4)
+        assert got[f] == {int(k[0]): int(v[0]) for k, v in zip(res["keys"], res["aggs"])}, f
+    ht.free()
+    del B
+    torch.cuda.empty_cache()
