@@ -31,10 +31,51 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(T, sample_rows, threads):
-    """Hive-restatement baseline (oracle) on the first sample_rows fact rows; median of 3."""
+def host_cores():
+    """The host threads the CPU baselines use: the cores this process may run on (its affinity
+    mask), capped by a cgroup CPU quota and by OMP_NUM_THREADS where the environment sets the
+    process's CPU share that way; with the CPU model name."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as fh:
+            q, per = fh.read().split()[:2]
+        if q != "max":
+            quota = max(1, -(-int(q) // int(per)))
+    except (OSError, ValueError):
+        pass
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    share = int(env) if env.isdigit() and int(env) > 0 else None
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"threads": min(x for x in (aff, quota, share) if x), "affinity_cpus": aff,
+            "cgroup_quota_cpus": quota, "omp_num_threads": share, "model": model}
+
+
+def median_of(fn, runs=5):
+    """One warm-up call, then the median wall time of `runs` calls (seconds)."""
+    fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def cpu_baseline(T, sample_rows, threads, one_thread_rows):
+    """Hive-restatement baseline (oracle) on the first fact rows: T map tasks over contiguous
+    slices (each with its own dimension tables and partial groups) + the mergepartial step, and
+    the same pipeline on one thread over a smaller sample. Median of 5 after a warm-up."""
     import oracle
-    from ghive_amd import abi
+    from ghive_amd import abi, workloads as W
     fact = [c[:sample_rows].cpu().numpy() for c in T.fact.columns]
     d, s, p = ([c.cpu().numpy() for c in t.columns] for t in (T.date, T.supplier, T.part))
     dims = [
@@ -45,24 +86,44 @@ def cpu_baseline(T, sample_rows, threads):
         dict(columns=p, key_col=0, payload_col=2, fact_key_col=3,
              filter=([abi.make_pred(abi.FILTER_EQ, 1, [1], data_type=2)], 0)),
     ]
-    spec = oracle.FragmentSpec(fact, dims, keys=[(abi.SRC_JOIN, 0), (abi.SRC_JOIN, 2)],
-                               aggs=[(abi.AGG_SUM, abi.EXPR_MUL, 5, 6)])
-    spec.run(nthreads=threads)  # warm-up
-    secs = sorted(spec.run(nthreads=threads)["seconds"] for _ in range(3))
-    return sample_rows / secs[1], secs[1]
+
+    def spec(rows):
+        return oracle.FragmentSpec([c[:rows] for c in fact], dims, keys=[(abi.SRC_JOIN, 0), (abi.SRC_JOIN, 2)],
+                                   aggs=[(abi.AGG_SUM, abi.EXPR_MUL, 5, 6)])
+
+    def med(sp, nt):
+        sp.run(nthreads=nt)  # warm-up
+        return float(np.median([sp.run(nthreads=nt)["seconds"] for _ in range(5)]))
+
+    secs = med(spec(sample_rows), threads)
+    secs1 = med(spec(one_thread_rows), 1)
+    rate = sample_rows / secs
+    return {"value": rate, "unit": "fact rows/s", "cores": threads, "kind": "port",
+            "gbs": rate * W.STAR_REFERENCED_BYTES_PER_ROW / 1e9, "seconds": secs,
+            "sample": f"first {sample_rows} fact rows of the same tables; C restatement of Hive 3.1 vectorized "
+                      f"map-join + hash group-by as {threads} map tasks on contiguous slices, each with its own "
+                      f"dimension hash tables and partial groups, + the mergepartial step; median of 5 after a warm-up",
+            "one_thread": {"value": one_thread_rows / secs1, "seconds": secs1,
+                           "sample": f"first {one_thread_rows} fact rows, one map task"}}
 
 
-def cpu_batch_baseline(B, threads, max_rows=200_000_000):
-    """Per-fragment CPU restatement (oracle FragmentSpec, one thread per fragment) over a pool of
-    `threads` workers, on the first fragments of the batch up to max_rows fact rows."""
+def cpu_batch_baseline(B, threads, max_rows=200_000_000, one_thread_rows=20_000_000):
+    """Per-fragment CPU restatement (oracle FragmentSpec, one thread per fragment = one map task)
+    over a pool of `threads` workers, on the first fragments of the batch up to max_rows fact rows;
+    and the first fragments up to one_thread_rows on one thread. Median of 5 after a warm-up."""
     import concurrent.futures as cf
     import oracle
-    from ghive_amd import abi
+    from ghive_amd import abi, workloads as W
     dim = [c.cpu().numpy() for c in B.dim.columns[:3]]
-    take, total = 0, 0
-    while take < len(B.rows) and total + B.rows[take] <= max_rows:
-        total += B.rows[take]
-        take += 1
+
+    def first(limit):
+        take, total = 0, 0
+        while take < len(B.rows) and total + B.rows[take] <= limit:
+            total += B.rows[take]
+            take += 1
+        return take, total
+
+    take, total = first(max_rows)
     end = B.starts[take - 1] + B.rows[take - 1] if take else 0
     fk, me, pr = (c[:end].cpu().numpy() for c in (B.fk, B.measure, B.price))
     specs = []
@@ -73,13 +134,18 @@ def cpu_batch_baseline(B, threads, max_rows=200_000_000):
             [dict(columns=dim, key_col=0, payload_col=2, fact_key_col=0,
                   filter=([abi.make_pred(abi.FILTER_EQ, 1, [c], data_type=2)], 0))],
             keys=[(abi.SRC_JOIN, 0)], aggs=[(abi.AGG_SUM, abi.EXPR_MUL, 1, 2)]))
-    t0 = time.perf_counter()
     with cf.ThreadPoolExecutor(threads) as ex:
-        list(ex.map(lambda sp: sp.run(nthreads=1), specs))
-    secs = time.perf_counter() - t0
-    return {"value": total / secs, "unit": "fact rows/s", "cores": threads, "kind": "port",
+        secs = median_of(lambda: list(ex.map(lambda sp: sp.run(nthreads=1), specs)))
+    take1, total1 = first(one_thread_rows)
+    secs1 = median_of(lambda: [sp.run(nthreads=1) for sp in specs[:take1]])
+    rate = total / secs
+    return {"value": rate, "unit": "fact rows/s", "cores": threads, "kind": "port",
+            "gbs": rate * W.SMALL_REFERENCED_BYTES_PER_ROW / 1e9,
             "sample": f"first {take} fragments ({total} fact rows), oracle restatement of Hive 3.1's vectorized "
-                      f"map-join + hash group-by, one thread per fragment on a pool of {threads}"}
+                      f"map-join + hash group-by, one map task per fragment on a pool of {threads} threads; "
+                      f"median of 5 after a warm-up",
+            "one_thread": {"value": total1 / secs1, "seconds": secs1,
+                           "sample": f"first {take1} fragments ({total1} fact rows) one after another on one thread"}}
 
 
 def batched_section(ctx, reps):
@@ -119,7 +185,7 @@ def batched_section(ctx, reps):
     torch.cuda.synchronize()
     plain_s = time.perf_counter() - t1
     identical = plain.run(collect=True) == got
-    cpu = cpu_batch_baseline(B, min(os.cpu_count() or 1, 16))
+    cpu = cpu_batch_baseline(B, host_cores()["threads"])
     log(f"batched cpu baseline: {cpu['value']:.3e} rows/s ({cpu['sample']})")
     p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
     out = {
@@ -142,38 +208,84 @@ def batched_section(ctx, reps):
     return out
 
 
-def cpu_skew_baseline(seed):
-    """The oracle's restatement of Hive's CPU operators (one thread each) on bounded samples of
-    the config-3 stages: map join of 10M Zipf(1.0) probe rows (10% misses) into 1M build keys;
-    ORDER BY (key, tiebreak) of 5M joined rows; GROUP BY of 10M rows over 1M distinct keys with
-    COUNT(*) and exact SUM(double)."""
+def cpu_skew_baseline(seed, threads):
+    """The oracle's restatement of Hive's CPU operators on bounded samples of the skewed-join stages,
+    on one thread and on `threads` threads as Tez runs them:
+    * map join: 10M Zipf(1.0) probe rows (10% misses) into 1M build keys; the threads probe
+      contiguous slices of one shared, read-only table (as LLAP's broadcast-table cache does);
+    * ORDER BY (key, tiebreak) of 2M joined rows: one thread sorts everything; at T threads each map
+      task sorts its slice and the single ORDER BY reducer merges the T sorted runs;
+    * GROUP BY of 4M rows over 400k distinct keys (10 rows a key, as at full size), COUNT(*) and
+      SUM(double) in Hive's double arithmetic: at T threads, T map tasks each aggregate a slice into
+      their own hash table, their partial groups are partitioned by key over T reducers, and each
+      reducer merges its share (mergepartial).
+    Each figure is the median of 5 after a warm-up; build time of the probe's table is reported
+    apart."""
+    import concurrent.futures as cf
     import numpy as np
     import torch
     import oracle
     from ghive_amd import abi, workloads as W
-    nb, npr, ns, ng = 1_000_000, 10_000_000, 5_000_000, 10_000_000
+    nb, npr, ns, ng, nk = 1_000_000, 10_000_000, 2_000_000, 4_000_000, 400_000
+    T = threads
+    split = lambda n: [(n * t // T, n * (t + 1) // T) for t in range(T)]
     bk = W.key62_torch(seed, torch.arange(1, nb + 1, dtype=torch.int64)).numpy()
     bp = np.arange(1, nb + 1, dtype=np.int64)
     pk, _ = oracle.gen_zipf_keys(seed, 0, 0, npr, nb, 1.0, 0.1)
     t0 = time.perf_counter()
-    rows, _ = oracle.mapjoin_inner(bk, pk, payload=bp)
-    tj = time.perf_counter() - t0
-    tb = oracle.gen_uniform_f32(seed, 1, 0, npr)
+    mj = oracle.MapJoinTable(bk, bp)
+    t_build = time.perf_counter() - t0
+    rows, _ = oracle.mapjoin_inner(bk, pk[: 4 * ns], payload=bp)
+    tb = oracle.gen_uniform_f32(seed, 1, 0, 4 * ns)
     sel = rows[:ns].astype(np.int64)
     jk, jt = np.ascontiguousarray(pk[sel]), np.ascontiguousarray(tb[sel])
-    t0 = time.perf_counter()
-    oracle.sort_perm([jk, jt], [0, 1], "++")
-    ts = time.perf_counter() - t0
-    keys = oracle.gen_uniform_i64(seed, 2, 0, ng, 1, 1_000_000)
+    keys = oracle.gen_uniform_i64(seed, 2, 0, ng, 1, nk)
     vals = oracle.gen_normal_f64(seed, 3, 0, ng)
-    t0 = time.perf_counter()
-    oracle.groupby([keys, vals], [0], [(abi.AGG_CNT, abi.COUNT_STAR), (abi.AGG_SUM, 1)])
-    tg = time.perf_counter() - t0
-    return {"probe_rows_per_s": npr / tj, "sort_rows_per_s": len(jk) / ts, "groupby_rows_per_s": ng / tg,
-            "cores": 1, "kind": "port",
-            "sample": "oracle (C restatement of Hive 3.1 VectorMapJoinFastLongHashTable, ORDER BY as a total "
-                      "order, hash group-by), one thread: 10M Zipf probe rows x 1M build keys; 5M-row sort; "
-                      "10M rows / 1M keys group-by"}
+    aggs = [(abi.AGG_CNT, abi.COUNT_STAR), (abi.AGG_SUM, 1)]
+    out = {"cores": T, "kind": "port"}
+    with cf.ThreadPoolExecutor(T) as ex:
+        probe1 = median_of(lambda: mj.probe(pk))
+        probeT = median_of(lambda: list(ex.map(lambda r: mj.probe(pk[r[0]:r[1]]), split(npr))))
+        sort1 = median_of(lambda: oracle.sort_perm([jk, jt], [0, 1], "++"))
+        phases = []
+        sortT = median_of(lambda: phases.append(oracle.sort_runs([jk, jt], [0, 1], "++", T)[1:]))
+        gby1 = median_of(lambda: oracle.groupby([keys, vals], [0], aggs, exact_double_sum=False))
+
+        def tez_groupby():
+            parts = [None] * T
+
+            def map_task(t):
+                b, e = split(ng)[t]
+                k = keys[b:e]
+                rep, (cnt, sm), _ = oracle.groupby([k, vals[b:e]], [0], aggs, exact_double_sum=False)
+                gk = k[rep]
+                dest = gk % T
+                order = np.argsort(dest, kind="stable")
+                parts[t] = (gk[order], cnt[order], sm[order], np.searchsorted(dest[order], np.arange(T + 1)))
+
+            def reduce_task(r):
+                cat = lambda i: np.concatenate([p[i][p[3][r]:p[3][r + 1]] for p in parts])
+                oracle.groupby([cat(0), cat(1), cat(2)], [0], [(abi.AGG_SUM, 1), (abi.AGG_SUM, 2)],
+                               exact_double_sum=False)
+
+            list(ex.map(map_task, range(T)))
+            list(ex.map(reduce_task, range(T)))
+
+        gbyT = median_of(tez_groupby)
+    mj.free()
+    ph = np.median(np.array(phases), axis=0)
+    out.update({
+        "probe_rows_per_s": npr / probeT, "probe_rows_per_s_one_thread": npr / probe1, "build_ms_one_thread": 1e3 * t_build,
+        "sort_rows_per_s": ns / sortT, "sort_rows_per_s_one_thread": ns / sort1,
+        "sort_phases_ms": {"map_side_run_sorts": 1e3 * float(ph[0]), "reducer_merge": 1e3 * float(ph[1])},
+        "groupby_rows_per_s": ng / gbyT, "groupby_rows_per_s_one_thread": ng / gby1,
+        "sample": "oracle (C restatement of Hive 3.1 VectorMapJoinFastLongHashTable, ORDER BY as a total order, "
+                  f"hash group-by) at {T} threads as Tez map tasks and at one thread: 10M Zipf probe rows x 1M "
+                  "build keys (probe only); 2M-row ORDER BY (map-side run sorts + one merging reducer); 4M rows / "
+                  "400k keys GROUP BY COUNT(*), SUM(double) (map-side hash aggregation + mergepartial over "
+                  f"{T} reducers); median of 5 after a warm-up",
+    })
+    return out
 
 
 def skew_section(ctx, reps):
@@ -239,7 +351,7 @@ def skew_section(ctx, reps):
         "groupby": {"rows": nrows, "groups": ng[0], "ms": 1e3 * t_gby, "rows_per_s": nrows / t_gby,
                     "gbs": gbs(b_gby, t_gby), "frac": gbs(b_gby, t_gby) / HBM_PEAK_GBS},
     }
-    out["cpu_baseline"] = cpu_skew_baseline(W.SEED_BASE + 3)
+    out["cpu_baseline"] = cpu_skew_baseline(W.SEED_BASE + 3, host_cores()["threads"])
     log(f"skew cpu baseline: {out['cpu_baseline']}")
     log(f"skew: build {1e3 * t_build:.1f} ms, probe {1e3 * t_probe:.1f} ms ({out['probe']['gbs']:.0f} GB/s), "
         f"sort {1e3 * t_sort:.1f} ms of {matches} rows, group-by {1e3 * t_gby:.1f} ms "
@@ -573,13 +685,11 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(os.cpu_count() or 1, 16)
+        hc = host_cores()
         sample = min(args.cpu_sample, args.rows)
-        rate, secs = cpu_baseline(T, sample, threads)
-        log(f"cpu baseline: {rate:.3e} rows/s ({secs:.2f}s per run, {threads} threads)")
-        cpu = {"value": rate, "unit": "fact rows/s", "cores": threads, "kind": "port",
-               "sample": f"first {sample} fact rows of the same tables, C restatement of Hive 3.1 "
-                         f"vectorized map-join + hash group-by, {threads} threads, median of 3"}
+        cpu = cpu_baseline(T, sample, hc["threads"], min(sample, 60_000_000))
+        cpu["host"] = hc
+        log(f"cpu baseline: {cpu}")
 
     plain_groups = {(int(y), int(b)): int(v) for y, b, v in zip(*[o.cpu().numpy() for o in out])}
     dropin = None

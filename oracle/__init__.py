@@ -66,6 +66,8 @@ def lib():
         L.orc_exact_sum.restype = C.c_double
         L.orc_sort.argtypes = [P, P, P, u32, P]
         L.orc_sort.restype = C.c_int
+        L.orc_sort_runs.argtypes = [P, P, P, u32, C.c_int, P, P, P]
+        L.orc_sort_runs.restype = C.c_int
         L.orc_fragment_run.argtypes = [P, C.c_int, P, P, P, u64, P, P]
         L.orc_fragment_run.restype = i64
         _lib = L
@@ -299,6 +301,37 @@ def mapjoin_inner(build_keys, probe_keys, build_sel=None, probe_sel=None, payloa
         L.orc_mj_free(h)
 
 
+class MapJoinTable:
+    """A built map-join hash table that several threads probe at once (the probe only reads the
+    table, and ctypes releases the GIL for the call): Hive map tasks probing one broadcast table."""
+
+    def __init__(self, build_keys, payload, load_factor=0.75):
+        self._bt = HostTable([build_keys, payload])
+        self.h = lib().orc_mj_build(C.byref(self._bt.cols[0]), None, len(build_keys),
+                                    C.byref(self._bt.cols[1]), load_factor)
+        if not self.h:
+            raise ValueError("oracle: unsupported key type")
+
+    def probe(self, probe_keys):
+        """Inner-join matches of probe_keys: their count."""
+        pt = HostTable([probe_keys])
+        need = C.c_uint64(0)
+        cap = max(len(probe_keys), 1)
+        while True:
+            rows = np.empty(cap, dtype=np.uint32)
+            pay = np.empty(cap, dtype=np.int64)
+            n = lib().orc_mj_probe(self.h, C.byref(pt.cols[0]), None, len(probe_keys), _ptr(rows),
+                                   _ptr(pay), cap, C.byref(need))
+            if n >= 0:
+                return n
+            cap = need.value
+
+    def free(self):
+        if self.h:
+            lib().orc_mj_free(self.h)
+            self.h = None
+
+
 def mapjoin_stats(build_keys, load_factor=0.75):
     bt = HostTable([build_keys])
     L = lib()
@@ -376,6 +409,20 @@ def sort_perm(columns, key_cols, orders):
     if lib().orc_sort(t.ref(), _ptr(k), o, len(key_cols), _ptr(out)) != 0:
         raise ValueError("oracle: unsupported sort")
     return out[: t.table.row_num].copy()
+
+
+def sort_runs(columns, key_cols, orders, nruns):
+    """ORDER BY as Tez runs it: nruns map-side run sorts in parallel, then one merging reducer.
+    Returns (permutation, seconds of the run sorts, seconds of the merge)."""
+    t = HostTable(columns)
+    k = np.ascontiguousarray(key_cols, dtype=np.uint32)
+    o = C.c_char_p("".join(orders).encode())
+    out = np.empty(max(t.table.row_num, 1), dtype=np.uint32)
+    rs, ms = C.c_double(0), C.c_double(0)
+    if lib().orc_sort_runs(t.ref(), _ptr(k), o, len(key_cols), nruns, _ptr(out), C.byref(rs),
+                           C.byref(ms)) != 0:
+        raise ValueError("oracle: unsupported sort")
+    return out[: t.table.row_num].copy(), rs.value, ms.value
 
 
 class FragmentSpec:

@@ -230,6 +230,14 @@ static void test_sort(uint64_t n) {
     if (perm[i] < n) seen[perm[i]] = 1;
     if (i) CHECK(a[perm[i - 1]] <= a[perm[i]]);  /* LONG NULL sorts as its sentinel: last */
   }
+  uint32_t *perm2 = malloc(4 * (n ? n : 1));
+  double rs = -1, ms = -1;
+  for (int runs = 1; runs <= 5; runs += 2) { /* run sorts + merge: the same permutation */
+    CHECK(orc_sort_runs(&t, keys, "+-", 2, runs, perm2, &rs, &ms) == 0);
+    CHECK(rs >= 0 && ms >= 0);
+    for (uint64_t i = 0; i < n; ++i) CHECK(perm2[i] == perm[i]);
+  }
+  free(perm2);
   free(a);
   free(b);
   free(perm);

@@ -937,6 +937,90 @@ int orc_sort(const ghive_table *table, const uint32_t *key_cols, const char *ord
   return 0;
 }
 
+/* ORDER BY as Tez runs it: each of nruns map tasks sorts its contiguous slice of the rows (the
+ * map-side sort of the shuffle), then the single ORDER BY reducer merges the sorted runs through a
+ * binary heap of run heads. The order is the total order of orc_sort, so the result is the same. */
+typedef struct {
+  const orc_sort_ctx *c;
+  uint32_t *perm;
+  uint64_t n;
+} orc_run_task;
+
+static void *run_sort_task(void *p) {
+  orc_run_task *t = (orc_run_task *)p;
+  qsort_r(t->perm, (size_t)t->n, sizeof(uint32_t), sort_cmp, (void *)t->c);
+  return NULL;
+}
+
+static double secs_between(struct timespec a, struct timespec b) {
+  return (double)(b.tv_sec - a.tv_sec) + 1e-9 * (double)(b.tv_nsec - a.tv_nsec);
+}
+
+int orc_sort_runs(const ghive_table *table, const uint32_t *key_cols, const char *orders,
+                  uint32_t nkeys, int nruns, uint32_t *out_perm, double *run_secs,
+                  double *merge_secs) {
+  for (uint32_t k = 0; k < nkeys; ++k) {
+    if (key_cols[k] >= table->ncols) return -2;
+  }
+  uint64_t n = table->row_num;
+  if (nruns < 1) nruns = 1;
+  if ((uint64_t)nruns > n) nruns = n ? (int)n : 1;
+  orc_sort_ctx c = {table, key_cols, orders, nkeys};
+  uint32_t *runs = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(n ? n : 1));
+  uint64_t *beg = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(nruns + 1));
+  orc_run_task *tasks = (orc_run_task *)malloc(sizeof(orc_run_task) * (size_t)nruns);
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nruns);
+  for (uint64_t i = 0; i < n; ++i) runs[i] = (uint32_t)i;
+  for (int r = 0; r <= nruns; ++r) beg[r] = n * (uint64_t)r / (uint64_t)nruns;
+  struct timespec t0, t1, t2;
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  for (int r = 0; r < nruns; ++r) {
+    tasks[r].c = &c;
+    tasks[r].perm = runs + beg[r];
+    tasks[r].n = beg[r + 1] - beg[r];
+    if (nruns == 1) run_sort_task(&tasks[r]);
+    else pthread_create(&th[r], NULL, run_sort_task, &tasks[r]);
+  }
+  if (nruns > 1)
+    for (int r = 0; r < nruns; ++r) pthread_join(th[r], NULL);
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  /* heap[0..hn) holds run indices ordered by their current head row */
+  int *heap = (int *)malloc(sizeof(int) * (size_t)nruns);
+  uint64_t *pos = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)nruns);
+  int hn = 0;
+  for (int r = 0; r < nruns; ++r) {
+    pos[r] = beg[r];
+    if (pos[r] == beg[r + 1]) continue;
+    int i = hn++;
+    heap[i] = r;
+    while (i > 0) { /* sift up */
+      int p = (i - 1) / 2;
+      if (sort_cmp(&runs[pos[heap[i]]], &runs[pos[heap[p]]], &c) >= 0) break;
+      int t = heap[i]; heap[i] = heap[p]; heap[p] = t;
+      i = p;
+    }
+  }
+  for (uint64_t o = 0; o < n; ++o) {
+    int r = heap[0];
+    out_perm[o] = runs[pos[r]++];
+    if (pos[r] == beg[r + 1]) heap[0] = heap[--hn];
+    int i = 0;
+    for (;;) { /* sift down */
+      int l = 2 * i + 1, m = i;
+      if (l < hn && sort_cmp(&runs[pos[heap[l]]], &runs[pos[heap[m]]], &c) < 0) m = l;
+      if (l + 1 < hn && sort_cmp(&runs[pos[heap[l + 1]]], &runs[pos[heap[m]]], &c) < 0) m = l + 1;
+      if (m == i) break;
+      int t = heap[i]; heap[i] = heap[m]; heap[m] = t;
+      i = m;
+    }
+  }
+  clock_gettime(CLOCK_MONOTONIC, &t2);
+  if (run_secs) *run_secs = secs_between(t0, t1);
+  if (merge_secs) *merge_secs = secs_between(t1, t2);
+  free(heap); free(pos); free(th); free(tasks); free(beg); free(runs);
+  return 0;
+}
+
 /* ------------------------------------------------------------------------------------ */
 /* Fragment = CS1 map task, run as T Tez-style tasks plus one mergepartial step.           */
 /* ------------------------------------------------------------------------------------ */

@@ -92,6 +92,12 @@ double orc_exact_sum(const double *x, uint64_t n);
 int orc_sort(const ghive_table *table, const uint32_t *key_cols, const char *orders,
              uint32_t nkeys, uint32_t *out_perm);
 
+/* The same ORDER BY as nruns map-side run sorts (one thread each) and one k-way merge (the single
+ * ORDER BY reducer); the wall time of each phase is returned. Used as the multi-core CPU baseline. */
+int orc_sort_runs(const ghive_table *table, const uint32_t *key_cols, const char *orders,
+                  uint32_t nkeys, int nruns, uint32_t *out_perm, double *run_secs,
+                  double *merge_secs);
+
 /* Fragment (CS1 map task): fact filter -> chain of broadcast map joins -> GROUP BY, run as
  * T "map tasks" over contiguous fact slices, each with its own copy of every dimension hash
  * table and its own partial aggregates, then one merge (the "mergepartial" reducer).

@@ -183,6 +183,14 @@ def test_mapjoin_brute_force_duplicates_nulls(orc):
     psel = np.nonzero(rng.random(20000) < 0.5)[0].astype(np.uint32)
     rows, vals = orc.mapjoin_inner(bk, pk, build_sel=bsel, probe_sel=psel)
     assert sorted(zip(rows.tolist(), vals.tolist())) == _brute_join(bk, pk, None, bsel, psel)
+    # one table probed from several threads at once: each slice's match count is its brute count
+    import concurrent.futures as cf
+    mj = orc.MapJoinTable(bk, pay)
+    cuts = [0, 1, 777, 9000, 20000]
+    with cf.ThreadPoolExecutor(4) as ex:
+        got = list(ex.map(lambda i: mj.probe(pk[cuts[i]:cuts[i + 1]]), range(4)))
+    mj.free()
+    assert got == [len(_brute_join(bk, pk[cuts[i]:cuts[i + 1]], pay)) for i in range(4)]
 
 
 def test_mapjoin_int_keys_and_expansion(orc):
@@ -456,6 +464,26 @@ def test_sort_total_order(orc):
     perm = orc.sort_perm([k, f], [0, 1], ["-", "+"])
     want = np.lexsort((np.arange(n), fkey, -k))
     assert np.array_equal(perm, want)
+
+
+@pytest.mark.parametrize("nruns", [1, 2, 7, 16])
+def test_sort_runs_merge_is_the_same_total_order(orc, nruns):
+    """Run sorts + one k-way merge (the multi-core CPU baseline's ORDER BY) give the permutation of
+    the one-pass sort and of numpy's lexsort, ties broken by row id across run boundaries."""
+    rng = np.random.default_rng(40 + nruns)
+    n = 20011
+    k = rng.integers(-30, 30, n).astype(np.int64)
+    f = rng.integers(0, 8, n).astype(np.float32)
+    f[::41] = np.nan
+    fkey = np.where(np.isnan(f), np.inf, f)
+    for orders, kk in ((["+", "+"], k), (["-", "+"], -k)):
+        perm, run_s, merge_s = orc.sort_runs([k, f], [0, 1], orders, nruns)
+        assert np.array_equal(perm, np.lexsort((np.arange(n), fkey, kk)))
+        assert np.array_equal(perm, orc.sort_perm([k, f], [0, 1], orders))
+        assert run_s >= 0 and merge_s >= 0
+    # fewer rows than runs, and no rows
+    assert np.array_equal(orc.sort_runs([k[:3]], [0], "-", nruns)[0], np.lexsort((np.arange(3), -k[:3])))
+    assert len(orc.sort_runs([k[:0]], [0], "+", nruns)[0]) == 0
 
 
 def test_sort_string_keys_against_python_order(orc):
