@@ -148,9 +148,10 @@ def cpu_batch_baseline(B, threads, max_rows=200_000_000, one_thread_rows=20_000_
                            "sample": f"first {take1} fragments ({total1} fact rows) one after another on one thread"}}
 
 
-def batched_section(ctx, reps):
+def batched_section(ctx, reps, fixed=None, with_cpu=True):
     """4096 lognormal fragments of the small query shape: autotuned persistent kernel replayed from
-    a HIP graph vs. 4096 plain per-fragment launches; identical outputs are checked."""
+    a HIP graph vs. 4096 plain per-fragment launches; identical outputs are checked. `fixed` (a
+    dict of tile_rows, blocks_per_cu, probe_mode, load_factor) replaces the autotuner's search."""
     import torch
     import numpy as np
     from ghive_amd import autotune, workloads as W
@@ -159,8 +160,11 @@ def batched_section(ctx, reps):
     torch.cuda.synchronize()
     total = int(sum(B.rows))
     log(f"batched: {len(B.rows)} fragments, {total} rows generated in {time.time() - t0:.1f}s")
-    best, results, search_s = autotune.tune_batch(
-        ctx, B, cache_path=os.path.join(ROOT, "gpurun_out", "batch_tuning.json"))
+    if fixed:
+        best, results, search_s = dict(fixed), [], 0.0
+    else:
+        best, results, search_s = autotune.tune_batch(
+            ctx, B, cache_path=os.path.join(ROOT, "gpurun_out", "batch_tuning.json"))
     log(f"batched: autotuner picked {best} after {len(results)} candidates in {search_s:.1f}s")
     ht = W.batch_dim_table(ctx, B, load_factor=best["load_factor"])
     bt = W.batch_create(ctx, B, ht, tile_rows=best["tile_rows"], blocks_per_cu=best["blocks_per_cu"],
@@ -185,8 +189,10 @@ def batched_section(ctx, reps):
     torch.cuda.synchronize()
     plain_s = time.perf_counter() - t1
     identical = plain.run(collect=True) == got
-    cpu = cpu_batch_baseline(B, host_cores()["threads"])
-    log(f"batched cpu baseline: {cpu['value']:.3e} rows/s ({cpu['sample']})")
+    cpu = None
+    if with_cpu:
+        cpu = cpu_batch_baseline(B, host_cores()["threads"])
+        log(f"batched cpu baseline: {cpu['value']:.3e} rows/s ({cpu['sample']})")
     p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
     out = {
         "workload": "4096 map-task fragments, rows lognormal(median 256k, sigma 1.32) clipped to "
@@ -288,7 +294,7 @@ def cpu_skew_baseline(seed, threads):
     return out
 
 
-def skew_section(ctx, reps):
+def skew_section(ctx, reps, with_cpu=True, gby_form="auto"):
     """Config 3 at full size: 64M-key build, 1e9 Zipf(1.0) probe rows with 10% misses, ORDER BY
     (key, float tiebreak) over the joined rows, and GROUP BY over 1e9 rows with 100M distinct int64
     keys (COUNT(*), exact SUM(double)). Wall time around each synchronous call, median of reps;
@@ -319,6 +325,7 @@ def skew_section(ctx, reps):
     jt = S.tiebreak[r64].contiguous()
     del rows, pay, r64, S
     t_sort, perm = timed(lambda: ops.sort(ctx, ops.Table([jk, jt]), [0, 1], ["+", "+"]))
+    si = ops.sort_info(ctx)
     del perm, jk, jt
     torch.cuda.empty_cache()
     keys, vals = W.highcard_tables(ctx)
@@ -330,7 +337,10 @@ def skew_section(ctx, reps):
         ng[0] = outs[0].numel()
         return None
 
+    # the group-by form: chosen by the library (auto), or forced to the global-table form (direct)
+    ctx.set_option("groupby.partition", {"auto": 1, "direct": 0, "partitioned": 2}[gby_form])
     t_gby, _ = timed(gby)
+    ctx.set_option("groupby.partition", 1)
     nrows = keys.numel()
     del keys, vals
     torch.cuda.empty_cache()
@@ -347,14 +357,22 @@ def skew_section(ctx, reps):
         "probe": {"rows": npr, "matches": matches, "ms": 1e3 * t_probe, "rows_per_s": npr / t_probe,
                   "gbs": gbs(b_probe, t_probe), "frac": gbs(b_probe, t_probe) / HBM_PEAK_GBS},
         "sort": {"rows": matches, "ms": 1e3 * t_sort, "rows_per_s": matches / t_sort,
-                 "gbs_lower_bound_bytes": gbs(b_sort, t_sort)},
-        "groupby": {"rows": nrows, "groups": ng[0], "ms": 1e3 * t_gby, "rows_per_s": nrows / t_gby,
+                 "gbs_lower_bound_bytes": gbs(b_sort, t_sort),
+                 # P x 2 x 16 B x N: P passes, each reading and writing a 16-byte (key, row id) record
+                 "passes": si["passes"], "key_words": si["key_words"], "embedded_ids": bool(si["embedded_ids"]),
+                 "gbs_passes": gbs(si["passes"] * 32 * matches, t_sort),
+                 "frac": gbs(si["passes"] * 32 * matches, t_sort) / HBM_PEAK_GBS,
+                 "scatter_bytes_moved": si["scatter_bytes"],
+                 "frac_bytes_moved": gbs(si["scatter_bytes"], t_sort) / HBM_PEAK_GBS},
+        "groupby": {"rows": nrows, "groups": ng[0], "form": gby_form, "ms": 1e3 * t_gby, "rows_per_s": nrows / t_gby,
                     "gbs": gbs(b_gby, t_gby), "frac": gbs(b_gby, t_gby) / HBM_PEAK_GBS},
     }
-    out["cpu_baseline"] = cpu_skew_baseline(W.SEED_BASE + 3, host_cores()["threads"])
-    log(f"skew cpu baseline: {out['cpu_baseline']}")
+    out["cpu_baseline"] = None
+    if with_cpu:
+        out["cpu_baseline"] = cpu_skew_baseline(W.SEED_BASE + 3, host_cores()["threads"])
+        log(f"skew cpu baseline: {out['cpu_baseline']}")
     log(f"skew: build {1e3 * t_build:.1f} ms, probe {1e3 * t_probe:.1f} ms ({out['probe']['gbs']:.0f} GB/s), "
-        f"sort {1e3 * t_sort:.1f} ms of {matches} rows, group-by {1e3 * t_gby:.1f} ms "
+        f"sort {1e3 * t_sort:.1f} ms of {matches} rows ({si['passes']} passes, frac {out['sort']['frac']:.3f}), group-by {1e3 * t_gby:.1f} ms "
         f"({out['groupby']['gbs']:.0f} GB/s, {ng[0]} groups)")
     return out
 
@@ -569,9 +587,13 @@ def main():
     ap.add_argument("--no-dropin", action="store_true", help="skip the run through ghive_gpu_process")
     ap.add_argument("--dropin-reps", type=int, default=2)
     ap.add_argument("--batched-reps", type=int, default=50)
+    ap.add_argument("--batched-config", default="",
+                    help="tile_rows,blocks_per_cu,probe_mode,load_factor instead of the autotuner's search")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving form")
     ap.add_argument("--no-skew", action="store_true", help="skip the skewed join / sort / group-by stages")
     ap.add_argument("--skew-reps", type=int, default=3)
+    ap.add_argument("--groupby-form", default="auto", choices=["auto", "direct", "partitioned"],
+                    help="form of the 1e9-row / 100M-key group-by (direct = one global hash table)")
     ap.add_argument("--serving-rates", default="50,200,1000,2000", help="offered request rates (req/s)")
     ap.add_argument("--serving-seconds", type=float, default=1.5)
     ap.add_argument("--serving-slots", type=int, default=8)
@@ -700,11 +722,15 @@ def main():
     T = None  # the 600M-row tables are not needed past this point
     torch.cuda.empty_cache()
     if not args.no_batched:
-        batched = batched_section(ctx, args.batched_reps)
+        fixed = None
+        if args.batched_config:
+            tr, bpc, mode, lf = args.batched_config.split(",")
+            fixed = dict(tile_rows=int(tr), blocks_per_cu=int(bpc), probe_mode=int(mode), load_factor=float(lf))
+        batched = batched_section(ctx, args.batched_reps, fixed, with_cpu=not args.no_cpu_baseline)
 
     skew = None
     if not args.no_skew:
-        skew = skew_section(ctx, args.skew_reps)
+        skew = skew_section(ctx, args.skew_reps, with_cpu=not args.no_cpu_baseline, gby_form=args.groupby_form)
 
     serving = None
     if not args.no_serving:

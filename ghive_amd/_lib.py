@@ -83,6 +83,7 @@ SIGNATURES = {
                                           C.POINTER(C.POINTER(C.c_int32))]),
     "ghive_gpu_result_free": (None, [P]),
     "ghive_last_process_times": (C.c_int, [P, C.POINTER(abi.ProcessTimes)]),
+    "ghive_last_sort_info": (C.c_int, [P, C.POINTER(abi.SortInfo)]),
 }
 
 

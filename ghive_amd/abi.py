@@ -125,6 +125,11 @@ class ProcessTimes(C.Structure):
                 ("fused", C.c_int32)]
 
 
+class SortInfo(C.Structure):
+    _fields_ = [("rows", C.c_uint64), ("key_words", C.c_uint32), ("passes", C.c_uint32),
+                ("embedded_ids", C.c_uint32), ("reserved", C.c_uint32), ("scatter_bytes", C.c_uint64)]
+
+
 # Oracle-side fragment description (oracle/ghive_oracle.h), kept here so both sides share layouts.
 class OrcDim(C.Structure):
     _fields_ = [("table", Table), ("filter", C.POINTER(Pred)), ("filter_nnodes", C.c_uint32),

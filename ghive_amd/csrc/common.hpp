@@ -78,6 +78,7 @@ struct ghive_ctx {
   ghive_stage_times times{};
   ghive_plan_exec_info plan_info{};        // what the last ghive_plan_execute did
   ghive_process_times process_times{};     // the last ghive_gpu_process
+  ghive_sort_info sort_info{};             // the last ghive_sort
   // the device address of this device's static scratch array (ctx.hip), looked up at init
   void *scratch = nullptr;
   ghive_tuning tune{};

@@ -253,6 +253,10 @@ static int radix_passes(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, 
                                                                           hist.as<uint32_t>(),
                                                                           counts.as<uint32_t>(), nblocks);
     GH_LAUNCH(ctx);
+    const bool drop = drop0_last && KW >= 2 && d == digits.back();
+    const uint64_t in_b = 8ull * KW + (ids ? 4 : 0), out_b = drop ? 8ull * (KW - 1) + 4 : in_b;
+    ctx->sort_info.passes += 1;
+    ctx->sort_info.scatter_bytes += n * (in_b + out_b);
     std::swap(kin, kout);
     std::swap(iin, iout);
   }
@@ -454,11 +458,15 @@ int ghive_sort(ghive_ctx *ctx, const ghive_table *table, const uint32_t *key_col
     bits += S.width[k];
   }
   if (bits > 256) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "sort: keys are %d bits wide (max 256)", bits);
+  memset(&ctx->sort_info, 0, sizeof ctx->sort_info);
+  ctx->sort_info.rows = n;
   if (n == 0) return GHIVE_OK;
   // when the last key word has 32 spare bits the row id rides in them (below every field): the
   // passes then move no separate id array, and equal keys stay in row-id order as before
   const int KW = (bits + 63) / 64;
   const bool embed = KW * 64 - bits >= 32;
+  ctx->sort_info.key_words = (uint32_t)KW;
+  ctx->sort_info.embedded_ids = embed ? 1u : 0u;
   int top = bits + (embed ? 32 : 0);
   for (uint32_t k = 0; k < nkeys; ++k) {
     top -= S.width[k];
@@ -532,6 +540,12 @@ int ghive_sort(ghive_ctx *ctx, const ghive_table *table, const uint32_t *key_col
     GH_HIP(ctx, hipMemcpyAsync(out_perm, res, 4ull * n, hipMemcpyDeviceToDevice, ctx->stream));
   }
   GH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return GHIVE_OK;
+}
+
+int ghive_last_sort_info(ghive_ctx *ctx, ghive_sort_info *out) {
+  if (!ctx || !out) return GHIVE_ERR_INVALID;
+  *out = ctx->sort_info;
   return GHIVE_OK;
 }
 

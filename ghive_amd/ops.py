@@ -448,6 +448,13 @@ def sort(ctx, table, key_cols, orders):
     return perm[: table.row_num]
 
 
+def sort_info(ctx):
+    """What the last sort on ctx did (ghive_last_sort_info): key words, passes, bytes moved."""
+    t = abi.SortInfo()
+    ctx.check(lib().ghive_last_sort_info(ctx.h, C.byref(t)))
+    return {f: getattr(t, f) for f, _ in abi.SortInfo._fields_ if f != "reserved"}
+
+
 def _column_of(t):
     c = abi.Column()
     c.type = _TYPE_OF_DTYPE[t.dtype]

@@ -363,6 +363,20 @@ int ghive_select_substr(ghive_ctx *ctx, const ghive_column *str, int32_t pos, in
 int ghive_sort(ghive_ctx *ctx, const ghive_table *table, const uint32_t *key_cols,
                const char *orders, uint32_t nkeys, uint32_t *out_perm);
 
+/* What the last ghive_sort on this context did (no counterpart in the reference; for measurement):
+ * the 64-bit key words built per row, the LSD passes run (an 8-bit digit whose byte is the same
+ * in every row is skipped), whether the row ids rode in spare key bits, and the bytes the scatter
+ * passes read and wrote in total (rows x bytes moved per row, summed over the passes). */
+typedef struct ghive_sort_info {
+  uint64_t rows;
+  uint32_t key_words;
+  uint32_t passes;
+  uint32_t embedded_ids;
+  uint32_t reserved;
+  uint64_t scatter_bytes;
+} ghive_sort_info;
+int ghive_last_sort_info(ghive_ctx *ctx, ghive_sort_info *out);
+
 /* ---- fused fragment: filter -> star joins -> expression -> group-by ------------------- *
  * One Tez map-task fragment of GHive's CS1 call stack (TableScan -> Filter -> MapJoin* ->
  * Select -> GroupBy -> ReduceSink), executed as one streaming pass over the fact columns. */

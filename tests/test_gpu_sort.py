@@ -86,3 +86,21 @@ def test_sort_large_sorted_and_permutation(ctx):
     ks, fs = k[perm], f[perm]
     assert bool(((ks[1:] > ks[:-1]) | ((ks[1:] == ks[:-1]) & (fs[1:] >= fs[:-1]))).all().item())
     assert bool((torch.bincount(perm, minlength=n) == 1).all().item())
+
+
+def test_sort_info_counts_the_passes_it_ran(ctx, orc):
+    """ghive_last_sort_info: a digit byte that is the same in every row costs no pass. LONG keys in
+    [0, 256): one varying byte, one pass moving an 8-byte key and a 4-byte id each way; INT keys in
+    [0, 256): the id rides in the low half of the word, one pass of 8 bytes each way; one value in
+    every row: no pass at all. The permutations still equal the oracle's."""
+    from ghive_amd import ops
+    n = 100_003
+    rng = np.random.default_rng(5)
+    for dtype, key_words, embedded, per_row in ((np.int64, 1, False, 24), (np.int32, 1, True, 16)):
+        k = rng.integers(0, 256, n).astype(dtype)
+        _check(ctx, orc, [k], [0], ["+"])
+        info = ops.sort_info(ctx)
+        assert info == {"rows": n, "key_words": key_words, "passes": 1, "embedded_ids": int(embedded),
+                        "scatter_bytes": per_row * n}
+    _check(ctx, orc, [np.full(n, 7, dtype=np.int64)], [0], ["-"])
+    assert ops.sort_info(ctx)["passes"] == 0
