@@ -64,6 +64,8 @@ struct ghive_tuning {
   int64_t gb_lds_budget = 64 * 1024;    // bytes of LDS for the group-by's private table
   int64_t gb_partition = 1;             // high-cardinality form: 0 never, 1 by size, 2 always
   int64_t plan_fuse = 1;                // 1: plan vertices of the map-join shape run as one fragment
+  int64_t gb_part_threads = 512;        // workgroup size of the group-by partition passes (256, 512, 1024)
+  int64_t sort_shape = 1;               // LSD tile: 0 256 x 32, 1 512 x 16, 2 512 x 32, 3 1024 x 8
 };
 
 struct ghive_ctx {

@@ -706,12 +706,15 @@ int group_table_init(ghive_ctx *ctx, GroupTableDev &T, uint64_t *keys, uint64_t 
 // Per pass: a per-tile digit histogram, one scan of the digit x tile matrix, then the scatter: every
 // element ranked among the equal digits of its wave with ballots (stable), the tile staged through
 // LDS in digit order and each digit's run written to consecutive addresses. A 9-bit pass uses
-// 4096-element tiles (52 KB of LDS, three workgroups per CU), an 8-bit pass 8192.
-template <int DB>
+// 4096-element tiles, an 8-bit pass 8192.
+// TH threads a workgroup (256, 512 or 1024, option groupby.part_threads; 512 by default, 4 ms
+// faster than 256 at 1e9 rows): the same tile in every case.
+template <int DB, int TH>
 struct Gbp {
   static constexpr int kBins = 1 << DB;
-  static constexpr int kPer = DB == 9 ? 16 : 32;
-  static constexpr uint32_t kTile = 256u * kPer;
+  static constexpr uint32_t kTile = DB == 9 ? 4096u : 8192u;
+  static constexpr int kPer = (int)(kTile / TH);
+  static constexpr int kBpt = kBins >= TH ? kBins / TH : 1;  // bins per thread (threads >= kBins: one)
   typedef typename std::conditional<DB == 9, uint16_t, uint8_t>::type Digit;
 };
 
@@ -736,33 +739,33 @@ __device__ __forceinline__ uint64_t gbp_hash(const GbpSrc &S, uint64_t e) {
   return mix64(pack_key(S.K, S.sel ? S.sel[e] : e));
 }
 
-template <int DB, int SRC>
-__global__ __launch_bounds__(256) void k_gbp_hist(GbpSrc S, uint64_t n, int shift, uint32_t *hist, uint32_t *counts,
-                                                  uint32_t nblocks) {
-  constexpr int B = Gbp<DB>::kBins;
+template <int DB, int SRC, int TH>
+__global__ __launch_bounds__(TH) void k_gbp_hist(GbpSrc S, uint64_t n, int shift, uint32_t *hist, uint32_t *counts,
+                                                 uint32_t nblocks) {
+  constexpr int B = Gbp<DB, TH>::kBins;
   __shared__ uint32_t hc[B];
-  for (int b = threadIdx.x; b < B; b += 256) hc[b] = 0;
+  for (int b = threadIdx.x; b < B; b += TH) hc[b] = 0;
   __syncthreads();
-  const uint64_t t0 = (uint64_t)blockIdx.x * Gbp<DB>::kTile;
+  const uint64_t t0 = (uint64_t)blockIdx.x * Gbp<DB, TH>::kTile;
 #pragma unroll 4
-  for (int j = 0; j < Gbp<DB>::kPer; ++j) {
-    const uint64_t e = t0 + (uint64_t)j * 256 + threadIdx.x;
+  for (int j = 0; j < Gbp<DB, TH>::kPer; ++j) {
+    const uint64_t e = t0 + (uint64_t)j * TH + threadIdx.x;
     if (e < n) atomicAdd(&hc[(gbp_hash<SRC>(S, e) >> shift) & (B - 1)], 1u);
   }
   __syncthreads();
-  for (int b = threadIdx.x; b < B; b += 256) {
+  for (int b = threadIdx.x; b < B; b += TH) {
     hist[(uint64_t)b * nblocks + blockIdx.x] = hc[b];
     counts[(uint64_t)blockIdx.x * B + b] = hc[b];
   }
 }
 
-template <int DB, int SRC, bool VAL>
-__global__ __launch_bounds__(256) void k_gbp_scatter(GbpSrc S, uint64_t n, int shift, const uint32_t *offs,
-                                                     const uint32_t *counts, uint32_t nblocks, uint64_t *h_out,
-                                                     uint64_t *v_out) {
-  constexpr int B = Gbp<DB>::kBins, kPer = Gbp<DB>::kPer, kWaves = 4, kBpt = B / 256;  // bins per thread
-  constexpr uint32_t kTile = Gbp<DB>::kTile, kSeg = (uint32_t)kPer * kWave;
-  typedef typename Gbp<DB>::Digit Digit;
+template <int DB, int SRC, bool VAL, int TH>
+__global__ __launch_bounds__(TH) void k_gbp_scatter(GbpSrc S, uint64_t n, int shift, const uint32_t *offs,
+                                                    const uint32_t *counts, uint32_t nblocks, uint64_t *h_out,
+                                                    uint64_t *v_out) {
+  constexpr int B = Gbp<DB, TH>::kBins, kPer = Gbp<DB, TH>::kPer, kWaves = TH / kWave, kBpt = Gbp<DB, TH>::kBpt;
+  constexpr uint32_t kTile = Gbp<DB, TH>::kTile, kSeg = (uint32_t)kPer * kWave;
+  typedef typename Gbp<DB, TH>::Digit Digit;
   __shared__ uint64_t stage[kTile];
   __shared__ Digit sdigit[kTile];
   __shared__ uint32_t lbase[B], goff[B];
@@ -771,19 +774,22 @@ __global__ __launch_bounds__(256) void k_gbp_scatter(GbpSrc S, uint64_t n, int s
   const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
   const uint64_t t0 = (uint64_t)blockIdx.x * kTile;
   const uint32_t tn = (uint32_t)(n - t0 < kTile ? n - t0 : kTile);
+  const bool has_bins = B >= TH || tid < (uint32_t)B;  // threads past the last bin hold none
   {  // this tile's first position of every digit, and the digit's global offset
     uint32_t c[kBpt], s = 0, total;
 #pragma unroll
     for (int q = 0; q < kBpt; ++q) {
-      c[q] = counts[(uint64_t)blockIdx.x * B + tid * kBpt + q];
+      c[q] = has_bins ? counts[(uint64_t)blockIdx.x * B + tid * kBpt + q] : 0u;
       s += c[q];
     }
     uint32_t lb = block_exclusive_scan_256(s, scan_tmp, total);
+    if (has_bins) {
 #pragma unroll
-    for (int q = 0; q < kBpt; ++q) {
-      lbase[tid * kBpt + q] = lb;
-      lb += c[q];
-      goff[tid * kBpt + q] = offs[(uint64_t)(tid * kBpt + q) * nblocks + blockIdx.x];
+      for (int q = 0; q < kBpt; ++q) {
+        lbase[tid * kBpt + q] = lb;
+        lb += c[q];
+        goff[tid * kBpt + q] = offs[(uint64_t)(tid * kBpt + q) * nblocks + blockIdx.x];
+      }
     }
   }
 #pragma unroll
@@ -835,6 +841,7 @@ __global__ __launch_bounds__(256) void k_gbp_scatter(GbpSrc S, uint64_t n, int s
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < kBpt; ++q) {  // exclusive prefix of each digit's counts over the waves
+    if (!has_bins) break;
     const uint32_t d = tid * kBpt + q;
     uint32_t acc = 0;
 #pragma unroll
@@ -861,7 +868,7 @@ __global__ __launch_bounds__(256) void k_gbp_scatter(GbpSrc S, uint64_t n, int s
       if (wave * kSeg + (uint32_t)j * kWave + lane < tn) stage[lpos[j]] = q ? vv[j] : hv[j];
     __syncthreads();
     uint64_t *out = q ? v_out : h_out;
-    for (uint32_t p = tid; p < tn; p += 256) {
+    for (uint32_t p = tid; p < tn; p += TH) {
       const uint32_t d = sdigit[p];
       out[(uint64_t)goff[d] + (p - lbase[d])] = stage[p];
     }
@@ -871,7 +878,8 @@ __global__ __launch_bounds__(256) void k_gbp_scatter(GbpSrc S, uint64_t n, int s
 // The stable passes over the top pbits bits of h, low digits first (at most 9 bits a pass, the
 // first pass as wide as the rest allow). h/v hold n words each; the partitioned (h, v) end in the
 // buffers *h_res / *v_res.
-static int gbp_partition(ghive_ctx *ctx, const GbpSrc &src, uint64_t n, int pbits, bool val, uint64_t *h0,
+template <int TH>
+static int gbp_partition_th(ghive_ctx *ctx, const GbpSrc &src, uint64_t n, int pbits, bool val, uint64_t *h0,
                          uint64_t *v0, uint64_t *h1, uint64_t *v1, uint64_t **h_res, uint64_t **v_res) {
   std::vector<int> widths;  // digit widths, low digit first (one 8-bit pass at least)
   for (int left = pbits < 8 ? 8 : pbits; left > 0;) {
@@ -880,7 +888,7 @@ static int gbp_partition(ghive_ctx *ctx, const GbpSrc &src, uint64_t n, int pbit
     widths.push_back(w > 9 ? 9 : w);
     left -= widths.back();
   }
-  const uint32_t nb_max = ceil_div(n, Gbp<9>::kTile);  // the smaller tile
+  const uint32_t nb_max = ceil_div(n, Gbp<9, TH>::kTile);  // the smaller tile
   DevBuf hist, counts;
   GH_TRY(hist.alloc(ctx, sizeof(uint32_t) * 512ull * nb_max));
   GH_TRY(counts.alloc(ctx, sizeof(uint32_t) * 512ull * nb_max));
@@ -899,7 +907,7 @@ static int gbp_partition(ghive_ctx *ctx, const GbpSrc &src, uint64_t n, int pbit
     // is the last one, else take DB bits starting at lo (the extra high bits are the next digit's,
     // which the later, stable passes order anyway)
     const int sh = shift + db > 64 ? 64 - db : shift;
-    const uint32_t tile = db == 9 ? Gbp<9>::kTile : Gbp<8>::kTile;
+    const uint32_t tile = db == 9 ? Gbp<9, TH>::kTile : Gbp<8, TH>::kTile;
     const uint32_t nb = ceil_div(n, tile);
     const uint32_t bins = db == 9 ? 512 : 256;
     if (!first) {
@@ -907,7 +915,7 @@ static int gbp_partition(ghive_ctx *ctx, const GbpSrc &src, uint64_t n, int pbit
       S.v = vin;
     }
     const int src = !first ? kSrcPass : (fast ? kSrcCol64 : kSrcCols);
-#define GBP_HIST(D, F) k_gbp_hist<D, F><<<nb, 256, 0, ctx->stream>>>(S, n, sh, hist.as<uint32_t>(), counts.as<uint32_t>(), nb)
+#define GBP_HIST(D, F) k_gbp_hist<D, F, TH><<<nb, TH, 0, ctx->stream>>>(S, n, sh, hist.as<uint32_t>(), counts.as<uint32_t>(), nb)
     if (db == 9) {
       if (src == kSrcCols) GBP_HIST(9, kSrcCols); else if (src == kSrcCol64) GBP_HIST(9, kSrcCol64); else GBP_HIST(9, kSrcPass);
     } else {
@@ -917,8 +925,8 @@ static int gbp_partition(ghive_ctx *ctx, const GbpSrc &src, uint64_t n, int pbit
     GH_LAUNCH(ctx);
     GH_TRY(exclusive_scan_u32(ctx, hist.as<uint32_t>(), hist.as<uint32_t>(), (uint64_t)bins * nb, nullptr));
 #define GBP_SCATTER(D, F, V)                                                                                   \
-  k_gbp_scatter<D, F, V><<<nb, 256, 0, ctx->stream>>>(S, n, sh, hist.as<uint32_t>(), counts.as<uint32_t>(), nb, \
-                                                      hout, vout)
+  k_gbp_scatter<D, F, V, TH><<<nb, TH, 0, ctx->stream>>>(S, n, sh, hist.as<uint32_t>(), counts.as<uint32_t>(), nb, \
+                                                         hout, vout)
     if (db == 9) {
       if (src == kSrcCols) { if (val) GBP_SCATTER(9, kSrcCols, true); else GBP_SCATTER(9, kSrcCols, false); }
       else if (src == kSrcCol64) { if (val) GBP_SCATTER(9, kSrcCol64, true); else GBP_SCATTER(9, kSrcCol64, false); }
@@ -939,6 +947,15 @@ static int gbp_partition(ghive_ctx *ctx, const GbpSrc &src, uint64_t n, int pbit
   *h_res = hin;
   *v_res = vin;
   return GHIVE_OK;
+}
+
+static int gbp_partition(ghive_ctx *ctx, const GbpSrc &src, uint64_t n, int pbits, bool val, uint64_t *h0,
+                         uint64_t *v0, uint64_t *h1, uint64_t *v1, uint64_t **h_res, uint64_t **v_res) {
+  switch (ctx->tune.gb_part_threads) {
+    case 256: return gbp_partition_th<256>(ctx, src, n, pbits, val, h0, v0, h1, v1, h_res, v_res);
+    case 1024: return gbp_partition_th<1024>(ctx, src, n, pbits, val, h0, v0, h1, v1, h_res, v_res);
+    default: return gbp_partition_th<512>(ctx, src, n, pbits, val, h0, v0, h1, v1, h_res, v_res);
+  }
 }
 
 // Runs the high-cardinality form: 2^pbits hash partitions, each aggregated in an LDS table of 2^lg

@@ -17,10 +17,12 @@
 
 namespace ghive {
 
+// Tile shapes of the LSD passes: TH threads x PER elements per thread (option sort.shape). The
+// default, 512 x 16, keeps the 8192-element tile of 256 x 32 (each digit's run on write-out is as
+// long) with twice the waves per workgroup and half the registers per lane: 0.9e9 rows, 12 passes
+// in 119 ms against 144 ms. 1024 x 8 measured the same; 512 x 32 (16384 elements) 140 ms.
 constexpr int kSortThreads = 256;
 constexpr int kSortPerThread = 32;
-constexpr uint64_t kSortTile = (uint64_t)kSortThreads * kSortPerThread;
-constexpr int kSortWaves = kSortThreads / kWave;
 
 struct SortKeySpec {
   int32_t nkeys;
@@ -101,55 +103,63 @@ __global__ void k_sort_keys(SortKeySpec S, uint64_t n, uint64_t *keys, uint32_t 
 
 // hist[digit * nblocks + block] (scanned afterwards into global offsets) and the same counts
 // block-major, counts[block * 256 + digit] (a tile's local offsets in the scatter).
-__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t *keys, uint64_t n, int word, int shift,
-                                                             uint32_t *hist, uint32_t *counts, uint32_t nblocks) {
+template <int TH, int PER>
+__global__ __launch_bounds__(TH) void k_radix_hist(const uint64_t *keys, uint64_t n, int word, int shift,
+                                                   uint32_t *hist, uint32_t *counts, uint32_t nblocks) {
   __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
+  for (uint32_t d = threadIdx.x; d < 256; d += TH) h[d] = 0;
   __syncthreads();
-  const uint64_t t0 = (uint64_t)blockIdx.x * kSortTile;
+  const uint64_t t0 = (uint64_t)blockIdx.x * (uint64_t)(TH * PER);
   const uint64_t *kw = keys + (uint64_t)word * n;
-  for (int j = 0; j < kSortPerThread; ++j) {
-    uint64_t e = t0 + (uint64_t)j * kSortThreads + threadIdx.x;
+  for (int j = 0; j < PER; ++j) {
+    uint64_t e = t0 + (uint64_t)j * TH + threadIdx.x;
     if (e < n) atomicAdd(&h[(kw[e] >> shift) & 255u], 1u);
   }
   __syncthreads();
-  hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
-  counts[(uint64_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];
+  for (uint32_t d = threadIdx.x; d < 256; d += TH) {
+    hist[(uint64_t)d * nblocks + blockIdx.x] = h[d];
+    counts[(uint64_t)blockIdx.x * 256 + d] = h[d];
+  }
 }
 
-// One stable pass over a tile of kSortTile elements. (1) Every element's position within the tile
+// One stable pass over a tile of TH x PER elements. (1) Every element's position within the tile
 // in digit order: wave64 ballots give the rank among equal digits of a wave, LDS counters carry it
 // across waves and iterations. (2) The tile is rewritten through a 32 KiB LDS buffer, one key
 // word (then the row ids) at a time, in digit order, so each digit's run leaves as consecutive
 // addresses (coalesced stores) at its global offset.
 // DROP0: the last pass over word 0, whose low 32 bits hold the row id: the later passes need only
 // the id, so word 0 leaves as a uint32 id array (idx_out) and words 1.. as a (KW - 1)-word array.
-template <int KW, bool IDS, bool DROP0 = false>
-__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *keys_in, const uint32_t *idx_in,
-                                                                uint64_t *keys_out, uint32_t *idx_out, uint64_t n,
-                                                                int word, int shift, const uint32_t *offs,
-                                                                const uint32_t *counts, uint32_t nblocks) {
-  __shared__ uint64_t stage[kSortTile];
-  __shared__ uint8_t sdigit[kSortTile];
+template <int KW, bool IDS, bool DROP0, int TH, int PER>
+__global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, const uint32_t *idx_in,
+                                                      uint64_t *keys_out, uint32_t *idx_out, uint64_t n,
+                                                      int word, int shift, const uint32_t *offs,
+                                                      const uint32_t *counts, uint32_t nblocks) {
+  constexpr uint32_t kTile = (uint32_t)TH * PER, kWaves = TH / kWave;
+  __shared__ uint64_t stage[kTile];
+  __shared__ uint8_t sdigit[kTile];
   __shared__ uint32_t lbase[256], goff[256];
-  __shared__ uint32_t wcnt[kSortWaves][256];
-  __shared__ uint32_t scan_tmp[kSortWaves];
-  constexpr uint32_t kSeg = kSortPerThread * kWave;  // each wave owns one contiguous segment of the tile
+  __shared__ uint32_t wcnt[kWaves][256];
+  __shared__ uint32_t scan_tmp[kWaves];
+  constexpr uint32_t kSeg = PER * kWave;  // each wave owns one contiguous segment of the tile
   const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
-  const uint64_t t0 = (uint64_t)blockIdx.x * kSortTile;
-  const uint32_t tn = (uint32_t)(n - t0 < kSortTile ? n - t0 : kSortTile);
+  const uint64_t t0 = (uint64_t)blockIdx.x * kTile;
+  const uint32_t tn = (uint32_t)(n - t0 < kTile ? n - t0 : kTile);
   uint32_t total;
-  const uint32_t lb = block_exclusive_scan_256(counts[(uint64_t)blockIdx.x * 256 + tid], scan_tmp, total);
-  lbase[tid] = lb;
-  goff[tid] = offs[(uint64_t)tid * nblocks + blockIdx.x];
+  // threads past the 256 digits add zeros at the end of the scan
+  const uint32_t lb = block_exclusive_scan_256(tid < 256 ? counts[(uint64_t)blockIdx.x * 256 + tid] : 0u, scan_tmp,
+                                               total);
+  if (tid < 256) {
+    lbase[tid] = lb;
+    goff[tid] = offs[(uint64_t)tid * nblocks + blockIdx.x];
+  }
 #pragma unroll
   for (int q = 0; q < 256 / kWave; ++q) wcnt[wave][lane + q * kWave] = 0;
   // all loads of the tile up front, element wave * kSeg + j * 64 + lane (coalesced per j); past the
   // end a lane re-reads the last element (ignored)
-  uint64_t kv[kSortPerThread][KW];
-  uint32_t id[kSortPerThread], lpos[kSortPerThread], dg[kSortPerThread];
+  uint64_t kv[PER][KW];
+  uint32_t id[PER], lpos[PER], dg[PER];
 #pragma unroll
-  for (int j = 0; j < kSortPerThread; ++j) {
+  for (int j = 0; j < PER; ++j) {
     const uint32_t el = wave * kSeg + (uint32_t)j * kWave + lane;
     const uint64_t e = t0 + (el < tn ? el : tn - 1);
 #pragma unroll
@@ -162,7 +172,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *
   // wave reads and then updates its row: no barrier between loads).
   const uint64_t lt = (1ull << lane) - 1ull;
 #pragma unroll
-  for (int j = 0; j < kSortPerThread; ++j) {
+  for (int j = 0; j < PER; ++j) {
     const bool valid = wave * kSeg + (uint32_t)j * kWave + lane < tn;
     uint32_t digit = 0;
 #pragma unroll
@@ -183,10 +193,10 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *
     dg[j] = digit;
   }
   __syncthreads();
-  {  // exclusive prefix of each digit's counts over the waves (segments are in wave order)
+  if (tid < 256) {  // exclusive prefix of each digit's counts over the waves (segments are in wave order)
     uint32_t acc = 0;
 #pragma unroll
-    for (int w = 0; w < kSortWaves; ++w) {
+    for (int w = 0; w < (int)kWaves; ++w) {
       const uint32_t c = wcnt[w][tid];
       wcnt[w][tid] = acc;
       acc += c;
@@ -194,7 +204,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *
   }
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < kSortPerThread; ++j) {
+  for (int j = 0; j < PER; ++j) {
     if (wave * kSeg + (uint32_t)j * kWave + lane < tn) {
       const uint32_t pos = lbase[dg[j]] + wcnt[wave][dg[j]] + lpos[j];
       lpos[j] = pos;
@@ -206,10 +216,10 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *
   for (int q = 0; q < (IDS ? KW + 1 : KW); ++q) {
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kSortPerThread; ++j)
+    for (int j = 0; j < PER; ++j)
       if (wave * kSeg + (uint32_t)j * kWave + lane < tn) stage[lpos[j]] = q < KW ? kv[j][q] : (uint64_t)id[j];
     __syncthreads();
-    for (uint32_t p = tid; p < tn; p += kSortThreads) {
+    for (uint32_t p = tid; p < tn; p += TH) {
       const uint32_t d = sdigit[p];
       const uint64_t g = (uint64_t)goff[d] + (p - lbase[d]);
       if (DROP0 && q == 0)
@@ -225,11 +235,11 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *
 // ids: carry the uint32 id arrays i0/i1 (false: the ids live inside the keys; i0/i1 unused).
 // drop0_last: the last pass writes word 0 as the uint32 id array (k_radix_scatter DROP0; needs
 // !ids, KW >= 2): its outputs are then *result (ids) and *keys_result (KW - 1 words).
-template <int KW>
-static int radix_passes(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, uint32_t *i0, uint32_t *i1,
-                        const std::vector<int> &digits, uint32_t **result, uint64_t **keys_result = nullptr,
-                        bool ids = true, bool drop0_last = false) {
-  const uint32_t nblocks = ceil_div(n, kSortTile);
+template <int KW, int TH, int PER>
+static int radix_passes_shape(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, uint32_t *i0, uint32_t *i1,
+                              const std::vector<int> &digits, uint32_t **result, uint64_t **keys_result, bool ids,
+                              bool drop0_last) {
+  const uint32_t nblocks = ceil_div(n, (uint64_t)TH * PER);
   DevBuf hist, counts;
   GH_TRY(hist.alloc(ctx, sizeof(uint32_t) * 256ull * nblocks));
   GH_TRY(counts.alloc(ctx, sizeof(uint32_t) * 256ull * nblocks));
@@ -237,23 +247,21 @@ static int radix_passes(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, 
   uint32_t *iin = i0, *iout = i1;
   for (int d : digits) {
     const int word = d / 8, shift = 8 * (d % 8);
-    k_radix_hist<<<nblocks, kSortThreads, 0, ctx->stream>>>(kin, n, word, shift, hist.as<uint32_t>(),
-                                                            counts.as<uint32_t>(), nblocks);
+    k_radix_hist<TH, PER><<<nblocks, TH, 0, ctx->stream>>>(kin, n, word, shift, hist.as<uint32_t>(),
+                                                           counts.as<uint32_t>(), nblocks);
     GH_LAUNCH(ctx);
     GH_TRY(exclusive_scan_u32(ctx, hist.as<uint32_t>(), hist.as<uint32_t>(), 256ull * nblocks, nullptr));
-    if (drop0_last && KW >= 2 && d == digits.back())
-      k_radix_scatter<KW, false, true><<<nblocks, kSortThreads, 0, ctx->stream>>>(
+    const bool drop = drop0_last && KW >= 2 && d == digits.back();
+    if (drop)
+      k_radix_scatter<KW, false, true, TH, PER><<<nblocks, TH, 0, ctx->stream>>>(
           kin, iin, kout, iout, n, word, shift, hist.as<uint32_t>(), counts.as<uint32_t>(), nblocks);
     else if (ids)
-      k_radix_scatter<KW, true><<<nblocks, kSortThreads, 0, ctx->stream>>>(kin, iin, kout, iout, n, word, shift,
-                                                                         hist.as<uint32_t>(), counts.as<uint32_t>(),
-                                                                         nblocks);
+      k_radix_scatter<KW, true, false, TH, PER><<<nblocks, TH, 0, ctx->stream>>>(
+          kin, iin, kout, iout, n, word, shift, hist.as<uint32_t>(), counts.as<uint32_t>(), nblocks);
     else
-      k_radix_scatter<KW, false><<<nblocks, kSortThreads, 0, ctx->stream>>>(kin, iin, kout, iout, n, word, shift,
-                                                                          hist.as<uint32_t>(),
-                                                                          counts.as<uint32_t>(), nblocks);
+      k_radix_scatter<KW, false, false, TH, PER><<<nblocks, TH, 0, ctx->stream>>>(
+          kin, iin, kout, iout, n, word, shift, hist.as<uint32_t>(), counts.as<uint32_t>(), nblocks);
     GH_LAUNCH(ctx);
-    const bool drop = drop0_last && KW >= 2 && d == digits.back();
     const uint64_t in_b = 8ull * KW + (ids ? 4 : 0), out_b = drop ? 8ull * (KW - 1) + 4 : in_b;
     ctx->sort_info.passes += 1;
     ctx->sort_info.scatter_bytes += n * (in_b + out_b);
@@ -263,6 +271,23 @@ static int radix_passes(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, 
   *result = iin;
   if (keys_result) *keys_result = kin;
   return GHIVE_OK;
+}
+
+template <int KW>
+static int radix_passes(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, uint32_t *i0, uint32_t *i1,
+                        const std::vector<int> &digits, uint32_t **result, uint64_t **keys_result = nullptr,
+                        bool ids = true, bool drop0_last = false) {
+  switch (ctx->tune.sort_shape) {
+    case 0:
+      return radix_passes_shape<KW, kSortThreads, kSortPerThread>(ctx, n, k0, k1, i0, i1, digits, result,
+                                                                  keys_result, ids, drop0_last);
+    case 2:
+      return radix_passes_shape<KW, 512, 32>(ctx, n, k0, k1, i0, i1, digits, result, keys_result, ids, drop0_last);
+    case 3:
+      return radix_passes_shape<KW, 1024, 8>(ctx, n, k0, k1, i0, i1, digits, result, keys_result, ids, drop0_last);
+    default:
+      return radix_passes_shape<KW, 512, 16>(ctx, n, k0, k1, i0, i1, digits, result, keys_result, ids, drop0_last);
+  }
 }
 
 // Stable LSD passes over the given 8-bit digits of one uint64 word per row, carrying a uint32 id

@@ -494,7 +494,11 @@ int ghive_fragment_execute_late(ghive_ctx *ctx, const ghive_fragment *frag, ghiv
  * bitmap + payload array instead of the hash table), "fragment.rows_per_thread" (4 or 8),
  * "fragment.block" (workgroup size 256, 512 or 1024), "fragment.generic" (0/1: always the general
  * fused kernel, not its dense-probe form), "fragment.pipe" (0/1: the pipelined dense-probe kernel
- * with direct group slots where the group keys' ranges allow it), "groupby.lds_budget" (bytes). */
+ * with direct group slots where the group keys' ranges allow it), "groupby.lds_budget" (bytes),
+ * "groupby.partition" (0 never, 1 by size, 2 always: the hash-partitioned high-cardinality form),
+ * "groupby.part_threads" (256, 512 or 1024: workgroup size of the partition passes; default 512),
+ * "plan.fuse" (0/1: run map-join vertices of a plan as one fused fragment), "sort.shape" (LSD tile
+ * of ghive_sort: 0 = 256 threads x 32 rows, 1 = 512 x 16 (default), 2 = 512 x 32, 3 = 1024 x 8). */
 int ghive_set_option(ghive_ctx *ctx, const char *name, int64_t value);
 int ghive_get_option(ghive_ctx *ctx, const char *name, int64_t *value);
 

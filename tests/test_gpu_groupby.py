@@ -255,8 +255,9 @@ def test_groupby_wide_keys_edges(ctx, orc):
     _compare(_as_dict(outs, 2), _oracle_dict(orc, cols, [0, 1], [(abi.AGG_CNT, abi.COUNT_STAR), (abi.AGG_SUM, 2)]))
 
 
+@pytest.mark.parametrize("threads", [256, 512, 1024])
 @pytest.mark.parametrize("aggset", ["count_star", "one_column", "two_columns"])
-def test_groupby_partitioned_value_forms(ctx, orc, aggset):
+def test_groupby_partitioned_value_forms(ctx, orc, aggset, threads):
     """The high-cardinality form reads aggregate inputs three ways: none (COUNT(*) only), one column
     carried through the partition passes, several columns read at the row id. LONG keys include -1,
     whose bits equal the EMPTY pattern (aggregated in the small global table), and NULL."""
@@ -274,6 +275,7 @@ def test_groupby_partitioned_value_forms(ctx, orc, aggset):
             "two_columns": [(abi.AGG_SUM, 1), (abi.AGG_MAX, 2), (abi.AGG_CNT, abi.COUNT_STAR)]}[aggset]
     cols = [k, x, y]
     ctx.set_option("groupby.partition", 2)
+    ctx.set_option("groupby.part_threads", threads)
     try:
         outs = _run(ctx, cols, [0], aggs)
         _compare(_as_dict(outs, 1), _oracle_dict(orc, cols, [0], aggs))
@@ -286,6 +288,7 @@ def test_groupby_partitioned_value_forms(ctx, orc, aggset):
         assert e.value.status == abi.ERR_CAPACITY
     finally:
         ctx.set_option("groupby.partition", 1)
+        ctx.set_option("groupby.part_threads", 512)
 
 
 def test_groupby_with_selection_and_empty(ctx, orc):

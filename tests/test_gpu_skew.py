@@ -43,11 +43,18 @@ def test_skewed_join_vs_oracle_then_sort(ctx, orc):
     ht.free()
 
 
-def test_high_cardinality_double_sum_vs_exact(ctx, orc):
+@pytest.mark.parametrize("threads", [256, 512, 1024])
+def test_high_cardinality_double_sum_vs_exact(ctx, orc, threads):
+    """4M rows, 1M groups through the partitioned form (one 9-bit partition pass), with either
+    workgroup size of the partition passes."""
     from ghive_amd import ops, workloads as W
     keys, vals = W.highcard_tables(ctx, rows=4_000_000, groups=1_000_000, seed=21)
-    outs = ops.GroupByPredicate([0], [(abi.AGG_CNT, abi.COUNT_STAR), (abi.AGG_SUM, 1)]).gpu_execute(
-        ctx, ops.Table([keys, vals]), capacity=1_100_000)
+    ctx.set_option("groupby.part_threads", threads)
+    try:
+        outs = ops.GroupByPredicate([0], [(abi.AGG_CNT, abi.COUNT_STAR), (abi.AGG_SUM, 1)]).gpu_execute(
+            ctx, ops.Table([keys, vals]), capacity=1_100_000)
+    finally:
+        ctx.set_option("groupby.part_threads", 512)
     k, c, s = (to_np(o) for o in outs)
     rep, v, nulls = orc.groupby([to_np(keys), to_np(vals)], [0], [(abi.AGG_CNT, abi.COUNT_STAR), (abi.AGG_SUM, 1)])
     hk = to_np(keys)[rep]

@@ -104,3 +104,23 @@ def test_sort_info_counts_the_passes_it_ran(ctx, orc):
                         "scatter_bytes": per_row * n}
     _check(ctx, orc, [np.full(n, 7, dtype=np.int64)], [0], ["-"])
     assert ops.sort_info(ctx)["passes"] == 0
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("n", [1, 8191, 8193, 16383, 16385, 300_007])
+def test_sort_tile_shapes(ctx, orc, shape, n):
+    """Every LSD tile shape (option sort.shape) gives the oracle's permutation, on sizes around
+    both tile lengths, with the row id inside the key (INT + FLOAT) and carried beside it (LONG +
+    FLOAT), and through the pass that drops word 0 (LONG + FLOAT + INT)."""
+    rng = np.random.default_rng(shape * 1000 + n % 997)
+    k = rng.integers(-3000, 3000, n).astype(np.int64)
+    f = rng.random(n).astype(np.float32)
+    f[::61] = np.nan
+    i32 = rng.integers(-5, 5, n).astype(np.int32)
+    ctx.set_option("sort.shape", shape)
+    try:
+        _check(ctx, orc, [i32, f], [0, 1], ["+", "-"])
+        _check(ctx, orc, [k, f], [0, 1], ["-", "+"])
+        _check(ctx, orc, [k, f, i32], [0, 1, 2], ["+", "+", "-"])
+    finally:
+        ctx.set_option("sort.shape", 1)
