@@ -5,7 +5,8 @@
 // sorted 8 bits at a time together with the row id. Digits that are equal in every row are
 // skipped. Each pass: per-tile LDS histogram -> scan of the digit x tile matrix -> stable scatter
 // ranked with wave64 ballots and staged through LDS in digit order, so that each digit's run is
-// written with consecutive addresses. Equal keys keep ascending row ids, so the result is a total
+// written with consecutive addresses. The scatter also writes the next pass's digit as a one-byte
+// plane, so every histogram after the first reads one byte per row instead of a key word. Equal keys keep ascending row ids, so the result is a total
 // order.
 #include <cstring>
 #include <vector>
@@ -122,6 +123,55 @@ __global__ __launch_bounds__(TH) void k_radix_hist(const uint64_t *keys, uint64_
   }
 }
 
+// The same counts from a digit plane: one byte per row, the digit this pass sorts on, written by
+// the previous pass's scatter (8x fewer bytes than the key word). The plane is padded to whole
+// tiles; bytes past n are not counted.
+template <int TH, int PER>
+__global__ __launch_bounds__(TH) void k_radix_hist_plane(const uint8_t *plane, uint64_t n, uint32_t *hist,
+                                                         uint32_t *counts, uint32_t nblocks) {
+  constexpr uint32_t kTile = (uint32_t)TH * PER;
+  __shared__ uint32_t h[256];
+  for (uint32_t d = threadIdx.x; d < 256; d += TH) h[d] = 0;
+  __syncthreads();
+  const uint64_t t0 = (uint64_t)blockIdx.x * kTile;
+  for (uint32_t c = threadIdx.x; c < kTile / 16; c += TH) {
+    const uint4 v = reinterpret_cast<const uint4 *>(plane + t0)[c];
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    const uint64_t e0 = t0 + 16ull * c;
+    const uint32_t m = e0 >= n ? 0u : (n - e0 >= 16 ? 16u : (uint32_t)(n - e0));  // rows of this lane
+    // equal neighbours are counted together (after the lower digits' passes equal keys are adjacent,
+    // and one LDS atomic per run replaces one per row); a wave whose 1024 rows all carry one digit
+    // adds them with a single atomic
+    const uint32_t d0 = w[0] & 255u;
+    const bool same = m == 16 && w[0] == d0 * 0x01010101u && w[1] == w[0] && w[2] == w[0] && w[3] == w[0];
+    const uint64_t all = ballot(true);
+    const uint32_t f = __builtin_amdgcn_readfirstlane(d0);
+    if (ballot(same && d0 == f) == all) {  // uniform across the wave
+      if (lane_id() == (uint32_t)__builtin_ctzll(all)) atomicAdd(&h[f], 16u * (uint32_t)__popcll(all));
+      continue;
+    }
+    uint32_t cur = d0, run = 0;
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+      const uint32_t d = (w[b >> 2] >> (8 * (b & 3))) & 255u;
+      if ((uint32_t)b < m) {
+        if (d != cur) {
+          atomicAdd(&h[cur], run);
+          cur = d;
+          run = 0;
+        }
+        ++run;
+      }
+    }
+    if (run) atomicAdd(&h[cur], run);
+  }
+  __syncthreads();
+  for (uint32_t d = threadIdx.x; d < 256; d += TH) {
+    hist[(uint64_t)d * nblocks + blockIdx.x] = h[d];
+    counts[(uint64_t)blockIdx.x * 256 + d] = h[d];
+  }
+}
+
 // One stable pass over a tile of TH x PER elements. (1) Every element's position within the tile
 // in digit order: wave64 ballots give the rank among equal digits of a wave, LDS counters carry it
 // across waves and iterations. (2) The tile is rewritten through a 32 KiB LDS buffer, one key
@@ -129,11 +179,14 @@ __global__ __launch_bounds__(TH) void k_radix_hist(const uint64_t *keys, uint64_
 // addresses (coalesced stores) at its global offset.
 // DROP0: the last pass over word 0, whose low 32 bits hold the row id: the later passes need only
 // the id, so word 0 leaves as a uint32 id array (idx_out) and words 1.. as a (KW - 1)-word array.
+// plane_out (optional): the next pass's digit, byte (next_shift / 8) of input word next_word, is
+// written beside every row as a one-byte plane for that pass's histogram.
 template <int KW, bool IDS, bool DROP0, int TH, int PER>
 __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, const uint32_t *idx_in,
                                                       uint64_t *keys_out, uint32_t *idx_out, uint64_t n,
                                                       int word, int shift, const uint32_t *offs,
-                                                      const uint32_t *counts, uint32_t nblocks) {
+                                                      const uint32_t *counts, uint32_t nblocks,
+                                                      uint8_t *plane_out, int next_word, int next_shift) {
   constexpr uint32_t kTile = (uint32_t)TH * PER, kWaves = TH / kWave;
   __shared__ uint64_t stage[kTile];
   __shared__ uint8_t sdigit[kTile];
@@ -228,6 +281,7 @@ __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, c
         keys_out[(uint64_t)(DROP0 ? q - 1 : q) * n + g] = stage[p];
       else
         idx_out[g] = (uint32_t)stage[p];
+      if (plane_out && q == next_word) plane_out[g] = (uint8_t)(stage[p] >> next_shift);
     }
   }
 }
@@ -235,34 +289,48 @@ __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, c
 // ids: carry the uint32 id arrays i0/i1 (false: the ids live inside the keys; i0/i1 unused).
 // drop0_last: the last pass writes word 0 as the uint32 id array (k_radix_scatter DROP0; needs
 // !ids, KW >= 2): its outputs are then *result (ids) and *keys_result (KW - 1 words).
+// planes (optional): two byte buffers of at least n rounded up to 8192 x 2 rows. *plane_cur: the
+// one that holds the first digit's plane (-1: none, the first histogram reads the keys); every
+// pass writes the next digit's plane into the other, and on return *plane_cur names the plane of
+// next_after (a digit of the same KW-word layout that a later call sorts on; -1: none).
 template <int KW, int TH, int PER>
 static int radix_passes_shape(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, uint32_t *i0, uint32_t *i1,
                               const std::vector<int> &digits, uint32_t **result, uint64_t **keys_result, bool ids,
-                              bool drop0_last) {
+                              bool drop0_last, uint8_t *const *planes, int *plane_cur, int next_after) {
   const uint32_t nblocks = ceil_div(n, (uint64_t)TH * PER);
   DevBuf hist, counts;
   GH_TRY(hist.alloc(ctx, sizeof(uint32_t) * 256ull * nblocks));
   GH_TRY(counts.alloc(ctx, sizeof(uint32_t) * 256ull * nblocks));
   uint64_t *kin = k0, *kout = k1;
   uint32_t *iin = i0, *iout = i1;
-  for (int d : digits) {
+  int cur = planes && plane_cur ? *plane_cur : -1;
+  for (size_t di = 0; di < digits.size(); ++di) {
+    const int d = digits[di];
     const int word = d / 8, shift = 8 * (d % 8);
-    k_radix_hist<TH, PER><<<nblocks, TH, 0, ctx->stream>>>(kin, n, word, shift, hist.as<uint32_t>(),
-                                                           counts.as<uint32_t>(), nblocks);
+    if (cur >= 0)
+      k_radix_hist_plane<TH, PER><<<nblocks, TH, 0, ctx->stream>>>(planes[cur], n, hist.as<uint32_t>(),
+                                                                   counts.as<uint32_t>(), nblocks);
+    else
+      k_radix_hist<TH, PER><<<nblocks, TH, 0, ctx->stream>>>(kin, n, word, shift, hist.as<uint32_t>(),
+                                                             counts.as<uint32_t>(), nblocks);
     GH_LAUNCH(ctx);
     GH_TRY(exclusive_scan_u32(ctx, hist.as<uint32_t>(), hist.as<uint32_t>(), 256ull * nblocks, nullptr));
-    const bool drop = drop0_last && KW >= 2 && d == digits.back();
+    const bool drop = drop0_last && KW >= 2 && di + 1 == digits.size();
+    const int next = di + 1 < digits.size() ? digits[di + 1] : next_after;
+    uint8_t *pout = planes && next >= 0 ? planes[cur == 0 ? 1 : 0] : nullptr;
+    const int nw = next >= 0 ? next / 8 : -1, ns = next >= 0 ? 8 * (next % 8) : 0;
     if (drop)
       k_radix_scatter<KW, false, true, TH, PER><<<nblocks, TH, 0, ctx->stream>>>(
-          kin, iin, kout, iout, n, word, shift, hist.as<uint32_t>(), counts.as<uint32_t>(), nblocks);
+          kin, iin, kout, iout, n, word, shift, hist.as<uint32_t>(), counts.as<uint32_t>(), nblocks, pout, nw, ns);
     else if (ids)
       k_radix_scatter<KW, true, false, TH, PER><<<nblocks, TH, 0, ctx->stream>>>(
-          kin, iin, kout, iout, n, word, shift, hist.as<uint32_t>(), counts.as<uint32_t>(), nblocks);
+          kin, iin, kout, iout, n, word, shift, hist.as<uint32_t>(), counts.as<uint32_t>(), nblocks, pout, nw, ns);
     else
       k_radix_scatter<KW, false, false, TH, PER><<<nblocks, TH, 0, ctx->stream>>>(
-          kin, iin, kout, iout, n, word, shift, hist.as<uint32_t>(), counts.as<uint32_t>(), nblocks);
+          kin, iin, kout, iout, n, word, shift, hist.as<uint32_t>(), counts.as<uint32_t>(), nblocks, pout, nw, ns);
     GH_LAUNCH(ctx);
-    const uint64_t in_b = 8ull * KW + (ids ? 4 : 0), out_b = drop ? 8ull * (KW - 1) + 4 : in_b;
+    cur = pout ? (cur == 0 ? 1 : 0) : -1;
+    const uint64_t in_b = 8ull * KW + (ids ? 4 : 0), out_b = (drop ? 8ull * (KW - 1) + 4 : in_b) + (pout ? 1 : 0);
     ctx->sort_info.passes += 1;
     ctx->sort_info.scatter_bytes += n * (in_b + out_b);
     std::swap(kin, kout);
@@ -270,24 +338,25 @@ static int radix_passes_shape(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t
   }
   *result = iin;
   if (keys_result) *keys_result = kin;
+  if (plane_cur) *plane_cur = cur;
   return GHIVE_OK;
 }
 
 template <int KW>
 static int radix_passes(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, uint32_t *i0, uint32_t *i1,
                         const std::vector<int> &digits, uint32_t **result, uint64_t **keys_result = nullptr,
-                        bool ids = true, bool drop0_last = false) {
+                        bool ids = true, bool drop0_last = false, uint8_t *const *planes = nullptr,
+                        int *plane_cur = nullptr, int next_after = -1) {
+#define GH_PASSES(TH, PER) \
+  radix_passes_shape<KW, TH, PER>(ctx, n, k0, k1, i0, i1, digits, result, keys_result, ids, drop0_last, planes, \
+                                  plane_cur, next_after)
   switch (ctx->tune.sort_shape) {
-    case 0:
-      return radix_passes_shape<KW, kSortThreads, kSortPerThread>(ctx, n, k0, k1, i0, i1, digits, result,
-                                                                  keys_result, ids, drop0_last);
-    case 2:
-      return radix_passes_shape<KW, 512, 32>(ctx, n, k0, k1, i0, i1, digits, result, keys_result, ids, drop0_last);
-    case 3:
-      return radix_passes_shape<KW, 1024, 8>(ctx, n, k0, k1, i0, i1, digits, result, keys_result, ids, drop0_last);
-    default:
-      return radix_passes_shape<KW, 512, 16>(ctx, n, k0, k1, i0, i1, digits, result, keys_result, ids, drop0_last);
+    case 0: return GH_PASSES(kSortThreads, kSortPerThread);
+    case 2: return GH_PASSES(512, 32);
+    case 3: return GH_PASSES(1024, 8);
+    default: return GH_PASSES(512, 16);
   }
+#undef GH_PASSES
 }
 
 // Stable LSD passes over the given 8-bit digits of one uint64 word per row, carrying a uint32 id
@@ -530,13 +599,25 @@ int ghive_sort(ghive_ctx *ctx, const ghive_table *table, const uint32_t *key_col
   // move (KW - 1) key words plus a 4-byte id instead of KW words.
   std::vector<int> d0, dhi;
   for (int d : digits) (d < 8 ? d0 : dhi).push_back(d);
+  // every pass after the first reads its histogram from a one-byte digit plane that the pass
+  // before it wrote (two planes, alternating)
+  DevBuf plane_buf;
+  uint8_t *planes[2] = {nullptr, nullptr};
+  int pcur = -1;
+  if (digits.size() > 1) {
+    const uint64_t pad = (n + 16383) / 16384 * 16384;
+    GH_TRY(plane_buf.alloc(ctx, 2 * pad));
+    planes[0] = plane_buf.as<uint8_t>();
+    planes[1] = planes[0] + pad;
+  }
   if (embed && KW >= 2 && !d0.empty() && !dhi.empty()) {
     uint32_t *ids = nullptr;
     uint64_t *keys = nullptr;
+    const int first_hi = dhi.front();  // in the KW-word layout of phase A
     switch (KW) {
-      case 2: GH_TRY(radix_passes<2>(ctx, n, K0, K1, out_perm, I1, d0, &ids, &keys, false, true)); break;
-      case 3: GH_TRY(radix_passes<3>(ctx, n, K0, K1, out_perm, I1, d0, &ids, &keys, false, true)); break;
-      default: GH_TRY(radix_passes<4>(ctx, n, K0, K1, out_perm, I1, d0, &ids, &keys, false, true)); break;
+      case 2: GH_TRY(radix_passes<2>(ctx, n, K0, K1, out_perm, I1, d0, &ids, &keys, false, true, planes, &pcur, first_hi)); break;
+      case 3: GH_TRY(radix_passes<3>(ctx, n, K0, K1, out_perm, I1, d0, &ids, &keys, false, true, planes, &pcur, first_hi)); break;
+      default: GH_TRY(radix_passes<4>(ctx, n, K0, K1, out_perm, I1, d0, &ids, &keys, false, true, planes, &pcur, first_hi)); break;
     }
     // the higher words now live at `keys` (KW - 1 words) with the ids beside them; the free key
     // buffer and the other id buffer take the next ping-pong
@@ -544,19 +625,19 @@ int ghive_sort(ghive_ctx *ctx, const ghive_table *table, const uint32_t *key_col
     uint32_t *iother = ids == out_perm ? I1 : out_perm;
     for (int &d : dhi) d -= 8;
     switch (KW) {
-      case 2: GH_TRY(radix_passes<1>(ctx, n, keys, other, ids, iother, dhi, &res, &kres, true)); break;
-      case 3: GH_TRY(radix_passes<2>(ctx, n, keys, other, ids, iother, dhi, &res, &kres, true)); break;
-      default: GH_TRY(radix_passes<3>(ctx, n, keys, other, ids, iother, dhi, &res, &kres, true)); break;
+      case 2: GH_TRY(radix_passes<1>(ctx, n, keys, other, ids, iother, dhi, &res, &kres, true, false, planes, &pcur)); break;
+      case 3: GH_TRY(radix_passes<2>(ctx, n, keys, other, ids, iother, dhi, &res, &kres, true, false, planes, &pcur)); break;
+      default: GH_TRY(radix_passes<3>(ctx, n, keys, other, ids, iother, dhi, &res, &kres, true, false, planes, &pcur)); break;
     }
     if (res != out_perm) GH_HIP(ctx, hipMemcpyAsync(out_perm, res, 4ull * n, hipMemcpyDeviceToDevice, ctx->stream));
     GH_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return GHIVE_OK;
   }
   switch (KW) {
-    case 1: GH_TRY(radix_passes<1>(ctx, n, K0, K1, out_perm, I1, digits, &res, &kres, !embed)); break;
-    case 2: GH_TRY(radix_passes<2>(ctx, n, K0, K1, out_perm, I1, digits, &res, &kres, !embed)); break;
-    case 3: GH_TRY(radix_passes<3>(ctx, n, K0, K1, out_perm, I1, digits, &res, &kres, !embed)); break;
-    default: GH_TRY(radix_passes<4>(ctx, n, K0, K1, out_perm, I1, digits, &res, &kres, !embed)); break;
+    case 1: GH_TRY(radix_passes<1>(ctx, n, K0, K1, out_perm, I1, digits, &res, &kres, !embed, false, planes, &pcur)); break;
+    case 2: GH_TRY(radix_passes<2>(ctx, n, K0, K1, out_perm, I1, digits, &res, &kres, !embed, false, planes, &pcur)); break;
+    case 3: GH_TRY(radix_passes<3>(ctx, n, K0, K1, out_perm, I1, digits, &res, &kres, !embed, false, planes, &pcur)); break;
+    default: GH_TRY(radix_passes<4>(ctx, n, K0, K1, out_perm, I1, digits, &res, &kres, !embed, false, planes, &pcur)); break;
   }
   if (embed) {  // the permutation is the low 32 bits of word 0 in sorted order
     k_low32<<<grid, 256, 0, ctx->stream>>>(kres, n, out_perm);

@@ -124,3 +124,14 @@ def test_sort_tile_shapes(ctx, orc, shape, n):
         _check(ctx, orc, [k, f, i32], [0, 1, 2], ["+", "+", "-"])
     finally:
         ctx.set_option("sort.shape", 1)
+
+
+def test_sort_info_counts_digit_planes(ctx, orc):
+    """Two varying bytes: the first pass writes the second pass's digit as a one-byte plane (one
+    byte a row more on write-out), the second histogram reads it; the order is the oracle's."""
+    from ghive_amd import ops
+    n = 70_001
+    k = np.random.default_rng(8).integers(0, 65536, n).astype(np.int64)
+    _check(ctx, orc, [k], [0], ["+"])
+    info = ops.sort_info(ctx)
+    assert info["passes"] == 2 and info["scatter_bytes"] == (24 + 1) * n + 24 * n
