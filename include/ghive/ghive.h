@@ -319,9 +319,12 @@ int ghive_agg_output_type(int32_t agg_type, int32_t input_type);
  * capacity int32) and shares the input blob (out data = input data). Output: keys then aggregates into out_cols
  * (types as ghive_agg_output_type; device buffers of `capacity` rows), groups in
  * unspecified order. An aggregate with no non-NULL input is NULL (Hive semantics).
- * SUM(LONG) wraps modulo 2^64 like a Java long. SUM/AVG(DOUBLE) are accumulated exactly
- * in a 128-bit fixed-point sum per group (order-independent, reproducible), then rounded
- * to nearest. */
+ * SUM(LONG) wraps modulo 2^64 like a Java long. SUM/AVG(DOUBLE): every value is rounded once to a
+ * multiple of 2^E, with 2^E about 2^-124 x max|x| x the row count, and these multiples are added
+ * exactly as 128-bit integers per group, so the result does not depend on row order and is
+ * reproducible bit for bit; its error is at most rows x 2^(E-1) before the final rounding to the
+ * nearest double (for 1e9 rows at most max|x| x 2^-64). A DOUBLE column with an infinite value
+ * is rejected (GHIVE_ERR_UNSUPPORTED); NaN is NULL. */
 int ghive_groupby(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel, uint64_t n,
                   const uint32_t *keys, uint32_t nkeys, const ghive_agg *aggs, uint32_t naggs,
                   ghive_column *out_cols, uint64_t capacity, uint64_t *out_groups);
@@ -396,10 +399,10 @@ typedef struct ghive_fragment_key {
 /* Aggregate over a fact expression: EXPR_COL a; EXPR_MUL a*b; EXPR_ONE counts rows. Inputs are
  * INT columns (16-byte aligned; the product is the int64 product, as SelectPredicate's LONG
  * multiply) or DOUBLE columns (both, for a product: the product rounded to a double, as Hive's
- * double multiply; read only for the rows that survive the joins). SUM over DOUBLE inputs is an
- * exact fixed-point sum of the values (LSB chosen from the columns' largest magnitudes and the row
- * count) rounded once to DOUBLE; MIN / MAX over DOUBLE give DOUBLE; every other aggregate gives
- * LONG. */
+ * double multiply; read only for the rows that survive the joins). SUM over DOUBLE inputs is the
+ * 128-bit fixed-point sum of ghive_groupby (each value rounded once to the LSB chosen from the
+ * columns' largest magnitudes and the row count, the rounded values added exactly), rounded once
+ * to DOUBLE; MIN / MAX over DOUBLE give DOUBLE; every other aggregate gives LONG. */
 typedef struct ghive_fragment_agg {
   int32_t type; /* GHIVE_AGG_SUM / CNT / MIN / MAX */
   int32_t expr;
