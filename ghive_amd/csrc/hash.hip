@@ -217,7 +217,9 @@ __global__ __launch_bounds__(kHashThreads) void k_hash_probe(
 }
 
 // Probe of a unique-key table (or the first match only, LEFT SEMI): at most one output per probe
-// row. A workgroup probes a tile of blockDim.x * R rows (row = tile base + q * blockDim.x + lane,
+// row. It takes the place of the reference's join front end, which sorts every input's row ids
+// with thrust::sort(sort_comparator) (gpu_impl/src/Operator/SortMergeJoinPredicate.cu:103-115)
+// before a CPU merge: here the build side is hashed once and each probe row is one table lookup. A workgroup probes a tile of blockDim.x * R rows (row = tile base + q * blockDim.x + lane,
 // coalesced key loads), counts its matches with wave ballots and one LDS scan, and reserves its
 // output range with ONE atomic per tile -- a per-wave append would put ~n/64 device-scope
 // atomics on a single counter, which serialise across the XCDs.

@@ -1,6 +1,6 @@
 // ORDER BY as a stable LSD radix sort (replaces SinkOperator's thrust::sort of row ids with a
 // type-generic comparator plus a CPU permutation gather, gpu_impl/src/Operator/SinkOperator.cu:
-// 135-288). Keys are turned into one order-preserving unsigned bit string per row (first key in
+// 135-288; the sort itself is thrust::sequence + thrust::sort(sort_comparator) at :218-222). Keys are turned into one order-preserving unsigned bit string per row (first key in
 // the most significant bits; DESC inverts its field; NaN is canonical and sorts after +inf), then
 // sorted 8 bits at a time together with the row id. Digits that are equal in every row are
 // skipped. Each pass: per-tile LDS histogram -> scan of the digit x tile matrix -> stable scatter
