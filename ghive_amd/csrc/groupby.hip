@@ -636,10 +636,16 @@ __global__ __launch_bounds__(kGbPartThreads) void k_gb_part_emit(KeySpec K, Grou
     }
     if (threadIdx.x == 0) base = atomicAdd(out_count, (unsigned long long)used);
     __syncthreads();
-    for (uint32_t s = threadIdx.x; s < lcap; s += blockDim.x) {
-      const uint64_t h = lds[s];
+    for (uint32_t s0 = 0; s0 < lcap; s0 += blockDim.x) {  // the same trip count in every lane
+      const uint32_t s = s0 + threadIdx.x;
+      const uint64_t h = s < lcap ? lds[s] : kEmptyKey;
+      // output rows: one LDS atomic per wave, ranks from the ballot
+      const uint64_t m = ballot(h != kEmptyKey);
+      uint32_t wbase = 0;
+      if (lane_id() == 0 && m) wbase = atomicAdd(&wpos, (uint32_t)__popcll(m));
+      wbase = __shfl(wbase, 0, kWave);
       if (h == kEmptyKey) continue;
-      const uint64_t idx = base + atomicAdd(&wpos, 1u);
+      const uint64_t idx = base + wbase + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
       if (idx >= cap) continue;
       emit_key(K, O, unmix64(h), idx);
       for (int a = 0; a < T.naggs; ++a) {
@@ -759,10 +765,36 @@ __global__ __launch_bounds__(TH) void k_gbp_hist(GbpSrc S, uint64_t n, int shift
   }
 }
 
+// An 8-bit pass's counts from the digit plane the previous pass wrote (one byte per row instead of
+// the 8-byte hash). The plane is padded to whole 8192-row tiles; bytes past n are not counted.
+template <int TH>
+__global__ __launch_bounds__(TH) void k_gbp_hist_plane(const uint8_t *plane, uint64_t n, uint32_t *hist,
+                                                       uint32_t *counts, uint32_t nblocks) {
+  constexpr uint32_t kTile = Gbp<8, TH>::kTile;
+  __shared__ uint32_t hc[256];
+  for (int b = threadIdx.x; b < 256; b += TH) hc[b] = 0;
+  __syncthreads();
+  const uint64_t t0 = (uint64_t)blockIdx.x * kTile;
+  for (uint32_t c = threadIdx.x; c < kTile / 16; c += TH) {
+    const uint4 v = reinterpret_cast<const uint4 *>(plane + t0)[c];
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    const uint64_t e0 = t0 + 16ull * c;
+#pragma unroll
+    for (int b = 0; b < 16; ++b)
+      if (e0 + b < n) atomicAdd(&hc[(w[b >> 2] >> (8 * (b & 3))) & 255u], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < 256; b += TH) {
+    hist[(uint64_t)b * nblocks + blockIdx.x] = hc[b];
+    counts[(uint64_t)blockIdx.x * 256 + b] = hc[b];
+  }
+}
+
+// plane_out (optional): the next pass's 8-bit digit, (h >> next_shift) & 255, as a byte per row.
 template <int DB, int SRC, bool VAL, int TH>
 __global__ __launch_bounds__(TH) void k_gbp_scatter(GbpSrc S, uint64_t n, int shift, const uint32_t *offs,
                                                     const uint32_t *counts, uint32_t nblocks, uint64_t *h_out,
-                                                    uint64_t *v_out) {
+                                                    uint64_t *v_out, uint8_t *plane_out, int next_shift) {
   constexpr int B = Gbp<DB, TH>::kBins, kPer = Gbp<DB, TH>::kPer, kWaves = TH / kWave, kBpt = Gbp<DB, TH>::kBpt;
   constexpr uint32_t kTile = Gbp<DB, TH>::kTile, kSeg = (uint32_t)kPer * kWave;
   typedef typename Gbp<DB, TH>::Digit Digit;
@@ -870,7 +902,9 @@ __global__ __launch_bounds__(TH) void k_gbp_scatter(GbpSrc S, uint64_t n, int sh
     uint64_t *out = q ? v_out : h_out;
     for (uint32_t p = tid; p < tn; p += TH) {
       const uint32_t d = sdigit[p];
-      out[(uint64_t)goff[d] + (p - lbase[d])] = stage[p];
+      const uint64_t g = (uint64_t)goff[d] + (p - lbase[d]);
+      out[g] = stage[p];
+      if (plane_out && q == 0) plane_out[g] = (uint8_t)(stage[p] >> next_shift);
     }
   }
 }
@@ -897,16 +931,30 @@ static int gbp_partition_th(ghive_ctx *ctx, const GbpSrc &src, uint64_t n, int p
   // every row: the plain-word first pass
   const bool fast = !src.sel && src.K.nkeys == 1 && src.K.shift[0] == 0 && src.K.type[0] == GHIVE_LONG &&
                     (!val || src.vtype == GHIVE_LONG || src.vtype == GHIVE_DOUBLE);
-  uint64_t *hin = nullptr, *vin = nullptr, *hout = h0, *vout = v0;
+  // each pass's digit width and shift: a narrower digit runs as an 8-bit one over its bits; a digit
+  // narrower than DB bits must not reach past bit 63, so it is shifted to end at 63 when it is the
+  // last one, else it takes DB bits starting at its low bit (the extra high bits are the next
+  // digit's, which the later, stable passes order anyway)
+  std::vector<int> dbs, shs;
   int lo = 64 - (pbits < 8 ? 8 : pbits);
+  for (int w : widths) {
+    const int db = w > 8 ? 9 : 8;
+    dbs.push_back(db);
+    shs.push_back(lo + db > 64 ? 64 - db : lo);
+    lo += w;
+  }
+  // an 8-bit pass after the first reads its counts from a byte plane the pass before it wrote
+  DevBuf plane_buf;
+  for (size_t p = 1; p < dbs.size() && !plane_buf.get(); ++p)
+    if (dbs[p] == 8) GH_TRY(plane_buf.alloc(ctx, (n + 8191) / 8192 * 8192));
+  uint64_t *hin = nullptr, *vin = nullptr, *hout = h0, *vout = v0;
+  bool have_plane = false;
   for (size_t p = 0; p < widths.size(); ++p) {
     const bool first = p == 0;
-    const int db = widths[p] > 8 ? 9 : 8;  // a narrower digit runs as an 8-bit one over its bits
-    const int shift = lo;
-    // a digit narrower than DB bits must not reach past bit 63: shift so that it ends at 63 when it
-    // is the last one, else take DB bits starting at lo (the extra high bits are the next digit's,
-    // which the later, stable passes order anyway)
-    const int sh = shift + db > 64 ? 64 - db : shift;
+    const int db = dbs[p], sh = shs[p];
+    const bool write_plane = p + 1 < dbs.size() && dbs[p + 1] == 8;
+    uint8_t *pout = write_plane ? plane_buf.as<uint8_t>() : nullptr;
+    const int nsh = write_plane ? shs[p + 1] : 0;
     const uint32_t tile = db == 9 ? Gbp<9, TH>::kTile : Gbp<8, TH>::kTile;
     const uint32_t nb = ceil_div(n, tile);
     const uint32_t bins = db == 9 ? 512 : 256;
@@ -916,7 +964,10 @@ static int gbp_partition_th(ghive_ctx *ctx, const GbpSrc &src, uint64_t n, int p
     }
     const int src = !first ? kSrcPass : (fast ? kSrcCol64 : kSrcCols);
 #define GBP_HIST(D, F) k_gbp_hist<D, F, TH><<<nb, TH, 0, ctx->stream>>>(S, n, sh, hist.as<uint32_t>(), counts.as<uint32_t>(), nb)
-    if (db == 9) {
+    if (have_plane && db == 8) {
+      k_gbp_hist_plane<TH><<<nb, TH, 0, ctx->stream>>>(plane_buf.as<uint8_t>(), n, hist.as<uint32_t>(),
+                                                       counts.as<uint32_t>(), nb);
+    } else if (db == 9) {
       if (src == kSrcCols) GBP_HIST(9, kSrcCols); else if (src == kSrcCol64) GBP_HIST(9, kSrcCol64); else GBP_HIST(9, kSrcPass);
     } else {
       if (src == kSrcCols) GBP_HIST(8, kSrcCols); else if (src == kSrcCol64) GBP_HIST(8, kSrcCol64); else GBP_HIST(8, kSrcPass);
@@ -926,7 +977,7 @@ static int gbp_partition_th(ghive_ctx *ctx, const GbpSrc &src, uint64_t n, int p
     GH_TRY(exclusive_scan_u32(ctx, hist.as<uint32_t>(), hist.as<uint32_t>(), (uint64_t)bins * nb, nullptr));
 #define GBP_SCATTER(D, F, V)                                                                                   \
   k_gbp_scatter<D, F, V, TH><<<nb, TH, 0, ctx->stream>>>(S, n, sh, hist.as<uint32_t>(), counts.as<uint32_t>(), nb, \
-                                                         hout, vout)
+                                                         hout, vout, pout, nsh)
     if (db == 9) {
       if (src == kSrcCols) { if (val) GBP_SCATTER(9, kSrcCols, true); else GBP_SCATTER(9, kSrcCols, false); }
       else if (src == kSrcCol64) { if (val) GBP_SCATTER(9, kSrcCol64, true); else GBP_SCATTER(9, kSrcCol64, false); }
@@ -938,7 +989,7 @@ static int gbp_partition_th(ghive_ctx *ctx, const GbpSrc &src, uint64_t n, int p
     }
 #undef GBP_SCATTER
     GH_LAUNCH(ctx);
-    lo += widths[p];
+    have_plane = write_plane;
     hin = hout;
     vin = vout;
     hout = hin == h0 ? h1 : h0;

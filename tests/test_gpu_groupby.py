@@ -291,6 +291,30 @@ def test_groupby_partitioned_value_forms(ctx, orc, aggset, threads):
         ctx.set_option("groupby.part_threads", 512)
 
 
+@pytest.mark.parametrize("threads", [256, 512])
+def test_groupby_two_partition_passes(ctx, threads):
+    """A group bound of 3M puts 2^11 partitions behind two 8-bit partition passes, the second of
+    which counts its digits from the byte plane the first one wrote. 5M rows over 1.5M keys:
+    COUNT(*) and SUM(LONG) exact against numpy, every key present once."""
+    rng = np.random.default_rng(threads)
+    n = 5_000_000
+    k = rng.integers(-2**40, 2**40, 1_500_000)[rng.integers(0, 1_500_000, n)].astype(np.int64)
+    y = rng.integers(-10**9, 10**9, n).astype(np.int64)
+    ctx.set_option("groupby.part_threads", threads)
+    try:
+        outs = _run(ctx, [k, y], [0], [(abi.AGG_CNT, abi.COUNT_STAR), (abi.AGG_SUM, 1)], capacity=3_000_000)
+    finally:
+        ctx.set_option("groupby.part_threads", 512)
+    uk, inv = np.unique(k, return_inverse=True)
+    cnt = np.bincount(inv)
+    sm = np.zeros(len(uk), dtype=np.int64)
+    np.add.at(sm, inv, y)
+    order = np.argsort(outs[0])
+    assert np.array_equal(outs[0][order], uk)
+    assert np.array_equal(outs[1][order], cnt)
+    assert np.array_equal(outs[2][order], sm)
+
+
 def test_groupby_with_selection_and_empty(ctx, orc):
     rng = np.random.default_rng(2)
     n = 100_000
