@@ -791,10 +791,14 @@ __global__ __launch_bounds__(TH) void k_gbp_hist_plane(const uint8_t *plane, uin
 }
 
 // plane_out (optional): the next pass's 8-bit digit, (h >> next_shift) & 255, as a byte per row.
+// part_max (optional, a first pass that reads the value column): the workgroup's max |x| over its
+// rows' non-NULL values, as the bits of a non-negative double, at part_max[blockIdx.x] -- the
+// exact SUM's scale comes from these without a pass of its own over the column.
 template <int DB, int SRC, bool VAL, int TH>
 __global__ __launch_bounds__(TH) void k_gbp_scatter(GbpSrc S, uint64_t n, int shift, const uint32_t *offs,
                                                     const uint32_t *counts, uint32_t nblocks, uint64_t *h_out,
-                                                    uint64_t *v_out, uint8_t *plane_out, int next_shift) {
+                                                    uint64_t *v_out, uint8_t *plane_out, int next_shift,
+                                                    unsigned long long *part_max) {
   constexpr int B = Gbp<DB, TH>::kBins, kPer = Gbp<DB, TH>::kPer, kWaves = TH / kWave, kBpt = Gbp<DB, TH>::kBpt;
   constexpr uint32_t kTile = Gbp<DB, TH>::kTile, kSeg = (uint32_t)kPer * kWave;
   typedef typename Gbp<DB, TH>::Digit Digit;
@@ -849,6 +853,29 @@ __global__ __launch_bounds__(TH) void k_gbp_scatter(GbpSrc S, uint64_t n, int sh
   if (SRC == kSrcCol64) {
 #pragma unroll
     for (int j = 0; j < kPer; ++j) hv[j] = mix64(hv[j]);
+  }
+  if (VAL && SRC != kSrcPass && part_max) {  // uniform: a kernel argument
+    __shared__ unsigned long long wmax[kWaves];
+    unsigned long long m = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      double d;
+      if (wave * kSeg + (uint32_t)j * kWave + lane < tn && bits_f64(vv[j], S.vtype, d)) {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(d));
+        m = b > m ? b : m;
+      }
+    }
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+      const unsigned long long o = __shfl_xor(m, d, kWave);
+      m = o > m ? o : m;
+    }
+    if (lane == 0) wmax[wave] = m;
+    __syncthreads();
+    if (tid == 0) {
+      for (int w = 1; w < kWaves; ++w) m = wmax[w] > m ? wmax[w] : m;
+      part_max[blockIdx.x] = m;
+    }
   }
   __syncthreads();
   const uint64_t lt = (1ull << lane) - 1ull;
@@ -914,7 +941,8 @@ __global__ __launch_bounds__(TH) void k_gbp_scatter(GbpSrc S, uint64_t n, int sh
 // buffers *h_res / *v_res.
 template <int TH>
 static int gbp_partition_th(ghive_ctx *ctx, const GbpSrc &src, uint64_t n, int pbits, bool val, uint64_t *h0,
-                         uint64_t *v0, uint64_t *h1, uint64_t *v1, uint64_t **h_res, uint64_t **v_res) {
+                         uint64_t *v0, uint64_t *h1, uint64_t *v1, uint64_t **h_res, uint64_t **v_res,
+                         unsigned long long *part_max) {
   std::vector<int> widths;  // digit widths, low digit first (one 8-bit pass at least)
   for (int left = pbits < 8 ? 8 : pbits; left > 0;) {
     const int passes = (left + 8) / 9;
@@ -955,6 +983,7 @@ static int gbp_partition_th(ghive_ctx *ctx, const GbpSrc &src, uint64_t n, int p
     const bool write_plane = p + 1 < dbs.size() && dbs[p + 1] == 8;
     uint8_t *pout = write_plane ? plane_buf.as<uint8_t>() : nullptr;
     const int nsh = write_plane ? shs[p + 1] : 0;
+    unsigned long long *pmax = first ? part_max : nullptr;
     const uint32_t tile = db == 9 ? Gbp<9, TH>::kTile : Gbp<8, TH>::kTile;
     const uint32_t nb = ceil_div(n, tile);
     const uint32_t bins = db == 9 ? 512 : 256;
@@ -977,7 +1006,7 @@ static int gbp_partition_th(ghive_ctx *ctx, const GbpSrc &src, uint64_t n, int p
     GH_TRY(exclusive_scan_u32(ctx, hist.as<uint32_t>(), hist.as<uint32_t>(), (uint64_t)bins * nb, nullptr));
 #define GBP_SCATTER(D, F, V)                                                                                   \
   k_gbp_scatter<D, F, V, TH><<<nb, TH, 0, ctx->stream>>>(S, n, sh, hist.as<uint32_t>(), counts.as<uint32_t>(), nb, \
-                                                         hout, vout, pout, nsh)
+                                                         hout, vout, pout, nsh, pmax)
     if (db == 9) {
       if (src == kSrcCols) { if (val) GBP_SCATTER(9, kSrcCols, true); else GBP_SCATTER(9, kSrcCols, false); }
       else if (src == kSrcCol64) { if (val) GBP_SCATTER(9, kSrcCol64, true); else GBP_SCATTER(9, kSrcCol64, false); }
@@ -1000,12 +1029,15 @@ static int gbp_partition_th(ghive_ctx *ctx, const GbpSrc &src, uint64_t n, int p
   return GHIVE_OK;
 }
 
+// part_max (optional): filled by the first pass, one entry per workgroup of it (at most
+// ceil(n / 4096) entries), see k_gbp_scatter.
 static int gbp_partition(ghive_ctx *ctx, const GbpSrc &src, uint64_t n, int pbits, bool val, uint64_t *h0,
-                         uint64_t *v0, uint64_t *h1, uint64_t *v1, uint64_t **h_res, uint64_t **v_res) {
+                         uint64_t *v0, uint64_t *h1, uint64_t *v1, uint64_t **h_res, uint64_t **v_res,
+                         unsigned long long *part_max) {
   switch (ctx->tune.gb_part_threads) {
-    case 256: return gbp_partition_th<256>(ctx, src, n, pbits, val, h0, v0, h1, v1, h_res, v_res);
-    case 1024: return gbp_partition_th<1024>(ctx, src, n, pbits, val, h0, v0, h1, v1, h_res, v_res);
-    default: return gbp_partition_th<512>(ctx, src, n, pbits, val, h0, v0, h1, v1, h_res, v_res);
+    case 256: return gbp_partition_th<256>(ctx, src, n, pbits, val, h0, v0, h1, v1, h_res, v_res, part_max);
+    case 1024: return gbp_partition_th<1024>(ctx, src, n, pbits, val, h0, v0, h1, v1, h_res, v_res, part_max);
+    default: return gbp_partition_th<512>(ctx, src, n, pbits, val, h0, v0, h1, v1, h_res, v_res, part_max);
   }
 }
 
@@ -1013,9 +1045,27 @@ static int gbp_partition(ghive_ctx *ctx, const GbpSrc &src, uint64_t n, int pbit
 // slots and written straight to the output columns O (rows in unspecified order). T carries the
 // aggregate specs. *failed: a partition held more groups than its LDS table (nothing usable was
 // written; use the direct form). *groups = groups found (CAPACITY when above capacity).
-int groupby_partitioned(ghive_ctx *ctx, const KeySpec &K, const GroupTableDev &Tspec, const uint32_t *sel,
+// The largest of n partial maxima (bits of non-negative doubles) into *out (one workgroup).
+__global__ __launch_bounds__(256) void k_max_u64(const unsigned long long *part, uint64_t n, unsigned long long *out) {
+  __shared__ unsigned long long wm[256 / kWave];
+  unsigned long long m = 0;
+  for (uint64_t i = threadIdx.x; i < n; i += 256) m = part[i] > m ? part[i] : m;
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    const unsigned long long o = __shfl_xor(m, d, kWave);
+    m = o > m ? o : m;
+  }
+  if (lane_id() == 0) wm[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 256 / kWave; ++w) m = wm[w] > m ? wm[w] : m;
+    *out = m;
+  }
+}
+
+int groupby_partitioned(ghive_ctx *ctx, const KeySpec &K, GroupTableDev &Tspec, const uint32_t *sel,
                         uint64_t n, uint32_t lcap, int pbits, const OutSpec &O, uint64_t capacity, bool *failed,
-                        uint64_t *groups) {
+                        uint64_t *groups, bool scale_pending) {
   *failed = false;
   *groups = 0;
   // the aggregate inputs: none (COUNT(*) only), one column carried through the passes, or several
@@ -1038,8 +1088,27 @@ int groupby_partitioned(ghive_ctx *ctx, const KeySpec &K, const GroupTableDev &T
       vdata = Tspec.aggs[a].data;
       vtype = Tspec.aggs[a].in_type;
     }
+  // exact double sums not yet scaled: over the carried column, max |x| comes out of the first
+  // partition pass; otherwise it is read here
+  bool fused_absmax = false;
+  for (int a = 0; a < Tspec.naggs && scale_pending; ++a) {
+    AggSpec &A = Tspec.aggs[a];
+    if (A.kind != AK_SUMD) continue;
+    if (val == 1 && A.col == vcol) {
+      fused_absmax = true;
+      continue;
+    }
+    double mx = 0;
+    GH_TRY(column_absmax(ctx, A.data, A.in_type, sel, n, &mx));
+    GH_TRY(sum_scale(ctx, mx, n, &A.scale_e));
+  }
   const int KW = val == 1 ? 2 : 1;
-  DevBuf k0, k1, i0, i1, starts, sk, sa, sc;
+  DevBuf k0, k1, i0, i1, starts, sk, sa, sc, pmax;
+  const uint64_t npmax = (n + 4095) / 4096;  // the first pass's workgroups, at most
+  if (fused_absmax) {
+    GH_TRY(pmax.alloc(ctx, 8 * npmax));
+    GH_TRY(fill_u64(ctx, pmax.as<uint64_t>(), 0, npmax));
+  }
   GH_TRY(k0.alloc(ctx, 8ull * KW * n));
   GH_TRY(k1.alloc(ctx, 8ull * KW * n));
   if (val == 2) {
@@ -1057,7 +1126,21 @@ int groupby_partitioned(ghive_ctx *ctx, const KeySpec &K, const GroupTableDev &T
     S.vdata = vdata;
     S.vtype = vtype;
     GH_TRY(gbp_partition(ctx, S, n, pbits, val == 1, k0.as<uint64_t>(), val == 1 ? k0.as<uint64_t>() + n : nullptr,
-                         k1.as<uint64_t>(), val == 1 ? k1.as<uint64_t>() + n : nullptr, &ks, &vs_part));
+                         k1.as<uint64_t>(), val == 1 ? k1.as<uint64_t>() + n : nullptr, &ks, &vs_part,
+                         fused_absmax ? reinterpret_cast<unsigned long long *>(pmax.get()) : nullptr));
+    if (fused_absmax) {
+      k_max_u64<<<1, 256, 0, ctx->stream>>>(reinterpret_cast<const unsigned long long *>(pmax.get()), npmax,
+                                            reinterpret_cast<unsigned long long *>(ctx->d_counters));
+      GH_LAUNCH(ctx);
+      uint64_t mb = 0;
+      GH_TRY(d2h(ctx, &mb, ctx->d_counters, sizeof mb));
+      double mx = 0;
+      memcpy(&mx, &mb, sizeof mx);
+      int32_t e = 0;
+      GH_TRY(sum_scale(ctx, mx, n, &e));
+      for (int a = 0; a < Tspec.naggs; ++a)
+        if (Tspec.aggs[a].kind == AK_SUMD) Tspec.aggs[a].scale_e = e;
+    }
   } else {  // several value columns: (hash, row id) pairs, the values read at the row id
     const uint64_t g = (n + 255) / 256, cap = (uint64_t)ctx->cus * 16;
     k_gb_part_keys<<<(uint32_t)(g > cap ? cap : g), 256, 0, ctx->stream>>>(K, sel, n, k0.as<uint64_t>(),
@@ -1249,8 +1332,13 @@ int ghive_groupby(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel,
     o.type = WK.type[k];
   }
   T.nwords = word;
-  // exact double sums: pick the fixed-point LSB from max |x| and the row count
-  for (uint32_t a = 0; a < naggs; ++a) {
+  // the high-cardinality form: a large group bound over many rows (or forced by the option)
+  const int64_t part_opt = ctx->tune.gb_partition;
+  const bool want_part = nkeys > 0 && n && part_opt != 0 &&
+                         (part_opt == 2 || (n >= (1ull << 22) && capacity >= (1ull << 20)));
+  // exact double sums: pick the fixed-point LSB from max |x| and the row count (the partitioned
+  // form does this itself, from its first pass where it can)
+  for (uint32_t a = 0; a < naggs && !want_part; ++a) {
     AggSpec &A = T.aggs[a];
     if (A.kind != AK_SUMD) continue;
     double mx = 0;
@@ -1294,12 +1382,9 @@ int ghive_groupby(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel,
     }
   }
   DevBuf dkeys, dacc, dcnt;
-  // the high-cardinality form: a large group bound over many rows (or forced by the option)
-  const int64_t part_opt = ctx->tune.gb_partition;
   bool partitioned = false;
   int st = GHIVE_OK;
-  if (nkeys > 0 && n && part_opt != 0 &&
-      (part_opt == 2 || (n >= (1ull << 22) && capacity >= (1ull << 20)))) {
+  if (want_part) {
     // LDS table slots per partition: a multiple of 512 that fits 150 KiB; partitions so that the
     // group bound fills each table to at most 60% on average (the emit falls back to the direct form
     // if a partition ever exceeds 7/8)
@@ -1308,7 +1393,7 @@ int ghive_groupby(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel,
     int pbits = 0;
     while (pbits < 24 && (double)capacity > 0.6 * (double)lcap * (double)(1ull << pbits)) ++pbits;
     bool failed = false;
-    st = groupby_partitioned(ctx, K, T, sel, n, lcap, pbits, O, capacity, &failed, out_groups);
+    st = groupby_partitioned(ctx, K, T, sel, n, lcap, pbits, O, capacity, &failed, out_groups, /*scale_pending=*/true);
     if (st != GHIVE_OK && st != GHIVE_ERR_CAPACITY) return st;
     partitioned = !failed;
   }

@@ -315,6 +315,28 @@ def test_groupby_two_partition_passes(ctx, threads):
     assert np.array_equal(outs[2][order], sm)
 
 
+@pytest.mark.parametrize("partition", [1, 2], ids=["direct", "partitioned"])
+def test_double_sum_rejects_infinity(ctx, partition):
+    """SUM over a DOUBLE column holding +-Inf has no fixed-point scale: both forms reject it with
+    UNSUPPORTED (the partitioned form finds the magnitude inside its first pass), and the context
+    keeps working."""
+    from ghive_amd._lib import GhiveError
+    n = 50_000
+    k = np.arange(n, dtype=np.int64) % 7919
+    x = np.random.default_rng(3).standard_normal(n)
+    x[12345] = -np.inf
+    ctx.set_option("groupby.partition", partition)
+    try:
+        with pytest.raises(GhiveError) as e:
+            _run(ctx, [k, x], [0], [(abi.AGG_SUM, 1)], capacity=n)
+        assert e.value.status == abi.ERR_UNSUPPORTED
+        x[12345] = 1.0
+        outs = _run(ctx, [k, x], [0], [(abi.AGG_SUM, 1), (abi.AGG_CNT, abi.COUNT_STAR)], capacity=n)
+        assert len(outs[0]) == 7919 and int(outs[2].sum()) == n
+    finally:
+        ctx.set_option("groupby.partition", 1)
+
+
 def test_groupby_with_selection_and_empty(ctx, orc):
     rng = np.random.default_rng(2)
     n = 100_000

@@ -161,3 +161,25 @@ def test_serve_error_does_not_stop_the_loop(ctx):
         assert _groups(srv.wait(good), 1) == want
     finally:
         srv.close()
+
+
+def test_async_built_table_used_first_by_a_server_slot(ctx):
+    """A dimension table built asynchronously on one context (filter count left on the device) and
+    used first by a server slot, another context on its own stream: the slot finishes the build
+    from the owner's stream, and the fragment's groups equal those of a synchronously built table."""
+    from ghive_amd import ops, workloads as W
+    T = W.small_tables(ctx, 2_000_000)
+    pred = ops.FilterPredicate([abi.make_pred(abi.FILTER_EQ, 1, [3], data_type=2)], 0)
+    sel, cnt = pred.process_async(ctx, T.dim)
+    ha = ops.HashTable(ctx, T.dim.columns[0], sel=sel, payload=T.dim.columns[2], count=cnt)
+    hs = W.small_dim_table(ctx, T.dim)
+    want = _groups(W.small_fragment(T.fact, hs).execute(ctx, capacity=64), 1)
+    srv = ops.Server(slots=2, hbm_budget=8 << 30, slot_mem_limit=2 << 30)
+    try:
+        t = srv.submit_fragment(W.small_fragment(T.fact, ha), capacity=64)
+        got = _groups(srv.wait(t), 1)
+    finally:
+        srv.close()
+    assert got == want and len(want) > 1
+    ha.free()
+    hs.free()
