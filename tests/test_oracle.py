@@ -581,3 +581,76 @@ def test_fragment_c1_closed_form(orc):
         got = {int(k[0]): int(v[0]) for k, v in zip(r["keys"], r["aggs"])}
         assert got == want
         assert r["joined"] == int(ok.sum())
+
+
+def test_fragment_star_three_joins_two_keys_brute_force(orc):
+    """The fragment path of the 600M-row star query (three filtered map joins, group keys from the
+    first and third dimension payloads, SUM(price * discount)) plus a fact filter, COUNT(*), MAX,
+    MIN and NULLs, on sparse shuffled dimension keys, against a dictionary join written here."""
+    rng = np.random.default_rng(7)
+    n = 6000
+
+    def dim(rows, key_space):
+        key = (rng.choice(key_space, size=rows, replace=False) + 1).astype(np.int32)
+        return key
+
+    dk, sk, pk = dim(300, 5000), dim(800, 40000), dim(1500, 90000)
+    year = rng.integers(1992, 1999, dk.size).astype(np.int32)
+    region = rng.integers(0, 5, sk.size).astype(np.int32)
+    pcat = rng.integers(0, 25, pk.size).astype(np.int32)
+    brand = rng.integers(0, 40, pk.size).astype(np.int32)
+    pcat[:200] = 1                        # enough part rows pass the category filter
+
+    def fks(keys, space):
+        f = np.where(rng.random(n) < 0.85, rng.choice(keys, n), rng.integers(1, space + 2, n)).astype(np.int32)
+        f[rng.random(n) < 0.03] = INT_NULL
+        return f
+
+    f_date, f_supp, f_part = fks(dk, 5000), fks(sk, 40000), fks(pk, 90000)
+    qty = rng.integers(1, 51, n).astype(np.int32)
+    price = rng.integers(100_000, 10_000_001, n).astype(np.int32)
+    disc = rng.integers(0, 11, n).astype(np.int32)
+    price[rng.random(n) < 0.02] = INT_NULL
+    disc[rng.random(n) < 0.02] = INT_NULL
+    fact = [f_date, f_supp, f_part, qty, price, disc]
+    P = abi.make_pred
+    spec = orc.FragmentSpec(
+        fact,
+        [dict(columns=[dk, year], key_col=0, payload_col=1, fact_key_col=0,
+              filter=([P(abi.FILTER_EQ_RANGE, 1, [1993, 1996], data_type=2)], 0)),
+         dict(columns=[sk, region], key_col=0, payload_col=1, fact_key_col=1,
+              filter=([P(abi.FILTER_EQ, 1, [1], data_type=2)], 0)),
+         dict(columns=[pk, pcat, brand], key_col=0, payload_col=2, fact_key_col=2,
+              filter=([P(abi.FILTER_EQ, 1, [1], data_type=2)], 0))],
+        keys=[(abi.SRC_JOIN, 0), (abi.SRC_JOIN, 2)],
+        aggs=[(abi.AGG_SUM, abi.EXPR_MUL, 4, 5), (abi.AGG_CNT, abi.EXPR_ONE, 0, 0),
+              (abi.AGG_MAX, abi.EXPR_COL, 4, 0), (abi.AGG_MIN, abi.EXPR_COL, 5, 0)],
+        fact_filter=([P(abi.FILTER_LT, 3, [25], data_type=2)], 0))
+    dmap = {int(k): int(y) for k, y in zip(dk, year) if 1993 <= y <= 1996}
+    smap = {int(k): int(r) for k, r in zip(sk, region) if r == 1}
+    pmap = {int(k): int(b) for k, c, b in zip(pk, pcat, brand) if c == 1}
+    groups, joined = {}, 0
+    for i in range(n):
+        if not qty[i] < 25:
+            continue
+        a, b, c = int(f_date[i]), int(f_supp[i]), int(f_part[i])
+        if a not in dmap or b not in smap or c not in pmap:
+            continue
+        joined += 1
+        g = groups.setdefault((dmap[a], pmap[c]), [0, 0, None, None])
+        if price[i] != INT_NULL and disc[i] != INT_NULL:
+            g[0] += int(price[i]) * int(disc[i])
+        g[1] += 1
+        if price[i] != INT_NULL:
+            g[2] = int(price[i]) if g[2] is None else max(g[2], int(price[i]))
+        if disc[i] != INT_NULL:
+            g[3] = int(disc[i]) if g[3] is None else min(g[3], int(disc[i]))
+    assert joined > 20 and len(groups) > 10
+    for nt in (1, 3):
+        r = spec.run(nthreads=nt)
+        assert r["joined"] == joined
+        got = {}
+        for k, v, z in zip(r["keys"], r["aggs"], r["nulls"]):
+            got[(int(k[0]), int(k[1]))] = [int(v[0]), int(v[1]), None if z[2] else int(v[2]),
+                                           None if z[3] else int(v[3])]
+        assert got == groups
