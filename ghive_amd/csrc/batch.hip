@@ -394,13 +394,13 @@ __device__ __forceinline__ void batch_sub_plain(const BatchDev &B, const BatchIt
     for (int q = 0; q < 4; ++q)
       if ((alive >> q) & 1u) pay[q] = B.dense[(uint32_t)((int64_t)k[q] - B.key_min)];
   } else {
+    bool live[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (!((alive >> q) & 1u)) continue;
-      int32_t p = 0;
-      if (!probe_unique8(B.slots, B.mask, B.log2, k[q], p)) alive &= ~(1u << q);
-      pay[q] = p;
-    }
+    for (int q = 0; q < 4; ++q) live[q] = (alive >> q) & 1u;
+    probe_unique8_rows<4>(B.slots, B.mask, B.log2, k, live, pay);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (!live[q]) alive &= ~(1u << q);
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {

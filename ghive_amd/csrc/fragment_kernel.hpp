@@ -206,14 +206,8 @@ __device__ __forceinline__ void frag_tile(const FragDev &F, const GroupTableDev 
       for (int q = 0; q < R; ++q) alive[q] = alive[q] && (w[q] & 1u);
     } else {
 #pragma unroll
-      for (int q = 0; q < R; ++q) {
-        int32_t p = 0;
-        if (alive[q]) {
-          const int32_t k = fk[j][q];
-          alive[q] = (uint32_t)k - kmin <= span && probe_unique8(J.slots, J.mask, J.log2, k, p);
-        }
-        pay[j][q] = p;
-      }
+      for (int q = 0; q < R; ++q) alive[q] = alive[q] && (uint32_t)fk[j][q] - kmin <= span;
+      probe_unique8_rows<R>(J.slots, J.mask, J.log2, fk[j], alive, pay[j]);
     }
   }
   uint32_t mask = 0;

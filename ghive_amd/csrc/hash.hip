@@ -219,14 +219,22 @@ __global__ __launch_bounds__(kHashThreads) void k_hash_probe(
 // Probe of a unique-key table (or the first match only, LEFT SEMI): at most one output per probe
 // row. It takes the place of the reference's join front end, which sorts every input's row ids
 // with thrust::sort(sort_comparator) (gpu_impl/src/Operator/SortMergeJoinPredicate.cu:103-115)
-// before a CPU merge: here the build side is hashed once and each probe row is one table lookup. A workgroup probes a tile of blockDim.x * R rows (row = tile base + q * blockDim.x + lane,
+// before a CPU merge: here the build side is hashed once and each probe row is one table lookup.
+// A workgroup probes a tile of blockDim.x * R rows (row = tile base + q * blockDim.x + lane,
 // coalesced key loads), counts its matches with wave ballots and one LDS scan, and reserves its
 // output range with ONE atomic per tile -- a per-wave append would put ~n/64 device-scope
 // atomics on a single counter, which serialise across the XCDs.
+//
+// The table is probed at random, so the kernel is bound by how many slot loads are in flight:
+// each lane first loads the keys of its R rows, then issues the R home-slot loads (one 8- or
+// 16-byte load per slot) before it examines any of them, and only the rows whose home slot holds
+// another key go on to further rounds, all R of them again issued together. KT is the key column
+// type (GHIVE_INT or GHIVE_LONG), L the slot layout (8 or 16 bytes).
 constexpr int kProbeThreads = 512;
-template <int R>
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+template <int R, int KT, int L>
 __global__ __launch_bounds__(kProbeThreads) void k_probe_unique(
-    DevHash h, const void *keys, int key_type, const uint32_t *sel, const int32_t *tup, uint64_t n,
+    DevHash h, const void *keys, const uint32_t *sel, const int32_t *tup, uint64_t n,
     int emit_index, uint32_t *out_a, int64_t *out_b, uint64_t cap, unsigned long long *counter) {
   __shared__ uint32_t wave_total[kProbeThreads / kWave];
   __shared__ unsigned long long tile_base;
@@ -234,54 +242,92 @@ __global__ __launch_bounds__(kProbeThreads) void k_probe_unique(
   const uint64_t tile_rows = (uint64_t)blockDim.x * R;
   const uint64_t ntiles = (n + tile_rows - 1) / tile_rows;
   for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    bool match[R];
-    int64_t payload[R];
-    uint64_t row[R];
+    uint64_t row[R], s[R];
+    int64_t k[R], p[R];
+    bool act[R], pend[R], match[R];
+    // key rows (each branch issues its R index loads together)
+    if (tup) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const uint64_t i = tile * tile_rows + (uint64_t)q * blockDim.x + threadIdx.x;
+        const int32_t t = i < n ? tup[i] : -1;
+        act[q] = t >= 0;
+        row[q] = (uint64_t)(t < 0 ? 0 : t);
+      }
+    } else if (sel) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const uint64_t i = tile * tile_rows + (uint64_t)q * blockDim.x + threadIdx.x;
+        act[q] = i < n;
+        row[q] = sel[act[q] ? i : n - 1];  // n >= 1: the kernel is launched only for rows
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const uint64_t i = tile * tile_rows + (uint64_t)q * blockDim.x + threadIdx.x;
+        act[q] = i < n;
+        row[q] = i;
+      }
+    }
+    // the keys of all R rows in flight together
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      const uint64_t i = tile * tile_rows + (uint64_t)q * blockDim.x + threadIdx.x;
-      bool active = i < n;
-      uint64_t r = 0;
-      if (active) {
-        if (tup) {
-          const int32_t t = tup[i];
-          active = t >= 0;
-          r = (uint64_t)(t < 0 ? 0 : t);
+      k[q] = 0;
+      if (act[q])
+        k[q] = KT == GHIVE_INT ? (int64_t)static_cast<const int32_t *>(keys)[row[q]]
+                               : static_cast<const int64_t *>(keys)[row[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      match[q] = false;
+      p[q] = -1;
+      const bool notnull = KT == GHIVE_INT ? !is_null_i32((int32_t)k[q]) : !is_null_i64(k[q]);
+      pend[q] = act[q] && notnull && k[q] >= h.key_min && k[q] <= h.key_max &&
+                (L == 16 || (k[q] >= INT32_MIN && k[q] <= INT32_MAX));
+      s[q] = pend[q] ? home_slot(h, k[q]) : 0;
+    }
+    // rounds of slot loads: every pending row's next slot, issued before any is examined; a
+    // table always keeps empty slots, and a row is never followed past a full cycle of the table
+    for (uint64_t round = 0; round <= h.mask; ++round) {
+      uint64_t w0[R], w1[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        if (L == 16) {
+          const u64x2 v = pend[q] ? *reinterpret_cast<const u64x2 *>(h.slots + 2 * s[q]) : u64x2{0, 0};
+          w0[q] = v.x;
+          w1[q] = v.y;
         } else {
-          r = sel ? sel[i] : i;
+          w0[q] = pend[q] ? h.slots[s[q]] : 0;
+          w1[q] = 0;
         }
       }
-      int64_t k = 0;
-      bool valid = active && col_load_i64(keys, key_type, r, k) && k >= h.key_min && k <= h.key_max;
-      if (valid && h.layout == 8 && (k < INT32_MIN || k > INT32_MAX)) valid = false;
-      bool hit = false;
-      int64_t p = -1;
-      if (valid) {
-        uint64_t s = home_slot(h, k);
-        for (uint64_t step = 0; step <= h.mask; ++step) {  // a table always has an empty slot
-          if (h.layout == 8) {
-            const uint64_t w = h.slots[s];
-            if ((uint32_t)w == 0x7fffffffu) break;
-            if ((int64_t)(int32_t)(uint32_t)w == k) {
-              hit = true;
-              p = (int64_t)(int32_t)(uint32_t)(w >> 32);
-              break;
-            }
-          } else {
-            const uint64_t kk = h.slots[2 * s];
-            if (kk == kEmpty16) break;
-            if ((int64_t)kk == k) {
-              hit = true;
-              p = (int64_t)h.slots[2 * s + 1];
-              break;
-            }
-          }
-          s = (s + 1) & h.mask;
+      bool more = false;
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        if (!pend[q]) continue;
+        int64_t kk, pp;
+        bool empty;
+        if (L == 16) {
+          kk = (int64_t)w0[q];
+          pp = (int64_t)w1[q];
+          empty = w0[q] == kEmpty16;
+        } else {
+          kk = (int64_t)(int32_t)(uint32_t)w0[q];
+          pp = (int64_t)(int32_t)(uint32_t)(w0[q] >> 32);
+          empty = (uint32_t)w0[q] == 0x7fffffffu;
+        }
+        if (!empty && kk == k[q]) {
+          match[q] = true;
+          p[q] = pp;
+          pend[q] = false;
+        } else if (empty) {
+          pend[q] = false;
+        } else {
+          s[q] = (s[q] + 1) & h.mask;
+          more = true;
         }
       }
-      match[q] = hit;
-      payload[q] = p;
-      row[q] = emit_index ? i : r;
+      if (!ballot(more)) break;
     }
     // offsets: per q a ballot; the tile's matches are laid out q-major, wave-major, lane-minor
     uint32_t below[R], wave_count = 0;
@@ -306,8 +352,9 @@ __global__ __launch_bounds__(kProbeThreads) void k_probe_unique(
     for (int q = 0; q < R; ++q) {
       const uint64_t pos = base + below[q];
       if (match[q] && pos < cap) {
-        out_a[pos] = (uint32_t)row[q];
-        out_b[pos] = payload[q];
+        const uint64_t i = tile * tile_rows + (uint64_t)q * blockDim.x + threadIdx.x;
+        out_a[pos] = (uint32_t)(emit_index ? i : row[q]);
+        out_b[pos] = p[q];
       }
     }
     __syncthreads();  // wave_total / tile_base are rewritten by the next tile
@@ -612,9 +659,21 @@ static int probe_run(ghive_ctx *ctx, const ghive_hash_table *h, const ghive_colu
     constexpr int R = 8;
     const uint64_t tiles = (n + (uint64_t)kProbeThreads * R - 1) / ((uint64_t)kProbeThreads * R);
     const uint64_t cap_grid = (uint64_t)ctx->cus * 8;
-    k_probe_unique<R><<<(uint32_t)(tiles < cap_grid ? tiles : cap_grid), kProbeThreads, 0, ctx->stream>>>(
-        dev_hash(h), key->data, key->type, sel, tup, n, emit_index, out_a, out_b, cap,
-        reinterpret_cast<unsigned long long *>(ctx->d_counters));
+    const uint32_t grid = (uint32_t)(tiles < cap_grid ? tiles : cap_grid);
+    auto *cnt = reinterpret_cast<unsigned long long *>(ctx->d_counters);
+    const DevHash d = dev_hash(h);
+    if (key->type == GHIVE_INT && h->layout == 8)
+      k_probe_unique<R, GHIVE_INT, 8><<<grid, kProbeThreads, 0, ctx->stream>>>(d, key->data, sel, tup, n, emit_index,
+                                                                              out_a, out_b, cap, cnt);
+    else if (key->type == GHIVE_INT)
+      k_probe_unique<R, GHIVE_INT, 16><<<grid, kProbeThreads, 0, ctx->stream>>>(d, key->data, sel, tup, n,
+                                                                               emit_index, out_a, out_b, cap, cnt);
+    else if (h->layout == 8)
+      k_probe_unique<R, GHIVE_LONG, 8><<<grid, kProbeThreads, 0, ctx->stream>>>(d, key->data, sel, tup, n,
+                                                                               emit_index, out_a, out_b, cap, cnt);
+    else
+      k_probe_unique<R, GHIVE_LONG, 16><<<grid, kProbeThreads, 0, ctx->stream>>>(d, key->data, sel, tup, n,
+                                                                                emit_index, out_a, out_b, cap, cnt);
     GH_LAUNCH(ctx);
   } else if (n) {
     k_hash_probe<<<grid_for(ctx, n), kHashThreads, 0, ctx->stream>>>(

@@ -84,21 +84,44 @@ __device__ __forceinline__ uint64_t home_slot(const DevHash &h, int64_t key) {
   return h.layout == 8 ? (uint64_t)hash_u32((uint32_t)key, h.log2) : hash_u64((uint64_t)key, h.log2);
 }
 
-// First match of a unique-key table; returns false on a miss.
-__device__ __forceinline__ bool probe_unique8(const uint64_t *slots, uint64_t mask, int log2,
-                                              int32_t key, int32_t &payload) {
-  uint64_t s = hash_u32((uint32_t)key, log2);
-  for (uint64_t step = 0; step <= mask; ++step) {  // a table always has empty slots; bounded anyway
-    uint64_t w = slots[s];
-    int32_t k = (int32_t)(uint32_t)w;
-    if (k == key) {
-      payload = (int32_t)(w >> 32);
-      return true;
-    }
-    if (k == 0x7fffffff) return false;
-    s = (s + 1) & mask;
+// R lookups of one thread in a unique-key L8 table at once: every row's home slot is loaded
+// before any is examined, and only the rows whose slot holds another key go on, again together,
+// so a lane keeps R slot loads in flight instead of one. alive[q] in: probe row q; out: matched.
+template <int R>
+__device__ __forceinline__ void probe_unique8_rows(const uint64_t *slots, uint64_t mask, int log2,
+                                                   const int32_t (&key)[R], bool (&alive)[R],
+                                                   int32_t (&payload)[R]) {
+  uint64_t s[R];
+  bool pend[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    pend[q] = alive[q];
+    s[q] = pend[q] ? hash_u32((uint32_t)key[q], log2) : 0;
+    alive[q] = false;
+    payload[q] = 0;
   }
-  return false;
+  for (uint64_t round = 0; round <= mask; ++round) {  // a table always has empty slots; bounded anyway
+    uint64_t w[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) w[q] = pend[q] ? slots[s[q]] : 0;
+    bool more = false;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      if (!pend[q]) continue;
+      const int32_t k = (int32_t)(uint32_t)w[q];
+      if (k == 0x7fffffff) {  // empty (NULL keys are never inserted)
+        pend[q] = false;
+      } else if (k == key[q]) {
+        alive[q] = true;
+        payload[q] = (int32_t)(w[q] >> 32);
+        pend[q] = false;
+      } else {
+        s[q] = (s[q] + 1) & mask;
+        more = true;
+      }
+    }
+    if (!more) break;
+  }
 }
 
 int hash_build(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, uint64_t n,
