@@ -183,3 +183,44 @@ def test_async_built_table_used_first_by_a_server_slot(ctx):
     assert got == want and len(want) > 1
     ha.free()
     hs.free()
+
+
+def test_serve_full_size_requests_identical_to_plain(ctx):
+    """The largest requests of the serving mix, all in flight together: a 256M-row sort, a group-by
+    over 16M rows with up to 10M distinct int64 keys, and 16M-row fragments with two filter
+    constants; each served result equals the same request run alone through the plain entry point
+    (outputs compared on the device; groups after sorting by key)."""
+    from ghive_amd import ops, workloads as W
+    I = W.serving_inputs(ctx)
+    reqs = [("sort", dict(rows=I.sort_keys.numel())),
+            ("groupby", dict(rows=I.gb_keys.numel())),
+            ("fragment", dict(start=0, rows=I.small.fact_rows, attr=3)),
+            ("fragment", dict(start=4096, rows=I.small.fact_rows - 4096, attr=5)),
+            ("sort", dict(rows=100_000_000))]
+    torch.cuda.synchronize()
+    srv = ops.Server(slots=4, hbm_budget=96 << 30, slot_mem_limit=32 << 30)
+    try:
+        tickets = [W.serving_submit(srv, I, k, p)[0] for k, p in reqs]
+        served = [srv.wait(t) for t in tickets]
+        assert srv.stats()["peak_inflight"] > 1
+    finally:
+        srv.close()
+
+    def canon(kind, outs):  # groups come out in no fixed order; a sort's permutation is the result
+        if kind == "sort":
+            return outs
+        order = torch.argsort(outs[0])
+        return [o[order] for o in outs]
+
+    for (kind, p), got in zip(reqs, served):
+        want = W.serving_plain(ctx, I, kind, p)
+        got, want = canon(kind, got), canon(kind, want)
+        assert len(got) == len(want)
+        for g, w in zip(got, want):
+            assert g.shape == w.shape and torch.equal(g, w), (kind, p)
+        if kind == "groupby":
+            assert 5_000_000 < got[0].numel() <= W.SERVE_GB_DISTINCT
+            assert int(got[2].sum()) == p["rows"]  # COUNT(*) over every group
+        if kind == "sort":
+            k = I.sort_keys[: p["rows"]][got[0].long()]
+            assert bool((k[1:] >= k[:-1]).all())
