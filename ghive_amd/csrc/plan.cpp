@@ -452,6 +452,272 @@ std::string attr_value(const std::string &block, const std::string &name) {
   return std::string();
 }
 
+// --- EXPLAIN EXTENDED alone: the operator tree ------------------------------------------------
+// The extended text holds the whole plan by itself: "Edges:" names each vertex's inputs with the
+// edge kind, and each vertex's "Map/Reduce Operator Tree:" nests every operator under the one that
+// feeds it. The tree built here is the one parse_tree builds from the formatted plan, with
+// operator ids numbered in order of appearance (TS_0, FIL_1, ...). A vertex's first operator
+// reads the vertex's SIMPLE / CUSTOM_SIMPLE / ONE_TO_ONE inputs (a merge join: every one, placed by
+// the tag of its sink); a map join's other inputs are the BROADCAST vertices its "input vertices:"
+// lines place. A table scan's outputs are the table's columns (its "columns" property) that its
+// vertex's operators name.
+int parse_extended_tree(ghive_plan &P, const std::string &text, std::string &err) {
+  static const std::regex edge_re(R"(^((Map|Reducer) \d+) <- (.*)$)");
+  static const std::regex edge_in_re(R"(((Map|Reducer|Union) \d+) \(([A-Z_]+)\))");
+  static const std::regex vertex_re(R"(^(Map|Reducer) \d+$)");
+  static const std::regex op_re(R"(^([a-zA-Z]+( [a-zA-Z]+)?) Operator$)");
+  static const std::regex name_re(R"(^name: (\w+)\.(\w+)$)");
+  static const std::regex cols_re(R"(^columns (\S+)$)");
+  static const std::regex ident_re(R"([A-Za-z_][A-Za-z0-9_]*)");
+  static const std::regex input_vertex_re(R"(^(\d+) ((Map|Reducer) \d+)$)");
+  const std::vector<std::string> lines = split_lines(text);
+  auto pad_of = [](const std::string &raw) { return (int)raw.find_first_not_of(' '); };
+  // edges: vertex -> (input vertex, edge kind)
+  std::map<std::string, std::vector<std::pair<std::string, std::string>>> edges;
+  // vertices in order, each with its lines
+  std::vector<std::pair<std::string, std::vector<std::string>>> vertices;
+  int epad = -1, vpad = -1;
+  for (const std::string &raw : lines) {
+    const std::string t = trim(raw);
+    if (t.empty()) continue;
+    const int pad = pad_of(raw);
+    if (t == "Edges:") {
+      epad = pad;
+      continue;
+    }
+    if (epad >= 0) {
+      std::smatch m;
+      if (pad > epad && std::regex_match(t, m, edge_re)) {
+        const std::string v = m[1].str(), rest = m[3].str();
+        for (auto it = std::sregex_iterator(rest.begin(), rest.end(), edge_in_re); it != std::sregex_iterator(); ++it) {
+          if ((*it)[2].str() == "Union") {
+            err = "plan: edge " + v + " <- " + (*it)[1].str() + ": UNION vertices are not supported";
+            return GHIVE_ERR_UNSUPPORTED;
+          }
+          edges[v].push_back({(*it)[1].str(), (*it)[3].str()});
+        }
+        continue;
+      }
+      epad = -1;
+    }
+    if (t == "Vertices:") {
+      vpad = pad;
+      continue;
+    }
+    if (vpad < 0) continue;
+    if (pad <= vpad) {
+      vpad = -1;
+      continue;
+    }
+    if (pad == vpad + 2 && std::regex_match(t, vertex_re)) {
+      vertices.push_back({t, {}});
+      continue;
+    }
+    if (!vertices.empty()) vertices.back().second.push_back(raw);
+  }
+  if (vertices.empty()) {
+    err = "plan: no vertex in the EXPLAIN EXTENDED text";
+    return GHIVE_ERR_INVALID;
+  }
+  struct VOp {
+    int node;
+    int pad;
+    int up;  // the operator that feeds it inside the vertex, else -1
+  };
+  std::map<std::string, std::vector<VOp>> vops;
+  std::map<std::string, std::vector<int>> sinks;
+  int counter = 0;
+  for (const auto &v : vertices) {
+    const std::string &vertex = v.first;
+    const std::vector<std::string> &vl = v.second;
+    // the table's name and columns (map vertices), from the Partition section
+    std::string table;
+    std::vector<std::string> table_cols;
+    for (const std::string &raw : vl) {
+      const std::string t = trim(raw);
+      std::smatch m;
+      if (table.empty() && std::regex_match(t, m, name_re)) table = m[1].str() + "@" + m[2].str();
+      if (table_cols.empty() && std::regex_match(t, m, cols_re)) table_cols = split_list(m[1].str());
+    }
+    int tpad = -1;
+    std::vector<VOp> &ops = vops[vertex];
+    std::vector<int> stack;  // indexes into ops
+    std::string all_text;    // the attribute text of every operator of the vertex
+    for (const std::string &raw : vl) {
+      const std::string t = trim(raw);
+      if (t.empty()) continue;
+      const int pad = pad_of(raw);
+      if (t == "Map Operator Tree:" || t == "Reduce Operator Tree:") {
+        tpad = pad;
+        continue;
+      }
+      if (tpad < 0) continue;
+      if (pad <= tpad) {  // the operator tree ended ("Execution mode:", "Path -> Alias:", ...)
+        tpad = -1;
+        continue;
+      }
+      std::smatch m;
+      std::string kind, prefix;
+      if (t == "TableScan") kind = "TableScan", prefix = "TS";
+      else if (t == "Limit") kind = "Limit", prefix = "LIM";
+      else if (std::regex_match(t, m, op_re)) {
+        const std::string k = m[1].str();
+        if (k == "Filter") kind = "Filter", prefix = "FIL";
+        else if (k == "Select") kind = "Select", prefix = "SEL";
+        else if (k == "Map Join") kind = "MapJoin", prefix = "MAPJOIN";
+        else if (k == "Merge Join") kind = "MergeJoin", prefix = "MERGEJOIN";
+        else if (k == "Group By") kind = "GroupBy", prefix = "GBY";
+        else if (k == "Reduce Output") kind = "ReduceSink", prefix = "RS";
+        else if (k == "File Output") kind = "FileSink", prefix = "FS";
+        else {
+          kind = k;
+          kind.erase(std::remove(kind.begin(), kind.end(), ' '), kind.end());
+          prefix = kind;
+          std::transform(prefix.begin(), prefix.end(), prefix.begin(), ::toupper);
+        }
+      }
+      if (kind.empty()) {  // an attribute line of the current operator
+        if (!ops.empty()) {
+          P.ops[ops.back().node].extended += raw + "\n";
+          all_text += t + "\n";
+        }
+        continue;
+      }
+      while (!stack.empty() && ops[stack.back()].pad >= pad) stack.pop_back();
+      Op o;
+      o.id = prefix + "_" + std::to_string(counter++);
+      o.kind = kind;
+      o.vertex = vertex;
+      o.extended = raw + "\n";
+      if (kind == "TableScan") o.table = table;
+      const int node = add_op(P, o);
+      ops.push_back({node, pad, stack.empty() ? -1 : ops[stack.back()].node});
+      stack.push_back((int)ops.size() - 1);
+      if (kind == "ReduceSink" || kind == "FileSink") sinks[vertex].push_back(node);
+    }
+    if (sinks[vertex].empty()) {
+      err = "plan: vertex " + vertex + " has no sink operator";
+      return GHIVE_ERR_INVALID;
+    }
+    if (sinks[vertex].size() > 1) {  // one result per vertex run, as GPUProcess returns
+      err = "plan: vertex " + vertex + " has " + std::to_string(sinks[vertex].size()) +
+            " sink operators (one result per vertex is supported)";
+      return GHIVE_ERR_UNSUPPORTED;
+    }
+    P.vertex_root[vertex] = sinks[vertex].back();
+    P.vertices.push_back(vertex);
+    // short attributes from the extended blocks
+    for (const VOp &vo : ops) {
+      Op &o = P.ops[vo.node];
+      if (o.kind == "TableScan") {
+        std::set<std::string> named;
+        for (auto it = std::sregex_iterator(all_text.begin(), all_text.end(), ident_re); it != std::sregex_iterator(); ++it)
+          named.insert(it->str());
+        for (const std::string &c : table_cols)
+          if (named.count(c)) o.outputs.push_back(c);
+      } else if (o.kind == "Select" || o.kind == "GroupBy" || o.kind == "MapJoin" || o.kind == "MergeJoin") {
+        o.outputs = attr_list(o.extended, "outputColumnNames");
+        if (o.kind == "GroupBy") {
+          o.agg_strs = attr_list(o.extended, "aggregations");
+          o.key_strs = attr_list(o.extended, "keys");
+        }
+      } else if (o.kind == "Filter") {
+        o.predicate = strip_type(attr_value(o.extended, "predicate"));
+      } else if (o.kind == "ReduceSink") {
+        for (const std::string &c : attr_list(o.extended, "Map-reduce partition columns"))
+          o.partition_cols.push_back(strip_type(c));
+      } else if (o.kind == "Limit") {
+        const std::string n = attr_value(o.extended, "Number of rows");
+        if (!n.empty()) o.limit = std::stoi(n);
+      }
+    }
+  }
+  // inputs: in-vertex upstream operators, input vertices' sinks
+  auto sink_of = [&](const std::string &v) -> int {
+    const auto it = P.vertex_root.find(v);
+    return it == P.vertex_root.end() ? -1 : it->second;
+  };
+  auto link = [&](int parent, int child) {
+    P.ops[parent].children.push_back(child);
+    if (P.ops[child].parent < 0) P.ops[child].parent = parent;
+  };
+  for (const auto &v : vertices) {
+    const std::string &vertex = v.first;
+    std::vector<std::pair<int, int>> tagged;  // (tag, sink) of the vertex's shuffled inputs
+    for (const auto &e : edges[vertex]) {
+      if (e.second == "BROADCAST_EDGE") continue;
+      const int s = sink_of(e.first);
+      if (s < 0) {
+        err = "plan: " + vertex + " reads unknown vertex " + e.first;
+        return GHIVE_ERR_INVALID;
+      }
+      const std::string tag = attr_value(P.ops[s].extended, "tag");
+      tagged.push_back({tag.empty() ? 0 : std::stoi(tag), s});
+    }
+    std::stable_sort(tagged.begin(), tagged.end(),
+                     [](const std::pair<int, int> &a, const std::pair<int, int> &b) { return a.first < b.first; });
+    for (const VOp &vo : vops[vertex]) {
+      Op &o = P.ops[vo.node];
+      if (o.kind == "TableScan") continue;
+      std::vector<int> ins;
+      if (o.kind == "MapJoin") {
+        // positions named by "input vertices:", the rest from inside the vertex (or its input)
+        std::map<int, int> at;
+        bool in_list = false;
+        for (const std::string &l : split_lines(o.extended)) {
+          const std::string t = trim(l);
+          std::smatch m;
+          if (t == "input vertices:") {
+            in_list = true;
+            continue;
+          }
+          if (in_list && std::regex_match(t, m, input_vertex_re)) {
+            const int s = sink_of(m[2].str());
+            if (s < 0) {
+              err = "plan: " + o.id + " reads unknown vertex " + m[2].str();
+              return GHIVE_ERR_INVALID;
+            }
+            at[std::stoi(m[1].str())] = s;
+            continue;
+          }
+          in_list = false;
+        }
+        const int big = vo.up >= 0 ? vo.up : (tagged.empty() ? -1 : tagged.front().second);
+        int n = 2;
+        for (const auto &a : at) n = std::max(n, a.first + 1);
+        for (int pos = 0; pos < n; ++pos) {
+          const auto it = at.find(pos);
+          ins.push_back(it != at.end() ? it->second : big);
+        }
+      } else if (vo.up >= 0) {
+        ins.push_back(vo.up);
+      } else {
+        for (const auto &t : tagged) ins.push_back(t.second);
+      }
+      for (int c : ins) {
+        if (c < 0) {
+          err = "plan: " + o.id + " in " + vertex + " has no input";
+          return GHIVE_ERR_INVALID;
+        }
+        link(vo.node, c);
+      }
+      if (o.kind == "MapJoin" || o.kind == "MergeJoin") {  // the input operators, in table order
+        std::string conds;
+        for (size_t i = 0; i < ins.size(); ++i) conds += (i ? "=" : "") + P.ops[ins[i]].id + "._col";
+        o.conds = conds;
+      }
+    }
+  }
+  // the root: the one file sink
+  for (size_t i = 0; i < P.ops.size(); ++i)
+    if (P.ops[i].kind == "FileSink") P.root = (int)i;
+  if (P.root < 0) {
+    err = "plan: no File Output Operator in the EXPLAIN EXTENDED text";
+    return GHIVE_ERR_INVALID;
+  }
+  return GHIVE_OK;
+}
+
 // --- parseExtended per operator -------------------------------------------------------------
 void resolve_sink(ghive_plan &P, Op &o) {
   if (!o.original_outputs.empty() || o.kind == "Union") return;  // done already
@@ -1689,7 +1955,8 @@ extern "C" {
 
 int ghive_plan_parse(const char *plan, const char *extended, const char *maintain_json, char *err, uint64_t errcap,
                      ghive_plan **out) {
-  if (!plan || !out) {
+  const bool extended_only = (!plan || !*plan) && extended && *extended;
+  if ((!plan && !extended_only) || !out) {
     set_err(err, errcap, "plan: NULL argument");
     return GHIVE_ERR_INVALID;
   }
@@ -1705,7 +1972,7 @@ int ghive_plan_parse(const char *plan, const char *extended, const char *maintai
         return GHIVE_ERR_INVALID;
       }
     }
-    int st = parse_tree(*P, plan, e);
+    int st = extended_only ? parse_extended_tree(*P, extended, e) : parse_tree(*P, plan, e);
     if (st != GHIVE_OK) {
       delete P;
       set_err(err, errcap, e);
@@ -1715,7 +1982,7 @@ int ghive_plan_parse(const char *plan, const char *extended, const char *maintai
     assign_vertices(*P, P->root, P->ops[P->root].vertex, seen);
     std::set<int> done;
     propagate_outputs(*P, P->root, done);
-    if (extended) link_extended(*P, extended_blocks(extended));
+    if (extended && !extended_only) link_extended(*P, extended_blocks(extended));
     std::set<int> resolved;
     resolve(*P, P->root, resolved);
     // the root sink performs the final ORDER BY with the order of the sink that feeds its vertex

@@ -824,13 +824,15 @@ def gpuresult_sequence(types):
 class Plan:
     """A query plan (ghive_plan_*): Hive's EXPLAIN and EXPLAIN EXTENDED texts (and the Java side's
     maintain-values JSON) parsed into every vertex's operator tree with resolved descriptors.
-    describe() returns {"root", "vertices": {vertex: sink id}, "operators": {id: {...}}}."""
+    describe() returns {"root", "vertices": {vertex: sink id}, "operators": {id: {...}}}.
+    plan_text None: the EXPLAIN EXTENDED text alone gives the tree."""
 
     def __init__(self, plan_text, extended_text=None, maintain_json=None):
         import json
         err = C.create_string_buffer(512)
         h = C.c_void_p()
-        st = lib().ghive_plan_parse(plan_text.encode(), extended_text.encode() if extended_text else None,
+        st = lib().ghive_plan_parse(plan_text.encode() if plan_text else None,
+                                    extended_text.encode() if extended_text else None,
                                     maintain_json.encode() if maintain_json else None, err, len(err), C.byref(h))
         if st != abi.OK:
             raise GhiveError(st, err.value.decode(errors="replace"))

@@ -575,7 +575,10 @@ void ghive_server_destroy(ghive_server *srv);
  * NULL), become the operator tree with each operator's resolved descriptor: filter predicate
  * text, select expressions, join input order / key columns / join types (BaseJoinType numbering)
  * / maintained columns, group-by key and aggregate columns (AggregationType numbering), sink key
- * columns and sort orders. Host only; needs no context. On failure `err` receives the message. */
+ * columns and sort orders. With `plan` NULL or empty, the EXPLAIN EXTENDED text alone gives the
+ * tree (its "Edges:" and each vertex's operator tree; operator ids are then numbered in order of
+ * appearance) -- for plans whose formatted text comes from another optimizer run. Host only;
+ * needs no context. On failure `err` receives the message. */
 typedef struct ghive_plan ghive_plan;
 int ghive_plan_parse(const char *plan, const char *extended, const char *maintain_json, char *err,
                      uint64_t errcap, ghive_plan **out);

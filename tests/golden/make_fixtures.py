@@ -207,7 +207,8 @@ def hive_groupby_kats():
 
 
 PLAN_FIXTURES = ["ssb_10_orc_transformed_1_1", "ssb_10_orc_transformed_2_1", "ssb_10_orc_transformed_3_1",
-                 "ssb_10_orc_transformed_4_1", "tpcds_orc_100_transformed_query3"] + \
+                 "ssb_10_orc_transformed_4_1", "tpcds_orc_100_transformed_query1",
+                 "tpcds_orc_100_transformed_query2", "tpcds_orc_100_transformed_query3"] + \
                 ["ssb_2_orc_" + q for q in ("1_1", "1_2", "1_3", "1_4", "2_1", "2_2", "2_3", "3_1", "3_2", "3_3",
                                             "3_4", "4_1", "4_2", "4_3")] + \
                 ["ssb_10_orc_transformed_" + q for q in ("1_2", "1_3", "2_2", "2_3", "3_2", "3_3", "3_4", "4_2",

@@ -155,3 +155,49 @@ def test_fused_form_recognised_from_the_plan():
         assert g["GBY_29"][:2] == ("Map 1", True)
     g = gbys("plans", "ssb_10_orc_transformed_2_1.txt", "ssb_10_orc_transformed_2_1_extended.txt")
     assert g["GBY_22"][1] is False and "MERGEJOIN_43" in g["GBY_22"][2]
+
+
+def test_tpcds_q3_from_the_extended_plan_alone():
+    """TPC-DS q3's EXPLAIN and EXPLAIN EXTENDED fixtures come from two optimizer runs (four
+    reducers and merge joins only in one, three reducers and a broadcast map join in the other),
+    so the extended text alone gives the tree: its Edges and operator trees, ids numbered in
+    order of appearance. Expected values read off tpcds_orc_100_transformed_query3_extended.txt."""
+    from ghive_amd import ops
+    with open(os.path.join(PLANS, "tpcds_orc_100_transformed_query3_extended.txt")) as fh:
+        p = ops.Plan(None, fh.read())
+    d = p.describe()
+    O = d["operators"]
+    assert d["root"] == "FS_20"
+    assert sorted(d["vertices"]) == ["Map 1", "Map 5", "Map 6", "Reducer 2", "Reducer 3", "Reducer 4"]
+    assert p.chain("Map 1") == ["RS_3", "SEL_2", "FIL_1", "TS_0"]
+    assert O["TS_0"]["table"] == "tpcds_orc_100_transformed@store_sales"
+    assert O["TS_0"]["outputs"] == ["ss_sold_date_sk", "ss_item_sk", "ss_sales_price"]
+    assert O["TS_4"]["outputs"] == ["i_item_sk", "i_brand_id", "i_brand", "i_manufact_id"]
+    assert O["FIL_9"]["predicate"] == "((d_moy = 11) and d_date_sk is not null)"
+    # Reducer 2: merge join of Map 1 (tag 0) and Map 5 (tag 1) on ss_item_sk = i_item_sk, then the
+    # broadcast map join with Map 6 on ss_sold_date_sk = d_date_sk, then the hash group-by
+    mj = O["MERGEJOIN_12"]
+    assert mj["vertex"] == "Reducer 2" and mj["join_inputs"] == ["RS_3", "RS_7"]
+    assert mj["join_cols"] == [[0], [0]] and mj["join_types"] == [[INNER, 0, 1]]
+    assert O["RS_3"]["outputs"] == ["_col1", "_col0", "_col2"]  # the reduce key first
+    mp = O["MAPJOIN_13"]
+    assert mp["join_inputs"] == ["MERGEJOIN_12", "RS_11"] and mp["join_cols"] == [[0], [0]]
+    assert mp["outputs"] == ["_col2", "_col4", "_col5", "_col8"]
+    g = O["GBY_14"]
+    assert g["keys"] == [3, 1, 2] and g["aggs"] == [[SUM, 0]] and g["mode"] == "hash"
+    assert O["GBY_16"]["keys"] == [0, 1, 2] and O["GBY_16"]["aggs"] == [[SUM, 3]]
+    assert O["RS_17"]["sort_orders"] == "+-+" and O["RS_17"]["key_offsets"] == [0, 3, 1]
+    assert O["LIM_19"]["limit"] == 100
+    assert O["FS_20"]["require_sort"] and O["FS_20"]["sort_orders"] == "+-+"
+
+
+@pytest.mark.parametrize("q,why", [(1, "Map 7 has 2 sink operators"), (2, "UNION vertices are not supported")])
+def test_tpcds_extended_plans_outside_the_vertex_model(q, why):
+    """q1 runs one map vertex into two reducers (two sinks, while a vertex run returns one result)
+    and q2 reads UNION vertices: both are refused with the reason, not parsed into a wrong tree."""
+    from ghive_amd import ops
+    with open(os.path.join(PLANS, f"tpcds_orc_100_transformed_query{q}_extended.txt")) as fh:
+        text = fh.read()
+    with pytest.raises(ops.GhiveError) as e:
+        ops.Plan(None, text)
+    assert e.value.status == abi.ERR_UNSUPPORTED and why in str(e.value)
