@@ -208,18 +208,24 @@ __global__ __launch_bounds__(256) void k_gb_part_keys(KeySpec K, const uint32_t 
   }
 }
 
-// starts[p] = first position of partition p (top pbits of h) in the sorted h; starts[P] = n.
+// starts[p] = first position of partition p (top pbits of h) in h, which the partition passes
+// leave ordered by those bits; starts[P] = n. One thread per partition runs a binary search
+// (~log2 n dependent loads, the upper levels shared through L2) instead of a pass over every h.
 __global__ void k_gb_part_bounds(const uint64_t *h, uint64_t n, int pbits, uint64_t *starts) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t P = 1ull << pbits;
-  for (; i < n; i += stride) {
-    const uint64_t p = pbits ? h[i] >> (64 - pbits) : 0;
-    const int64_t prev = i ? (int64_t)(pbits ? h[i - 1] >> (64 - pbits) : 0) : -1;
-    for (int64_t q = prev + 1; q <= (int64_t)p; ++q) starts[q] = i;  // empty partitions start here too
-    if (i == n - 1)
-      for (uint64_t q = p + 1; q <= P; ++q) starts[q] = n;
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > P) return;
+  uint64_t lo = 0, hi = n;  // the first i whose partition is >= q
+  if (q == P) {
+    lo = n;
+  } else if (pbits) {
+    while (lo < hi) {
+      const uint64_t mid = lo + (hi - lo) / 2;
+      if ((h[mid] >> (64 - pbits)) < q) lo = mid + 1;
+      else hi = mid;
+    }
   }
+  starts[q] = lo;
 }
 
 // ---- wide keys: composite keys wider than 64 bits, more than kMaxKeys columns, or STRING -------
@@ -1049,7 +1055,18 @@ static int gbp_partition(ghive_ctx *ctx, const GbpSrc &src, uint64_t n, int pbit
 __global__ __launch_bounds__(256) void k_max_u64(const unsigned long long *part, uint64_t n, unsigned long long *out) {
   __shared__ unsigned long long wm[256 / kWave];
   unsigned long long m = 0;
-  for (uint64_t i = threadIdx.x; i < n; i += 256) m = part[i] > m ? part[i] : m;
+  // one workgroup: 16 independent loads in flight per lane, not one (the reduction is latency-bound)
+  constexpr int U = 16;
+  for (uint64_t i0 = 0; i0 < n; i0 += 256 * U) {
+    unsigned long long x[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t i = i0 + (uint64_t)u * 256 + threadIdx.x;
+      x[u] = i < n ? part[i] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) m = x[u] > m ? x[u] : m;
+  }
 #pragma unroll
   for (int d = kWave / 2; d > 0; d >>= 1) {
     const unsigned long long o = __shfl_xor(m, d, kWave);
@@ -1153,8 +1170,8 @@ int groupby_partitioned(ghive_ctx *ctx, const KeySpec &K, GroupTableDev &Tspec, 
                               digits, &ks, &ids, true));
   }
   {
-    const uint64_t g = (n + 255) / 256, cap = (uint64_t)ctx->cus * 16;
-    k_gb_part_bounds<<<(uint32_t)(g > cap ? cap : g), 256, 0, ctx->stream>>>(ks, n, pbits, starts.as<uint64_t>());
+    const uint64_t parts = (1ull << pbits) + 1;
+    k_gb_part_bounds<<<(uint32_t)((parts + 255) / 256), 256, 0, ctx->stream>>>(ks, n, pbits, starts.as<uint64_t>());
     GH_LAUNCH(ctx);
   }
   // the small global table that takes the one key equal to the EMPTY pattern
