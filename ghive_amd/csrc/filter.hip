@@ -42,12 +42,15 @@ __global__ __launch_bounds__(kFilterThreads) void k_filter_mask(DevPred P, uint6
   if (threadIdx.x == 0) tile_counts[tile] = total;
 }
 
-// The same masks for whole tiles when every leaf compares a 4-byte INT column with literals
-// (AND / OR inside): each lane reads 4 consecutive rows with one 16-byte load per leaf column and
-// the 4 row groups of its tile together, then evaluates the nodes on the 16 values in registers.
-// A lane's 4 result bits are a nibble of the row-order mask word that 16 consecutive lanes share.
+// The same masks when every leaf compares a 4-byte INT column with literals (AND / OR inside):
+// each lane reads 4 consecutive rows with one 16-byte load per leaf column and the 4 row groups of
+// its tile together, then evaluates the nodes on the 16 values in registers. A lane's 4 result
+// bits are a nibble of the row-order mask word that 16 consecutive lanes share. The table's last
+// tile may be partial: there a lane whose 4 rows run past n loads the rows that exist one by one
+// (the rest read as NULL, which no comparison passes), and mask words past the table stay unwritten.
 constexpr int kFilterGroups = kFilterTile / (kFilterThreads * 4);  // 4 groups of 1024 rows
-__global__ __launch_bounds__(kFilterThreads) void k_filter_mask_i32(DevPred P, uint64_t *masks, uint32_t *tile_counts) {
+__global__ __launch_bounds__(kFilterThreads) void k_filter_mask_i32(DevPred P, uint64_t n, uint64_t *masks,
+                                                                    uint32_t *tile_counts) {
   __shared__ uint32_t lds[kFilterThreads / kWave];
   const uint64_t tile0 = (uint64_t)blockIdx.x * kFilterTile;
   uint32_t bits[kFilterGroups][4];
@@ -61,7 +64,18 @@ __global__ __launch_bounds__(kFilterThreads) void k_filter_mask_i32(DevPred P, u
       const int32_t *col = static_cast<const int32_t *>(P.col_data[nd.a]) + tile0 + 4 * threadIdx.x;
       v4i x[kFilterGroups];
 #pragma unroll
-      for (int g = 0; g < kFilterGroups; ++g) x[g] = *reinterpret_cast<const v4i *>(col + (uint64_t)g * kFilterThreads * 4);
+      for (int g = 0; g < kFilterGroups; ++g) {
+        const uint64_t r = tile0 + (uint64_t)g * kFilterThreads * 4 + 4 * threadIdx.x;
+        if (r + 4 <= n) {
+          x[g] = *reinterpret_cast<const v4i *>(col + (uint64_t)g * kFilterThreads * 4);
+        } else {  // the table's partial last tile
+          const int32_t *c = col + (uint64_t)g * kFilterThreads * 4;
+          x[g].x = r < n ? c[0] : 0x7fffffff;
+          x[g].y = r + 1 < n ? c[1] : 0x7fffffff;
+          x[g].z = r + 2 < n ? c[2] : 0x7fffffff;
+          x[g].w = 0x7fffffff;
+        }
+      }
 #pragma unroll
       for (int g = 0; g < kFilterGroups; ++g) {
         const int32_t v[4] = {x[g].x, x[g].y, x[g].z, x[g].w};
@@ -91,16 +105,23 @@ __global__ __launch_bounds__(kFilterThreads) void k_filter_mask_i32(DevPred P, u
     uint64_t word = (uint64_t)nib << (4 * (threadIdx.x & 15));
 #pragma unroll
     for (int d = 1; d < 16; d <<= 1) word |= __shfl_xor(word, d, kWave);
-    if ((threadIdx.x & 15) == 0) masks[(tile0 >> 6) + (uint64_t)g * (kFilterThreads / 16) + threadIdx.x / 16] = word;
+    const uint64_t wi = (tile0 >> 6) + (uint64_t)g * (kFilterThreads / 16) + threadIdx.x / 16;
+    if ((threadIdx.x & 15) == 0 && (wi << 6) < n) masks[wi] = word;
   }
   uint32_t total;
   (void)block_exclusive_scan_256(cnt, lds, total);
   if (threadIdx.x == 0) tile_counts[blockIdx.x] = total;
 }
 
+// Tile offsets: tile_offs (the scanned counts), or -- self_scan, for tables of at most
+// kSelfScanTiles tiles -- each workgroup sums the counts of the tiles before it itself (a few KiB
+// from L2), and the last one writes the total to *total: no scan launches.
+constexpr uint32_t kSelfScanTiles = 4096;
 __global__ __launch_bounds__(kFilterThreads) void k_filter_write(const uint64_t *masks, uint64_t nwords,
-                                                                 const uint32_t *tile_offs, uint32_t *out) {
+                                                                 const uint32_t *tile_offs, int self_scan,
+                                                                 uint64_t *total, uint32_t *out) {
   __shared__ uint32_t word_pre[kWordsPerTile];
+  __shared__ uint32_t red[kFilterThreads / kWave];
   const uint64_t w0 = (uint64_t)blockIdx.x * kWordsPerTile;
   if (threadIdx.x < kWordsPerTile) {
     uint64_t w = w0 + threadIdx.x;
@@ -108,8 +129,18 @@ __global__ __launch_bounds__(kFilterThreads) void k_filter_write(const uint64_t 
     uint32_t inc = wave_inclusive_scan(c);  // threads 0..63 are exactly wave 0
     word_pre[threadIdx.x] = inc - c;
   }
-  __syncthreads();
-  const uint32_t base = tile_offs[blockIdx.x];
+  uint32_t base;
+  if (self_scan) {
+    uint32_t part = 0;
+    for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kFilterThreads) part += tile_offs[t];
+    uint32_t sum;
+    (void)block_exclusive_scan_256(part, red, sum);  // synchronises the workgroup
+    base = sum;
+    if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = (uint64_t)sum + tile_offs[blockIdx.x];
+  } else {
+    __syncthreads();
+    base = tile_offs[blockIdx.x];
+  }
   const uint32_t wave = threadIdx.x / kWave, lane = lane_id();
   for (uint32_t q = wave; q < kWordsPerTile; q += kFilterThreads / kWave) {
     uint64_t w = w0 + q;
@@ -150,8 +181,8 @@ static int filter_run(ghive_ctx *ctx, const ghive_table *t, const ghive_pred *no
   DevBuf masks, counts;
   GH_TRY(masks.alloc(ctx, sizeof(uint64_t) * nwords));
   GH_TRY(counts.alloc(ctx, sizeof(uint32_t) * ntiles));
-  // whole tiles through the INT form when every leaf compares an aligned INT column, the rest
-  // (and every other predicate) through the general one
+  // every tile through the INT form when every leaf compares an aligned INT column, every other
+  // predicate through the general one
   bool int_form = P.nnodes > 0;
   for (int i = 0; i < P.nnodes; ++i) {
     const DevPredNode &nd = P.nodes[i];
@@ -159,9 +190,10 @@ static int filter_run(ghive_ctx *ctx, const ghive_table *t, const ghive_pred *no
     int_form = int_form && nd.op == kLeafInt && P.col_type[nd.a] == GHIVE_INT &&
                ((uintptr_t)P.col_data[nd.a] & 15) == 0;
   }
-  const uint64_t full = int_form ? n / kFilterTile : 0;
+  const uint64_t full = int_form ? ntiles : 0;  // the INT form takes the partial last tile too
   if (full) {
-    k_filter_mask_i32<<<(uint32_t)full, kFilterThreads, 0, ctx->stream>>>(P, masks.as<uint64_t>(), counts.as<uint32_t>());
+    k_filter_mask_i32<<<(uint32_t)full, kFilterThreads, 0, ctx->stream>>>(P, n, masks.as<uint64_t>(),
+                                                                         counts.as<uint32_t>());
     GH_LAUNCH(ctx);
   }
   if (full < ntiles) {
@@ -170,10 +202,11 @@ static int filter_run(ghive_ctx *ctx, const ghive_table *t, const ghive_pred *no
     GH_LAUNCH(ctx);
   }
   // the asynchronous form takes the count where the scan leaves its total, with no copy after
-  GH_TRY(exclusive_scan_u32(ctx, counts.as<uint32_t>(), counts.as<uint32_t>(), ntiles,
-                            count_dev ? count_dev : ctx->d_counters));
-  k_filter_write<<<ntiles, kFilterThreads, 0, ctx->stream>>>(masks.as<uint64_t>(), nwords,
-                                                             counts.as<uint32_t>(), out_sel);
+  uint64_t *total = count_dev ? count_dev : ctx->d_counters;
+  const int self_scan = ntiles <= kSelfScanTiles;
+  if (!self_scan) GH_TRY(exclusive_scan_u32(ctx, counts.as<uint32_t>(), counts.as<uint32_t>(), ntiles, total));
+  k_filter_write<<<ntiles, kFilterThreads, 0, ctx->stream>>>(masks.as<uint64_t>(), nwords, counts.as<uint32_t>(),
+                                                             self_scan, total, out_sel);
   GH_LAUNCH(ctx);
   if (count_dev) return GHIVE_OK;
   GH_TRY(d2h(ctx, out_count, ctx->d_counters, sizeof(uint64_t)));

@@ -386,7 +386,7 @@ int fragment_run(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_c
   GH_TRY(mem.get(3 * sizeof(unsigned long long), &dcnt));  // entries, overflow, rows joined
   djoined = static_cast<unsigned long long *>(dcnt) + 2;
   GH_TRY(group_table_init(ctx, P.T, static_cast<uint64_t *>(dkeys), static_cast<uint64_t *>(dacc),
-                          static_cast<unsigned long long *>(dcnt)));
+                          static_cast<unsigned long long *>(dcnt), 3));  // + rows joined
   const uint64_t tile_rows = (uint64_t)P.threads * (uint64_t)P.rows;
   if (P.F.n % tile_rows) {
     void *stage = nullptr;
@@ -394,7 +394,6 @@ int fragment_run(ghive_ctx *ctx, const ghive_fragment *frag, ghive_column *out_c
     P.F.stage = static_cast<int32_t *>(stage);
   }
   P.T.capacity = capacity;
-  GH_TRY(fill_u64(ctx, static_cast<uint64_t *>(djoined), 0, 1));
   if (frag->nkeys == 0) {  // one group, even over no rows
     if (P.pipe_form) GH_TRY(fill_u64(ctx, P.T.keys, 0ull, 1));  // direct slot 0 holds key 0
     else GH_TRY(group_table_preinsert(ctx, P.T, 0ull));

@@ -690,24 +690,41 @@ int group_table_alloc(ghive_ctx *ctx, GroupTableDev &T, uint64_t capacity, DevBu
   return group_table_init(ctx, T, keys.as<uint64_t>(), acc.as<uint64_t>(), counters.as<unsigned long long>());
 }
 
-int group_table_init(ghive_ctx *ctx, GroupTableDev &T, uint64_t *keys, uint64_t *acc, unsigned long long *counters) {
-  const uint64_t slots = T.stride - 1;
+// The whole initial state of a group table in one launch: keys EMPTY (the extra slot 0), every
+// record's accumulator words at their identities (init[w]: MIN / MAX start from the extreme
+// value, the rest from 0), and the table's counters 0.
+struct TableInit {
+  uint64_t init[32];  // per record word (rec <= 32)
+};
+__global__ void k_group_table_init(uint64_t *keys, uint64_t *acc, uint64_t stride, uint32_t rec, TableInit I,
+                                   unsigned long long *counters, uint32_t ncounters) {
+  const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = i0; i < stride; i += step) keys[i] = i + 1 < stride ? kEmptyKey : 0ull;
+  for (uint64_t i = i0; i < stride * rec; i += step) acc[i] = I.init[i % rec];
+  if (i0 < ncounters) counters[i0] = 0ull;
+}
+
+int group_table_init(ghive_ctx *ctx, GroupTableDev &T, uint64_t *keys, uint64_t *acc, unsigned long long *counters,
+                     uint32_t ncounters) {
   T.keys = keys;
   T.acc = acc;
   T.nent = counters;
   T.overflow = T.nent + 1;
-  GH_TRY(fill_u64(ctx, T.keys, kEmptyKey, slots));
-  GH_TRY(fill_u64(ctx, T.keys + slots, 0, 1));
-  GH_TRY(fill_u64(ctx, reinterpret_cast<uint64_t *>(T.nent), 0, 2));
-  GH_TRY(fill_u64(ctx, T.acc, 0, T.stride * (uint64_t)T.rec));
+  TableInit I;
+  memset(&I, 0, sizeof I);
+  if (T.rec > 32) return set_error(ctx, GHIVE_ERR_UNSUPPORTED, "group table: %d words per record", T.rec);
   for (int a = 0; a < T.naggs; ++a) {  // MIN / MAX start from their identities
     const AggSpec &A = T.aggs[a];
-    uint64_t v = 0;
-    if (A.kind == AK_MINI) v = 0x7fffffffffffffffull;
-    if (A.kind == AK_MAXI) v = 0x8000000000000000ull;
-    if (A.kind == AK_MIND) v = ~0ull;
-    if (v) GH_TRY(fill_u64_strided(ctx, T.acc + A.word0, v, T.stride, (uint64_t)T.rec));
+    if (A.kind == AK_MINI) I.init[A.word0] = 0x7fffffffffffffffull;
+    if (A.kind == AK_MAXI) I.init[A.word0] = 0x8000000000000000ull;
+    if (A.kind == AK_MIND) I.init[A.word0] = ~0ull;
   }
+  const uint64_t words = T.stride * (uint64_t)T.rec;
+  const uint64_t g = (words + 255) / 256, cap = (uint64_t)ctx->cus * 8;
+  k_group_table_init<<<(uint32_t)(g < 1 ? 1 : (g > cap ? cap : g)), 256, 0, ctx->stream>>>(
+      T.keys, T.acc, T.stride, (uint32_t)T.rec, I, counters, ncounters < 2 ? 2 : ncounters);
+  GH_LAUNCH(ctx);
   return GHIVE_OK;
 }
 

@@ -36,7 +36,9 @@ int lds_log2_for(int nwords, int budget_bytes);
 // bytes of its key and accumulator arrays (the two counters take 16 more bytes).
 void group_table_size(GroupTableDev &T, uint64_t capacity, uint64_t *key_bytes, uint64_t *acc_bytes);
 // Initialises a table sized by group_table_size in caller memory (fill kernels only).
-int group_table_init(ghive_ctx *ctx, GroupTableDev &T, uint64_t *keys, uint64_t *acc, unsigned long long *counters);
+// ncounters: table counters zeroed with it (entries, overflow, then any the caller keeps after them)
+int group_table_init(ghive_ctx *ctx, GroupTableDev &T, uint64_t *keys, uint64_t *acc, unsigned long long *counters,
+                     uint32_t ncounters = 2);
 // Allocates and initialises the global table for `capacity` groups (T.aggs/naggs/nwords set).
 int group_table_alloc(ghive_ctx *ctx, GroupTableDev &T, uint64_t capacity, DevBuf &keys, DevBuf &acc,
                       DevBuf &counters);

@@ -2,6 +2,7 @@
 // (gpu_impl/src/Operator/SortMergeJoinPredicate.cu:8-461 produces the same result_offsets
 // with a GPU sort plus a single-threaded CPU merge; here both sides stay on the device).
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "common.hpp"
@@ -361,28 +362,52 @@ __global__ __launch_bounds__(kProbeThreads) void k_probe_unique(
   }
 }
 
-// bitmap bit and dense payload of every inserted key (unique keys, so no write races on dense)
-__global__ __launch_bounds__(kHashThreads) void k_dense_fill(const void *keys, int key_type, const uint32_t *sel,
-                                                             uint64_t n, const uint64_t *n_dev, const void *pay,
-                                                             int pay_type, int64_t key_min, uint64_t range,
-                                                             uint32_t *bitmap, int32_t *dense) {
-  if (n_dev && *n_dev < n) n = *n_dev;
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Dense companions of up to kDenseJobs tables per launch (blockIdx.y = table): one launch clears
+// their bitmaps, the next sets the bit and dense payload of every inserted key (unique keys, so no
+// write races on dense).
+constexpr int kDenseJobs = 8;
+struct DenseJob {
+  const void *key;
+  const uint32_t *sel;
+  const uint64_t *n_dev;  // the row count on the device (at most n), optional
+  const void *pay;
+  uint64_t n;
+  int64_t key_min;
+  uint64_t range;
+  uint32_t *bitmap;
+  int32_t *dense;
+  int key_type, pay_type;
+};
+struct DenseJobs {
+  DenseJob j[kDenseJobs];
+};
+
+__global__ __launch_bounds__(kHashThreads) void k_dense_clear(DenseJobs J) {
+  const DenseJob &d = J.j[blockIdx.y];
+  const uint64_t words = (d.range + 31) / 32;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) {
-    uint64_t r = sel ? sel[i] : i;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride) d.bitmap[i] = 0u;
+}
+
+__global__ __launch_bounds__(kHashThreads) void k_dense_fill(DenseJobs J) {
+  const DenseJob &d = J.j[blockIdx.y];
+  uint64_t n = d.n;
+  if (d.n_dev && *d.n_dev < n) n = *d.n_dev;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint64_t r = d.sel ? d.sel[i] : i;
     int64_t k;
-    if (!col_load_i64(keys, key_type, r, k)) continue;
-    uint64_t idx = (uint64_t)(k - key_min);
-    if (idx >= range) continue;
+    if (!col_load_i64(d.key, d.key_type, r, k)) continue;
+    const uint64_t idx = (uint64_t)(k - d.key_min);
+    if (idx >= d.range) continue;
     int64_t p;
-    if (pay) {
-      if (!col_load_i64(pay, pay_type, r, p)) p = 0x7fffffffll;
+    if (d.pay) {
+      if (!col_load_i64(d.pay, d.pay_type, r, p)) p = 0x7fffffffll;
     } else {
       p = (int64_t)r;
     }
-    atomicOr(&bitmap[idx >> 5], 1u << (idx & 31));
-    dense[idx] = (int32_t)p;
+    atomicOr(&d.bitmap[idx >> 5], 1u << (idx & 31));
+    d.dense[idx] = (int32_t)p;
   }
 }
 
@@ -418,9 +443,12 @@ static uint32_t grid_for(ghive_ctx *ctx, uint64_t n, uint32_t threads = kHashThr
 // companion of a unique INT table whose keys span a compact range (surrogate keys of a star
 // schema): an existence bitmap over [key_min, key_max] and a payload array indexed by
 // key - key_min. A probe is then one bit test (LDS-resident when small) plus, for survivors, one
-// load -- no probe chain. Rows: sel[0 .. min(n, *n_dev)) (n_dev optional).
-static int finish_build(ghive_ctx *ctx, ghive_hash_table *h, const uint64_t (&host)[kStatWords], const void *key,
-                        const uint32_t *sel, uint64_t n, const uint64_t *n_dev, const void *pay, int pay_type) {
+// load -- no probe chain. Rows: sel[0 .. min(n, *n_dev)) (n_dev optional). When the table gets a
+// companion, *job describes its fill (dense_launch) and *has_job is set.
+static void finish_stats(ghive_ctx *ctx, ghive_hash_table *h, const uint64_t (&host)[kStatWords], const void *key,
+                         const uint32_t *sel, uint64_t n, const uint64_t *n_dev, const void *pay, int pay_type,
+                         DenseJob *job, bool *has_job) {
+  *has_job = false;
   h->entries = host[0];
   h->unique = host[1] == 0;
   h->key_min = (int64_t)host[2];
@@ -429,7 +457,7 @@ static int finish_build(ghive_ctx *ctx, ghive_hash_table *h, const uint64_t (&ho
   h->pay_max = (int64_t)host[5];
   const uint64_t range = h->entries ? (uint64_t)(h->key_max - h->key_min) + 1 : 0;
   if (!(h->layout == 8 && h->unique && h->entries && range <= (1ull << 27) && range <= 64 * h->entries + 65536))
-    return GHIVE_OK;
+    return;
   const uint64_t nwords = (range + 31) / 32;
   h->bitmap = static_cast<uint32_t *>(ctx->arena->alloc(nwords * 4));
   h->dense = static_cast<int32_t *>(ctx->arena->alloc(range * 4));
@@ -438,14 +466,39 @@ static int finish_build(ghive_ctx *ctx, ghive_hash_table *h, const uint64_t (&ho
     if (h->dense) ctx->arena->free(h->dense);
     h->bitmap = nullptr;
     h->dense = nullptr;
-    return GHIVE_OK;
+    return;
   }
   h->range = range;
-  GH_TRY(fill_u32(ctx, h->bitmap, 0u, nwords));
-  k_dense_fill<<<grid_for(ctx, n), kHashThreads, 0, ctx->stream>>>(key, h->key_type, sel, n, n_dev, pay, pay_type,
-                                                                   h->key_min, range, h->bitmap, h->dense);
-  if (hipGetLastError() != hipSuccess) return set_error(ctx, GHIVE_ERR_HIP, "hash_build: dense launch failed");
+  *job = DenseJob{key, sel, n_dev, pay, n, h->key_min, range, h->bitmap, h->dense, h->key_type, pay_type};
+  *has_job = true;
+}
+
+// The fills of njobs dense companions: one clearing launch and one filling launch per
+// kDenseJobs tables.
+static int dense_launch(ghive_ctx *ctx, const DenseJob *jobs, int njobs) {
+  for (int b = 0; b < njobs; b += kDenseJobs) {
+    const int nb = std::min(njobs - b, kDenseJobs);
+    DenseJobs J;
+    memset(&J, 0, sizeof J);
+    uint64_t words = 1, rows = 1;
+    for (int i = 0; i < nb; ++i) {
+      J.j[i] = jobs[b + i];
+      words = std::max(words, (jobs[b + i].range + 31) / 32);
+      rows = std::max(rows, jobs[b + i].n);
+    }
+    k_dense_clear<<<dim3(grid_for(ctx, words), nb), kHashThreads, 0, ctx->stream>>>(J);
+    k_dense_fill<<<dim3(grid_for(ctx, rows), nb), kHashThreads, 0, ctx->stream>>>(J);
+    if (hipGetLastError() != hipSuccess) return set_error(ctx, GHIVE_ERR_HIP, "hash_build: dense launch failed");
+  }
   return GHIVE_OK;
+}
+
+static int finish_build(ghive_ctx *ctx, ghive_hash_table *h, const uint64_t (&host)[kStatWords], const void *key,
+                        const uint32_t *sel, uint64_t n, const uint64_t *n_dev, const void *pay, int pay_type) {
+  DenseJob job;
+  bool has_job = false;
+  finish_stats(ctx, h, host, key, sel, n, n_dev, pay, pay_type, &job, &has_job);
+  return has_job ? dense_launch(ctx, &job, 1) : GHIVE_OK;
 }
 
 // Rows sel[0 .. n) (n_dev: the count on the device, at most n). Synchronous unless `async`:
@@ -624,6 +677,8 @@ int hash_tables_ready(ghive_ctx *caller, const ghive_hash_table *const *hs, int 
         st = set_error(caller, GHIVE_ERR_HIP, "hash table: statistics read-back failed");
     if (st == GHIVE_OK && hipStreamSynchronize(caller->stream) != hipSuccess)
       st = set_error(caller, GHIVE_ERR_HIP, "hash table: statistics read-back failed");
+    std::vector<DenseJob> jobs;  // the dense companions of these tables, filled together
+    std::vector<ghive_hash_table *> done;
     for (size_t i = 0; i < todo.size(); ++i) {
       ghive_hash_table *h = todo[i];
       if (st != GHIVE_OK) {
@@ -635,14 +690,21 @@ int hash_tables_ready(ghive_ctx *caller, const ghive_hash_table *const *hs, int 
       caller->arena->free(h->stats_buf);  // the read-back above has completed
       h->stats_buf = nullptr;
       h->stats_dev = nullptr;
-      const int s = finish_build(caller, h, host, h->in_key, h->in_sel, h->in_n_max, h->in_n_dev, h->in_pay,
-                                 h->in_pay_type);
-      if (s != GHIVE_OK) {
-        st = fail_table(caller, h, s);
-        continue;
-      }
-      h->pending.store(false);
+      DenseJob job;
+      bool has_job = false;
+      finish_stats(caller, h, host, h->in_key, h->in_sel, h->in_n_max, h->in_n_dev, h->in_pay, h->in_pay_type, &job,
+                   &has_job);
+      if (has_job) jobs.push_back(job);
+      done.push_back(h);
     }
+    if (st == GHIVE_OK && !jobs.empty()) {
+      const int s = dense_launch(caller, jobs.data(), (int)jobs.size());
+      if (s != GHIVE_OK) {
+        for (ghive_hash_table *h : done) fail_table(caller, h, s);
+        return s;
+      }
+    }
+    for (ghive_hash_table *h : done) h->pending.store(false);
     if (st != GHIVE_OK) return st;
   }
   return GHIVE_OK;
