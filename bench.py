@@ -356,6 +356,13 @@ def skew_section(ctx, reps, with_cpu=True, gby_form="auto"):
         "build": {"rows": nb, "ms": 1e3 * t_build, "rows_per_s": nb / t_build, "gbs": gbs(b_build, t_build)},
         "probe": {"rows": npr, "matches": matches, "ms": 1e3 * t_probe, "rows_per_s": npr / t_probe,
                   "gbs": gbs(b_probe, t_probe), "frac": gbs(b_probe, t_probe) / HBM_PEAK_GBS},
+        # build + probe as one stage: probe keys in, build (key, payload) in, matches out as this
+        # build writes them (4-byte probe row id + 8-byte payload); bytes_16 counts each match as a
+        # 16-byte (key, payload) pair, the form of the join's algorithmic bytes in its specification
+        "join": {"ms": 1e3 * (t_build + t_probe), "bytes": b_build + b_probe,
+                 "gbs": gbs(b_build + b_probe, t_build + t_probe),
+                 "frac": gbs(b_build + b_probe, t_build + t_probe) / HBM_PEAK_GBS,
+                 "bytes_16": npr * 8 + nb * 16 + matches * 16},
         "sort": {"rows": matches, "ms": 1e3 * t_sort, "rows_per_s": matches / t_sort,
                  "gbs_lower_bound_bytes": gbs(b_sort, t_sort),
                  # P x 2 x 16 B x N: P passes, each reading and writing a 16-byte (key, row id) record
