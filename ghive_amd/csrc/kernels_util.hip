@@ -160,6 +160,31 @@ __global__ __launch_bounds__(256) void k_stream_read(const v4u *p, uint64_t n16,
   if (x == 0x9e3779b9u) atomicAdd(sink, 1ull);  // keeps the loads alive; almost never taken
 }
 
+// The fused star-join kernel's pattern without its work: src holds kStreamCols equal columns and
+// each workgroup streams the same contiguous range of every column, a 16-byte non-temporal load per
+// column per lane, two row groups in flight.
+constexpr int kStreamCols = 5;
+__global__ __launch_bounds__(256) void k_stream_read_cols(const v4u *p, uint64_t n16, unsigned long long *sink) {
+  const uint64_t b0 = n16 * blockIdx.x / gridDim.x, b1 = n16 * (blockIdx.x + 1) / gridDim.x;
+  v4u acc = {0u, 0u, 0u, 0u};
+  uint64_t i = b0 + threadIdx.x;
+  for (; i + blockDim.x < b1; i += 2 * blockDim.x) {
+    v4u x[2 * kStreamCols];
+#pragma unroll
+    for (int c = 0; c < kStreamCols; ++c) {
+      x[2 * c] = __builtin_nontemporal_load(p + (uint64_t)c * n16 + i);
+      x[2 * c + 1] = __builtin_nontemporal_load(p + (uint64_t)c * n16 + i + blockDim.x);
+    }
+#pragma unroll
+    for (int c = 0; c < 2 * kStreamCols; ++c) acc ^= x[c];
+  }
+  for (; i < b1; i += blockDim.x)
+#pragma unroll
+    for (int c = 0; c < kStreamCols; ++c) acc ^= __builtin_nontemporal_load(p + (uint64_t)c * n16 + i);
+  uint32_t x = acc.x ^ acc.y ^ acc.z ^ acc.w;
+  if (x == 0x9e3779b9u) atomicAdd(sink, 1ull);  // keeps the loads alive; almost never taken
+}
+
 __global__ __launch_bounds__(256) void k_stream_copy(const v4u *src, v4u *dst, uint64_t n16) {
   const uint64_t b0 = n16 * blockIdx.x / gridDim.x, b1 = n16 * (blockIdx.x + 1) / gridDim.x;
   uint64_t i = b0 + threadIdx.x;
@@ -177,8 +202,9 @@ extern "C" int ghive_bench_stream(ghive_ctx *ctx, void *dst, const void *src, ui
                                   int iters, float *ms_per_iter) {
   using namespace ghive;
   GH_CHECK_CTX(ctx);
-  if (!src || (mode == 1 && !dst) || bytes < 16 || (bytes & 15) || iters < 1 || !ms_per_iter ||
-      ((uintptr_t)src & 15) || (dst && ((uintptr_t)dst & 15)))
+  if (!src || (mode == 1 && !dst) || mode < 0 || mode > 2 || bytes < 16 || (bytes & 15) || iters < 1 ||
+      !ms_per_iter || ((uintptr_t)src & 15) || (dst && ((uintptr_t)dst & 15)) ||
+      (mode == 2 && bytes % (16 * kStreamCols)))
     return set_error(ctx, GHIVE_ERR_INVALID, "bench_stream: bad arguments");
   const uint64_t n16 = bytes / 16;
   const uint32_t grid = (uint32_t)ctx->cus * 8;
@@ -187,6 +213,9 @@ extern "C" int ghive_bench_stream(ghive_ctx *ctx, void *dst, const void *src, ui
     if (mode == 0)
       k_stream_read<<<grid, 256, 0, ctx->stream>>>(static_cast<const v4u *>(src), n16,
                                                     reinterpret_cast<unsigned long long *>(ctx->d_counters));
+    else if (mode == 2)
+      k_stream_read_cols<<<grid, 256, 0, ctx->stream>>>(static_cast<const v4u *>(src), n16 / kStreamCols,
+                                                         reinterpret_cast<unsigned long long *>(ctx->d_counters));
     else
       k_stream_copy<<<grid, 256, 0, ctx->stream>>>(static_cast<const v4u *>(src), static_cast<v4u *>(dst), n16);
     GH_LAUNCH(ctx);

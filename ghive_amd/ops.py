@@ -613,7 +613,8 @@ class Batch:
 
 
 def bench_stream(ctx, nbytes, mode=0, iters=10):
-    """Achievable HBM bandwidth (GB/s): mode 0 read, mode 1 copy (counts read + write)."""
+    """Achievable HBM bandwidth (GB/s): mode 0 read, mode 1 copy (counts read + write), mode 2 read
+    as five columns streamed side by side (nbytes a multiple of 80)."""
     src = torch.empty(nbytes // 4, dtype=torch.int32, device=f"cuda:{ctx.device}").fill_(1)
     dst = torch.empty_like(src) if mode == 1 else None
     ms = C.c_float()

@@ -120,8 +120,10 @@ int ghive_memcpy(ghive_ctx *ctx, void *dst, const void *src, uint64_t bytes, int
 int ghive_mem_stats(ghive_ctx *ctx, uint64_t *in_use, uint64_t *peak, uint64_t *limit,
                     uint64_t *reserved);
 /* STREAM-style probe of achievable HBM bandwidth (no counterpart in the reference): mode 0
- * reads `bytes` from src, mode 1 copies src -> dst; 16-byte aligned, iters launches, mean time
- * per launch from HIP events. Calibrates roofline fractions and PMC byte counters. */
+ * reads `bytes` from src, mode 1 copies src -> dst, mode 2 reads src as five equal columns with
+ * every workgroup streaming the same row range of each (the fused star-join kernel's pattern;
+ * bytes a multiple of 80); 16-byte aligned, iters launches, mean time per launch from HIP events.
+ * Calibrates roofline fractions and PMC byte counters. */
 int ghive_bench_stream(ghive_ctx *ctx, void *dst, const void *src, uint64_t bytes, int mode,
                        int iters, float *ms_per_iter);
 

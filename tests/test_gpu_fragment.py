@@ -390,6 +390,14 @@ def test_bandwidth_probe_copies_and_reads(ctx, nbytes):
     assert torch.equal(dst, src)
     ctx.check(lib().ghive_bench_stream(ctx.h, None, C.c_void_p(src.data_ptr()), nbytes, 0, 2, C.byref(ms)))
     assert ms.value > 0
+    # the five-column form takes whole 80-byte rows of its five columns, and rejects the rest
+    st = lib().ghive_bench_stream(ctx.h, None, C.c_void_p(src.data_ptr()), nbytes, 2, 2, C.byref(ms))
+    if nbytes % 80:
+        assert st == abi.ERR_INVALID
+    else:
+        ctx.check(st)
+    ctx.check(lib().ghive_bench_stream(ctx.h, None, C.c_void_p(src.data_ptr()), nbytes - nbytes % 80, 2, 2,
+                                       C.byref(ms)) if nbytes >= 80 else abi.OK)
 
 
 def test_async_filter_and_build_match_the_waiting_forms(ctx, orc):
