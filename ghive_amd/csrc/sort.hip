@@ -51,6 +51,104 @@ __device__ __forceinline__ uint64_t field_bits(int type, const void *p, uint64_t
   }
 }
 
+// field_bits of key k for R rows at once: the type is chosen once and the R loads issue together.
+template <int R>
+__device__ __forceinline__ void field_bits_rows(const SortKeySpec &S, int k, const uint64_t (&r)[R],
+                                                uint64_t (&b)[R]) {
+  const void *p = S.data[k];
+  switch (S.type[k]) {
+    case GHIVE_INT: {
+      int32_t x[R];
+#pragma unroll
+      for (int j = 0; j < R; ++j) x[j] = static_cast<const int32_t *>(p)[r[j]];
+#pragma unroll
+      for (int j = 0; j < R; ++j) b[j] = (uint32_t)x[j] ^ 0x80000000u;
+      break;
+    }
+    case GHIVE_LONG: {
+      int64_t x[R];
+#pragma unroll
+      for (int j = 0; j < R; ++j) x[j] = static_cast<const int64_t *>(p)[r[j]];
+#pragma unroll
+      for (int j = 0; j < R; ++j) b[j] = (uint64_t)x[j] ^ 0x8000000000000000ull;
+      break;
+    }
+    case GHIVE_FLOAT: {
+      float x[R];
+#pragma unroll
+      for (int j = 0; j < R; ++j) x[j] = static_cast<const float *>(p)[r[j]];
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const uint32_t u = __builtin_isnan(x[j]) ? 0x7fc00000u : (x[j] == 0.0f ? 0u : __float_as_uint(x[j]));
+        b[j] = (u >> 31) ? (uint32_t)~u : (u | 0x80000000u);
+      }
+      break;
+    }
+    default: {
+      double x[R];
+#pragma unroll
+      for (int j = 0; j < R; ++j) x[j] = static_cast<const double *>(p)[r[j]];
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const uint64_t u = __builtin_isnan(x[j]) ? 0x7ff8000000000000ull
+                                                 : (x[j] == 0.0 ? 0ull : (uint64_t)__double_as_longlong(x[j]));
+        b[j] = (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+      }
+      break;
+    }
+  }
+}
+
+// The composite keys of R rows as KW words each (w[j][0] least significant), key by key with each
+// key's R loads in flight together; embed_id: the row id fills the low 32 bits of word 0.
+template <int KW, int R>
+__device__ __forceinline__ void sort_words_rows(const SortKeySpec &S, const uint64_t (&r)[R], int embed_id,
+                                                uint64_t (&w)[R][KW]) {
+#pragma unroll
+  for (int j = 0; j < R; ++j)
+#pragma unroll
+    for (int q = 0; q < KW; ++q) w[j][q] = 0;
+  for (int k = 0; k < S.nkeys; ++k) {
+    uint64_t b[R];
+    field_bits_rows<R>(S, k, r, b);
+    const uint64_t mask = S.width[k] < 64 ? (1ull << S.width[k]) - 1 : ~0ull, flip = S.desc[k] ? ~0ull : 0ull;
+    const int sh = S.shift[k], word = sh / 64, off = sh % 64;
+    const bool spill = off + S.width[k] > 64;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const uint64_t v = (b[j] ^ flip) & mask;
+#pragma unroll
+      for (int q = 0; q < KW; ++q) {
+        if (q == word) w[j][q] |= v << off;
+        if (q == word + 1 && spill) w[j][q] |= v >> (64 - off);
+      }
+    }
+  }
+  if (embed_id)
+#pragma unroll
+    for (int j = 0; j < R; ++j) w[j][0] |= (uint32_t)r[j];
+}
+
+// The composite key of row r as KW words (w[0] least significant); embed_id: the row id fills the
+// low 32 bits of word 0 (the fields start at bit 32).
+template <int KW>
+__device__ __forceinline__ void sort_words(const SortKeySpec &S, uint64_t r, int embed_id, uint64_t (&w)[KW]) {
+#pragma unroll
+  for (int q = 0; q < KW; ++q) w[q] = 0;
+  for (int k = 0; k < S.nkeys; ++k) {
+    uint64_t b = field_bits(S.type[k], S.data[k], r);
+    if (S.desc[k]) b = ~b;
+    if (S.width[k] < 64) b &= (1ull << S.width[k]) - 1;
+    const int sh = S.shift[k], word = sh / 64, off = sh % 64;
+#pragma unroll
+    for (int q = 0; q < KW; ++q) {
+      if (q == word) w[q] |= b << off;
+      if (q == word + 1 && off + S.width[k] > 64) w[q] |= b >> (64 - off);
+    }
+  }
+  if (embed_id) w[0] |= (uint32_t)r;
+}
+
 // embed_id: the row id fills the low 32 bits of word 0 (the fields start at bit 32), so the passes
 // carry no separate id array; otherwise idx[i] = i.
 template <int KW>
@@ -66,20 +164,7 @@ __global__ void k_sort_keys(SortKeySpec S, uint64_t n, uint64_t *keys, uint32_t 
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) {
     uint64_t w[KW];  // w[0] = least significant 64 bits
-#pragma unroll
-    for (int q = 0; q < KW; ++q) w[q] = 0;
-    for (int k = 0; k < S.nkeys; ++k) {
-      uint64_t b = field_bits(S.type[k], S.data[k], i);
-      if (S.desc[k]) b = ~b;
-      if (S.width[k] < 64) b &= (1ull << S.width[k]) - 1;
-      const int sh = S.shift[k], word = sh / 64, off = sh % 64;
-#pragma unroll
-      for (int q = 0; q < KW; ++q) {
-        if (q == word) w[q] |= b << off;
-        if (q == word + 1 && off + S.width[k] > 64) w[q] |= b >> (64 - off);
-      }
-    }
-    if (embed_id) w[0] |= (uint32_t)i;
+    sort_words<KW>(S, i, embed_id, w);
 #pragma unroll
     for (int q = 0; q < KW; ++q) {
       keys[(uint64_t)q * n + i] = w[q];
@@ -116,6 +201,34 @@ __global__ __launch_bounds__(TH) void k_radix_hist(const uint64_t *keys, uint64_
     uint64_t e = t0 + (uint64_t)j * TH + threadIdx.x;
     if (e < n) atomicAdd(&h[(kw[e] >> shift) & 255u], 1u);
   }
+  __syncthreads();
+  for (uint32_t d = threadIdx.x; d < 256; d += TH) {
+    hist[(uint64_t)d * nblocks + blockIdx.x] = h[d];
+    counts[(uint64_t)blockIdx.x * 256 + d] = h[d];
+  }
+}
+
+// The same counts for the first pass of a sort, whose digit is the lowest byte of the last key
+// field (column S.data[k]), read from that column: no composite keys exist yet. Every load of the
+// tile is issued before its LDS counts.
+template <int TH, int PER>
+__global__ __launch_bounds__(TH) void k_radix_hist_key(SortKeySpec S, int k, uint64_t n, uint32_t *hist,
+                                                       uint32_t *counts, uint32_t nblocks) {
+  __shared__ uint32_t h[256];
+  for (uint32_t d = threadIdx.x; d < 256; d += TH) h[d] = 0;
+  __syncthreads();
+  const uint64_t t0 = (uint64_t)blockIdx.x * (uint64_t)(TH * PER);
+  uint64_t r[PER], b[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const uint64_t e = t0 + (uint64_t)j * TH + threadIdx.x;
+    r[j] = e < n ? e : n - 1;
+  }
+  field_bits_rows<PER>(S, k, r, b);
+  const uint32_t flip = S.desc[k] ? 255u : 0u;
+#pragma unroll
+  for (int j = 0; j < PER; ++j)
+    if (t0 + (uint64_t)j * TH + threadIdx.x < n) atomicAdd(&h[((uint32_t)b[j] ^ flip) & 255u], 1u);
   __syncthreads();
   for (uint32_t d = threadIdx.x; d < 256; d += TH) {
     hist[(uint64_t)d * nblocks + blockIdx.x] = h[d];
@@ -181,12 +294,17 @@ __global__ __launch_bounds__(TH) void k_radix_hist_plane(const uint8_t *plane, u
 // the id, so word 0 leaves as a uint32 id array (idx_out) and words 1.. as a (KW - 1)-word array.
 // plane_out (optional): the next pass's digit, byte (next_shift / 8) of input word next_word, is
 // written beside every row as a one-byte plane for that pass's histogram.
-template <int KW, bool IDS, bool DROP0, int TH, int PER>
+// COLS: the sort's first pass. It builds each row's composite key from the key columns (S, embed)
+// instead of reading key words, its row ids are the row numbers, and every workgroup leaves the OR
+// and AND of each key word over its tile in orand_part[block * 2 KW ..] (which later digits vary).
+template <int KW, bool IDS, bool DROP0, int TH, int PER, bool COLS = false>
 __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, const uint32_t *idx_in,
                                                       uint64_t *keys_out, uint32_t *idx_out, uint64_t n,
                                                       int word, int shift, const uint32_t *offs,
                                                       const uint32_t *counts, uint32_t nblocks,
-                                                      uint8_t *plane_out, int next_word, int next_shift) {
+                                                      uint8_t *plane_out, int next_word, int next_shift,
+                                                      SortKeySpec S = SortKeySpec{}, int embed = 0,
+                                                      unsigned long long *orand_part = nullptr) {
   constexpr uint32_t kTile = (uint32_t)TH * PER, kWaves = TH / kWave;
   __shared__ uint64_t stage[kTile];
   __shared__ uint8_t sdigit[kTile];
@@ -211,13 +329,59 @@ __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, c
   // end a lane re-reads the last element (ignored)
   uint64_t kv[PER][KW];
   uint32_t id[PER], lpos[PER], dg[PER];
+  if constexpr (COLS) {
+    uint64_t rows[PER];
 #pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const uint32_t el = wave * kSeg + (uint32_t)j * kWave + lane;
-    const uint64_t e = t0 + (el < tn ? el : tn - 1);
+    for (int j = 0; j < PER; ++j) {
+      const uint32_t el = wave * kSeg + (uint32_t)j * kWave + lane;
+      rows[j] = t0 + (el < tn ? el : tn - 1);
+      id[j] = (uint32_t)rows[j];
+    }
+    sort_words_rows<KW, PER>(S, rows, embed, kv);
+  } else {
 #pragma unroll
-    for (int q = 0; q < KW; ++q) kv[j][q] = keys_in[(uint64_t)q * n + e];
-    id[j] = IDS ? idx_in[e] : 0;
+    for (int j = 0; j < PER; ++j) {
+      const uint32_t el = wave * kSeg + (uint32_t)j * kWave + lane;
+      const uint64_t e = t0 + (el < tn ? el : tn - 1);
+#pragma unroll
+      for (int q = 0; q < KW; ++q) kv[j][q] = keys_in[(uint64_t)q * n + e];
+      id[j] = IDS ? idx_in[e] : 0;
+    }
+  }
+  if constexpr (COLS) {  // OR / AND of every key word over the tile's rows
+    __shared__ unsigned long long ored[kWaves][2 * KW];
+    unsigned long long vor[KW], vand[KW];
+#pragma unroll
+    for (int q = 0; q < KW; ++q) {
+      vor[q] = 0ull;
+      vand[q] = ~0ull;
+    }
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+      if (wave * kSeg + (uint32_t)j * kWave + lane < tn)
+#pragma unroll
+        for (int q = 0; q < KW; ++q) {
+          vor[q] |= kv[j][q];
+          vand[q] &= kv[j][q];
+        }
+#pragma unroll
+    for (int q = 0; q < KW; ++q) {
+#pragma unroll
+      for (int d = kWave / 2; d > 0; d >>= 1) {
+        vor[q] |= __shfl_xor(vor[q], d, kWave);
+        vand[q] &= __shfl_xor(vand[q], d, kWave);
+      }
+      if (lane == 0) {
+        ored[wave][2 * q] = vor[q];
+        ored[wave][2 * q + 1] = vand[q];
+      }
+    }
+    __syncthreads();
+    if (tid < 2 * KW) {
+      unsigned long long r = ored[0][tid];
+      for (uint32_t w = 1; w < kWaves; ++w) r = (tid & 1) ? (r & ored[w][tid]) : (r | ored[w][tid]);
+      orand_part[(uint64_t)blockIdx.x * 2 * KW + tid] = r;
+    }
   }
   __syncthreads();
   // Rank every element among the equal digits of its wave's segment, in element order: ballots give
@@ -283,6 +447,91 @@ __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, c
         idx_out[g] = (uint32_t)stage[p];
       if (plane_out && q == next_word) plane_out[g] = (uint8_t)(stage[p] >> next_shift);
     }
+  }
+}
+
+// out[2q] = OR, out[2q + 1] = AND of key word q over the first pass's per-tile partials (one
+// workgroup of kOrandThreads; eight loads in flight per lane).
+constexpr int kOrandThreads = 1024;
+__global__ __launch_bounds__(kOrandThreads) void k_orand_reduce(const unsigned long long *part, uint32_t nblocks,
+                                                                int words2, unsigned long long *out) {
+  __shared__ unsigned long long red[kOrandThreads / kWave][8];
+  unsigned long long acc[8];
+  for (int q = 0; q < 8; ++q) acc[q] = (q & 1) ? ~0ull : 0ull;
+  constexpr int U = 8;
+  for (uint32_t b0 = 0; b0 < nblocks; b0 += kOrandThreads * U) {
+    for (int q = 0; q < words2; ++q) {
+      unsigned long long x[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t b = b0 + (uint32_t)u * kOrandThreads + threadIdx.x;
+        x[u] = b < nblocks ? part[(uint64_t)b * words2 + q] : ((q & 1) ? ~0ull : 0ull);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc[q] = (q & 1) ? (acc[q] & x[u]) : (acc[q] | x[u]);
+    }
+  }
+  for (int q = 0; q < words2; ++q) {
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+      const unsigned long long o = __shfl_xor(acc[q], d, kWave);
+      acc[q] = (q & 1) ? (acc[q] & o) : (acc[q] | o);
+    }
+    if (lane_id() == 0) red[threadIdx.x / kWave][q] = acc[q];
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < words2) {
+    const int q = (int)threadIdx.x;
+    unsigned long long r = red[0][q];
+    for (int w = 1; w < kOrandThreads / kWave; ++w) r = (q & 1) ? (r & red[w][q]) : (r | red[w][q]);
+    out[q] = r;
+  }
+}
+
+// The sort's first pass: a stable pass on `digit` (the lowest byte of the last key field), reading
+// the key columns themselves (no key-building pass before it): keys_out receives the KW-word
+// composite keys in that digit's order, ids_out the row ids when they are not embedded, plane_out
+// (optional) the plane of digit `next` for the next pass, and oa[2 KW] the OR / AND of every word.
+template <int KW, int TH, int PER>
+static int sort_first_pass(ghive_ctx *ctx, const SortKeySpec &S, bool embed, uint64_t n, int digit, uint64_t *keys_out,
+                           uint32_t *ids_out, uint8_t *plane_out, int next, unsigned long long *oa) {
+  const uint32_t nblocks = ceil_div(n, (uint64_t)TH * PER);
+  DevBuf hist, counts, part;
+  GH_TRY(hist.alloc(ctx, sizeof(uint32_t) * 256ull * nblocks));
+  GH_TRY(counts.alloc(ctx, sizeof(uint32_t) * 256ull * nblocks));
+  GH_TRY(part.alloc(ctx, sizeof(unsigned long long) * 2ull * KW * nblocks));
+  k_radix_hist_key<TH, PER><<<nblocks, TH, 0, ctx->stream>>>(S, S.nkeys - 1, n, hist.as<uint32_t>(),
+                                                             counts.as<uint32_t>(), nblocks);
+  GH_LAUNCH(ctx);
+  GH_TRY(exclusive_scan_u32(ctx, hist.as<uint32_t>(), hist.as<uint32_t>(), 256ull * nblocks, nullptr));
+  const int word = digit / 8, shift = 8 * (digit % 8);
+  const int nw = next >= 0 ? next / 8 : -1, ns = next >= 0 ? 8 * (next % 8) : 0;
+  if (embed)
+    k_radix_scatter<KW, false, false, TH, PER, true><<<nblocks, TH, 0, ctx->stream>>>(
+        nullptr, nullptr, keys_out, nullptr, n, word, shift, hist.as<uint32_t>(), counts.as<uint32_t>(), nblocks,
+        plane_out, nw, ns, S, 1, part.as<unsigned long long>());
+  else
+    k_radix_scatter<KW, true, false, TH, PER, true><<<nblocks, TH, 0, ctx->stream>>>(
+        nullptr, nullptr, keys_out, ids_out, n, word, shift, hist.as<uint32_t>(), counts.as<uint32_t>(), nblocks,
+        plane_out, nw, ns, S, 0, part.as<unsigned long long>());
+  GH_LAUNCH(ctx);
+  k_orand_reduce<<<1, kOrandThreads, 0, ctx->stream>>>(part.as<unsigned long long>(), nblocks, 2 * KW, oa);
+  GH_LAUNCH(ctx);
+  int in_b = 0;
+  for (int k = 0; k < S.nkeys; ++k) in_b += S.width[k] / 8;
+  ctx->sort_info.passes += 1;
+  ctx->sort_info.scatter_bytes += n * ((uint64_t)in_b + 8ull * KW + (embed ? 0 : 4) + (plane_out ? 1 : 0));
+  return GHIVE_OK;
+}
+
+template <int KW>
+static int sort_first(ghive_ctx *ctx, const SortKeySpec &S, bool embed, uint64_t n, int digit, uint64_t *keys_out,
+                      uint32_t *ids_out, uint8_t *plane_out, int next, unsigned long long *oa) {
+  switch (ctx->tune.sort_shape) {
+    case 0: return sort_first_pass<KW, kSortThreads, kSortPerThread>(ctx, S, embed, n, digit, keys_out, ids_out, plane_out, next, oa);
+    case 2: return sort_first_pass<KW, 512, 32>(ctx, S, embed, n, digit, keys_out, ids_out, plane_out, next, oa);
+    case 3: return sort_first_pass<KW, 1024, 8>(ctx, S, embed, n, digit, keys_out, ids_out, plane_out, next, oa);
+    default: return sort_first_pass<KW, 512, 16>(ctx, S, embed, n, digit, keys_out, ids_out, plane_out, next, oa);
   }
 }
 
@@ -571,26 +820,39 @@ int ghive_sort(ghive_ctx *ctx, const ghive_table *table, const uint32_t *key_col
   GH_TRY(k1.alloc(ctx, 8ull * KW * n));
   GH_TRY(i1.alloc(ctx, 4ull * n));
   GH_TRY(orand.alloc(ctx, 8 * sizeof(uint64_t)));
-  uint64_t init[8] = {0, ~0ull, 0, ~0ull, 0, ~0ull, 0, ~0ull};
-  GH_TRY(h2d(ctx, orand.get(), init, sizeof init));
   uint64_t g = (n + 255) / 256, cap = (uint64_t)ctx->cus * 8;
   uint32_t grid = (uint32_t)(g > cap ? cap : g);
   unsigned long long *oa_d = orand.as<unsigned long long>();
+  // every pass after the first reads its histogram from a one-byte digit plane that the pass
+  // before it wrote (two planes, alternating)
+  DevBuf plane_buf;
+  uint8_t *planes[2] = {nullptr, nullptr};
+  const uint64_t pad = (n + 16383) / 16384 * 16384;
+  GH_TRY(plane_buf.alloc(ctx, 2 * pad));
+  planes[0] = plane_buf.as<uint8_t>();
+  planes[1] = planes[0] + pad;
+  // The first pass sorts on the lowest byte of the last key field, reading the key columns: it
+  // builds the composite keys as it scatters them, so no pass writes them first. Whether the later
+  // digits vary is known only after it (the OR / AND of every word): a first digit equal in every
+  // row costs one pass that keeps the order, never a wrong result. Its plane is the next byte's.
+  const int first = S.shift[nkeys - 1] / 8;
+  const int guess = first + 1 < 8 * KW ? first + 1 : -1;
   switch (KW) {
-    case 1: k_sort_keys<1><<<grid, 256, 0, ctx->stream>>>(S, n, k0.as<uint64_t>(), out_perm, oa_d, embed); break;
-    case 2: k_sort_keys<2><<<grid, 256, 0, ctx->stream>>>(S, n, k0.as<uint64_t>(), out_perm, oa_d, embed); break;
-    case 3: k_sort_keys<3><<<grid, 256, 0, ctx->stream>>>(S, n, k0.as<uint64_t>(), out_perm, oa_d, embed); break;
-    default: k_sort_keys<4><<<grid, 256, 0, ctx->stream>>>(S, n, k0.as<uint64_t>(), out_perm, oa_d, embed); break;
+    case 1: GH_TRY(sort_first<1>(ctx, S, embed, n, first, k0.as<uint64_t>(), out_perm, planes[0], guess, oa_d)); break;
+    case 2: GH_TRY(sort_first<2>(ctx, S, embed, n, first, k0.as<uint64_t>(), out_perm, planes[0], guess, oa_d)); break;
+    case 3: GH_TRY(sort_first<3>(ctx, S, embed, n, first, k0.as<uint64_t>(), out_perm, planes[0], guess, oa_d)); break;
+    default: GH_TRY(sort_first<4>(ctx, S, embed, n, first, k0.as<uint64_t>(), out_perm, planes[0], guess, oa_d)); break;
   }
-  GH_LAUNCH(ctx);
   uint64_t oa[8];
   GH_TRY(d2h(ctx, oa, orand.get(), sizeof oa));
-  std::vector<int> digits;
+  std::vector<int> digits;  // the digits still to sort on, lowest first
   for (int q = 0; q < KW; ++q) {
     uint64_t varying = oa[2 * q] ^ oa[2 * q + 1];
     for (int b = 0; b < 8; ++b)
-      if (((varying >> (8 * b)) & 255u) && !(embed && q == 0 && b < 4)) digits.push_back(8 * q + b);
+      if (((varying >> (8 * b)) & 255u) && !(embed && q == 0 && b < 4) && 8 * q + b != first)
+        digits.push_back(8 * q + b);
   }
+  int pcur = !digits.empty() && digits.front() == guess ? 0 : -1;
   uint32_t *res = out_perm;
   uint64_t *K0 = k0.as<uint64_t>(), *K1 = k1.as<uint64_t>(), *kres = K0;
   uint32_t *I1 = i1.as<uint32_t>();
@@ -599,16 +861,11 @@ int ghive_sort(ghive_ctx *ctx, const ghive_table *table, const uint32_t *key_col
   // move (KW - 1) key words plus a 4-byte id instead of KW words.
   std::vector<int> d0, dhi;
   for (int d : digits) (d < 8 ? d0 : dhi).push_back(d);
-  // every pass after the first reads its histogram from a one-byte digit plane that the pass
-  // before it wrote (two planes, alternating)
-  DevBuf plane_buf;
-  uint8_t *planes[2] = {nullptr, nullptr};
-  int pcur = -1;
-  if (digits.size() > 1) {
-    const uint64_t pad = (n + 16383) / 16384 * 16384;
-    GH_TRY(plane_buf.alloc(ctx, 2 * pad));
-    planes[0] = plane_buf.as<uint8_t>();
-    planes[1] = planes[0] + pad;
+  if (embed && KW >= 2 && d0.empty() && !dhi.empty()) {
+    // the first pass was word 0's only varying digit: one pass on the next (constant) byte of word
+    // 0 keeps the order and leaves the ids as their own array
+    d0.push_back(first + 1);
+    pcur = -1;
   }
   if (embed && KW >= 2 && !d0.empty() && !dhi.empty()) {
     uint32_t *ids = nullptr;

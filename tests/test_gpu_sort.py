@@ -89,21 +89,28 @@ def test_sort_large_sorted_and_permutation(ctx):
 
 
 def test_sort_info_counts_the_passes_it_ran(ctx, orc):
-    """ghive_last_sort_info: a digit byte that is the same in every row costs no pass. LONG keys in
-    [0, 256): one varying byte, one pass moving an 8-byte key and a 4-byte id each way; INT keys in
-    [0, 256): the id rides in the low half of the word, one pass of 8 bytes each way; one value in
-    every row: no pass at all. The permutations still equal the oracle's."""
+    """ghive_last_sort_info: the first pass sorts on the lowest key byte while it builds the
+    composite keys from the key columns, and any later digit byte that is the same in every row
+    costs no pass. Keys in [0, 256) vary in that byte only: one pass. LONG: the 8-byte column in,
+    an 8-byte key word and a 4-byte id out, plus the 1-byte digit plane of the next byte; INT: the
+    4-byte column in, one 8-byte word out (the id in its low half) and the plane. Keys that vary
+    in a higher byte (LONG, multiples of 256): that byte's pass too, 8 + 4 bytes each way reading
+    the plane. One value in every row: the first pass alone. The permutations equal the oracle's."""
     from ghive_amd import ops
     n = 100_003
     rng = np.random.default_rng(5)
-    for dtype, key_words, embedded, per_row in ((np.int64, 1, False, 24), (np.int32, 1, True, 16)):
+    for dtype, key_words, embedded, per_row in ((np.int64, 1, False, 8 + 12 + 1), (np.int32, 1, True, 4 + 8 + 1)):
         k = rng.integers(0, 256, n).astype(dtype)
         _check(ctx, orc, [k], [0], ["+"])
         info = ops.sort_info(ctx)
         assert info == {"rows": n, "key_words": key_words, "passes": 1, "embedded_ids": int(embedded),
                         "scatter_bytes": per_row * n}
+    k = rng.integers(0, 256, n).astype(np.int64) * 256
+    _check(ctx, orc, [k], [0], ["+"])
+    info = ops.sort_info(ctx)
+    assert info["passes"] == 2 and info["scatter_bytes"] == (8 + 12 + 1) * n + (12 + 12) * n
     _check(ctx, orc, [np.full(n, 7, dtype=np.int64)], [0], ["-"])
-    assert ops.sort_info(ctx)["passes"] == 0
+    assert ops.sort_info(ctx)["passes"] == 1
 
 
 @pytest.mark.parametrize("shape", [0, 1, 2, 3])
@@ -127,11 +134,12 @@ def test_sort_tile_shapes(ctx, orc, shape, n):
 
 
 def test_sort_info_counts_digit_planes(ctx, orc):
-    """Two varying bytes: the first pass writes the second pass's digit as a one-byte plane (one
-    byte a row more on write-out), the second histogram reads it; the order is the oracle's."""
+    """Two varying bytes: the first pass (the 8-byte key column in, key word and id out) writes the
+    second pass's digit as a one-byte plane (one byte a row more on write-out), the second histogram
+    reads it; the order is the oracle's."""
     from ghive_amd import ops
     n = 70_001
     k = np.random.default_rng(8).integers(0, 65536, n).astype(np.int64)
     _check(ctx, orc, [k], [0], ["+"])
     info = ops.sort_info(ctx)
-    assert info["passes"] == 2 and info["scatter_bytes"] == (24 + 1) * n + 24 * n
+    assert info["passes"] == 2 and info["scatter_bytes"] == (8 + 12 + 1) * n + 24 * n
