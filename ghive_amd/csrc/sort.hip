@@ -2,8 +2,10 @@
 // type-generic comparator plus a CPU permutation gather, gpu_impl/src/Operator/SinkOperator.cu:
 // 135-288; the sort itself is thrust::sequence + thrust::sort(sort_comparator) at :218-222). Keys are turned into one order-preserving unsigned bit string per row (first key in
 // the most significant bits; DESC inverts its field; NaN is canonical and sorts after +inf), then
-// sorted 8 bits at a time together with the row id. Digits that are equal in every row are
-// skipped. Each pass: per-tile LDS histogram -> scan of the digit x tile matrix -> stable scatter
+// sorted 8 bits at a time together with the row id. The first pass builds those bit strings from
+// the key columns as it scatters them (on the lowest key byte) and collects which bytes vary;
+// later digits that are equal in every row are skipped. Each pass: per-tile LDS histogram -> scan
+// of the digit x tile matrix -> stable scatter
 // ranked with wave64 ballots and staged through LDS in digit order, so that each digit's run is
 // written with consecutive addresses. The scatter also writes the next pass's digit as a one-byte
 // plane, so every histogram after the first reads one byte per row instead of a key word. Equal keys keep ascending row ids, so the result is a total
@@ -34,24 +36,9 @@ struct SortKeySpec {
   const void *data[8];
 };
 
-__device__ __forceinline__ uint64_t field_bits(int type, const void *p, uint64_t r) {
-  switch (type) {
-    case GHIVE_INT: return (uint32_t) static_cast<const int32_t *>(p)[r] ^ 0x80000000u;
-    case GHIVE_LONG: return (uint64_t) static_cast<const int64_t *>(p)[r] ^ 0x8000000000000000ull;
-    case GHIVE_FLOAT: {
-      float f = static_cast<const float *>(p)[r];
-      uint32_t b = __builtin_isnan(f) ? 0x7fc00000u : (f == 0.0f ? 0u : __float_as_uint(f));
-      return (b >> 31) ? (uint32_t)~b : (b | 0x80000000u);
-    }
-    default: {
-      double d = static_cast<const double *>(p)[r];
-      uint64_t b = __builtin_isnan(d) ? 0x7ff8000000000000ull : (d == 0.0 ? 0ull : (uint64_t)__double_as_longlong(d));
-      return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-    }
-  }
-}
-
-// field_bits of key k for R rows at once: the type is chosen once and the R loads issue together.
+// The order-preserving bits of key k (its column's type) for R rows at once: the type is chosen
+// once and the R loads issue together. INT / LONG: sign bit flipped; FLOAT / DOUBLE: sign-magnitude
+// made monotone, -0 as +0, every NaN canonical (above +inf).
 template <int R>
 __device__ __forceinline__ void field_bits_rows(const SortKeySpec &S, int k, const uint64_t (&r)[R],
                                                 uint64_t (&b)[R]) {
@@ -127,64 +114,6 @@ __device__ __forceinline__ void sort_words_rows(const SortKeySpec &S, const uint
   if (embed_id)
 #pragma unroll
     for (int j = 0; j < R; ++j) w[j][0] |= (uint32_t)r[j];
-}
-
-// The composite key of row r as KW words (w[0] least significant); embed_id: the row id fills the
-// low 32 bits of word 0 (the fields start at bit 32).
-template <int KW>
-__device__ __forceinline__ void sort_words(const SortKeySpec &S, uint64_t r, int embed_id, uint64_t (&w)[KW]) {
-#pragma unroll
-  for (int q = 0; q < KW; ++q) w[q] = 0;
-  for (int k = 0; k < S.nkeys; ++k) {
-    uint64_t b = field_bits(S.type[k], S.data[k], r);
-    if (S.desc[k]) b = ~b;
-    if (S.width[k] < 64) b &= (1ull << S.width[k]) - 1;
-    const int sh = S.shift[k], word = sh / 64, off = sh % 64;
-#pragma unroll
-    for (int q = 0; q < KW; ++q) {
-      if (q == word) w[q] |= b << off;
-      if (q == word + 1 && off + S.width[k] > 64) w[q] |= b >> (64 - off);
-    }
-  }
-  if (embed_id) w[0] |= (uint32_t)r;
-}
-
-// embed_id: the row id fills the low 32 bits of word 0 (the fields start at bit 32), so the passes
-// carry no separate id array; otherwise idx[i] = i.
-template <int KW>
-__global__ void k_sort_keys(SortKeySpec S, uint64_t n, uint64_t *keys, uint32_t *idx, unsigned long long *orand,
-                            int embed_id) {
-  uint64_t vor[KW], vand[KW];
-#pragma unroll
-  for (int q = 0; q < KW; ++q) {
-    vor[q] = 0;
-    vand[q] = ~0ull;
-  }
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) {
-    uint64_t w[KW];  // w[0] = least significant 64 bits
-    sort_words<KW>(S, i, embed_id, w);
-#pragma unroll
-    for (int q = 0; q < KW; ++q) {
-      keys[(uint64_t)q * n + i] = w[q];
-      vor[q] |= w[q];
-      vand[q] &= w[q];
-    }
-    if (!embed_id) idx[i] = (uint32_t)i;
-  }
-#pragma unroll
-  for (int q = 0; q < KW; ++q) {
-#pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1) {
-      vor[q] |= __shfl_xor((unsigned long long)vor[q], d, kWave);
-      vand[q] &= __shfl_xor((unsigned long long)vand[q], d, kWave);
-    }
-    if (lane_id() == 0) {
-      atomicOr(&orand[2 * q], (unsigned long long)vor[q]);
-      atomicAnd(&orand[2 * q + 1], (unsigned long long)vand[q]);
-    }
-  }
 }
 
 // hist[digit * nblocks + block] (scanned afterwards into global offsets) and the same counts
