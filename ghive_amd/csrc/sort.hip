@@ -235,7 +235,10 @@ __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, c
                                                       SortKeySpec S = SortKeySpec{}, int embed = 0,
                                                       unsigned long long *orand_part = nullptr) {
   constexpr uint32_t kTile = (uint32_t)TH * PER, kWaves = TH / kWave;
-  __shared__ uint64_t stage[kTile];
+  // one key word and its row id leave as one 16-byte LDS record (one staging round, not two) when
+  // the tile's records fit LDS that way
+  constexpr bool kPacked = KW == 1 && IDS && !DROP0 && kTile <= 8192;
+  __shared__ __attribute__((aligned(16))) uint64_t stage[kPacked ? 2 * kTile : kTile];
   __shared__ uint8_t sdigit[kTile];
   __shared__ uint32_t lbase[256], goff[256];
   __shared__ uint32_t wcnt[kWaves][256];
@@ -357,6 +360,24 @@ __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, c
       sdigit[pos] = (uint8_t)dg[j];
     }
   }
+  if constexpr (kPacked) {
+    typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+      if (wave * kSeg + (uint32_t)j * kWave + lane < tn)
+        *reinterpret_cast<u64x2 *>(stage + 2 * lpos[j]) = u64x2{kv[j][0], (unsigned long long)id[j]};
+    __syncthreads();
+    for (uint32_t p = tid; p < tn; p += TH) {
+      const uint32_t d = sdigit[p];
+      const uint64_t g = (uint64_t)goff[d] + (p - lbase[d]);
+      const u64x2 v = *reinterpret_cast<const u64x2 *>(stage + 2 * p);
+      keys_out[g] = v.x;
+      idx_out[g] = (uint32_t)v.y;
+      if (plane_out && next_word == 0) plane_out[g] = (uint8_t)(v.x >> next_shift);
+    }
+    return;
+  }
   // write-out, one word at a time through the stage buffer (then the ids, when carried)
 #pragma unroll
   for (int q = 0; q < (IDS ? KW + 1 : KW); ++q) {
@@ -380,40 +401,34 @@ __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, c
 }
 
 // out[2q] = OR, out[2q + 1] = AND of key word q over the first pass's per-tile partials (one
-// workgroup of kOrandThreads; eight loads in flight per lane).
+// workgroup; the loading lanes are a multiple of 2 KW, so each always meets the same word of the
+// contiguous partials array; eight loads in flight).
 constexpr int kOrandThreads = 1024;
 __global__ __launch_bounds__(kOrandThreads) void k_orand_reduce(const unsigned long long *part, uint32_t nblocks,
                                                                 int words2, unsigned long long *out) {
-  __shared__ unsigned long long red[kOrandThreads / kWave][8];
-  unsigned long long acc[8];
-  for (int q = 0; q < 8; ++q) acc[q] = (q & 1) ? ~0ull : 0ull;
+  __shared__ unsigned long long red[kOrandThreads];
+  const uint32_t t = threadIdx.x;
+  const uint32_t nt = kOrandThreads - kOrandThreads % (uint32_t)words2;  // lanes that load: a multiple of 2 KW
+  const bool is_and = (t % (uint32_t)words2) & 1u;
+  unsigned long long acc = is_and ? ~0ull : 0ull;
+  const uint64_t total = (uint64_t)nblocks * (uint32_t)words2;
   constexpr int U = 8;
-  for (uint32_t b0 = 0; b0 < nblocks; b0 += kOrandThreads * U) {
-    for (int q = 0; q < words2; ++q) {
-      unsigned long long x[U];
+  for (uint64_t i0 = 0; i0 < total; i0 += (uint64_t)nt * U) {
+    unsigned long long x[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const uint32_t b = b0 + (uint32_t)u * kOrandThreads + threadIdx.x;
-        x[u] = b < nblocks ? part[(uint64_t)b * words2 + q] : ((q & 1) ? ~0ull : 0ull);
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) acc[q] = (q & 1) ? (acc[q] & x[u]) : (acc[q] | x[u]);
+    for (int u = 0; u < U; ++u) {
+      const uint64_t i = i0 + (uint64_t)u * nt + t;
+      x[u] = t < nt && i < total ? part[i] : (is_and ? ~0ull : 0ull);
     }
-  }
-  for (int q = 0; q < words2; ++q) {
 #pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1) {
-      const unsigned long long o = __shfl_xor(acc[q], d, kWave);
-      acc[q] = (q & 1) ? (acc[q] & o) : (acc[q] | o);
-    }
-    if (lane_id() == 0) red[threadIdx.x / kWave][q] = acc[q];
+    for (int u = 0; u < U; ++u) acc = is_and ? (acc & x[u]) : (acc | x[u]);
   }
+  red[t] = acc;
   __syncthreads();
-  if ((int)threadIdx.x < words2) {
-    const int q = (int)threadIdx.x;
-    unsigned long long r = red[0][q];
-    for (int w = 1; w < kOrandThreads / kWave; ++w) r = (q & 1) ? (r & red[w][q]) : (r | red[w][q]);
-    out[q] = r;
+  if ((int)t < words2) {
+    unsigned long long r = red[t];
+    for (uint32_t j = t + (uint32_t)words2; j < kOrandThreads; j += (uint32_t)words2) r = is_and ? (r & red[j]) : (r | red[j]);
+    out[t] = r;
   }
 }
 

@@ -57,6 +57,12 @@ def test_sort_composite_mixed_orders(ctx, orc):
     c = (rng.integers(0, 4, n) / 2).astype(np.float64)
     _check(ctx, orc, [a, b, c], [0, 1, 2], ["-", "+", "-"])
     _check(ctx, orc, [a, b, c], [2, 0], ["+", "+"])
+    # three and four LONG keys: 192 and 256 bits, three and four key words with the row ids beside
+    # them (no spare bits), the first pass building every word
+    d = rng.integers(-2**62, 2**62, n).astype(np.int64)
+    e = rng.integers(0, 3, n).astype(np.int64)
+    _check(ctx, orc, [b, e, d], [0, 1, 2], ["+", "-", "+"])
+    _check(ctx, orc, [e, b, d, e], [0, 1, 2, 3], ["-", "+", "-", "+"])
 
 
 @pytest.mark.parametrize("n", [1, 4097, 200_003])
