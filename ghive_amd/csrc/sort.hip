@@ -226,7 +226,8 @@ __global__ __launch_bounds__(TH) void k_radix_hist_plane(const uint8_t *plane, u
 // COLS: the sort's first pass. It builds each row's composite key from the key columns (S, embed)
 // instead of reading key words, its row ids are the row numbers, and every workgroup leaves the OR
 // and AND of each key word over its tile in orand_part[block * 2 KW ..] (which later digits vary).
-template <int KW, bool IDS, bool DROP0, int TH, int PER, bool COLS = false>
+// PACK: stage a key word and its row id as one 16-byte LDS record where that fits (option sort.pack).
+template <int KW, bool IDS, bool DROP0, int TH, int PER, bool COLS = false, bool PACK = true>
 __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, const uint32_t *idx_in,
                                                       uint64_t *keys_out, uint32_t *idx_out, uint64_t n,
                                                       int word, int shift, const uint32_t *offs,
@@ -236,8 +237,9 @@ __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, c
                                                       unsigned long long *orand_part = nullptr) {
   constexpr uint32_t kTile = (uint32_t)TH * PER, kWaves = TH / kWave;
   // one key word and its row id leave as one 16-byte LDS record (one staging round, not two) when
-  // the tile's records fit LDS that way
-  constexpr bool kPacked = KW == 1 && IDS && !DROP0 && kTile <= 8192;
+  // the tile's records fit LDS that way and the workgroup holds a CU's LDS anyway (512 threads or
+  // more: one workgroup per CU either way; at 256 threads the 8-byte staging keeps two)
+  constexpr bool kPacked = PACK && KW == 1 && IDS && !DROP0 && kTile <= 8192 && TH >= 512;
   __shared__ __attribute__((aligned(16))) uint64_t stage[kPacked ? 2 * kTile : kTile];
   __shared__ uint8_t sdigit[kTile];
   __shared__ uint32_t lbase[256], goff[256];
@@ -515,8 +517,11 @@ static int radix_passes_shape(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t
     if (drop)
       k_radix_scatter<KW, false, true, TH, PER><<<nblocks, TH, 0, ctx->stream>>>(
           kin, iin, kout, iout, n, word, shift, hist.as<uint32_t>(), counts.as<uint32_t>(), nblocks, pout, nw, ns);
-    else if (ids)
+    else if (ids && ctx->tune.sort_pack)
       k_radix_scatter<KW, true, false, TH, PER><<<nblocks, TH, 0, ctx->stream>>>(
+          kin, iin, kout, iout, n, word, shift, hist.as<uint32_t>(), counts.as<uint32_t>(), nblocks, pout, nw, ns);
+    else if (ids)
+      k_radix_scatter<KW, true, false, TH, PER, false, false><<<nblocks, TH, 0, ctx->stream>>>(
           kin, iin, kout, iout, n, word, shift, hist.as<uint32_t>(), counts.as<uint32_t>(), nblocks, pout, nw, ns);
     else
       k_radix_scatter<KW, false, false, TH, PER><<<nblocks, TH, 0, ctx->stream>>>(
