@@ -537,13 +537,16 @@ __global__ void k_groupby_finalize(KeySpec K, GroupTableDev T, OutSpec O, unsign
 // caller uses the direct form. Then the workgroup reserves its range of output rows with one
 // atomic and writes its groups there: no global table is built. The one key equal to the EMPTY
 // pattern is aggregated in the small global table T instead, emitted afterwards.
+// sum_nocount: the carried value column holds no NULL and no exact double SUM is an AVG: every
+// group then has a non-NULL value, and a SUM's non-NULL count word is neither updated nor read
+// (one LDS atomic a row less: the LDS atomics bound this kernel).
 template <int VAL>
 __global__ __launch_bounds__(kGbPartThreads) void k_gb_part_emit(KeySpec K, GroupTableDev T, const uint64_t *starts,
                                                                  const uint64_t *hs, const uint64_t *vs,
                                                                  const uint32_t *ids, uint32_t lcap, uint32_t nparts,
                                                                  unsigned long long *fail,
                                                                  unsigned long long *out_count, OutSpec O,
-                                                                 uint64_t cap) {
+                                                                 uint64_t cap, int sum_nocount) {
   // One resident workgroup per CU takes partitions blockIdx.x, blockIdx.x + gridDim.x, ... in turn
   // (its table fills most of the CU's LDS; a new workgroup per partition would leave the CU idle
   // between them). The partition's table: keys[lcap] (EMPTY = all ones), then the accumulator
@@ -631,7 +634,18 @@ __global__ __launch_bounds__(kGbPartThreads) void k_gb_part_emit(KeySpec K, Grou
         for (int a = 0; a < T.naggs; ++a) {
           const AggSpec &A = T.aggs[a];
           const AggVal v = VAL == 2 ? agg_value(A, r) : agg_value_bits(A, vbits);
-          acc_apply(A, [&](int j) { return lds + (size_t)(1 + A.word0 + j) * lstride + slot; }, v);
+          auto word = [&](int j) { return lds + (size_t)(1 + A.word0 + j) * lstride + slot; };
+          if (sum_nocount && A.kind == AK_SUMD) {
+            if (!v.valid) continue;
+            uint64_t lo = v.a, hi = v.b;
+            if (v.neg) {
+              lo = ~lo + 1ull;
+              hi = ~hi + (lo == 0 ? 1ull : 0ull);
+            }
+            add128(word(0), word(1), lo, hi);
+            continue;
+          }
+          acc_apply(A, word, v);
         }
       }
     }
@@ -656,7 +670,9 @@ __global__ __launch_bounds__(kGbPartThreads) void k_gb_part_emit(KeySpec K, Grou
       emit_key(K, O, unmix64(h), idx);
       for (int a = 0; a < T.naggs; ++a) {
         const AggSpec &A = T.aggs[a];
-        finalize_words(A, [&](int j) { return lds[(size_t)(1 + A.word0 + j) * lstride + s]; }, O.col[K.nkeys + a], idx);
+        const bool one = sum_nocount && A.kind == AK_SUMD;  // the count word reads as non-zero
+        finalize_words(A, [&](int j) { return one && j == 2 ? 1ull : lds[(size_t)(1 + A.word0 + j) * lstride + s]; },
+                       O.col[K.nkeys + a], idx);
       }
     }
     __syncthreads();  // the table is reset for the next partition only after every slot was read
@@ -815,8 +831,10 @@ __global__ __launch_bounds__(TH) void k_gbp_hist_plane(const uint8_t *plane, uin
 
 // plane_out (optional): the next pass's 8-bit digit, (h >> next_shift) & 255, as a byte per row.
 // part_max (optional, a first pass that reads the value column): the workgroup's max |x| over its
-// rows' non-NULL values, as the bits of a non-negative double, at part_max[blockIdx.x] -- the
-// exact SUM's scale comes from these without a pass of its own over the column.
+// rows' non-NULL values, as the bits of a non-negative double, at part_max[blockIdx.x], with bit 63
+// (kNullSeen, never set in such bits) set when one of its rows is NULL -- the exact SUM's scale,
+// and whether a group can lack a non-NULL value, come from these without a pass over the column.
+constexpr unsigned long long kNullSeen = 1ull << 63;
 template <int DB, int SRC, bool VAL, int TH>
 __global__ __launch_bounds__(TH) void k_gbp_scatter(GbpSrc S, uint64_t n, int shift, const uint32_t *offs,
                                                     const uint32_t *counts, uint32_t nblocks, uint64_t *h_out,
@@ -880,12 +898,17 @@ __global__ __launch_bounds__(TH) void k_gbp_scatter(GbpSrc S, uint64_t n, int sh
   if (VAL && SRC != kSrcPass && part_max) {  // uniform: a kernel argument
     __shared__ unsigned long long wmax[kWaves];
     unsigned long long m = 0;
+    bool null_row = false;
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
       double d;
-      if (wave * kSeg + (uint32_t)j * kWave + lane < tn && bits_f64(vv[j], S.vtype, d)) {
-        const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(d));
-        m = b > m ? b : m;
+      if (wave * kSeg + (uint32_t)j * kWave + lane < tn) {
+        if (bits_f64(vv[j], S.vtype, d)) {
+          const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(d));
+          m = b > m ? b : m;
+        } else {
+          null_row = true;
+        }
       }
     }
 #pragma unroll
@@ -893,11 +916,17 @@ __global__ __launch_bounds__(TH) void k_gbp_scatter(GbpSrc S, uint64_t n, int sh
       const unsigned long long o = __shfl_xor(m, d, kWave);
       m = o > m ? o : m;
     }
-    if (lane == 0) wmax[wave] = m;
+    if (lane == 0) wmax[wave] = m | (ballot(null_row) ? kNullSeen : 0ull);
     __syncthreads();
     if (tid == 0) {
-      for (int w = 1; w < kWaves; ++w) m = wmax[w] > m ? wmax[w] : m;
-      part_max[blockIdx.x] = m;
+      unsigned long long nf = 0;
+      m = 0;
+      for (int w = 0; w < kWaves; ++w) {
+        const unsigned long long x = wmax[w] & ~kNullSeen;
+        m = x > m ? x : m;
+        nf |= wmax[w] & kNullSeen;
+      }
+      part_max[blockIdx.x] = m | nf;
     }
   }
   __syncthreads();
@@ -1069,9 +1098,10 @@ static int gbp_partition(ghive_ctx *ctx, const GbpSrc &src, uint64_t n, int pbit
 // aggregate specs. *failed: a partition held more groups than its LDS table (nothing usable was
 // written; use the direct form). *groups = groups found (CAPACITY when above capacity).
 // The largest of n partial maxima (bits of non-negative doubles) into *out (one workgroup).
+// *out = the max of the partials' low 63 bits, with kNullSeen set when any partial has it.
 __global__ __launch_bounds__(256) void k_max_u64(const unsigned long long *part, uint64_t n, unsigned long long *out) {
   __shared__ unsigned long long wm[256 / kWave];
-  unsigned long long m = 0;
+  unsigned long long m = 0, nf = 0;
   // one workgroup: 16 independent loads in flight per lane, not one (the reduction is latency-bound)
   constexpr int U = 16;
   for (uint64_t i0 = 0; i0 < n; i0 += 256 * U) {
@@ -1082,18 +1112,28 @@ __global__ __launch_bounds__(256) void k_max_u64(const unsigned long long *part,
       x[u] = i < n ? part[i] : 0ull;
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) m = x[u] > m ? x[u] : m;
+    for (int u = 0; u < U; ++u) {
+      nf |= x[u] & kNullSeen;
+      const unsigned long long v = x[u] & ~kNullSeen;
+      m = v > m ? v : m;
+    }
   }
 #pragma unroll
   for (int d = kWave / 2; d > 0; d >>= 1) {
     const unsigned long long o = __shfl_xor(m, d, kWave);
     m = o > m ? o : m;
+    nf |= __shfl_xor(nf, d, kWave);
   }
-  if (lane_id() == 0) wm[threadIdx.x / kWave] = m;
+  if (lane_id() == 0) wm[threadIdx.x / kWave] = m | nf;
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int w = 1; w < 256 / kWave; ++w) m = wm[w] > m ? wm[w] : m;
-    *out = m;
+    m = 0;
+    for (int w = 0; w < 256 / kWave; ++w) {
+      const unsigned long long v = wm[w] & ~kNullSeen;
+      m = v > m ? v : m;
+      nf |= wm[w] & kNullSeen;
+    }
+    *out = m | nf;
   }
 }
 
@@ -1137,6 +1177,7 @@ int groupby_partitioned(ghive_ctx *ctx, const KeySpec &K, GroupTableDev &Tspec, 
     GH_TRY(sum_scale(ctx, mx, n, &A.scale_e));
   }
   const int KW = val == 1 ? 2 : 1;
+  bool null_seen = true;  // until the first pass shows that the carried column has no NULL
   DevBuf k0, k1, i0, i1, starts, sk, sa, sc, pmax;
   const uint64_t npmax = (n + 4095) / 4096;  // the first pass's workgroups, at most
   if (fused_absmax) {
@@ -1168,6 +1209,8 @@ int groupby_partitioned(ghive_ctx *ctx, const KeySpec &K, GroupTableDev &Tspec, 
       GH_LAUNCH(ctx);
       uint64_t mb = 0;
       GH_TRY(d2h(ctx, &mb, ctx->d_counters, sizeof mb));
+      null_seen = (mb & kNullSeen) != 0;
+      mb &= ~kNullSeen;
       double mx = 0;
       memcpy(&mx, &mb, sizeof mx);
       int32_t e = 0;
@@ -1200,25 +1243,28 @@ int groupby_partitioned(ghive_ctx *ctx, const KeySpec &K, GroupTableDev &Tspec, 
   GH_TRY(fill_u64(ctx, ctx->d_counters, 0, 2));
   const uint32_t P = (uint32_t)(1ull << pbits);
   const uint32_t egrid = std::min<uint32_t>(P, (uint32_t)ctx->cus);  // one resident workgroup per CU
+  int sum_nocount = !null_seen;
+  for (int a = 0; a < T.naggs; ++a)
+    if (T.aggs[a].kind == AK_SUMD && T.aggs[a].is_avg) sum_nocount = 0;
   const uint64_t *vs = vs_part;
   switch (val) {
     case 0:
       (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_gb_part_emit<0>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
       k_gb_part_emit<0><<<egrid, kGbPartThreads, lds_bytes, ctx->stream>>>(K, T, starts.as<uint64_t>(), ks, vs, ids, lcap, P,
-                                                                           fail, count, O, capacity);
+                                                                           fail, count, O, capacity, sum_nocount);
       break;
     case 1:
       (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_gb_part_emit<1>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
       k_gb_part_emit<1><<<egrid, kGbPartThreads, lds_bytes, ctx->stream>>>(K, T, starts.as<uint64_t>(), ks, vs, ids, lcap, P,
-                                                                           fail, count, O, capacity);
+                                                                           fail, count, O, capacity, sum_nocount);
       break;
     default:
       (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_gb_part_emit<2>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
       k_gb_part_emit<2><<<egrid, kGbPartThreads, lds_bytes, ctx->stream>>>(K, T, starts.as<uint64_t>(), ks, vs, ids, lcap, P,
-                                                                           fail, count, O, capacity);
+                                                                           fail, count, O, capacity, sum_nocount);
       break;
   }
   GH_LAUNCH(ctx);

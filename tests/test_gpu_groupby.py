@@ -292,6 +292,35 @@ def test_groupby_partitioned_value_forms(ctx, orc, aggset, threads):
 
 
 @pytest.mark.parametrize("threads", [256, 512])
+def test_groupby_partitioned_double_sum_without_nulls(ctx, orc, threads):
+    """A carried DOUBLE column with no NULL: the first partition pass records that no row is NULL,
+    and the emit keeps no non-NULL count for SUM (every group has a value). Against the oracle:
+    SUM, COUNT(*), MIN and COUNT(x) of values over seven decades, keys with -1 (the EMPTY pattern's
+    bits, aggregated in the small global table) and NULL; then the same keys with one NULL value
+    added (the count-keeping form) on a selection."""
+    rng = np.random.default_rng(threads + 7)
+    n = 400_000
+    k = rng.integers(-1000, 50_000, n).astype(np.int64)
+    k[::97] = -1
+    k[::89] = LONG_NULL
+    x = rng.standard_normal(n) * 10.0 ** rng.integers(-3, 4, n)
+    aggs = [(abi.AGG_SUM, 1), (abi.AGG_CNT, abi.COUNT_STAR), (abi.AGG_MIN, 1), (abi.AGG_CNT, 1)]
+    ctx.set_option("groupby.partition", 2)
+    ctx.set_option("groupby.part_threads", threads)
+    try:
+        cols = [k, x]
+        _compare(_as_dict(_run(ctx, cols, [0], aggs), 1), _oracle_dict(orc, cols, [0], aggs))
+        x2 = x.copy()
+        x2[12345] = np.nan
+        cols = [k, x2]
+        sel = np.nonzero(rng.random(n) < 0.7)[0].astype(np.uint32)
+        _compare(_as_dict(_run(ctx, cols, [0], aggs, sel=sel), 1), _oracle_dict(orc, cols, [0], aggs, sel=sel))
+    finally:
+        ctx.set_option("groupby.partition", 1)
+        ctx.set_option("groupby.part_threads", 512)
+
+
+@pytest.mark.parametrize("threads", [256, 512])
 def test_groupby_two_partition_passes(ctx, threads):
     """A group bound of 3M puts 2^11 partitions behind two 8-bit partition passes, the second of
     which counts its digits from the byte plane the first one wrote. 5M rows over 1.5M keys:
