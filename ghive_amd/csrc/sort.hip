@@ -216,9 +216,8 @@ __global__ __launch_bounds__(TH) void k_radix_hist_plane(const uint8_t *plane, u
 
 // One stable pass over a tile of TH x PER elements. (1) Every element's position within the tile
 // in digit order: wave64 ballots give the rank among equal digits of a wave, LDS counters carry it
-// across waves and iterations. (2) The tile is rewritten through a 32 KiB LDS buffer, one key
-// word (then the row ids) at a time, in digit order, so each digit's run leaves as consecutive
-// addresses (coalesced stores) at its global offset.
+// across waves and iterations. (2) The tile is rewritten through an LDS buffer in digit order, so
+// each digit's run leaves as consecutive addresses (coalesced stores) at its global offset.
 // DROP0: the last pass over word 0, whose low 32 bits hold the row id: the later passes need only
 // the id, so word 0 leaves as a uint32 id array (idx_out) and words 1.. as a (KW - 1)-word array.
 // plane_out (optional): the next pass's digit, byte (next_shift / 8) of input word next_word, is
@@ -226,7 +225,11 @@ __global__ __launch_bounds__(TH) void k_radix_hist_plane(const uint8_t *plane, u
 // COLS: the sort's first pass. It builds each row's composite key from the key columns (S, embed)
 // instead of reading key words, its row ids are the row numbers, and every workgroup leaves the OR
 // and AND of each key word over its tile in orand_part[block * 2 KW ..] (which later digits vary).
-// PACK: stage a key word and its row id as one 16-byte LDS record where that fits (option sort.pack).
+// PACK: a row's two words are staged as one 16-byte LDS record (one staging round, not one per word)
+// where that fits and was measured faster (option sort.pack): one key word and its id, and the pass
+// that drops word 0 (two key words, the id inside word 0). Two key words that both leave as key
+// words (10.3 ms against 9.8 ms per pass over 0.9e9 rows) and the other shapes stage one 8-byte
+// word at a time.
 template <int KW, bool IDS, bool DROP0, int TH, int PER, bool COLS = false, bool PACK = true>
 __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, const uint32_t *idx_in,
                                                       uint64_t *keys_out, uint32_t *idx_out, uint64_t n,
@@ -236,10 +239,11 @@ __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, c
                                                       SortKeySpec S = SortKeySpec{}, int embed = 0,
                                                       unsigned long long *orand_part = nullptr) {
   constexpr uint32_t kTile = (uint32_t)TH * PER, kWaves = TH / kWave;
-  // one key word and its row id leave as one 16-byte LDS record (one staging round, not two) when
-  // the tile's records fit LDS that way and the workgroup holds a CU's LDS anyway (512 threads or
-  // more: one workgroup per CU either way; at 256 threads the 8-byte staging keeps two)
-  constexpr bool kPacked = PACK && KW == 1 && IDS && !DROP0 && kTile <= 8192 && TH >= 512;
+  // 16-byte records when the tile's records fit LDS that way and the workgroup holds a CU's LDS
+  // anyway (512 threads or more: one workgroup per CU either way; at 256 threads the 8-byte staging
+  // keeps two)
+  constexpr bool kPacked = PACK && ((KW == 1 && IDS) || (KW == 2 && DROP0)) && kTile <= 8192 && TH >= 512;
+  typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
   __shared__ __attribute__((aligned(16))) uint64_t stage[kPacked ? 2 * kTile : kTile];
   __shared__ uint8_t sdigit[kTile];
   __shared__ uint32_t lbase[256], goff[256];
@@ -247,22 +251,22 @@ __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, c
   __shared__ uint32_t scan_tmp[kWaves];
   constexpr uint32_t kSeg = PER * kWave;  // each wave owns one contiguous segment of the tile
   const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
-  const uint64_t t0 = (uint64_t)blockIdx.x * kTile;
+  uint64_t kv[PER][KW];
+  uint32_t id[PER], lpos[PER], dg[PER];
+  const uint32_t tile = blockIdx.x;
+  const uint64_t t0 = (uint64_t)tile * kTile;
   const uint32_t tn = (uint32_t)(n - t0 < kTile ? n - t0 : kTile);
   uint32_t total;
   // threads past the 256 digits add zeros at the end of the scan
-  const uint32_t lb = block_exclusive_scan_256(tid < 256 ? counts[(uint64_t)blockIdx.x * 256 + tid] : 0u, scan_tmp,
-                                               total);
+  const uint32_t lb = block_exclusive_scan_256(tid < 256 ? counts[(uint64_t)tile * 256 + tid] : 0u, scan_tmp, total);
   if (tid < 256) {
     lbase[tid] = lb;
-    goff[tid] = offs[(uint64_t)tid * nblocks + blockIdx.x];
+    goff[tid] = offs[(uint64_t)tid * nblocks + tile];
   }
 #pragma unroll
   for (int q = 0; q < 256 / kWave; ++q) wcnt[wave][lane + q * kWave] = 0;
   // all loads of the tile up front, element wave * kSeg + j * 64 + lane (coalesced per j); past the
   // end a lane re-reads the last element (ignored)
-  uint64_t kv[PER][KW];
-  uint32_t id[PER], lpos[PER], dg[PER];
   if constexpr (COLS) {
     uint64_t rows[PER];
 #pragma unroll
@@ -314,7 +318,7 @@ __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, c
     if (tid < 2 * KW) {
       unsigned long long r = ored[0][tid];
       for (uint32_t w = 1; w < kWaves; ++w) r = (tid & 1) ? (r & ored[w][tid]) : (r | ored[w][tid]);
-      orand_part[(uint64_t)blockIdx.x * 2 * KW + tid] = r;
+      orand_part[(uint64_t)tile * 2 * KW + tid] = r;
     }
   }
   __syncthreads();
@@ -363,20 +367,32 @@ __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, c
     }
   }
   if constexpr (kPacked) {
-    typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < PER; ++j)
-      if (wave * kSeg + (uint32_t)j * kWave + lane < tn)
-        *reinterpret_cast<u64x2 *>(stage + 2 * lpos[j]) = u64x2{kv[j][0], (unsigned long long)id[j]};
+      if (wave * kSeg + (uint32_t)j * kWave + lane < tn) {
+        u64x2 rec;
+        if constexpr (IDS) rec = u64x2{kv[j][0], (unsigned long long)id[j]};
+        else rec = u64x2{kv[j][0], kv[j][KW - 1]};
+        *reinterpret_cast<u64x2 *>(stage + 2 * lpos[j]) = rec;
+      }
     __syncthreads();
+    const bool plane = plane_out && next_word >= 0 && next_word < KW;
     for (uint32_t p = tid; p < tn; p += TH) {
       const uint32_t d = sdigit[p];
       const uint64_t g = (uint64_t)goff[d] + (p - lbase[d]);
       const u64x2 v = *reinterpret_cast<const u64x2 *>(stage + 2 * p);
-      keys_out[g] = v.x;
-      idx_out[g] = (uint32_t)v.y;
-      if (plane_out && next_word == 0) plane_out[g] = (uint8_t)(v.x >> next_shift);
+      if constexpr (DROP0) {  // word 0 (the id in its low half) -> ids, word 1 -> the one-word keys
+        idx_out[g] = (uint32_t)v.x;
+        keys_out[g] = v.y;
+      } else if constexpr (IDS) {
+        keys_out[g] = v.x;
+        idx_out[g] = (uint32_t)v.y;
+      } else {
+        keys_out[g] = v.x;
+        keys_out[n + g] = v.y;
+      }
+      if (plane) plane_out[g] = (uint8_t)((next_word == 0 ? v.x : v.y) >> next_shift);
     }
     return;
   }
@@ -400,6 +416,7 @@ __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, c
       if (plane_out && q == next_word) plane_out[g] = (uint8_t)(stage[p] >> next_shift);
     }
   }
+  return;
 }
 
 // out[2q] = OR, out[2q + 1] = AND of key word q over the first pass's per-tile partials (one
@@ -434,6 +451,27 @@ __global__ __launch_bounds__(kOrandThreads) void k_orand_reduce(const unsigned l
   }
 }
 
+// One scatter pass over key words (not the first pass): the shapes that stage 16-byte LDS records
+// take them when sort.pack is on.
+template <int KW, bool IDS, bool DROP0, int TH, int PER>
+static int scatter_pass(ghive_ctx *ctx, const uint64_t *kin, const uint32_t *iin, uint64_t *kout, uint32_t *iout,
+                        uint64_t n, int word, int shift, const uint32_t *offs, const uint32_t *counts,
+                        uint32_t nblocks, uint8_t *pout, int nw, int ns) {
+  constexpr bool kPackable = ((KW == 1 && IDS) || (KW == 2 && DROP0)) && TH >= 512 && TH * PER <= 8192;
+  if constexpr (kPackable) {
+    if (ctx->tune.sort_pack) {
+      k_radix_scatter<KW, IDS, DROP0, TH, PER, false, true><<<nblocks, TH, 0, ctx->stream>>>(
+          kin, iin, kout, iout, n, word, shift, offs, counts, nblocks, pout, nw, ns);
+      GH_LAUNCH(ctx);
+      return GHIVE_OK;
+    }
+  }
+  k_radix_scatter<KW, IDS, DROP0, TH, PER, false, false><<<nblocks, TH, 0, ctx->stream>>>(
+      kin, iin, kout, iout, n, word, shift, offs, counts, nblocks, pout, nw, ns);
+  GH_LAUNCH(ctx);
+  return GHIVE_OK;
+}
+
 // The sort's first pass: a stable pass on `digit` (the lowest byte of the last key field), reading
 // the key columns themselves (no key-building pass before it): keys_out receives the KW-word
 // composite keys in that digit's order, ids_out the row ids when they are not embedded, plane_out
@@ -453,11 +491,11 @@ static int sort_first_pass(ghive_ctx *ctx, const SortKeySpec &S, bool embed, uin
   const int word = digit / 8, shift = 8 * (digit % 8);
   const int nw = next >= 0 ? next / 8 : -1, ns = next >= 0 ? 8 * (next % 8) : 0;
   if (embed)
-    k_radix_scatter<KW, false, false, TH, PER, true><<<nblocks, TH, 0, ctx->stream>>>(
+    k_radix_scatter<KW, false, false, TH, PER, true, false><<<nblocks, TH, 0, ctx->stream>>>(
         nullptr, nullptr, keys_out, nullptr, n, word, shift, hist.as<uint32_t>(), counts.as<uint32_t>(), nblocks,
         plane_out, nw, ns, S, 1, part.as<unsigned long long>());
   else
-    k_radix_scatter<KW, true, false, TH, PER, true><<<nblocks, TH, 0, ctx->stream>>>(
+    k_radix_scatter<KW, true, false, TH, PER, true, false><<<nblocks, TH, 0, ctx->stream>>>(
         nullptr, nullptr, keys_out, ids_out, n, word, shift, hist.as<uint32_t>(), counts.as<uint32_t>(), nblocks,
         plane_out, nw, ns, S, 0, part.as<unsigned long long>());
   GH_LAUNCH(ctx);
@@ -514,19 +552,16 @@ static int radix_passes_shape(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t
     const int next = di + 1 < digits.size() ? digits[di + 1] : next_after;
     uint8_t *pout = planes && next >= 0 ? planes[cur == 0 ? 1 : 0] : nullptr;
     const int nw = next >= 0 ? next / 8 : -1, ns = next >= 0 ? 8 * (next % 8) : 0;
+    const uint32_t *ho = hist.as<uint32_t>(), *co = counts.as<uint32_t>();
     if (drop)
-      k_radix_scatter<KW, false, true, TH, PER><<<nblocks, TH, 0, ctx->stream>>>(
-          kin, iin, kout, iout, n, word, shift, hist.as<uint32_t>(), counts.as<uint32_t>(), nblocks, pout, nw, ns);
-    else if (ids && ctx->tune.sort_pack)
-      k_radix_scatter<KW, true, false, TH, PER><<<nblocks, TH, 0, ctx->stream>>>(
-          kin, iin, kout, iout, n, word, shift, hist.as<uint32_t>(), counts.as<uint32_t>(), nblocks, pout, nw, ns);
+      GH_TRY((scatter_pass<KW, false, true, TH, PER>(ctx, kin, iin, kout, iout, n, word, shift, ho, co, nblocks, pout,
+                                                      nw, ns)));
     else if (ids)
-      k_radix_scatter<KW, true, false, TH, PER, false, false><<<nblocks, TH, 0, ctx->stream>>>(
-          kin, iin, kout, iout, n, word, shift, hist.as<uint32_t>(), counts.as<uint32_t>(), nblocks, pout, nw, ns);
+      GH_TRY((scatter_pass<KW, true, false, TH, PER>(ctx, kin, iin, kout, iout, n, word, shift, ho, co, nblocks, pout,
+                                                      nw, ns)));
     else
-      k_radix_scatter<KW, false, false, TH, PER><<<nblocks, TH, 0, ctx->stream>>>(
-          kin, iin, kout, iout, n, word, shift, hist.as<uint32_t>(), counts.as<uint32_t>(), nblocks, pout, nw, ns);
-    GH_LAUNCH(ctx);
+      GH_TRY((scatter_pass<KW, false, false, TH, PER>(ctx, kin, iin, kout, iout, n, word, shift, ho, co, nblocks,
+                                                       pout, nw, ns)));
     cur = pout ? (cur == 0 ? 1 : 0) : -1;
     const uint64_t in_b = 8ull * KW + (ids ? 4 : 0), out_b = (drop ? 8ull * (KW - 1) + 4 : in_b) + (pout ? 1 : 0);
     ctx->sort_info.passes += 1;

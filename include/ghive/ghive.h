@@ -504,7 +504,8 @@ int ghive_fragment_execute_late(ghive_ctx *ctx, const ghive_fragment *frag, ghiv
  * "groupby.part_threads" (256, 512 or 1024: workgroup size of the partition passes; default 512),
  * "plan.fuse" (0/1: run map-join vertices of a plan as one fused fragment), "sort.shape" (LSD tile
  * of ghive_sort: 0 = 256 threads x 32 rows, 1 = 512 x 16 (default), 2 = 512 x 32, 3 = 1024 x 8),
- * "sort.pack" (0/1: the passes that carry one key word and a row id stage both as one 16-byte LDS
+ * "sort.pack" (0/1: the passes that carry one key word and a row id, and the pass that turns two
+ * key words with the row id inside into one key word and an id, stage a row as one 16-byte LDS
  * record, on tiles of 8192 rows and 512 threads or more; default 1). */
 int ghive_set_option(ghive_ctx *ctx, const char *name, int64_t value);
 int ghive_get_option(ghive_ctx *ctx, const char *name, int64_t *value);

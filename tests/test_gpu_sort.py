@@ -123,9 +123,9 @@ def test_sort_info_counts_the_passes_it_ran(ctx, orc):
 @pytest.mark.parametrize("n", [1, 8191, 8193, 16383, 16385, 300_007])
 def test_sort_tile_shapes(ctx, orc, shape, n):
     """Every LSD tile shape (option sort.shape), with and without 16-byte LDS records (sort.pack),
-    gives the oracle's permutation, on sizes around
-    both tile lengths, with the row id inside the key (INT + FLOAT) and carried beside it (LONG +
-    FLOAT), and through the pass that drops word 0 (LONG + FLOAT + INT)."""
+    gives the oracle's permutation, on sizes around both tile lengths: the row id carried
+    beside one key word (INT + FLOAT) and beside two (LONG + FLOAT + INT), and inside two key words
+    through the pass that drops word 0 (LONG + FLOAT)."""
     rng = np.random.default_rng(shape * 1000 + n % 997)
     k = rng.integers(-3000, 3000, n).astype(np.int64)
     f = rng.random(n).astype(np.float32)
@@ -133,7 +133,7 @@ def test_sort_tile_shapes(ctx, orc, shape, n):
     i32 = rng.integers(-5, 5, n).astype(np.int32)
     ctx.set_option("sort.shape", shape)
     try:
-        for pack in (1, 0):  # key word and id staged as one 16-byte LDS record, or one after the other
+        for pack in (1, 0):  # a row's two words staged as one 16-byte LDS record, or one after the other
             ctx.set_option("sort.pack", pack)
             _check(ctx, orc, [i32, f], [0, 1], ["+", "-"])
             _check(ctx, orc, [k, f], [0, 1], ["-", "+"])
