@@ -67,6 +67,7 @@ struct ghive_tuning {
   int64_t gb_part_threads = 512;        // workgroup size of the group-by partition passes (256, 512, 1024)
   int64_t sort_shape = 1;               // LSD tile: 0 256 x 32, 1 512 x 16, 2 512 x 32, 3 1024 x 8
   int64_t sort_pack = 1;                // 1: key word + id (and the word-0-dropping pass) as 16-byte LDS records
+  int64_t sort_merge = 1;               // 1: key word + id become one word once the low half's digits are done
 };
 
 struct ghive_ctx {

@@ -369,6 +369,8 @@ int ghive_set_option(ghive_ctx *ctx, const char *name, int64_t value) {
     ctx->tune.sort_shape = value;
   } else if (k == "sort.pack") {
     ctx->tune.sort_pack = value ? 1 : 0;
+  } else if (k == "sort.merge") {
+    ctx->tune.sort_merge = value ? 1 : 0;
   } else if (k == "groupby.partition") {
     if (value < 0 || value > 2) return set_error(ctx, GHIVE_ERR_INVALID, "option %s: %lld (0, 1 or 2)", name, (long long)value);
     ctx->tune.gb_partition = value;
@@ -398,6 +400,7 @@ int ghive_get_option(ghive_ctx *ctx, const char *name, int64_t *value) {
   else if (k == "plan.fuse") *value = ctx->tune.plan_fuse;
   else if (k == "sort.shape") *value = ctx->tune.sort_shape;
   else if (k == "sort.pack") *value = ctx->tune.sort_pack;
+  else if (k == "sort.merge") *value = ctx->tune.sort_merge;
   else if (k == "groupby.part_threads") *value = ctx->tune.gb_part_threads;
   else return set_error(ctx, GHIVE_ERR_INVALID, "unknown option '%s'", name);
   return GHIVE_OK;

@@ -219,7 +219,8 @@ __global__ __launch_bounds__(TH) void k_radix_hist_plane(const uint8_t *plane, u
 // across waves and iterations. (2) The tile is rewritten through an LDS buffer in digit order, so
 // each digit's run leaves as consecutive addresses (coalesced stores) at its global offset.
 // DROP0: the last pass over word 0, whose low 32 bits hold the row id: the later passes need only
-// the id, so word 0 leaves as a uint32 id array (idx_out) and words 1.. as a (KW - 1)-word array.
+// the id, so word 0 leaves as a uint32 id array (idx_out) and words 1.. as a (KW - 1)-word array
+// (KW 1: the sort's last pass, which writes the permutation itself).
 // plane_out (optional): the next pass's digit, byte (next_shift / 8) of input word next_word, is
 // written beside every row as a one-byte plane for that pass's histogram.
 // COLS: the sort's first pass. It builds each row's composite key from the key columns (S, embed)
@@ -230,7 +231,10 @@ __global__ __launch_bounds__(TH) void k_radix_hist_plane(const uint8_t *plane, u
 // that drops word 0 (two key words, the id inside word 0). Two key words that both leave as key
 // words (10.3 ms against 9.8 ms per pass over 0.9e9 rows) and the other shapes stage one 8-byte
 // word at a time.
-template <int KW, bool IDS, bool DROP0, int TH, int PER, bool COLS = false, bool PACK = true>
+// MERGE (one key word with the ids beside it, the pass on the last digit of the word's low half):
+// the later digits all lie in the high half, so each row leaves as one word, its high half above its
+// id, and the passes after it move 8 bytes a row instead of 12 (option sort.merge).
+template <int KW, bool IDS, bool DROP0, int TH, int PER, bool COLS = false, bool PACK = true, bool MERGE = false>
 __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, const uint32_t *idx_in,
                                                       uint64_t *keys_out, uint32_t *idx_out, uint64_t n,
                                                       int word, int shift, const uint32_t *offs,
@@ -242,7 +246,8 @@ __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, c
   // 16-byte records when the tile's records fit LDS that way and the workgroup holds a CU's LDS
   // anyway (512 threads or more: one workgroup per CU either way; at 256 threads the 8-byte staging
   // keeps two)
-  constexpr bool kPacked = PACK && ((KW == 1 && IDS) || (KW == 2 && DROP0)) && kTile <= 8192 && TH >= 512;
+  static_assert(!MERGE || (KW == 1 && IDS && !DROP0 && !COLS), "MERGE: one key word with the ids beside it");
+  constexpr bool kPacked = !MERGE && PACK && ((KW == 1 && IDS) || (KW == 2 && DROP0)) && kTile <= 8192 && TH >= 512;
   typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
   __shared__ __attribute__((aligned(16))) uint64_t stage[kPacked ? 2 * kTile : kTile];
   __shared__ uint8_t sdigit[kTile];
@@ -396,20 +401,22 @@ __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, c
     }
     return;
   }
-  // write-out, one word at a time through the stage buffer (then the ids, when carried)
+  // write-out, one word at a time through the stage buffer (then the ids, when carried; MERGE: one
+  // round of the merged words)
 #pragma unroll
-  for (int q = 0; q < (IDS ? KW + 1 : KW); ++q) {
+  for (int q = 0; q < (MERGE ? 1 : IDS ? KW + 1 : KW); ++q) {
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < PER; ++j)
-      if (wave * kSeg + (uint32_t)j * kWave + lane < tn) stage[lpos[j]] = q < KW ? kv[j][q] : (uint64_t)id[j];
+      if (wave * kSeg + (uint32_t)j * kWave + lane < tn)
+        stage[lpos[j]] = MERGE ? (kv[j][0] & 0xFFFFFFFF00000000ull) | id[j] : q < KW ? kv[j][q] : (uint64_t)id[j];
     __syncthreads();
     for (uint32_t p = tid; p < tn; p += TH) {
       const uint32_t d = sdigit[p];
       const uint64_t g = (uint64_t)goff[d] + (p - lbase[d]);
       if (DROP0 && q == 0)
         idx_out[g] = (uint32_t)stage[p];
-      else if (q < KW)
+      else if (MERGE || q < KW)
         keys_out[(uint64_t)(DROP0 ? q - 1 : q) * n + g] = stage[p];
       else
         idx_out[g] = (uint32_t)stage[p];
@@ -453,10 +460,16 @@ __global__ __launch_bounds__(kOrandThreads) void k_orand_reduce(const unsigned l
 
 // One scatter pass over key words (not the first pass): the shapes that stage 16-byte LDS records
 // take them when sort.pack is on.
-template <int KW, bool IDS, bool DROP0, int TH, int PER>
+template <int KW, bool IDS, bool DROP0, int TH, int PER, bool MERGE = false>
 static int scatter_pass(ghive_ctx *ctx, const uint64_t *kin, const uint32_t *iin, uint64_t *kout, uint32_t *iout,
                         uint64_t n, int word, int shift, const uint32_t *offs, const uint32_t *counts,
                         uint32_t nblocks, uint8_t *pout, int nw, int ns) {
+  if constexpr (MERGE) {
+    k_radix_scatter<KW, IDS, DROP0, TH, PER, false, false, true><<<nblocks, TH, 0, ctx->stream>>>(
+        kin, iin, kout, iout, n, word, shift, offs, counts, nblocks, pout, nw, ns);
+    GH_LAUNCH(ctx);
+    return GHIVE_OK;
+  }
   constexpr bool kPackable = ((KW == 1 && IDS) || (KW == 2 && DROP0)) && TH >= 512 && TH * PER <= 8192;
   if constexpr (kPackable) {
     if (ctx->tune.sort_pack) {
@@ -521,15 +534,18 @@ static int sort_first(ghive_ctx *ctx, const SortKeySpec &S, bool embed, uint64_t
 
 // ids: carry the uint32 id arrays i0/i1 (false: the ids live inside the keys; i0/i1 unused).
 // drop0_last: the last pass writes word 0 as the uint32 id array (k_radix_scatter DROP0; needs
-// !ids, KW >= 2): its outputs are then *result (ids) and *keys_result (KW - 1 words).
+// !ids): its outputs are then *result (ids) and *keys_result (KW - 1 words).
 // planes (optional): two byte buffers of at least n rounded up to 8192 x 2 rows. *plane_cur: the
 // one that holds the first digit's plane (-1: none, the first histogram reads the keys); every
 // pass writes the next digit's plane into the other, and on return *plane_cur names the plane of
 // next_after (a digit of the same KW-word layout that a later call sorts on; -1: none).
+// merge_last (KW 1 with ids, the digits all in the low half): the last pass writes each word's high
+// half above its id (k_radix_scatter MERGE); the result is then *keys_result alone.
 template <int KW, int TH, int PER>
 static int radix_passes_shape(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, uint32_t *i0, uint32_t *i1,
                               const std::vector<int> &digits, uint32_t **result, uint64_t **keys_result, bool ids,
-                              bool drop0_last, uint8_t *const *planes, int *plane_cur, int next_after) {
+                              bool drop0_last, uint8_t *const *planes, int *plane_cur, int next_after,
+                              bool merge_last) {
   const uint32_t nblocks = ceil_div(n, (uint64_t)TH * PER);
   DevBuf hist, counts;
   GH_TRY(hist.alloc(ctx, sizeof(uint32_t) * 256ull * nblocks));
@@ -548,22 +564,29 @@ static int radix_passes_shape(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t
                                                              counts.as<uint32_t>(), nblocks);
     GH_LAUNCH(ctx);
     GH_TRY(exclusive_scan_u32(ctx, hist.as<uint32_t>(), hist.as<uint32_t>(), 256ull * nblocks, nullptr));
-    const bool drop = drop0_last && KW >= 2 && di + 1 == digits.size();
+    const bool drop = drop0_last && !ids && di + 1 == digits.size();
     const int next = di + 1 < digits.size() ? digits[di + 1] : next_after;
     uint8_t *pout = planes && next >= 0 ? planes[cur == 0 ? 1 : 0] : nullptr;
     const int nw = next >= 0 ? next / 8 : -1, ns = next >= 0 ? 8 * (next % 8) : 0;
     const uint32_t *ho = hist.as<uint32_t>(), *co = counts.as<uint32_t>();
+    const bool merge = KW == 1 && ids && merge_last && di + 1 == digits.size();
+    if constexpr (KW == 1) {
+      if (merge)
+        GH_TRY((scatter_pass<1, true, false, TH, PER, true>(ctx, kin, iin, kout, iout, n, word, shift, ho, co, nblocks,
+                                                            pout, nw, ns)));
+    }
     if (drop)
       GH_TRY((scatter_pass<KW, false, true, TH, PER>(ctx, kin, iin, kout, iout, n, word, shift, ho, co, nblocks, pout,
                                                       nw, ns)));
-    else if (ids)
+    else if (ids && !merge)
       GH_TRY((scatter_pass<KW, true, false, TH, PER>(ctx, kin, iin, kout, iout, n, word, shift, ho, co, nblocks, pout,
                                                       nw, ns)));
-    else
+    else if (!ids)
       GH_TRY((scatter_pass<KW, false, false, TH, PER>(ctx, kin, iin, kout, iout, n, word, shift, ho, co, nblocks,
                                                        pout, nw, ns)));
     cur = pout ? (cur == 0 ? 1 : 0) : -1;
-    const uint64_t in_b = 8ull * KW + (ids ? 4 : 0), out_b = (drop ? 8ull * (KW - 1) + 4 : in_b) + (pout ? 1 : 0);
+    const uint64_t in_b = 8ull * KW + (ids ? 4 : 0);
+    const uint64_t out_b = (drop ? 8ull * (KW - 1) + 4 : merge ? 8ull : in_b) + (pout ? 1 : 0);
     ctx->sort_info.passes += 1;
     ctx->sort_info.scatter_bytes += n * (in_b + out_b);
     std::swap(kin, kout);
@@ -579,10 +602,10 @@ template <int KW>
 static int radix_passes(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, uint32_t *i0, uint32_t *i1,
                         const std::vector<int> &digits, uint32_t **result, uint64_t **keys_result = nullptr,
                         bool ids = true, bool drop0_last = false, uint8_t *const *planes = nullptr,
-                        int *plane_cur = nullptr, int next_after = -1) {
+                        int *plane_cur = nullptr, int next_after = -1, bool merge_last = false) {
 #define GH_PASSES(TH, PER) \
   radix_passes_shape<KW, TH, PER>(ctx, n, k0, k1, i0, i1, digits, result, keys_result, ids, drop0_last, planes, \
-                                  plane_cur, next_after)
+                                  plane_cur, next_after, merge_last)
   switch (ctx->tune.sort_shape) {
     case 0: return GH_PASSES(kSortThreads, kSortPerThread);
     case 2: return GH_PASSES(512, 32);
@@ -603,6 +626,37 @@ int radix_sort_u64_ids(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, u
 int radix_sort_u64x2_ids(ghive_ctx *ctx, uint64_t n, uint64_t *k0, uint64_t *k1, uint32_t *i0, uint32_t *i1,
                          const std::vector<int> &digits, uint64_t **keys_out, uint32_t **ids_out, bool ids) {
   return radix_passes<2>(ctx, n, k0, k1, i0, i1, digits, ids_out, keys_out, ids);
+}
+
+// The two id buffers in the order that makes the last of p passes (p >= 1) write its ids into
+// `target`: a pass reads the first and writes the second, and they swap after every pass.
+static void ids_last_into(uint32_t *target, uint32_t *spare, size_t p, uint32_t **i0, uint32_t **i1) {
+  const bool swapped = (p - 1) % 2 == 1;
+  *i0 = swapped ? target : spare;
+  *i1 = swapped ? spare : target;
+}
+
+// The ORDER BY's passes over one key word with the row ids beside it (keys K0 / ids I0, ping-pong
+// partners K1 / I1; out is I0 or I1). When digits remain in both halves of the word (and option
+// sort.merge is on), the pass on the low half's last digit writes each row as one word, its high
+// half above its id, and the passes over the high half then move 8 bytes a row instead of 12, the
+// last of them writing the ids alone into out. *ids: where the permutation ends (out after a merge).
+static int passes_one_word_ids(ghive_ctx *ctx, uint64_t n, uint64_t *K0, uint64_t *K1, uint32_t *I0, uint32_t *I1,
+                               uint32_t *out, const std::vector<int> &digits, uint8_t *const *planes, int *pcur,
+                               uint32_t **ids) {
+  std::vector<int> lo, hi;
+  for (int d : digits) (d < 4 ? lo : hi).push_back(d);
+  uint64_t *kres = nullptr;
+  if (!ctx->tune.sort_merge || lo.empty() || hi.empty())
+    return radix_passes<1>(ctx, n, K0, K1, I0, I1, digits, ids, &kres, true, false, planes, pcur);
+  uint64_t *words = nullptr;
+  uint32_t *unused = nullptr;
+  GH_TRY(radix_passes<1>(ctx, n, K0, K1, I0, I1, lo, &unused, &words, true, false, planes, pcur, hi.front(),
+                         /*merge_last=*/true));
+  uint32_t *j0 = nullptr, *j1 = nullptr;  // the ids now ride in the words: both buffers are free
+  ids_last_into(out, out == I0 ? I1 : I0, hi.size(), &j0, &j1);
+  return radix_passes<1>(ctx, n, words, words == K0 ? K1 : K0, j0, j1, hi, ids, &kres, false, /*drop0_last=*/true,
+                         planes, pcur);
 }
 
 // ---- STRING keys: dense ranks in Hive's string order -------------------------------------------
@@ -866,7 +920,7 @@ int ghive_sort(ghive_ctx *ctx, const ghive_table *table, const uint32_t *key_col
     uint32_t *iother = ids == out_perm ? I1 : out_perm;
     for (int &d : dhi) d -= 8;
     switch (KW) {
-      case 2: GH_TRY(radix_passes<1>(ctx, n, keys, other, ids, iother, dhi, &res, &kres, true, false, planes, &pcur)); break;
+      case 2: GH_TRY(passes_one_word_ids(ctx, n, keys, other, ids, iother, out_perm, dhi, planes, &pcur, &res)); break;
       case 3: GH_TRY(radix_passes<2>(ctx, n, keys, other, ids, iother, dhi, &res, &kres, true, false, planes, &pcur)); break;
       default: GH_TRY(radix_passes<3>(ctx, n, keys, other, ids, iother, dhi, &res, &kres, true, false, planes, &pcur)); break;
     }
@@ -874,13 +928,20 @@ int ghive_sort(ghive_ctx *ctx, const ghive_table *table, const uint32_t *key_col
     GH_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return GHIVE_OK;
   }
+  // ids inside the keys: the last pass writes them alone, as the permutation, into out_perm
+  const bool drop_last = embed && !digits.empty();
+  uint32_t *j0 = out_perm, *j1 = I1;
+  if (drop_last) ids_last_into(out_perm, I1, digits.size(), &j0, &j1);
   switch (KW) {
-    case 1: GH_TRY(radix_passes<1>(ctx, n, K0, K1, out_perm, I1, digits, &res, &kres, !embed, false, planes, &pcur)); break;
-    case 2: GH_TRY(radix_passes<2>(ctx, n, K0, K1, out_perm, I1, digits, &res, &kres, !embed, false, planes, &pcur)); break;
-    case 3: GH_TRY(radix_passes<3>(ctx, n, K0, K1, out_perm, I1, digits, &res, &kres, !embed, false, planes, &pcur)); break;
-    default: GH_TRY(radix_passes<4>(ctx, n, K0, K1, out_perm, I1, digits, &res, &kres, !embed, false, planes, &pcur)); break;
+    case 1:
+      if (embed) GH_TRY(radix_passes<1>(ctx, n, K0, K1, j0, j1, digits, &res, &kres, false, drop_last, planes, &pcur));
+      else GH_TRY(passes_one_word_ids(ctx, n, K0, K1, out_perm, I1, out_perm, digits, planes, &pcur, &res));
+      break;
+    case 2: GH_TRY(radix_passes<2>(ctx, n, K0, K1, j0, j1, digits, &res, &kres, !embed, drop_last, planes, &pcur)); break;
+    case 3: GH_TRY(radix_passes<3>(ctx, n, K0, K1, j0, j1, digits, &res, &kres, !embed, drop_last, planes, &pcur)); break;
+    default: GH_TRY(radix_passes<4>(ctx, n, K0, K1, j0, j1, digits, &res, &kres, !embed, drop_last, planes, &pcur)); break;
   }
-  if (embed) {  // the permutation is the low 32 bits of word 0 in sorted order
+  if (embed && !drop_last) {  // no pass after the first: the permutation is the low 32 bits of word 0
     k_low32<<<grid, 256, 0, ctx->stream>>>(kres, n, out_perm);
     GH_LAUNCH(ctx);
   } else if (res != out_perm) {

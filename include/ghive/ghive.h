@@ -506,7 +506,10 @@ int ghive_fragment_execute_late(ghive_ctx *ctx, const ghive_fragment *frag, ghiv
  * of ghive_sort: 0 = 256 threads x 32 rows, 1 = 512 x 16 (default), 2 = 512 x 32, 3 = 1024 x 8),
  * "sort.pack" (0/1: the passes that carry one key word and a row id, and the pass that turns two
  * key words with the row id inside into one key word and an id, stage a row as one 16-byte LDS
- * record, on tiles of 8192 rows and 512 threads or more; default 1). */
+ * record, on tiles of 8192 rows and 512 threads or more; default 1), "sort.merge" (0/1: a sort on
+ * one key word with the row ids beside it, whose digits lie in both halves of the word, writes each
+ * row as one word, the high half above the id, after the low half's last digit, so the passes over
+ * the high half move 8 bytes a row instead of 12; default 1). */
 int ghive_set_option(ghive_ctx *ctx, const char *name, int64_t value);
 int ghive_get_option(ghive_ctx *ctx, const char *name, int64_t *value);
 

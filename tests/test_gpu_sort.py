@@ -123,7 +123,8 @@ def test_sort_info_counts_the_passes_it_ran(ctx, orc):
 @pytest.mark.parametrize("n", [1, 8191, 8193, 16383, 16385, 300_007])
 def test_sort_tile_shapes(ctx, orc, shape, n):
     """Every LSD tile shape (option sort.shape), with and without 16-byte LDS records (sort.pack),
-    gives the oracle's permutation, on sizes around both tile lengths: the row id carried
+    with and without the pass that merges a key word's high half and its row id into one word
+    (sort.merge), gives the oracle's permutation, on sizes around both tile lengths: the row id carried
     beside one key word (INT + FLOAT) and beside two (LONG + FLOAT + INT), and inside two key words
     through the pass that drops word 0 (LONG + FLOAT)."""
     rng = np.random.default_rng(shape * 1000 + n % 997)
@@ -133,14 +134,16 @@ def test_sort_tile_shapes(ctx, orc, shape, n):
     i32 = rng.integers(-5, 5, n).astype(np.int32)
     ctx.set_option("sort.shape", shape)
     try:
-        for pack in (1, 0):  # a row's two words staged as one 16-byte LDS record, or one after the other
+        for pack, merge in ((1, 1), (0, 1), (1, 0)):
             ctx.set_option("sort.pack", pack)
+            ctx.set_option("sort.merge", merge)
             _check(ctx, orc, [i32, f], [0, 1], ["+", "-"])
             _check(ctx, orc, [k, f], [0, 1], ["-", "+"])
             _check(ctx, orc, [k, f, i32], [0, 1, 2], ["+", "+", "-"])
     finally:
         ctx.set_option("sort.shape", 1)
         ctx.set_option("sort.pack", 1)
+        ctx.set_option("sort.merge", 1)
 
 
 def test_sort_info_counts_digit_planes(ctx, orc):
@@ -153,3 +156,30 @@ def test_sort_info_counts_digit_planes(ctx, orc):
     _check(ctx, orc, [k], [0], ["+"])
     info = ops.sort_info(ctx)
     assert info["passes"] == 2 and info["scatter_bytes"] == (8 + 12 + 1) * n + 24 * n
+
+
+def test_sort_merges_the_row_id_into_the_key_word(ctx, orc):
+    """LONG keys over the whole range: the first pass (byte 0, the 8-byte column in, key word + id
+    out, 21 bytes a row with the next plane), bytes 1 and 2 (key word + id both ways, 25), byte 3 as
+    the merging pass (12 in, one 8-byte word out: 21), bytes 4-6 on the merged words (17) and byte 7
+    (the merged words in, the 4-byte ids out as the permutation: 12): 155 bytes a row over 8 passes
+    against 195 without the merge (option sort.merge 0). Both orders equal the oracle's, also with
+    duplicates and a second key (INT + FLOAT: one key word, the id beside it)."""
+    from ghive_amd import ops
+    n = 100_003
+    rng = np.random.default_rng(11)
+    k = rng.integers(-2**63, 2**63 - 1, n, dtype=np.int64)
+    k[::5] = k[::5] % 977
+    try:
+        for merge, per_row in ((1, 155), (0, 195)):
+            ctx.set_option("sort.merge", merge)
+            for order in ("+", "-"):
+                _check(ctx, orc, [k], [0], [order])
+                info = ops.sort_info(ctx)
+                assert info["passes"] == 8 and info["scatter_bytes"] == per_row * n
+            i = rng.integers(-2**31, 2**31 - 1, n).astype(np.int32)
+            f = rng.standard_normal(n).astype(np.float32)
+            f[::13] = np.nan
+            _check(ctx, orc, [i, f], [0, 1], ["-", "+"])
+    finally:
+        ctx.set_option("sort.merge", 1)
