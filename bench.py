@@ -601,6 +601,8 @@ def main():
     ap.add_argument("--skew-reps", type=int, default=3)
     ap.add_argument("--sort-shape", type=int, default=-1, help="LSD tile of the sort (option sort.shape)")
     ap.add_argument("--sort-pack", type=int, default=-1, help="16-byte LDS records in the sort (option sort.pack)")
+    ap.add_argument("--sort-occupancy", type=int, default=0,
+                    help="workgroups per CU of the sort's 8-byte-staging passes (option sort.occupancy)")
     ap.add_argument("--part-threads", type=int, default=0, help="group-by partition workgroup size (256/512)")
     ap.add_argument("--groupby-form", default="auto", choices=["auto", "direct", "partitioned"],
                     help="form of the 1e9-row / 100M-key group-by (direct = one global hash table)")
@@ -637,6 +639,8 @@ def main():
         ctx.set_option("sort.shape", args.sort_shape)
     if args.sort_pack >= 0:
         ctx.set_option("sort.pack", args.sort_pack)
+    if args.sort_occupancy:
+        ctx.set_option("sort.occupancy", args.sort_occupancy)
     if args.part_threads:
         ctx.set_option("groupby.part_threads", args.part_threads)
     variant = ("late" if args.late else "early") + ("/hash" if args.no_dense else

@@ -65,9 +65,11 @@ struct ghive_tuning {
   int64_t gb_partition = 1;             // high-cardinality form: 0 never, 1 by size, 2 always
   int64_t plan_fuse = 1;                // 1: plan vertices of the map-join shape run as one fragment
   int64_t gb_part_threads = 512;        // workgroup size of the group-by partition passes (256, 512, 1024)
+  int64_t gb_part_occupancy = 2;        // workgroups per CU of its 512-thread 8-bit passes (1 or 2)
   int64_t sort_shape = 1;               // LSD tile: 0 256 x 32, 1 512 x 16, 2 512 x 32, 3 1024 x 8
-  int64_t sort_pack = 1;                // 1: key word + id (and the word-0-dropping pass) as 16-byte LDS records
+  int64_t sort_pack = 0;                // 1: key word + id (and the word-0-dropping pass) as 16-byte LDS records
   int64_t sort_merge = 1;               // 1: key word + id become one word once the low half's digits are done
+  int64_t sort_occupancy = 2;           // workgroups per CU of the 8-byte-staging 512 x 16 passes (1 or 2)
 };
 
 struct ghive_ctx {

@@ -364,6 +364,9 @@ int ghive_set_option(ghive_ctx *ctx, const char *name, int64_t value) {
     if (value != 256 && value != 512 && value != 1024)
       return set_error(ctx, GHIVE_ERR_INVALID, "option %s: %lld (256, 512 or 1024)", name, (long long)value);
     ctx->tune.gb_part_threads = value;
+  } else if (k == "groupby.part_occupancy") {
+    if (value != 1 && value != 2) return set_error(ctx, GHIVE_ERR_INVALID, "option %s: %lld (1 or 2)", name, (long long)value);
+    ctx->tune.gb_part_occupancy = value;
   } else if (k == "sort.shape") {
     if (value < 0 || value > 3) return set_error(ctx, GHIVE_ERR_INVALID, "option %s: %lld (0..3)", name, (long long)value);
     ctx->tune.sort_shape = value;
@@ -371,6 +374,9 @@ int ghive_set_option(ghive_ctx *ctx, const char *name, int64_t value) {
     ctx->tune.sort_pack = value ? 1 : 0;
   } else if (k == "sort.merge") {
     ctx->tune.sort_merge = value ? 1 : 0;
+  } else if (k == "sort.occupancy") {
+    if (value != 1 && value != 2) return set_error(ctx, GHIVE_ERR_INVALID, "option %s: %lld (1 or 2)", name, (long long)value);
+    ctx->tune.sort_occupancy = value;
   } else if (k == "groupby.partition") {
     if (value < 0 || value > 2) return set_error(ctx, GHIVE_ERR_INVALID, "option %s: %lld (0, 1 or 2)", name, (long long)value);
     ctx->tune.gb_partition = value;
@@ -401,7 +407,9 @@ int ghive_get_option(ghive_ctx *ctx, const char *name, int64_t *value) {
   else if (k == "sort.shape") *value = ctx->tune.sort_shape;
   else if (k == "sort.pack") *value = ctx->tune.sort_pack;
   else if (k == "sort.merge") *value = ctx->tune.sort_merge;
+  else if (k == "sort.occupancy") *value = ctx->tune.sort_occupancy;
   else if (k == "groupby.part_threads") *value = ctx->tune.gb_part_threads;
+  else if (k == "groupby.part_occupancy") *value = ctx->tune.gb_part_occupancy;
   else return set_error(ctx, GHIVE_ERR_INVALID, "unknown option '%s'", name);
   return GHIVE_OK;
 }

@@ -834,9 +834,12 @@ __global__ __launch_bounds__(TH) void k_gbp_hist_plane(const uint8_t *plane, uin
 // rows' non-NULL values, as the bits of a non-negative double, at part_max[blockIdx.x], with bit 63
 // (kNullSeen, never set in such bits) set when one of its rows is NULL -- the exact SUM's scale,
 // and whether a group can lack a non-NULL value, come from these without a pass over the column.
+// OCC: workgroups per CU the kernel is compiled for (the launch bound's second figure is waves per
+// SIMD, TH / 256 per workgroup; 2 holds a 512-thread pass to 128 VGPRs, and its 8-bit tile's LDS
+// is under 80 KiB, so two share a CU; option groupby.part_occupancy).
 constexpr unsigned long long kNullSeen = 1ull << 63;
-template <int DB, int SRC, bool VAL, int TH>
-__global__ __launch_bounds__(TH) void k_gbp_scatter(GbpSrc S, uint64_t n, int shift, const uint32_t *offs,
+template <int DB, int SRC, bool VAL, int TH, int OCC = 1>
+__global__ __launch_bounds__(TH, TH / 256 * OCC) void k_gbp_scatter(GbpSrc S, uint64_t n, int shift, const uint32_t *offs,
                                                     const uint32_t *counts, uint32_t nblocks, uint64_t *h_out,
                                                     uint64_t *v_out, uint8_t *plane_out, int next_shift,
                                                     unsigned long long *part_max) {
@@ -846,7 +849,7 @@ __global__ __launch_bounds__(TH) void k_gbp_scatter(GbpSrc S, uint64_t n, int sh
   __shared__ uint64_t stage[kTile];
   __shared__ Digit sdigit[kTile];
   __shared__ uint32_t lbase[B], goff[B];
-  __shared__ uint32_t wcnt[kWaves][B];
+  __shared__ uint16_t wcnt[kWaves][B];  // per-wave digit counts, then their prefix over waves (< 2^16)
   __shared__ uint32_t scan_tmp[kWaves];
   const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
   const uint64_t t0 = (uint64_t)blockIdx.x * kTile;
@@ -874,7 +877,9 @@ __global__ __launch_bounds__(TH) void k_gbp_scatter(GbpSrc S, uint64_t n, int sh
   // every load of the tile first: element wave * kSeg + j * 64 + lane; past the end a lane
   // re-reads the last element (ignored)
   uint64_t hv[kPer], vv[kPer];
-  uint32_t lpos[kPer], dg[kPer];
+  // lpos: a row's rank among its wave's equal digits, with the digit in bits 16.., then its position
+  // in the tile (one register a row: registers decide whether two workgroups share a CU)
+  uint32_t lpos[kPer];
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const uint32_t el = wave * kSeg + (uint32_t)j * kWave + lane;
@@ -944,10 +949,9 @@ __global__ __launch_bounds__(TH) void k_gbp_scatter(GbpSrc S, uint64_t n, int sh
     const uint32_t before = wcnt[wave][digit];
     const uint32_t rank = (uint32_t)__popcll(peers & lt);
     __builtin_amdgcn_wave_barrier();
-    if (valid && rank == 0) wcnt[wave][digit] = before + (uint32_t)__popcll(peers);
+    if (valid && rank == 0) wcnt[wave][digit] = (uint16_t)(before + (uint32_t)__popcll(peers));
     __builtin_amdgcn_wave_barrier();
-    lpos[j] = before + rank;
-    dg[j] = digit;
+    lpos[j] = digit << 16 | (before + rank);  // before + rank < kSeg <= 2^16
   }
   __syncthreads();
 #pragma unroll
@@ -958,7 +962,7 @@ __global__ __launch_bounds__(TH) void k_gbp_scatter(GbpSrc S, uint64_t n, int sh
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) {
       const uint32_t c = wcnt[w][d];
-      wcnt[w][d] = acc;
+      wcnt[w][d] = (uint16_t)acc;
       acc += c;
     }
   }
@@ -966,9 +970,10 @@ __global__ __launch_bounds__(TH) void k_gbp_scatter(GbpSrc S, uint64_t n, int sh
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     if (wave * kSeg + (uint32_t)j * kWave + lane < tn) {
-      const uint32_t pos = lbase[dg[j]] + wcnt[wave][dg[j]] + lpos[j];
+      const uint32_t digit = lpos[j] >> 16;
+      const uint32_t pos = lbase[digit] + wcnt[wave][digit] + (lpos[j] & 0xFFFFu);
       lpos[j] = pos;
-      sdigit[pos] = (Digit)dg[j];
+      sdigit[pos] = (Digit)digit;
     }
   }
 #pragma unroll
@@ -1056,9 +1061,20 @@ static int gbp_partition_th(ghive_ctx *ctx, const GbpSrc &src, uint64_t n, int p
 #undef GBP_HIST
     GH_LAUNCH(ctx);
     GH_TRY(exclusive_scan_u32(ctx, hist.as<uint32_t>(), hist.as<uint32_t>(), (uint64_t)bins * nb, nullptr));
-#define GBP_SCATTER(D, F, V)                                                                                   \
-  k_gbp_scatter<D, F, V, TH><<<nb, TH, 0, ctx->stream>>>(S, n, sh, hist.as<uint32_t>(), counts.as<uint32_t>(), nb, \
-                                                         hout, vout, pout, nsh, pmax)
+#define GBP_SCATTER(D, F, V)                                                                                     \
+  do {                                                                                                          \
+    bool two_ = false;                                                                                          \
+    if constexpr (D == 8 && TH == 512) {                                                                         \
+      if (ctx->tune.gb_part_occupancy == 2) {                                                                   \
+        k_gbp_scatter<D, F, V, TH, 2><<<nb, TH, 0, ctx->stream>>>(S, n, sh, hist.as<uint32_t>(),                 \
+                                                                  counts.as<uint32_t>(), nb, hout, vout, pout, nsh, pmax); \
+        two_ = true;                                                                                            \
+      }                                                                                                         \
+    }                                                                                                           \
+    if (!two_)                                                                                                  \
+      k_gbp_scatter<D, F, V, TH><<<nb, TH, 0, ctx->stream>>>(S, n, sh, hist.as<uint32_t>(), counts.as<uint32_t>(),   \
+                                                             nb, hout, vout, pout, nsh, pmax);                  \
+  } while (0)
     if (db == 9) {
       if (src == kSrcCols) { if (val) GBP_SCATTER(9, kSrcCols, true); else GBP_SCATTER(9, kSrcCols, false); }
       else if (src == kSrcCol64) { if (val) GBP_SCATTER(9, kSrcCol64, true); else GBP_SCATTER(9, kSrcCol64, false); }

@@ -234,8 +234,13 @@ __global__ __launch_bounds__(TH) void k_radix_hist_plane(const uint8_t *plane, u
 // MERGE (one key word with the ids beside it, the pass on the last digit of the word's low half):
 // the later digits all lie in the high half, so each row leaves as one word, its high half above its
 // id, and the passes after it move 8 bytes a row instead of 12 (option sort.merge).
-template <int KW, bool IDS, bool DROP0, int TH, int PER, bool COLS = false, bool PACK = true, bool MERGE = false>
-__global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, const uint32_t *idx_in,
+// OCC: workgroups per CU the kernel is compiled for (2: at most 128 VGPRs; the 8-byte staging of a
+// 512 x 16 tile holds under 80 KiB of LDS, so two workgroups share a CU and one's loads and stores
+// overlap the other's ranking and barriers; option sort.occupancy).
+// (The launch bound's second figure is waves per SIMD: a workgroup of TH threads puts TH / 256 on each.)
+template <int KW, bool IDS, bool DROP0, int TH, int PER, bool COLS = false, bool PACK = true, bool MERGE = false,
+          int OCC = 1>
+__global__ __launch_bounds__(TH, TH / 256 * OCC) void k_radix_scatter(const uint64_t *keys_in, const uint32_t *idx_in,
                                                       uint64_t *keys_out, uint32_t *idx_out, uint64_t n,
                                                       int word, int shift, const uint32_t *offs,
                                                       const uint32_t *counts, uint32_t nblocks,
@@ -252,12 +257,14 @@ __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, c
   __shared__ __attribute__((aligned(16))) uint64_t stage[kPacked ? 2 * kTile : kTile];
   __shared__ uint8_t sdigit[kTile];
   __shared__ uint32_t lbase[256], goff[256];
-  __shared__ uint32_t wcnt[kWaves][256];
+  __shared__ uint16_t wcnt[kWaves][256];  // per-wave digit counts, then their prefix over waves (< 2^16)
   __shared__ uint32_t scan_tmp[kWaves];
   constexpr uint32_t kSeg = PER * kWave;  // each wave owns one contiguous segment of the tile
   const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
   uint64_t kv[PER][KW];
-  uint32_t id[PER], lpos[PER], dg[PER];
+  // lpos: a row's rank among its wave's equal digits, with the digit in bits 16..23, then its
+  // position in the tile (one register a row: registers decide whether two workgroups share a CU)
+  uint32_t id[PER], lpos[PER];
   const uint32_t tile = blockIdx.x;
   const uint64_t t0 = (uint64_t)tile * kTile;
   const uint32_t tn = (uint32_t)(n - t0 < kTile ? n - t0 : kTile);
@@ -347,10 +354,9 @@ __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, c
     const uint32_t before = wcnt[wave][digit];
     const uint32_t rank = (uint32_t)__popcll(peers & lt);
     __builtin_amdgcn_wave_barrier();
-    if (valid && rank == 0) wcnt[wave][digit] = before + (uint32_t)__popcll(peers);
+    if (valid && rank == 0) wcnt[wave][digit] = (uint16_t)(before + (uint32_t)__popcll(peers));
     __builtin_amdgcn_wave_barrier();
-    lpos[j] = before + rank;
-    dg[j] = digit;
+    lpos[j] = digit << 16 | (before + rank);  // before + rank < kSeg <= 2^16
   }
   __syncthreads();
   if (tid < 256) {  // exclusive prefix of each digit's counts over the waves (segments are in wave order)
@@ -358,7 +364,7 @@ __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, c
 #pragma unroll
     for (int w = 0; w < (int)kWaves; ++w) {
       const uint32_t c = wcnt[w][tid];
-      wcnt[w][tid] = acc;
+      wcnt[w][tid] = (uint16_t)acc;
       acc += c;
     }
   }
@@ -366,9 +372,10 @@ __global__ __launch_bounds__(TH) void k_radix_scatter(const uint64_t *keys_in, c
 #pragma unroll
   for (int j = 0; j < PER; ++j) {
     if (wave * kSeg + (uint32_t)j * kWave + lane < tn) {
-      const uint32_t pos = lbase[dg[j]] + wcnt[wave][dg[j]] + lpos[j];
+      const uint32_t digit = lpos[j] >> 16;
+      const uint32_t pos = lbase[digit] + wcnt[wave][digit] + (lpos[j] & 0xFFFFu);
       lpos[j] = pos;
-      sdigit[pos] = (uint8_t)dg[j];
+      sdigit[pos] = (uint8_t)digit;
     }
   }
   if constexpr (kPacked) {
@@ -464,13 +471,7 @@ template <int KW, bool IDS, bool DROP0, int TH, int PER, bool MERGE = false>
 static int scatter_pass(ghive_ctx *ctx, const uint64_t *kin, const uint32_t *iin, uint64_t *kout, uint32_t *iout,
                         uint64_t n, int word, int shift, const uint32_t *offs, const uint32_t *counts,
                         uint32_t nblocks, uint8_t *pout, int nw, int ns) {
-  if constexpr (MERGE) {
-    k_radix_scatter<KW, IDS, DROP0, TH, PER, false, false, true><<<nblocks, TH, 0, ctx->stream>>>(
-        kin, iin, kout, iout, n, word, shift, offs, counts, nblocks, pout, nw, ns);
-    GH_LAUNCH(ctx);
-    return GHIVE_OK;
-  }
-  constexpr bool kPackable = ((KW == 1 && IDS) || (KW == 2 && DROP0)) && TH >= 512 && TH * PER <= 8192;
+  constexpr bool kPackable = !MERGE && ((KW == 1 && IDS) || (KW == 2 && DROP0)) && TH >= 512 && TH * PER <= 8192;
   if constexpr (kPackable) {
     if (ctx->tune.sort_pack) {
       k_radix_scatter<KW, IDS, DROP0, TH, PER, false, true><<<nblocks, TH, 0, ctx->stream>>>(
@@ -479,7 +480,17 @@ static int scatter_pass(ghive_ctx *ctx, const uint64_t *kin, const uint32_t *iin
       return GHIVE_OK;
     }
   }
-  k_radix_scatter<KW, IDS, DROP0, TH, PER, false, false><<<nblocks, TH, 0, ctx->stream>>>(
+  // 8-byte staging of a 512 x 16 tile: two workgroups per CU fit its LDS
+  constexpr bool kTwo = TH == 512 && PER == 16 && KW <= 2;
+  if constexpr (kTwo) {
+    if (ctx->tune.sort_occupancy == 2) {
+      k_radix_scatter<KW, IDS, DROP0, TH, PER, false, false, MERGE, 2><<<nblocks, TH, 0, ctx->stream>>>(
+          kin, iin, kout, iout, n, word, shift, offs, counts, nblocks, pout, nw, ns);
+      GH_LAUNCH(ctx);
+      return GHIVE_OK;
+    }
+  }
+  k_radix_scatter<KW, IDS, DROP0, TH, PER, false, false, MERGE><<<nblocks, TH, 0, ctx->stream>>>(
       kin, iin, kout, iout, n, word, shift, offs, counts, nblocks, pout, nw, ns);
   GH_LAUNCH(ctx);
   return GHIVE_OK;

@@ -502,11 +502,15 @@ int ghive_fragment_execute_late(ghive_ctx *ctx, const ghive_fragment *frag, ghiv
  * with direct group slots where the group keys' ranges allow it), "groupby.lds_budget" (bytes),
  * "groupby.partition" (0 never, 1 by size, 2 always: the hash-partitioned high-cardinality form),
  * "groupby.part_threads" (256, 512 or 1024: workgroup size of the partition passes; default 512),
+ * "groupby.part_occupancy" (1/2: workgroups per CU its 512-thread 8-bit passes are compiled for; default 2),
  * "plan.fuse" (0/1: run map-join vertices of a plan as one fused fragment), "sort.shape" (LSD tile
  * of ghive_sort: 0 = 256 threads x 32 rows, 1 = 512 x 16 (default), 2 = 512 x 32, 3 = 1024 x 8),
  * "sort.pack" (0/1: the passes that carry one key word and a row id, and the pass that turns two
  * key words with the row id inside into one key word and an id, stage a row as one 16-byte LDS
- * record, on tiles of 8192 rows and 512 threads or more; default 1), "sort.merge" (0/1: a sort on
+ * record, on tiles of 8192 rows and 512 threads or more, one workgroup per CU; default 0: the
+ * 8-byte staging lets two workgroups share a CU), "sort.occupancy" (1/2: workgroups per CU the
+ * 8-byte-staging passes of the 512 x 16 tile are compiled for; 2 holds them to 128 VGPRs; default
+ * 2), "sort.merge" (0/1: a sort on
  * one key word with the row ids beside it, whose digits lie in both halves of the word, writes each
  * row as one word, the high half above the id, after the low half's last digit, so the passes over
  * the high half move 8 bytes a row instead of 12; default 1). */

@@ -255,12 +255,14 @@ def test_groupby_wide_keys_edges(ctx, orc):
     _compare(_as_dict(outs, 2), _oracle_dict(orc, cols, [0, 1], [(abi.AGG_CNT, abi.COUNT_STAR), (abi.AGG_SUM, 2)]))
 
 
-@pytest.mark.parametrize("threads", [256, 512, 1024])
+@pytest.mark.parametrize("threads,occ", [(256, 1), (512, 2), (512, 1), (1024, 1)])
 @pytest.mark.parametrize("aggset", ["count_star", "one_column", "two_columns"])
-def test_groupby_partitioned_value_forms(ctx, orc, aggset, threads):
+def test_groupby_partitioned_value_forms(ctx, orc, aggset, threads, occ):
     """The high-cardinality form reads aggregate inputs three ways: none (COUNT(*) only), one column
-    carried through the partition passes, several columns read at the row id. LONG keys include -1,
-    whose bits equal the EMPTY pattern (aggregated in the small global table), and NULL."""
+    carried through the partition passes, several columns read at the row id; its partition passes
+    at every workgroup size, the 512-thread ones compiled for one or two workgroups per CU (option
+    groupby.part_occupancy). LONG keys include -1, whose bits equal the EMPTY pattern (aggregated in
+    the small global table), and NULL."""
     rng = np.random.default_rng(len(aggset))
     n = 500_000
     k = rng.integers(-1000, 60_000, n).astype(np.int64)
@@ -276,6 +278,7 @@ def test_groupby_partitioned_value_forms(ctx, orc, aggset, threads):
     cols = [k, x, y]
     ctx.set_option("groupby.partition", 2)
     ctx.set_option("groupby.part_threads", threads)
+    ctx.set_option("groupby.part_occupancy", occ)
     try:
         outs = _run(ctx, cols, [0], aggs)
         _compare(_as_dict(outs, 1), _oracle_dict(orc, cols, [0], aggs))
@@ -289,6 +292,7 @@ def test_groupby_partitioned_value_forms(ctx, orc, aggset, threads):
     finally:
         ctx.set_option("groupby.partition", 1)
         ctx.set_option("groupby.part_threads", 512)
+        ctx.set_option("groupby.part_occupancy", 2)
 
 
 @pytest.mark.parametrize("threads", [256, 512])

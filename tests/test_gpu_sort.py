@@ -124,7 +124,8 @@ def test_sort_info_counts_the_passes_it_ran(ctx, orc):
 def test_sort_tile_shapes(ctx, orc, shape, n):
     """Every LSD tile shape (option sort.shape), with and without 16-byte LDS records (sort.pack),
     with and without the pass that merges a key word's high half and its row id into one word
-    (sort.merge), gives the oracle's permutation, on sizes around both tile lengths: the row id carried
+    (sort.merge), compiled for one or two workgroups per CU (sort.occupancy), gives the oracle's
+    permutation, on sizes around both tile lengths: the row id carried
     beside one key word (INT + FLOAT) and beside two (LONG + FLOAT + INT), and inside two key words
     through the pass that drops word 0 (LONG + FLOAT)."""
     rng = np.random.default_rng(shape * 1000 + n % 997)
@@ -134,16 +135,18 @@ def test_sort_tile_shapes(ctx, orc, shape, n):
     i32 = rng.integers(-5, 5, n).astype(np.int32)
     ctx.set_option("sort.shape", shape)
     try:
-        for pack, merge in ((1, 1), (0, 1), (1, 0)):
+        for pack, merge, occ in ((0, 1, 2), (1, 1, 2), (0, 1, 1), (0, 0, 2)):
             ctx.set_option("sort.pack", pack)
             ctx.set_option("sort.merge", merge)
+            ctx.set_option("sort.occupancy", occ)
             _check(ctx, orc, [i32, f], [0, 1], ["+", "-"])
             _check(ctx, orc, [k, f], [0, 1], ["-", "+"])
             _check(ctx, orc, [k, f, i32], [0, 1, 2], ["+", "+", "-"])
     finally:
         ctx.set_option("sort.shape", 1)
-        ctx.set_option("sort.pack", 1)
+        ctx.set_option("sort.pack", 0)
         ctx.set_option("sort.merge", 1)
+        ctx.set_option("sort.occupancy", 2)
 
 
 def test_sort_info_counts_digit_planes(ctx, orc):
