@@ -539,8 +539,11 @@ __global__ void k_groupby_finalize(KeySpec K, GroupTableDev T, OutSpec O, unsign
 // pattern is aggregated in the small global table T instead, emitted afterwards.
 // sum_nocount: the carried value column holds no NULL and no exact double SUM is an AVG: every
 // group then has a non-NULL value, and a SUM's non-NULL count word is neither updated nor read
-// (one LDS atomic a row less: the LDS atomics bound this kernel).
-template <int VAL>
+// (one LDS atomic a row less).
+// NA (1..4): the number of aggregates, fixed when the kernel is compiled, so that every loop over
+// them unrolls and their descriptors stay in scalar registers (the per-row descriptor reloads and
+// switches were most of the kernel's instructions); 0: T.naggs at run time.
+template <int VAL, int NA>
 __global__ __launch_bounds__(kGbPartThreads) void k_gb_part_emit(KeySpec K, GroupTableDev T, const uint64_t *starts,
                                                                  const uint64_t *hs, const uint64_t *vs,
                                                                  const uint32_t *ids, uint32_t lcap, uint32_t nparts,
@@ -557,16 +560,24 @@ __global__ __launch_bounds__(kGbPartThreads) void k_gb_part_emit(KeySpec K, Grou
   __shared__ unsigned long long base;
   const uint32_t lstride = lcap + 1;
   const uint32_t limit = lcap - lcap / 8;
+  auto for_aggs = [&](auto &&f) {  // f(a) for every aggregate a
+    if constexpr (NA > 0) {
+#pragma unroll
+      for (int a = 0; a < NA; ++a) f(a);
+    } else {
+      for (int a = 0; a < T.naggs; ++a) f(a);
+    }
+  };
   for (uint32_t part = blockIdx.x; part < nparts; part += gridDim.x) {
     for (uint32_t s = threadIdx.x; s < lstride; s += blockDim.x) lds[s] = kEmptyKey;
-    for (int a = 0; a < T.naggs; ++a) {
+    for_aggs([&](int a) {
       const AggSpec &A = T.aggs[a];
       for (int w = 0; w < agg_words(A.kind); ++w) {
         const uint64_t v = init_word(A.kind, w);
         uint64_t *p = lds + (size_t)(1 + A.word0 + w) * lstride;
         for (uint32_t s = threadIdx.x; s < lstride; s += blockDim.x) p[s] = v;
       }
-    }
+    });
     if (threadIdx.x == 0) {
       used = 0;
       full = 0;
@@ -597,11 +608,11 @@ __global__ __launch_bounds__(kGbPartThreads) void k_gb_part_emit(KeySpec K, Grou
         if (h == kEmptyKey) {  // the one key whose hash is the EMPTY pattern: the small global table
           const int64_t g = gt_find_insert(T, unmix64(h));
           if (g < 0) continue;
-          for (int a = 0; a < T.naggs; ++a) {
+          for_aggs([&](int a) {
             const AggSpec &A = T.aggs[a];
             const AggVal v = VAL == 2 ? agg_value(A, r) : agg_value_bits(A, vbits);
             acc_apply(A, [&](int j) { return acc_word(T, A.word0 + j, (uint64_t)g); }, v);
-          }
+          });
           continue;
         }
         // slot: the low 32 bits of h scaled to [0, lcap), then linear probing
@@ -631,22 +642,22 @@ __global__ __launch_bounds__(kGbPartThreads) void k_gb_part_emit(KeySpec K, Grou
           full = 1;
           break;
         }
-        for (int a = 0; a < T.naggs; ++a) {
+        for_aggs([&](int a) {
           const AggSpec &A = T.aggs[a];
           const AggVal v = VAL == 2 ? agg_value(A, r) : agg_value_bits(A, vbits);
           auto word = [&](int j) { return lds + (size_t)(1 + A.word0 + j) * lstride + slot; };
           if (sum_nocount && A.kind == AK_SUMD) {
-            if (!v.valid) continue;
+            if (!v.valid) return;
             uint64_t lo = v.a, hi = v.b;
             if (v.neg) {
               lo = ~lo + 1ull;
               hi = ~hi + (lo == 0 ? 1ull : 0ull);
             }
             add128(word(0), word(1), lo, hi);
-            continue;
+            return;
           }
           acc_apply(A, word, v);
-        }
+        });
       }
     }
     __syncthreads();
@@ -668,12 +679,12 @@ __global__ __launch_bounds__(kGbPartThreads) void k_gb_part_emit(KeySpec K, Grou
       const uint64_t idx = base + wbase + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
       if (idx >= cap) continue;
       emit_key(K, O, unmix64(h), idx);
-      for (int a = 0; a < T.naggs; ++a) {
+      for_aggs([&](int a) {
         const AggSpec &A = T.aggs[a];
         const bool one = sum_nocount && A.kind == AK_SUMD;  // the count word reads as non-zero
         finalize_words(A, [&](int j) { return one && j == 2 ? 1ull : lds[(size_t)(1 + A.word0 + j) * lstride + s]; },
                        O.col[K.nkeys + a], idx);
-      }
+      });
     }
     __syncthreads();  // the table is reset for the next partition only after every slot was read
   }
@@ -1263,26 +1274,31 @@ int groupby_partitioned(ghive_ctx *ctx, const KeySpec &K, GroupTableDev &Tspec, 
   for (int a = 0; a < T.naggs; ++a)
     if (T.aggs[a].kind == AK_SUMD && T.aggs[a].is_avg) sum_nocount = 0;
   const uint64_t *vs = vs_part;
-  switch (val) {
-    case 0:
-      (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_gb_part_emit<0>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-      k_gb_part_emit<0><<<egrid, kGbPartThreads, lds_bytes, ctx->stream>>>(K, T, starts.as<uint64_t>(), ks, vs, ids, lcap, P,
-                                                                           fail, count, O, capacity, sum_nocount);
-      break;
-    case 1:
-      (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_gb_part_emit<1>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-      k_gb_part_emit<1><<<egrid, kGbPartThreads, lds_bytes, ctx->stream>>>(K, T, starts.as<uint64_t>(), ks, vs, ids, lcap, P,
-                                                                           fail, count, O, capacity, sum_nocount);
-      break;
-    default:
-      (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_gb_part_emit<2>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
-      k_gb_part_emit<2><<<egrid, kGbPartThreads, lds_bytes, ctx->stream>>>(K, T, starts.as<uint64_t>(), ks, vs, ids, lcap, P,
-                                                                           fail, count, O, capacity, sum_nocount);
-      break;
+  // VAL x the aggregate count (1..4 compiled, 0 = any)
+  const int na = T.naggs >= 1 && T.naggs <= 4 ? T.naggs : 0;
+#define GB_EMIT(V, N)                                                                                           \
+  do {                                                                                                          \
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_gb_part_emit<V, N>),                             \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);                       \
+    k_gb_part_emit<V, N><<<egrid, kGbPartThreads, lds_bytes, ctx->stream>>>(K, T, starts.as<uint64_t>(), ks, vs, ids, \
+                                                                            lcap, P, fail, count, O, capacity,    \
+                                                                            sum_nocount);                         \
+  } while (0)
+#define GB_EMIT_N(V)        \
+  switch (na) {             \
+    case 1: GB_EMIT(V, 1); break; \
+    case 2: GB_EMIT(V, 2); break; \
+    case 3: GB_EMIT(V, 3); break; \
+    case 4: GB_EMIT(V, 4); break; \
+    default: GB_EMIT(V, 0); break; \
   }
+  switch (val) {
+    case 0: GB_EMIT_N(0); break;
+    case 1: GB_EMIT_N(1); break;
+    default: GB_EMIT_N(2); break;
+  }
+#undef GB_EMIT_N
+#undef GB_EMIT
   GH_LAUNCH(ctx);
   uint64_t f = 0;
   GH_TRY(d2h(ctx, &f, fail, sizeof f));
