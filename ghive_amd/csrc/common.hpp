@@ -67,6 +67,7 @@ struct ghive_tuning {
   int64_t gb_part_threads = 512;        // workgroup size of the group-by partition passes (256, 512, 1024)
   int64_t gb_part_occupancy = 2;        // workgroups per CU of its 512-thread 8-bit passes (1 or 2)
   int64_t sort_shape = 1;               // LSD tile: 0 256 x 32, 1 512 x 16, 2 512 x 32, 3 1024 x 8
+  int64_t hash_regional_rows = 1 << 22;  // builds of at least this many rows take the regional form (0: never)
   int64_t sort_pack = 0;                // 1: key word + id (and the word-0-dropping pass) as 16-byte LDS records
   int64_t sort_merge = 1;               // 1: key word + id become one word once the low half's digits are done
   int64_t sort_occupancy = 2;           // workgroups per CU of the 8-byte-staging 512 x 16 passes (1 or 2)
@@ -174,6 +175,11 @@ class DevBuf {
     void *p = p_;
     p_ = nullptr;
     return p;
+  }
+  void adopt(ghive_ctx *ctx, void *p) {  // takes over a block from ctx's arena
+    reset();
+    ctx_ = ctx;
+    p_ = p;
   }
   template <class T>
   T *as() const { return static_cast<T *>(p_); }

@@ -372,6 +372,9 @@ int ghive_set_option(ghive_ctx *ctx, const char *name, int64_t value) {
     ctx->tune.sort_shape = value;
   } else if (k == "sort.pack") {
     ctx->tune.sort_pack = value ? 1 : 0;
+  } else if (k == "hash.regional_rows") {
+    if (value < 0) return set_error(ctx, GHIVE_ERR_INVALID, "option %s: %lld (>= 0)", name, (long long)value);
+    ctx->tune.hash_regional_rows = value;
   } else if (k == "sort.merge") {
     ctx->tune.sort_merge = value ? 1 : 0;
   } else if (k == "sort.occupancy") {
@@ -407,6 +410,7 @@ int ghive_get_option(ghive_ctx *ctx, const char *name, int64_t *value) {
   else if (k == "sort.shape") *value = ctx->tune.sort_shape;
   else if (k == "sort.pack") *value = ctx->tune.sort_pack;
   else if (k == "sort.merge") *value = ctx->tune.sort_merge;
+  else if (k == "hash.regional_rows") *value = ctx->tune.hash_regional_rows;
   else if (k == "sort.occupancy") *value = ctx->tune.sort_occupancy;
   else if (k == "groupby.part_threads") *value = ctx->tune.gb_part_threads;
   else if (k == "groupby.part_occupancy") *value = ctx->tune.gb_part_occupancy;

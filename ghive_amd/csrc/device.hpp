@@ -283,12 +283,25 @@ __host__ __device__ __forceinline__ uint64_t zipf_try(const ZipfParams &p, doubl
 }
 
 // ---- hashing -------------------------------------------------------------------------------
-// Fibonacci hashing: multiply by 2^64/phi and keep the top bits (Knuth, TAOCP 6.4).
+// Fibonacci hashing: multiply by 2^64/phi and keep the top bits (Knuth, TAOCP 6.4). The
+// multipliers are odd, so the full product determines the key (kFibInv*: their inverses modulo
+// 2^64 / 2^32, by Newton's iteration x <- x (2 - a x), which doubles the correct low bits).
+constexpr uint64_t kFibMul64 = 0x9E3779B97F4A7C15ull;
+constexpr uint32_t kFibMul32 = 0x9E3779B1u;
+constexpr uint64_t inverse_odd64(uint64_t a) {
+  uint64_t x = a;  // correct to 3 bits for any odd a
+  for (int i = 0; i < 6; ++i) x *= 2 - a * x;
+  return x;
+}
+constexpr uint64_t kFibInv64 = inverse_odd64(kFibMul64);
+constexpr uint32_t kFibInv32 = (uint32_t)inverse_odd64(kFibMul32);
+static_assert(kFibMul64 * kFibInv64 == 1ull, "inverse modulo 2^64");
+static_assert((uint32_t)(kFibMul32 * kFibInv32) == 1u, "inverse modulo 2^32");
 __device__ __forceinline__ uint64_t hash_u64(uint64_t k, int log2_slots) {
-  return (k * 0x9E3779B97F4A7C15ull) >> (64 - log2_slots);
+  return (k * kFibMul64) >> (64 - log2_slots);
 }
 __device__ __forceinline__ uint32_t hash_u32(uint32_t k, int log2_slots) {
-  return (uint32_t)(((uint64_t)(k * 0x9E3779B1u)) >> (32 - log2_slots));
+  return (uint32_t)(((uint64_t)(k * kFibMul32)) >> (32 - log2_slots));
 }
 
 }  // namespace ghive

@@ -7,6 +7,7 @@
 
 #include "common.hpp"
 #include "device.hpp"
+#include "groupby.hpp"
 #include "hash.hpp"
 #include "scan.hpp"
 
@@ -35,6 +36,39 @@ __device__ __forceinline__ long long wave_max_i64(long long v) {
 // [2] key min, [3] key max, [4] payload min, [5] payload max (k_build_stats combines them; no
 // atomics on shared counters, which serialise across the XCDs).
 constexpr int kStatWords = 6;
+
+// One workgroup's statistics from its lanes' values into part[blockIdx.x * kStatWords ..]; every
+// thread of the workgroup calls it (it holds a barrier).
+template <int TH>
+__device__ void stat_partial(unsigned long long *part, uint32_t inserted, uint32_t dup, long long kmin, long long kmax,
+                             long long pmin, long long pmax) {
+  kmin = wave_min_i64(kmin);
+  kmax = wave_max_i64(kmax);
+  pmin = wave_min_i64(pmin);
+  pmax = wave_max_i64(pmax);
+  const unsigned long long ins = wave_sum_u64(inserted);
+  const bool anydup = ballot(dup != 0) != 0;
+  __shared__ long long red[TH / kWave][kStatWords];
+  const int wv = threadIdx.x / kWave;
+  if (lane_id() == 0) {
+    red[wv][0] = (long long)ins;
+    red[wv][1] = anydup;
+    red[wv][2] = kmin;
+    red[wv][3] = kmax;
+    red[wv][4] = pmin;
+    red[wv][5] = pmax;
+  }
+  __syncthreads();
+  if (threadIdx.x < kStatWords) {
+    const int w = threadIdx.x;
+    long long v = red[0][w];
+    for (int j = 1; j < TH / kWave; ++j) {
+      const long long o = red[j][w];
+      v = w < 2 ? v + o : ((w & 1) == 0 ? (o < v ? o : v) : (o > v ? o : v));  // sums, mins, maxes
+    }
+    part[(uint64_t)blockIdx.x * kStatWords + w] = (unsigned long long)v;
+  }
+}
 // n_dev (optional): the row count on the device (a filter's count), at most n.
 __global__ __launch_bounds__(kHashThreads) void k_hash_build(
     const void *keys, int key_type, const uint32_t *sel, uint64_t n, const uint64_t *n_dev, const void *pay,
@@ -81,32 +115,7 @@ __global__ __launch_bounds__(kHashThreads) void k_hash_build(
     pmin = p < pmin ? p : pmin;
     pmax = p > pmax ? p : pmax;
   }
-  kmin = wave_min_i64(kmin);
-  kmax = wave_max_i64(kmax);
-  pmin = wave_min_i64(pmin);
-  pmax = wave_max_i64(pmax);
-  const unsigned long long ins = wave_sum_u64(inserted);
-  const bool anydup = ballot(dup != 0) != 0;
-  __shared__ long long red[kHashThreads / kWave][kStatWords];
-  const int wv = threadIdx.x / kWave;
-  if (lane_id() == 0) {
-    red[wv][0] = (long long)ins;
-    red[wv][1] = anydup;
-    red[wv][2] = kmin;
-    red[wv][3] = kmax;
-    red[wv][4] = pmin;
-    red[wv][5] = pmax;
-  }
-  __syncthreads();
-  if (threadIdx.x < kStatWords) {
-    const int w = threadIdx.x;
-    long long v = red[0][w];
-    for (int j = 1; j < (int)(blockDim.x / kWave); ++j) {
-      const long long o = red[j][w];
-      v = w < 2 ? v + o : ((w & 1) == 0 ? (o < v ? o : v) : (o > v ? o : v));  // sums, mins, maxes
-    }
-    part[(uint64_t)blockIdx.x * kStatWords + w] = (unsigned long long)v;
-  }
+  stat_partial<kHashThreads>(part, inserted, dup, kmin, kmax, pmin, pmax);
 }
 
 // Combines the per-workgroup build statistics into stats[0..5] (one workgroup).
@@ -142,6 +151,177 @@ __global__ __launch_bounds__(kHashThreads) void k_build_stats(const unsigned lon
     }
     stats[w] = (unsigned long long)r;
   }
+}
+
+// ---- regional build: large tables without device-scope atomics ---------------------------------
+// A key's home slot is the top log2 bits of its Fibonacci product w (k * kFibMul64, or for L8 the
+// 32-bit product in w's high half), and the product determines the key. Rows sorted by the top
+// bits of w therefore arrive grouped by table region (2^kRegionLog2 consecutive slots). One
+// workgroup builds each region in LDS (linear probing inside the region with LDS compare-and-swap)
+// and writes it out whole, so the table needs no fill pass and no insert reaches past the XCD's L2;
+// device-scope atomics on this GPU execute at the memory side, and a per-key global CAS loop moved
+// a 128-byte line per insert. A key whose probe runs past its region's last slot goes to an
+// overflow list and is inserted afterwards by the ordinary global loop from its home slot: every
+// slot from there to the region's end is occupied, so its probe sequence is the one a sequential
+// build would give it.
+constexpr int kRegionLog2 = 13;
+constexpr int kRegionThreads = 512;
+
+// W[i] = row i's payload (the build row id without a payload column), W[n + i] = its product
+// (NULL keys: the product of the empty marker, which the region build skips); statistics partials
+// as k_hash_build writes them (no duplicate flag: the region build finds duplicates).
+__global__ __launch_bounds__(kHashThreads) void k_build_words(const void *keys, int key_type, const uint32_t *sel,
+                                                             uint64_t n, const void *pay, int pay_type, int layout,
+                                                             uint64_t *W, unsigned long long *part) {
+  long long kmin = LLONG_MAX, kmax = LLONG_MIN, pmin = LLONG_MAX, pmax = LLONG_MIN;
+  uint32_t inserted = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint64_t r = sel ? sel[i] : i;
+    int64_t k;
+    const bool valid = col_load_i64(keys, key_type, r, k);
+    int64_t p;
+    if (pay) {
+      if (!col_load_i64(pay, pay_type, r, p)) p = pay_type == GHIVE_INT ? 0x7fffffffll : 0x7fffffffffffffffll;
+    } else {
+      p = (int64_t)r;
+    }
+    if (!valid) k = layout == 8 ? (int64_t)0x7fffffff : (int64_t)kEmpty16;
+    W[i] = (uint64_t)p;
+    W[n + i] = layout == 8 ? (uint64_t)((uint32_t)k * kFibMul32) << 32 : (uint64_t)k * kFibMul64;
+    if (!valid) continue;
+    ++inserted;
+    kmin = k < kmin ? k : kmin;
+    kmax = k > kmax ? k : kmax;
+    pmin = p < pmin ? p : pmin;
+    pmax = p > pmax ? p : pmax;
+  }
+  stat_partial<kHashThreads>(part, inserted, 0, kmin, kmax, pmin, pmax);
+}
+
+// starts[r] = the first row whose product's top rbits are >= r, for r in [0, 2^rbits] (one binary
+// search per region over the sorted products).
+__global__ void k_region_starts(const uint64_t *w, uint64_t n, int rbits, uint64_t *starts) {
+  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t nr = 1ull << rbits;
+  if (r > nr) return;
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if ((w[mid] >> (64 - rbits)) < r) lo = mid + 1;
+    else hi = mid;
+  }
+  starts[r] = lo;
+}
+
+// The key of product w (decoded by the multiplier's inverse).
+template <int L>
+__device__ __forceinline__ int64_t region_key(uint64_t w) {
+  if constexpr (L == 16) return (int64_t)(w * kFibInv64);
+  else return (int64_t)(int32_t)((uint32_t)(w >> 32) * kFibInv32);
+}
+
+// Regions blockIdx.x, + gridDim.x, ... of a table of 2^log2 slots from the sorted rows (W: payloads,
+// W + n: products; starts from k_region_starts). ov (2 x ov_cap words: products, then payloads)
+// takes the rows that run past their region, *ov_count their number.
+template <int L>
+__global__ __launch_bounds__(kRegionThreads) void k_region_build(const uint64_t *W, uint64_t n, const uint64_t *starts,
+                                                                 uint32_t nregions, int log2, uint64_t *table,
+                                                                 uint64_t *ov, unsigned long long *ov_count,
+                                                                 uint64_t ov_cap, unsigned long long *part) {
+  constexpr uint32_t kSlots = 1u << kRegionLog2;
+  __shared__ __attribute__((aligned(16))) uint64_t tk[kSlots];  // L16 keys; L8 slot words
+  __shared__ __attribute__((aligned(16))) uint64_t tp[L == 16 ? kSlots : 1];  // L16 payloads
+  constexpr uint64_t kEmpty = L == 16 ? kEmpty16 : kEmpty8;
+  uint32_t dup = 0;
+  for (uint32_t reg = blockIdx.x; reg < nregions; reg += gridDim.x) {
+    for (uint32_t s = threadIdx.x; s < kSlots; s += kRegionThreads) {
+      tk[s] = kEmpty;
+      if constexpr (L == 16) tp[s] = kEmpty16;  // an empty slot's payload word, as the fill pass leaves it
+    }
+    __syncthreads();
+    const uint64_t b = starts[reg], e = starts[reg + 1];
+    const uint64_t base = (uint64_t)reg << kRegionLog2;
+    for (uint64_t i = b + threadIdx.x; i < e; i += kRegionThreads) {
+      const uint64_t w = W[n + i], p = W[i];
+      const int64_t k = region_key<L>(w);
+      if (L == 16 ? (uint64_t)k == kEmpty16 : (int32_t)k == 0x7fffffff) continue;  // a NULL key: never joins
+      uint32_t s = (uint32_t)((w >> (64 - log2)) - base);  // home slot inside the region
+      bool placed = false;
+      for (; s < kSlots; ++s) {
+        if constexpr (L == 16) {
+          const unsigned long long old = atomicCAS((unsigned long long *)&tk[s], kEmpty16, (unsigned long long)k);
+          if (old == kEmpty16) {
+            tp[s] = p;
+            placed = true;
+            break;
+          }
+          if ((int64_t)old == k) dup = 1;
+        } else {
+          const uint64_t word = (uint64_t)(uint32_t)(int32_t)k | ((uint64_t)(uint32_t)p << 32);
+          const unsigned long long old = atomicCAS((unsigned long long *)&tk[s], kEmpty8, word);
+          if (old == kEmpty8) {
+            placed = true;
+            break;
+          }
+          if ((int32_t)(uint32_t)old == (int32_t)k) dup = 1;
+        }
+      }
+      if (!placed) {
+        const unsigned long long o = atomicAdd(ov_count, 1ull);
+        if (o < ov_cap) {
+          ov[o] = w;
+          ov[ov_cap + o] = p;
+        }
+      }
+    }
+    __syncthreads();
+    if constexpr (L == 16) {  // {key, payload} pairs as 16-byte stores
+      typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+      u64x2 *out = reinterpret_cast<u64x2 *>(table) + base;
+      for (uint32_t s = threadIdx.x; s < kSlots; s += kRegionThreads) out[s] = u64x2{tk[s], tp[s]};
+    } else {
+      for (uint32_t s = threadIdx.x; s < kSlots; s += kRegionThreads) table[base + s] = tk[s];
+    }
+    __syncthreads();  // the region is written out before the next one reuses the LDS table
+  }
+  stat_partial<kRegionThreads>(part, 0, dup, LLONG_MAX, LLONG_MIN, LLONG_MAX, LLONG_MIN);
+}
+
+// The overflow rows of the region build, inserted by the global loop from their home slots.
+__global__ __launch_bounds__(kHashThreads) void k_overflow_insert(const uint64_t *ov, uint64_t ov_cap,
+                                                                 const unsigned long long *ov_count, int layout,
+                                                                 uint64_t *slots, uint64_t mask, int log2,
+                                                                 unsigned long long *part) {
+  const uint64_t cnt = *ov_count < ov_cap ? *ov_count : ov_cap;
+  uint32_t dup = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += stride) {
+    const uint64_t w = ov[i], p = ov[ov_cap + i];
+    uint64_t s = w >> (64 - log2);
+    if (layout == 8) {
+      const int32_t k = (int32_t)region_key<8>(w);
+      const uint64_t word = (uint64_t)(uint32_t)k | ((uint64_t)(uint32_t)p << 32);
+      for (uint64_t step = 0; step <= mask; ++step) {
+        const unsigned long long old = atomicCAS((unsigned long long *)&slots[s], kEmpty8, word);
+        if (old == kEmpty8) break;
+        if ((int32_t)(uint32_t)old == k) dup = 1;
+        s = (s + 1) & mask;
+      }
+    } else {
+      const int64_t k = region_key<16>(w);
+      for (uint64_t step = 0; step <= mask; ++step) {
+        const unsigned long long old = atomicCAS((unsigned long long *)&slots[2 * s], kEmpty16, (unsigned long long)k);
+        if (old == kEmpty16) {
+          slots[2 * s + 1] = p;
+          break;
+        }
+        if ((int64_t)old == k) dup = 1;
+        s = (s + 1) & mask;
+      }
+    }
+  }
+  stat_partial<kHashThreads>(part, 0, dup, LLONG_MAX, LLONG_MIN, LLONG_MAX, LLONG_MIN);
 }
 
 // mode 0 inner: every match; 1 left semi: the first match only; 2 left outer: every match, or
@@ -503,6 +683,43 @@ static int finish_build(ghive_ctx *ctx, ghive_hash_table *h, const uint64_t (&ho
 
 // Rows sel[0 .. n) (n_dev: the count on the device, at most n). Synchronous unless `async`:
 // then the statistics stay on the device and the table is finished on its first use.
+// The regional build's launches (scratch allocated by the caller: W0 and W1 2n words each, starts
+// 2^rbits + 1 words, one counter word, part (g1 + g2 + g3 + 1) x kStatWords words): products and
+// payloads, the stable passes over the products' top bytes, region starts, regions, overflow rows,
+// then the combined statistics at stats.
+static int launch_regional(ghive_ctx *ctx, ghive_hash_table *h, const ghive_column *key, const uint32_t *sel,
+                           uint64_t n, const void *pay, int pay_type, uint64_t *W0, uint64_t *W1, uint64_t *starts,
+                           unsigned long long *ov_count, unsigned long long *part, uint32_t g1, uint32_t g2,
+                           uint32_t g3, unsigned long long *stats) {
+  const int rbits = h->log2_slots - kRegionLog2;
+  const uint32_t nregions = 1u << rbits;
+  k_build_words<<<g1, kHashThreads, 0, ctx->stream>>>(key->data, key->type, sel, n, pay, pay_type, h->layout, W0, part);
+  GH_LAUNCH(ctx);
+  std::vector<int> digits;  // the bytes of word 1 (the products) that hold their top rbits
+  for (int b = (64 - rbits) / 8; b < 8; ++b) digits.push_back(8 + b);
+  uint64_t *words = nullptr;
+  uint32_t *unused = nullptr;
+  GH_TRY(radix_sort_u64x2_ids(ctx, n, W0, W1, nullptr, nullptr, digits, &words, &unused, false));
+  uint64_t *ov = words == W0 ? W1 : W0;  // the other buffer: 2n words for overflow rows
+  GH_TRY(fill_u64(ctx, reinterpret_cast<uint64_t *>(ov_count), 0, 1));
+  k_region_starts<<<(nregions + 1 + 255) / 256, 256, 0, ctx->stream>>>(words + n, n, rbits, starts);
+  GH_LAUNCH(ctx);
+  uint64_t *table = static_cast<uint64_t *>(h->data);
+  if (h->layout == 16)
+    k_region_build<16><<<g2, kRegionThreads, 0, ctx->stream>>>(words, n, starts, nregions, h->log2_slots, table, ov,
+                                                                ov_count, n, part + (uint64_t)g1 * kStatWords);
+  else
+    k_region_build<8><<<g2, kRegionThreads, 0, ctx->stream>>>(words, n, starts, nregions, h->log2_slots, table, ov,
+                                                               ov_count, n, part + (uint64_t)g1 * kStatWords);
+  GH_LAUNCH(ctx);
+  k_overflow_insert<<<g3, kHashThreads, 0, ctx->stream>>>(ov, n, ov_count, h->layout, table, h->slots - 1,
+                                                         h->log2_slots, part + (uint64_t)(g1 + g2) * kStatWords);
+  GH_LAUNCH(ctx);
+  k_build_stats<<<1, kHashThreads, 0, ctx->stream>>>(part, g1 + g2 + g3, stats);
+  GH_LAUNCH(ctx);
+  return GHIVE_OK;
+}
+
 static int hash_build_core(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel, uint64_t n,
                            const uint64_t *n_dev, const ghive_column *payload, float load_factor, bool async,
                            ghive_hash_table **out) {
@@ -537,10 +754,31 @@ static int hash_build_core(ghive_ctx *ctx, const ghive_column *key, const uint32
     return set_error(ctx, GHIVE_ERR_OOM, "hash_build: cannot allocate %llu slots", (unsigned long long)h->slots);
   }
   int st = GHIVE_OK;
-  if (h->layout == 8) {
+  // A large synchronous build runs in the regional form (every slot written by its region, no fill
+  // pass); its scratch lives until the statistics are read back. Too little memory for the scratch:
+  // the global form.
+  DevBuf rw0, rw1, rstarts, rcount;
+  bool regional = !async && !n_dev && ctx->tune.hash_regional_rows > 0 && n >= (uint64_t)ctx->tune.hash_regional_rows &&
+                  h->log2_slots >= kRegionLog2 && h->log2_slots <= (h->layout == 8 ? 32 : 48);
+  if (regional) {
+    const uint64_t nregions = 1ull << (h->log2_slots - kRegionLog2);
+    void *a = ctx->arena->alloc(16 * n), *b = ctx->arena->alloc(16 * n), *c = ctx->arena->alloc(8 * (nregions + 1)),
+         *d = ctx->arena->alloc(8);
+    regional = a && b && c && d;
+    for (void *p : {a, b, c, d})
+      if (p && !regional) ctx->arena->free(p);
+    if (regional) {
+      rw0.adopt(ctx, a);
+      rw1.adopt(ctx, b);
+      rstarts.adopt(ctx, c);
+      rcount.adopt(ctx, d);
+    }
+  }
+  if (regional) {
+  } else if (h->layout == 8) {
     st = fill_u64(ctx, static_cast<uint64_t *>(h->data), kEmpty8, h->slots);
   } else {
-    // {key = LONG_MAX, payload = 0}: fill keys and payloads as one u64 array then fix payloads
+    // {key = LONG_MAX, payload = LONG_MAX}: keys and payloads filled as one u64 array
     st = fill_u64(ctx, static_cast<uint64_t *>(h->data), kEmpty16, words);
   }
   const void *pay = payload ? payload->data : nullptr;
@@ -550,10 +788,27 @@ static int hash_build_core(ghive_ctx *ctx, const ghive_column *key, const uint32
                                (uint64_t)LLONG_MIN};
   if (st == GHIVE_OK && n) {
     const uint32_t grid = grid_for(ctx, n);
+    // regional form: partials of the product pass, the region workgroups and the overflow pass
+    uint32_t g2 = 0, g3 = 0;
+    if (regional) {
+      int per_cu = 0;
+      const void *kfn = h->layout == 16 ? reinterpret_cast<const void *>(k_region_build<16>)
+                                        : reinterpret_cast<const void *>(k_region_build<8>);
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kRegionThreads, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+      const uint64_t nregions = 1ull << (h->log2_slots - kRegionLog2), slots = (uint64_t)ctx->cus * (uint32_t)per_cu;
+      g2 = (uint32_t)(nregions < slots ? nregions : slots);
+      g3 = (uint32_t)ctx->cus * 2;
+    }
+    const uint32_t nparts = grid + g2 + g3;
     DevBuf part;
-    st = part.alloc(ctx, ((uint64_t)grid + 1) * kStatWords * sizeof(uint64_t));
-    unsigned long long *stats = st == GHIVE_OK ? part.as<unsigned long long>() + (uint64_t)grid * kStatWords : nullptr;
-    if (st == GHIVE_OK) {
+    st = part.alloc(ctx, ((uint64_t)nparts + 1) * kStatWords * sizeof(uint64_t));
+    unsigned long long *stats = st == GHIVE_OK ? part.as<unsigned long long>() + (uint64_t)nparts * kStatWords : nullptr;
+    if (st == GHIVE_OK && regional) {
+      st = launch_regional(ctx, h, key, sel, n, pay, pay_type, rw0.as<uint64_t>(), rw1.as<uint64_t>(),
+                           rstarts.as<uint64_t>(), rcount.as<unsigned long long>(), part.as<unsigned long long>(),
+                           grid, g2, g3, stats);
+    } else if (st == GHIVE_OK) {
       k_hash_build<<<grid, kHashThreads, 0, ctx->stream>>>(key->data, key->type, sel, n, n_dev, pay, pay_type, h->layout,
                                                            static_cast<uint64_t *>(h->data), h->slots - 1, h->log2_slots,
                                                            part.as<unsigned long long>());

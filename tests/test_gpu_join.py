@@ -196,3 +196,103 @@ def test_generate_result_types_and_maintained_columns(ctx, orc):
         for g, x in zip(got, want):
             assert g.dtype == x.dtype
             assert np.array_equal(g, x, equal_nan=g.dtype.kind == "f")
+
+
+@pytest.fixture
+def regional(ctx):
+    """Every build in the test takes the regional form (option hash.regional_rows 1)."""
+    ctx.set_option("hash.regional_rows", 1)
+    yield
+    ctx.set_option("hash.regional_rows", 1 << 22)
+
+
+@pytest.mark.parametrize("ktype", ["INT", "LONG"])
+@pytest.mark.parametrize("payload", [None, "INT", "LONG"])
+def test_regional_build_vs_oracle(ctx, orc, regional, ktype, payload):
+    """The regional build (rows sorted by table region, each region of 8192 slots built in LDS):
+    duplicate keys, NULL keys, every key and payload type; 65536 slots = 8 regions. The probe's
+    (row, payload) multiset equals the oracle's and the statistics match the rows."""
+    from ghive_amd import ops
+    rng = np.random.default_rng(17)
+    dt = np.int32 if ktype == "INT" else np.int64
+    null = INT_NULL if ktype == "INT" else LONG_NULL
+    bk = rng.integers(-3000, 3000, 30000).astype(dt)
+    bk[::50] = null
+    pk = rng.integers(-4000, 4000, 200000).astype(dt)
+    pk[::31] = null
+    pay = None
+    if payload == "INT":
+        pay = rng.integers(-2**31, 2**31 - 1, 30000).astype(np.int32)
+    elif payload == "LONG":
+        pay = rng.integers(-2**62, 2**62, 30000).astype(np.int64)
+    ht = ops.HashTable(ctx, to_dev(bk), payload=None if pay is None else to_dev(pay))
+    info = ht.info()
+    live = bk[bk != null]
+    assert not info["unique"] and info["entries"] == len(live) and info["slots"] == 65536
+    rows, vals = ht.probe(to_dev(pk))
+    want_rows, want_vals = orc.mapjoin_inner(bk, pk, payload=pay)
+    assert _pairs(to_np(rows), to_np(vals)) == _pairs(want_rows, want_vals)
+    ht.free()
+
+
+def test_regional_build_unique_with_selections(ctx, orc, regional):
+    """Unique INT keys through a build selection at load factors 0.25, 0.5 and 0.9 (the last fills
+    regions enough that clusters run past a region's end into the overflow pass)."""
+    from ghive_amd import ops
+    rng = np.random.default_rng(5)
+    bk = (rng.permutation(900000)[:400000] * 3 + 1).astype(np.int32)
+    pk = rng.integers(0, 2700000, 1_000_000).astype(np.int32)
+    bsel = np.nonzero(rng.random(len(bk)) < 0.7)[0].astype(np.uint32)
+    psel = np.nonzero(rng.random(len(pk)) < 0.6)[0].astype(np.uint32)
+    for lf in (0.25, 0.5, 0.9):
+        ht = ops.HashTable(ctx, to_dev(bk), sel=to_dev(bsel.astype(np.int32)), load_factor=lf)
+        info = ht.info()
+        assert info["unique"] and info["entries"] == len(bsel)
+        rows, vals = ht.probe(to_dev(pk), sel=to_dev(psel.astype(np.int32)))
+        want = orc.mapjoin_inner(bk, pk, build_sel=bsel, probe_sel=psel)
+        assert _pairs(to_np(rows), to_np(vals)) == _pairs(*want)
+        ht.free()
+
+
+def test_regional_build_overflow_and_wrap(ctx):
+    """Keys made to share home slots at the end of region 3 and at the end of the last region (the
+    table's last slots): their clusters run past the region, into the overflow pass, and past the
+    table's end to slot 0. 65536 slots (20000 keys at load factor 0.5). Every key is found with its
+    payload, absent keys are not, and the regional and global forms give the same probe output and
+    statistics."""
+    from ghive_amd import ops
+    M = 1 << 64
+    C = 0x9E3779B97F4A7C15
+    inv = pow(C, -1, M)
+    log2 = 16
+    rng = np.random.default_rng(23)
+
+    def key_at(slot):  # a key whose home slot is `slot`: its product's top log2 bits
+        w = (slot << (64 - log2)) | int(rng.integers(0, 1 << (64 - log2)))
+        k = (w * inv) % M
+        return k - M if k >= 1 << 63 else k
+
+    crafted = [key_at(4 * 8192 - 1 - int(rng.integers(0, 8))) for _ in range(300)]
+    crafted += [key_at((1 << log2) - 1 - int(rng.integers(0, 8))) for _ in range(300)]
+    rest = rng.integers(-2**62, 2**62, 20000 - len(crafted)).astype(np.int64)
+    bk = np.unique(np.concatenate([np.array(crafted, dtype=np.int64), rest]))
+    rng.shuffle(bk)
+    pay = rng.integers(-2**62, 2**62, len(bk)).astype(np.int64)
+    absent = rng.integers(-2**62, 2**62, 5000).astype(np.int64)
+    absent = absent[~np.isin(absent, bk)]
+    pk = np.concatenate([bk, absent])
+    out = {}
+    for rows_min in (1, 0):  # regional, then global
+        ctx.set_option("hash.regional_rows", rows_min)
+        try:
+            ht = ops.HashTable(ctx, to_dev(bk), payload=to_dev(pay), load_factor=0.5)
+            info = ht.info()
+            rows, vals = ht.probe(to_dev(pk))
+            out[rows_min] = (info, _pairs(to_np(rows), to_np(vals)))
+            ht.free()
+        finally:
+            ctx.set_option("hash.regional_rows", 1 << 22)
+    info, pairs = out[1]
+    assert info["unique"] and info["entries"] == len(bk) and info["slots"] == 1 << log2
+    assert pairs == sorted(zip(range(len(bk)), pay.tolist()))
+    assert out[1] == out[0]
