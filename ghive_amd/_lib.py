@@ -55,6 +55,7 @@ SIGNATURES = {
                                    C.POINTER(P)]),
     "ghive_hash_build_async": (C.c_int, [P, C.POINTER(abi.Column), P, P, u64, C.POINTER(abi.Column),
                                          C.c_float, C.POINTER(P)]),
+    "ghive_hash_build_filtered": (C.c_int, [P, C.POINTER(abi.FilteredBuild), u32, C.POINTER(P)]),
     "ghive_hash_table_info": (C.c_int, [P, PU64, PU64, PU64, C.POINTER(i32)]),
     "ghive_hash_table_free": (None, [P, P]),
     "ghive_hash_probe": (C.c_int, [P, P, C.POINTER(abi.Column), P, u64, P, P, u64, PU64]),

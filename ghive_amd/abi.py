@@ -48,6 +48,14 @@ class Table(C.Structure):
                 ("row_num", C.c_uint64)]
 
 
+class FilteredBuild(C.Structure):
+    _fields_ = [("table", C.c_void_p), ("nodes", C.c_void_p), ("nnodes", C.c_uint32), ("root", C.c_uint32),
+                ("key_col", C.c_uint32), ("payload_col", C.c_int32), ("load_factor", C.c_float),
+                ("reserved", C.c_uint32)]
+
+
+MAX_FILTERED_BUILDS = 8
+
 PRED_STR_MAX = 64
 
 

@@ -9,6 +9,7 @@
 #include "device.hpp"
 #include "groupby.hpp"
 #include "hash.hpp"
+#include "pred_host.hpp"
 #include "scan.hpp"
 
 namespace ghive {
@@ -118,9 +119,8 @@ __global__ __launch_bounds__(kHashThreads) void k_hash_build(
   stat_partial<kHashThreads>(part, inserted, dup, kmin, kmax, pmin, pmax);
 }
 
-// Combines the per-workgroup build statistics into stats[0..5] (one workgroup).
-__global__ __launch_bounds__(kHashThreads) void k_build_stats(const unsigned long long *part, uint32_t nparts,
-                                                             unsigned long long *stats) {
+// Combines the per-workgroup build statistics into stats[0..5] (one workgroup, kHashThreads).
+__device__ void build_stats_combine(const unsigned long long *part, uint32_t nparts, unsigned long long *stats) {
   long long v[kStatWords] = {0, 0, LLONG_MAX, LLONG_MIN, LLONG_MAX, LLONG_MIN};
   for (uint32_t b = threadIdx.x; b < nparts; b += blockDim.x) {
     const unsigned long long *p = part + (uint64_t)b * kStatWords;
@@ -151,6 +151,90 @@ __global__ __launch_bounds__(kHashThreads) void k_build_stats(const unsigned lon
     }
     stats[w] = (unsigned long long)r;
   }
+}
+__global__ __launch_bounds__(kHashThreads) void k_build_stats(const unsigned long long *part, uint32_t nparts,
+                                                             unsigned long long *stats) {
+  build_stats_combine(part, nparts, stats);
+}
+
+// ---- filtered builds: a table's filter and its hash build in one pass -------------------------
+// Several tables' fills and statistics each run as one launch for all of them (blockIdx.y or
+// blockIdx.x = table), their filter-and-insert passes one launch per table (the predicate travels
+// by value).
+struct TableFill {
+  uint64_t *p;
+  uint64_t words;
+  uint64_t value;
+};
+struct TableFills {
+  TableFill t[GHIVE_MAX_FILTERED_BUILDS];
+};
+__global__ void k_fill_tables(TableFills F) {
+  const TableFill &f = F.t[blockIdx.y];
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < f.words; i += stride) f.p[i] = f.value;
+}
+
+struct TableStats {
+  const unsigned long long *part;
+  uint32_t nparts;
+  unsigned long long *stats;
+};
+struct TableStatsAll {
+  TableStats t[GHIVE_MAX_FILTERED_BUILDS];
+};
+__global__ __launch_bounds__(kHashThreads) void k_build_stats_tables(TableStatsAll S) {
+  build_stats_combine(S.t[blockIdx.x].part, S.t[blockIdx.x].nparts, S.t[blockIdx.x].stats);
+}
+
+// Rows r < rows that pass P (nnodes 0: every row) and whose key is not NULL: inserted (the loop of
+// k_hash_build), with the statistics partials of k_hash_build. No list of the rows is kept: a
+// counter shared by every wave would serialise their appends at the memory side.
+__global__ __launch_bounds__(kHashThreads) void k_filtered_build(DevPred P, const void *keys, int key_type,
+                                                                const void *pay, int pay_type, uint64_t rows,
+                                                                int layout, uint64_t *slots, uint64_t mask, int log2,
+                                                                unsigned long long *part) {
+  long long kmin = LLONG_MAX, kmax = LLONG_MIN, pmin = LLONG_MAX, pmax = LLONG_MIN;
+  uint32_t inserted = 0, dup = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += stride) {
+    int64_t k = 0;
+    if (P.nnodes && !eval_pred(P, r)) continue;
+    if (!col_load_i64(keys, key_type, r, k)) continue;  // NULL keys never join
+    int64_t p;
+    if (pay) {
+      if (!col_load_i64(pay, pay_type, r, p)) p = pay_type == GHIVE_INT ? 0x7fffffffll : 0x7fffffffffffffffll;
+    } else {
+      p = (int64_t)r;
+    }
+    if (layout == 8) {
+      const uint64_t word = (uint64_t)(uint32_t)(int32_t)k | ((uint64_t)(uint32_t)p << 32);
+      uint64_t s = hash_u32((uint32_t)k, log2);
+      for (uint64_t step = 0; step <= mask; ++step) {
+        const unsigned long long old = atomicCAS((unsigned long long *)&slots[s], kEmpty8, word);
+        if (old == kEmpty8) break;
+        if ((int32_t)(uint32_t)old == (int32_t)k) dup = 1;
+        s = (s + 1) & mask;
+      }
+    } else {
+      uint64_t s = hash_u64((uint64_t)k, log2);
+      for (uint64_t step = 0; step <= mask; ++step) {
+        const unsigned long long old = atomicCAS((unsigned long long *)&slots[2 * s], kEmpty16, (unsigned long long)k);
+        if (old == kEmpty16) {
+          slots[2 * s + 1] = (uint64_t)p;
+          break;
+        }
+        if ((int64_t)old == k) dup = 1;
+        s = (s + 1) & mask;
+      }
+    }
+    ++inserted;
+    kmin = k < kmin ? k : kmin;
+    kmax = k > kmax ? k : kmax;
+    pmin = p < pmin ? p : pmin;
+    pmax = p > pmax ? p : pmax;
+  }
+  stat_partial<kHashThreads>(part, inserted, dup, kmin, kmax, pmin, pmax);
 }
 
 // ---- regional build: large tables without device-scope atomics ---------------------------------
@@ -551,7 +635,8 @@ struct DenseJob {
   const uint32_t *sel;
   const uint64_t *n_dev;  // the row count on the device (at most n), optional
   const void *pay;
-  uint64_t n;
+  uint64_t n;             // rows, or slots of slots8
+  const uint64_t *slots8; // set: the entries are the L8 table's n slots, not build rows
   int64_t key_min;
   uint64_t range;
   uint32_t *bitmap;
@@ -574,6 +659,17 @@ __global__ __launch_bounds__(kHashThreads) void k_dense_fill(DenseJobs J) {
   uint64_t n = d.n;
   if (d.n_dev && *d.n_dev < n) n = *d.n_dev;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  if (d.slots8) {  // every occupied slot: (int32 key, int32 payload)
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+      const uint64_t w = d.slots8[i];
+      if (w == kEmpty8) continue;
+      const uint64_t idx = (uint64_t)((int64_t)(int32_t)(uint32_t)w - d.key_min);
+      if (idx >= d.range) continue;
+      atomicOr(&d.bitmap[idx >> 5], 1u << (idx & 31));
+      d.dense[idx] = (int32_t)(uint32_t)(w >> 32);
+    }
+    return;
+  }
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const uint64_t r = d.sel ? d.sel[i] : i;
     int64_t k;
@@ -649,7 +745,11 @@ static void finish_stats(ghive_ctx *ctx, ghive_hash_table *h, const uint64_t (&h
     return;
   }
   h->range = range;
-  *job = DenseJob{key, sel, n_dev, pay, n, h->key_min, range, h->bitmap, h->dense, h->key_type, pay_type};
+  if (h->dense_from_slots)
+    *job = DenseJob{key, nullptr, nullptr, pay, h->slots, static_cast<const uint64_t *>(h->data), h->key_min, range,
+                    h->bitmap, h->dense, h->key_type, pay_type};
+  else
+    *job = DenseJob{key, sel, n_dev, pay, n, nullptr, h->key_min, range, h->bitmap, h->dense, h->key_type, pay_type};
   *has_job = true;
 }
 
@@ -1036,6 +1136,107 @@ int ghive_hash_build_async(ghive_ctx *ctx, const ghive_column *key, const uint32
   if (!sel && n_max != key->row_num)
     return set_error(ctx, GHIVE_ERR_INVALID, "hash_build_async: n_max != rows without a selection");
   return hash_build_core(ctx, key, sel, n_max, n_dev, payload, load_factor, true, out);
+}
+
+int ghive_hash_build_filtered(ghive_ctx *ctx, const ghive_filtered_build *jobs, uint32_t n, ghive_hash_table **out) {
+  GH_CHECK_CTX(ctx);
+  if (!jobs || !out) return set_error(ctx, GHIVE_ERR_INVALID, "hash_build_filtered: NULL argument");
+  if (n == 0 || n > GHIVE_MAX_FILTERED_BUILDS)
+    return set_error(ctx, GHIVE_ERR_INVALID, "hash_build_filtered: %u builds (1..%d)", n, GHIVE_MAX_FILTERED_BUILDS);
+  for (uint32_t i = 0; i < n; ++i) out[i] = nullptr;
+  // every build is checked before anything is allocated or launched
+  std::vector<DevPred> preds(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    const ghive_filtered_build &J = jobs[i];
+    GH_TRY(check_table(ctx, J.table, "hash_build_filtered"));
+    const ghive_table &T = *J.table;
+    if (J.key_col >= T.ncols || !is_int_type(T.columns[J.key_col].type))
+      return set_error(ctx, GHIVE_ERR_INVALID, "hash_build_filtered: build %u: key column %u (an INT/LONG column)", i,
+                       J.key_col);
+    if (J.payload_col >= 0 && ((uint32_t)J.payload_col >= T.ncols || !is_int_type(T.columns[J.payload_col].type)))
+      return set_error(ctx, GHIVE_ERR_INVALID, "hash_build_filtered: build %u: payload column %d (an INT/LONG column or -1)",
+                       i, J.payload_col);
+    if (J.load_factor > 0.95f)
+      return set_error(ctx, GHIVE_ERR_INVALID, "hash_build_filtered: build %u: load factor %.3f > 0.95", i, J.load_factor);
+    memset(&preds[i], 0, sizeof(DevPred));
+    if (J.nnodes) GH_TRY(flatten_pred(ctx, J.table, J.nodes, J.nnodes, J.root, preds[i]));
+  }
+  // tables sized for every row (as an asynchronous build over a selection of unknown count); per
+  // build one block of statistics partials; the dense companion comes from the slots
+  std::vector<ghive_hash_table *> hs;
+  std::vector<uint32_t> grids(n);
+  auto undo = [&](int st) {
+    for (ghive_hash_table *h : hs) {
+      if (h->data) ctx->arena->free(h->data);
+      if (h->stats_buf) ctx->arena->free(h->stats_buf);
+      delete h;
+    }
+    return st;
+  };
+  TableFills F;
+  TableStatsAll S;
+  memset(&F, 0, sizeof F);
+  memset(&S, 0, sizeof S);
+  uint64_t max_words = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const ghive_filtered_build &J = jobs[i];
+    const ghive_column &key = J.table->columns[J.key_col];
+    const ghive_column *pay = J.payload_col >= 0 ? &J.table->columns[J.payload_col] : nullptr;
+    const uint64_t rows = J.table->row_num;
+    auto *h = new ghive_hash_table();
+    hs.push_back(h);
+    h->owner = ctx;
+    h->key_type = key.type;
+    h->source_rows = rows;
+    h->payload_kind = pay ? (pay->type == GHIVE_INT ? 1 : 2) : 0;
+    h->layout = (key.type == GHIVE_INT && h->payload_kind != 2) ? 8 : 16;
+    const float lf = J.load_factor > 0.f ? J.load_factor : 0.5f;
+    const uint64_t want = (uint64_t)((double)(rows ? rows : 1) / lf) + 1;
+    h->slots = next_pow2(want < 64 ? 64 : want);
+    h->log2_slots = __builtin_ctzll(h->slots);
+    const uint64_t words = h->slots * (h->layout == 8 ? 1 : 2);
+    grids[i] = grid_for(ctx, rows ? rows : 1);
+    h->data = ctx->arena->alloc(words * 8);
+    h->stats_buf = ctx->arena->alloc(((uint64_t)grids[i] + 1) * kStatWords * sizeof(uint64_t));
+    if (!h->data || !h->stats_buf)
+      return undo(set_error(ctx, GHIVE_ERR_OOM, "hash_build_filtered: cannot allocate build %u (%llu slots)", i,
+                            (unsigned long long)h->slots));
+    auto *part = static_cast<unsigned long long *>(h->stats_buf);
+    F.t[i] = TableFill{static_cast<uint64_t *>(h->data), words, h->layout == 8 ? kEmpty8 : kEmpty16};
+    S.t[i] = TableStats{part, grids[i], part + (uint64_t)grids[i] * kStatWords};
+    max_words = words > max_words ? words : max_words;
+    h->stats_dev = S.t[i].stats;
+    h->in_key = key.data;
+    h->in_pay = pay ? pay->data : nullptr;
+    h->in_pay_type = pay ? pay->type : 0;
+    h->in_sel = nullptr;
+    h->in_n_dev = nullptr;
+    h->in_n_max = rows;
+    h->dense_from_slots = true;
+  }
+  const uint64_t fg = (max_words + 1023) / 1024, fcap = (uint64_t)ctx->cus * 8;
+  k_fill_tables<<<dim3((uint32_t)(fg < 1 ? 1 : (fg > fcap ? fcap : fg)), n), 256, 0, ctx->stream>>>(F);
+  if (hipGetLastError() != hipSuccess) return undo(set_error(ctx, GHIVE_ERR_HIP, "hash_build_filtered: launch failed"));
+  for (uint32_t i = 0; i < n; ++i) {
+    const ghive_filtered_build &J = jobs[i];
+    const ghive_column &key = J.table->columns[J.key_col];
+    const ghive_column *pay = J.payload_col >= 0 ? &J.table->columns[J.payload_col] : nullptr;
+    ghive_hash_table *h = hs[i];
+    k_filtered_build<<<grids[i], kHashThreads, 0, ctx->stream>>>(
+        preds[i], key.data, key.type, pay ? pay->data : nullptr, pay ? pay->type : 0, J.table->row_num, h->layout,
+        static_cast<uint64_t *>(h->data), h->slots - 1, h->log2_slots, static_cast<unsigned long long *>(h->stats_buf));
+  }
+  k_build_stats_tables<<<n, kHashThreads, 0, ctx->stream>>>(S);
+  if (hipGetLastError() != hipSuccess) {
+    (void)hipStreamSynchronize(ctx->stream);  // no kernel may still write the memory given back
+    return undo(set_error(ctx, GHIVE_ERR_HIP, "hash_build_filtered: launch failed"));
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    hs[i]->pending = true;
+    ctx->tables.insert(hs[i]);
+    out[i] = hs[i];
+  }
+  return GHIVE_OK;
 }
 
 int ghive_hash_table_info(const ghive_hash_table *ht, uint64_t *entries, uint64_t *slots,

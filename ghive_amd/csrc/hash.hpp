@@ -36,6 +36,8 @@ struct ghive_hash_table {
   // companion built, on the table's first use); the build inputs are kept for that.
   std::atomic<bool> pending{false};
   void *stats_buf = nullptr;              // arena block: per-workgroup partials, then the statistics
+  bool dense_from_slots = false;          // the dense companion is filled from the table's slots (a
+                                          // filtered build keeps no list of its rows)
   const unsigned long long *stats_dev = nullptr;
   const void *in_key = nullptr, *in_pay = nullptr;
   int in_pay_type = 0;

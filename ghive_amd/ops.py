@@ -235,6 +235,30 @@ class FilterPredicate:
         return out, cnt
 
 
+def filtered_hash_tables(ctx, builds):
+    """Hash tables of several build sides, each filtered and built in one pass
+    (ghive_hash_build_filtered): builds = [(table, FilterPredicate or None, key_col, payload_col or
+    None, load_factor)]. Equal to FilterPredicate.process_async then HashTable(..., count=...) per
+    build; the statistics stay on the device until a table's first use."""
+    n = len(builds)
+    if not 1 <= n <= abi.MAX_FILTERED_BUILDS:
+        raise ValueError(f"{n} builds (1..{abi.MAX_FILTERED_BUILDS})")
+    jobs = (abi.FilteredBuild * n)()
+    keep = []
+    for i, (table, pred, key_col, pay_col, lf) in enumerate(builds):
+        jobs[i].table = C.addressof(table.table)
+        jobs[i].nodes = C.addressof(pred._arr) if pred is not None and pred.nodes else None
+        jobs[i].nnodes = len(pred.nodes) if pred is not None else 0
+        jobs[i].root = pred.root if pred is not None else 0
+        jobs[i].key_col = key_col
+        jobs[i].payload_col = -1 if pay_col is None else pay_col
+        jobs[i].load_factor = lf
+        keep.append((table, pred))
+    hs = (C.c_void_p * n)()
+    ctx.check(lib().ghive_hash_build_filtered(ctx.h, jobs, n, hs))
+    return [HashTable(ctx, keep[i], _handle=C.c_void_p(hs[i])) for i in range(n)]
+
+
 class FilterOperator:
     """FilterOperator: EXPLAIN predicate text + the operator's output columns."""
 
@@ -275,11 +299,15 @@ class FilterOperator:
 class HashTable:
     """Device hash table of a build side (unique or duplicate keys)."""
 
-    def __init__(self, ctx, key, sel=None, payload=None, load_factor=0.5, count=None):
+    def __init__(self, ctx, key, sel=None, payload=None, load_factor=0.5, count=None, _handle=None):
         """count: a 1-element device tensor holding how many entries of sel to insert (from
         FilterPredicate.process_async): the build then leaves its statistics on the device until
         the table's first use, so that filter, build and the consumer wait once."""
         self.ctx = ctx
+        if _handle is not None:  # built elsewhere (filtered_hash_tables); key: what the table keeps alive
+            self._keep = key
+            self.h = _handle
+            return
         kt = Table([key] + ([payload] if payload is not None else []))
         self._keep = (kt, key, payload, sel, count)
         n = sel.numel() if sel is not None else key.numel()

@@ -103,14 +103,20 @@ def star_tables(ctx, fact_rows=600_000_000, seed=SEED_BASE + 2):
 
 
 def star_hash_tables(ctx, T, year_lo=1992, year_hi=1993, region=1, category=1, load_factor=0.5,
-                     wait_each=False):
+                     wait_each=False, fused=True):
     """Filtered dimension hash tables: date (-> year), supplier (existence), part (-> brand).
-    By default each filter hands its count to the build on the device (no host round trip until
-    the tables are first used); wait_each=True reads every count on the host first."""
+    By default the three filters and builds run as one filtered build (ghive_hash_build_filtered:
+    a pass per table filters and inserts, one launch fills and one reduces all three tables'
+    statistics); fused=False runs each filter then its build, the filter handing its count to the
+    build on the device (no host round trip until the tables are first used); wait_each=True
+    reads every count on the host first."""
     P = ops.FilterPredicate
     preds = [(T.date, [abi.make_pred(abi.FILTER_EQ_RANGE, 1, [year_lo, year_hi], data_type=2)], 1),
              (T.supplier, [abi.make_pred(abi.FILTER_EQ, 1, [region], data_type=2)], 1),
              (T.part, [abi.make_pred(abi.FILTER_EQ, 1, [category], data_type=2)], 2)]
+    if fused and not wait_each:
+        return tuple(ops.filtered_hash_tables(ctx, [(dim, P(nodes, 0), 0, pay_col, load_factor)
+                                                    for dim, nodes, pay_col in preds]))
     tables = []
     for dim, nodes, pay_col in preds:
         if wait_each:

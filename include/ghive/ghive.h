@@ -251,6 +251,29 @@ int ghive_hash_build(ghive_ctx *ctx, const ghive_column *key, const uint32_t *se
 int ghive_hash_build_async(ghive_ctx *ctx, const ghive_column *key, const uint32_t *sel,
                            const uint64_t *n_dev, uint64_t n_max, const ghive_column *payload,
                            float load_factor, ghive_hash_table **out);
+/* One build side of ghive_hash_build_filtered: a table's filter (as ghive_filter takes it; nnodes
+ * 0: every row) and the hash build over the rows that pass. */
+typedef struct ghive_filtered_build {
+  const ghive_table *table;  /* device columns */
+  const ghive_pred *nodes;
+  uint32_t nnodes;
+  uint32_t root;
+  uint32_t key_col;     /* INT or LONG column of table */
+  int32_t payload_col;  /* INT or LONG column of table, or -1: the build row id */
+  float load_factor;    /* <= 0: 0.5 */
+  uint32_t reserved;
+} ghive_filtered_build;
+#define GHIVE_MAX_FILTERED_BUILDS 8
+/* Filters and builds n (1..GHIVE_MAX_FILTERED_BUILDS) tables at once: one pass over each table
+ * evaluates the filter and inserts the passing rows with non-NULL keys, and every table's fill
+ * and statistics run as one launch for all of them, with no wait on the host. Each out[i] equals
+ * ghive_hash_build_async over ghive_filter_async's selection of jobs[i] (same entries, payloads,
+ * statistics and dense companion; sized for the table's rows) and is finished the same way, on
+ * its first use. The tables, their predicates' columns and the key and payload columns must stay
+ * valid until that first use. No counterpart in the reference, which filters on the device and
+ * builds its map-join tables on the host (FilterOperator.cu:36-93, exec/tez/HashTableLoader.java:277,440-468). */
+int ghive_hash_build_filtered(ghive_ctx *ctx, const ghive_filtered_build *jobs, uint32_t n,
+                              ghive_hash_table **out);
 int ghive_hash_table_info(const ghive_hash_table *ht, uint64_t *entries, uint64_t *slots,
                           uint64_t *bytes, int32_t *unique_keys);
 void ghive_hash_table_free(ghive_ctx *ctx, ghive_hash_table *ht);

@@ -56,6 +56,7 @@ def test_struct_layouts_match_header(tmp_path):
         "ghive_plan_exec_info": abi.PlanExecInfo,
         "ghive_process_times": abi.ProcessTimes,
         "ghive_sort_info": abi.SortInfo,
+        "ghive_filtered_build": abi.FilteredBuild,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ghive/ghive.h"', "int main(void){"]
     for cname, py in structs.items():

@@ -296,3 +296,68 @@ def test_regional_build_overflow_and_wrap(ctx):
     assert info["unique"] and info["entries"] == len(bk) and info["slots"] == 1 << log2
     assert pairs == sorted(zip(range(len(bk)), pay.tolist()))
     assert out[1] == out[0]
+
+
+def test_filtered_build_equals_filter_then_build(ctx):
+    """ghive_hash_build_filtered over four build sides at once (INT key with an INT payload, LONG key
+    with a LONG payload, INT key with the row id, LONG key with no filter; NULL keys; AND / OR
+    trees; a filter that keeps nothing) against filter_async + the asynchronous build of each: the
+    same entries, slots and uniqueness, the same probe output, and the same dense companion (a
+    fragment over the tables gives the same groups)."""
+    from ghive_amd import ops
+    rng = np.random.default_rng(41)
+    n = 300_000
+    ki = rng.permutation(n).astype(np.int32) + 1
+    ki[::97] = INT_NULL
+    kl = rng.integers(-2**60, 2**60, n).astype(np.int64)
+    kl[::89] = LONG_NULL
+    a = rng.integers(0, 7, n).astype(np.int32)
+    b = rng.integers(-100, 100, n).astype(np.int64)
+    pi = rng.integers(-2**31, 2**31 - 1, n).astype(np.int32)
+    T = ops.Table([to_dev(ki), to_dev(kl), to_dev(a), to_dev(b), to_dev(pi)])
+    P = ops.FilterPredicate
+    eq = P([abi.make_pred(abi.FILTER_EQ, 2, [3], data_type=2)], 0)
+    tree = P([abi.make_pred(abi.FILTER_EQ_RANGE, 2, [1, 2], data_type=2), abi.make_pred(abi.FILTER_GT, 3, [40]),
+              abi.make_pred(abi.FILTER_AND, children=(0, 1)), abi.make_pred(abi.FILTER_LT, 3, [-90]),
+              abi.make_pred(abi.FILTER_OR, children=(2, 3))], 4)
+    none = P([abi.make_pred(abi.FILTER_EQ, 2, [99], data_type=2)], 0)
+    builds = [(T, eq, 0, 4, 0.5), (T, tree, 1, 3, 0.5), (T, eq, 0, None, 0.75), (T, None, 1, None, 0.5),
+              (T, none, 0, 4, 0.5)]
+    fused = ops.filtered_hash_tables(ctx, builds)
+    probe_i = to_dev(np.concatenate([ki, rng.integers(0, n + 1000, 50_000).astype(np.int32)]))
+    probe_l = to_dev(np.concatenate([kl, rng.integers(-2**60, 2**60, 50_000).astype(np.int64)]))
+    for (tab, pred, kc, pc, lf), hf in zip(builds, fused):
+        if pred is None:
+            sel, cnt = None, None
+            hs = ops.HashTable(ctx, tab.columns[kc], payload=None if pc is None else tab.columns[pc], load_factor=lf)
+        else:
+            sel, cnt = pred.process_async(ctx, tab)
+            hs = ops.HashTable(ctx, tab.columns[kc], sel=sel, payload=None if pc is None else tab.columns[pc],
+                               load_factor=lf, count=cnt)
+        fi, si = hf.info(), hs.info()
+        assert fi["entries"] == si["entries"] and fi["unique"] == si["unique"]
+        if pred is not None:
+            assert fi["slots"] == si["slots"]
+        probe = probe_i if kc == 0 else probe_l
+        rf, vf = hf.probe(probe)
+        rs, vs = hs.probe(probe)
+        assert _pairs(to_np(rf), to_np(vf)) == _pairs(to_np(rs), to_np(vs))
+        hs.free()
+        hf.free()
+
+
+def test_star_fragment_over_filtered_builds(ctx):
+    """The star-join fragment over the dimension tables of one filtered build (the default) and over
+    filter-then-build tables: the same groups and sums."""
+    from ghive_amd import workloads as W
+    T = W.star_tables(ctx, fact_rows=3_000_000, seed=77)
+    res = []
+    for fused in (True, False):
+        hd, hs, hp = W.star_hash_tables(ctx, T, fused=fused)
+        out = W.star_fragment(T.fact, hd, hs, hp).execute(ctx, capacity=8192)
+        keys = np.stack([to_np(o).astype(np.int64) for o in out[:-1]], axis=1)
+        order = np.lexsort(keys.T[::-1])
+        res.append([to_np(o)[order].tolist() for o in out])
+        for h in (hd, hs, hp):
+            h.free()
+    assert res[0] == res[1] and len(res[0][0]) > 0
