@@ -565,6 +565,14 @@ class Fragment:
         self.f = f
         self.nkeys, self.naggs = len(keys), len(aggs)
 
+    def rebind(self, tables):
+        """The same fragment over new dimension tables (one HashTable per join, in join order)."""
+        if len(tables) != len(self.joins_py):
+            raise ValueError("one table per join")
+        self.joins_py = [(c, ht) for (c, _), ht in zip(self.joins_py, tables)]
+        for i, ht in enumerate(tables):
+            self.jarr[i].table = ht.h.value
+
     def execute(self, ctx, capacity=4096, late=False, outs=None, noalloc=False):  # noqa: D102 (see class doc)
         return self._execute(ctx, capacity, late, outs, noalloc)
 

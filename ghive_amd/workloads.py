@@ -102,6 +102,16 @@ def star_tables(ctx, fact_rows=600_000_000, seed=SEED_BASE + 2):
     return StarTables(star_fact(ctx, fact_rows, seed), date, supplier, part, customer, seed, fact_rows)
 
 
+def star_builds(T, year_lo=1992, year_hi=1993, region=1, category=1, load_factor=0.5):
+    """The star query's three build sides as ops.filtered_hash_tables takes them: (dimension,
+    filter, key column, payload column, load factor) for date (-> year), supplier (existence) and
+    part (-> brand). Built once, run every step (a prepared plan)."""
+    P = ops.FilterPredicate
+    return [(T.date, P([abi.make_pred(abi.FILTER_EQ_RANGE, 1, [year_lo, year_hi], data_type=2)], 0), 0, 1, load_factor),
+            (T.supplier, P([abi.make_pred(abi.FILTER_EQ, 1, [region], data_type=2)], 0), 0, 1, load_factor),
+            (T.part, P([abi.make_pred(abi.FILTER_EQ, 1, [category], data_type=2)], 0), 0, 2, load_factor)]
+
+
 def star_hash_tables(ctx, T, year_lo=1992, year_hi=1993, region=1, category=1, load_factor=0.5,
                      wait_each=False, fused=True):
     """Filtered dimension hash tables: date (-> year), supplier (existence), part (-> brand).
@@ -115,8 +125,7 @@ def star_hash_tables(ctx, T, year_lo=1992, year_hi=1993, region=1, category=1, l
              (T.supplier, [abi.make_pred(abi.FILTER_EQ, 1, [region], data_type=2)], 1),
              (T.part, [abi.make_pred(abi.FILTER_EQ, 1, [category], data_type=2)], 2)]
     if fused and not wait_each:
-        return tuple(ops.filtered_hash_tables(ctx, [(dim, P(nodes, 0), 0, pay_col, load_factor)
-                                                    for dim, nodes, pay_col in preds]))
+        return tuple(ops.filtered_hash_tables(ctx, star_builds(T, year_lo, year_hi, region, category, load_factor)))
     tables = []
     for dim, nodes, pay_col in preds:
         if wait_each:
