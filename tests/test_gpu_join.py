@@ -348,16 +348,35 @@ def test_filtered_build_equals_filter_then_build(ctx):
 
 def test_star_fragment_over_filtered_builds(ctx):
     """The star-join fragment over the dimension tables of one filtered build (the default) and over
-    filter-then-build tables: the same groups and sums."""
-    from ghive_amd import workloads as W
+    filter-then-build tables: the same groups and sums; then one fragment descriptor rebound to
+    fresh tables for each of three more runs (as the benchmark's step does), into the same result
+    buffers, with the same result each time."""
+    import torch
+    from ghive_amd import ops, workloads as W
     T = W.star_tables(ctx, fact_rows=3_000_000, seed=77)
+
+    def canon(out):
+        keys = np.stack([to_np(o).astype(np.int64) for o in out[:-1]], axis=1)
+        order = np.lexsort(keys.T[::-1])
+        return [to_np(o)[order].tolist() for o in out]
+
     res = []
     for fused in (True, False):
         hd, hs, hp = W.star_hash_tables(ctx, T, fused=fused)
-        out = W.star_fragment(T.fact, hd, hs, hp).execute(ctx, capacity=8192)
-        keys = np.stack([to_np(o).astype(np.int64) for o in out[:-1]], axis=1)
-        order = np.lexsort(keys.T[::-1])
-        res.append([to_np(o)[order].tolist() for o in out])
+        res.append(canon(W.star_fragment(T.fact, hd, hs, hp).execute(ctx, capacity=8192)))
         for h in (hd, hs, hp):
             h.free()
     assert res[0] == res[1] and len(res[0][0]) > 0
+    builds = W.star_builds(T)
+    frag = None
+    outs = [torch.empty(8192, dtype=torch.int32, device="cuda") for _ in range(2)] + \
+           [torch.empty(8192, dtype=torch.int64, device="cuda")]
+    for _ in range(3):
+        tabs = ops.filtered_hash_tables(ctx, builds)
+        if frag is None:
+            frag = W.star_fragment(T.fact, *tabs)
+        else:
+            frag.rebind(tabs)
+        assert canon(frag.execute(ctx, capacity=8192, outs=outs)) == res[0]
+        for h in tabs:
+            h.free()
