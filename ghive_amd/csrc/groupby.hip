@@ -1253,8 +1253,11 @@ int groupby_partitioned(ghive_ctx *ctx, const KeySpec &K, GroupTableDev &Tspec, 
     std::vector<int> digits;  // the 8-bit digits of the hash that cover its top pbits bits
     for (int d = pbits ? (64 - pbits) / 8 : 8; d < 8; ++d) digits.push_back(d);
     ks = k0.as<uint64_t>();
-    GH_TRY(radix_sort_u64_ids(ctx, n, k0.as<uint64_t>(), k1.as<uint64_t>(), i0.as<uint32_t>(), i1.as<uint32_t>(),
-                              digits, &ks, &ids, true));
+    const ghive_sort_info last_sort = ctx->sort_info;  // the last ORDER BY's record stays as it was
+    const int sst = radix_sort_u64_ids(ctx, n, k0.as<uint64_t>(), k1.as<uint64_t>(), i0.as<uint32_t>(),
+                                       i1.as<uint32_t>(), digits, &ks, &ids, true);
+    ctx->sort_info = last_sort;
+    GH_TRY(sst);
   }
   {
     const uint64_t parts = (1ull << pbits) + 1;

@@ -799,7 +799,10 @@ static int launch_regional(ghive_ctx *ctx, ghive_hash_table *h, const ghive_colu
   for (int b = (64 - rbits) / 8; b < 8; ++b) digits.push_back(8 + b);
   uint64_t *words = nullptr;
   uint32_t *unused = nullptr;
-  GH_TRY(radix_sort_u64x2_ids(ctx, n, W0, W1, nullptr, nullptr, digits, &words, &unused, false));
+  const ghive_sort_info last_sort = ctx->sort_info;  // the last ORDER BY's record stays as it was
+  const int sst = radix_sort_u64x2_ids(ctx, n, W0, W1, nullptr, nullptr, digits, &words, &unused, false);
+  ctx->sort_info = last_sort;
+  GH_TRY(sst);
   uint64_t *ov = words == W0 ? W1 : W0;  // the other buffer: 2n words for overflow rows
   GH_TRY(fill_u64(ctx, reinterpret_cast<uint64_t *>(ov_count), 0, 1));
   k_region_starts<<<(nregions + 1 + 255) / 256, 256, 0, ctx->stream>>>(words + n, n, rbits, starts);

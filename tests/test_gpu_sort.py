@@ -186,3 +186,28 @@ def test_sort_merges_the_row_id_into_the_key_word(ctx, orc):
             _check(ctx, orc, [i, f], [0, 1], ["-", "+"])
     finally:
         ctx.set_option("sort.merge", 1)
+
+
+def test_sort_info_is_left_by_builds_and_group_bys(ctx, orc):
+    """ghive_last_sort_info describes the last ORDER BY: a regional hash build and a partitioned
+    group-by after it (both run radix passes of their own) leave it unchanged."""
+    from ghive_amd import ops
+    from conftest import to_dev
+    rng = np.random.default_rng(2)
+    k = rng.integers(-2**40, 2**40, 100_003).astype(np.int64)
+    _check(ctx, orc, [k], [0], ["+"])
+    before = ops.sort_info(ctx)
+    ctx.set_option("hash.regional_rows", 1)
+    try:
+        ht = ops.HashTable(ctx, to_dev(rng.integers(0, 2**50, 50_000).astype(np.int64)))
+        ht.info()
+        ht.free()
+    finally:
+        ctx.set_option("hash.regional_rows", 1 << 22)
+    ctx.set_option("groupby.partition", 2)
+    try:
+        ops.GroupByPredicate([0], [(abi.AGG_CNT, abi.COUNT_STAR)]).gpu_execute(
+            ctx, ops.Table([to_dev(k)]), capacity=200_000)
+    finally:
+        ctx.set_option("groupby.partition", 1)
+    assert ops.sort_info(ctx) == before
