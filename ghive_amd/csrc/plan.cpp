@@ -15,6 +15,7 @@
 // group-by over a reduce input is offset by that sink's key count at parse time (the reference
 // reads the key count of the child's runtime result).
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <cstring>
 #include <functional>
@@ -1586,7 +1587,38 @@ struct Executor {
       for (uint64_t f : host)
         if (f) return give_up("a LONG column holds values beyond 32 bits");
     }
-    for (size_t j = 0; j < S.joins.size(); ++j) {
+    // dimension tables below the regional-build size: built by one call (one fill launch, one
+    // statistics launch for all of them); a larger one takes the regional form, one by one
+    bool batch = S.joins.size() >= 1 && S.joins.size() <= GHIVE_MAX_FILTERED_BUILDS;
+    for (size_t j = 0; j < S.joins.size() && batch; ++j)
+      batch = ctx->tune.hash_regional_rows <= 0 || dims[j].rows < (uint64_t)ctx->tune.hash_regional_rows;
+    if (batch) {
+      std::vector<std::array<ghive_column, 2>> dcols(S.joins.size());
+      std::vector<ghive_table> dt(S.joins.size());
+      std::vector<ghive_filtered_build> jobs(S.joins.size());
+      std::vector<ghive_hash_table *> hs(S.joins.size(), nullptr);
+      for (size_t j = 0; j < S.joins.size(); ++j) {
+        const bool pay = S.joins[j].dim_pay >= 0;
+        dcols[j][0] = dkey[j];
+        if (pay) dcols[j][1] = dpay[j];
+        for (uint32_t c = 0; c < (pay ? 2u : 1u); ++c) dcols[j][c].row_num = dims[j].rows;
+        dt[j] = ghive_table{dcols[j].data(), pay ? 2u : 1u, 0, dims[j].rows};
+        memset(&jobs[j], 0, sizeof jobs[j]);
+        jobs[j].table = &dt[j];
+        jobs[j].key_col = 0;
+        jobs[j].payload_col = pay ? 1 : -1;
+        jobs[j].load_factor = 0.5f;
+      }
+      if ((st = ghive_hash_build_filtered(ctx, jobs.data(), (uint32_t)jobs.size(), hs.data())) != GHIVE_OK) {
+        if (st == GHIVE_ERR_UNSUPPORTED) return give_up(ghive_last_error(ctx));
+        return op_fail(st);
+      }
+      for (size_t j = 0; j < S.joins.size(); ++j) {
+        tables.push_back(hs[j]);
+        fj[j].table = hs[j];
+      }
+    }
+    for (size_t j = 0; j < S.joins.size() && !batch; ++j) {
       ghive_hash_table *h = nullptr;
       st = ghive::hash_build_deferred(ctx, &dkey[j], nullptr, dims[j].rows, S.joins[j].dim_pay >= 0 ? &dpay[j] : nullptr,
                                       0.5f, &h);

@@ -97,6 +97,14 @@ def test_star_map_vertex_fused_through_gpu_process(ctx, orc, n):
     finally:
         ctx.set_option("plan.fuse", 1)
     assert {(int(y), int(b)): int(s) for y, b, s in zip(*cols2)} == got
+    # dimension tables at the regional-build size are built one by one, not in one batched call
+    ctx.set_option("hash.regional_rows", 1)
+    try:
+        cols3, _, _ = ops.gpu_process(ctx, plan, tables, "Map 1")
+        assert ops.process_times(ctx)["fused"] == 1
+    finally:
+        ctx.set_option("hash.regional_rows", 1 << 22)
+    assert {(int(y), int(b)): int(s) for y, b, s in zip(*cols3)} == got
     # and the reduce vertex finishes the query
     final, _, _ = ops.gpu_process(ctx, plan, [("Map 1", cols)], "Reducer 5")
     assert {(int(y), int(b)): int(s) for y, b, s in zip(*final)} == got
