@@ -303,7 +303,9 @@ def skew_section(ctx, reps, with_cpu=True, gby_form="auto"):
     import numpy as np
     from ghive_amd import abi, ops, workloads as W
 
-    def timed(fn):
+    reps_ms = {}  # every repetition's wall time, per stage
+
+    def timed(fn, name):
         ts, out = [], None
         for _ in range(reps):
             out = None
@@ -312,19 +314,20 @@ def skew_section(ctx, reps, with_cpu=True, gby_form="auto"):
             out = fn()
             torch.cuda.synchronize()
             ts.append(time.perf_counter() - t0)
+        reps_ms[name] = [round(1e3 * t, 3) for t in ts]
         return float(np.median(ts)), out
 
     S = W.skew_tables(ctx)
     nb, npr = S.build_keys.numel(), S.probe_keys.numel()
-    t_build, ht = timed(lambda: ops.HashTable(ctx, S.build_keys, payload=S.build_pay, load_factor=0.5))
-    t_probe, (rows, pay) = timed(lambda: ht.probe(S.probe_keys, capacity=npr))
+    t_build, ht = timed(lambda: ops.HashTable(ctx, S.build_keys, payload=S.build_pay, load_factor=0.5), "build")
+    t_probe, (rows, pay) = timed(lambda: ht.probe(S.probe_keys, capacity=npr), "probe")
     matches = rows.numel()
     ht.free()
     r64 = rows.to(torch.int64) & 0xFFFFFFFF
     jk = S.probe_keys[r64].contiguous()
     jt = S.tiebreak[r64].contiguous()
     del rows, pay, r64, S
-    t_sort, perm = timed(lambda: ops.sort(ctx, ops.Table([jk, jt]), [0, 1], ["+", "+"]))
+    t_sort, perm = timed(lambda: ops.sort(ctx, ops.Table([jk, jt]), [0, 1], ["+", "+"]), "sort")
     si = ops.sort_info(ctx)
     del perm, jk, jt
     torch.cuda.empty_cache()
@@ -339,7 +342,7 @@ def skew_section(ctx, reps, with_cpu=True, gby_form="auto"):
 
     # the group-by form: chosen by the library (auto), or forced to the global-table form (direct)
     ctx.set_option("groupby.partition", {"auto": 1, "direct": 0, "partitioned": 2}[gby_form])
-    t_gby, _ = timed(gby)
+    t_gby, _ = timed(gby, "groupby")
     ctx.set_option("groupby.partition", 1)
     nrows = keys.numel()
     del keys, vals
@@ -353,6 +356,7 @@ def skew_section(ctx, reps, with_cpu=True, gby_form="auto"):
         "workload": "64M unique int64 build keys (int64 payload) x 1e9 probe rows, Zipf(1.0) over build "
                     "ranks + 10% misses; ORDER BY key, float32 tiebreak over the joined rows; GROUP BY "
                     "1e9 rows, 100M distinct int64 keys, COUNT(*), SUM(double N(0,1)) exact",
+        "reps_ms": reps_ms,  # the stage times above are the medians of these
         "build": {"rows": nb, "ms": 1e3 * t_build, "rows_per_s": nb / t_build, "gbs": gbs(b_build, t_build)},
         "probe": {"rows": npr, "matches": matches, "ms": 1e3 * t_probe, "rows_per_s": npr / t_probe,
                   "gbs": gbs(b_probe, t_probe), "frac": gbs(b_probe, t_probe) / HBM_PEAK_GBS},
