@@ -342,7 +342,9 @@ def skew_section(ctx, reps, with_cpu=True, gby_form="auto"):
 
     # the group-by form: chosen by the library (auto), or forced to the global-table form (direct)
     ctx.set_option("groupby.partition", {"auto": 1, "direct": 0, "partitioned": 2}[gby_form])
+    fb0 = ctx.get_option("groupby.partition_fallbacks")
     t_gby, _ = timed(gby, "groupby")
+    fallbacks = ctx.get_option("groupby.partition_fallbacks") - fb0
     ctx.set_option("groupby.partition", 1)
     nrows = keys.numel()
     del keys, vals
@@ -376,7 +378,8 @@ def skew_section(ctx, reps, with_cpu=True, gby_form="auto"):
                  "scatter_bytes_moved": si["scatter_bytes"],
                  "frac_bytes_moved": gbs(si["scatter_bytes"], t_sort) / HBM_PEAK_GBS},
         "groupby": {"rows": nrows, "groups": ng[0], "form": gby_form, "ms": 1e3 * t_gby, "rows_per_s": nrows / t_gby,
-                    "gbs": gbs(b_gby, t_gby), "frac": gbs(b_gby, t_gby) / HBM_PEAK_GBS},
+                    "gbs": gbs(b_gby, t_gby), "frac": gbs(b_gby, t_gby) / HBM_PEAK_GBS,
+                    "partition_fallbacks": fallbacks},  # repetitions that ran the direct form after the partitioned one
     }
     out["cpu_baseline"] = None
     if with_cpu:

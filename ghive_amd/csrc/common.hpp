@@ -86,6 +86,7 @@ struct ghive_ctx {
   ghive_plan_exec_info plan_info{};        // what the last ghive_plan_execute did
   ghive_process_times process_times{};     // the last ghive_gpu_process
   ghive_sort_info sort_info{};             // the last ghive_sort
+  uint64_t gb_part_fallbacks = 0;          // partitioned group-bys whose result was discarded for the direct form
   // the device address of this device's static scratch array (ctx.hip), looked up at init
   void *scratch = nullptr;
   ghive_tuning tune{};

@@ -414,6 +414,7 @@ int ghive_get_option(ghive_ctx *ctx, const char *name, int64_t *value) {
   else if (k == "sort.occupancy") *value = ctx->tune.sort_occupancy;
   else if (k == "groupby.part_threads") *value = ctx->tune.gb_part_threads;
   else if (k == "groupby.part_occupancy") *value = ctx->tune.gb_part_occupancy;
+  else if (k == "groupby.partition_fallbacks") *value = (int64_t)ctx->gb_part_fallbacks;
   else return set_error(ctx, GHIVE_ERR_INVALID, "unknown option '%s'", name);
   return GHIVE_OK;
 }

@@ -1511,6 +1511,7 @@ int ghive_groupby(ghive_ctx *ctx, const ghive_table *table, const uint32_t *sel,
     st = groupby_partitioned(ctx, K, T, sel, n, lcap, pbits, O, capacity, &failed, out_groups, /*scale_pending=*/true);
     if (st != GHIVE_OK && st != GHIVE_ERR_CAPACITY) return st;
     partitioned = !failed;
+    if (failed) ++ctx->gb_part_fallbacks;
   }
   if (!partitioned) GH_TRY(group_table_alloc(ctx, T, capacity, dkeys, dacc, dcnt));
   if (nkeys == 0) GH_TRY(group_table_preinsert(ctx, T, 0ull));  // a global aggregate: one group, even over no rows

@@ -526,6 +526,8 @@ int ghive_fragment_execute_late(ghive_ctx *ctx, const ghive_fragment *frag, ghiv
  * "groupby.partition" (0 never, 1 by size, 2 always: the hash-partitioned high-cardinality form),
  * "groupby.part_threads" (256, 512 or 1024: workgroup size of the partition passes; default 512),
  * "groupby.part_occupancy" (1/2: workgroups per CU its 512-thread 8-bit passes are compiled for; default 2),
+ * "groupby.partition_fallbacks" (read only, ghive_get_option: how many partitioned group-bys of this
+ * context had a partition overflow its LDS table and ran the direct form instead),
  * "hash.regional_rows" (synchronous hash builds of at least this many rows sort the rows by table
  * region and build each region of 8192 slots in one workgroup's LDS, with no fill pass and no
  * device-scope atomics but for keys that run past their region; 0 = never; default 4194304),

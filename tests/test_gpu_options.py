@@ -21,7 +21,7 @@ def test_option_defaults_round_trips_and_rejections(ctx):
         assert ctx.get_option(name) == value
         ctx.set_option(name, defaults[name])
     for name, bad in (("sort.occupancy", 3), ("groupby.part_occupancy", 0), ("hash.regional_rows", -1),
-                      ("sort.shape", 4), ("no.such.option", 1)):
+                      ("sort.shape", 4), ("no.such.option", 1), ("groupby.partition_fallbacks", 0)):
         with pytest.raises(GhiveError) as e:
             ctx.set_option(name, bad)
         assert e.value.status == abi.ERR_INVALID
@@ -54,3 +54,27 @@ def test_filtered_build_rejects_bad_arguments(ctx):
     (h,) = ops.filtered_hash_tables(ctx, [(T, pred, 0, None, 0.5)])  # and the valid call
     assert h.info()["entries"] == 989
     h.free()
+
+
+def test_partition_overflow_counts_one_fallback(ctx):
+    """The partitioned group-by sizes its partitions from the group bound: with a bound far below
+    the real group count its one partition overflows the LDS table, the direct form runs instead
+    (and reports the bound exceeded), and the read-only counter records it; a bound that holds
+    every group runs partitioned and adds nothing."""
+    from ghive_amd import ops
+    from ghive_amd._lib import GhiveError
+    n = 300_000
+    keys = to_dev(np.arange(n, dtype=np.int64) * 7919 + 11)
+    gb = ops.GroupByPredicate([0], [(abi.AGG_CNT, abi.COUNT_STAR)])
+    ctx.set_option("groupby.partition", 2)
+    try:
+        f0 = ctx.get_option("groupby.partition_fallbacks")
+        with pytest.raises(GhiveError) as e:
+            gb.gpu_execute(ctx, ops.Table([keys]), capacity=1000)
+        assert e.value.status == abi.ERR_CAPACITY
+        assert ctx.get_option("groupby.partition_fallbacks") == f0 + 1
+        out = gb.gpu_execute(ctx, ops.Table([keys]), capacity=n)
+        assert out[0].numel() == n and int(out[1].sum()) == n
+        assert ctx.get_option("groupby.partition_fallbacks") == f0 + 1
+    finally:
+        ctx.set_option("groupby.partition", 1)
